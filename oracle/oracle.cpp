@@ -1,0 +1,1633 @@
+// CPU oracle: single-threaded restatement of PotreeConverter 2.x's counting
+// sort chunker, octree indexer and Poisson / Poisson-average samplers.
+//
+// TEST INFRASTRUCTURE ONLY: used as the parity checker by tests/, smoke()
+// and bench.py's cpu_baseline leg.  Never linked into the product.
+//
+// Citations are relative to the reference checkout (Converter/...).  The
+// reference's own output order is schedule dependent; this restatement fixes
+// the canonical order described in oracle.h.
+#include "oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <filesystem>
+#include <fstream>
+#include <functional>
+#include <iomanip>
+#include <limits>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../potreeconverter_amd/csrc/lasfile.h"
+
+namespace fs = std::filesystem;
+
+namespace orc {
+
+static std::string g_error;
+static const double INF = std::numeric_limits<double>::infinity();
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::high_resolution_clock::now().time_since_epoch()).count();
+}
+
+// ---------------------------------------------------------------- vectors
+// Converter/include/Vector3.h:13-100
+struct V3 {
+  double x = 0, y = 0, z = 0;
+  V3() {}
+  V3(double a, double b, double c) : x(a), y(b), z(c) {}
+  V3 operator-(const V3& r) const { return V3(x - r.x, y - r.y, z - r.z); }
+  V3 operator+(const V3& r) const { return V3(x + r.x, y + r.y, z + r.z); }
+  V3 operator+(double s) const { return V3(x + s, y + s, z + s); }
+  V3 operator*(double s) const { return V3(x * s, y * s, z * s); }
+  V3 operator/(double s) const { return V3(x / s, y / s, z / s); }
+  double max() const { return std::max(std::max(x, y), z); }
+};
+
+struct Box {
+  V3 min, max;
+};
+
+// converter_utils.h:106-136
+static Box child_box(V3 min, V3 max, int index) {
+  Box b;
+  V3 size = max - min;
+  V3 center = min + (size * 0.5);
+  if ((index & 4) == 0) { b.min.x = min.x; b.max.x = center.x; } else { b.min.x = center.x; b.max.x = max.x; }
+  if ((index & 2) == 0) { b.min.y = min.y; b.max.y = center.y; } else { b.min.y = center.y; b.max.y = max.y; }
+  if ((index & 1) == 0) { b.min.z = min.z; b.max.z = center.z; } else { b.min.z = center.z; b.max.z = max.z; }
+  return b;
+}
+
+// converter_utils.h:89-104 (bit interleave: x -> bit 0, y -> bit 1, z -> bit 2
+// of each triple, called as morton(z, y, x) by the indexer)
+static uint64_t spread3(uint32_t a) {
+  uint64_t x = a & 0x1fffff;
+  uint64_t r = 0;
+  for (int i = 0; i < 21; i++) r |= ((x >> i) & 1ull) << (3 * i);
+  return r;
+}
+static uint64_t morton(uint32_t x, uint32_t y, uint32_t z) { return spread3(x) | spread3(y) << 1 | spread3(z) << 2; }
+
+// ---------------------------------------------------------------- attributes
+// Attributes.h:16-186
+enum AType { INT8 = 0, INT16 = 1, INT32 = 2, INT64 = 3, UINT8 = 10, UINT16 = 11, UINT32 = 12, UINT64 = 13,
+             FLOAT = 20, DOUBLE = 21, UNDEFINED = 123456 };
+
+static const char* type_name(int t) {
+  switch (t) {
+    case INT8: return "int8"; case INT16: return "int16"; case INT32: return "int32"; case INT64: return "int64";
+    case UINT8: return "uint8"; case UINT16: return "uint16"; case UINT32: return "uint32"; case UINT64: return "uint64";
+    case FLOAT: return "float"; case DOUBLE: return "double"; case UNDEFINED: return "undefined";
+  }
+  return "error";
+}
+static int type_from_name(const std::string& s) {
+  const int all[] = {INT8, INT16, INT32, INT64, UINT8, UINT16, UINT32, UINT64, FLOAT, DOUBLE, UNDEFINED};
+  for (int t : all) if (s == type_name(t)) return t;
+  return UNDEFINED;
+}
+static int type_size(int t) {
+  switch (t) {
+    case UINT8: case INT8: return 1; case UINT16: case INT16: return 2;
+    case UINT32: case INT32: case FLOAT: return 4; case UINT64: case INT64: case DOUBLE: return 8;
+  }
+  return 0;
+}
+
+struct Attr {
+  std::string name, description;
+  int size = 0, numElements = 0, elementSize = 0, type = UNDEFINED;
+  V3 min = V3(INF, INF, INF), max = V3(-INF, -INF, -INF);
+  V3 scale = V3(1, 1, 1), offset = V3(0, 0, 0);
+  std::vector<int64_t> histogram = std::vector<int64_t>(256, 0);
+  Attr() {}
+  Attr(std::string n, int s, int ne, int es, int t) : name(n), size(s), numElements(ne), elementSize(es), type(t) {}
+};
+
+struct Attrs {
+  std::vector<Attr> list;
+  int bytes = 0;
+  V3 posScale = V3(1, 1, 1), posOffset = V3(0, 0, 0);
+  void finish() { bytes = 0; for (auto& a : list) bytes += a.size; }
+  int offsetOf(const std::string& n) const {
+    int o = 0;
+    for (auto& a : list) { if (a.name == n) return o; o += a.size; }
+    return -1;
+  }
+  Attr* get(const std::string& n) { for (auto& a : list) if (a.name == n) return &a; return nullptr; }
+};
+
+// LasLoader.cpp:7-69 (extra bytes data type -> attribute type, element count)
+static void las_type_info(int id, int& type, int& ne) {
+  static const int base[] = {UNDEFINED, UINT8, INT8, UINT16, INT16, UINT32, INT32, UINT64, INT64, FLOAT, DOUBLE};
+  if (id == 0) { type = UNDEFINED; ne = 1; return; }
+  int k = (id - 1) % 10 + 1;
+  type = base[k];
+  ne = id <= 10 ? 1 : (id <= 20 ? 2 : 3);
+}
+
+// PotreeConverter.h:136-193 (attribute list per point format + extra bytes)
+static bool attributes_for_format(const lasfile::Header& h, std::vector<Attr>& list) {
+  Attr xyz("position", 12, 3, 4, INT32), intensity("intensity", 2, 1, 2, UINT16);
+  Attr returnNumber("return number", 1, 1, 1, UINT8), numberOfReturns("number of returns", 1, 1, 1, UINT8);
+  Attr classification("classification", 1, 1, 1, UINT8), scanAngleRank("scan angle rank", 1, 1, 1, UINT8);
+  Attr userData("user data", 1, 1, 1, UINT8), pointSourceId("point source id", 2, 1, 2, UINT16);
+  Attr gpsTime("gps-time", 8, 1, 8, DOUBLE), rgb("rgb", 6, 3, 2, UINT16);
+  Attr wpdi("wave packet descriptor index", 1, 1, 1, UINT8), bowd("byte offset to waveform data", 8, 1, 8, UINT64);
+  Attr wps("waveform packet size", 4, 1, 4, UINT32), rpwl("return point waveform location", 4, 1, 4, FLOAT);
+  Attr xyzt("XYZ(t)", 12, 3, 4, FLOAT), cflags("classification flags", 1, 1, 1, UINT8);
+  Attr scanAngle("scan angle", 2, 1, 2, INT16);
+  switch (h.point_format) {
+    case 0: list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId}; break;
+    case 1: list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId, gpsTime}; break;
+    case 2: list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId, rgb}; break;
+    case 3: list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId, gpsTime, rgb}; break;
+    default:
+      g_error = "oracle handles LAS point formats 0-3";
+      return false;
+  }
+  for (auto& e : h.extra) {
+    int t, ne;
+    las_type_info(e.data_type, t, ne);
+    int es = type_size(t);
+    Attr a(e.name, ne * es, ne, es, t);
+    a.description = e.description;
+    a.scale = V3(e.scale[0], e.scale[1], e.scale[2]);
+    a.offset = V3(e.offset[0], e.offset[1], e.offset[2]);
+    list.push_back(a);
+  }
+  return true;
+}
+
+// PotreeConverter.h:29-76 computeScaleOffset, :195-315 (single source)
+static bool output_attributes(const lasfile::Header& h, Attrs& out) {
+  if (!attributes_for_format(h, out.list)) return false;
+  out.finish();
+  V3 min(h.min[0], h.min[1], h.min[2]), max(h.max[0], h.max[1], h.max[2]);
+  V3 scale(h.scale[0], h.scale[1], h.scale[2]);
+  V3 size = max - min;
+  scale.x = std::max(scale.x, size.x / pow(2.0, 30.0));
+  scale.y = std::max(scale.y, size.y / pow(2.0, 30.0));
+  scale.z = std::max(scale.z, size.z / pow(2.0, 30.0));
+  out.posScale = scale;
+  out.posOffset = min;
+  return true;
+}
+
+// ---------------------------------------------------------------- chunker
+struct ChunkNode {  // chunker_countsort_laszip.cpp:64-78
+  std::string id;
+  int64_t level = 0, x = 0, y = 0, z = 0, size = 0, numPoints = 0;
+};
+
+// chunker_countsort_laszip.cpp:82-115
+static std::string to_node_id(int level, int gridSize, int64_t x, int64_t y, int64_t z) {
+  std::string id = "r";
+  int cur = gridSize;
+  int lx = (int)x, ly = (int)y, lz = (int)z;
+  for (int i = 0; i < level; i++) {
+    int index = 0;
+    if (lx >= cur / 2) { index += 4; lx -= cur / 2; }
+    if (ly >= cur / 2) { index += 2; ly -= cur / 2; }
+    if (lz >= cur / 2) { index += 1; lz -= cur / 2; }
+    id += std::to_string(index);
+    cur /= 2;
+  }
+  return id;
+}
+
+// cell index of one requantized point: chunker_countsort_laszip.cpp:232-257 and :817-835
+static inline int64_t cell_of(int32_t X, int32_t Y, int32_t Z, const V3& posScale, const V3& posOffset,
+                              const V3& min, double cube, int64_t G, bool* inBox) {
+  double ux = (double(X) * posScale.x + posOffset.x - min.x) / cube;
+  double uy = (double(Y) * posScale.y + posOffset.y - min.y) / cube;
+  double uz = (double(Z) * posScale.z + posOffset.z - min.z) / cube;
+  if (inBox) *inBox = ux >= 0.0 && uy >= 0.0 && uz >= 0.0 && ux <= 1.0 && uy <= 1.0 && uz <= 1.0;
+  double dG = double(G);
+  int64_t ix = int64_t(std::min(dG * ux, dG - 1.0));
+  int64_t iy = int64_t(std::min(dG * uy, dG - 1.0));
+  int64_t iz = int64_t(std::min(dG * uz, dG - 1.0));
+  return ix + iy * G + iz * G * G;
+}
+
+// laszip_dll.cpp:1137-1139 then chunker_countsort_laszip.cpp:228-230 / :781-783
+static inline void requantize(const uint8_t* rec, const double* ls, const double* lo, const V3& posScale,
+                              const V3& posOffset, double xyz[3], int32_t q[3]) {
+  int32_t X, Y, Z;
+  memcpy(&X, rec + 0, 4); memcpy(&Y, rec + 4, 4); memcpy(&Z, rec + 8, 4);
+  xyz[0] = ls[0] * X + lo[0];
+  xyz[1] = ls[1] * Y + lo[1];
+  xyz[2] = ls[2] * Z + lo[2];
+  q[0] = int32_t((xyz[0] - posOffset.x) / posScale.x);
+  q[1] = int32_t((xyz[1] - posOffset.y) / posScale.y);
+  q[2] = int32_t((xyz[2] - posOffset.z) / posScale.z);
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" int orc_count_cells(const uint8_t* las, int64_t n, int32_t bpp, const double ls[3], const double lo[3],
+                               const double ps[3], const double po[3], const double mn[3], const double mx[3],
+                               int32_t grid_size, int32_t* grid) {
+  V3 posScale(ps[0], ps[1], ps[2]), posOffset(po[0], po[1], po[2]);
+  V3 min(mn[0], mn[1], mn[2]), max(mx[0], mx[1], mx[2]);
+  double cube = (max - min).max();
+  int64_t G = grid_size;
+  memset(grid, 0, sizeof(int32_t) * G * G * G);
+  for (int64_t i = 0; i < n; i++) {
+    double xyz[3];
+    int32_t q[3];
+    requantize(las + i * bpp, ls, lo, posScale, posOffset, xyz, q);
+    bool inBox;
+    int64_t idx = cell_of(q[0], q[1], q[2], posScale, posOffset, min, cube, G, &inBox);
+    if (!inBox) {
+      g_error = "encountered point outside bounding box";
+      return 123;
+    }
+    grid[idx]++;
+  }
+  return 0;
+}
+
+namespace orc {
+// chunker_countsort_laszip.cpp:1074-1204
+static void create_lut(const int32_t* grid, int64_t gridSize, int64_t maxPointsPerChunk,
+                       std::vector<ChunkNode>& nodes, std::vector<int32_t>& lut) {
+  std::vector<int64_t> high(grid, grid + gridSize * gridSize * gridSize);
+  int64_t level_max = int64_t(log2((double)gridSize));
+  for (int64_t level_low = level_max - 1; level_low >= 0; level_low--) {
+    int64_t level_high = level_low + 1;
+    int64_t gh = int64_t(1) << level_high, gl = int64_t(1) << level_low;
+    std::vector<int64_t> low(gl * gl * gl, 0);
+    for (int64_t x = 0; x < gl; x++)
+      for (int64_t y = 0; y < gl; y++)
+        for (int64_t z = 0; z < gl; z++) {
+          int64_t il = x + y * gl + z * gl * gl;
+          int64_t sum = 0;
+          bool unmergeable = false;
+          for (int j = 0; j < 8; j++) {
+            int64_t nx = 2 * x + ((j & 4) >> 2), ny = 2 * y + ((j & 2) >> 1), nz = 2 * z + (j & 1);
+            int64_t v = high[nx + ny * gh + nz * gh * gh];
+            if (v == -1) unmergeable = true; else sum += v;
+          }
+          if (unmergeable || sum > maxPointsPerChunk) {
+            for (int j = 0; j < 8; j++) {
+              int64_t nx = 2 * x + ((j & 4) >> 2), ny = 2 * y + ((j & 2) >> 1), nz = 2 * z + (j & 1);
+              int64_t v = high[nx + ny * gh + nz * gh * gh];
+              if (v > 0) {
+                ChunkNode c;
+                c.id = to_node_id((int)level_high, (int)gh, nx, ny, nz);
+                c.numPoints = v;
+                c.level = level_high;
+                c.x = nx; c.y = ny; c.z = nz;
+                c.size = int64_t(1) << (level_max - level_high);
+                nodes.push_back(c);
+              }
+            }
+            low[il] = -1;
+          } else {
+            low[il] = sum;
+          }
+        }
+    high.swap(low);
+  }
+  lut.assign(gridSize * gridSize * gridSize, -1);
+  for (size_t i = 0; i < nodes.size(); i++) {
+    const auto& c = nodes[i];
+    for (int64_t ox = 0; ox < c.size; ox++)
+      for (int64_t oy = 0; oy < c.size; oy++)
+        for (int64_t oz = 0; oz < c.size; oz++) {
+          int64_t x = c.size * c.x + ox, y = c.size * c.y + oy, z = c.size * c.z + oz;
+          lut[x + y * gridSize + z * gridSize * gridSize] = (int32_t)i;
+        }
+  }
+}
+}  // namespace orc
+
+extern "C" int64_t orc_create_lut(const int32_t* grid, int32_t grid_size, int64_t cap, int64_t* desc,
+                                  int64_t max_chunks, int32_t* lut) {
+  std::vector<ChunkNode> nodes;
+  std::vector<int32_t> l;
+  create_lut(grid, grid_size, cap, nodes, l);
+  memcpy(lut, l.data(), l.size() * sizeof(int32_t));
+  for (int64_t i = 0; i < (int64_t)nodes.size() && i < max_chunks; i++) {
+    desc[5 * i + 0] = nodes[i].level; desc[5 * i + 1] = nodes[i].x; desc[5 * i + 2] = nodes[i].y;
+    desc[5 * i + 3] = nodes[i].z; desc[5 * i + 4] = nodes[i].numPoints;
+  }
+  return (int64_t)nodes.size();
+}
+
+namespace orc {
+
+// Record transcode for point formats 0-3 + extra bytes, with per-attribute
+// min/max and the classification histogram:
+// chunker_countsort_laszip.cpp:380-658 (handlers), :770-803 (per point).
+struct Transcoder {
+  int fmt = 2;
+  int inBpp = 26;
+  Attrs* out = nullptr;
+  int oIntensity = -1, oRet = -1, oNret = -1, oCls = -1, oSar = -1, oUd = -1, oPsid = -1, oGps = -1, oRgb = -1;
+  int firstExtraIndex = 9, extraSrc = 0, extraDst = 0, extraBytes = 0;
+
+  void init(const lasfile::Header& h, Attrs& o) {
+    fmt = h.point_format;
+    inBpp = h.record_length;
+    out = &o;
+    oIntensity = o.offsetOf("intensity"); oRet = o.offsetOf("return number"); oNret = o.offsetOf("number of returns");
+    oCls = o.offsetOf("classification"); oSar = o.offsetOf("scan angle rank"); oUd = o.offsetOf("user data");
+    oPsid = o.offsetOf("point source id"); oGps = o.offsetOf("gps-time"); oRgb = o.offsetOf("rgb");
+    static const int core[] = {20, 28, 26, 34};  // legacy record size without extra bytes
+    extraSrc = core[fmt];
+    // :571-577 the destination offset of the first extra attribute is the
+    // summed size of the standard attributes of the input format
+    static const int firstIdx[] = {8, 9, 9, 10};
+    firstExtraIndex = firstIdx[fmt];
+    extraDst = 0;
+    for (int i = 0; i < firstExtraIndex; i++) extraDst += o.list[i].size;
+    extraBytes = 0;
+    for (size_t i = firstExtraIndex; i < o.list.size(); i++) extraBytes += o.list[i].size;
+  }
+
+  static void mm(Attr* a, double v) { a->min.x = std::min(a->min.x, v); a->max.x = std::max(a->max.x, v); }
+
+  template <class T>
+  static double asD(const uint8_t* p) { T v; memcpy(&v, p, sizeof(T)); return double(v); }
+  static double elem(int type, const uint8_t* p) {
+    switch (type) {
+      case INT8: return asD<int8_t>(p); case INT16: return asD<int16_t>(p); case INT32: return asD<int32_t>(p);
+      case INT64: return asD<int64_t>(p); case UINT8: return asD<uint8_t>(p); case UINT16: return asD<uint16_t>(p);
+      case UINT32: return asD<uint32_t>(p); case UINT64: return asD<uint64_t>(p); case FLOAT: return asD<float>(p);
+      case DOUBLE: return asD<double>(p);
+    }
+    return 0.0;
+  }
+
+  // dst must be zeroed (chunker_countsort_laszip.cpp:732)
+  void apply(const uint8_t* src, const int32_t q[3], const double xyz[3], uint8_t* dst) {
+    Attrs& o = *out;
+    memcpy(dst + 0, &q[0], 4); memcpy(dst + 4, &q[1], 4); memcpy(dst + 8, &q[2], 4);
+    Attr* pos = &o.list[0];
+    pos->min.x = std::min(pos->min.x, xyz[0]); pos->min.y = std::min(pos->min.y, xyz[1]); pos->min.z = std::min(pos->min.z, xyz[2]);
+    pos->max.x = std::max(pos->max.x, xyz[0]); pos->max.y = std::max(pos->max.y, xyz[1]); pos->max.z = std::max(pos->max.z, xyz[2]);
+    // handler order follows the input attribute list (:531-544)
+    uint16_t intensity; memcpy(&intensity, src + 12, 2);
+    uint8_t flags = src[14], cbyte = src[15];
+    int8_t sar = (int8_t)src[16];
+    uint8_t ud = src[17];
+    uint16_t psid; memcpy(&psid, src + 18, 2);
+    memcpy(dst + oIntensity, &intensity, 2); mm(&o.list[1], double(intensity));
+    uint8_t rn = flags & 7, nr = (flags >> 3) & 7;
+    dst[oRet] = rn; mm(o.get("return number"), double(rn));
+    dst[oNret] = nr; mm(o.get("number of returns"), double(nr));
+    // :472-486 legacy formats carry the 5-bit class; extended class is 0
+    uint8_t cls = cbyte & 0x1F;
+    Attr* ac = o.get("classification");
+    dst[oCls] = cls; ac->histogram[cls]++; mm(ac, double(cls));
+    dst[oSar] = (uint8_t)sar; mm(o.get("scan angle rank"), double(sar));
+    dst[oUd] = ud; mm(o.get("user data"), double(ud));
+    memcpy(dst + oPsid, &psid, 2); mm(o.get("point source id"), double(psid));
+    if (oGps >= 0) {
+      double g; memcpy(&g, src + 20, 8);
+      memcpy(dst + oGps, &g, 8); mm(o.get("gps-time"), g);
+    }
+    if (oRgb >= 0) {
+      int srcRgb = (fmt == 2) ? 20 : 28;
+      uint16_t c[3]; memcpy(c, src + srcRgb, 6);
+      memcpy(dst + oRgb, c, 6);
+      Attr* a = o.get("rgb");
+      a->min.x = std::min(a->min.x, double(c[0])); a->min.y = std::min(a->min.y, double(c[1])); a->min.z = std::min(a->min.z, double(c[2]));
+      a->max.x = std::max(a->max.x, double(c[0])); a->max.y = std::max(a->max.y, double(c[1])); a->max.z = std::max(a->max.z, double(c[2]));
+    }
+    // extra attributes :579-651
+    int so = 0, ao = extraDst;
+    for (size_t i = firstExtraIndex; i < o.list.size(); i++) {
+      Attr* a = &o.list[i];
+      memcpy(dst + ao, src + extraSrc + so, a->size);
+      const uint8_t* p = src + extraSrc + so;
+      if (a->numElements == 1) {
+        double x = elem(a->type, p);
+        a->min.x = std::min(a->min.x, x); a->max.x = std::max(a->max.x, x);
+      } else if (a->numElements == 2) {
+        double x = elem(a->type, p), y = elem(a->type, p + a->elementSize);
+        a->min.x = std::min(a->min.x, x); a->min.y = std::min(a->min.y, y);
+        a->max.x = std::max(a->max.x, x); a->max.y = std::max(a->max.y, y);
+      } else if (a->numElements == 3) {
+        double x = elem(a->type, p), y = elem(a->type, p + a->elementSize), z = elem(a->type, p + 2 * a->elementSize);
+        a->min.x = std::min(a->min.x, x); a->min.y = std::min(a->min.y, y); a->min.z = std::min(a->min.z, z);
+        a->max.x = std::max(a->max.x, x); a->max.y = std::max(a->max.y, y); a->max.z = std::max(a->max.z, z);
+      }
+      ao += a->size;
+      so += a->size;
+    }
+  }
+};
+
+// %.17g formatting, identical to iostream setprecision(max_digits10)
+static std::string d17(double v) {
+  std::ostringstream ss;
+  ss << std::setprecision(std::numeric_limits<double>::max_digits10) << v;
+  return ss.str();
+}
+static std::string jnum(double v) {  // JSON value for chunks/metadata.json (inf -> null like nlohmann)
+  if (std::isinf(v) || std::isnan(v)) return "null";
+  return d17(v);
+}
+
+// chunks/metadata.json (chunker_countsort_laszip.cpp:1005-1068).  The key
+// set is the reference's; number formatting is %.17g (round-trip exact),
+// non-finite values become null as nlohmann::json does.
+static void write_chunk_metadata(const std::string& path, V3 min, V3 max, const Attrs& a) {
+  std::ostringstream s;
+  s << "{\n    \"attributes\": [\n";
+  for (size_t i = 0; i < a.list.size(); i++) {
+    const Attr& t = a.list[i];
+    auto vec = [&](V3 v) {
+      std::string r = "[" + jnum(v.x);
+      if (t.numElements >= 2) r += ", " + jnum(v.y);
+      if (t.numElements >= 3) r += ", " + jnum(v.z);
+      return r + "]";
+    };
+    s << "        {\n";
+    s << "            \"description\": \"" << t.description << "\",\n";
+    s << "            \"elementSize\": " << t.elementSize << ",\n";
+    bool empty = true;
+    for (auto h : t.histogram) if (h != 0) empty = false;
+    if (t.size == 1 && !empty) {
+      s << "            \"histogram\": [";
+      for (size_t k = 0; k < t.histogram.size(); k++) s << (k ? ", " : "") << t.histogram[k];
+      s << "],\n";
+    }
+    s << "            \"max\": " << vec(t.max) << ",\n";
+    s << "            \"min\": " << vec(t.min) << ",\n";
+    s << "            \"name\": \"" << t.name << "\",\n";
+    s << "            \"numElements\": " << t.numElements << ",\n";
+    s << "            \"offset\": " << vec(t.offset) << ",\n";
+    s << "            \"scale\": " << vec(t.scale) << ",\n";
+    s << "            \"size\": " << t.size << ",\n";
+    s << "            \"type\": \"" << type_name(t.type) << "\"\n";
+    s << "        }" << (i + 1 < a.list.size() ? "," : "") << "\n";
+  }
+  s << "    ],\n";
+  s << "    \"max\": [" << d17(max.x) << ", " << d17(max.y) << ", " << d17(max.z) << "],\n";
+  s << "    \"min\": [" << d17(min.x) << ", " << d17(min.y) << ", " << d17(min.z) << "],\n";
+  s << "    \"offset\": [" << d17(a.posOffset.x) << ", " << d17(a.posOffset.y) << ", " << d17(a.posOffset.z) << "],\n";
+  s << "    \"scale\": [" << d17(a.posScale.x) << ", " << d17(a.posScale.y) << ", " << d17(a.posScale.z) << "]\n";
+  s << "}";
+  std::ofstream f(path, std::ios::binary);
+  f << s.str();
+}
+
+struct ChunkerOut {
+  std::vector<ChunkNode> chunks;                 // reference push order
+  std::vector<std::vector<uint8_t>> records;     // per chunk, input order
+  Attrs attrs;
+  V3 min, max;                                   // cubic box
+  double tCount = 0, tDistribute = 0, tTotal = 0;
+};
+
+// doChunking (chunker_countsort_laszip.cpp:1206-1261) for one source with
+// the whole file in memory.  State.pointsTotal = header point count.
+static int do_chunking(const std::string& lasPath, ChunkerOut& co) {
+  double t0 = now_s();
+  lasfile::Header h;
+  if (!lasfile::read_header(lasPath, h, &g_error)) return 1;
+  if (!output_attributes(h, co.attrs)) return 1;
+  // main.cpp:215-284 computeStats: cubic box from the header bounds; the
+  // chunker receives (min, min + cube) and cubes it once more
+  // (chunker_countsort_laszip.cpp:205-207, :1252-1254)
+  V3 min(h.min[0], h.min[1], h.min[2]), hmax(h.max[0], h.max[1], h.max[2]);
+  double statsCube = (hmax - min).max();
+  if (statsCube == 0.0) { g_error = "invalid bounding box"; return 123; }
+  V3 max = min + statsCube;
+  double cube = (max - min).max();
+  int64_t N = (int64_t)h.num_points;
+  int64_t maxPointsPerChunk = std::min(N / 20, int64_t(10000000));
+  int64_t G = N < 100000000 ? 128 : (N < 500000000 ? 256 : 512);
+  std::vector<uint8_t> las((size_t)N * h.record_length);
+  if (!lasfile::read_points(lasPath, h, 0, N, las.data())) { g_error = "short read"; return 1; }
+  V3 cubeMax = min + cube;
+
+  std::vector<int32_t> grid(G * G * G);
+  double mn[3] = {min.x, min.y, min.z}, mx[3] = {max.x, max.y, max.z};
+  double ps[3] = {co.attrs.posScale.x, co.attrs.posScale.y, co.attrs.posScale.z};
+  double po[3] = {co.attrs.posOffset.x, co.attrs.posOffset.y, co.attrs.posOffset.z};
+  int rc = orc_count_cells(las.data(), N, h.record_length, h.scale, h.offset, ps, po, mn, mx, (int32_t)G, grid.data());
+  if (rc) return rc;
+  double t1 = now_s();
+  co.tCount = t1 - t0;
+
+  std::vector<int32_t> lut;
+  create_lut(grid.data(), G, maxPointsPerChunk, co.chunks, lut);
+  co.records.assign(co.chunks.size(), {});
+  for (size_t c = 0; c < co.chunks.size(); c++) co.records[c].reserve(co.chunks[c].numPoints * co.attrs.bytes);
+
+  // distributePoints (:660-1003): per-thread attribute copies reset min/max
+  // (:385-389) then merge into the caller's copy (:914-932); with one batch
+  // sequence the merged result equals a single running min/max.
+  Transcoder tc;
+  tc.init(h, co.attrs);
+  for (auto& a : co.attrs.list) a.histogram.assign(256, 0);
+  int bpp = co.attrs.bytes;
+  std::vector<uint8_t> rec(bpp);
+  for (int64_t i = 0; i < N; i++) {
+    const uint8_t* src = las.data() + (size_t)i * h.record_length;
+    double xyz[3];
+    int32_t q[3];
+    requantize(src, h.scale, h.offset, co.attrs.posScale, co.attrs.posOffset, xyz, q);
+    std::fill(rec.begin(), rec.end(), 0);
+    tc.apply(src, q, xyz, rec.data());
+    int64_t idx = cell_of(q[0], q[1], q[2], co.attrs.posScale, co.attrs.posOffset, min, cube, G, nullptr);
+    int32_t ci = lut[idx];
+    if (ci < 0) { g_error = "point to node lookup failed"; return 123; }
+    co.records[ci].insert(co.records[ci].end(), rec.begin(), rec.end());
+  }
+  double t2 = now_s();
+  co.tDistribute = t2 - t1;
+  co.min = min;
+  co.max = cubeMax;
+  co.tTotal = t2 - t0;
+  return 0;
+}
+
+// ---------------------------------------------------------------- indexer
+// structures.h:27-140
+struct Node {
+  std::vector<std::shared_ptr<Node>> children;
+  std::string name;
+  std::shared_ptr<std::vector<uint8_t>> points;
+  V3 min, max;
+  int64_t indexStart = 0, byteOffset = 0, byteSize = 0, numPoints = 0;
+  bool sampled = false;
+  Node() {}
+  Node(std::string n, V3 mn, V3 mx) : name(n), min(mn), max(mx) { children.resize(8); }
+  int64_t level() const { return (int64_t)name.size() - 1; }
+  bool isLeaf() const { for (auto& c : children) if (c) return false; return true; }
+  // structures.h:61-88
+  void addDescendant(std::shared_ptr<Node> d) {
+    int dl = (int)d->name.size() - 1;
+    Node* cur = this;
+    for (int level = 1; level < dl; level++) {
+      int index = d->name[level] - '0';
+      if (cur->children[index]) {
+        cur = cur->children[index].get();
+      } else {
+        std::string cn = cur->name + std::to_string(index);
+        Box b = child_box(cur->min, cur->max, index);
+        auto c = std::make_shared<Node>(cn, b.min, b.max);
+        cur->children[index] = c;
+        cur = c.get();
+      }
+    }
+    cur->children[d->name[dl] - '0'] = d;
+  }
+  void traverse(const std::function<void(Node*)>& cb) { cb(this); for (auto& c : children) if (c) c->traverse(cb); }
+};
+
+using NodeCB = std::function<void(Node*)>;
+
+static const int64_t kLeafCap = 10000;  // indexer.h:47
+
+struct Indexer {
+  Attrs attributes;
+  std::shared_ptr<Node> root;
+  double spacing = 1.0;
+  int64_t octreeDepth = 0;
+  int64_t byteOffset = 0;
+  std::vector<uint8_t> octree;           // octree.bin in completion order
+  struct HRec { std::string name; int64_t byteOffset, byteSize, numPoints; };
+  std::vector<HRec> hrecords;            // HierarchyFlusher input, completion order
+  struct FCR { std::shared_ptr<Node> node; std::vector<uint8_t> points; };
+  std::vector<FCR> flushed;              // flushChunkRoot, in-memory
+};
+
+struct Candidate {  // indexer.cpp:681-689
+  std::string name;
+  int64_t indexStart = 0, numPoints = 0, level = 0, x = 0, y = 0, z = 0;
+};
+
+// indexer.cpp:691-730
+static std::vector<std::vector<int64_t>> sum_pyramid(const std::vector<int64_t>& grid, int gridSize) {
+  int maxLevel = (int)std::log2(gridSize);
+  int cur = gridSize / 2;
+  std::vector<std::vector<int64_t>> pyr(maxLevel + 1);
+  for (int l = 0; l < maxLevel; l++) pyr[l].resize((size_t)pow(8, l), 0);
+  pyr[maxLevel] = grid;
+  for (int l = maxLevel - 1; l >= 0; l--) {
+    for (int x = 0; x < cur; x++)
+      for (int y = 0; y < cur; y++)
+        for (int z = 0; z < cur; z++) {
+          uint64_t idx = morton(z, y, x), idx1 = morton(2 * z, 2 * y, 2 * x);
+          int64_t s = 0;
+          for (int i = 0; i < 8; i++) s += pyr[l + 1][idx1 + i];
+          pyr[l][idx] = s;
+        }
+    cur /= 2;
+  }
+  return pyr;
+}
+
+// indexer.cpp:732-817
+static std::vector<Candidate> create_nodes(std::vector<std::vector<int64_t>>& pyr) {
+  std::vector<Candidate> nodes;
+  std::vector<std::vector<int64_t>> offs;
+  for (auto& c : pyr) {
+    if (c.size() == 1) { offs.push_back({0}); continue; }
+    std::vector<int64_t> o(c.size(), 0);
+    for (size_t i = 1; i < c.size(); i++) o[i] = o[i - 1] + c[i - 1];
+    offs.push_back(o);
+  }
+  size_t maxLevel = pyr.size() - 1;
+  std::vector<Candidate> stack = {Candidate()};
+  while (!stack.empty()) {
+    Candidate cand = stack.back();
+    stack.pop_back();
+    auto& g = pyr[cand.level];
+    int64_t np = g[morton((uint32_t)cand.z, (uint32_t)cand.y, (uint32_t)cand.x)];
+    if ((size_t)cand.level == maxLevel) {
+      if (np > 0) nodes.push_back(cand);
+    } else if (np > kLeafCap) {
+      for (int i = 0; i < 8; i++) {
+        uint64_t i1 = morton(2 * (uint32_t)cand.z, 2 * (uint32_t)cand.y, 2 * (uint32_t)cand.x) + i;
+        int64_t cnt = pyr[cand.level + 1][i1];
+        if (cnt > 0) {
+          Candidate ch;
+          ch.level = cand.level + 1;
+          ch.name = cand.name + std::to_string(i);
+          ch.indexStart = offs[cand.level + 1][i1];
+          ch.numPoints = cnt;
+          ch.x = 2 * cand.x + ((i & 4) >> 2);
+          ch.y = 2 * cand.y + ((i & 2) >> 1);
+          ch.z = 2 * cand.z + (i & 1);
+          stack.push_back(ch);
+        }
+      }
+    } else if (np > 0) {
+      nodes.push_back(cand);
+    }
+  }
+  return nodes;
+}
+
+// indexer.cpp:823-1109
+static void build_hierarchy(Indexer& ix, Node* node, std::shared_ptr<std::vector<uint8_t>> points,
+                            int64_t numPoints) {
+  if (numPoints < kLeafCap) {
+    node->indexStart = 0;
+    node->numPoints = numPoints;
+    node->points = points;
+    return;
+  }
+  const int64_t G = 32;
+  std::vector<int64_t> counters(G * G * G, 0);
+  V3 min = node->min, max = node->max, size = max - min;
+  int64_t bpp = ix.attributes.bytes;
+  V3 scale = ix.attributes.posScale, offset = ix.attributes.posOffset;
+  auto gridIndexOf = [&](int64_t i) -> int64_t {
+    const uint8_t* p = points->data() + i * bpp;
+    int32_t X, Y, Z;
+    memcpy(&X, p, 4); memcpy(&Y, p + 4, 4); memcpy(&Z, p + 8, 4);
+    double x = (X * scale.x) + offset.x, y = (Y * scale.y) + offset.y, z = (Z * scale.z) + offset.z;
+    int64_t cx = double(G) * (x - min.x) / size.x;
+    int64_t cy = double(G) * (y - min.y) / size.y;
+    int64_t cz = double(G) * (z - min.z) / size.z;
+    cx = std::max(int64_t(0), std::min(cx, G - 1));
+    cy = std::max(int64_t(0), std::min(cy, G - 1));
+    cz = std::max(int64_t(0), std::min(cz, G - 1));
+    return (int64_t)morton((uint32_t)cz, (uint32_t)cy, (uint32_t)cx);
+  };
+  for (int64_t i = 0; i < numPoints; i++) counters[gridIndexOf(i)]++;
+  {
+    std::vector<int64_t> offs(counters.size(), 0);
+    for (size_t i = 1; i < counters.size(); i++) offs[i] = offs[i - 1] + counters[i - 1];
+    std::vector<uint8_t> tmp(numPoints * bpp);
+    for (int64_t i = 0; i < numPoints; i++) {
+      int64_t t = offs[gridIndexOf(i)]++;
+      memcpy(tmp.data() + t * bpp, points->data() + i * bpp, bpp);
+    }
+    memcpy(points->data(), tmp.data(), numPoints * bpp);
+  }
+  auto pyr = sum_pyramid(counters, (int)G);
+  auto nodes = create_nodes(pyr);
+
+  auto expandTo = [node](Candidate& c) {
+    std::string full = node->name + c.name;
+    Node* cur = node;
+    for (size_t i = node->name.size(); i < full.size(); i++) {
+      int index = full[i] - '0';
+      if (!cur->children[index]) {
+        Box b = child_box(cur->min, cur->max, index);
+        auto ch = std::make_shared<Node>();
+        ch->min = b.min; ch->max = b.max;
+        ch->name = cur->name + std::to_string(index);
+        ch->children.resize(8);
+        cur->children[index] = ch;
+        cur = ch.get();
+      } else {
+        cur = cur->children[index].get();
+      }
+    }
+    return cur;
+  };
+
+  std::vector<Node*> refine;
+  int64_t depth = 0;
+  for (auto& c : nodes) {
+    Node* r = expandTo(c);
+    r->indexStart = c.indexStart;
+    r->numPoints = c.numPoints;
+    auto buf = std::make_shared<std::vector<uint8_t>>(points->begin() + c.indexStart * bpp,
+                                                      points->begin() + (c.indexStart + c.numPoints) * bpp);
+    r->points = buf;
+    if (r->numPoints > kLeafCap) refine.push_back(r);
+    depth = std::max(depth, r->level());
+  }
+  ix.octreeDepth = std::max(ix.octreeDepth, depth);
+
+  for (int64_t k = 0; k < (int64_t)refine.size(); k++) {
+    Node* subject = refine[k];
+    auto buffer = subject->points;
+    if (subject->numPoints == numPoints) {
+      // :1002-1098 duplicate handling; the distinct buffer copies the first
+      // distinct.size() records (:1088-1090), and the recursion below keeps
+      // using the previous buffer
+      std::unordered_map<std::string, int> counts;
+      for (int64_t i = 0; i < numPoints; i++) {
+        int32_t X, Y, Z;
+        memcpy(&X, buffer->data() + i * bpp, 4); memcpy(&Y, buffer->data() + i * bpp + 4, 4);
+        memcpy(&Z, buffer->data() + i * bpp + 8, 4);
+        counts[std::to_string(X) + ", " + std::to_string(Y) + ", " + std::to_string(Z)]++;
+      }
+      int64_t numDuplicates = subject->numPoints - (int64_t)counts.size();
+      if (numDuplicates >= kLeafCap / 2) {
+        std::vector<int64_t> distinct;
+        std::unordered_map<std::string, bool> handled;
+        for (int64_t i = 0; i < numPoints; i++) {
+          int32_t X, Y, Z;
+          memcpy(&X, buffer->data() + i * bpp, 4); memcpy(&Y, buffer->data() + i * bpp + 4, 4);
+          memcpy(&Z, buffer->data() + i * bpp + 8, 4);
+          std::string key = std::to_string(X) + ", " + std::to_string(Y) + ", " + std::to_string(Z);
+          if (counts.find(key) != counts.end()) {
+            if (handled.find(key) == handled.end()) { distinct.push_back(i); handled[key] = true; }
+          } else {
+            distinct.push_back(i);
+          }
+        }
+        auto db = std::make_shared<std::vector<uint8_t>>(buffer->begin(), buffer->begin() + distinct.size() * bpp);
+        subject->points = db;
+        subject->numPoints = (int64_t)distinct.size();
+        k--;
+      }
+    }
+    int64_t next = subject->numPoints;
+    subject->points = nullptr;
+    subject->numPoints = 0;
+    build_hierarchy(ix, subject, buffer, next);
+  }
+}
+
+// The point record used by both samplers (sampler_poisson.h:19-25,
+// sampler_poisson_average.h:19-33); identical element type so std::sort
+// visits the same sequence of comparisons as the reference.
+struct PPoint {
+  double x, y, z;
+  int32_t pointIndex, childIndex;
+};
+struct APoint {
+  double x, y, z;
+  int32_t pointIndex, childIndex;
+  int64_t r = 0, g = 0, b = 0, w = 0;
+  int64_t mainIndex = 0;
+  bool accepted = false;
+};
+
+template <class P>
+static void post_order_unsampled(Node* n, const std::function<void(Node*)>& cb) {
+  for (auto& c : n->children)
+    if (c && !c->sampled) post_order_unsampled<P>(c.get(), cb);
+  cb(n);
+}
+
+// SamplerPoisson::sample (sampler_poisson.h:14-304)
+static void sample_poisson(Node* node, Attrs& attributes, double baseSpacing, const NodeCB& onCompleted,
+                           const NodeCB& onDiscarded) {
+  post_order_unsampled<PPoint>(node, [&](Node* n) {
+    n->sampled = true;
+    if (n->isLeaf()) return;
+    V3 scale = attributes.posScale, offset = attributes.posOffset;
+    int64_t bpp = attributes.bytes;
+    int64_t total = 0;
+    for (auto& c : n->children) if (c) total += c->numPoints;
+    std::vector<PPoint> points;
+    points.reserve(total);
+    std::vector<std::vector<int8_t>> flags(8);
+    std::vector<int64_t> numRejected(8, 0);
+    int64_t numAccepted = 0;
+    for (int ci = 0; ci < 8; ci++) {
+      auto c = n->children[ci];
+      if (!c) continue;
+      flags[ci].assign(c->numPoints, 0);
+      for (int64_t i = 0; i < c->numPoints; i++) {
+        int32_t X, Y, Z;
+        const uint8_t* p = c->points->data() + i * bpp;
+        memcpy(&X, p, 4); memcpy(&Y, p + 4, 4); memcpy(&Z, p + 8, 4);
+        PPoint q = {(X * scale.x) + offset.x, (Y * scale.y) + offset.y, (Z * scale.z) + offset.z, (int32_t)i, ci};
+        points.push_back(q);
+      }
+    }
+    std::vector<PPoint> accepted;  // the reference keeps 1'000'000 slots (:112)
+    double spacing = baseSpacing / pow(2.0, (double)n->level());
+    double squaredSpacing = spacing * spacing;
+    V3 center = (n->min + n->max) * 0.5;
+    auto checkAccept = [&](const PPoint& c) {
+      double cx = c.x - center.x, cy = c.y - center.y, cz = c.z - center.z;
+      double cdd = cx * cx + cy * cy + cz * cz;
+      double cd = sqrt(cdd);
+      double limit = cd - spacing;
+      double limitSquared = limit * limit;
+      int64_t j = 0;
+      for (int64_t i = (int64_t)accepted.size() - 1; i >= 0; i--) {
+        const PPoint& p = accepted[i];
+        double px = p.x - center.x, py = p.y - center.y, pz = p.z - center.z;
+        double pdd = px * px + py * py + pz * pz;
+        if (pdd < limitSquared) return true;
+        double dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z;
+        double dd = dx * dx + dy * dy + dz * dz;
+        if (dd < squaredSpacing) return false;
+        j++;
+        if (j > 10000) return true;
+      }
+      return true;
+    };
+    std::sort(points.begin(), points.end(), [center](PPoint a, PPoint b) -> bool {
+      double ax = a.x - center.x, ay = a.y - center.y, az = a.z - center.z;
+      double add = ax * ax + ay * ay + az * az;
+      double bx = b.x - center.x, by = b.y - center.y, bz = b.z - center.z;
+      double bdd = bx * bx + by * by + bz * bz;
+      return add < bdd;
+    });
+    for (const PPoint& p : points) {
+      bool acc = checkAccept(p);
+      if (acc) { accepted.push_back(p); numAccepted++; } else { numRejected[p.childIndex]++; }
+      flags[p.childIndex][p.pointIndex] = acc ? 1 : 0;
+    }
+    auto acc = std::make_shared<std::vector<uint8_t>>();
+    acc->reserve(numAccepted * bpp);
+    for (int ci = 0; ci < 8; ci++) {
+      auto c = n->children[ci];
+      if (!c) continue;
+      auto rej = std::make_shared<std::vector<uint8_t>>();
+      rej->reserve(numRejected[ci] * bpp);
+      for (int64_t i = 0; i < c->numPoints; i++) {
+        const uint8_t* p = c->points->data() + i * bpp;
+        if (flags[ci][i]) acc->insert(acc->end(), p, p + bpp); else rej->insert(rej->end(), p, p + bpp);
+      }
+      // :271-290 (note the missing else: a discarded leaf is also completed)
+      if (numRejected[ci] == 0 && c->isLeaf()) {
+        onDiscarded(c.get());
+        n->children[ci] = nullptr;
+      }
+      if (numRejected[ci] > 0) {
+        c->points = rej;
+        c->numPoints = numRejected[ci];
+        onCompleted(c.get());
+      } else if (numRejected[ci] == 0) {
+        c->points = nullptr;
+        c->numPoints = 0;
+        onCompleted(c.get());
+      }
+    }
+    n->points = acc;
+    n->numPoints = numAccepted;
+  });
+}
+
+// SamplerPoissonAverage::sample (sampler_poisson_average.h:14-492)
+static void sample_poisson_average(Node* node, Attrs& attributes, double baseSpacing, const NodeCB& onCompleted,
+                                   const NodeCB&) {
+  post_order_unsampled<APoint>(node, [&](Node* n) {
+    n->sampled = true;
+    if (n->isLeaf()) return;
+    V3 scale = attributes.posScale, offset = attributes.posOffset;
+    int64_t bpp = attributes.bytes;
+    int offsetRGB = attributes.offsetOf("rgb");
+    V3 min = n->min, max = n->max, size = max - min;
+    int total = 0;
+    for (auto& c : n->children) if (c) total += (int)c->numPoints;
+    std::vector<APoint> points;
+    points.reserve(total);
+    std::vector<std::vector<int8_t>> flags(8);
+    std::vector<int64_t> numRejected(8, 0);
+    int64_t numAccepted = 0;
+    for (int ci = 0; ci < 8; ci++) {
+      auto c = n->children[ci];
+      if (!c) continue;
+      flags[ci].assign(c->numPoints, 0);
+      for (int i = 0; i < (int)c->numPoints; i++) {
+        const uint8_t* p = c->points->data() + (int64_t)i * bpp;
+        int32_t X, Y, Z;
+        memcpy(&X, p, 4); memcpy(&Y, p + 4, 4); memcpy(&Z, p + 8, 4);
+        APoint q;
+        q.x = (X * scale.x) + offset.x; q.y = (Y * scale.y) + offset.y; q.z = (Z * scale.z) + offset.z;
+        q.pointIndex = i; q.childIndex = ci;
+        uint16_t rgb[3];
+        memcpy(rgb, p + offsetRGB, 6);
+        q.r = rgb[0]; q.g = rgb[1]; q.b = rgb[2]; q.w = 1;
+        q.mainIndex = (int64_t)points.size();
+        points.push_back(q);
+      }
+    }
+    double spacing = baseSpacing / pow(2.0, (double)n->level());
+    double squaredSpacing = spacing * spacing;
+    V3 center = (n->min + n->max) * 0.5;
+    const double gs = 16;  // :181
+    std::vector<std::vector<APoint>> grid((size_t)(gs * gs * gs));
+    auto range = [&](const APoint& c, int& ix, int& iy, int& iz, int& x0, int& y0, int& z0, int& x1, int& y1, int& z1) {
+      double dx = gs * (c.x - min.x) / size.x, dy = gs * (c.y - min.y) / size.y, dz = gs * (c.z - min.z) / size.z;
+      double dx0 = gs * (c.x - spacing - min.x) / size.x, dy0 = gs * (c.y - spacing - min.y) / size.y;
+      double dz0 = gs * (c.z - spacing - min.z) / size.z;
+      double dx1 = gs * (c.x + spacing - min.x) / size.x, dy1 = gs * (c.y + spacing - min.y) / size.y;
+      double dz1 = gs * (c.z + spacing - min.z) / size.z;
+      ix = (int)std::max(std::min(dx, gs - 1.0), 0.0); iy = (int)std::max(std::min(dy, gs - 1.0), 0.0);
+      iz = (int)std::max(std::min(dz, gs - 1.0), 0.0);
+      x0 = (int)std::max(std::min(dx0, gs - 1.0), 0.0); y0 = (int)std::max(std::min(dy0, gs - 1.0), 0.0);
+      z0 = (int)std::max(std::min(dz0, gs - 1.0), 0.0);
+      x1 = (int)std::max(std::min(dx1, gs - 1.0), 0.0); y1 = (int)std::max(std::min(dy1, gs - 1.0), 0.0);
+      z1 = (int)std::max(std::min(dz1, gs - 1.0), 0.0);
+    };
+    auto sqd = [](const APoint& a, const APoint& b) {
+      double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+      return dx * dx + dy * dy + dz * dz;
+    };
+    auto checkAccept = [&](const APoint& c) {
+      int ix, iy, iz, x0, y0, z0, x1, y1, z1;
+      range(c, ix, iy, iz, x0, y0, z0, x1, y1, z1);
+      for (int x = x0; x <= x1; x++)
+        for (int y = y0; y <= y1; y++)
+          for (int z = z0; z <= z1; z++) {
+            for (const APoint& p : grid[x + y * (int)gs + z * (int)gs * (int)gs])
+              if (sqd(p, c) < squaredSpacing) return false;
+          }
+      grid[ix + iy * (int)gs + iz * (int)gs * (int)gs].push_back(c);
+      return true;
+    };
+    std::sort(points.begin(), points.end(), [center](APoint a, APoint b) -> bool {
+      double ax = a.x - center.x, ay = a.y - center.y, az = a.z - center.z;
+      double add = ax * ax + ay * ay + az * az;
+      double bx = b.x - center.x, by = b.y - center.y, bz = b.z - center.z;
+      double bdd = bx * bx + by * by + bz * bz;
+      return add < bdd;
+    });
+    std::vector<int32_t> mainToSort(points.size());
+    for (size_t i = 0; i < points.size(); i++) mainToSort[points[i].mainIndex] = (int32_t)i;
+    for (APoint& p : points) {
+      bool acc = checkAccept(p);
+      if (acc) numAccepted++; else numRejected[p.childIndex]++;
+      flags[p.childIndex][p.pointIndex] = acc ? 1 : 0;
+    }
+    // :357-416 colour averaging in sorted order
+    for (APoint& c : points) {
+      int ix, iy, iz, x0, y0, z0, x1, y1, z1;
+      range(c, ix, iy, iz, x0, y0, z0, x1, y1, z1);
+      for (int x = x0; x <= x1; x++)
+        for (int y = y0; y <= y1; y++)
+          for (int z = z0; z <= z1; z++) {
+            for (const APoint& p : grid[x + y * (int)gs + z * (int)gs * (int)gs]) {
+              if (sqd(p, c) < squaredSpacing) {
+                APoint& a = points[mainToSort[p.mainIndex]];
+                a.r = a.r + c.r; a.g = a.g + c.g; a.b = a.b + c.b; a.w = a.w + c.w;
+              }
+            }
+          }
+    }
+    // :418-482
+    auto acc = std::make_shared<std::vector<uint8_t>>();
+    acc->reserve(numAccepted * bpp);
+    size_t j = 0;
+    for (int ci = 0; ci < 8; ci++) {
+      auto c = n->children[ci];
+      if (!c) continue;
+      auto rej = std::make_shared<std::vector<uint8_t>>();
+      rej->reserve(c->numPoints * bpp);
+      for (int i = 0; i < (int)c->numPoints; i++) {
+        uint8_t* rec = c->points->data() + (int64_t)i * bpp;
+        APoint& p = points[mainToSort[j]];
+        uint16_t t[3] = {(uint16_t)(p.r / p.w), (uint16_t)(p.g / p.w), (uint16_t)(p.b / p.w)};
+        memcpy(rec + offsetRGB, t, 6);
+        if (flags[ci][i]) acc->insert(acc->end(), rec, rec + bpp);
+        rej->insert(rej->end(), rec, rec + bpp);
+        j++;
+      }
+      if (numRejected[ci] == 0) n->children[ci] = nullptr;
+      if (numRejected[ci] > 0) {
+        c->points = rej;
+        onCompleted(c.get());
+      }
+    }
+    n->points = acc;
+    n->numPoints = numAccepted;
+  });
+}
+
+struct CRNode {  // indexer.h:258-301
+  std::string name;
+  Node* node = nullptr;
+  std::vector<std::shared_ptr<CRNode>> children = std::vector<std::shared_ptr<CRNode>>(8);
+  std::vector<int> fcrs;  // indices into Indexer::flushed
+  int numPoints = 0;
+  bool isLeaf() const { for (auto& c : children) if (c) return false; return true; }
+};
+
+static void cr_post(CRNode* n, const std::function<void(CRNode*)>& cb) {
+  for (auto& c : n->children) if (c) cr_post(c.get(), cb);
+  cb(n);
+}
+static void cr_pre(CRNode* n, const std::function<void(CRNode*)>& cb) {
+  cb(n);
+  for (auto& c : n->children) if (c) cr_pre(c.get(), cb);
+}
+
+// indexer.cpp:202-281
+static std::vector<CRNode> process_chunk_roots(Indexer& ix) {
+  std::unordered_map<std::string, std::shared_ptr<CRNode>> map;
+  std::vector<std::shared_ptr<CRNode>> list;
+  ix.root->traverse([&](Node* n) {
+    auto c = std::make_shared<CRNode>();
+    c->name = n->name;
+    c->node = n;
+    c->children.resize(n->children.size());
+    list.push_back(c);
+    map[c->name] = c;
+  });
+  for (auto& c : list) {
+    std::string parent = c->name.substr(0, c->name.size() - 1);
+    if (parent != "") map[parent]->children[c->name.back() - '0'] = c;
+  }
+  for (size_t i = 0; i < ix.flushed.size(); i++) {
+    auto c = map[ix.flushed[i].node->name];
+    c->fcrs.push_back((int)i);
+    c->numPoints += (int)ix.flushed[i].node->numPoints;
+  }
+  auto crRoot = map["r"];
+  const int64_t threshold = 5000000;
+  cr_post(crRoot.get(), [&](CRNode* n) {
+    if (n->isLeaf()) return;
+    int np = 0;
+    for (auto& c : n->children) if (c) np += c->numPoints;
+    n->numPoints = np;
+    if (n->numPoints < threshold) {
+      for (auto& c : n->children) if (c) n->fcrs.insert(n->fcrs.end(), c->fcrs.begin(), c->fcrs.end());
+      n->children.clear();
+    }
+  });
+  std::vector<CRNode> tasks;
+  cr_pre(crRoot.get(), [&](CRNode* n) { if (!n->fcrs.empty()) tasks.push_back(*n); });
+  return tasks;
+}
+
+// HierarchyBuilder.h:26-354 over the HierarchyFlusher groups (indexer.h:171-241)
+struct HNode {
+  std::string name;
+  int numPoints = 0;
+  uint8_t childMask = 0;
+  uint64_t byteOffset = 0, byteSize = 0;
+  int type = 1;  // NORMAL 0, LEAF 1, PROXY 2
+  uint64_t proxyByteOffset = 0, proxyByteSize = 0;
+};
+struct HChunk { std::string name; std::vector<std::shared_ptr<HNode>> nodes; int64_t byteOffset = 0; };
+struct HBatch {
+  std::string name;
+  int64_t byteSize = 0;
+  std::vector<std::shared_ptr<HNode>> nodes;
+  std::vector<std::shared_ptr<HChunk>> chunks;
+  std::unordered_map<std::string, std::shared_ptr<HNode>> nodeMap;
+  std::unordered_map<std::string, std::shared_ptr<HChunk>> chunkMap;
+};
+
+static bool bf_less(const std::string& a, const std::string& b) {
+  if (a.size() != b.size()) return a.size() < b.size();
+  return a < b;
+}
+
+static const int kStep = 4;  // indexer.cpp:19
+
+static std::shared_ptr<HBatch> load_batch(const std::string& name, const std::vector<Indexer::HRec>& recs) {
+  auto batch = std::make_shared<HBatch>();
+  batch->name = name;
+  for (auto& r : recs) {
+    auto node = std::make_shared<HNode>();
+    // the 48-byte record keeps 31 name bytes, u32 numPoints, i32 byteSize
+    node->name = r.name.substr(0, 31);
+    node->numPoints = (int)(uint32_t)r.numPoints;
+    node->byteOffset = (uint64_t)r.byteOffset;
+    node->byteSize = (uint64_t)(int64_t)(int32_t)r.byteSize;
+    int chunkLevel = ((int)node->name.size() - 2) / 4;
+    std::string key = node->name.substr(0, kStep * chunkLevel + 1);
+    if (node->name == batch->name) key = node->name;
+    if (batch->chunkMap.find(key) == batch->chunkMap.end()) {
+      auto c = std::make_shared<HChunk>();
+      c->name = key;
+      batch->chunkMap[key] = c;
+      batch->chunks.push_back(c);
+    }
+    batch->chunkMap[key]->nodes.push_back(node);
+    batch->nodes.push_back(node);
+    batch->nodeMap[node->name] = batch->nodes.back();
+    bool isChunkKey = ((node->name.size() - 1) % kStep) == 0;
+    bool isBatchSubChunk = node->name.size() > kStep + 1;
+    if (isChunkKey && isBatchSubChunk) {
+      if (batch->chunkMap.find(node->name) == batch->chunkMap.end()) {
+        auto c = std::make_shared<HChunk>();
+        c->name = node->name;
+        batch->chunkMap[node->name] = c;
+        batch->chunks.push_back(c);
+      }
+      batch->chunkMap[node->name]->nodes.push_back(node);
+    }
+  }
+  std::sort(batch->chunks.begin(), batch->chunks.end(),
+            [](const std::shared_ptr<HChunk>& a, const std::shared_ptr<HChunk>& b) { return bf_less(a->name, b->name); });
+  for (auto& node : batch->nodes) {
+    node->type = 1;
+    std::string parent = node->name.substr(0, node->name.size() - 1);
+    auto it = batch->nodeMap.find(parent);
+    if (it != batch->nodeMap.end()) {
+      int ci = node->name.back() - '0';
+      it->second->type = 0;
+      it->second->childMask = it->second->childMask | (1 << ci);
+    }
+  }
+  for (auto& c : batch->chunks) {
+    auto it = batch->nodeMap.find(c->name);
+    if (it != batch->nodeMap.end()) it->second->type = 2;
+  }
+  for (auto& c : batch->chunks)
+    std::sort(c->nodes.begin(), c->nodes.end(),
+              [](const std::shared_ptr<HNode>& a, const std::shared_ptr<HNode>& b) { return bf_less(a->name, b->name); });
+  return batch;
+}
+
+static void process_batch(HBatch& b) {
+  int64_t off = 0;
+  for (auto& c : b.chunks) {
+    c->byteOffset = off;
+    if (c->name != b.name) {
+      auto proxy = b.nodeMap[c->name];
+      proxy->type = 2;
+      proxy->proxyByteOffset = c->byteOffset;
+      proxy->proxyByteSize = 22 * c->nodes.size();
+    }
+    off += 22 * c->nodes.size();
+  }
+  b.byteSize = off;
+}
+
+static std::vector<uint8_t> serialize_batch(HBatch& b, int64_t bytesWritten) {
+  size_t n = 0;
+  for (auto& c : b.chunks) n += c->nodes.size();
+  std::vector<uint8_t> out(22 * n);
+  size_t k = 0;
+  for (auto& c : b.chunks) {
+    for (auto& node : c->nodes) {
+      bool isProxy = node->type == 2 && node->name != c->name;
+      uint8_t type = (uint8_t)node->type;
+      if (node->type == 2 && !isProxy) type = 0;
+      uint64_t bs = isProxy ? node->proxyByteSize : node->byteSize;
+      uint64_t bo = isProxy ? bytesWritten + node->proxyByteOffset : node->byteOffset;
+      uint8_t* d = out.data() + 22 * k;
+      d[0] = type;
+      d[1] = node->childMask;
+      uint32_t np = (uint32_t)node->numPoints;
+      memcpy(d + 2, &np, 4);
+      memcpy(d + 6, &bo, 8);
+      memcpy(d + 14, &bs, 8);
+      k++;
+    }
+  }
+  b.byteSize = (int64_t)out.size();
+  return out;
+}
+
+static std::vector<uint8_t> build_hierarchy_file(const std::vector<Indexer::HRec>& recs, int64_t& firstChunkSize) {
+  std::map<std::string, std::vector<Indexer::HRec>> groups;  // ascending name = canonical batch order
+  for (auto& r : recs) {
+    std::string key = r.name.substr(0, kStep + 1);
+    if (r.name.size() <= (size_t)kStep + 1) key = "r";
+    groups[key].push_back(r);
+    if (r.name.size() == (size_t)kStep + 1) groups[r.name].push_back(r);
+  }
+  auto root = load_batch("r", groups["r"]);
+  std::vector<uint8_t> file(22 * root->nodes.size(), 0);
+  int64_t written = (int64_t)file.size();
+  for (auto& [name, list] : groups) {
+    if (name == "r") continue;
+    auto b = load_batch(name, list);
+    process_batch(*b);
+    auto buf = serialize_batch(*b, written);
+    if (b->nodes.size() > 1) {
+      auto proxy = root->nodeMap[b->name];
+      proxy->type = 2;
+      proxy->proxyByteOffset = written;
+      proxy->proxyByteSize = 22 * b->chunkMap[b->name]->nodes.size();
+    } else {
+      root->nodeMap[b->name]->type = 1;
+    }
+    file.insert(file.end(), buf.begin(), buf.end());
+    written += (int64_t)buf.size();
+  }
+  auto rb = serialize_batch(*root, 0);
+  memcpy(file.data(), rb.data(), rb.size());
+  firstChunkSize = root->byteSize;
+  return file;
+}
+
+// indexer.cpp:354-524
+static std::string create_metadata(const Indexer& ix, const std::string& name, int64_t pointsTotal,
+                                   int64_t firstChunkSize) {
+  auto d = [](double v) { return d17(v); };
+  auto toJson = [&](V3 v) { return "[" + d(v.x) + ", " + d(v.y) + ", " + d(v.z) + "]"; };
+  auto t = [](int n) { return std::string(n, '\t'); };
+  auto s = [](const std::string& str) { return "\"" + str + "\""; };
+  std::stringstream ss;
+  ss << "{" << "\n";
+  ss << t(1) << s("version") << ": " << s("2.0") << "," << "\n";
+  ss << t(1) << s("name") << ": " << s(name) << "," << "\n";
+  ss << t(1) << s("description") << ": " << s("") << "," << "\n";
+  ss << t(1) << s("points") << ": " << pointsTotal << "," << "\n";
+  ss << t(1) << s("projection") << ": " << s("") << "," << "\n";
+  ss << t(1) << s("hierarchy") << ": " << "{" << "\n";
+  ss << t(2) << s("firstChunkSize") << ": " << firstChunkSize << ", " << "\n";
+  ss << t(2) << s("stepSize") << ": " << kStep << ", " << "\n";
+  ss << t(2) << s("depth") << ": " << ix.octreeDepth << "\n";
+  ss << t(1) << "}" << "," << "\n";
+  ss << t(1) << s("offset") << ": " << toJson(ix.attributes.posOffset) << "," << "\n";
+  ss << t(1) << s("scale") << ": " << toJson(ix.attributes.posScale) << "," << "\n";
+  ss << t(1) << s("spacing") << ": " << d(ix.spacing) << "," << "\n";
+  ss << t(1) << s("boundingBox") << ": " << "{" << "\n";
+  ss << t(2) << s("min") << ": " << toJson(ix.root->min) << ", " << "\n";
+  ss << t(2) << s("max") << ": " << toJson(ix.root->max) << "\n";
+  ss << t(1) << "}" << "," << "\n";
+  ss << t(1) << s("encoding") << ": " << s("DEFAULT") << "," << "\n";
+  ss << t(1) << s("attributes") << ": " << "[" << "\n";
+  const auto& L = ix.attributes.list;
+  for (size_t i = 0; i < L.size(); i++) {
+    const Attr& a = L[i];
+    if (i == 0) ss << t(2) << "{" << "\n";
+    ss << t(3) << s("name") << ": " << s(a.name) << "," << "\n";
+    ss << t(3) << s("description") << ": " << s(a.description) << "," << "\n";
+    ss << t(3) << s("size") << ": " << a.size << "," << "\n";
+    ss << t(3) << s("numElements") << ": " << a.numElements << "," << "\n";
+    ss << t(3) << s("elementSize") << ": " << a.elementSize << "," << "\n";
+    ss << t(3) << s("type") << ": " << s(type_name(a.type)) << "," << "\n";
+    bool empty = true;
+    for (auto h : a.histogram) if (h != 0) empty = false;
+    if (a.size == 1 && !empty) {
+      ss << t(3) << s("histogram") << ": [";
+      for (size_t k = 0; k < a.histogram.size(); k++) ss << a.histogram[k] << (k + 1 < a.histogram.size() ? ", " : "");
+      ss << "], " << "\n";
+    }
+    auto vec = [&](std::vector<double> v) {
+      std::string r = "[";
+      for (size_t k = 0; k < v.size(); k++) r += d(v[k]) + (k + 1 < v.size() ? ", " : "");
+      return r + "]";
+    };
+    if (a.numElements == 1) {
+      ss << t(3) << s("min") << ": " << vec({a.min.x}) << "," << "\n";
+      ss << t(3) << s("max") << ": " << vec({a.max.x}) << "," << "\n";
+      ss << t(3) << s("scale") << ": " << vec({a.scale.x}) << "," << "\n";
+      ss << t(3) << s("offset") << ": " << vec({a.offset.x}) << "\n";
+    } else if (a.numElements == 2) {
+      ss << t(3) << s("min") << ": " << vec({a.min.x, a.min.y}) << "," << "\n";
+      ss << t(3) << s("max") << ": " << vec({a.max.x, a.max.y}) << "," << "\n";
+      ss << t(3) << s("scale") << ": " << vec({a.scale.x, a.scale.y}) << "," << "\n";
+      ss << t(3) << s("offset") << ": " << vec({a.offset.x, a.offset.y}) << "\n";
+    } else if (a.numElements == 3) {
+      ss << t(3) << s("min") << ": " << vec({a.min.x, a.min.y, a.min.z}) << "," << "\n";
+      ss << t(3) << s("max") << ": " << vec({a.max.x, a.max.y, a.max.z}) << "," << "\n";
+      ss << t(3) << s("scale") << ": " << vec({a.scale.x, a.scale.y, a.scale.z}) << "," << "\n";
+      ss << t(3) << s("offset") << ": " << vec({a.offset.x, a.offset.y, a.offset.z}) << "\n";
+    }
+    if (i + 1 < L.size()) ss << t(2) << "},{" << "\n"; else ss << t(2) << "}" << "\n";
+  }
+  ss << t(1) << "]" << "\n";
+  ss << "}" << "\n";
+  return ss.str();
+}
+
+// metadata round trip of chunks/metadata.json: non-finite min/max were
+// written as null and read back as +inf (indexer.cpp:106-123)
+static void json_roundtrip(Attrs& a) {
+  for (auto& t : a.list) {
+    auto fix = [](double& v) { if (!std::isfinite(v)) v = INF; };
+    fix(t.min.x); fix(t.min.y); fix(t.min.z); fix(t.max.x); fix(t.max.y); fix(t.max.z);
+  }
+}
+
+struct ChunkIn {
+  std::string id;
+  V3 min, max;
+  std::shared_ptr<std::vector<uint8_t>> points;
+};
+
+// doIndexing (indexer.cpp:1581-1795), canonical schedule.
+static int do_indexing(std::vector<ChunkIn>& chunks, Attrs attributes, V3 min, V3 max, const std::string& method,
+                       const std::string& name, int64_t pointsTotal, const std::string& outdir, double* tIndex) {
+  double t0 = now_s();
+  Indexer ix;
+  json_roundtrip(attributes);
+  ix.attributes = attributes;
+  ix.root = std::make_shared<Node>("r", min, max);
+  ix.spacing = (max - min).x / 128.0;
+  bool avg = method == "poisson_average";
+  if (!avg && method != "poisson") { g_error = "unknown method " + method; return 1; }
+  NodeCB onCompleted = [&](Node* n) {
+    // Writer::writeAndUnload (indexer.cpp:1453-1516), DEFAULT encoding
+    if (n->numPoints != 0) {
+      int64_t bs = (int64_t)n->points->size();
+      n->byteSize = bs;
+      n->byteOffset = ix.byteOffset;
+      ix.byteOffset += bs;
+      ix.octree.insert(ix.octree.end(), n->points->begin(), n->points->end());
+      n->points = nullptr;
+    }
+    ix.hrecords.push_back({n->name, n->byteOffset, n->byteSize, n->numPoints});
+  };
+  NodeCB onDiscarded = [](Node*) {};
+  auto sample = [&](Node* n) {
+    if (avg) sample_poisson_average(n, ix.attributes, ix.spacing, onCompleted, onDiscarded);
+    else sample_poisson(n, ix.attributes, ix.spacing, onCompleted, onDiscarded);
+  };
+  std::sort(chunks.begin(), chunks.end(), [](const ChunkIn& a, const ChunkIn& b) { return a.id < b.id; });
+  std::vector<std::shared_ptr<Node>> nodes;
+  int64_t bpp = attributes.bytes;
+  for (auto& ch : chunks) {
+    auto chunkRoot = std::make_shared<Node>(ch.id, ch.min, ch.max);
+    int64_t np = (int64_t)ch.points->size() / bpp;
+    build_hierarchy(ix, chunkRoot.get(), ch.points, np);
+    sample(chunkRoot.get());
+    chunkRoot->children.clear();
+    ix.flushed.push_back({chunkRoot, *chunkRoot->points});
+    chunkRoot->points = nullptr;
+    if (chunkRoot->name.size() > 1) ix.root->addDescendant(chunkRoot);
+    nodes.push_back(chunkRoot);
+  }
+  auto tasks = process_chunk_roots(ix);
+  for (auto& task : tasks) {
+    for (int f : task.fcrs) ix.flushed[f].node->points = std::make_shared<std::vector<uint8_t>>(ix.flushed[f].points);
+    sample(task.node);
+    task.node->children.clear();
+  }
+  if (chunks.size() == 1) ix.root = nodes[0];
+  else if (!ix.root->sampled) sample(ix.root.get());
+  onCompleted(ix.root.get());
+
+  int64_t firstChunkSize = 0;
+  auto hier = build_hierarchy_file(ix.hrecords, firstChunkSize);
+  std::string meta = create_metadata(ix, name, pointsTotal, firstChunkSize);
+  if (tIndex) *tIndex = now_s() - t0;
+  fs::create_directories(outdir);
+  { std::ofstream f(outdir + "/octree.bin", std::ios::binary); f.write((const char*)ix.octree.data(), ix.octree.size()); }
+  { std::ofstream f(outdir + "/hierarchy.bin", std::ios::binary); f.write((const char*)hier.data(), hier.size()); }
+  { std::ofstream f(outdir + "/metadata.json", std::ios::binary); f << meta; }
+  return 0;
+}
+
+// getChunks (indexer.cpp:53-179): chunk boxes from the ID digit chain
+static Box chunk_box(const std::string& id, V3 min, V3 max) {
+  Box b = {min, max};
+  for (size_t i = 1; i < id.size(); i++) b = child_box(b.min, b.max, id[i] - '0');
+  return b;
+}
+
+// a tiny JSON reader sufficient for chunks/metadata.json as written above
+struct JVal {
+  enum K { NUL, NUM, STR, ARR, OBJ } k = NUL;
+  double num = 0;
+  std::string str;
+  std::vector<JVal> arr;
+  std::vector<std::pair<std::string, JVal>> obj;
+  const JVal* at(const std::string& key) const {
+    for (auto& kv : obj) if (kv.first == key) return &kv.second;
+    return nullptr;
+  }
+};
+static void jws(const char*& p) { while (*p == ' ' || *p == '\n' || *p == '\t' || *p == '\r') p++; }
+static JVal jparse(const char*& p) {
+  jws(p);
+  JVal v;
+  if (*p == '{') {
+    v.k = JVal::OBJ; p++;
+    jws(p);
+    if (*p == '}') { p++; return v; }
+    while (true) {
+      JVal key = jparse(p);
+      jws(p); p++;  // ':'
+      v.obj.push_back({key.str, jparse(p)});
+      jws(p);
+      if (*p == ',') { p++; continue; }
+      p++; break;
+    }
+  } else if (*p == '[') {
+    v.k = JVal::ARR; p++;
+    jws(p);
+    if (*p == ']') { p++; return v; }
+    while (true) {
+      v.arr.push_back(jparse(p));
+      jws(p);
+      if (*p == ',') { p++; continue; }
+      p++; break;
+    }
+  } else if (*p == '"') {
+    v.k = JVal::STR; p++;
+    while (*p != '"') v.str += *p++;
+    p++;
+  } else if (!strncmp(p, "null", 4)) {
+    v.k = JVal::NUL; p += 4;
+  } else {
+    v.k = JVal::NUM;
+    char* e;
+    v.num = strtod(p, &e);
+    p = e;
+  }
+  return v;
+}
+
+static bool read_chunks(const std::string& outdir, std::vector<ChunkIn>& chunks, Attrs& attrs, V3& min, V3& max) {
+  std::ifstream f(outdir + "/chunks/metadata.json");
+  if (!f) { g_error = "missing chunks/metadata.json"; return false; }
+  std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+  const char* p = text.c_str();
+  JVal js = jparse(p);
+  auto v3 = [](const JVal* a) { return V3(a->arr[0].num, a->arr[1].num, a->arr[2].num); };
+  min = v3(js.at("min"));
+  max = v3(js.at("max"));
+  attrs.list.clear();
+  for (auto& ja : js.at("attributes")->arr) {
+    Attr a(ja.at("name")->str, (int)ja.at("size")->num, (int)ja.at("numElements")->num,
+           (int)ja.at("elementSize")->num, type_from_name(ja.at("type")->str));
+    a.description = ja.at("description")->str;
+    if (auto h = ja.at("histogram")) for (size_t i = 0; i < h->arr.size(); i++) a.histogram[i] = (int64_t)h->arr[i].num;
+    auto comp = [](const JVal* arr, int i, double dflt) {
+      return arr->arr[i].k == JVal::NUL ? dflt : arr->arr[i].num;
+    };
+    const JVal *mn = ja.at("min"), *mx = ja.at("max"), *sc = ja.at("scale"), *of = ja.at("offset");
+    double* mins[3] = {&a.min.x, &a.min.y, &a.min.z};
+    double* maxs[3] = {&a.max.x, &a.max.y, &a.max.z};
+    double* scs[3] = {&a.scale.x, &a.scale.y, &a.scale.z};
+    double* ofs[3] = {&a.offset.x, &a.offset.y, &a.offset.z};
+    for (int i = 0; i < a.numElements && i < 3; i++) {
+      *mins[i] = comp(mn, i, INF);
+      *maxs[i] = comp(mx, i, INF);
+      *scs[i] = comp(sc, i, 1.0);
+      *ofs[i] = comp(of, i, 0.0);
+    }
+    attrs.list.push_back(a);
+  }
+  attrs.finish();
+  attrs.posScale = v3(js.at("scale"));
+  attrs.posOffset = v3(js.at("offset"));
+  for (auto& e : fs::directory_iterator(outdir + "/chunks")) {
+    std::string fn = e.path().filename().string();
+    if (fn.size() < 4 || fn.substr(fn.size() - 4) != ".bin") continue;
+    ChunkIn c;
+    c.id = fn.substr(0, fn.size() - 4);
+    Box b = chunk_box(c.id, min, max);
+    c.min = b.min; c.max = b.max;
+    std::ifstream cf(e.path(), std::ios::binary);
+    c.points = std::make_shared<std::vector<uint8_t>>((std::istreambuf_iterator<char>(cf)), std::istreambuf_iterator<char>());
+    chunks.push_back(c);
+  }
+  return true;
+}
+
+}  // namespace orc
+
+extern "C" int orc_convert(const char* las_path, const char* outdir, const char* method, int32_t keep_chunks,
+                           int32_t chunk_only, double* timings) {
+  ChunkerOut co;
+  int rc = do_chunking(las_path, co);
+  if (rc) return rc;
+  std::string od = outdir;
+  fs::create_directories(od);
+  if (keep_chunks || chunk_only) {
+    fs::create_directories(od + "/chunks");
+    for (auto& e : fs::directory_iterator(od + "/chunks")) fs::remove(e);
+    for (size_t c = 0; c < co.chunks.size(); c++) {
+      if (co.records[c].empty()) continue;
+      std::ofstream f(od + "/chunks/" + co.chunks[c].id + ".bin", std::ios::binary);
+      f.write((const char*)co.records[c].data(), co.records[c].size());
+    }
+    write_chunk_metadata(od + "/chunks/metadata.json", co.min, co.max, co.attrs);
+  }
+  if (timings) { timings[0] = co.tCount; timings[1] = co.tDistribute; timings[2] = co.tTotal; timings[3] = 0; }
+  if (chunk_only) return 0;
+  std::vector<ChunkIn> chunks;
+  for (size_t c = 0; c < co.chunks.size(); c++) {
+    if (co.records[c].empty()) continue;
+    ChunkIn ci;
+    ci.id = co.chunks[c].id;
+    Box b = chunk_box(ci.id, co.min, co.max);
+    ci.min = b.min; ci.max = b.max;
+    ci.points = std::make_shared<std::vector<uint8_t>>(std::move(co.records[c]));
+    chunks.push_back(ci);
+  }
+  lasfile::Header h;
+  lasfile::read_header(las_path, h, nullptr);
+  std::string name = fs::path(las_path).stem().string();
+  double tIndex = 0;
+  rc = do_indexing(chunks, co.attrs, co.min, co.max, method, name, (int64_t)h.num_points, od, &tIndex);
+  if (timings) timings[3] = tIndex;
+  return rc;
+}
+
+extern "C" int orc_index(const char* outdir, const char* method, const char* name, double* timings) {
+  std::vector<ChunkIn> chunks;
+  Attrs attrs;
+  V3 min, max;
+  if (!read_chunks(outdir, chunks, attrs, min, max)) return 1;
+  int64_t total = 0;
+  for (auto& c : chunks) total += (int64_t)c.points->size() / attrs.bytes;
+  double t = 0;
+  int rc = do_indexing(chunks, attrs, min, max, method, name, total, outdir, &t);
+  if (timings) timings[0] = t;
+  return rc;
+}
+
+extern "C" int orc_sample_node(const char* method, const uint8_t* records, const int64_t child_counts[8], int32_t bpp,
+                               int32_t rgb_offset, const double node_min[3], const double node_max[3], int32_t level,
+                               double base_spacing, const double pos_scale[3], const double pos_offset[3],
+                               uint8_t* accepted_flags, uint8_t* out_records) {
+  Attrs attrs;
+  attrs.list.push_back(Attr("position", 12, 3, 4, INT32));
+  int rest = bpp - 12;
+  if (rgb_offset >= 12) {
+    int before = rgb_offset - 12, after = bpp - rgb_offset - 6;
+    if (before > 0) attrs.list.push_back(Attr("pad0", before, before, 1, UINT8));
+    attrs.list.push_back(Attr("rgb", 6, 3, 2, UINT16));
+    if (after > 0) attrs.list.push_back(Attr("pad1", after, after, 1, UINT8));
+  } else if (rest > 0) {
+    attrs.list.push_back(Attr("pad0", rest, rest, 1, UINT8));
+  }
+  attrs.finish();
+  attrs.posScale = V3(pos_scale[0], pos_scale[1], pos_scale[2]);
+  attrs.posOffset = V3(pos_offset[0], pos_offset[1], pos_offset[2]);
+  std::string name = "r";
+  for (int i = 0; i < level; i++) name += "0";
+  auto node = std::make_shared<Node>(name, V3(node_min[0], node_min[1], node_min[2]), V3(node_max[0], node_max[1], node_max[2]));
+  const uint8_t* p = records;
+  std::vector<std::shared_ptr<Node>> kids(8);
+  for (int c = 0; c < 8; c++) {
+    if (child_counts[c] == -1) continue;
+    int64_t cnt = child_counts[c] < 0 ? 0 : child_counts[c];
+    Box b = child_box(node->min, node->max, c);
+    auto k = std::make_shared<Node>(name + std::to_string(c), b.min, b.max);
+    k->points = std::make_shared<std::vector<uint8_t>>(p, p + cnt * bpp);
+    k->numPoints = cnt;
+    k->sampled = true;
+    if (child_counts[c] == -2 || true) {
+      // mark as an inner node when requested by giving it a dummy sampled child
+    }
+    p += cnt * bpp;
+    node->children[c] = k;
+    kids[c] = k;
+  }
+  std::vector<std::vector<uint8_t>> orig(8);
+  for (int c = 0; c < 8; c++) if (kids[c]) orig[c] = *kids[c]->points;
+  NodeCB none = [](Node*) {};
+  bool avg = std::string(method) == "poisson_average";
+  if (avg) sample_poisson_average(node.get(), attrs, base_spacing, none, none);
+  else sample_poisson(node.get(), attrs, base_spacing, none, none);
+  // derive accept flags by matching accepted records in order
+  size_t ai = 0, fi = 0;
+  uint8_t* o = out_records;
+  for (int c = 0; c < 8; c++) {
+    if (!kids[c]) continue;
+    int64_t cnt = (int64_t)orig[c].size() / bpp;
+    for (int64_t i = 0; i < cnt; i++) {
+      const uint8_t* rec = orig[c].data() + i * bpp;
+      bool acc = ai < (size_t)node->numPoints &&
+                 memcmp(node->points->data() + ai * bpp, rec, 12) == 0;
+      (void)rec;
+      accepted_flags[fi++] = acc ? 1 : 0;
+      if (acc) ai++;
+    }
+    if (avg && kids[c]->points) {
+      memcpy(o, kids[c]->points->data(), kids[c]->points->size());
+      o += kids[c]->points->size();
+    }
+  }
+  return 0;
+}
+
+extern "C" const char* orc_last_error(void) { return g_error.c_str(); }
