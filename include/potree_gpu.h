@@ -1,0 +1,98 @@
+/*
+ * C ABI of the MI355X (gfx950) counting-sort chunker and octree indexer.
+ *
+ * Plain pointers, sizes and status codes only.  Every entry point returns 0
+ * on success; 123 mirrors the reference converter's fatal exit code (point
+ * outside the bounding box, failed chunk lookup, bad hierarchy); any other
+ * nonzero value is an error whose text pcg_last_error() returns.  Device
+ * pointers are HIP device allocations on device 0; `stream` arguments are
+ * hipStream_t values passed as void* (NULL = the library's own stream).
+ *
+ * Each declaration names the reference interface it replaces
+ * (paths relative to the PotreeConverter 2.x checkout).
+ */
+#ifndef POTREE_GPU_H
+#define POTREE_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Selects HIP device 0 and creates the library stream.  Called implicitly by
+ * every other entry point. */
+int pcg_init(void);
+
+/* Text of the last error (thread-unsafe, like the reference's logger). */
+const char* pcg_last_error(void);
+
+/* Phase 1 + phase 2 from a LAS file, the drop-in for the two calls at
+ * Converter/src/main.cpp:353 (chunker_countsort_laszip::doChunking) and
+ * Converter/src/main.cpp:380-390 (indexer::doIndexing with
+ * SamplerPoisson / SamplerPoissonAverage).  method: "poisson" or
+ * "poisson_average".  Writes outdir/{octree.bin, hierarchy.bin,
+ * metadata.json}; flags bit 0 keeps outdir/chunks/ (--keep-chunks).
+ * timings (nullable, 8 doubles): chunking count, chunking distribute,
+ * chunking total, indexing, octree build, sampling (seconds), nodes sorted
+ * in libstdc++ order, nodes resolved sequentially. */
+int pcg_convert(const char* las_path, const char* outdir, const char* method, int32_t flags, double* timings);
+
+/* Phase 1 only (chunker_countsort_laszip::doChunking,
+ * Converter/include/chunker_countsort_laszip.h:19): writes outdir/chunks/
+ * (<id>.bin + metadata.json) exactly as the reference chunker's contract. */
+int pcg_do_chunking(const char* las_path, const char* outdir, double* timings);
+
+/* Phase 2 only (indexer::doIndexing, Converter/include/indexer.h:379) on an
+ * existing outdir/chunks/ written by either chunker. */
+int pcg_do_indexing(const char* outdir, const char* method, const char* name, int64_t points_total,
+                    double* timings);
+
+/* Per-cell counts of the chunker's counting pass
+ * (countPointsInCells, Converter/src/chunker_countsort_laszip.cpp:131-361)
+ * over n LAS records of bpp bytes at d_las.  d_grid receives grid_size^3
+ * int32 counts (zeroed by the call).  min/max are the non-cubic bounds
+ * handed to doChunking. */
+int pcg_count_cells(const void* d_las, int64_t n, int32_t bpp, const double las_scale[3],
+                    const double las_offset[3], const double pos_scale[3], const double pos_offset[3],
+                    const double min[3], const double max[3], int32_t grid_size, int32_t* d_grid, void* stream);
+
+/* Device-resident conversion for benchmarks and servers: the LAS records
+ * are already in HBM (d_las, n records, header fields in `las_header`, a
+ * 375-byte LAS 1.4 header image).  Produces the octree in HBM and, when
+ * outdir is non-NULL, writes the three output files.  stats (nullable,
+ * 24 doubles): [0] count s, [1] LUT+distribute s, [2] chunking s,
+ * [3] indexing s, [4] octree build s, [5] sampling s, [6] octree assembly s,
+ * [7] octree.bin bytes, [8] nodes, [9] sampled nodes, [10] candidates,
+ * [11] FP64 re-checks, [12] FP32 pair tests, [13] max FP32 relative error
+ * of re-checked pairs, [14] nodes sorted in libstdc++ order, [15] nodes
+ * resolved sequentially, [16] chunks, [17] octree rounds, [18] depth. */
+int pcg_convert_device(const void* d_las, int64_t n, const uint8_t* las_header, const char* method,
+                       const char* outdir, double* stats);
+
+/* Device memory helpers for callers without a HIP runtime binding.
+ * kind: 0 host->device, 1 device->host, 2 device->device. */
+int pcg_device_alloc(void** ptr, uint64_t bytes);
+int pcg_device_free(void* ptr);
+int pcg_copy(void* dst, const void* src, uint64_t bytes, int32_t kind);
+int pcg_synchronize(void);
+
+/* Synthetic LAS generator (not part of the reference): preset 0 = 500 m
+ * tile, 1 = 5 km tile in UTM-like coordinates.  `params` points at a
+ * caller-owned buffer of at least 256 bytes. */
+int pcg_lasgen_preset(int32_t preset, int64_t n, uint64_t seed, void* params);
+int32_t pcg_lasgen_record_length(const void* params);
+void pcg_lasgen_fill(const void* params, int64_t first, int64_t count, uint8_t* out);
+int pcg_lasgen_write(const void* params, const char* path);
+/* Generates records [first, first+count) into d_out on the GPU and folds the
+ * integer X/Y/Z extents into d_qminmax[6] (min xyz, max xyz). */
+int pcg_lasgen_device(const void* params, int64_t first, int64_t count, uint8_t* d_out, int32_t* d_qminmax);
+/* 375-byte LAS 1.4 header image for generated points with the given
+ * integer extents. */
+int pcg_lasgen_header(const void* params, const int32_t qminmax[6], uint8_t* header375);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* POTREE_GPU_H */

@@ -1,0 +1,97 @@
+"""MI355X (gfx950) counting-sort chunker + octree indexer, PotreeConverter 2.x compatible.
+
+The compute path is libpotree_gpu.so (hand-written HIP kernels plus the C++
+drop-ins for doChunking / doIndexing).  This module is a thin ctypes view
+of its C ABI (include/potree_gpu.h); it never falls back to a CPU
+implementation: a missing library raises at import of `lib()`.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpotree_gpu.so")
+_lib = None
+
+
+class PotreeGpuError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads the native library (built by potreeconverter_amd/build.py)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PotreeGpuError(f"{LIB_PATH} is missing; run potreeconverter_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        c = ctypes
+        L.pcg_init.restype = c.c_int
+        L.pcg_last_error.restype = c.c_char_p
+        L.pcg_convert.argtypes = [c.c_char_p, c.c_char_p, c.c_char_p, c.c_int32, c.POINTER(c.c_double)]
+        L.pcg_do_chunking.argtypes = [c.c_char_p, c.c_char_p, c.POINTER(c.c_double)]
+        L.pcg_do_indexing.argtypes = [c.c_char_p, c.c_char_p, c.c_char_p, c.c_int64, c.POINTER(c.c_double)]
+        L.pcg_count_cells.argtypes = [c.c_void_p, c.c_int64, c.c_int32] + [c.POINTER(c.c_double)] * 6 + [
+            c.c_int32, c.c_void_p, c.c_void_p]
+        L.pcg_convert_device.argtypes = [c.c_void_p, c.c_int64, c.c_void_p, c.c_char_p, c.c_char_p,
+                                         c.POINTER(c.c_double)]
+        L.pcg_lasgen_preset.argtypes = [c.c_int32, c.c_int64, c.c_uint64, c.c_void_p]
+        L.pcg_lasgen_record_length.argtypes = [c.c_void_p]
+        L.pcg_lasgen_record_length.restype = c.c_int32
+        L.pcg_lasgen_fill.argtypes = [c.c_void_p, c.c_int64, c.c_int64, c.c_void_p]
+        L.pcg_lasgen_fill.restype = None
+        L.pcg_lasgen_write.argtypes = [c.c_void_p, c.c_char_p]
+        L.pcg_lasgen_device.argtypes = [c.c_void_p, c.c_int64, c.c_int64, c.c_void_p, c.c_void_p]
+        L.pcg_lasgen_header.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise PotreeGpuError(f"error {rc}: {lib().pcg_last_error().decode(errors='replace')}")
+
+
+class LasGen:
+    """Parameters of the seeded synthetic LAS generator (preset 0: 500 m tile,
+    preset 1: 5 km tile at UTM-like coordinates)."""
+
+    def __init__(self, preset, n, seed=0, extra_bytes=False):
+        self.buf = ctypes.create_string_buffer(256)
+        _check(lib().pcg_lasgen_preset(preset, n, seed, self.buf))
+        if extra_bytes:
+            # LasGenParams.extra_bytes sits after seed, n, extent, origin[3],
+            # offset[3], inv_scale[3], amplitude, num_buildings, tile_points
+            off = 8 + 8 + 8 + 24 * 3 + 8 + 4 + 4
+            ctypes.c_int32.from_buffer(self.buf, off).value = 1
+        self.n = n
+
+    @property
+    def record_length(self):
+        return lib().pcg_lasgen_record_length(self.buf)
+
+    def records(self, first=0, count=None):
+        count = self.n - first if count is None else count
+        out = ctypes.create_string_buffer(count * self.record_length)
+        lib().pcg_lasgen_fill(self.buf, first, count, out)
+        return out.raw
+
+    def write(self, path):
+        _check(lib().pcg_lasgen_write(self.buf, str(path).encode()))
+
+
+def convert(las_path, outdir, method="poisson", keep_chunks=False):
+    t = (ctypes.c_double * 8)()
+    _check(lib().pcg_convert(str(las_path).encode(), str(outdir).encode(), method.encode(), 1 if keep_chunks else 0, t))
+    return list(t)
+
+
+def do_chunking(las_path, outdir):
+    t = (ctypes.c_double * 8)()
+    _check(lib().pcg_do_chunking(str(las_path).encode(), str(outdir).encode(), t))
+    return list(t)
+
+
+def do_indexing(outdir, method="poisson", name="", points_total=0):
+    t = (ctypes.c_double * 8)()
+    _check(lib().pcg_do_indexing(str(outdir).encode(), method.encode(), name.encode(), points_total, t))
+    return list(t)

@@ -1,0 +1,311 @@
+// Attribute schema, GPU transcode tables and chunk metadata text.
+#include "attributes.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <sstream>
+
+namespace pcg {
+
+const char* attribute_type_name(AttributeType t) {
+  switch (t) {
+    case AttributeType::INT8: return "int8";
+    case AttributeType::INT16: return "int16";
+    case AttributeType::INT32: return "int32";
+    case AttributeType::INT64: return "int64";
+    case AttributeType::UINT8: return "uint8";
+    case AttributeType::UINT16: return "uint16";
+    case AttributeType::UINT32: return "uint32";
+    case AttributeType::UINT64: return "uint64";
+    case AttributeType::FLOAT: return "float";
+    case AttributeType::DOUBLE: return "double";
+    case AttributeType::UNDEFINED: return "undefined";
+  }
+  return "error";
+}
+
+AttributeType attribute_type_from_name(const std::string& s) {
+  const AttributeType all[] = {AttributeType::INT8, AttributeType::INT16, AttributeType::INT32, AttributeType::INT64,
+                               AttributeType::UINT8, AttributeType::UINT16, AttributeType::UINT32,
+                               AttributeType::UINT64, AttributeType::FLOAT, AttributeType::DOUBLE};
+  for (auto t : all) if (s == attribute_type_name(t)) return t;
+  return AttributeType::UNDEFINED;
+}
+
+int attribute_type_size(AttributeType t) {
+  switch (t) {
+    case AttributeType::INT8: case AttributeType::UINT8: return 1;
+    case AttributeType::INT16: case AttributeType::UINT16: return 2;
+    case AttributeType::INT32: case AttributeType::UINT32: case AttributeType::FLOAT: return 4;
+    case AttributeType::INT64: case AttributeType::UINT64: case AttributeType::DOUBLE: return 8;
+    default: return 0;
+  }
+}
+
+static StatKind stat_kind_of(AttributeType t) {
+  switch (t) {
+    case AttributeType::INT8: return ST_I8;
+    case AttributeType::INT16: return ST_I16;
+    case AttributeType::INT32: return ST_I32;
+    case AttributeType::INT64: return ST_I64;
+    case AttributeType::UINT8: return ST_U8;
+    case AttributeType::UINT16: return ST_U16;
+    case AttributeType::UINT32: return ST_U32;
+    case AttributeType::UINT64: return ST_U64;
+    case AttributeType::FLOAT: return ST_F32;
+    case AttributeType::DOUBLE: return ST_F64;
+    default: return ST_NONE;
+  }
+}
+
+bool output_attributes_for(const lasfile::Header& h, Attributes& out, ChunkerParams& P) {
+  using T = AttributeType;
+  Attribute xyz("position", 12, 3, 4, T::INT32), intensity("intensity", 2, 1, 2, T::UINT16);
+  Attribute returnNumber("return number", 1, 1, 1, T::UINT8), numberOfReturns("number of returns", 1, 1, 1, T::UINT8);
+  Attribute classification("classification", 1, 1, 1, T::UINT8), scanAngleRank("scan angle rank", 1, 1, 1, T::UINT8);
+  Attribute userData("user data", 1, 1, 1, T::UINT8), pointSourceId("point source id", 2, 1, 2, T::UINT16);
+  Attribute gpsTime("gps-time", 8, 1, 8, T::DOUBLE), rgb("rgb", 6, 3, 2, T::UINT16);
+  std::vector<Attribute> list;
+  int fmt = h.point_format;
+  switch (fmt) {
+    case 0: list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId}; break;
+    case 1: list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId, gpsTime}; break;
+    case 2: list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId, rgb}; break;
+    case 3: list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId, gpsTime, rgb}; break;
+    default:
+      set_error("LAS point data format " + std::to_string(fmt) + " is not supported by the GPU chunker (formats 0-3)");
+      return false;
+  }
+  const size_t numStandard = list.size();
+  // extra bytes (PotreeConverter.h:80-133, LasLoader.cpp:7-69)
+  for (auto& e : h.extra) {
+    static const T base[] = {T::UNDEFINED, T::UINT8, T::INT8, T::UINT16, T::INT16, T::UINT32, T::INT32, T::UINT64,
+                             T::INT64, T::FLOAT, T::DOUBLE};
+    T t = T::UNDEFINED;
+    int ne = 1;
+    if (e.data_type > 0 && e.data_type <= 30) {
+      t = base[(e.data_type - 1) % 10 + 1];
+      ne = e.data_type <= 10 ? 1 : (e.data_type <= 20 ? 2 : 3);
+    }
+    int es = attribute_type_size(t);
+    Attribute a(e.name, ne * es, ne, es, t);
+    a.description = e.description;
+    a.scale = Vec3(e.scale[0], e.scale[1], e.scale[2]);
+    a.offset = Vec3(e.offset[0], e.offset[1], e.offset[2]);
+    list.push_back(a);
+  }
+  out = Attributes(list);
+  Vec3 min(h.min[0], h.min[1], h.min[2]), max(h.max[0], h.max[1], h.max[2]);
+  Vec3 scale(h.scale[0], h.scale[1], h.scale[2]);
+  Vec3 size = max - min;
+  scale.x = std::max(scale.x, size.x / pow(2.0, 30.0));
+  scale.y = std::max(scale.y, size.y / pow(2.0, 30.0));
+  scale.z = std::max(scale.z, size.z / pow(2.0, 30.0));
+  out.posScale = scale;
+  out.posOffset = min;
+
+  // transcode + statistics tables (chunker_countsort_laszip.cpp:391-651)
+  memset(&P, 0, sizeof(P));
+  P.bpp_in = h.record_length;
+  P.bpp_out = out.bytes;
+  auto op = [&](int src, int dst, int len, int mode) {
+    P.ops[P.num_ops++] = TranscodeOp{src, dst, len, mode};
+  };
+  auto st = [&](int src, int kind) { P.stats[P.num_stats++] = StatOp{src, kind}; };
+  st(0, ST_I32); st(4, ST_I32); st(8, ST_I32);
+  op(12, out.getOffset("intensity"), 2, TC_COPY); st(12, ST_U16);
+  op(14, out.getOffset("return number"), 1, TC_LOW3); st(14, ST_LOW3);
+  op(14, out.getOffset("number of returns"), 1, TC_MID3); st(14, ST_MID3);
+  op(15, out.getOffset("classification"), 1, TC_LOW5); st(15, ST_LOW5);
+  op(16, out.getOffset("scan angle rank"), 1, TC_COPY); st(16, ST_I8);
+  op(17, out.getOffset("user data"), 1, TC_COPY); st(17, ST_U8);
+  op(18, out.getOffset("point source id"), 2, TC_COPY); st(18, ST_U16);
+  if (fmt == 1 || fmt == 3) { op(20, out.getOffset("gps-time"), 8, TC_COPY); st(20, ST_F64); }
+  if (fmt == 2 || fmt == 3) {
+    int src = fmt == 2 ? 20 : 28;
+    op(src, out.getOffset("rgb"), 6, TC_COPY);
+    st(src, ST_U16); st(src + 2, ST_U16); st(src + 4, ST_U16);
+  }
+  P.class_src = 15;
+  static const int core[] = {20, 28, 26, 34};
+  int srcOff = core[fmt];
+  int dstOff = 0;
+  for (size_t i = 0; i < numStandard; i++) dstOff += out.list[i].size;
+  int extraLen = 0;
+  for (size_t i = numStandard; i < out.list.size(); i++) {
+    const Attribute& a = out.list[i];
+    StatKind k = stat_kind_of(a.type);
+    for (int e = 0; e < a.numElements && e < 3 && k != ST_NONE; e++) {
+      if (P.num_stats >= kMaxStats) { set_error("too many attribute components"); return false; }
+      st(srcOff + extraLen + e * a.elementSize, k);
+    }
+    extraLen += a.size;
+  }
+  if (extraLen > 0) op(srcOff, dstOff, extraLen, TC_COPY);
+  if (srcOff + extraLen > h.record_length) {
+    set_error("LAS record shorter than its declared attributes");
+    return false;
+  }
+  return true;
+}
+
+std::string fmt17(double v) {
+  char buf[64];
+  if (std::isinf(v)) return v > 0 ? "inf" : "-inf";
+  if (std::isnan(v)) return "nan";
+  snprintf(buf, sizeof(buf), "%.17g", v);
+  return buf;
+}
+
+static std::string jnum(double v) { return std::isfinite(v) ? fmt17(v) : "null"; }
+
+std::string chunk_metadata_json(const Vec3& min, const Vec3& max, const Attributes& a) {
+  std::ostringstream s;
+  s << "{\n    \"attributes\": [\n";
+  for (size_t i = 0; i < a.list.size(); i++) {
+    const Attribute& t = a.list[i];
+    auto vec = [&](const Vec3& v) {
+      std::string r = "[" + jnum(v.x);
+      if (t.numElements >= 2) r += ", " + jnum(v.y);
+      if (t.numElements >= 3) r += ", " + jnum(v.z);
+      return r + "]";
+    };
+    s << "        {\n";
+    s << "            \"description\": \"" << t.description << "\",\n";
+    s << "            \"elementSize\": " << t.elementSize << ",\n";
+    bool empty = std::all_of(t.histogram.begin(), t.histogram.end(), [](int64_t h) { return h == 0; });
+    if (t.size == 1 && !empty) {
+      s << "            \"histogram\": [";
+      for (size_t k = 0; k < t.histogram.size(); k++) s << (k ? ", " : "") << t.histogram[k];
+      s << "],\n";
+    }
+    s << "            \"max\": " << vec(t.max) << ",\n";
+    s << "            \"min\": " << vec(t.min) << ",\n";
+    s << "            \"name\": \"" << t.name << "\",\n";
+    s << "            \"numElements\": " << t.numElements << ",\n";
+    s << "            \"offset\": " << vec(t.offset) << ",\n";
+    s << "            \"scale\": " << vec(t.scale) << ",\n";
+    s << "            \"size\": " << t.size << ",\n";
+    s << "            \"type\": \"" << attribute_type_name(t.type) << "\"\n";
+    s << "        }" << (i + 1 < a.list.size() ? "," : "") << "\n";
+  }
+  s << "    ],\n";
+  s << "    \"max\": [" << fmt17(max.x) << ", " << fmt17(max.y) << ", " << fmt17(max.z) << "],\n";
+  s << "    \"min\": [" << fmt17(min.x) << ", " << fmt17(min.y) << ", " << fmt17(min.z) << "],\n";
+  s << "    \"offset\": [" << fmt17(a.posOffset.x) << ", " << fmt17(a.posOffset.y) << ", " << fmt17(a.posOffset.z) << "],\n";
+  s << "    \"scale\": [" << fmt17(a.posScale.x) << ", " << fmt17(a.posScale.y) << ", " << fmt17(a.posScale.z) << "]\n";
+  s << "}";
+  return s.str();
+}
+
+// ---- minimal JSON reader for chunks/metadata.json ----
+namespace {
+struct J {
+  enum K { NUL, NUM, STR, ARR, OBJ } k = NUL;
+  double num = 0;
+  std::string str;
+  std::vector<J> arr;
+  std::vector<std::pair<std::string, J>> obj;
+  const J* at(const std::string& key) const {
+    for (auto& kv : obj) if (kv.first == key) return &kv.second;
+    return nullptr;
+  }
+};
+void ws(const char*& p) { while (*p == ' ' || *p == '\n' || *p == '\t' || *p == '\r') p++; }
+bool parse(const char*& p, J& v) {
+  ws(p);
+  if (*p == '{') {
+    v.k = J::OBJ; p++; ws(p);
+    if (*p == '}') { p++; return true; }
+    while (*p) {
+      J key, val;
+      if (!parse(p, key) || key.k != J::STR) return false;
+      ws(p);
+      if (*p++ != ':') return false;
+      if (!parse(p, val)) return false;
+      v.obj.push_back({key.str, val});
+      ws(p);
+      if (*p == ',') { p++; continue; }
+      if (*p == '}') { p++; return true; }
+      return false;
+    }
+    return false;
+  }
+  if (*p == '[') {
+    v.k = J::ARR; p++; ws(p);
+    if (*p == ']') { p++; return true; }
+    while (*p) {
+      J e;
+      if (!parse(p, e)) return false;
+      v.arr.push_back(e);
+      ws(p);
+      if (*p == ',') { p++; continue; }
+      if (*p == ']') { p++; return true; }
+      return false;
+    }
+    return false;
+  }
+  if (*p == '"') {
+    v.k = J::STR; p++;
+    while (*p && *p != '"') v.str += *p++;
+    if (*p != '"') return false;
+    p++;
+    return true;
+  }
+  if (!strncmp(p, "null", 4)) { v.k = J::NUL; p += 4; return true; }
+  char* e = nullptr;
+  v.k = J::NUM;
+  v.num = strtod(p, &e);
+  if (e == p) return false;
+  p = e;
+  return true;
+}
+}  // namespace
+
+bool parse_chunk_metadata(const std::string& text, Vec3& min, Vec3& max, Attributes& a) {
+  const char* p = text.c_str();
+  J js;
+  if (!parse(p, js) || js.k != J::OBJ) { set_error("malformed chunks/metadata.json"); return false; }
+  auto v3 = [](const J* x, Vec3& out) {
+    if (!x || x->arr.size() < 3) return false;
+    out = Vec3(x->arr[0].num, x->arr[1].num, x->arr[2].num);
+    return true;
+  };
+  Vec3 sc, of;
+  if (!v3(js.at("min"), min) || !v3(js.at("max"), max) || !v3(js.at("scale"), sc) || !v3(js.at("offset"), of)) {
+    set_error("chunks/metadata.json lacks min/max/scale/offset");
+    return false;
+  }
+  std::vector<Attribute> list;
+  const J* ja = js.at("attributes");
+  if (!ja) { set_error("chunks/metadata.json lacks attributes"); return false; }
+  for (auto& o : ja->arr) {
+    Attribute t(o.at("name")->str, (int)o.at("size")->num, (int)o.at("numElements")->num,
+                (int)o.at("elementSize")->num, attribute_type_from_name(o.at("type")->str));
+    t.description = o.at("description") ? o.at("description")->str : "";
+    if (const J* h = o.at("histogram"))
+      for (size_t i = 0; i < h->arr.size() && i < 256; i++) t.histogram[i] = (int64_t)h->arr[i].num;
+    // indexer.cpp:106-123: a missing (null) min or max reads back as +inf
+    auto comp = [](const J* arr, int i, double dflt) {
+      if (!arr || (int)arr->arr.size() <= i || arr->arr[i].k == J::NUL) return dflt;
+      return arr->arr[i].num;
+    };
+    for (int i = 0; i < t.numElements && i < 3; i++) {
+      t.min[i] = comp(o.at("min"), i, INFINITY);
+      t.max[i] = comp(o.at("max"), i, INFINITY);
+      t.scale[i] = comp(o.at("scale"), i, 1.0);
+      t.offset[i] = comp(o.at("offset"), i, 0.0);
+    }
+    list.push_back(t);
+  }
+  a = Attributes(list);
+  a.posScale = sc;
+  a.posOffset = of;
+  return true;
+}
+
+}  // namespace pcg

@@ -1,0 +1,258 @@
+// extern "C" entry points (include/potree_gpu.h).
+#include <string.h>
+
+#include <filesystem>
+#include <string>
+#include <vector>
+
+#include "../../include/potree_gpu.h"
+#include "chunker.h"
+#include "converter.h"
+#include "indexing.h"
+#include "lasgen.h"
+
+namespace fs = std::filesystem;
+using namespace pcg;
+
+static_assert(sizeof(LasGenParams) <= 256, "LasGenParams must fit the caller's 256-byte buffer");
+
+extern "C" {
+
+int pcg_init(void) { return ensure_device(); }
+
+const char* pcg_last_error(void) { return last_error().c_str(); }
+
+static int load_las_to_device(const char* las_path, lasfile::Header& h, uint8_t** d_las) {
+  std::string err;
+  if (!lasfile::read_header(las_path, h, &err)) {
+    set_error(err);
+    return 1;
+  }
+  std::vector<uint8_t> host((size_t)h.num_points * h.record_length);
+  if (!host.empty() && !lasfile::read_points(las_path, h, 0, h.num_points, host.data())) {
+    set_error(std::string("short read of point records: ") + las_path);
+    return 1;
+  }
+  PCG_CHECK(hipMalloc(d_las, host.size() + 64));
+  if (!host.empty()) PCG_CHECK(hipMemcpy(*d_las, host.data(), host.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int pcg_do_chunking(const char* las_path, const char* outdir, double* timings) {
+  if (ensure_device()) return 1;
+  lasfile::Header h;
+  uint8_t* d_las = nullptr;
+  int rc = load_las_to_device(las_path, h, &d_las);
+  if (rc) { if (d_las) (void)hipFree(d_las); return rc; }
+  DeviceChunks dc;
+  rc = gpu_chunking(d_las, h, dc);
+  (void)hipFree(d_las);
+  if (rc) { dc.release(); return rc; }
+  rc = write_chunk_files(dc, outdir);
+  if (timings) { timings[0] = dc.t_count; timings[1] = dc.t_distribute; timings[2] = dc.t_total; }
+  dc.release();
+  return rc;
+}
+
+static void fill_timings(double* t, const DeviceChunks& dc, const IndexResult& r, double tWrite) {
+  (void)tWrite;
+  if (!t) return;
+  t[0] = dc.t_count;
+  t[1] = dc.t_distribute;
+  t[2] = dc.t_total;
+  t[3] = r.t_total;
+  t[4] = r.t_build;
+  t[5] = r.t_sample;
+  t[6] = (double)r.counters[3];
+  t[7] = (double)r.counters[4];
+}
+
+int pcg_convert(const char* las_path, const char* outdir, const char* method, int32_t flags, double* timings) {
+  if (ensure_device()) return 1;
+  lasfile::Header h;
+  uint8_t* d_las = nullptr;
+  int rc = load_las_to_device(las_path, h, &d_las);
+  if (rc) { if (d_las) (void)hipFree(d_las); return rc; }
+  DeviceChunks dc;
+  rc = gpu_chunking(d_las, h, dc);
+  (void)hipFree(d_las);
+  if (rc) { dc.release(); return rc; }
+  fs::create_directories(outdir);
+  if (flags & 1) {
+    rc = write_chunk_files(dc, outdir);
+    if (rc) { dc.release(); return rc; }
+  }
+  IndexResult r;
+  std::string name = fs::path(las_path).stem().string();
+  rc = gpu_indexing(dc, method, name, (int64_t)h.num_points, r);
+  if (!rc) rc = write_index_files(r, outdir);
+  fill_timings(timings, dc, r, 0.0);
+  dc.release();
+  r.release();
+  return rc;
+}
+
+int pcg_do_indexing(const char* outdir, const char* method, const char* name, int64_t points_total,
+                    double* timings) {
+  if (ensure_device()) return 1;
+  DeviceChunks dc;
+  int rc = read_chunk_files(outdir, dc);
+  if (rc) { dc.release(); return rc; }
+  IndexResult r;
+  rc = gpu_indexing(dc, method, name ? name : "", points_total > 0 ? points_total : dc.numPoints, r);
+  if (!rc) rc = write_index_files(r, outdir);
+  fill_timings(timings, dc, r, 0.0);
+  dc.release();
+  r.release();
+  return rc;
+}
+
+int pcg_convert_device(const void* d_las, int64_t n, const uint8_t* las_header, const char* method,
+                       const char* outdir, double* stats) {
+  if (ensure_device()) return 1;
+  lasfile::Header h;
+  h.point_format = las_header[104] & 0x3F;
+  h.record_length = lasfile::rd<uint16_t>(las_header + 105);
+  h.num_points = (uint64_t)n;
+  for (int i = 0; i < 3; i++) h.scale[i] = lasfile::rd<double>(las_header + 131 + 8 * i);
+  for (int i = 0; i < 3; i++) h.offset[i] = lasfile::rd<double>(las_header + 155 + 8 * i);
+  h.max[0] = lasfile::rd<double>(las_header + 179); h.min[0] = lasfile::rd<double>(las_header + 187);
+  h.max[1] = lasfile::rd<double>(las_header + 195); h.min[1] = lasfile::rd<double>(las_header + 203);
+  h.max[2] = lasfile::rd<double>(las_header + 211); h.min[2] = lasfile::rd<double>(las_header + 219);
+  DeviceChunks dc;
+  int rc = gpu_chunking((const uint8_t*)d_las, h, dc);
+  if (rc) { dc.release(); return rc; }
+  IndexResult r;
+  rc = gpu_indexing(dc, method, "device", n, r);
+  if (!rc && outdir) rc = write_index_files(r, outdir);
+  if (stats) {
+    stats[0] = dc.t_count;
+    stats[1] = dc.t_distribute;
+    stats[2] = dc.t_total;
+    stats[3] = r.t_total;
+    stats[4] = r.t_build;
+    stats[5] = r.t_sample;
+    stats[6] = r.t_assemble;
+    stats[7] = (double)r.octreeBytes;
+    stats[8] = (double)r.numNodes;
+    stats[9] = (double)r.numSampled;
+    stats[10] = (double)r.counters[5];  // candidates
+    stats[11] = (double)r.counters[0];  // fp64 re-checks
+    stats[12] = (double)r.counters[1];  // fp32 pair tests
+    uint32_t eb = (uint32_t)r.counters[2];
+    float e;
+    memcpy(&e, &eb, 4);
+    stats[13] = (double)e;              // max |dd32 - dd64| / spacing^2 on re-checked pairs
+    stats[14] = (double)r.counters[3];  // nodes sorted in libstdc++ order
+    stats[15] = (double)r.counters[4];  // nodes resolved sequentially
+    stats[16] = (double)dc.chunks.size();
+    stats[17] = (double)r.buildRounds;
+    stats[18] = (double)r.octreeDepth;
+  }
+  dc.release();
+  r.release();
+  return rc;
+}
+
+int pcg_device_alloc(void** ptr, uint64_t bytes) {
+  if (ensure_device()) return 1;
+  PCG_CHECK(hipMalloc(ptr, bytes ? bytes : 16));
+  return 0;
+}
+
+int pcg_device_free(void* ptr) {
+  if (ptr) PCG_CHECK(hipFree(ptr));
+  return 0;
+}
+
+int pcg_copy(void* dst, const void* src, uint64_t bytes, int32_t kind) {
+  if (ensure_device()) return 1;
+  hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+  PCG_CHECK(hipMemcpy(dst, src, bytes, k));
+  return 0;
+}
+
+int pcg_synchronize(void) {
+  if (ensure_device()) return 1;
+  PCG_CHECK(hipDeviceSynchronize());
+  return 0;
+}
+
+int pcg_count_cells(const void* d_las, int64_t n, int32_t bpp, const double las_scale[3], const double las_offset[3],
+                    const double pos_scale[3], const double pos_offset[3], const double min[3], const double max[3],
+                    int32_t grid_size, int32_t* d_grid, void* stream) {
+  if (ensure_device()) return 1;
+  hipStream_t s = stream ? (hipStream_t)stream : device().stream;
+  ChunkerParams P;
+  memset(&P, 0, sizeof(P));
+  Vec3 mn(min[0], min[1], min[2]), mx(max[0], max[1], max[2]);
+  for (int a = 0; a < 3; a++) {
+    P.las_scale[a] = las_scale[a]; P.las_offset[a] = las_offset[a];
+    P.pos_scale[a] = pos_scale[a]; P.pos_offset[a] = pos_offset[a];
+    P.min[a] = min[a];
+  }
+  P.cube = (mx - mn).max();
+  P.grid_size = grid_size;
+  P.bpp_in = bpp;
+  P.class_src = -1;
+  if (chunker_set_params(P, s)) return 1;
+  int64_t cells = (int64_t)grid_size * grid_size * grid_size;
+  void *stats = nullptr, *hist = nullptr, *err = nullptr;
+  PCG_CHECK(hipMalloc(&stats, sizeof(uint64_t) * 2 * kMaxStats));
+  PCG_CHECK(hipMalloc(&hist, 256 * sizeof(unsigned long long)));
+  PCG_CHECK(hipMalloc(&err, 16));
+  PCG_CHECK(hipMemsetAsync(d_grid, 0, sizeof(int32_t) * cells, s));
+  PCG_CHECK(hipMemsetAsync(err, 0, 16, s));
+  int rc = chunker_count((const uint8_t*)d_las, n, d_grid, (uint64_t*)stats, (uint64_t*)stats + kMaxStats,
+                         (unsigned long long*)hist, (int32_t*)err, device().num_cus, s);
+  int32_t herr = 0;
+  if (!rc) {
+    PCG_CHECK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s));
+    PCG_CHECK(hipStreamSynchronize(s));
+  }
+  (void)hipFree(stats); (void)hipFree(hist); (void)hipFree(err);
+  if (rc) return rc;
+  if (herr & 1) {
+    set_error("encountered point outside bounding box.");
+    return 123;
+  }
+  return 0;
+}
+
+int pcg_lasgen_device(const void* params, int64_t first, int64_t count, uint8_t* d_out, int32_t* d_qminmax) {
+  if (ensure_device()) return 1;
+  return lasgen_device(*(const LasGenParams*)params, first, count, d_out, d_qminmax, device().stream);
+}
+
+void pcg_lasgen_bounds(const LasGenParams* p, const int32_t qmin[3], const int32_t qmax[3], double mn[3], double mx[3]);
+
+int pcg_lasgen_header(const void* params, const int32_t qminmax[6], uint8_t* header375) {
+  const LasGenParams* p = (const LasGenParams*)params;
+  lasfile::Header h;
+  h.point_format = 2;
+  h.record_length = (uint16_t)(p->extra_bytes ? 33 : 26);
+  h.num_points = (uint64_t)p->n;
+  for (int i = 0; i < 3; i++) {
+    h.scale[i] = 1.0 / p->inv_scale[i];
+    h.offset[i] = p->offset[i];
+  }
+  pcg_lasgen_bounds(p, qminmax, qminmax + 3, h.min, h.max);
+  memset(header375, 0, 375);
+  memcpy(header375, "LASF", 4);
+  header375[24] = 1;
+  header375[25] = 4;
+  lasfile::wr<uint16_t>(header375 + 94, 375);
+  lasfile::wr<uint32_t>(header375 + 96, 375);
+  header375[104] = h.point_format;
+  lasfile::wr<uint16_t>(header375 + 105, h.record_length);
+  lasfile::wr<uint32_t>(header375 + 107, h.num_points <= 0xFFFFFFFFull ? (uint32_t)h.num_points : 0u);
+  for (int i = 0; i < 3; i++) lasfile::wr<double>(header375 + 131 + 8 * i, h.scale[i]);
+  for (int i = 0; i < 3; i++) lasfile::wr<double>(header375 + 155 + 8 * i, h.offset[i]);
+  lasfile::wr<double>(header375 + 179, h.max[0]); lasfile::wr<double>(header375 + 187, h.min[0]);
+  lasfile::wr<double>(header375 + 195, h.max[1]); lasfile::wr<double>(header375 + 203, h.min[1]);
+  lasfile::wr<double>(header375 + 211, h.max[2]); lasfile::wr<double>(header375 + 219, h.min[2]);
+  lasfile::wr<uint64_t>(header375 + 247, h.num_points);
+  return 0;
+}
+
+}  // extern "C"

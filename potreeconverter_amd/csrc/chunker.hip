@@ -1,0 +1,551 @@
+// Counting-sort chunker on gfx950: LAS records in HBM -> output records
+// grouped by chunk, in input order inside each chunk.
+//
+// Restates Converter/src/chunker_countsort_laszip.cpp:
+//   k_count           :131-361  per-cell counts (+ attribute min/max and the
+//                               classification histogram of :380-658, which
+//                               depend only on the input records)
+//   k_pyramid_level   :1101-1183 image-pyramid merge
+//   k_chunk_cells     :1144-1172 cells that become chunks
+//   k_lut_fill        :1187-1199 LUT fill
+//   k_tile_hist       :838-876  per-tile chunk counts
+//   k_scatter         :770-904  transcode + stable scatter
+// All geometry is IEEE binary64 with the reference's evaluation order; the
+// TU is compiled with -ffp-contract=off.
+#include "chunker.h"
+#include "lasgen.h"
+#include "pcg_common.h"
+#include "scan.h"
+
+namespace pcg {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / 64;
+constexpr int kCountSteps = 16;             // 64-point steps per wave in k_count
+constexpr int kHashSize = 2048;             // LDS hash slots in k_count
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr int kMaxBppIn = 128;
+
+__constant__ ChunkerParams c_params;
+
+// Loads `count` records (bpp bytes each) starting at record `first` into a
+// 16-byte aligned LDS area using this wave's lanes; returns the byte shift
+// of the first record inside the area.  Reads never pass `total_bytes`.
+__device__ inline uint32_t wave_load_records(const uint8_t* __restrict__ src, uint64_t total_bytes, int64_t first,
+                                             int count, int bpp, uint8_t* lds) {
+  const int lane = threadIdx.x & 63;
+  uint64_t b0 = (uint64_t)first * bpp;
+  uint64_t a0 = b0 & ~15ull;
+  uint64_t end = b0 + (uint64_t)count * bpp;
+  int nvec = (int)((end - a0 + 15) >> 4);
+  for (int v = lane; v < nvec; v += 64) {
+    uint64_t addr = a0 + (uint64_t)v * 16;
+    if (addr + 16 <= total_bytes) {
+      *(uint4*)(lds + v * 16) = *(const uint4*)(src + addr);
+    } else {
+      for (int b = 0; b < 16; b++) lds[v * 16 + b] = (addr + b < total_bytes) ? src[addr + b] : 0;
+    }
+  }
+  return (uint32_t)(b0 - a0);
+}
+
+__device__ inline int32_t ld_i32(const uint8_t* p) {
+  return (int32_t)((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24));
+}
+
+// laszip coordinate (scale * X + offset), requantization and the cell of
+// one point: chunker_countsort_laszip.cpp:224-257
+__device__ inline int64_t cell_of_record(const uint8_t* rec, const ChunkerParams& P, bool& inBox, int32_t q[3]) {
+  const int32_t G = P.grid_size;
+  const double dG = (double)G;
+  int64_t idx = 0;
+  inBox = true;
+  int64_t mul = 1;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    int32_t X = ld_i32(rec + 4 * a);
+    double x = P.las_scale[a] * X + P.las_offset[a];
+    int32_t Xq = (int32_t)((x - P.pos_offset[a]) / P.pos_scale[a]);
+    double u = ((double)Xq * P.pos_scale[a] + P.pos_offset[a] - P.min[a]) / P.cube;
+    inBox = inBox && u >= 0.0 && u <= 1.0;
+    double gu = dG * u;
+    double lim = dG - 1.0;
+    int64_t ic = (int64_t)(lim < gu ? lim : gu);
+    idx += ic * mul;
+    mul *= G;
+    q[a] = Xq;
+  }
+  return idx;
+}
+
+__device__ inline double stat_value(const uint8_t* rec, int kind) {
+  switch (kind) {
+    case ST_U8: return (double)rec[0];
+    case ST_I8: return (double)(int8_t)rec[0];
+    case ST_U16: return (double)(uint16_t)(rec[0] | (rec[1] << 8));
+    case ST_I16: return (double)(int16_t)(rec[0] | (rec[1] << 8));
+    case ST_U32: return (double)(uint32_t)ld_i32(rec);
+    case ST_I32: return (double)ld_i32(rec);
+    case ST_U64: {
+      uint64_t v = (uint64_t)(uint32_t)ld_i32(rec) | ((uint64_t)(uint32_t)ld_i32(rec + 4) << 32);
+      return (double)v;
+    }
+    case ST_I64: {
+      uint64_t v = (uint64_t)(uint32_t)ld_i32(rec) | ((uint64_t)(uint32_t)ld_i32(rec + 4) << 32);
+      return (double)(int64_t)v;
+    }
+    case ST_F32: {
+      uint32_t b = (uint32_t)ld_i32(rec);
+      float f;
+      __builtin_memcpy(&f, &b, 4);
+      return (double)f;
+    }
+    case ST_F64: {
+      uint64_t v = (uint64_t)(uint32_t)ld_i32(rec) | ((uint64_t)(uint32_t)ld_i32(rec + 4) << 32);
+      double d;
+      __builtin_memcpy(&d, &v, 8);
+      return d;
+    }
+    case ST_LOW3: return (double)(rec[0] & 7);
+    case ST_MID3: return (double)((rec[0] >> 3) & 7);
+    case ST_LOW5: return (double)(rec[0] & 31);
+  }
+  return 0.0;
+}
+
+// K1: per-cell counts with an LDS hash that merges repeated cells before the
+// global atomics, plus attribute min/max (ordered uint64 bits) and the
+// classification histogram.  Each wave walks 64-point steps.
+__global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ las, int64_t n,
+                                                   int32_t* __restrict__ grid, uint64_t* __restrict__ stat_min,
+                                                   uint64_t* __restrict__ stat_max,
+                                                   unsigned long long* __restrict__ class_hist,
+                                                   int32_t* __restrict__ err) {
+  const ChunkerParams& P = c_params;
+  __shared__ uint32_t s_key[kHashSize];
+  __shared__ uint32_t s_cnt[kHashSize];
+  __shared__ uint32_t s_hist[256];
+  __shared__ uint64_t s_smin[kWavesPerBlock][kMaxStats];
+  __shared__ uint64_t s_smax[kWavesPerBlock][kMaxStats];
+  __shared__ __attribute__((aligned(16))) uint8_t s_rec[kWavesPerBlock][64 * kMaxBppIn + 32];
+  for (int i = threadIdx.x; i < kHashSize; i += kBlock) { s_key[i] = kEmpty; s_cnt[i] = 0; }
+  for (int i = threadIdx.x; i < 256; i += kBlock) s_hist[i] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int bpp = P.bpp_in;
+  const uint64_t total = (uint64_t)n * bpp;
+  uint64_t my_min = ~0ull, my_max = 0ull;  // stat `lane` of this wave
+  uint8_t* rec_area = s_rec[wave];
+  const int64_t steps_total = (n + 63) / 64;
+  const int64_t wave_global = (int64_t)blockIdx.x * kWavesPerBlock + wave;
+  const int64_t waves_total = (int64_t)gridDim.x * kWavesPerBlock;
+  bool bad = false;
+  for (int64_t step = wave_global; step < steps_total; step += waves_total) {
+    int64_t first = step * 64;
+    int cnt = (int)((n - first) < 64 ? (n - first) : 64);
+    uint32_t shift = wave_load_records(las, total, first, cnt, bpp, rec_area);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if (lane < cnt) {
+      const uint8_t* rec = rec_area + shift + lane * bpp;
+      bool inBox;
+      int32_t q[3];
+      uint32_t key = (uint32_t)cell_of_record(rec, P, inBox, q);
+      if (!inBox) bad = true;
+      else {
+        uint32_t h = (key * 2654435761u) >> 21;  // 11 bits
+        bool done = false;
+        for (int probe = 0; probe < 8 && !done; probe++) {
+          uint32_t slot = (h + probe) & (kHashSize - 1);
+          uint32_t cur = s_key[slot];
+          if (cur == kEmpty) {
+            cur = atomicCAS(&s_key[slot], kEmpty, key);
+            if (cur == kEmpty) cur = key;
+          }
+          if (cur == key) {
+            atomicAdd(&s_cnt[slot], 1u);
+            done = true;
+          }
+        }
+        if (!done) atomicAdd(&grid[key], 1);
+      }
+      if (P.class_src >= 0) atomicAdd(&s_hist[rec[P.class_src] & 31], 1u);
+    }
+    // attribute statistics: lane s folds stat s over the step's records
+    if (lane < P.num_stats) {
+      const StatOp op = P.stats[lane];
+      for (int r = 0; r < cnt; r++) {
+        const uint8_t* rec = rec_area + shift + r * bpp;
+        uint64_t b = ordered_bits(stat_value(rec + op.src, op.kind));
+        my_min = b < my_min ? b : my_min;
+        my_max = b > my_max ? b : my_max;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  }
+  if (bad) atomicOr(err, 1);
+  if (lane < kMaxStats) { s_smin[wave][lane] = my_min; s_smax[wave][lane] = my_max; }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHashSize; i += kBlock) {
+    uint32_t k = s_key[i];
+    if (k != kEmpty && s_cnt[i]) atomicAdd(&grid[k], (int32_t)s_cnt[i]);
+  }
+  for (int i = threadIdx.x; i < 256; i += kBlock)
+    if (s_hist[i]) atomicAdd(&class_hist[i], (unsigned long long)s_hist[i]);
+  if (threadIdx.x < P.num_stats) {
+    uint64_t mn = ~0ull, mx = 0;
+    for (int w = 0; w < kWavesPerBlock; w++) {
+      mn = s_smin[w][threadIdx.x] < mn ? s_smin[w][threadIdx.x] : mn;
+      mx = s_smax[w][threadIdx.x] > mx ? s_smax[w][threadIdx.x] : mx;
+    }
+    atomicMin((unsigned long long*)&stat_min[threadIdx.x], (unsigned long long)mn);
+    atomicMax((unsigned long long*)&stat_max[threadIdx.x], (unsigned long long)mx);
+  }
+}
+
+// K2a: one pyramid level (chunker_countsort_laszip.cpp:1112-1180)
+__global__ void k_pyramid_level(const int32_t* __restrict__ high, int32_t* __restrict__ low, int64_t gl,
+                                int64_t cap) {
+  int64_t cells = gl * gl * gl;
+  int64_t gh = gl * 2;
+  for (int64_t il = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; il < cells; il += (int64_t)gridDim.x * blockDim.x) {
+    int64_t x = il % gl, y = (il / gl) % gl, z = il / (gl * gl);
+    int64_t sum = 0;
+    bool unmergeable = false;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      int64_t nx = 2 * x + ((j >> 2) & 1), ny = 2 * y + ((j >> 1) & 1), nz = 2 * z + (j & 1);
+      int32_t v = high[nx + ny * gh + nz * gh * gh];
+      if (v == -1) unmergeable = true; else sum += v;
+    }
+    low[il] = (unmergeable || sum > cap) ? -1 : (int32_t)sum;
+  }
+}
+
+// K2b: cells of level l (values `cur`) with count > 0 under an unmergeable
+// parent (`parent`, level l-1) become chunks (:1144-1172)
+__global__ void k_chunk_cells(const int32_t* __restrict__ cur, const int32_t* __restrict__ parent, int64_t g,
+                              int32_t level, int64_t* __restrict__ out, uint32_t* __restrict__ counter,
+                              uint32_t max_out) {
+  int64_t cells = g * g * g;
+  int64_t gp = g / 2;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cells; i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t v = cur[i];
+    if (v <= 0) continue;
+    int64_t x = i % g, y = (i / g) % g, z = i / (g * g);
+    int32_t pv = parent[(x / 2) + (y / 2) * gp + (z / 2) * gp * gp];
+    if (pv != -1) continue;
+    uint32_t slot = atomicAdd(counter, 1u);
+    if (slot < max_out) {
+      out[5 * slot + 0] = level; out[5 * slot + 1] = x; out[5 * slot + 2] = y; out[5 * slot + 3] = z;
+      out[5 * slot + 4] = v;
+    }
+  }
+}
+
+// K2c: LUT[cell] = chunk index for every fine cell covered by a chunk
+__global__ void k_lut_fill(const int64_t* __restrict__ chunks, int64_t num_chunks, int32_t level_max, int64_t G,
+                           int32_t* __restrict__ lut) {
+  for (int64_t c = blockIdx.x; c < num_chunks; c += gridDim.x) {
+    int64_t level = chunks[4 * c + 0], x = chunks[4 * c + 1], y = chunks[4 * c + 2], z = chunks[4 * c + 3];
+    int64_t s = (int64_t)1 << (level_max - level);
+    int64_t cells = s * s * s;
+    for (int64_t k = threadIdx.x; k < cells; k += blockDim.x) {
+      int64_t ox = k % s, oy = (k / s) % s, oz = k / (s * s);
+      lut[(s * x + ox) + (s * y + oy) * G + (s * z + oz) * G * G] = (int32_t)c;
+    }
+  }
+}
+
+// K3a: per-tile chunk histogram, written chunk-major: M[c * tiles + t]
+__global__ __launch_bounds__(kBlock) void k_tile_hist(const uint8_t* __restrict__ las, int64_t n, int32_t tile_size,
+                                                       const int32_t* __restrict__ lut, int32_t num_chunks,
+                                                       int64_t tiles, uint32_t* __restrict__ M,
+                                                       int32_t* __restrict__ err) {
+  extern __shared__ uint32_t s_hist[];  // num_chunks entries (or none: then global atomics)
+  __shared__ __attribute__((aligned(16))) uint8_t s_rec[kWavesPerBlock][64 * kMaxBppIn + 32];
+  const ChunkerParams& P = c_params;
+  const bool lds_hist = num_chunks <= 8192;
+  const int64_t t = blockIdx.x;
+  if (lds_hist)
+    for (int i = threadIdx.x; i < num_chunks; i += kBlock) s_hist[i] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int bpp = P.bpp_in;
+  const uint64_t total = (uint64_t)n * bpp;
+  const int64_t t0 = t * tile_size;
+  const int64_t t1 = (t0 + tile_size) < n ? (t0 + tile_size) : n;
+  bool bad = false;
+  for (int64_t first = t0 + (int64_t)wave * 64; first < t1; first += kBlock) {
+    int cnt = (int)((t1 - first) < 64 ? (t1 - first) : 64);
+    uint32_t shift = wave_load_records(las, total, first, cnt, bpp, s_rec[wave]);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if (lane < cnt) {
+      bool inBox;
+      int32_t q[3];
+      int64_t cell = cell_of_record(s_rec[wave] + shift + lane * bpp, P, inBox, q);
+      int32_t c = inBox ? lut[cell] : -1;
+      if (c < 0) bad = true;
+      else if (lds_hist) atomicAdd(&s_hist[c], 1u);
+      else atomicAdd(&M[(int64_t)c * tiles + t], 1u);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  }
+  if (bad) atomicOr(err, 2);
+  __syncthreads();
+  if (lds_hist)
+    for (int i = threadIdx.x; i < num_chunks; i += kBlock) M[(int64_t)i * tiles + t] = s_hist[i];
+}
+
+// peers of this lane with the same key in the wave (match-any by key bits)
+__device__ inline uint64_t wave_match(uint32_t key, int bits, uint64_t active) {
+  uint64_t peers = active;
+  for (int b = 0; b < bits; b++) {
+    uint64_t m = __ballot((key >> b) & 1);
+    peers &= ((key >> b) & 1) ? m : ~m;
+  }
+  return peers;
+}
+
+// K3b: transcode + stable scatter.  Each wave owns a contiguous quarter of
+// the tile; pass 1 counts per chunk per wave, pass 2 places each point at
+// M[c][t] + (earlier waves' count) + (running count) + (rank in step).
+__global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ las, int64_t n, int32_t tile_size,
+                                                     const int32_t* __restrict__ lut, int32_t num_chunks,
+                                                     int32_t key_bits, int64_t tiles,
+                                                     const uint32_t* __restrict__ M, uint8_t* __restrict__ out) {
+  extern __shared__ uint32_t s_cnt[];  // [kWavesPerBlock][num_chunks]
+  __shared__ __attribute__((aligned(16))) uint8_t s_rec[kWavesPerBlock][64 * kMaxBppIn + 32];
+  const ChunkerParams& P = c_params;
+  const int64_t t = blockIdx.x;
+  for (int i = threadIdx.x; i < kWavesPerBlock * num_chunks; i += kBlock) s_cnt[i] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int bpp = P.bpp_in, bpo = P.bpp_out;
+  const uint64_t total = (uint64_t)n * bpp;
+  const int64_t t0 = t * tile_size;
+  const int64_t t1 = (t0 + tile_size) < n ? (t0 + tile_size) : n;
+  const int64_t per_wave = (tile_size + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int64_t w0 = t0 + wave * per_wave;
+  const int64_t w1 = (w0 + per_wave) < t1 ? (w0 + per_wave) : t1;
+  uint32_t* my_cnt = s_cnt + wave * num_chunks;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // pass 1: counts
+  for (int64_t first = w0; first < w1; first += 64) {
+    int cnt = (int)((w1 - first) < 64 ? (w1 - first) : 64);
+    uint32_t shift = wave_load_records(las, total, first, cnt, bpp, s_rec[wave]);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    uint32_t c = 0;
+    bool valid = lane < cnt;
+    if (valid) {
+      bool inBox;
+      int32_t q[3];
+      int64_t cell = cell_of_record(s_rec[wave] + shift + lane * bpp, P, inBox, q);
+      int32_t cc = inBox ? lut[cell] : -1;
+      valid = cc >= 0;
+      c = valid ? (uint32_t)cc : 0u;
+    }
+    uint64_t active = __ballot(valid);
+    uint64_t peers = wave_match(c, key_bits, active);
+    if (valid && (peers & lt_mask) == 0) my_cnt[c] += (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  }
+  __syncthreads();
+  // exclusive prefix over waves, seeded with the global tile offset
+  for (int c = threadIdx.x; c < num_chunks; c += kBlock) {
+    uint32_t run = M[(int64_t)c * tiles + t];
+    for (int w = 0; w < kWavesPerBlock; w++) {
+      uint32_t v = s_cnt[w * num_chunks + c];
+      s_cnt[w * num_chunks + c] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  // pass 2: transcode + scatter
+  for (int64_t first = w0; first < w1; first += 64) {
+    int cnt = (int)((w1 - first) < 64 ? (w1 - first) : 64);
+    uint32_t shift = wave_load_records(las, total, first, cnt, bpp, s_rec[wave]);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    bool valid = lane < cnt;
+    const uint8_t* rec = s_rec[wave] + shift + lane * bpp;
+    uint32_t c = 0;
+    int32_t q[3] = {0, 0, 0};
+    if (valid) {
+      bool inBox;
+      int64_t cell = cell_of_record(rec, P, inBox, q);
+      int32_t cc = inBox ? lut[cell] : -1;
+      valid = cc >= 0;
+      c = valid ? (uint32_t)cc : 0u;
+    }
+    uint64_t active = __ballot(valid);
+    uint64_t peers = wave_match(c, key_bits, active);
+    uint32_t base = valid ? my_cnt[c] : 0;
+    __builtin_amdgcn_wave_barrier();
+    if (valid && (peers & lt_mask) == 0) my_cnt[c] = base + (uint32_t)__popcll(peers);
+    if (valid) {
+      uint64_t pos = (uint64_t)base + (uint64_t)__popcll(peers & lt_mask);
+      uint8_t* dst = out + pos * bpo;
+      uint8_t tmp[4];
+      for (int a = 0; a < 3; a++) {
+        __builtin_memcpy(tmp, &q[a], 4);
+        dst[4 * a + 0] = tmp[0]; dst[4 * a + 1] = tmp[1]; dst[4 * a + 2] = tmp[2]; dst[4 * a + 3] = tmp[3];
+      }
+      for (int o = 0; o < P.num_ops; o++) {
+        const TranscodeOp op = P.ops[o];
+        if (op.mode == TC_COPY) {
+          for (int b = 0; b < op.len; b++) dst[op.dst + b] = rec[op.src + b];
+        } else {
+          uint8_t v = rec[op.src];
+          dst[op.dst] = op.mode == TC_LOW3 ? (v & 7) : (op.mode == TC_MID3 ? ((v >> 3) & 7) : (v & 31));
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  }
+}
+
+__global__ void k_lasgen(LasGenParams p, const LasGenBuilding* __restrict__ bld, int64_t first, int64_t count,
+                         uint8_t* __restrict__ out, int32_t* __restrict__ qmin, int32_t* __restrict__ qmax) {
+  __shared__ LasGenBuilding s_b[64];
+  for (int i = threadIdx.x; i < p.num_buildings && i < 64; i += blockDim.x) s_b[i] = bld[i];
+  __syncthreads();
+  const int rl = p.extra_bytes ? 33 : 26;
+  int32_t mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+  uint8_t rec[40];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+    lasgen_point(p, s_b, first + i, rec);
+    uint8_t* d = out + i * rl;
+    for (int b = 0; b < rl; b++) d[b] = rec[b];
+    for (int a = 0; a < 3; a++) {
+      int32_t v = ld_i32(rec + 4 * a);
+      mn[a] = v < mn[a] ? v : mn[a];
+      mx[a] = v > mx[a] ? v : mx[a];
+    }
+  }
+  for (int a = 0; a < 3; a++) {
+    for (int d = 32; d >= 1; d >>= 1) {
+      int32_t o1 = __shfl_xor(mn[a], d, 64), o2 = __shfl_xor(mx[a], d, 64);
+      mn[a] = o1 < mn[a] ? o1 : mn[a];
+      mx[a] = o2 > mx[a] ? o2 : mx[a];
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+    for (int a = 0; a < 3; a++) { atomicMin(&qmin[a], mn[a]); atomicMax(&qmax[a], mx[a]); }
+  }
+}
+
+}  // namespace
+
+int chunker_set_params(const ChunkerParams& p, hipStream_t stream) {
+  if (p.bpp_in > kMaxBppIn) {
+    set_error("LAS record length above 128 bytes is not supported");
+    return 1;
+  }
+  PCG_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_params), &p, sizeof(p), 0, hipMemcpyHostToDevice, stream));
+  return 0;
+}
+
+int chunker_count(const uint8_t* d_las, int64_t n, int32_t* d_grid, uint64_t* d_stat_min, uint64_t* d_stat_max,
+                  unsigned long long* d_class_hist, int32_t* d_err, int num_cus, hipStream_t stream) {
+  int64_t steps = (n + 63) / 64;
+  int64_t blocks = (steps + kWavesPerBlock * kCountSteps - 1) / (kWavesPerBlock * kCountSteps);
+  int64_t max_blocks = (int64_t)num_cus * 8;
+  if (blocks > max_blocks) blocks = max_blocks;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_count, dim3((unsigned)blocks), dim3(kBlock), 0, stream, d_las, n, d_grid, d_stat_min,
+                     d_stat_max, d_class_hist, d_err);
+  PCG_CHECK(hipGetLastError());
+  return 0;
+}
+
+int chunker_pyramid(int32_t* const* d_levels, int32_t level_max, int64_t cap, hipStream_t stream) {
+  for (int32_t l = level_max - 1; l >= 0; l--) {
+    int64_t gl = (int64_t)1 << l;
+    int64_t cells = gl * gl * gl;
+    int64_t blocks = (cells + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_pyramid_level, dim3((unsigned)blocks), dim3(256), 0, stream, d_levels[l + 1], d_levels[l],
+                       gl, cap);
+    PCG_CHECK(hipGetLastError());
+  }
+  return 0;
+}
+
+int chunker_chunk_cells(int32_t* const* d_levels, int32_t level_max, int64_t* d_out, uint32_t* d_counter,
+                        uint32_t max_out, hipStream_t stream) {
+  for (int32_t l = 1; l <= level_max; l++) {
+    int64_t g = (int64_t)1 << l;
+    int64_t cells = g * g * g;
+    int64_t blocks = (cells + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_chunk_cells, dim3((unsigned)blocks), dim3(256), 0, stream, d_levels[l], d_levels[l - 1],
+                       g, l, d_out, d_counter, max_out);
+    PCG_CHECK(hipGetLastError());
+  }
+  return 0;
+}
+
+int chunker_lut_fill(const int64_t* d_chunks, int64_t num_chunks, int32_t level_max, int32_t* d_lut,
+                     hipStream_t stream) {
+  int64_t G = (int64_t)1 << level_max;
+  PCG_CHECK(hipMemsetAsync(d_lut, 0xFF, sizeof(int32_t) * G * G * G, stream));
+  if (num_chunks == 0) return 0;
+  int64_t blocks = num_chunks < 4096 ? num_chunks : 4096;
+  hipLaunchKernelGGL(k_lut_fill, dim3((unsigned)blocks), dim3(256), 0, stream, d_chunks, num_chunks, level_max, G,
+                     d_lut);
+  PCG_CHECK(hipGetLastError());
+  return 0;
+}
+
+int chunker_distribute(const uint8_t* d_las, int64_t n, const int32_t* d_lut, int32_t num_chunks, int32_t tile_size,
+                       uint32_t* d_M, void* d_scan_ws, uint8_t* d_out, int32_t* d_err, hipStream_t stream) {
+  int64_t tiles = (n + tile_size - 1) / tile_size;
+  if (tiles == 0) return 0;
+  size_t hist_lds = num_chunks <= 8192 ? sizeof(uint32_t) * num_chunks : 0;
+  if (num_chunks > 8192) PCG_CHECK(hipMemsetAsync(d_M, 0, sizeof(uint32_t) * tiles * num_chunks, stream));
+  hipLaunchKernelGGL(k_tile_hist, dim3((unsigned)tiles), dim3(kBlock), hist_lds, stream, d_las, n, tile_size, d_lut,
+                     num_chunks, tiles, d_M, d_err);
+  PCG_CHECK(hipGetLastError());
+  if (exclusive_scan_u32(d_M, tiles * num_chunks, d_scan_ws, nullptr, stream)) return 1;
+  int key_bits = 1;
+  while ((1 << key_bits) < num_chunks) key_bits++;
+  size_t cnt_lds = sizeof(uint32_t) * kWavesPerBlock * num_chunks;
+  if (cnt_lds > 96 * 1024) {
+    set_error("too many chunks for the LDS scatter counters");
+    return 1;
+  }
+  hipLaunchKernelGGL(k_scatter, dim3((unsigned)tiles), dim3(kBlock), cnt_lds, stream, d_las, n, tile_size, d_lut,
+                     num_chunks, key_bits, tiles, d_M, d_out);
+  PCG_CHECK(hipGetLastError());
+  return 0;
+}
+
+int lasgen_device(const LasGenParams& p, int64_t first, int64_t count, uint8_t* d_out, int32_t* d_qminmax,
+                  hipStream_t stream) {
+  LasGenBuilding bld[64];
+  int nb = p.num_buildings < 64 ? p.num_buildings : 64;
+  for (int b = 0; b < nb; b++) bld[b] = lasgen_building(p, b);
+  LasGenBuilding* d_bld = nullptr;
+  PCG_CHECK(hipMalloc(&d_bld, sizeof(bld)));
+  PCG_CHECK(hipMemcpyAsync(d_bld, bld, sizeof(bld), hipMemcpyHostToDevice, stream));
+  int64_t blocks = (count + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_lasgen, dim3((unsigned)blocks), dim3(256), 0, stream, p, d_bld, first, count, d_out, d_qminmax,
+                     d_qminmax + 3);
+  PCG_CHECK(hipGetLastError());
+  PCG_CHECK(hipStreamSynchronize(stream));
+  PCG_CHECK(hipFree(d_bld));
+  return 0;
+}
+
+}  // namespace pcg

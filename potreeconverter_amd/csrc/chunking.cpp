@@ -1,0 +1,339 @@
+// Device-resident doChunking: count -> pyramid/LUT -> stable scatter.
+// Restates Converter/src/chunker_countsort_laszip.cpp:1206-1261 with the
+// per-point work on gfx950 (chunker.hip) and the chunk list on the host.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <filesystem>
+#include <fstream>
+
+#include "chunker.h"
+#include "converter.h"
+#include "scan.h"
+
+namespace fs = std::filesystem;
+
+namespace pcg {
+
+static std::string g_error;
+void set_error(const std::string& msg) { g_error = msg; }
+const std::string& last_error() { return g_error; }
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+Device& device() {
+  static Device d;
+  return d;
+}
+
+int ensure_device() {
+  static bool ready = false;
+  if (ready) return 0;
+  Device& d = device();
+  int count = 0;
+  PCG_CHECK(hipGetDeviceCount(&count));
+  if (count < 1) {
+    set_error("no HIP device available");
+    return 1;
+  }
+  PCG_CHECK(hipSetDevice(d.id));
+  hipDeviceProp_t prop;
+  PCG_CHECK(hipGetDeviceProperties(&prop, d.id));
+  d.num_cus = prop.multiProcessorCount;
+  PCG_CHECK(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
+  ready = true;
+  return 0;
+}
+
+void DeviceChunks::release() {
+  if (d_records) (void)hipFree(d_records);
+  d_records = nullptr;
+}
+
+void child_box(const Vec3& min, const Vec3& max, int index, Vec3& cmin, Vec3& cmax) {
+  // converter_utils.h:106-136
+  Vec3 size = max - min;
+  Vec3 center = min + (size * 0.5);
+  cmin.x = (index & 4) ? center.x : min.x; cmax.x = (index & 4) ? max.x : center.x;
+  cmin.y = (index & 2) ? center.y : min.y; cmax.y = (index & 2) ? max.y : center.y;
+  cmin.z = (index & 1) ? center.z : min.z; cmax.z = (index & 1) ? max.z : center.z;
+}
+
+void chunk_box(const std::string& id, const Vec3& min, const Vec3& max, Vec3& cmin, Vec3& cmax) {
+  cmin = min;
+  cmax = max;
+  for (size_t i = 1; i < id.size(); i++) {
+    Vec3 a, b;
+    child_box(cmin, cmax, id[i] - '0', a, b);
+    cmin = a;
+    cmax = b;
+  }
+}
+
+// chunker_countsort_laszip.cpp:82-115
+static std::string node_id(int level, int64_t gridSize, int64_t x, int64_t y, int64_t z) {
+  std::string id = "r";
+  int64_t cur = gridSize;
+  for (int i = 0; i < level; i++) {
+    int index = 0;
+    if (x >= cur / 2) { index += 4; x -= cur / 2; }
+    if (y >= cur / 2) { index += 2; y -= cur / 2; }
+    if (z >= cur / 2) { index += 1; z -= cur / 2; }
+    id += char('0' + index);
+    cur /= 2;
+  }
+  return id;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  template <class T>
+  T* as() { return (T*)p; }
+};
+
+int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& out) {
+  if (ensure_device()) return 1;
+  Device& dev = device();
+  hipStream_t s = dev.stream;
+  double t0 = now_s();
+  ChunkerParams P;
+  out.release();
+  if (!output_attributes_for(h, out.attributes, P)) return 1;
+  Attributes& A = out.attributes;
+
+  // main.cpp:215-284 (computeStats) then chunker_countsort_laszip.cpp:205-207
+  Vec3 hmin(h.min[0], h.min[1], h.min[2]), hmax(h.max[0], h.max[1], h.max[2]);
+  double statsCube = (hmax - hmin).max();
+  if (statsCube == 0.0) {
+    set_error("invalid bounding box. at least one axis has a size of zero.");
+    return 123;
+  }
+  Vec3 smax = hmin + statsCube;
+  double cube = (smax - hmin).max();
+  int64_t N = (int64_t)h.num_points;
+  if (N >= (int64_t)4000000000LL) {
+    set_error("point counts of 4e9 and above are not supported");
+    return 1;
+  }
+  out.pointsTotal = N;
+  out.numPoints = N;
+  out.bpp = A.bytes;
+  out.min = hmin;
+  out.max = hmin + cube;
+  // :1210-1220
+  int64_t cap = std::min(N / 20, (int64_t)10000000);
+  int32_t G = N < 100000000 ? 128 : (N < 500000000 ? 256 : 512);
+  int32_t Lmax = G == 128 ? 7 : (G == 256 ? 8 : 9);
+  out.gridSize = G;
+  out.maxPointsPerChunk = cap;
+  for (int a = 0; a < 3; a++) {
+    P.las_scale[a] = h.scale[a];
+    P.las_offset[a] = h.offset[a];
+    P.pos_scale[a] = A.posScale[a];
+    P.pos_offset[a] = A.posOffset[a];
+    P.min[a] = hmin[a];
+  }
+  P.cube = cube;
+  P.grid_size = G;
+  if (chunker_set_params(P, s)) return 1;
+
+  const int64_t cells = (int64_t)G * G * G;
+  DevBuf grid, stats, hist, err;
+  PCG_CHECK(hipMalloc(&grid.p, sizeof(int32_t) * cells));
+  PCG_CHECK(hipMalloc(&stats.p, sizeof(uint64_t) * 2 * kMaxStats));
+  PCG_CHECK(hipMalloc(&hist.p, sizeof(unsigned long long) * 256));
+  PCG_CHECK(hipMalloc(&err.p, sizeof(int32_t) * 4));
+  PCG_CHECK(hipMemsetAsync(grid.p, 0, sizeof(int32_t) * cells, s));
+  PCG_CHECK(hipMemsetAsync(stats.p, 0xFF, sizeof(uint64_t) * kMaxStats, s));
+  PCG_CHECK(hipMemsetAsync(stats.as<uint64_t>() + kMaxStats, 0, sizeof(uint64_t) * kMaxStats, s));
+  PCG_CHECK(hipMemsetAsync(hist.p, 0, sizeof(unsigned long long) * 256, s));
+  PCG_CHECK(hipMemsetAsync(err.p, 0, sizeof(int32_t) * 4, s));
+  if (chunker_count(d_las, N, grid.as<int32_t>(), stats.as<uint64_t>(), stats.as<uint64_t>() + kMaxStats,
+                    hist.as<unsigned long long>(), err.as<int32_t>(), dev.num_cus, s))
+    return 1;
+  int32_t herr[4];
+  uint64_t hstats[2 * kMaxStats];
+  unsigned long long hhist[256];
+  PCG_CHECK(hipMemcpyAsync(herr, err.p, sizeof(herr), hipMemcpyDeviceToHost, s));
+  PCG_CHECK(hipMemcpyAsync(hstats, stats.p, sizeof(hstats), hipMemcpyDeviceToHost, s));
+  PCG_CHECK(hipMemcpyAsync(hhist, hist.p, sizeof(hhist), hipMemcpyDeviceToHost, s));
+  PCG_CHECK(hipStreamSynchronize(s));
+  if (herr[0] & 1) {
+    set_error("encountered point outside bounding box.");
+    return 123;
+  }
+  double t1 = now_s();
+  out.t_count = t1 - t0;
+
+  // attribute min/max (chunker_countsort_laszip.cpp:385-389, :914-932);
+  // position uses the decoded coordinate, monotone in the stored integer
+  {
+    int si = 0;
+    auto mn = [&](int k) { return from_ordered_bits(hstats[k]); };
+    auto mx = [&](int k) { return from_ordered_bits(hstats[kMaxStats + k]); };
+    Attribute& pos = A.list[0];
+    for (int a = 0; a < 3; a++) {
+      pos.min[a] = h.scale[a] * mn(si) + h.offset[a];
+      pos.max[a] = h.scale[a] * mx(si) + h.offset[a];
+      si++;
+    }
+    for (size_t i = 1; i < A.list.size(); i++) {
+      Attribute& at = A.list[i];
+      if (at.type == AttributeType::UNDEFINED || at.size == 0) continue;
+      int comps = at.numElements < 3 ? at.numElements : 3;
+      for (int e = 0; e < comps; e++) {
+        if (N > 0) { at.min[e] = mn(si); at.max[e] = mx(si); }
+        si++;
+      }
+    }
+    Attribute* cls = A.get("classification");
+    if (cls) for (int i = 0; i < 256; i++) cls->histogram[i] = (int64_t)hhist[i];
+  }
+
+  // pyramid + chunk cells (:1074-1183)
+  std::vector<DevBuf> levels(Lmax);
+  std::vector<int32_t*> lv(Lmax + 1);
+  for (int l = 0; l < Lmax; l++) {
+    int64_t g = (int64_t)1 << l;
+    PCG_CHECK(hipMalloc(&levels[l].p, sizeof(int32_t) * g * g * g));
+    lv[l] = levels[l].as<int32_t>();
+  }
+  lv[Lmax] = grid.as<int32_t>();
+  if (chunker_pyramid(lv.data(), Lmax, cap, s)) return 1;
+  uint32_t maxOut = 1 << 16;
+  std::vector<int64_t> raw;
+  uint32_t numChunks = 0;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    DevBuf list, counter;
+    PCG_CHECK(hipMalloc(&list.p, sizeof(int64_t) * 5 * maxOut));
+    PCG_CHECK(hipMalloc(&counter.p, sizeof(uint32_t)));
+    PCG_CHECK(hipMemsetAsync(counter.p, 0, sizeof(uint32_t), s));
+    if (chunker_chunk_cells(lv.data(), Lmax, list.as<int64_t>(), counter.as<uint32_t>(), maxOut, s)) return 1;
+    PCG_CHECK(hipMemcpyAsync(&numChunks, counter.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    PCG_CHECK(hipStreamSynchronize(s));
+    if (numChunks <= maxOut) {
+      raw.resize((size_t)5 * numChunks);
+      PCG_CHECK(hipMemcpy(raw.data(), list.p, sizeof(int64_t) * raw.size(), hipMemcpyDeviceToHost));
+      break;
+    }
+    maxOut = numChunks;
+  }
+  out.chunks.clear();
+  for (uint32_t c = 0; c < numChunks; c++) {
+    ChunkDesc d;
+    d.level = (int32_t)raw[5 * c + 0];
+    d.x = raw[5 * c + 1]; d.y = raw[5 * c + 2]; d.z = raw[5 * c + 3];
+    d.count = raw[5 * c + 4];
+    d.id = node_id(d.level, (int64_t)1 << d.level, d.x, d.y, d.z);
+    out.chunks.push_back(d);
+  }
+  std::sort(out.chunks.begin(), out.chunks.end(), [](const ChunkDesc& a, const ChunkDesc& b) { return a.id < b.id; });
+  std::vector<int64_t> sorted(4 * out.chunks.size());
+  int64_t off = 0;
+  for (size_t c = 0; c < out.chunks.size(); c++) {
+    ChunkDesc& d = out.chunks[c];
+    sorted[4 * c + 0] = d.level; sorted[4 * c + 1] = d.x; sorted[4 * c + 2] = d.y; sorted[4 * c + 3] = d.z;
+    d.offset = off;
+    off += d.count;
+    chunk_box(d.id, out.min, out.max, d.min, d.max);
+  }
+  if (off != N) {
+    set_error("chunk counts do not add up to the point count");
+    return 1;
+  }
+  DevBuf dchunks, lut;
+  PCG_CHECK(hipMalloc(&dchunks.p, sizeof(int64_t) * (sorted.size() + 4)));
+  PCG_CHECK(hipMemcpyAsync(dchunks.p, sorted.data(), sizeof(int64_t) * sorted.size(), hipMemcpyHostToDevice, s));
+  PCG_CHECK(hipMalloc(&lut.p, sizeof(int32_t) * cells));
+  if (chunker_lut_fill(dchunks.as<int64_t>(), (int64_t)out.chunks.size(), Lmax, lut.as<int32_t>(), s)) return 1;
+  levels.clear();
+
+  // distribute (:660-1003)
+  const int32_t tile = 4096;
+  int64_t tiles = (N + tile - 1) / tile;
+  int32_t C = (int32_t)out.chunks.size();
+  DevBuf M, ws;
+  PCG_CHECK(hipMalloc(&M.p, sizeof(uint32_t) * std::max<int64_t>(1, tiles * C)));
+  PCG_CHECK(hipMalloc(&ws.p, scan_workspace_bytes(tiles * C)));
+  PCG_CHECK(hipMalloc(&out.d_records, (size_t)N * out.bpp + 64));
+  int covered = 12;
+  for (int o = 0; o < P.num_ops; o++) covered += P.ops[o].len;
+  if (covered < out.bpp) PCG_CHECK(hipMemsetAsync(out.d_records, 0, (size_t)N * out.bpp, s));
+  if (chunker_distribute(d_las, N, lut.as<int32_t>(), C, tile, M.as<uint32_t>(), ws.p, out.d_records,
+                         err.as<int32_t>(), s))
+    return 1;
+  PCG_CHECK(hipMemcpyAsync(herr, err.p, sizeof(herr), hipMemcpyDeviceToHost, s));
+  PCG_CHECK(hipStreamSynchronize(s));
+  if (herr[0] & 2) {
+    set_error("point to node lookup failed, no node found.");
+    return 123;
+  }
+  double t2 = now_s();
+  out.t_distribute = t2 - t1;
+  out.t_total = t2 - t0;
+  return 0;
+}
+
+int write_chunk_files(const DeviceChunks& dc, const std::string& targetDir) {
+  std::string dir = targetDir + "/chunks";
+  fs::create_directories(dir);
+  for (const auto& e : fs::directory_iterator(dir)) fs::remove(e);
+  std::vector<uint8_t> host((size_t)dc.numPoints * dc.bpp);
+  if (!host.empty()) PCG_CHECK(hipMemcpy(host.data(), dc.d_records, host.size(), hipMemcpyDeviceToHost));
+  for (const auto& c : dc.chunks) {
+    if (c.count == 0) continue;
+    std::ofstream f(dir + "/" + c.id + ".bin", std::ios::binary);
+    f.write((const char*)host.data() + c.offset * dc.bpp, (std::streamsize)(c.count * dc.bpp));
+  }
+  std::ofstream m(dir + "/metadata.json", std::ios::binary);
+  m << chunk_metadata_json(dc.min, dc.max, dc.attributes);
+  return 0;
+}
+
+int read_chunk_files(const std::string& targetDir, DeviceChunks& out) {
+  if (ensure_device()) return 1;
+  std::string dir = targetDir + "/chunks";
+  std::ifstream mf(dir + "/metadata.json", std::ios::binary);
+  if (!mf) {
+    set_error("missing " + dir + "/metadata.json");
+    return 1;
+  }
+  std::string text((std::istreambuf_iterator<char>(mf)), std::istreambuf_iterator<char>());
+  out.release();
+  if (!parse_chunk_metadata(text, out.min, out.max, out.attributes)) return 1;
+  out.bpp = out.attributes.bytes;
+  out.chunks.clear();
+  for (const auto& e : fs::directory_iterator(dir)) {
+    std::string fn = e.path().filename().string();
+    if (fn.size() < 4 || fn.substr(fn.size() - 4) != ".bin") continue;
+    ChunkDesc d;
+    d.id = fn.substr(0, fn.size() - 4);
+    d.count = (int64_t)fs::file_size(e.path()) / out.bpp;
+    out.chunks.push_back(d);
+  }
+  std::sort(out.chunks.begin(), out.chunks.end(), [](const ChunkDesc& a, const ChunkDesc& b) { return a.id < b.id; });
+  int64_t off = 0;
+  for (auto& d : out.chunks) {
+    d.offset = off;
+    off += d.count;
+    d.level = (int32_t)d.id.size() - 1;
+    chunk_box(d.id, out.min, out.max, d.min, d.max);
+  }
+  out.numPoints = off;
+  out.pointsTotal = off;
+  std::vector<uint8_t> host((size_t)off * out.bpp);
+  for (auto& d : out.chunks) {
+    std::ifstream f(dir + "/" + d.id + ".bin", std::ios::binary);
+    f.read((char*)host.data() + d.offset * out.bpp, (std::streamsize)(d.count * out.bpp));
+  }
+  PCG_CHECK(hipMalloc(&out.d_records, host.size() + 64));
+  if (!host.empty()) PCG_CHECK(hipMemcpy(out.d_records, host.data(), host.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+}  // namespace pcg

@@ -1,0 +1,59 @@
+// Host orchestration of the device-resident chunker and indexer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "attributes.h"
+#include "lasfile.h"
+
+namespace pcg {
+
+struct ChunkDesc {
+  std::string id;     // "r" + one octal digit per level (x -> 4, y -> 2, z -> 1)
+  int32_t level = 0;
+  int64_t x = 0, y = 0, z = 0;
+  int64_t count = 0;
+  int64_t offset = 0;  // first record of the chunk in DeviceChunks::d_records
+  Vec3 min, max;       // childBoundingBoxOf chain from the cubic box
+};
+
+struct DeviceChunks {
+  uint8_t* d_records = nullptr;  // numPoints * bpp bytes, grouped by chunk (ascending ID)
+  int64_t numPoints = 0;
+  int32_t bpp = 0;
+  std::vector<ChunkDesc> chunks;
+  Attributes attributes;  // posScale/posOffset, min/max, classification histogram
+  Vec3 min, max;          // cubic bounding box written to chunks/metadata.json
+  int64_t pointsTotal = 0;
+  int32_t gridSize = 0;
+  int64_t maxPointsPerChunk = 0;
+  double t_count = 0, t_distribute = 0, t_total = 0;
+  void release();
+};
+
+struct Device {
+  int id = 0;
+  int num_cus = 256;
+  hipStream_t stream = nullptr;
+};
+
+Device& device();
+int ensure_device();
+
+// doChunking on LAS records already resident in HBM (header describes them).
+int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& out);
+
+// Writes chunks/<id>.bin + chunks/metadata.json like the reference chunker.
+int write_chunk_files(const DeviceChunks& dc, const std::string& targetDir);
+
+// Reads chunks/ written by any chunker (the reference's or ours) into HBM.
+int read_chunk_files(const std::string& targetDir, DeviceChunks& out);
+
+// Box of a chunk from its ID digits (indexer.cpp:161-170).
+void chunk_box(const std::string& id, const Vec3& min, const Vec3& max, Vec3& cmin, Vec3& cmax);
+void child_box(const Vec3& min, const Vec3& max, int index, Vec3& cmin, Vec3& cmax);
+
+}  // namespace pcg

@@ -1,0 +1,450 @@
+// Device-resident doIndexing (Converter/src/indexer.cpp:1581-1795).
+//
+// 1. Octree build: every chunk with >= 10 000 points is refined in rounds
+//    of the 32^3 counting sort (octree.hip); the host turns the realized
+//    cells into the node tree (expandTo, indexer.cpp:915-947).
+// 2. Sampling: the whole tree (root, nodes above the chunk roots, chunk
+//    subtrees) is sampled bottom-up one level per launch (sampler.hip).
+//    Per-node results depend only on the children's states, so this equals
+//    the reference's per-chunk post-order followed by the chunk-root phase.
+// 3. Canonical completion order (per chunk in ascending ID, post-order,
+//    children 0..7; then the chunk-root tasks of processChunkRoots in
+//    pre-order; then the root), octree.bin assembled in HBM by one gather,
+//    hierarchy.bin and metadata.json on the host.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <functional>
+#include <map>
+#include <unordered_map>
+
+#include "indexing.h"
+#include "octree.h"
+#include "sampler.h"
+#include "scan.h"
+
+namespace pcg {
+
+namespace {
+
+constexpr int64_t kLeafCap = 10000;  // indexer.h:47
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct TNode {
+  std::string name;
+  Vec3 min, max;
+  int32_t children[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+  int32_t parent = -1;
+  int64_t off = -1;      // leaf records: first record in the chunk buffer
+  int64_t count = 0;     // leaf records: count
+  bool chunkRoot = false;
+  bool upper = false;    // root or intermediate node above the chunk roots
+  bool hasChildren() const {
+    for (int c = 0; c < 8; c++) if (children[c] >= 0) return true;
+    return false;
+  }
+};
+
+struct Tree {
+  std::vector<TNode> nodes;
+  int32_t add(const std::string& name, const Vec3& mn, const Vec3& mx, int32_t parent) {
+    TNode t;
+    t.name = name;
+    t.min = mn;
+    t.max = mx;
+    t.parent = parent;
+    nodes.push_back(t);
+    int32_t id = (int32_t)nodes.size() - 1;
+    if (parent >= 0) nodes[parent].children[name.back() - '0'] = id;
+    return id;
+  }
+  // child of `id` with digit c, created with childBoundingBoxOf if missing
+  int32_t child(int32_t id, int c) {
+    if (nodes[id].children[c] >= 0) return nodes[id].children[c];
+    Vec3 a, b;
+    child_box(nodes[id].min, nodes[id].max, c, a, b);
+    return add(nodes[id].name + char('0' + c), a, b, id);
+  }
+};
+
+struct DevMem {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevMem() { if (p) (void)hipFree(p); }
+  int alloc(size_t b) {
+    if (p && bytes >= b) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    PCG_CHECK(hipMalloc(&p, b ? b : 16));
+    bytes = b;
+    return 0;
+  }
+  template <class T>
+  T* as() const { return (T*)p; }
+};
+
+}  // namespace
+
+void IndexResult::release() {
+  if (d_octree) (void)hipFree(d_octree);
+  d_octree = nullptr;
+}
+
+int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string& name, int64_t pointsTotal,
+                 IndexResult& out) {
+  if (ensure_device()) return 1;
+  hipStream_t s = device().stream;
+  const double t0 = now_s();
+  out.release();
+  out = IndexResult();
+  const bool avg = method == "poisson_average";
+  if (!avg && method != "poisson") {
+    set_error("unknown sampling method: " + method);
+    return 1;
+  }
+  Attributes attributes = dc.attributes;
+  // chunks/metadata.json round trip: non-finite min/max read back as +inf
+  for (auto& a : attributes.list)
+    for (int i = 0; i < 3; i++) {
+      if (!std::isfinite(a.min[i])) a.min[i] = INFINITY;
+      if (!std::isfinite(a.max[i])) a.max[i] = INFINITY;
+    }
+  const int32_t bpp = dc.bpp;
+  const int rgbOffset = attributes.getOffset("rgb");
+  if (avg && rgbOffset < 0) {
+    set_error("poisson_average needs an rgb attribute");
+    return 1;
+  }
+  const int64_t N = dc.numPoints;
+
+  // ---------------------------------------------------------------- tree
+  Tree tree;
+  tree.add("r", dc.min, dc.max, -1);
+  tree.nodes[0].upper = true;
+  std::vector<int32_t> chunkRoots;
+  for (const auto& c : dc.chunks) {
+    int32_t id;
+    if (c.id == "r") {
+      id = 0;
+    } else {
+      int32_t cur = 0;
+      for (size_t l = 1; l + 1 < c.id.size(); l++) {
+        cur = tree.child(cur, c.id[l] - '0');
+        tree.nodes[cur].upper = true;
+      }
+      // the chunk box comes from the ID chain of the cubic box, as getChunks
+      Vec3 mn, mx;
+      chunk_box(c.id, dc.min, dc.max, mn, mx);
+      id = tree.add(c.id, mn, mx, cur);
+    }
+    tree.nodes[id].chunkRoot = true;
+    tree.nodes[id].upper = false;
+    tree.nodes[id].off = c.offset;
+    tree.nodes[id].count = c.count;
+    chunkRoots.push_back(id);
+  }
+
+  // ------------------------------------------------------ octree build
+  const double tb0 = now_s();
+  DevMem bufB, keysA, keysB, dM, ws, dsegs, dtiles, dnodes, dcounter;
+  struct Seg { int32_t node; int64_t start, count; };
+  std::vector<Seg> segs;
+  for (int32_t id : chunkRoots)
+    if (tree.nodes[id].count >= kLeafCap) segs.push_back({id, tree.nodes[id].off, tree.nodes[id].count});
+  if (!segs.empty()) {
+    if (bufB.alloc((size_t)N * bpp + 64) || keysA.alloc((size_t)N * 2 + 64) || keysB.alloc((size_t)N * 2 + 64) ||
+        dcounter.alloc(16))
+      return 1;
+  }
+  OctreeParams op;
+  for (int a = 0; a < 3; a++) { op.scale[a] = attributes.posScale[a]; op.offset[a] = attributes.posOffset[a]; }
+  const int64_t tileSize = 4096;
+  int64_t octreeDepth = 0;
+  while (!segs.empty()) {
+    out.buildRounds++;
+    std::vector<SegDesc> sd(segs.size());
+    std::vector<TileDesc> td;
+    int64_t before = 0, tilesBefore = 0;
+    for (size_t i = 0; i < segs.size(); i++) {
+      const TNode& t = tree.nodes[segs[i].node];
+      SegDesc& d = sd[i];
+      d.start = segs[i].start;
+      d.count = segs[i].count;
+      for (int a = 0; a < 3; a++) { d.min[a] = t.min[a]; d.size[a] = t.max[a] - t.min[a]; }
+      d.delta = d.start - before;
+      int64_t nt = (d.count + tileSize - 1) / tileSize;
+      for (int64_t k = 0; k < nt; k++) {
+        TileDesc x;
+        x.seg = (int32_t)i;
+        x.tile_in_seg = (int32_t)k;
+        x.start = d.start + k * tileSize;
+        x.count = std::min(tileSize, d.count - k * tileSize);
+        x.m_base = 256 * tilesBefore;
+        x.seg_tiles = nt;
+        td.push_back(x);
+      }
+      tilesBefore += nt;
+      before += d.count;
+    }
+    const int64_t mTotal = 256 * tilesBefore;
+    const uint32_t maxNodes = (uint32_t)std::min<int64_t>(segs.size() * 32768ll, 1ll << 26);
+    if (dsegs.alloc(sd.size() * sizeof(SegDesc)) || dtiles.alloc(td.size() * sizeof(TileDesc)) ||
+        dM.alloc((size_t)mTotal * 4 + 64) || ws.alloc(scan_workspace_bytes(mTotal)) ||
+        dnodes.alloc((size_t)maxNodes * sizeof(NodeOut)))
+      return 1;
+    PCG_CHECK(hipMemcpyAsync(dsegs.p, sd.data(), sd.size() * sizeof(SegDesc), hipMemcpyHostToDevice, s));
+    PCG_CHECK(hipMemcpyAsync(dtiles.p, td.data(), td.size() * sizeof(TileDesc), hipMemcpyHostToDevice, s));
+    PCG_CHECK(hipMemsetAsync(dcounter.p, 0, 16, s));
+    if (octree_round(dc.d_records, bufB.as<uint8_t>(), keysA.as<uint16_t>(), keysB.as<uint16_t>(), bpp, op,
+                     dsegs.as<SegDesc>(), (int32_t)sd.size(), dtiles.as<TileDesc>(), (int64_t)td.size(), mTotal,
+                     dM.as<uint32_t>(), ws.p, dnodes.as<NodeOut>(), dcounter.as<uint32_t>(), maxNodes, s))
+      return 1;
+    uint32_t numOut = 0;
+    PCG_CHECK(hipMemcpyAsync(&numOut, dcounter.p, 4, hipMemcpyDeviceToHost, s));
+    PCG_CHECK(hipStreamSynchronize(s));
+    if (numOut > maxNodes) {
+      set_error("octree round produced more nodes than its buffer holds");
+      return 1;
+    }
+    std::vector<NodeOut> no(numOut);
+    if (numOut) PCG_CHECK(hipMemcpy(no.data(), dnodes.p, numOut * sizeof(NodeOut), hipMemcpyDeviceToHost));
+    std::vector<Seg> next;
+    for (const NodeOut& o : no) {
+      const Seg& sg = segs[o.seg];
+      int32_t id = sg.node;
+      for (int i = 0; i < o.level; i++) id = tree.child(id, (o.morton >> (3 * (o.level - 1 - i))) & 7);
+      TNode& t = tree.nodes[id];
+      t.off = sg.start + o.start;
+      t.count = o.count;
+      octreeDepth = std::max<int64_t>(octreeDepth, (int64_t)t.name.size() - 1);
+      if ((int64_t)o.count > kLeafCap) {
+        if ((int64_t)o.count == sg.count) {
+          // indexer.cpp:1002-1098 (all points in one 1/32^5 cell): the
+          // reference's duplicate-removal path is not implemented here
+          set_error("a node of " + std::to_string(o.count) + " points could not be split (" + t.name +
+                    "); the duplicate-point path of the reference is not supported");
+          return 1;
+        }
+        next.push_back({id, t.off, t.count});
+      }
+    }
+    // refined nodes hold no records themselves
+    for (const Seg& n2 : next) { tree.nodes[n2.node].off = -1; tree.nodes[n2.node].count = 0; }
+    segs.swap(next);
+  }
+  // segment roots that were split keep no records of their own
+  for (int32_t id : chunkRoots) {
+    TNode& t = tree.nodes[id];
+    if (t.count >= kLeafCap && t.hasChildren()) { t.off = -1; t.count = 0; }
+  }
+  PCG_CHECK(hipStreamSynchronize(s));
+  const double tb1 = now_s();
+  out.t_build = tb1 - tb0;
+  out.octreeDepth = octreeDepth;
+
+  // ------------------------------------------------------------ sampling
+  const int32_t numNodes = (int32_t)tree.nodes.size();
+  out.numNodes = numNodes;
+  std::vector<NodeInfo> info(numNodes);
+  std::vector<NodeState> state(numNodes);
+  int maxLevel = 0;
+  for (int32_t i = 0; i < numNodes; i++) {
+    const TNode& t = tree.nodes[i];
+    NodeInfo& f = info[i];
+    for (int c = 0; c < 8; c++) f.children[c] = t.children[c];
+    for (int a = 0; a < 3; a++) { f.min[a] = t.min[a]; f.max[a] = t.max[a]; }
+    f.level = (int32_t)t.name.size() - 1;
+    f.pad = 0;
+    maxLevel = std::max(maxLevel, f.level);
+    NodeState& st = state[i];
+    st.ptr = (t.off >= 0 && t.count > 0) ? dc.d_records + (size_t)t.off * bpp : nullptr;
+    st.count = t.off >= 0 ? t.count : 0;
+    st.accepted = st.count;
+    st.flags = 0;
+    st.pad = 0;
+  }
+  std::vector<std::vector<int32_t>> levels(maxLevel + 1);
+  const bool singleChunk = chunkRoots.size() == 1;
+  for (int32_t i = 0; i < numNodes; i++)
+    if (tree.nodes[i].hasChildren() && !(singleChunk && tree.nodes[i].upper)) levels[info[i].level].push_back(i);
+  int64_t maxLevelNodes = 1;
+  for (auto& l : levels) maxLevelNodes = std::max<int64_t>(maxLevelNodes, (int64_t)l.size());
+  DevMem dinfo, dstate, dwork, dlist, arena, scratch, tops, counters;
+  const int methodId = avg ? METHOD_POISSON_AVERAGE : METHOD_POISSON;
+  uint64_t arenaCap = (uint64_t)N * bpp * (avg ? 4 : 3) + (1u << 20);
+  // per node: linear arrays plus a hash table of at most 4n slots (the
+  // power of two >= 2n) or the Poisson-average minimum of 4096 slots
+  uint64_t scratchCap = (uint64_t)N * (avg ? 170 : 100) + (uint64_t)maxLevelNodes * (sample_scratch_bytes(1, methodId) + 40960);
+  if (dinfo.alloc(sizeof(NodeInfo) * numNodes) || dstate.alloc(sizeof(NodeState) * numNodes) ||
+      dwork.alloc(sizeof(NodeWork) * maxLevelNodes) || dlist.alloc(sizeof(int32_t) * maxLevelNodes) ||
+      arena.alloc(arenaCap) || scratch.alloc(scratchCap) || tops.alloc(64) || counters.alloc(64))
+    return 1;
+  PCG_CHECK(hipMemcpyAsync(dinfo.p, info.data(), sizeof(NodeInfo) * numNodes, hipMemcpyHostToDevice, s));
+  PCG_CHECK(hipMemcpyAsync(dstate.p, state.data(), sizeof(NodeState) * numNodes, hipMemcpyHostToDevice, s));
+  PCG_CHECK(hipMemsetAsync(tops.p, 0, 64, s));
+  PCG_CHECK(hipMemsetAsync(counters.p, 0, 64, s));
+  SampleParams sp;
+  memset(&sp, 0, sizeof(sp));
+  for (int a = 0; a < 3; a++) { sp.scale[a] = attributes.posScale[a]; sp.offset[a] = attributes.posOffset[a]; }
+  const double spacing = (dc.max - dc.min).x / 128.0;  // indexer.cpp:1603
+  sp.base_spacing = spacing;
+  sp.bpp = bpp;
+  sp.rgb_offset = rgbOffset;
+  sp.method = methodId;
+  sp.arena = arena.as<uint8_t>();
+  sp.arena_cap = arenaCap;
+  sp.arena_top = tops.as<uint64_t>();
+  sp.scratch = scratch.as<uint8_t>();
+  sp.scratch_cap = scratchCap;
+  sp.scratch_top = tops.as<uint64_t>() + 1;
+  sp.counters = counters.as<unsigned long long>();
+  std::vector<uint32_t> workFlags;
+  int64_t sampled = 0;
+  for (int l = maxLevel; l >= 0; l--) {
+    const auto& lst = levels[l];
+    if (lst.empty()) continue;
+    sampled += (int64_t)lst.size();
+    PCG_CHECK(hipMemcpyAsync(dlist.p, lst.data(), sizeof(int32_t) * lst.size(), hipMemcpyHostToDevice, s));
+    PCG_CHECK(hipMemsetAsync(tops.as<uint64_t>() + 1, 0, 8, s));
+    if (sample_level(dlist.as<int32_t>(), (int32_t)lst.size(), dinfo.as<NodeInfo>(), dstate.as<NodeState>(),
+                     dwork.as<NodeWork>(), sp, s))
+      return 1;
+    // the level list buffer is reused by the next level: wait for this one
+    PCG_CHECK(hipStreamSynchronize(s));
+  }
+  out.numSampled = sampled;
+  PCG_CHECK(hipMemcpy(state.data(), dstate.p, sizeof(NodeState) * numNodes, hipMemcpyDeviceToHost));
+  unsigned long long hc[8];
+  PCG_CHECK(hipMemcpy(hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost));
+  for (int i = 0; i < 8; i++) out.counters[i] = hc[i];
+  if (hc[6] != 0) {
+    set_error("sampling arena or scratch exhausted in " + std::to_string(hc[6]) + " nodes");
+    return 1;
+  }
+  const double ts1 = now_s();
+  out.t_sample = ts1 - tb1;
+
+  // --------------------------------------------- canonical completion order
+  std::vector<uint8_t> sampledFlag(numNodes, 0);
+  std::vector<int32_t> order;  // node ids in onNodeCompleted order
+  auto complete_children = [&](int32_t id) {
+    for (int c = 0; c < 8; c++) {
+      int32_t ch = tree.nodes[id].children[c];
+      if (ch >= 0 && (state[ch].flags & NS_COMPLETED)) order.push_back(ch);
+    }
+  };
+  // sample(): post-order over not-yet-sampled children (sampler_poisson.h:27-36)
+  // phase 1 samples each chunk subtree; afterwards the chunk roots are
+  // leaves of the upper tree (their children were cleared, indexer.cpp:1675)
+  std::function<void(int32_t, bool)> sample_rec = [&](int32_t id, bool phase1) {
+    if (!phase1 && tree.nodes[id].chunkRoot) {
+      sampledFlag[id] = 1;
+      return;
+    }
+    for (int c = 0; c < 8; c++) {
+      int32_t ch = tree.nodes[id].children[c];
+      if (ch < 0 || sampledFlag[ch]) continue;
+      sample_rec(ch, phase1);
+    }
+    sampledFlag[id] = 1;
+    if (tree.nodes[id].hasChildren()) complete_children(id);
+  };
+  for (int32_t id : chunkRoots) sample_rec(id, true);  // ascending chunk ID
+  // processChunkRoots (indexer.cpp:202-281): subtrees of the upper tree with
+  // fewer than 5M chunk-root points become one task; tasks in pre-order
+  {
+    std::vector<int64_t> crPoints(numNodes, 0);
+    std::vector<uint8_t> merged(numNodes, 0), hasFcr(numNodes, 0);
+    for (int32_t id : chunkRoots) {
+      crPoints[id] = (int64_t)(int32_t)state[id].accepted;
+      hasFcr[id] = 1;
+    }
+    std::function<void(int32_t)> post = [&](int32_t id) {
+      if (!tree.nodes[id].upper) return;
+      bool leaf = true;
+      for (int c = 0; c < 8; c++) {
+        int32_t ch = tree.nodes[id].children[c];
+        if (ch >= 0) { post(ch); leaf = false; }
+      }
+      if (leaf) return;
+      int32_t np = 0;
+      for (int c = 0; c < 8; c++) {
+        int32_t ch = tree.nodes[id].children[c];
+        if (ch >= 0) np += (int32_t)crPoints[ch];
+      }
+      crPoints[id] = np;
+      if (np < 5000000) {
+        merged[id] = 1;
+        for (int c = 0; c < 8; c++) {
+          int32_t ch = tree.nodes[id].children[c];
+          if (ch >= 0 && hasFcr[ch]) hasFcr[id] = 1;
+        }
+      }
+    };
+    post(0);
+    std::vector<int32_t> tasks;
+    std::function<void(int32_t)> pre = [&](int32_t id) {
+      if (hasFcr[id]) tasks.push_back(id);
+      if (merged[id]) return;  // its children were cleared in the CR tree
+      if (!tree.nodes[id].upper) return;
+      for (int c = 0; c < 8; c++) {
+        int32_t ch = tree.nodes[id].children[c];
+        if (ch >= 0) pre(ch);
+      }
+    };
+    pre(0);
+    for (int32_t t : tasks) sample_rec(t, false);
+  }
+  if (chunkRoots.size() == 1) {
+    // indexer.cpp:1736-1739: the single chunk root becomes the root
+  } else if (!sampledFlag[0]) {
+    sample_rec(0, false);
+  }
+  int32_t rootId = chunkRoots.size() == 1 ? chunkRoots[0] : 0;
+  order.push_back(rootId);
+
+  // byte offsets (Writer::writeAndUnload, indexer.cpp:1453-1516)
+  std::vector<CopyDesc> copies;
+  uint64_t offset = 0;
+  out.completed.clear();
+  for (int32_t id : order) {
+    CompletedNode cn;
+    cn.name = tree.nodes[id].name;
+    const NodeState& st = state[id];
+    cn.numPoints = st.count;
+    if (st.count != 0) {
+      uint64_t bytes = (uint64_t)st.count * bpp;
+      cn.byteOffset = (int64_t)offset;
+      cn.byteSize = (int64_t)bytes;
+      copies.push_back(CopyDesc{st.ptr, offset, bytes});
+      offset += bytes;
+    }
+    out.completed.push_back(cn);
+  }
+  out.octreeBytes = offset;
+  PCG_CHECK(hipMalloc(&out.d_octree, offset + 64));
+  DevMem dcopies;
+  if (dcopies.alloc(copies.size() * sizeof(CopyDesc) + 16)) return 1;
+  if (!copies.empty()) {
+    PCG_CHECK(hipMemcpyAsync(dcopies.p, copies.data(), copies.size() * sizeof(CopyDesc), hipMemcpyHostToDevice, s));
+    if (gather_ranges(dcopies.as<CopyDesc>(), (int64_t)copies.size(), out.d_octree, s)) return 1;
+  }
+  PCG_CHECK(hipStreamSynchronize(s));
+  const double ta1 = now_s();
+  out.t_assemble = ta1 - ts1;
+
+  // root name is the chunk id when there is a single chunk: keep its name
+  out.hierarchy = build_hierarchy_bin(out.completed, out.firstChunkSize);
+  out.metadata = create_metadata_json(attributes, dc.min, dc.max, spacing, name, pointsTotal, out.firstChunkSize,
+                                      octreeDepth);
+  out.t_total = now_s() - t0;
+  return 0;
+}
+
+}  // namespace pcg

@@ -1,0 +1,49 @@
+// Device-resident doIndexing (octree build + bottom-up sampling + writers).
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "converter.h"
+
+namespace pcg {
+
+// One onNodeCompleted call (indexer.cpp:1605-1608), in completion order.
+struct CompletedNode {
+  std::string name;
+  int64_t numPoints = 0;
+  int64_t byteOffset = 0;
+  int64_t byteSize = 0;
+};
+
+struct IndexResult {
+  uint8_t* d_octree = nullptr;   // octree.bin image in HBM
+  uint64_t octreeBytes = 0;
+  std::vector<CompletedNode> completed;
+  std::vector<uint8_t> hierarchy;  // hierarchy.bin
+  std::string metadata;            // metadata.json
+  int64_t octreeDepth = 0;
+  int64_t firstChunkSize = 0;
+  double t_build = 0, t_sample = 0, t_assemble = 0, t_total = 0;
+  uint64_t counters[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t numNodes = 0, numSampled = 0, buildRounds = 0;
+  void release();
+};
+
+// method: "poisson" | "poisson_average".  `name`/`pointsTotal` go to
+// metadata.json as in createMetadata (indexer.cpp:354-524).
+int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string& name, int64_t pointsTotal,
+                 IndexResult& out);
+
+int write_index_files(const IndexResult& r, const std::string& targetDir);
+
+// HierarchyFlusher + HierarchyBuilder over the completion records
+// (indexer.h:120-245, HierarchyBuilder.h:26-354); batches in ascending name.
+std::vector<uint8_t> build_hierarchy_bin(const std::vector<CompletedNode>& recs, int64_t& firstChunkSize);
+
+std::string create_metadata_json(const Attributes& attributes, const Vec3& min, const Vec3& max, double spacing,
+                                 const std::string& name, int64_t pointsTotal, int64_t firstChunkSize,
+                                 int64_t octreeDepth);
+
+}  // namespace pcg

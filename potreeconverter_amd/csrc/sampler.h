@@ -1,0 +1,83 @@
+// Bottom-up Poisson / Poisson-average node sampling on gfx950 (sampler.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pcg {
+
+enum SampleMethod : int32_t { METHOD_POISSON = 0, METHOD_POISSON_AVERAGE = 1 };
+
+// node state flags
+enum : int32_t {
+  NS_COMPLETED = 1,   // onNodeCompleted was called for the node
+  NS_DISCARDED = 2,   // removed from its parent without completion (Poisson-average)
+  NS_SAMPLED = 4,
+};
+
+// per-node work flags (set by the kernels)
+enum : uint32_t {
+  NW_EXACT_SORT = 1,   // equal-distance candidates within spacing: libstdc++ sort order needed
+  NW_SEQ_ACCEPT = 2,   // the 10 001-check window of the Poisson scan may bind
+  NW_SEQ_AVERAGE = 4,  // two accepted points inside each other's colour range
+  NW_OVERFLOW = 8,     // scratch or arena exhausted
+};
+
+struct NodeInfo {
+  int32_t children[8];  // node ids, -1 = none
+  double min[3];
+  double max[3];
+  int32_t level;        // name.size() - 1
+  int32_t pad;
+};
+
+struct NodeState {
+  uint8_t* ptr;      // current records
+  int64_t count;     // current numPoints
+  int64_t accepted;  // numPoints right after the node's own sampling
+  int32_t flags;     // NS_*
+  int32_t pad;
+};
+
+struct NodeWork {
+  uint8_t* scratch;
+  int64_t n;              // candidates
+  int64_t child_start[9]; // candidate index of each child's first point
+  uint32_t flags;         // NW_*
+  uint32_t pad;
+};
+
+struct SampleParams {
+  double scale[3];
+  double offset[3];
+  double base_spacing;
+  int32_t bpp;
+  int32_t rgb_offset;
+  int32_t method;
+  int32_t pad;
+  uint8_t* arena;          // record arena
+  uint64_t arena_cap;      // bytes
+  uint64_t* arena_top;     // bytes used (device counter)
+  uint8_t* scratch;        // per-level scratch
+  uint64_t scratch_cap;
+  uint64_t* scratch_top;
+  unsigned long long* counters;  // [0] fp64 re-checks, [1] pair tests, [2] max |dd32-dd64|/s^2 bits,
+                                 // [3] exact-sort nodes, [4] sequential nodes, [5] candidates,
+                                 // [6] nodes that ran out of scratch or arena
+};
+
+// Samples every node of `nodes` (all of one level, children already done).
+int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_info, NodeState* d_state,
+                 NodeWork* d_work, const SampleParams& p, hipStream_t s);
+
+// Scratch bytes for one node of n candidates.
+uint64_t sample_scratch_bytes(int64_t n, int method);
+
+// Copies each (src, bytes) range to dst + offset (one workgroup per range).
+struct CopyDesc {
+  const uint8_t* src;
+  uint64_t dst_offset;
+  uint64_t bytes;
+};
+int gather_ranges(const CopyDesc* d_desc, int64_t num, uint8_t* d_dst, hipStream_t s);
+
+}  // namespace pcg

@@ -1,0 +1,859 @@
+// Poisson and Poisson-average node sampling on gfx950, one workgroup per
+// node, all nodes of one octree level per launch.
+//
+// Restates Converter/include/sampler_poisson.h:14-304 and
+// sampler_poisson_average.h:14-492 for one inner node:
+//   k_prepare      gather the children's points in child-then-index order
+//                  (sampler_poisson.h:82-108), squared distance to the node
+//                  centre, stable radix sort by float(distance) with an
+//                  exact (double, index) fix-up, detection of equal
+//                  distances that can change a decision
+//   k_exact_sort   libstdc++ std::sort order for those nodes (:182-203)
+//   k_accept       greedy acceptance (:133-180 / :184-281) in windows of
+//                  256 sorted candidates against a hash grid of 2*spacing
+//                  cells, FP32 distance filter with FP64 re-check near the
+//                  threshold; Poisson-average colour sums (:357-416) with
+//                  int64 atomics
+//   k_sequential   exact scans for nodes where the windowed form could
+//                  differ (10 001-check cap, accepted points in each
+//                  other's colour range)
+//   k_partition    accepted records to the node, the rest back to the
+//                  children in their order (:247-291 / :418-482)
+#include "lstd_sort.h"
+#include "pcg_common.h"
+#include "sampler.h"
+
+namespace pcg {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr uint32_t kNil = 0xFFFFFFFFu;
+
+__device__ inline int32_t ld_i32(const uint8_t* p) {
+  return (int32_t)((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24));
+}
+__device__ inline uint16_t ld_u16(const uint8_t* p) { return (uint16_t)(p[0] | (p[1] << 8)); }
+
+__host__ __device__ inline uint64_t align16(uint64_t v) { return (v + 15) & ~15ull; }
+
+__host__ __device__ inline uint32_t hash_cap(int64_t n, int method) {
+  uint32_t h = 64;
+  while ((int64_t)h < 2 * n) h <<= 1;
+  if (method == METHOD_POISSON_AVERAGE && h < 4096) h = 4096;
+  return h;
+}
+
+struct Layout {
+  int32_t *X, *Y, *Z;
+  double* add;       // main order
+  uint32_t *keyA, *idxA, *keyB, *idxB;
+  double* sadd;      // sorted order
+  uint32_t* sidx;    // sorted -> main (points at idxA or idxB)
+  uint8_t* acc;      // sorted order
+  uint32_t* cell16;  // main order, packed 16^3 cell (average)
+  uint16_t* rgb;     // main order, own colour (average)
+  uint32_t *hkey, *hhead, *next;
+  uint32_t H;
+  long long* sums;   // sorted order, 8 per candidate (average)
+};
+
+__host__ __device__ inline uint64_t layout_bytes(int64_t n, int method) {
+  uint64_t H = hash_cap(n, method);
+  uint64_t b = 3 * align16(4 * n) + align16(8 * n) + 4 * align16(4 * n) + align16(8 * n) + align16(n) +
+               align16(4 * n) + 2 * align16(4 * H) + align16(4 * n);
+  if (method == METHOD_POISSON_AVERAGE) b += align16(64 * n) + align16(6 * n);
+  return b + 64;
+}
+
+__device__ inline Layout make_layout(uint8_t* base, int64_t n, int method) {
+  Layout L;
+  uint8_t* p = base;
+  auto take = [&](uint64_t bytes) { uint8_t* r = p; p += align16(bytes); return r; };
+  L.X = (int32_t*)take(4 * n);
+  L.Y = (int32_t*)take(4 * n);
+  L.Z = (int32_t*)take(4 * n);
+  L.add = (double*)take(8 * n);
+  L.keyA = (uint32_t*)take(4 * n);
+  L.idxA = (uint32_t*)take(4 * n);
+  L.keyB = (uint32_t*)take(4 * n);
+  L.idxB = (uint32_t*)take(4 * n);
+  L.sadd = (double*)take(8 * n);
+  L.acc = (uint8_t*)take(n);
+  L.cell16 = (uint32_t*)take(4 * n);
+  L.H = hash_cap(n, method);
+  L.hkey = (uint32_t*)take(4 * (uint64_t)L.H);
+  L.hhead = (uint32_t*)take(4 * (uint64_t)L.H);
+  L.next = (uint32_t*)take(4 * n);
+  L.sums = method == METHOD_POISSON_AVERAGE ? (long long*)take(64 * n) : nullptr;
+  L.rgb = method == METHOD_POISSON_AVERAGE ? (uint16_t*)take(6 * n) : nullptr;
+  L.sidx = L.idxA;
+  return L;
+}
+
+__device__ inline double coord(int32_t v, double scale, double offset) { return ((double)v * scale) + offset; }
+
+struct NodeGeom {
+  double cx, cy, cz;        // (min + max) * 0.5
+  double spacing, s2;
+  double min[3], size[3];
+};
+
+__device__ inline NodeGeom node_geom(const NodeInfo& info, const SampleParams& P) {
+  NodeGeom g;
+  g.cx = (info.min[0] + info.max[0]) * 0.5;
+  g.cy = (info.min[1] + info.max[1]) * 0.5;
+  g.cz = (info.min[2] + info.max[2]) * 0.5;
+  // sampler_poisson.h:114 spacing = baseSpacing / pow(2, level) (exact: power of two)
+  double p2 = 1.0;
+  for (int i = 0; i < info.level; i++) p2 *= 2.0;
+  g.spacing = P.base_spacing / p2;
+  g.s2 = g.spacing * g.spacing;
+  for (int a = 0; a < 3; a++) { g.min[a] = info.min[a]; g.size[a] = info.max[a] - info.min[a]; }
+  return g;
+}
+
+// the 16^3 accept-grid cell of a position and the range searched around it
+// (sampler_poisson_average.h:186-208)
+__device__ inline int cell16_axis(double v, double mn, double size) {
+  double d = 16.0 * (v - mn) / size;
+  d = d < 15.0 ? d : 15.0;
+  d = d > 0.0 ? d : 0.0;
+  return (int)d;
+}
+
+__device__ inline uint32_t pack16(int x, int y, int z) { return (uint32_t)(x | (y << 4) | (z << 8)); }
+
+__device__ inline void range16(const NodeGeom& g, double x, double y, double z, int lo[3], int hi[3]) {
+  double v[3] = {x, y, z};
+  for (int a = 0; a < 3; a++) {
+    lo[a] = cell16_axis(v[a] - g.spacing, g.min[a], g.size[a]);
+    hi[a] = cell16_axis(v[a] + g.spacing, g.min[a], g.size[a]);
+  }
+}
+
+__device__ inline bool in_range16(uint32_t c, const int lo[3], const int hi[3]) {
+  int x = c & 15, y = (c >> 4) & 15, z = (c >> 8) & 15;
+  return x >= lo[0] && x <= hi[0] && y >= lo[1] && y <= hi[1] && z >= lo[2] && z <= hi[2];
+}
+
+// hash-grid cell (2 * spacing) of a coordinate, shifted by one so points on
+// or just outside the box boundary stay in range
+__device__ inline int gcell(double v, double mn, double cs) {
+  double d = (v - mn) / cs;
+  int64_t i = (int64_t)__builtin_floor(d) + 1;
+  return (int)(i < 0 ? 0 : (i > 127 ? 127 : i));
+}
+__device__ inline uint32_t gkey(int x, int y, int z) { return (uint32_t)(x | (y << 7) | (z << 14)); }
+__device__ inline uint32_t ghash(uint32_t k, uint32_t H) { return (k * 2654435761u) & (H - 1); }
+
+__device__ inline uint32_t grid_find(const Layout& L, uint32_t key) {
+  uint32_t slot = ghash(key, L.H);
+  for (uint32_t probe = 0; probe < L.H; probe++) {
+    uint32_t k = L.hkey[slot];
+    if (k == key) return L.hhead[slot];
+    if (k == kNil) return kNil;
+    slot = (slot + 1) & (L.H - 1);
+  }
+  return kNil;
+}
+
+__device__ inline uint64_t wave_match8(uint32_t key, uint64_t active) {
+  uint64_t peers = active;
+#pragma unroll
+  for (int b = 0; b < 8; b++) {
+    uint64_t m = __ballot((key >> b) & 1);
+    peers &= ((key >> b) & 1) ? m : ~m;
+  }
+  return peers;
+}
+
+// ---------------------------------------------------------------- prepare
+__global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
+                                                     const NodeState* __restrict__ states, NodeWork* __restrict__ work,
+                                                     SampleParams P) {
+  __shared__ int64_t s_cs[9];
+  __shared__ const uint8_t* s_cptr[8];
+  __shared__ uint8_t* s_scr;
+  __shared__ uint32_t s_hist[256];
+  __shared__ uint32_t s_base[256];
+  __shared__ uint32_t s_wc[kWaves][256];
+  __shared__ int s_skip;
+  const int k = blockIdx.x;
+  const NodeInfo info = infos[nodes[k]];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) {
+    s_cs[0] = 0;
+    for (int c = 0; c < 8; c++) {
+      int ch = info.children[c];
+      int64_t cnt = ch >= 0 ? states[ch].count : 0;
+      s_cptr[c] = ch >= 0 ? states[ch].ptr : nullptr;
+      s_cs[c + 1] = s_cs[c] + cnt;
+    }
+    int64_t n = s_cs[8];
+    uint64_t bytes = layout_bytes(n, P.method);
+    uint64_t off = atomicAdd((unsigned long long*)P.scratch_top, (unsigned long long)bytes);
+    NodeWork w;
+    w.n = n;
+    for (int c = 0; c < 9; c++) w.child_start[c] = s_cs[c];
+    w.flags = 0;
+    w.pad = 0;
+    if (off + bytes > P.scratch_cap) {
+      w.flags = NW_OVERFLOW;
+      w.scratch = nullptr;
+      s_scr = nullptr;
+      atomicAdd(&P.counters[6], 1ull);
+    } else {
+      w.scratch = P.scratch + off;
+      s_scr = w.scratch;
+    }
+    work[k] = w;
+  }
+  __syncthreads();
+  if (!s_scr) return;
+  const int64_t n = s_cs[8];
+  Layout L = make_layout(s_scr, n, P.method);
+  NodeGeom g = node_geom(info, P);
+  // gather (sampler_poisson.h:82-108)
+  for (int64_t j = tid; j < n; j += kBlock) {
+    int c = 0;
+    while (c < 7 && j >= s_cs[c + 1]) c++;
+    const uint8_t* r = s_cptr[c] + (j - s_cs[c]) * P.bpp;
+    int32_t X = ld_i32(r), Y = ld_i32(r + 4), Z = ld_i32(r + 8);
+    double x = coord(X, P.scale[0], P.offset[0]), y = coord(Y, P.scale[1], P.offset[1]),
+           z = coord(Z, P.scale[2], P.offset[2]);
+    double ax = x - g.cx, ay = y - g.cy, az = z - g.cz;
+    double add = ax * ax + ay * ay + az * az;
+    L.X[j] = X; L.Y[j] = Y; L.Z[j] = Z;
+    L.add[j] = add;
+    float f = (float)add;
+    uint32_t fb;
+    __builtin_memcpy(&fb, &f, 4);
+    L.keyA[j] = fb;
+    L.idxA[j] = (uint32_t)j;
+    if (P.method == METHOD_POISSON_AVERAGE) {
+      L.cell16[j] = pack16(cell16_axis(x, g.min[0], g.size[0]), cell16_axis(y, g.min[1], g.size[1]),
+                           cell16_axis(z, g.min[2], g.size[2]));
+      L.rgb[3 * j + 0] = ld_u16(r + P.rgb_offset);
+      L.rgb[3 * j + 1] = ld_u16(r + P.rgb_offset + 2);
+      L.rgb[3 * j + 2] = ld_u16(r + P.rgb_offset + 4);
+      for (int q = 0; q < 8; q++) L.sums[8 * j + q] = 0;
+    }
+  }
+  __syncthreads();
+  // stable LSD radix sort on float(add) bits, 4 x 8 bits
+  uint32_t *kc = L.keyA, *ic = L.idxA, *ko = L.keyB, *io = L.idxB;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  for (int pass = 0; pass < 4; pass++) {
+    const int shift = 8 * pass;
+    s_hist[tid] = 0;
+    __syncthreads();
+    for (int64_t j = tid; j < n; j += kBlock) atomicAdd(&s_hist[(kc[j] >> shift) & 255], 1u);
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t run = 0;
+      s_skip = 0;
+      for (int b = 0; b < 256; b++) {
+        if ((int64_t)s_hist[b] == n) s_skip = 1;
+        s_base[b] = run;
+        run += s_hist[b];
+      }
+    }
+    __syncthreads();
+    if (s_skip) continue;
+    for (int64_t b0 = 0; b0 < n; b0 += kBlock) {
+      for (int w = 0; w < kWaves; w++) s_wc[w][tid] = 0;
+      __syncthreads();
+      int64_t j = b0 + tid;
+      bool valid = j < n;
+      uint32_t key = valid ? kc[j] : 0;
+      uint32_t d = (key >> shift) & 255;
+      uint64_t peers = wave_match8(d, __ballot(valid));
+      if (valid && (peers & lt) == 0) s_wc[wave][d] = (uint32_t)__popcll(peers);
+      __syncthreads();
+      if (valid) {
+        uint32_t pos = s_base[d] + (uint32_t)__popcll(peers & lt);
+        for (int w = 0; w < wave; w++) pos += s_wc[w][d];
+        ko[pos] = key;
+        io[pos] = ic[j];
+      }
+      __syncthreads();
+      s_base[tid] += s_wc[0][tid] + s_wc[1][tid] + s_wc[2][tid] + s_wc[3][tid];
+      __syncthreads();
+    }
+    uint32_t* t = kc; kc = ko; ko = t;
+    t = ic; ic = io; io = t;
+  }
+  // runs of equal float keys: order by (double add, main index)
+  for (int64_t p = tid; p < n; p += kBlock) {
+    bool start = (p == 0 || kc[p - 1] != kc[p]) && p + 1 < n && kc[p + 1] == kc[p];
+    if (!start) continue;
+    int64_t e = p + 1;
+    while (e < n && kc[e] == kc[p]) e++;
+    for (int64_t i = p + 1; i < e; i++) {
+      uint32_t v = ic[i];
+      double dv = L.add[v];
+      int64_t q = i - 1;
+      while (q >= p && (L.add[ic[q]] > dv || (L.add[ic[q]] == dv && ic[q] > v))) {
+        ic[q + 1] = ic[q];
+        q--;
+      }
+      ic[q + 1] = v;
+    }
+  }
+  __syncthreads();
+  if (ic != L.idxA)
+    for (int64_t p = tid; p < n; p += kBlock) L.idxA[p] = ic[p];
+  __syncthreads();
+  for (int64_t p = tid; p < n; p += kBlock) L.sadd[p] = L.add[L.idxA[p]];
+  __syncthreads();
+  // equal distances: the reference's unstable sort decides their order;
+  // it matters only if two of them can exclude each other
+  bool conflict = false;
+  for (int64_t p = tid; p + 1 < n; p += kBlock) {
+    if (!(L.sadd[p] == L.sadd[p + 1]) || (p > 0 && L.sadd[p - 1] == L.sadd[p])) continue;
+    int64_t e = p + 1;
+    while (e < n && L.sadd[e] == L.sadd[p]) e++;
+    if (e - p > 64) { conflict = true; continue; }
+    for (int64_t a = p; a < e && !conflict; a++) {
+      for (int64_t b = a + 1; b < e && !conflict; b++) {
+        uint32_t ia = L.idxA[a], ib = L.idxA[b];
+        double xa = coord(L.X[ia], P.scale[0], P.offset[0]), ya = coord(L.Y[ia], P.scale[1], P.offset[1]),
+               za = coord(L.Z[ia], P.scale[2], P.offset[2]);
+        double xb = coord(L.X[ib], P.scale[0], P.offset[0]), yb = coord(L.Y[ib], P.scale[1], P.offset[1]),
+               zb = coord(L.Z[ib], P.scale[2], P.offset[2]);
+        double dx = xa - xb, dy = ya - yb, dz = za - zb;
+        double dd = dx * dx + dy * dy + dz * dz;
+        if (!(dd < g.s2)) continue;
+        if (P.method == METHOD_POISSON) {
+          double cd = __builtin_sqrt(L.sadd[p]);
+          double lim = cd - g.spacing;
+          if (L.sadd[p] >= lim * lim) conflict = true;
+        } else {
+          int lo[3], hi[3];
+          range16(g, xa, ya, za, lo, hi);
+          if (in_range16(L.cell16[ib], lo, hi)) conflict = true;
+          range16(g, xb, yb, zb, lo, hi);
+          if (in_range16(L.cell16[ia], lo, hi)) conflict = true;
+        }
+      }
+    }
+  }
+  if (conflict) atomicOr(&work[k].flags, (uint32_t)NW_EXACT_SORT);
+}
+
+// ------------------------------------------------------------- exact sort
+__global__ void k_exact_sort(const int32_t* __restrict__ nodes, NodeWork* __restrict__ work, SampleParams P,
+                             unsigned long long* counters) {
+  const int k = blockIdx.x;
+  NodeWork w = work[k];
+  if (!(w.flags & NW_EXACT_SORT) || (w.flags & NW_OVERFLOW)) return;
+  Layout L = make_layout(w.scratch, w.n, P.method);
+  if (threadIdx.x == 0) {
+    atomicAdd(&counters[3], 1ull);
+    for (int64_t j = 0; j < w.n; j++) L.idxA[j] = (uint32_t)j;
+    lstd::sort(L.idxA, w.n, lstd::KeyLess{L.add});
+  }
+  __syncthreads();
+  for (int64_t p = threadIdx.x; p < w.n; p += blockDim.x) L.sadd[p] = L.add[L.idxA[p]];
+}
+
+// ------------------------------------------------------------- acceptance
+__device__ inline double dd64_pair(const Layout& L, uint32_t a, uint32_t b, const SampleParams& P) {
+  double dx = coord(L.X[a], P.scale[0], P.offset[0]) - coord(L.X[b], P.scale[0], P.offset[0]);
+  double dy = coord(L.Y[a], P.scale[1], P.offset[1]) - coord(L.Y[b], P.scale[1], P.offset[1]);
+  double dz = coord(L.Z[a], P.scale[2], P.offset[2]) - coord(L.Z[b], P.scale[2], P.offset[2]);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// exact "dd < spacing^2" through an FP32 filter on integer deltas; the
+// FP64 reference expression is evaluated only inside a +-1e-3 band
+__device__ inline bool within_spacing(const Layout& L, uint32_t a, uint32_t b, const SampleParams& P, float sc[3],
+                                      float s2f, double s2, unsigned long long& rechecks, float& maxerr) {
+  float dx = (float)((int64_t)L.X[a] - (int64_t)L.X[b]) * sc[0];
+  float dy = (float)((int64_t)L.Y[a] - (int64_t)L.Y[b]) * sc[1];
+  float dz = (float)((int64_t)L.Z[a] - (int64_t)L.Z[b]) * sc[2];
+  float d32 = dx * dx + dy * dy + dz * dz;
+  if (d32 > s2f * 1.001f) return false;
+  if (d32 < s2f * 0.999f) return true;
+  rechecks++;
+  double d64 = dd64_pair(L, a, b, P);
+  float e = (float)(__builtin_fabs((double)d32 - d64) / s2);
+  maxerr = e > maxerr ? e : maxerr;
+  return d64 < s2;
+}
+
+__global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
+                                                    NodeWork* __restrict__ work, SampleParams P) {
+  __shared__ unsigned long long s_dec[4], s_acc[4];
+  __shared__ int s_done;
+  __shared__ double s_minlim;
+  const int k = blockIdx.x;
+  NodeWork w = work[k];
+  if (w.flags & NW_OVERFLOW) return;
+  const int64_t n = w.n;
+  const int tid = threadIdx.x;
+  Layout L = make_layout(w.scratch, n, P.method);
+  const NodeInfo info = infos[nodes[k]];
+  NodeGeom g = node_geom(info, P);
+  const bool avg = P.method == METHOD_POISSON_AVERAGE;
+  const double cs = 2.0 * g.spacing;
+  const double sq = g.spacing * (1.0 + 1e-9);
+  float sc[3] = {(float)P.scale[0], (float)P.scale[1], (float)P.scale[2]};
+  const float s2f = (float)g.s2;
+  unsigned long long rechecks = 0, tests = 0;
+  float maxerr = 0.f;
+  // hash grid over all candidates, entries are sorted positions
+  for (uint32_t i = tid; i < L.H; i += kBlock) { L.hkey[i] = kNil; L.hhead[i] = kNil; }
+  __syncthreads();
+  for (int64_t p = tid; p < n; p += kBlock) {
+    uint32_t j = L.idxA[p];
+    double x = coord(L.X[j], P.scale[0], P.offset[0]), y = coord(L.Y[j], P.scale[1], P.offset[1]),
+           z = coord(L.Z[j], P.scale[2], P.offset[2]);
+    uint32_t key = gkey(gcell(x, g.min[0], cs), gcell(y, g.min[1], cs), gcell(z, g.min[2], cs));
+    uint32_t slot = ghash(key, L.H);
+    while (true) {
+      uint32_t prev = atomicCAS(&L.hkey[slot], kNil, key);
+      if (prev == kNil || prev == key) break;
+      slot = (slot + 1) & (L.H - 1);
+    }
+    L.next[p] = atomicExch(&L.hhead[slot], (uint32_t)p);
+    L.acc[p] = 0;
+  }
+  __syncthreads();
+  // greedy acceptance, 256 sorted candidates per window
+  for (int64_t w0 = 0; w0 < n; w0 += kBlock) {
+    const int64_t p = w0 + tid;
+    const bool valid = p < n;
+    unsigned long long in_mask[4] = {0, 0, 0, 0};
+    bool rej = false;
+    double limSq = 0.0;
+    if (tid < 4) { s_dec[tid] = 0; s_acc[tid] = 0; }
+    if (valid) {
+      uint32_t j = L.idxA[p];
+      double x = coord(L.X[j], P.scale[0], P.offset[0]), y = coord(L.Y[j], P.scale[1], P.offset[1]),
+             z = coord(L.Z[j], P.scale[2], P.offset[2]);
+      int lo16[3], hi16[3];
+      if (!avg) {
+        double cd = __builtin_sqrt(L.sadd[p]);
+        double lim = cd - g.spacing;
+        limSq = lim * lim;
+      } else {
+        range16(g, x, y, z, lo16, hi16);
+      }
+      int lx = gcell(x - sq, g.min[0], cs), hx = gcell(x + sq, g.min[0], cs);
+      int ly = gcell(y - sq, g.min[1], cs), hy = gcell(y + sq, g.min[1], cs);
+      int lz = gcell(z - sq, g.min[2], cs), hz = gcell(z + sq, g.min[2], cs);
+      for (int cz = lz; cz <= hz; cz++)
+        for (int cy = ly; cy <= hy; cy++)
+          for (int cx = lx; cx <= hx; cx++) {
+            for (uint32_t q = grid_find(L, gkey(cx, cy, cz)); q != kNil; q = L.next[q]) {
+              if ((int64_t)q >= p) continue;
+              if ((int64_t)q < w0 && !L.acc[q]) continue;
+              uint32_t jq = L.idxA[q];
+              if (!avg) {
+                if (L.sadd[q] < limSq) continue;  // behind the early exit (sampler_poisson.h:160)
+              } else {
+                if (!in_range16(L.cell16[jq], lo16, hi16)) continue;
+              }
+              tests++;
+              if (!within_spacing(L, jq, j, P, sc, s2f, g.s2, rechecks, maxerr)) continue;
+              if ((int64_t)q < w0) rej = true;
+              else in_mask[(q - w0) >> 6] |= 1ull << ((q - w0) & 63);
+            }
+          }
+    }
+    if (!avg && tid == 0) s_minlim = 1e308;
+    __syncthreads();
+    if (!avg && valid) {
+      // conservative bound for the 10 001-check window: the candidates with
+      // distance in [limSq, add) bound the accepted points it would scan
+      unsigned long long bits = __double_as_longlong(limSq);
+      atomicMin((unsigned long long*)&s_minlim, bits);
+    }
+    // resolve dependencies inside the window
+    bool decided = !valid;
+    bool accepted = false;
+    while (true) {
+      if (!decided) {
+        bool ready = true;
+        for (int b = 0; b < 4; b++) if (in_mask[b] & ~s_dec[b]) ready = false;
+        if (ready) {
+          bool blocked = rej;
+          for (int b = 0; b < 4; b++) if (in_mask[b] & s_acc[b]) blocked = true;
+          accepted = !blocked;
+          decided = true;
+        }
+      }
+      __syncthreads();
+      if (decided && valid) {
+        atomicOr(&s_dec[tid >> 6], 1ull << (tid & 63));
+        if (accepted) atomicOr(&s_acc[tid >> 6], 1ull << (tid & 63));
+      }
+      if (tid == 0) s_done = 1;
+      __syncthreads();
+      if (!decided) s_done = 0;
+      __syncthreads();
+      if (s_done) break;
+    }
+    if (valid) L.acc[p] = accepted ? 1 : 0;
+    if (!avg && tid == 0 && w0 + kBlock > 10001) {
+      double minlim = s_minlim;
+      // lower_bound of minlim over sadd[0, p_last]
+      int64_t lo = 0, hi = (w0 + kBlock < n ? w0 + kBlock : n);
+      int64_t last = hi - 1;
+      while (lo < hi) {
+        int64_t mid = (lo + hi) / 2;
+        if (L.sadd[mid] < minlim) lo = mid + 1; else hi = mid;
+      }
+      if (last - lo > 10000) atomicOr(&work[k].flags, (uint32_t)NW_SEQ_ACCEPT);
+    }
+    __syncthreads();
+  }
+  atomicAdd(&P.counters[0], rechecks);
+  atomicAdd(&P.counters[1], tests);
+  if (maxerr > 0.f) atomicMax((unsigned int*)&P.counters[2], __float_as_uint(maxerr));
+}
+
+
+// Poisson-average colour sums: every rejected candidate C adds its own colour
+// (weight 1) to each accepted P with P inside C's 16^3 range and dd < s^2,
+// into P's "before" or "after" slots by sorted rank.  With accepted points
+// pairwise outside each other's range the reference's sequential pass
+// (sampler_poisson_average.h:369-415) then yields
+//   P = 2 * (own + before) + after.
+__global__ __launch_bounds__(kBlock) void k_avg_colors(const int32_t* __restrict__ nodes,
+                                                        const NodeInfo* __restrict__ infos,
+                                                        NodeWork* __restrict__ work, SampleParams P) {
+  const int k = blockIdx.x;
+  NodeWork w = work[k];
+  if (w.flags & NW_OVERFLOW) return;
+  const int64_t n = w.n;
+  Layout L = make_layout(w.scratch, n, P.method);
+  NodeGeom g = node_geom(infos[nodes[k]], P);
+  const double cs = 2.0 * g.spacing, sq = g.spacing * (1.0 + 1e-9);
+  bool violation = false;
+  for (int64_t p = threadIdx.x; p < n; p += kBlock) {
+    uint32_t j = L.idxA[p];
+    double x = coord(L.X[j], P.scale[0], P.offset[0]), y = coord(L.Y[j], P.scale[1], P.offset[1]),
+           z = coord(L.Z[j], P.scale[2], P.offset[2]);
+    int lo16[3], hi16[3];
+    range16(g, x, y, z, lo16, hi16);
+    int lx = gcell(x - sq, g.min[0], cs), hx = gcell(x + sq, g.min[0], cs);
+    int ly = gcell(y - sq, g.min[1], cs), hy = gcell(y + sq, g.min[1], cs);
+    int lz = gcell(z - sq, g.min[2], cs), hz = gcell(z + sq, g.min[2], cs);
+    const bool self_acc = L.acc[p];
+    const long long r = L.rgb[3 * j], gg = L.rgb[3 * j + 1], b = L.rgb[3 * j + 2];
+    for (int cz = lz; cz <= hz; cz++)
+      for (int cy = ly; cy <= hy; cy++)
+        for (int cx = lx; cx <= hx; cx++)
+          for (uint32_t q = grid_find(L, gkey(cx, cy, cz)); q != kNil; q = L.next[q]) {
+            if ((int64_t)q == p || !L.acc[q]) continue;
+            uint32_t jq = L.idxA[q];
+            if (!in_range16(L.cell16[jq], lo16, hi16)) continue;
+            if (!(dd64_pair(L, jq, j, P) < g.s2)) continue;
+            if (self_acc) { violation = true; continue; }
+            unsigned long long* d = (unsigned long long*)(L.sums + 8 * (uint64_t)q + (p < (int64_t)q ? 0 : 4));
+            atomicAdd(&d[0], (unsigned long long)r);
+            atomicAdd(&d[1], (unsigned long long)gg);
+            atomicAdd(&d[2], (unsigned long long)b);
+            atomicAdd(&d[3], 1ull);
+          }
+  }
+  if (violation) atomicOr(&work[k].flags, (uint32_t)NW_SEQ_AVERAGE);
+}
+
+// Exact sequential forms for the nodes flagged above (single lane).
+__global__ void k_sequential(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
+                             NodeWork* __restrict__ work, SampleParams P) {
+  const int k = blockIdx.x;
+  NodeWork w = work[k];
+  if ((w.flags & NW_OVERFLOW) || !(w.flags & (NW_SEQ_ACCEPT | NW_SEQ_AVERAGE))) return;
+  if (threadIdx.x != 0) return;
+  atomicAdd(&P.counters[4], 1ull);
+  const int64_t n = w.n;
+  Layout L = make_layout(w.scratch, n, P.method);
+  NodeGeom g = node_geom(infos[nodes[k]], P);
+  if (P.method == METHOD_POISSON) {
+    // sampler_poisson.h:133-180 over the accepted list kept in `next`
+    int64_t na = 0;
+    for (int64_t p = 0; p < n; p++) {
+      uint32_t j = L.idxA[p];
+      double cd = __builtin_sqrt(L.sadd[p]);
+      double limit = cd - g.spacing;
+      double limitSquared = limit * limit;
+      bool accept = true;
+      int64_t cnt = 0;
+      for (int64_t i = na - 1; i >= 0; i--) {
+        uint32_t q = L.next[i];
+        if (L.sadd[q] < limitSquared) break;
+        if (dd64_pair(L, L.idxA[q], j, P) < g.s2) { accept = false; break; }
+        cnt++;
+        if (cnt > 10000) break;
+      }
+      L.acc[p] = accept ? 1 : 0;
+      if (accept) L.next[na++] = (uint32_t)p;
+    }
+    return;
+  }
+  // sampler_poisson_average.h:184-230 and :357-416 with per-cell lists in
+  // push order (head in hkey, tail in hhead)
+  for (int c = 0; c < 4096; c++) { L.hkey[c] = kNil; L.hhead[c] = kNil; }
+  for (int64_t p = 0; p < n; p++) {
+    uint32_t j = L.idxA[p];
+    double x = coord(L.X[j], P.scale[0], P.offset[0]), y = coord(L.Y[j], P.scale[1], P.offset[1]),
+           z = coord(L.Z[j], P.scale[2], P.offset[2]);
+    int lo[3], hi[3];
+    range16(g, x, y, z, lo, hi);
+    bool accept = true;
+    for (int cx = lo[0]; cx <= hi[0] && accept; cx++)
+      for (int cy = lo[1]; cy <= hi[1] && accept; cy++)
+        for (int cz = lo[2]; cz <= hi[2] && accept; cz++)
+          for (uint32_t q = L.hkey[cx + cy * 16 + cz * 256]; q != kNil; q = L.next[q])
+            if (dd64_pair(L, L.idxA[q], j, P) < g.s2) { accept = false; break; }
+    L.acc[p] = accept ? 1 : 0;
+    L.next[p] = kNil;
+    if (accept) {
+      uint32_t c = L.cell16[j];
+      int cell = (c & 15) + ((c >> 4) & 15) * 16 + ((c >> 8) & 15) * 256;
+      if (L.hkey[cell] == kNil) L.hkey[cell] = (uint32_t)p; else L.next[L.hhead[cell]] = (uint32_t)p;
+      L.hhead[cell] = (uint32_t)p;
+    }
+  }
+  for (int64_t p = 0; p < n; p++) {
+    uint32_t j = L.idxA[p];
+    L.sums[8 * p + 0] = L.rgb[3 * j]; L.sums[8 * p + 1] = L.rgb[3 * j + 1];
+    L.sums[8 * p + 2] = L.rgb[3 * j + 2]; L.sums[8 * p + 3] = 1;
+  }
+  for (int64_t p = 0; p < n; p++) {
+    uint32_t j = L.idxA[p];
+    double x = coord(L.X[j], P.scale[0], P.offset[0]), y = coord(L.Y[j], P.scale[1], P.offset[1]),
+           z = coord(L.Z[j], P.scale[2], P.offset[2]);
+    int lo[3], hi[3];
+    range16(g, x, y, z, lo, hi);
+    for (int cx = lo[0]; cx <= hi[0]; cx++)
+      for (int cy = lo[1]; cy <= hi[1]; cy++)
+        for (int cz = lo[2]; cz <= hi[2]; cz++)
+          for (uint32_t q = L.hkey[cx + cy * 16 + cz * 256]; q != kNil; q = L.next[q])
+            if (dd64_pair(L, L.idxA[q], j, P) < g.s2)
+              for (int c = 0; c < 4; c++) L.sums[8 * q + c] += L.sums[8 * p + c];
+  }
+}
+
+// Writes the node's accepted records and the children's remainders and
+// updates the node states (sampler_poisson.h:247-294,
+// sampler_poisson_average.h:418-482).
+__global__ __launch_bounds__(kBlock) void k_partition(const int32_t* __restrict__ nodes,
+                                                       const NodeInfo* __restrict__ infos,
+                                                       NodeState* __restrict__ states, NodeWork* __restrict__ work,
+                                                       SampleParams P) {
+  __shared__ int64_t s_acc[8];
+  __shared__ uint8_t* s_out[9];
+  __shared__ const uint8_t* s_src[8];
+  __shared__ int s_fail;
+  __shared__ uint32_t s_wsum[kWaves];
+  const int k = blockIdx.x;
+  const int node = nodes[k];
+  NodeWork w = work[k];
+  if (w.flags & NW_OVERFLOW) return;
+  const NodeInfo info = infos[node];
+  const int64_t n = w.n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool avg = P.method == METHOD_POISSON_AVERAGE;
+  const bool seq_avg = (w.flags & NW_SEQ_AVERAGE) != 0;
+  Layout L = make_layout(w.scratch, n, P.method);
+  uint32_t* inv = L.keyB;
+  for (int64_t p = tid; p < n; p += kBlock) inv[L.idxA[p]] = (uint32_t)p;
+  if (tid < 8) {
+    s_acc[tid] = 0;
+    int ch = info.children[tid];
+    s_src[tid] = ch >= 0 ? states[ch].ptr : nullptr;
+  }
+  __syncthreads();
+  for (int c = 0; c < 8; c++) {
+    unsigned long long cnt = 0;
+    for (int64_t j = w.child_start[c] + tid; j < w.child_start[c + 1]; j += kBlock) cnt += L.acc[inv[j]];
+    if (cnt) atomicAdd((unsigned long long*)&s_acc[c], cnt);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int64_t total = 0;
+    for (int c = 0; c < 8; c++) total += s_acc[c];
+    uint64_t bytes = (uint64_t)total * P.bpp;
+    for (int c = 0; c < 8; c++) {
+      int64_t cnt = w.child_start[c + 1] - w.child_start[c];
+      int64_t rej = cnt - s_acc[c];
+      if (info.children[c] >= 0 && rej > 0) bytes += (uint64_t)(avg ? cnt : rej) * P.bpp;
+    }
+    uint64_t off = atomicAdd((unsigned long long*)P.arena_top, (unsigned long long)bytes);
+    s_fail = off + bytes > P.arena_cap;
+    if (s_fail) {
+      atomicOr(&work[k].flags, (uint32_t)NW_OVERFLOW);
+      atomicAdd(&P.counters[6], 1ull);
+    } else {
+      uint8_t* o = P.arena + off;
+      s_out[8] = o;
+      o += (uint64_t)total * P.bpp;
+      for (int c = 0; c < 8; c++) {
+        int64_t cnt = w.child_start[c + 1] - w.child_start[c];
+        int64_t rej = cnt - s_acc[c];
+        s_out[c] = nullptr;
+        if (info.children[c] >= 0 && rej > 0) {
+          s_out[c] = o;
+          o += (uint64_t)(avg ? cnt : rej) * P.bpp;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (s_fail) return;
+  // child accepted-prefix at each child start
+  int64_t acc_start[8];
+  {
+    int64_t r = 0;
+    for (int c = 0; c < 8; c++) { acc_start[c] = r; r += s_acc[c]; }
+  }
+  const int bpp = P.bpp;
+  int64_t run = 0;  // accepted before this chunk
+  for (int64_t b0 = 0; b0 < n; b0 += kBlock) {
+    int64_t j = b0 + tid;
+    bool valid = j < n;
+    uint32_t q = valid ? inv[j] : 0;
+    bool a = valid && L.acc[q];
+    uint64_t bal = __ballot(a);
+    uint32_t before_in_wave = (uint32_t)__popcll(bal & (lane == 0 ? 0ull : (~0ull >> (64 - lane))));
+    if (lane == 0) s_wsum[wave] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+    for (int v = 0; v < kWaves; v++) { if (v < wave) woff += s_wsum[v]; tot += s_wsum[v]; }
+    if (valid) {
+      int c = 0;
+      while (c < 7 && j >= w.child_start[c + 1]) c++;
+      int64_t local = j - w.child_start[c];
+      int64_t apos = run + woff + before_in_wave;
+      const uint8_t* src = s_src[c] + local * bpp;
+      uint16_t col[3];
+      if (avg && a) {
+        long long rs[3], ws;
+        if (seq_avg) {
+          for (int ch = 0; ch < 3; ch++) rs[ch] = L.sums[8 * q + ch];
+          ws = L.sums[8 * q + 3];
+        } else {
+          for (int ch = 0; ch < 3; ch++)
+            rs[ch] = 2 * ((long long)L.rgb[3 * j + ch] + L.sums[8 * q + ch]) + L.sums[8 * q + 4 + ch];
+          ws = 2 * (1 + L.sums[8 * q + 3]) + L.sums[8 * q + 7];
+        }
+        for (int ch = 0; ch < 3; ch++) col[ch] = (uint16_t)(rs[ch] / ws);
+      }
+      auto put = [&](uint8_t* dst) {
+        for (int b = 0; b < bpp; b++) dst[b] = src[b];
+        if (avg && a) {
+          for (int ch = 0; ch < 3; ch++) {
+            dst[P.rgb_offset + 2 * ch] = (uint8_t)col[ch];
+            dst[P.rgb_offset + 2 * ch + 1] = (uint8_t)(col[ch] >> 8);
+          }
+        }
+      };
+      if (a) put(s_out[8] + apos * bpp);
+      if (avg) {
+        if (s_out[c]) put(s_out[c] + local * bpp);
+      } else if (!a) {
+        int64_t rpos = local - (apos - acc_start[c]);
+        put(s_out[c] + rpos * bpp);
+      }
+    }
+    run += tot;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    int64_t total = 0;
+    for (int c = 0; c < 8; c++) total += s_acc[c];
+    NodeState& me = states[node];
+    me.ptr = s_out[8];
+    me.count = total;
+    me.accepted = total;
+    me.flags |= NS_SAMPLED;
+    for (int c = 0; c < 8; c++) {
+      int ch = info.children[c];
+      if (ch < 0) continue;
+      int64_t cnt = w.child_start[c + 1] - w.child_start[c];
+      int64_t rej = cnt - s_acc[c];
+      NodeState& cs = states[ch];
+      if (avg) {
+        if (rej == 0) {
+          cs.flags |= NS_DISCARDED;
+        } else {
+          cs.ptr = s_out[c];
+          cs.flags |= NS_COMPLETED;
+        }
+      } else {
+        cs.ptr = rej > 0 ? s_out[c] : nullptr;
+        cs.count = rej;
+        cs.flags |= NS_COMPLETED;
+      }
+    }
+    atomicAdd(&P.counters[5], (unsigned long long)n);
+  }
+}
+
+}  // namespace
+
+}  // namespace pcg
+
+namespace pcg {
+
+int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_info, NodeState* d_state,
+                 NodeWork* d_work, const SampleParams& p, hipStream_t s) {
+  if (num_nodes == 0) return 0;
+  dim3 grid((unsigned)num_nodes);
+  hipLaunchKernelGGL(k_prepare, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_state, d_work, p);
+  PCG_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_exact_sort, grid, dim3(64), 0, s, d_nodes, d_work, p, p.counters);
+  PCG_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_accept, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p);
+  PCG_CHECK(hipGetLastError());
+  if (p.method == METHOD_POISSON_AVERAGE) {
+    hipLaunchKernelGGL(k_avg_colors, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p);
+    PCG_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_sequential, grid, dim3(64), 0, s, d_nodes, d_info, d_work, p);
+  PCG_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_partition, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_state, d_work, p);
+  PCG_CHECK(hipGetLastError());
+  return 0;
+}
+
+uint64_t sample_scratch_bytes(int64_t n, int method) { return layout_bytes(n, method); }
+
+namespace {
+// byte ranges are arbitrary (records of any size): copy 4-byte words when
+// source and destination share their alignment, bytes otherwise
+__global__ __launch_bounds__(256) void k_gather_ranges(const CopyDesc* __restrict__ desc, uint8_t* __restrict__ dst) {
+  const CopyDesc d = desc[blockIdx.x];
+  const uint8_t* s = d.src;
+  uint8_t* o = dst + d.dst_offset;
+  uint64_t n = d.bytes;
+  if ((((uintptr_t)s ^ (uintptr_t)o) & 3) == 0) {
+    uint64_t head = (4 - ((uintptr_t)s & 3)) & 3;
+    if (head > n) head = n;
+    if (threadIdx.x < head) o[threadIdx.x] = s[threadIdx.x];
+    uint64_t words = (n - head) / 4;
+    const uint32_t* s4 = (const uint32_t*)(s + head);
+    uint32_t* o4 = (uint32_t*)(o + head);
+    for (uint64_t i = threadIdx.x; i < words; i += blockDim.x) o4[i] = s4[i];
+    for (uint64_t i = head + words * 4 + threadIdx.x; i < n; i += blockDim.x) o[i] = s[i];
+  } else {
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) o[i] = s[i];
+  }
+}
+}  // namespace
+
+int gather_ranges(const CopyDesc* d_desc, int64_t num, uint8_t* d_dst, hipStream_t s) {
+  if (num == 0) return 0;
+  hipLaunchKernelGGL(k_gather_ranges, dim3((unsigned)num), dim3(256), 0, s, d_desc, d_dst);
+  PCG_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace pcg
