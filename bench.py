@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Throughput of the gfx950 counting-sort chunker + octree indexer + Poisson sampler.
+
+One step = doChunking + doIndexing (Poisson) of a seeded synthetic LAS point
+set that is already resident in HBM (5 km x 5 km tile in UTM-like
+coordinates, LAS 1.4 point format 2, scale 0.001) up to the octree.bin image
+in HBM plus hierarchy.bin / metadata.json on the host.  No files are written
+inside the timed region.
+
+Prints one JSON line (see README for the fields).  `cpu_baseline` times the
+single-threaded CPU oracle (a restatement of the reference algorithm) on a
+bounded prefix of the same point stream, on this machine's host.
+"""
+import argparse
+import ctypes
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+STATS = 24
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--points", type=int, default=200_000_000)
+    ap.add_argument("--method", default="poisson", choices=["poisson", "poisson_average"])
+    ap.add_argument("--cpu-points", type=int, default=20_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(points, seed, method):
+    """Times the CPU oracle on the first `points` records of the same stream."""
+    import potreeconverter_amd as pc
+
+    ocl = os.path.join(HERE, "oracle", "_build", "liboracle.so")
+    if not os.path.exists(ocl):
+        return None
+    O = ctypes.CDLL(ocl)
+    O.orc_convert.argtypes = [ctypes.c_char_p] * 3 + [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+    g = pc.LasGen(1, 200_000_000, seed=seed)
+    g.n = points
+    # generate the prefix of the 200M-point stream and write it as its own LAS file
+    with tempfile.TemporaryDirectory() as td:
+        las = os.path.join(td, "sample.las")
+        ctypes.c_int64.from_buffer(g.buf, 8).value = points  # LasGenParams.n (tiling stays that of 200M)
+        # keep the 200M tiling: tiles_per_side / num_tiles were fixed by the preset call
+        pc.lib().pcg_lasgen_write(g.buf, las.encode())
+        t = (ctypes.c_double * 8)()
+        out = os.path.join(td, "out")
+        t0 = time.time()
+        rc = O.orc_convert(las.encode(), out.encode(), method.encode(), 0, 0, t)
+        wall = time.time() - t0
+        if rc != 0:
+            return None
+        secs = t[2] + t[3]
+    return {
+        "value": points / secs / 1e6,
+        "unit": "Mpts/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"first {points} points of the same generated stream; chunking {t[2]:.2f}s + indexing {t[3]:.2f}s "
+                  f"(wall {wall:.1f}s incl. file read/write)",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md, "
+                                   "--gpus above 1 is not supported"}))
+        sys.exit(2)
+    import potreeconverter_amd as pc
+
+    L = pc.lib()
+    n = args.points
+    g = pc.LasGen(1, n, seed=args.seed)
+    rl = g.record_length
+    d_las = ctypes.c_void_p()
+    pc._check(L.pcg_device_alloc(ctypes.byref(d_las), n * rl + 64))
+    qmm = (ctypes.c_int32 * 6)(2**31 - 1, 2**31 - 1, 2**31 - 1, -2**31, -2**31, -2**31)
+    d_q = ctypes.c_void_p()
+    pc._check(L.pcg_device_alloc(ctypes.byref(d_q), 64))
+    pc._check(L.pcg_copy(d_q, qmm, 24, 0))
+    pc._check(L.pcg_lasgen_device(g.buf, 0, n, d_las, d_q))
+    pc._check(L.pcg_copy(qmm, d_q, 24, 1))
+    header = ctypes.create_string_buffer(375)
+    pc._check(L.pcg_lasgen_header(g.buf, qmm, header))
+    stats = (ctypes.c_double * STATS)()
+
+    def step():
+        pc._check(L.pcg_convert_device(d_las, n, header, args.method.encode(), None, stats))
+
+    for _ in range(args.warmup):
+        step()
+    pc._check(L.pcg_synchronize())
+    per = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        step()
+        per.append(list(stats))
+    pc._check(L.pcg_synchronize())
+    elapsed = time.perf_counter() - t0
+    ms = elapsed / args.steps * 1e3
+    value = n / (elapsed / args.steps) / 1e6
+
+    # roofline of the chunking pass (count + distribute): 79 B/pt algorithmic
+    # bytes for 26-byte input records and 27-byte output records
+    bpp_in, bpp_out = rl, rl + 1
+    chunk_bytes = n * (2 * bpp_in + bpp_out)
+    chunk_s = sum(p[2] for p in per) / len(per)
+    kern = {
+        "count_ms": sum(p[19] for p in per) / len(per),
+        "distribute_ms": sum(p[20] for p in per) / len(per),
+    }
+    kern_s = (kern["count_ms"] + kern["distribute_ms"]) / 1e3
+    achieved = chunk_bytes / kern_s / 1e9 if kern_s > 0 else 0.0
+    last = per[-1]
+    cpu = None if args.no_cpu else cpu_baseline(min(args.cpu_points, n), args.seed, args.method)
+    line = {
+        "metric": "Mpts/s count-sort+octree+Poisson at 200M pts, batched vs plain; % HBM peak",
+        "value": round(value, 3),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{n} pts synthetic LAS PDRF2 5x5 km (offset 500000,5000000,0; scale 0.001), "
+                        f"terrain fBm + 20 buildings, seed {args.seed}",
+            "points": n,
+            "method": args.method,
+            "path": "batched (all nodes of a level per launch)",
+            "grid": 128 if n < 100_000_000 else (256 if n < 500_000_000 else 512),
+        },
+        "phases_s": {
+            "chunking": round(last[2], 4), "count": round(last[0], 4), "lut+distribute": round(last[1], 4),
+            "indexing": round(last[3], 4), "octree_build": round(last[4], 4), "sampling": round(last[5], 4),
+            "octree_assembly": round(last[6], 4),
+        },
+        "counters": {
+            "chunks": int(last[16]), "nodes": int(last[8]), "sampled_nodes": int(last[9]),
+            "candidates": int(last[10]), "fp32_pair_tests": int(last[12]), "fp64_rechecks": int(last[11]),
+            "max_fp32_rel_err": last[13], "libstdcxx_order_nodes": int(last[14]), "sequential_nodes": int(last[15]),
+            "octree_rounds": int(last[17]), "depth": int(last[18]), "octree_bytes": int(last[7]),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "chunking pass (k_count + k_tile_hist + k_scatter)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes": chunk_bytes,
+            "kernel_ms": round(kern_s * 1e3, 4),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    L.pcg_device_free(d_las)
+    L.pcg_device_free(d_q)
+
+
+if __name__ == "__main__":
+    main()

@@ -148,6 +148,8 @@ int pcg_convert_device(const void* d_las, int64_t n, const uint8_t* las_header, 
     stats[16] = (double)dc.chunks.size();
     stats[17] = (double)r.buildRounds;
     stats[18] = (double)r.octreeDepth;
+    stats[19] = (double)dc.ms_count_kernel;
+    stats[20] = (double)dc.ms_distribute_kernels;
   }
   dc.release();
   r.release();

@@ -153,9 +153,14 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
   PCG_CHECK(hipMemsetAsync(stats.as<uint64_t>() + kMaxStats, 0, sizeof(uint64_t) * kMaxStats, s));
   PCG_CHECK(hipMemsetAsync(hist.p, 0, sizeof(unsigned long long) * 256, s));
   PCG_CHECK(hipMemsetAsync(err.p, 0, sizeof(int32_t) * 4, s));
+  static hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  if (!ev[0])
+    for (int i = 0; i < 4; i++) PCG_CHECK(hipEventCreate(&ev[i]));
+  PCG_CHECK(hipEventRecord(ev[0], s));
   if (chunker_count(d_las, N, grid.as<int32_t>(), stats.as<uint64_t>(), stats.as<uint64_t>() + kMaxStats,
                     hist.as<unsigned long long>(), err.as<int32_t>(), dev.num_cus, s))
     return 1;
+  PCG_CHECK(hipEventRecord(ev[1], s));
   int32_t herr[4];
   uint64_t hstats[2 * kMaxStats];
   unsigned long long hhist[256];
@@ -264,15 +269,19 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
   int covered = 12;
   for (int o = 0; o < P.num_ops; o++) covered += P.ops[o].len;
   if (covered < out.bpp) PCG_CHECK(hipMemsetAsync(out.d_records, 0, (size_t)N * out.bpp, s));
+  PCG_CHECK(hipEventRecord(ev[2], s));
   if (chunker_distribute(d_las, N, lut.as<int32_t>(), C, tile, M.as<uint32_t>(), ws.p, out.d_records,
                          err.as<int32_t>(), s))
     return 1;
+  PCG_CHECK(hipEventRecord(ev[3], s));
   PCG_CHECK(hipMemcpyAsync(herr, err.p, sizeof(herr), hipMemcpyDeviceToHost, s));
   PCG_CHECK(hipStreamSynchronize(s));
   if (herr[0] & 2) {
     set_error("point to node lookup failed, no node found.");
     return 123;
   }
+  PCG_CHECK(hipEventElapsedTime(&out.ms_count_kernel, ev[0], ev[1]));
+  PCG_CHECK(hipEventElapsedTime(&out.ms_distribute_kernels, ev[2], ev[3]));
   double t2 = now_s();
   out.t_distribute = t2 - t1;
   out.t_total = t2 - t0;

@@ -31,6 +31,7 @@ struct DeviceChunks {
   int32_t gridSize = 0;
   int64_t maxPointsPerChunk = 0;
   double t_count = 0, t_distribute = 0, t_total = 0;
+  float ms_count_kernel = 0, ms_distribute_kernels = 0;  // HIP events on the library stream
   void release();
 };
 

@@ -1,0 +1,64 @@
+"""Writes the golden fixtures of the indexer path from the reference itself.
+
+For each case: generate the seeded LAS file, chunk it with the CPU oracle
+(the reference chunker cannot be built here: laszip needs a CMake-generated
+header), index those chunks with the REFERENCE indexer built from the
+reference sources (oracle/build_ref.sh -> oracle/_ref/ref_indexer), and store
+its metadata.json plus a per-node digest of hierarchy.bin / octree.bin
+(type, childMask, numPoints, byteSize, sha256 of the node's records).
+The reference writes nodes in thread-completion order, so the digest is
+keyed by node name rather than by file offset.
+
+Run from the repository root:  python tests/golden/make_golden.py
+"""
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from helpers import REF_INDEXER, digest_output, make_las, oracle_convert  # noqa: E402
+
+CASES = [
+    # name, preset, points, seed, extra bytes, method
+    ("c0_2m_poisson", 0, 2_000_000, 0, False, "poisson"),
+    ("c0_2m_poisson_average", 0, 2_000_000, 0, False, "poisson_average"),
+    ("c0_400k_extra_poisson_average", 0, 400_000, 5, True, "poisson_average"),
+    ("c1_1m_utm_poisson", 1, 1_000_000, 1, False, "poisson"),
+    ("c0_150k_poisson", 0, 150_000, 2, False, "poisson"),
+]
+
+
+def main():
+    assert os.path.exists(REF_INDEXER), "build the reference indexer first: bash oracle/build_ref.sh"
+    td = tempfile.mkdtemp()
+    try:
+        for name, preset, n, seed, eb, method in CASES:
+            las = os.path.join(td, name + ".las")
+            make_las(las, preset, n, seed, eb)
+            work = os.path.join(td, name)
+            oracle_convert(las, work, method, keep_chunks=True, chunk_only=True)
+            subprocess.run([REF_INDEXER, work, method, name, str(n)], check=True, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL)
+            with open(os.path.join(work, "metadata.json")) as f:
+                meta = f.read()
+            fixture = {
+                "case": {"preset": preset, "points": n, "seed": seed, "extra_bytes": eb, "method": method},
+                "producer": "reference indexer (Converter/src/indexer.cpp) on oracle-chunked input",
+                "metadata_json": meta,
+                "nodes": digest_output(work),
+            }
+            with open(os.path.join(HERE, name + ".json"), "w") as f:
+                json.dump(fixture, f, indent=0, sort_keys=True)
+            print(name, len(fixture["nodes"]), "nodes")
+    finally:
+        shutil.rmtree(td, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    main()

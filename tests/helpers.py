@@ -1,0 +1,56 @@
+"""Shared helpers for the tests: oracle loading, LAS generation, digests."""
+import ctypes
+import hashlib
+import os
+
+from potree_format import canonical_nodes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+REF_INDEXER = os.path.join(ROOT, "oracle", "_ref", "ref_indexer")
+_oracle = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        O = ctypes.CDLL(ORACLE_LIB)
+        O.orc_convert.argtypes = [ctypes.c_char_p] * 3 + [ctypes.c_int32, ctypes.c_int32,
+                                                          ctypes.POINTER(ctypes.c_double)]
+        O.orc_index.argtypes = [ctypes.c_char_p] * 3 + [ctypes.POINTER(ctypes.c_double)]
+        O.orc_last_error.restype = ctypes.c_char_p
+        O.orc_count_cells.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32] + \
+            [ctypes.POINTER(ctypes.c_double)] * 6 + [ctypes.c_int32, ctypes.c_void_p]
+        O.orc_create_lut.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p,
+                                     ctypes.c_int64, ctypes.c_void_p]
+        O.orc_create_lut.restype = ctypes.c_int64
+        _oracle = O
+    return _oracle
+
+
+def oracle_convert(las, outdir, method, keep_chunks=False, chunk_only=False):
+    t = (ctypes.c_double * 8)()
+    rc = oracle().orc_convert(str(las).encode(), str(outdir).encode(), method.encode(), int(keep_chunks),
+                              int(chunk_only), t)
+    assert rc == 0, oracle().orc_last_error()
+    return list(t)
+
+
+def make_las(path, preset, n, seed, extra_bytes=False):
+    import potreeconverter_amd as pc
+    g = pc.LasGen(preset, n, seed=seed, extra_bytes=extra_bytes)
+    g.write(str(path))
+    return g
+
+
+def digest_output(outdir):
+    """name -> [type, childMask, numPoints, byteSize, sha256(node bytes)]"""
+    out = {}
+    for name, (typ, mask, npts, size, data) in canonical_nodes(outdir).items():
+        out[name] = [typ, mask, npts, size, hashlib.sha256(data).hexdigest()]
+    return out
+
+
+def read(path):
+    with open(path, "rb") as f:
+        return f.read()

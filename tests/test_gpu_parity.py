@@ -1,0 +1,166 @@
+"""gfx950 path against the CPU oracle, byte for byte, through the C ABI.
+
+The oracle is pinned to the reference indexer by tests/test_oracle_golden.py;
+here the GPU conversion must produce the same octree.bin, hierarchy.bin and
+metadata.json bytes (both sides write the canonical node order), and the GPU
+chunker the same chunk files.
+"""
+import ctypes
+import filecmp
+import json
+import os
+
+import numpy as np
+import pytest
+
+import potreeconverter_amd as pc
+from helpers import digest_output, make_las, oracle, oracle_convert, read
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # preset, points, seed, extra bytes
+    (0, 150_000, 2, False),
+    (0, 2_000_000, 0, False),
+    (0, 400_000, 5, True),
+    (1, 1_000_000, 1, False),
+]
+
+
+def same_files(a, b, names):
+    return {n: filecmp.cmp(os.path.join(a, n), os.path.join(b, n), shallow=False) for n in names}
+
+
+@pytest.mark.parametrize("preset,n,seed,eb", CASES, ids=[f"p{c[0]}_{c[1]}_s{c[2]}{'_eb' if c[3] else ''}" for c in CASES])
+@pytest.mark.parametrize("method", ["poisson", "poisson_average"])
+def test_convert_matches_oracle(preset, n, seed, eb, method, tmp_path):
+    print(f"case preset={preset} n={n} seed={seed} extra={eb} method={method}", flush=True)
+    las = tmp_path / "in.las"
+    make_las(las, preset, n, seed, eb)
+    cpu, gpu = tmp_path / "cpu", tmp_path / "gpu"
+    oracle_convert(las, cpu, method)
+    pc.convert(las, gpu, method)
+    res = same_files(cpu, gpu, ["octree.bin", "hierarchy.bin", "metadata.json"])
+    assert all(res.values()), res
+
+
+def test_chunking_matches_oracle(tmp_path):
+    las = tmp_path / "in.las"
+    make_las(las, 1, 800_000, 4)
+    cpu, gpu = tmp_path / "cpu", tmp_path / "gpu"
+    oracle_convert(las, cpu, "poisson", chunk_only=True)
+    pc.do_chunking(las, gpu)
+    a, b = sorted(os.listdir(cpu / "chunks")), sorted(os.listdir(gpu / "chunks"))
+    assert a == b
+    assert all(same_files(cpu / "chunks", gpu / "chunks", a).values())
+
+
+def test_indexing_of_reference_format_chunks(tmp_path):
+    """doIndexing on chunks/ written to the reference's file contract."""
+    las = tmp_path / "in.las"
+    make_las(las, 0, 900_000, 8)
+    cpu = tmp_path / "cpu"
+    oracle_convert(las, cpu, "poisson", keep_chunks=True)
+    gpu = tmp_path / "gpu"
+    os.makedirs(gpu)
+    os.system(f"cp -r {cpu}/chunks {gpu}/")
+    pc.do_indexing(gpu, "poisson", "in", 900_000)
+    assert all(same_files(cpu, gpu, ["octree.bin", "hierarchy.bin", "metadata.json"]).values())
+
+
+def test_count_cells_entry_point():
+    L = pc.lib()
+    g = pc.LasGen(0, 300_000, seed=6)
+    recs = g.records()
+    n, rl = 300_000, 26
+    d = ctypes.c_void_p()
+    assert L.pcg_device_alloc(ctypes.byref(d), len(recs) + 64) == 0
+    assert L.pcg_copy(d, recs, len(recs), 0) == 0
+    a = np.frombuffer(recs, dtype=np.uint8).reshape(n, rl)[:, :12].copy().view(np.int32).reshape(n, 3)
+    scale, off = [0.001] * 3, [0.0] * 3
+    mn = [(a[:, k].min() * 0.001) for k in range(3)]
+    mx = [(a[:, k].max() * 0.001) for k in range(3)]
+    dp = lambda v: (ctypes.c_double * 3)(*v)
+    G = 128
+    dg = ctypes.c_void_p()
+    assert L.pcg_device_alloc(ctypes.byref(dg), 4 * G ** 3) == 0
+    rc = L.pcg_count_cells(d, n, rl, dp(scale), dp(off), dp(scale), dp(mn), dp(mn), dp(mx), G, dg, None)
+    assert rc == 0, L.pcg_last_error()
+    got = np.zeros(G ** 3, dtype=np.int32)
+    assert L.pcg_copy(got.ctypes.data_as(ctypes.c_void_p), dg, 4 * G ** 3, 1) == 0
+    want = np.zeros(G ** 3, dtype=np.int32)
+    buf = ctypes.create_string_buffer(recs, len(recs))
+    rc = oracle().orc_count_cells(buf, n, rl, dp(scale), dp(off), dp(scale), dp(mn), dp(mn), dp(mx), G,
+                                  want.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 0
+    assert np.array_equal(got, want)
+    L.pcg_device_free(d)
+    L.pcg_device_free(dg)
+
+
+def test_out_of_box_point_is_rejected_like_the_reference(tmp_path):
+    """A header whose bounds exclude a point fails with the reference's 123."""
+    las = tmp_path / "bad.las"
+    make_las(las, 0, 50_000, 1)
+    raw = bytearray(read(las))
+    # shrink max x below the data
+    import struct
+    mx = struct.unpack_from("<d", raw, 179)[0]
+    mnx = struct.unpack_from("<d", raw, 187)[0]
+    struct.pack_into("<d", raw, 179, mnx + (mx - mnx) * 0.5)
+    # keep the cube from the other axes small too
+    for o_max, o_min in ((195, 203), (211, 219)):
+        lo = struct.unpack_from("<d", raw, o_min)[0]
+        struct.pack_into("<d", raw, o_max, lo + (mx - mnx) * 0.25)
+    las.write_bytes(bytes(raw))
+    t = (ctypes.c_double * 8)()
+    rc = pc.lib().pcg_do_chunking(str(las).encode(), str(tmp_path / "o").encode(), t)
+    assert rc == 123
+
+
+def test_gpu_generator_equals_cpu_generator():
+    L = pc.lib()
+    g = pc.LasGen(1, 1_000_000, seed=7)
+    cpu = g.records()
+    d = ctypes.c_void_p()
+    assert L.pcg_device_alloc(ctypes.byref(d), len(cpu) + 64) == 0
+    q = (ctypes.c_int32 * 6)(2**31 - 1, 2**31 - 1, 2**31 - 1, -2**31, -2**31, -2**31)
+    dq = ctypes.c_void_p()
+    assert L.pcg_device_alloc(ctypes.byref(dq), 64) == 0
+    assert L.pcg_copy(dq, q, 24, 0) == 0
+    assert L.pcg_lasgen_device(g.buf, 0, 1_000_000, d, dq) == 0
+    out = ctypes.create_string_buffer(len(cpu))
+    assert L.pcg_copy(out, d, len(cpu), 1) == 0
+    assert out.raw == cpu
+    L.pcg_device_free(d)
+    L.pcg_device_free(dq)
+
+
+def test_large_run_is_deterministic_and_conserves_points(tmp_path):
+    """20M points: two device runs give identical bytes; Poisson keeps every point once."""
+    L = pc.lib()
+    n = 20_000_000
+    g = pc.LasGen(1, n, seed=0)
+    d = ctypes.c_void_p()
+    assert L.pcg_device_alloc(ctypes.byref(d), n * 26 + 64) == 0
+    q = (ctypes.c_int32 * 6)(2**31 - 1, 2**31 - 1, 2**31 - 1, -2**31, -2**31, -2**31)
+    dq = ctypes.c_void_p()
+    assert L.pcg_device_alloc(ctypes.byref(dq), 64) == 0
+    assert L.pcg_copy(dq, q, 24, 0) == 0
+    assert L.pcg_lasgen_device(g.buf, 0, n, d, dq) == 0
+    assert L.pcg_copy(q, dq, 24, 1) == 0
+    hdr = ctypes.create_string_buffer(375)
+    assert L.pcg_lasgen_header(g.buf, q, hdr) == 0
+    stats = (ctypes.c_double * 24)()
+    outs = []
+    for k in range(2):
+        o = tmp_path / f"run{k}"
+        rc = L.pcg_convert_device(d, n, hdr, b"poisson", str(o).encode(), stats)
+        assert rc == 0, L.pcg_last_error()
+        outs.append(o)
+    assert all(same_files(outs[0], outs[1], ["octree.bin", "hierarchy.bin", "metadata.json"]).values())
+    meta = json.load(open(outs[0] / "metadata.json"))
+    assert meta["points"] == n
+    assert os.path.getsize(outs[0] / "octree.bin") == 27 * n
+    L.pcg_device_free(d)
+    L.pcg_device_free(dq)

@@ -1,0 +1,59 @@
+"""The CPU oracle against the reference indexer's own output.
+
+Fixtures in tests/golden/ were written by the reference indexer built from
+the reference sources (see tests/golden/make_golden.py).  The oracle must
+reproduce metadata.json byte for byte and every node's type, child mask,
+point count, byte size and record bytes.
+"""
+import glob
+import json
+import os
+import shutil
+import subprocess
+
+import pytest
+
+from helpers import REF_INDEXER, digest_output, make_las, oracle_convert
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.json")))
+
+
+@pytest.mark.parametrize("fixture", GOLDEN, ids=[os.path.basename(p)[:-5] for p in GOLDEN])
+def test_oracle_matches_reference_fixture(fixture, tmp_path):
+    with open(fixture) as f:
+        fx = json.load(f)
+    case = fx["case"]
+    name = os.path.basename(fixture)[:-5]
+    las = tmp_path / (name + ".las")
+    make_las(las, case["preset"], case["points"], case["seed"], case["extra_bytes"])
+    out = tmp_path / "out"
+    oracle_convert(las, out, case["method"])
+    with open(out / "metadata.json") as f:
+        assert f.read() == fx["metadata_json"]
+    got = digest_output(str(out))
+    assert set(got) == set(fx["nodes"])
+    bad = [n for n in got if got[n] != fx["nodes"][n]]
+    assert not bad, f"{len(bad)} nodes differ, e.g. {bad[:5]}"
+
+
+def test_fixtures_discriminate_methods():
+    """The digests are sensitive enough to tell the two samplers apart."""
+    a = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "c0_2m_poisson.json")))
+    b = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "c0_2m_poisson_average.json")))
+    assert a["nodes"].keys() == b["nodes"].keys()
+    assert sum(a["nodes"][k] != b["nodes"][k] for k in a["nodes"]) > 100
+
+
+@pytest.mark.skipif(not os.path.exists(REF_INDEXER), reason="reference indexer not built (oracle/build_ref.sh)")
+@pytest.mark.parametrize("method", ["poisson", "poisson_average"])
+def test_oracle_matches_live_reference(method, tmp_path):
+    las = tmp_path / "live.las"
+    make_las(las, 0, 700_000, 11)
+    ref = tmp_path / "ref"
+    oracle_convert(las, ref, method, keep_chunks=True, chunk_only=True)
+    subprocess.run([REF_INDEXER, str(ref), method, "live", "700000"], check=True, stdout=subprocess.DEVNULL,
+                   stderr=subprocess.DEVNULL)
+    orc = tmp_path / "orc"
+    oracle_convert(las, orc, method)
+    assert open(ref / "metadata.json").read() == open(orc / "metadata.json").read()
+    assert digest_output(str(ref)) == digest_output(str(orc))
