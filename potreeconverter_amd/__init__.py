@@ -34,6 +34,10 @@ def lib():
             c.c_int32, c.c_void_p, c.c_void_p]
         L.pcg_convert_device.argtypes = [c.c_void_p, c.c_int64, c.c_void_p, c.c_char_p, c.c_char_p,
                                          c.POINTER(c.c_double)]
+        L.pcg_device_alloc.argtypes = [c.POINTER(c.c_void_p), c.c_uint64]
+        L.pcg_device_free.argtypes = [c.c_void_p]
+        L.pcg_copy.argtypes = [c.c_void_p, c.c_void_p, c.c_uint64, c.c_int32]
+        L.pcg_synchronize.argtypes = []
         L.pcg_lasgen_preset.argtypes = [c.c_int32, c.c_int64, c.c_uint64, c.c_void_p]
         L.pcg_lasgen_record_length.argtypes = [c.c_void_p]
         L.pcg_lasgen_record_length.restype = c.c_int32
@@ -42,6 +46,10 @@ def lib():
         L.pcg_lasgen_write.argtypes = [c.c_void_p, c.c_char_p]
         L.pcg_lasgen_device.argtypes = [c.c_void_p, c.c_int64, c.c_int64, c.c_void_p, c.c_void_p]
         L.pcg_lasgen_header.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p]
+        for name in ("pcg_convert", "pcg_do_chunking", "pcg_do_indexing", "pcg_count_cells", "pcg_convert_device",
+                     "pcg_device_alloc", "pcg_device_free", "pcg_copy", "pcg_synchronize", "pcg_lasgen_preset",
+                     "pcg_lasgen_write", "pcg_lasgen_device", "pcg_lasgen_header", "pcg_init"):
+            getattr(L, name).restype = c.c_int
         _lib = L
     return _lib
 

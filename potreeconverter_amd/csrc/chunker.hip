@@ -141,8 +141,13 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ la
   const int64_t steps_total = (n + 63) / 64;
   const int64_t wave_global = (int64_t)blockIdx.x * kWavesPerBlock + wave;
   const int64_t waves_total = (int64_t)gridDim.x * kWavesPerBlock;
+  // contiguous step ranges: consecutive points of a LAS file are spatially
+  // coherent, so the block's hash absorbs most repeats of a cell
+  const int64_t per_wave = (steps_total + waves_total - 1) / waves_total;
+  const int64_t step0 = wave_global * per_wave;
+  const int64_t step1 = (step0 + per_wave) < steps_total ? (step0 + per_wave) : steps_total;
   bool bad = false;
-  for (int64_t step = wave_global; step < steps_total; step += waves_total) {
+  for (int64_t step = step0; step < step1; step++) {
     int64_t first = step * 64;
     int cnt = (int)((n - first) < 64 ? (n - first) : 64);
     uint32_t shift = wave_load_records(las, total, first, cnt, bpp, rec_area);

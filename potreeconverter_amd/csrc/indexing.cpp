@@ -280,7 +280,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   uint64_t arenaCap = (uint64_t)N * bpp * (avg ? 4 : 3) + (1u << 20);
   // per node: linear arrays plus a hash table of at most 4n slots (the
   // power of two >= 2n) or the Poisson-average minimum of 4096 slots
-  uint64_t scratchCap = (uint64_t)N * (avg ? 170 : 100) + (uint64_t)maxLevelNodes * (sample_scratch_bytes(1, methodId) + 40960);
+  uint64_t scratchCap = (uint64_t)N * (avg ? 240 : 170) + (uint64_t)maxLevelNodes * (sample_scratch_bytes(1, methodId) + 40960);
   if (dinfo.alloc(sizeof(NodeInfo) * numNodes) || dstate.alloc(sizeof(NodeState) * numNodes) ||
       dwork.alloc(sizeof(NodeWork) * maxLevelNodes) || dlist.alloc(sizeof(int32_t) * maxLevelNodes) ||
       arena.alloc(arenaCap) || scratch.alloc(scratchCap) || tops.alloc(64) || counters.alloc(64))

@@ -20,6 +20,7 @@ enum : uint32_t {
   NW_SEQ_ACCEPT = 2,   // the 10 001-check window of the Poisson scan may bind
   NW_SEQ_AVERAGE = 4,  // two accepted points inside each other's colour range
   NW_OVERFLOW = 8,     // scratch or arena exhausted
+  NW_MANY_NBR = 16,    // a candidate has more earlier neighbours than the list holds
 };
 
 struct NodeInfo {

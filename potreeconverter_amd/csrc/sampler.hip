@@ -30,6 +30,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kNil = 0xFFFFFFFFu;
+constexpr int kMaxNbr = 16;  // earlier neighbours kept per candidate
 
 __device__ inline int32_t ld_i32(const uint8_t* p) {
   return (int32_t)((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24));
@@ -57,6 +58,8 @@ struct Layout {
   uint32_t *hkey, *hhead, *next;
   uint32_t H;
   long long* sums;   // sorted order, 8 per candidate (average)
+  uint32_t* nbr;     // sorted order, kMaxNbr earlier neighbours per candidate
+  uint8_t* nbrCount;
 };
 
 __host__ __device__ inline uint64_t layout_bytes(int64_t n, int method) {
@@ -64,6 +67,7 @@ __host__ __device__ inline uint64_t layout_bytes(int64_t n, int method) {
   uint64_t b = 3 * align16(4 * n) + align16(8 * n) + 4 * align16(4 * n) + align16(8 * n) + align16(n) +
                align16(4 * n) + 2 * align16(4 * H) + align16(4 * n);
   if (method == METHOD_POISSON_AVERAGE) b += align16(64 * n) + align16(6 * n);
+  b += align16(4 * kMaxNbr * n) + align16(n);
   return b + 64;
 }
 
@@ -88,6 +92,8 @@ __device__ inline Layout make_layout(uint8_t* base, int64_t n, int method) {
   L.next = (uint32_t*)take(4 * n);
   L.sums = method == METHOD_POISSON_AVERAGE ? (long long*)take(64 * n) : nullptr;
   L.rgb = method == METHOD_POISSON_AVERAGE ? (uint16_t*)take(6 * n) : nullptr;
+  L.nbr = (uint32_t*)take(4 * kMaxNbr * n);
+  L.nbrCount = (uint8_t*)take(n);
   L.sidx = L.idxA;
   return L;
 }
@@ -344,19 +350,309 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
 }
 
 // ------------------------------------------------------------- exact sort
-__global__ void k_exact_sort(const int32_t* __restrict__ nodes, NodeWork* __restrict__ work, SampleParams P,
-                             unsigned long long* counters) {
+// The reference orders candidates with libstdc++ std::sort, which places
+// equal distances in an order fixed by its moves.  Introsort's final
+// insertion sort is stable, so that order is the order of the equal keys
+// after the partition phase.  The partition phase is replayed here: every
+// median-of-three Hoare partition is evaluated by one wavefront from prefix
+// counts (the k-th element >= pivot from the left swaps with the k-th
+// element <= pivot from the right while the former lies left of the
+// latter), segments are shared between the four waves through an LDS
+// stack, and past the depth limit a segment is heap-sorted by one lane.
+
+struct PairArr {
+  double* K;
+  uint32_t* H;
+};
+
+__device__ inline void pair_swap(PairArr a, int64_t i, int64_t j) {
+  double k = a.K[i]; a.K[i] = a.K[j]; a.K[j] = k;
+  uint32_t h = a.H[i]; a.H[i] = a.H[j]; a.H[j] = h;
+}
+
+__device__ void pair_push_heap(PairArr a, int64_t f, int64_t hole, int64_t top, double vk, uint32_t vh) {
+  int64_t parent = (hole - 1) / 2;
+  while (hole > top && a.K[f + parent] < vk) {
+    a.K[f + hole] = a.K[f + parent];
+    a.H[f + hole] = a.H[f + parent];
+    hole = parent;
+    parent = (hole - 1) / 2;
+  }
+  a.K[f + hole] = vk;
+  a.H[f + hole] = vh;
+}
+
+__device__ void pair_adjust_heap(PairArr a, int64_t f, int64_t hole, int64_t len, double vk, uint32_t vh) {
+  const int64_t top = hole;
+  int64_t second = hole;
+  while (second < (len - 1) / 2) {
+    second = 2 * (second + 1);
+    if (a.K[f + second] < a.K[f + second - 1]) second--;
+    a.K[f + hole] = a.K[f + second];
+    a.H[f + hole] = a.H[f + second];
+    hole = second;
+  }
+  if ((len & 1) == 0 && second == (len - 2) / 2) {
+    second = 2 * (second + 1);
+    a.K[f + hole] = a.K[f + second - 1];
+    a.H[f + hole] = a.H[f + second - 1];
+    hole = second - 1;
+  }
+  pair_push_heap(a, f, hole, top, vk, vh);
+}
+
+__device__ void pair_heap_sort(PairArr a, int64_t f, int64_t len) {
+  if (len >= 2) {
+    int64_t parent = (len - 2) / 2;
+    while (true) {
+      pair_adjust_heap(a, f, parent, len, a.K[f + parent], a.H[f + parent]);
+      if (parent == 0) break;
+      parent--;
+    }
+  }
+  for (int64_t last = len; last > 1;) {
+    --last;
+    double vk = a.K[f + last];
+    uint32_t vh = a.H[f + last];
+    a.K[f + last] = a.K[f];
+    a.H[f + last] = a.H[f];
+    pair_adjust_heap(a, f, 0, last, vk, vh);
+  }
+}
+
+// pair versions of the libstdc++ partition loop for one lane
+__device__ void pair_partition_loop(PairArr a, int64_t first, int64_t last, int64_t depth) {
+  struct Fr { int64_t f, l, d; };
+  Fr st[64];
+  int sp = 0;
+  st[sp++] = Fr{first, last, depth};
+  while (sp > 0) {
+    Fr fr = st[--sp];
+    int64_t f = fr.f, l = fr.l, d = fr.d;
+    while (l - f > 16) {
+      if (d == 0) {
+        pair_heap_sort(a, f, l - f);
+        break;
+      }
+      --d;
+      int64_t mid = f + (l - f) / 2, x = f + 1, y = mid, z = l - 1;
+      double kx = a.K[x], ky = a.K[y], kz = a.K[z];
+      int64_t pick;
+      if (kx < ky) pick = (ky < kz) ? y : ((kx < kz) ? z : x);
+      else pick = (kx < kz) ? x : ((ky < kz) ? z : y);
+      pair_swap(a, f, pick);
+      const double pv = a.K[f];
+      int64_t lo = f + 1, hi = l;
+      while (true) {
+        while (a.K[lo] < pv) ++lo;
+        --hi;
+        while (pv < a.K[hi]) --hi;
+        if (!(lo < hi)) break;
+        pair_swap(a, lo, hi);
+        ++lo;
+      }
+      if (sp < 64) st[sp++] = Fr{lo, l, d};
+      l = lo;
+    }
+  }
+}
+
+// Block-wide unguarded partition of [f+1, l) around the pivot at f: the
+// k-th element >= pivot from the left swaps with the k-th element <= pivot
+// from the right while the former lies left of the latter; the cut is the
+// (K+1)-th such left element or the K-th right element, whichever is first.
+// All control flow is uniform over the block.
+struct PartShared {
+  uint32_t wge[4], wle[4];
+  unsigned long long total;
+  long long fail;
+  unsigned long long kc;
+};
+
+__device__ int64_t block_partition(PairArr a, uint32_t* posLe, uint32_t* posGe, int64_t f, int64_t l, PartShared& sh) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const uint64_t me = 1ull << lane;
+  const double pv = a.K[f];
+  if (tid == 0) { sh.total = 0; sh.fail = INT64_MAX; sh.kc = 0; }
+  __syncthreads();
+  unsigned long long myLe = 0;
+  for (int64_t i = f + 1 + tid; i < l; i += kBlock) myLe += !(pv < a.K[i]);
+  if (myLe) atomicAdd(&sh.total, myLe);
+  __syncthreads();
+  const int64_t leT = (int64_t)sh.total;
+  int64_t cG = 0, cL = 0;
+  for (int64_t b = f + 1; b < l; b += kBlock) {
+    const int64_t i = b + tid;
+    const bool in = i < l;
+    const double v = in ? a.K[i] : pv;
+    const bool ge = in && !(v < pv), le = in && !(pv < v);
+    const uint64_t bg = __ballot(ge), bl = __ballot(le);
+    if (lane == 0) { sh.wge[wave] = (uint32_t)__popcll(bg); sh.wle[wave] = (uint32_t)__popcll(bl); }
+    __syncthreads();
+    int64_t preG = cG, preL = cL, totG = 0, totL = 0;
+    for (int w = 0; w < 4; w++) {
+      if (w < wave) { preG += sh.wge[w]; preL += sh.wle[w]; }
+      totG += sh.wge[w];
+      totL += sh.wle[w];
+    }
+    if (le) posLe[f + preL + __popcll(bl & lt)] = (uint32_t)(i - f);
+    if (ge) {
+      int64_t k = preG + __popcll(bg & lt) + 1;
+      int64_t leIncl = preL + __popcll(bl & (lt | me));
+      if ((leT - leIncl) >= k) {
+        posGe[f + k - 1] = (uint32_t)(i - f);
+        atomicAdd(&sh.kc, 1ull);
+      } else {
+        atomicMin(&sh.fail, (long long)i);
+      }
+    }
+    cG += totG;
+    cL += totL;
+    __syncthreads();
+  }
+  const int64_t kc = (int64_t)sh.kc;
+  for (int64_t k = tid; k < kc; k += kBlock) {
+    int64_t i = f + posGe[f + k];
+    int64_t j = f + posLe[f + leT - 1 - k];
+    pair_swap(a, i, j);
+  }
+  __syncthreads();
+  const int64_t fail = (int64_t)sh.fail;
+  int64_t cut;
+  if (kc == 0) {
+    cut = fail;
+  } else {
+    int64_t rK = f + posLe[f + leT - kc];
+    cut = fail < rK ? fail : rK;
+  }
+  __syncthreads();
+  return cut;
+}
+
+__global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict__ nodes, NodeWork* __restrict__ work,
+                                                        SampleParams P, unsigned long long* counters) {
+  constexpr int kSmall = 128;     // segments up to this size go to one lane each
+  constexpr int kList = 1024;     // segments per level
+  constexpr int kDeferred = 4096; // segments handed to single lanes
+  __shared__ int32_t s_cf[kList], s_cl[kList], s_cd[kList];
+  __shared__ int32_t s_nf[kList], s_nl[kList], s_nd[kList];
+  __shared__ int32_t s_df[kDeferred], s_dl[kDeferred], s_dd[kDeferred];
+  __shared__ int s_nc, s_nn, s_nd_count, s_bad;
+  __shared__ PartShared sh;
   const int k = blockIdx.x;
   NodeWork w = work[k];
   if (!(w.flags & NW_EXACT_SORT) || (w.flags & NW_OVERFLOW)) return;
-  Layout L = make_layout(w.scratch, w.n, P.method);
-  if (threadIdx.x == 0) {
+  const int64_t n = w.n;
+  Layout L = make_layout(w.scratch, n, P.method);
+  PairArr a{L.sadd, L.idxB};
+  uint32_t* posLe = L.keyB;
+  uint32_t* posGe = L.keyA;
+  const int tid = threadIdx.x;
+  for (int64_t i = tid; i < n; i += kBlock) { a.K[i] = L.add[i]; a.H[i] = (uint32_t)i; }
+  if (tid == 0) {
     atomicAdd(&counters[3], 1ull);
-    for (int64_t j = 0; j < w.n; j++) L.idxA[j] = (uint32_t)j;
-    lstd::sort(L.idxA, w.n, lstd::KeyLess{L.add});
+    s_nc = 0; s_nd_count = 0; s_bad = 0;
+    int64_t d = 0;
+    for (int64_t m = n; m > 1; m >>= 1) d++;
+    if (n > 16) {
+      if (n > kSmall) { s_cf[0] = 0; s_cl[0] = (int32_t)n; s_cd[0] = (int32_t)(2 * d); s_nc = 1; }
+      else { s_df[0] = 0; s_dl[0] = (int32_t)n; s_dd[0] = (int32_t)(2 * d); s_nd_count = 1; }
+    }
   }
   __syncthreads();
-  for (int64_t p = threadIdx.x; p < w.n; p += blockDim.x) L.sadd[p] = L.add[L.idxA[p]];
+  // levels of the recursion: every segment of a level is partitioned by the
+  // whole block; both parts continue with the decremented depth
+  for (int level = 0; level < 128 && s_nc > 0 && !s_bad; level++) {
+    if (tid == 0) s_nn = 0;
+    __syncthreads();
+    const int nc = s_nc;
+    for (int c = 0; c < nc; c++) {
+      int64_t f = s_cf[c], l = s_cl[c];
+      int depth = s_cd[c];
+      if (depth == 0) {
+        if (tid == 0) {
+          if (s_nd_count < kDeferred) { s_df[s_nd_count] = (int32_t)f; s_dl[s_nd_count] = (int32_t)l; s_dd[s_nd_count] = 0; s_nd_count++; }
+          else s_bad = 1;
+        }
+        __syncthreads();
+        continue;
+      }
+      if (tid == 0) {
+        int64_t mid = f + (l - f) / 2, x = f + 1, y = mid, z = l - 1;
+        double kx = a.K[x], ky = a.K[y], kz = a.K[z];
+        int64_t pick;
+        if (kx < ky) pick = (ky < kz) ? y : ((kx < kz) ? z : x);
+        else pick = (kx < kz) ? x : ((ky < kz) ? z : y);
+        pair_swap(a, f, pick);
+      }
+      __syncthreads();
+      int64_t cut = block_partition(a, posLe, posGe, f, l, sh);
+      if (tid == 0) {
+        if (cut <= f || cut >= l) {
+          s_bad = 1;  // cannot happen for a median-of-three pivot; stay bounded
+        } else {
+          int64_t pf[2] = {f, cut}, pl[2] = {cut, l};
+          for (int side = 0; side < 2; side++) {
+            int64_t sz = pl[side] - pf[side];
+            if (sz <= 16) continue;
+            if (sz > kSmall && s_nn < kList) {
+              s_nf[s_nn] = (int32_t)pf[side]; s_nl[s_nn] = (int32_t)pl[side]; s_nd[s_nn] = depth - 1; s_nn++;
+            } else if (sz <= kSmall && s_nd_count < kDeferred) {
+              s_df[s_nd_count] = (int32_t)pf[side]; s_dl[s_nd_count] = (int32_t)pl[side]; s_dd[s_nd_count] = depth - 1;
+              s_nd_count++;
+            } else {
+              s_bad = 1;
+            }
+          }
+        }
+      }
+      __syncthreads();
+      if (s_bad) break;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int c = 0; c < s_nn; c++) { s_cf[c] = s_nf[c]; s_cl[c] = s_nl[c]; s_cd[c] = s_nd[c]; }
+      s_nc = s_nn;
+    }
+    __syncthreads();
+  }
+  if (s_nc > 0) s_bad = 1;
+  __syncthreads();
+  if (s_bad) {
+    // lists exhausted: replay the whole sort on one lane (same result)
+    if (tid == 0) {
+      for (int64_t i = 0; i < n; i++) L.idxB[i] = (uint32_t)i;
+      lstd::sort(L.idxB, n, lstd::KeyLess{L.add});
+      for (int64_t i = 0; i < n; i++) L.sadd[i] = L.add[L.idxB[i]];
+    }
+    __syncthreads();
+  } else {
+    // small segments: the rest of the partition loop, one lane each
+    for (int d = tid; d < s_nd_count; d += kBlock) pair_partition_loop(a, s_df[d], s_dl[d], s_dd[d]);
+    __syncthreads();
+  }
+  // final positions after the partition phase, then order each run of equal
+  // distances (from the radix order in idxA) by those positions
+  for (int64_t i = tid; i < n; i += kBlock) posLe[a.H[i]] = (uint32_t)i;
+  __syncthreads();
+  for (int64_t p = tid; p < n; p += kBlock) L.sadd[p] = L.add[L.idxA[p]];
+  __syncthreads();
+  for (int64_t p = tid; p < n; p += kBlock) {
+    if (p > 0 && L.sadd[p - 1] == L.sadd[p]) continue;
+    int64_t e = p + 1;
+    while (e < n && L.sadd[e] == L.sadd[p]) e++;
+    for (int64_t i = p + 1; i < e; i++) {
+      uint32_t v = L.idxA[i];
+      uint32_t pv = posLe[v];
+      int64_t q = i - 1;
+      while (q >= p && posLe[L.idxA[q]] > pv) {
+        L.idxA[q + 1] = L.idxA[q];
+        q--;
+      }
+      L.idxA[q + 1] = v;
+    }
+  }
 }
 
 // ------------------------------------------------------------- acceptance
@@ -384,19 +680,20 @@ __device__ inline bool within_spacing(const Layout& L, uint32_t a, uint32_t b, c
   return d64 < s2;
 }
 
-__global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
-                                                    NodeWork* __restrict__ work, SampleParams P) {
-  __shared__ unsigned long long s_dec[4], s_acc[4];
-  __shared__ int s_done;
-  __shared__ double s_minlim;
+// Hash grid of all candidates (entries are sorted positions) and, per
+// candidate, the earlier candidates that can reject it: within spacing,
+// and (Poisson) not behind the scan's early exit (sampler_poisson.h:160) or
+// (Poisson-average) inside its 16^3 search range.  Fully parallel.
+__global__ __launch_bounds__(kBlock) void k_neighbors(const int32_t* __restrict__ nodes,
+                                                       const NodeInfo* __restrict__ infos,
+                                                       NodeWork* __restrict__ work, SampleParams P) {
   const int k = blockIdx.x;
   NodeWork w = work[k];
   if (w.flags & NW_OVERFLOW) return;
   const int64_t n = w.n;
   const int tid = threadIdx.x;
   Layout L = make_layout(w.scratch, n, P.method);
-  const NodeInfo info = infos[nodes[k]];
-  NodeGeom g = node_geom(info, P);
+  NodeGeom g = node_geom(infos[nodes[k]], P);
   const bool avg = P.method == METHOD_POISSON_AVERAGE;
   const double cs = 2.0 * g.spacing;
   const double sq = g.spacing * (1.0 + 1e-9);
@@ -404,7 +701,6 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
   const float s2f = (float)g.s2;
   unsigned long long rechecks = 0, tests = 0;
   float maxerr = 0.f;
-  // hash grid over all candidates, entries are sorted positions
   for (uint32_t i = tid; i < L.H; i += kBlock) { L.hkey[i] = kNil; L.hhead[i] = kNil; }
   __syncthreads();
   for (int64_t p = tid; p < n; p += kBlock) {
@@ -422,7 +718,74 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
     L.acc[p] = 0;
   }
   __syncthreads();
-  // greedy acceptance, 256 sorted candidates per window
+  bool many = false;
+  for (int64_t p = tid; p < n; p += kBlock) {
+    uint32_t j = L.idxA[p];
+    double x = coord(L.X[j], P.scale[0], P.offset[0]), y = coord(L.Y[j], P.scale[1], P.offset[1]),
+           z = coord(L.Z[j], P.scale[2], P.offset[2]);
+    int lo16[3], hi16[3];
+    double limSq = 0.0;
+    if (!avg) {
+      double cd = __builtin_sqrt(L.sadd[p]);
+      double lim = cd - g.spacing;
+      limSq = lim * lim;
+    } else {
+      range16(g, x, y, z, lo16, hi16);
+    }
+    int cnt = 0;
+    uint32_t* out = L.nbr + (uint64_t)p * kMaxNbr;
+    int lx = gcell(x - sq, g.min[0], cs), hx = gcell(x + sq, g.min[0], cs);
+    int ly = gcell(y - sq, g.min[1], cs), hy = gcell(y + sq, g.min[1], cs);
+    int lz = gcell(z - sq, g.min[2], cs), hz = gcell(z + sq, g.min[2], cs);
+    for (int cz = lz; cz <= hz; cz++)
+      for (int cy = ly; cy <= hy; cy++)
+        for (int cx = lx; cx <= hx; cx++)
+          for (uint32_t q = grid_find(L, gkey(cx, cy, cz)); q != kNil; q = L.next[q]) {
+            if ((int64_t)q >= p) continue;
+            uint32_t jq = L.idxA[q];
+            if (!avg) {
+              if (L.sadd[q] < limSq) continue;
+            } else {
+              if (!in_range16(L.cell16[jq], lo16, hi16)) continue;
+            }
+            tests++;
+            if (!within_spacing(L, jq, j, P, sc, s2f, g.s2, rechecks, maxerr)) continue;
+            if (cnt < kMaxNbr) out[cnt] = q;
+            cnt++;
+          }
+    if (cnt > kMaxNbr) many = true;
+    L.nbrCount[p] = (uint8_t)(cnt < kMaxNbr ? cnt : kMaxNbr);
+  }
+  if (many) atomicOr(&work[k].flags, (uint32_t)NW_MANY_NBR);
+  atomicAdd(&P.counters[0], rechecks);
+  atomicAdd(&P.counters[1], tests);
+  if (maxerr > 0.f) atomicMax((unsigned int*)&P.counters[2], __float_as_uint(maxerr));
+}
+
+// Greedy acceptance in sorted order, 256 candidates per window: a candidate
+// is rejected by an accepted neighbour of an earlier window; inside the
+// window the dependencies are resolved in rounds.
+__global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
+                                                    NodeWork* __restrict__ work, SampleParams P) {
+  __shared__ unsigned long long s_dec[4], s_acc[4];
+  __shared__ int s_done;
+  __shared__ double s_minlim;
+  const int k = blockIdx.x;
+  NodeWork w = work[k];
+  if (w.flags & NW_OVERFLOW) return;
+  const int64_t n = w.n;
+  const int tid = threadIdx.x;
+  Layout L = make_layout(w.scratch, n, P.method);
+  const NodeInfo info = infos[nodes[k]];
+  NodeGeom g = node_geom(info, P);
+  const bool avg = P.method == METHOD_POISSON_AVERAGE;
+  const bool inline_search = (w.flags & NW_MANY_NBR) != 0;
+  const double cs = 2.0 * g.spacing;
+  const double sq = g.spacing * (1.0 + 1e-9);
+  float sc[3] = {(float)P.scale[0], (float)P.scale[1], (float)P.scale[2]};
+  const float s2f = (float)g.s2;
+  unsigned long long rechecks = 0, tests = 0;
+  float maxerr = 0.f;
   for (int64_t w0 = 0; w0 < n; w0 += kBlock) {
     const int64_t p = w0 + tid;
     const bool valid = p < n;
@@ -431,48 +794,54 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
     double limSq = 0.0;
     if (tid < 4) { s_dec[tid] = 0; s_acc[tid] = 0; }
     if (valid) {
-      uint32_t j = L.idxA[p];
-      double x = coord(L.X[j], P.scale[0], P.offset[0]), y = coord(L.Y[j], P.scale[1], P.offset[1]),
-             z = coord(L.Z[j], P.scale[2], P.offset[2]);
-      int lo16[3], hi16[3];
       if (!avg) {
         double cd = __builtin_sqrt(L.sadd[p]);
         double lim = cd - g.spacing;
         limSq = lim * lim;
-      } else {
-        range16(g, x, y, z, lo16, hi16);
       }
-      int lx = gcell(x - sq, g.min[0], cs), hx = gcell(x + sq, g.min[0], cs);
-      int ly = gcell(y - sq, g.min[1], cs), hy = gcell(y + sq, g.min[1], cs);
-      int lz = gcell(z - sq, g.min[2], cs), hz = gcell(z + sq, g.min[2], cs);
-      for (int cz = lz; cz <= hz; cz++)
-        for (int cy = ly; cy <= hy; cy++)
-          for (int cx = lx; cx <= hx; cx++) {
-            for (uint32_t q = grid_find(L, gkey(cx, cy, cz)); q != kNil; q = L.next[q]) {
-              if ((int64_t)q >= p) continue;
-              if ((int64_t)q < w0 && !L.acc[q]) continue;
-              uint32_t jq = L.idxA[q];
-              if (!avg) {
-                if (L.sadd[q] < limSq) continue;  // behind the early exit (sampler_poisson.h:160)
-              } else {
-                if (!in_range16(L.cell16[jq], lo16, hi16)) continue;
+      if (!inline_search) {
+        const int cnt = L.nbrCount[p];
+        const uint32_t* nb = L.nbr + (uint64_t)p * kMaxNbr;
+        for (int i = 0; i < cnt; i++) {
+          uint32_t q = nb[i];
+          if ((int64_t)q < w0) rej = rej || L.acc[q];
+          else in_mask[(q - w0) >> 6] |= 1ull << ((q - w0) & 63);
+        }
+      } else {
+        uint32_t j = L.idxA[p];
+        double x = coord(L.X[j], P.scale[0], P.offset[0]), y = coord(L.Y[j], P.scale[1], P.offset[1]),
+               z = coord(L.Z[j], P.scale[2], P.offset[2]);
+        int lo16[3], hi16[3];
+        if (avg) range16(g, x, y, z, lo16, hi16);
+        int lx = gcell(x - sq, g.min[0], cs), hx = gcell(x + sq, g.min[0], cs);
+        int ly = gcell(y - sq, g.min[1], cs), hy = gcell(y + sq, g.min[1], cs);
+        int lz = gcell(z - sq, g.min[2], cs), hz = gcell(z + sq, g.min[2], cs);
+        for (int cz = lz; cz <= hz; cz++)
+          for (int cy = ly; cy <= hy; cy++)
+            for (int cx = lx; cx <= hx; cx++)
+              for (uint32_t q = grid_find(L, gkey(cx, cy, cz)); q != kNil; q = L.next[q]) {
+                if ((int64_t)q >= p) continue;
+                if ((int64_t)q < w0 && !L.acc[q]) continue;
+                uint32_t jq = L.idxA[q];
+                if (!avg) {
+                  if (L.sadd[q] < limSq) continue;
+                } else {
+                  if (!in_range16(L.cell16[jq], lo16, hi16)) continue;
+                }
+                tests++;
+                if (!within_spacing(L, jq, j, P, sc, s2f, g.s2, rechecks, maxerr)) continue;
+                if ((int64_t)q < w0) rej = true;
+                else in_mask[(q - w0) >> 6] |= 1ull << ((q - w0) & 63);
               }
-              tests++;
-              if (!within_spacing(L, jq, j, P, sc, s2f, g.s2, rechecks, maxerr)) continue;
-              if ((int64_t)q < w0) rej = true;
-              else in_mask[(q - w0) >> 6] |= 1ull << ((q - w0) & 63);
-            }
-          }
+      }
     }
     if (!avg && tid == 0) s_minlim = 1e308;
     __syncthreads();
     if (!avg && valid) {
       // conservative bound for the 10 001-check window: the candidates with
-      // distance in [limSq, add) bound the accepted points it would scan
-      unsigned long long bits = __double_as_longlong(limSq);
-      atomicMin((unsigned long long*)&s_minlim, bits);
+      // distance in [limSq, add) bound the accepted points the scan visits
+      atomicMin((unsigned long long*)&s_minlim, (unsigned long long)__double_as_longlong(limSq));
     }
-    // resolve dependencies inside the window
     bool decided = !valid;
     bool accepted = false;
     while (true) {
@@ -500,7 +869,6 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
     if (valid) L.acc[p] = accepted ? 1 : 0;
     if (!avg && tid == 0 && w0 + kBlock > 10001) {
       double minlim = s_minlim;
-      // lower_bound of minlim over sadd[0, p_last]
       int64_t lo = 0, hi = (w0 + kBlock < n ? w0 + kBlock : n);
       int64_t last = hi - 1;
       while (lo < hi) {
@@ -511,11 +879,12 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
     }
     __syncthreads();
   }
-  atomicAdd(&P.counters[0], rechecks);
-  atomicAdd(&P.counters[1], tests);
-  if (maxerr > 0.f) atomicMax((unsigned int*)&P.counters[2], __float_as_uint(maxerr));
+  if (inline_search) {
+    atomicAdd(&P.counters[0], rechecks);
+    atomicAdd(&P.counters[1], tests);
+    if (maxerr > 0.f) atomicMax((unsigned int*)&P.counters[2], __float_as_uint(maxerr));
+  }
 }
-
 
 // Poisson-average colour sums: every rejected candidate C adds its own colour
 // (weight 1) to each accepted P with P inside C's 16^3 range and dd < s^2,
@@ -809,7 +1178,9 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   dim3 grid((unsigned)num_nodes);
   hipLaunchKernelGGL(k_prepare, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_state, d_work, p);
   PCG_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_exact_sort, grid, dim3(64), 0, s, d_nodes, d_work, p, p.counters);
+  hipLaunchKernelGGL(k_exact_sort, grid, dim3(kBlock), 0, s, d_nodes, d_work, p, p.counters);
+  PCG_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_neighbors, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p);
   PCG_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_accept, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p);
   PCG_CHECK(hipGetLastError());
