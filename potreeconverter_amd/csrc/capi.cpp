@@ -34,7 +34,7 @@ static int load_las_to_device(const char* las_path, lasfile::Header& h, uint8_t*
     return 1;
   }
   PCG_CHECK(hipMalloc(d_las, host.size() + 64));
-  if (!host.empty()) PCG_CHECK(hipMemcpy(*d_las, host.data(), host.size(), hipMemcpyHostToDevice));
+  if (!host.empty()) PCG_CHECK(stream_copy(*d_las, host.data(), host.size(), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -170,7 +170,7 @@ int pcg_device_free(void* ptr) {
 int pcg_copy(void* dst, const void* src, uint64_t bytes, int32_t kind) {
   if (ensure_device()) return 1;
   hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
-  PCG_CHECK(hipMemcpy(dst, src, bytes, k));
+  PCG_CHECK(stream_copy(dst, src, bytes, k));
   return 0;
 }
 

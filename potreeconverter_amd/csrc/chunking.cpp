@@ -223,7 +223,7 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
     PCG_CHECK(hipStreamSynchronize(s));
     if (numChunks <= maxOut) {
       raw.resize((size_t)5 * numChunks);
-      PCG_CHECK(hipMemcpy(raw.data(), list.p, sizeof(int64_t) * raw.size(), hipMemcpyDeviceToHost));
+      PCG_CHECK(stream_copy(raw.data(), list.p, sizeof(int64_t) * raw.size(), hipMemcpyDeviceToHost));
       break;
     }
     maxOut = numChunks;
@@ -293,7 +293,7 @@ int write_chunk_files(const DeviceChunks& dc, const std::string& targetDir) {
   fs::create_directories(dir);
   for (const auto& e : fs::directory_iterator(dir)) fs::remove(e);
   std::vector<uint8_t> host((size_t)dc.numPoints * dc.bpp);
-  if (!host.empty()) PCG_CHECK(hipMemcpy(host.data(), dc.d_records, host.size(), hipMemcpyDeviceToHost));
+  if (!host.empty()) PCG_CHECK(stream_copy(host.data(), dc.d_records, host.size(), hipMemcpyDeviceToHost));
   for (const auto& c : dc.chunks) {
     if (c.count == 0) continue;
     std::ofstream f(dir + "/" + c.id + ".bin", std::ios::binary);
@@ -341,7 +341,7 @@ int read_chunk_files(const std::string& targetDir, DeviceChunks& out) {
     f.read((char*)host.data() + d.offset * out.bpp, (std::streamsize)(d.count * out.bpp));
   }
   PCG_CHECK(hipMalloc(&out.d_records, host.size() + 64));
-  if (!host.empty()) PCG_CHECK(hipMemcpy(out.d_records, host.data(), host.size(), hipMemcpyHostToDevice));
+  if (!host.empty()) PCG_CHECK(stream_copy(out.d_records, host.data(), host.size(), hipMemcpyHostToDevice));
   return 0;
 }
 

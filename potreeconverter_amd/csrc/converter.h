@@ -44,6 +44,16 @@ struct Device {
 Device& device();
 int ensure_device();
 
+// Blocking copy ordered after everything queued on the library stream.  The
+// stream is non-blocking, so a plain hipMemcpy (null stream) would not wait
+// for it.
+inline hipError_t stream_copy(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  hipStream_t s = device().stream;
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(s);
+}
+
 // doChunking on LAS records already resident in HBM (header describes them).
 int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& out);
 

@@ -213,7 +213,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
       return 1;
     }
     std::vector<NodeOut> no(numOut);
-    if (numOut) PCG_CHECK(hipMemcpy(no.data(), dnodes.p, numOut * sizeof(NodeOut), hipMemcpyDeviceToHost));
+    if (numOut) PCG_CHECK(stream_copy(no.data(), dnodes.p, numOut * sizeof(NodeOut), hipMemcpyDeviceToHost));
     std::vector<Seg> next;
     for (const NodeOut& o : no) {
       const Seg& sg = segs[o.seg];
@@ -278,13 +278,23 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   DevMem dinfo, dstate, dwork, dlist, arena, scratch, tops, counters;
   const int methodId = avg ? METHOD_POISSON_AVERAGE : METHOD_POISSON;
   uint64_t arenaCap = (uint64_t)N * bpp * (avg ? 4 : 3) + (1u << 20);
-  // per node: linear arrays plus a hash table of at most 4n slots (the
-  // power of two >= 2n) or the Poisson-average minimum of 4096 slots
-  uint64_t scratchCap = (uint64_t)N * (avg ? 240 : 170) + (uint64_t)maxLevelNodes * (sample_scratch_bytes(1, methodId) + 40960);
+  // every candidate of a level is a point of the chunk records
+  uint64_t scratchCap = sample_scratch_bound(N, maxLevelNodes, methodId);
+  // all level lists in one upload, deepest level first
+  std::vector<int32_t> allLevels;
+  std::vector<size_t> levelOffset(maxLevel + 1, 0);
+  for (int l = maxLevel; l >= 0; l--) {
+    levelOffset[l] = allLevels.size();
+    allLevels.insert(allLevels.end(), levels[l].begin(), levels[l].end());
+  }
+  DevMem ditems;
   if (dinfo.alloc(sizeof(NodeInfo) * numNodes) || dstate.alloc(sizeof(NodeState) * numNodes) ||
-      dwork.alloc(sizeof(NodeWork) * maxLevelNodes) || dlist.alloc(sizeof(int32_t) * maxLevelNodes) ||
-      arena.alloc(arenaCap) || scratch.alloc(scratchCap) || tops.alloc(64) || counters.alloc(64))
+      dwork.alloc(sizeof(NodeWork) * maxLevelNodes) || dlist.alloc(sizeof(int32_t) * (allLevels.size() + 1)) ||
+      ditems.alloc(sizeof(uint32_t) * (maxLevelNodes + 1)) || arena.alloc(arenaCap) || scratch.alloc(scratchCap) ||
+      tops.alloc(64) || counters.alloc(64))
     return 1;
+  if (!allLevels.empty())
+    PCG_CHECK(hipMemcpyAsync(dlist.p, allLevels.data(), sizeof(int32_t) * allLevels.size(), hipMemcpyHostToDevice, s));
   PCG_CHECK(hipMemcpyAsync(dinfo.p, info.data(), sizeof(NodeInfo) * numNodes, hipMemcpyHostToDevice, s));
   PCG_CHECK(hipMemcpyAsync(dstate.p, state.data(), sizeof(NodeState) * numNodes, hipMemcpyHostToDevice, s));
   PCG_CHECK(hipMemsetAsync(tops.p, 0, 64, s));
@@ -304,24 +314,22 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   sp.scratch_cap = scratchCap;
   sp.scratch_top = tops.as<uint64_t>() + 1;
   sp.counters = counters.as<unsigned long long>();
+  sp.item_start = ditems.as<uint32_t>();
   std::vector<uint32_t> workFlags;
   int64_t sampled = 0;
   for (int l = maxLevel; l >= 0; l--) {
     const auto& lst = levels[l];
     if (lst.empty()) continue;
     sampled += (int64_t)lst.size();
-    PCG_CHECK(hipMemcpyAsync(dlist.p, lst.data(), sizeof(int32_t) * lst.size(), hipMemcpyHostToDevice, s));
     PCG_CHECK(hipMemsetAsync(tops.as<uint64_t>() + 1, 0, 8, s));
-    if (sample_level(dlist.as<int32_t>(), (int32_t)lst.size(), dinfo.as<NodeInfo>(), dstate.as<NodeState>(),
-                     dwork.as<NodeWork>(), sp, s))
+    if (sample_level(dlist.as<int32_t>() + levelOffset[l], (int32_t)lst.size(), dinfo.as<NodeInfo>(),
+                     dstate.as<NodeState>(), dwork.as<NodeWork>(), sp, s))
       return 1;
-    // the level list buffer is reused by the next level: wait for this one
-    PCG_CHECK(hipStreamSynchronize(s));
   }
   out.numSampled = sampled;
-  PCG_CHECK(hipMemcpy(state.data(), dstate.p, sizeof(NodeState) * numNodes, hipMemcpyDeviceToHost));
+  PCG_CHECK(stream_copy(state.data(), dstate.p, sizeof(NodeState) * numNodes, hipMemcpyDeviceToHost));
   unsigned long long hc[8];
-  PCG_CHECK(hipMemcpy(hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost));
+  PCG_CHECK(stream_copy(hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost));
   for (int i = 0; i < 8; i++) out.counters[i] = hc[i];
   if (hc[6] != 0) {
     set_error("sampling arena or scratch exhausted in " + std::to_string(hc[6]) + " nodes");

@@ -61,6 +61,7 @@ struct SampleParams {
   uint8_t* scratch;        // per-level scratch
   uint64_t scratch_cap;
   uint64_t* scratch_top;
+  uint32_t* item_start;    // num_nodes + 1 entries: work items of the level
   unsigned long long* counters;  // [0] fp64 re-checks, [1] pair tests, [2] max |dd32-dd64|/s^2 bits,
                                  // [3] exact-sort nodes, [4] sequential nodes, [5] candidates,
                                  // [6] nodes that ran out of scratch or arena
@@ -72,6 +73,9 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
 
 // Scratch bytes for one node of n candidates.
 uint64_t sample_scratch_bytes(int64_t n, int method);
+// Scratch bytes that cover any level of `max_nodes` nodes holding at most
+// `candidates` candidates in total.
+uint64_t sample_scratch_bound(int64_t candidates, int64_t max_nodes, int method);
 
 // Copies each (src, bytes) range to dst + offset (one workgroup per range).
 struct CopyDesc {

@@ -31,6 +31,8 @@ constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kNil = 0xFFFFFFFFu;
 constexpr int kMaxNbr = 16;  // earlier neighbours kept per candidate
+constexpr uint8_t kNbrOverflow = 255;
+constexpr int kNeighborBlocks = 4096;  // grid of the level-wide neighbour search
 
 __device__ inline int32_t ld_i32(const uint8_t* p) {
   return (int32_t)((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24));
@@ -51,50 +53,71 @@ struct Layout {
   double* add;       // main order
   uint32_t *keyA, *idxA, *keyB, *idxB;
   double* sadd;      // sorted order
-  uint32_t* sidx;    // sorted -> main (points at idxA or idxB)
+  uint32_t* sidx;    // sorted -> main (points at idxA)
   uint8_t* acc;      // sorted order
   uint32_t* cell16;  // main order, packed 16^3 cell (average)
   uint16_t* rgb;     // main order, own colour (average)
-  uint32_t *hkey, *hhead, *next;
   uint32_t H;
+  uint32_t *hkey, *hstart, *hend;  // cell table: 2*spacing cell -> [start, end) in cell order
+  uint32_t *hhead, *next;          // lists of the sequential forms
+  uint32_t* eP;                    // cell order -> sorted position (ascending inside a cell)
+  int32_t *eX, *eY, *eZ;           // cell order coordinates
+  double* eAdd;                    // cell order squared distance to the centre (Poisson)
+  uint32_t* eC16;                  // cell order 16^3 cell (average)
   long long* sums;   // sorted order, 8 per candidate (average)
   uint32_t* nbr;     // sorted order, kMaxNbr earlier neighbours per candidate
-  uint8_t* nbrCount;
+  uint8_t* nbrCount; // kNbrOverflow: more than kMaxNbr
 };
 
+// Carves the per-node scratch into the arrays above; with base == nullptr it
+// only measures.  Returns the bytes used.
+__host__ __device__ inline uint64_t layout_carve(uint8_t* base, int64_t n, int method, Layout* L) {
+  uint64_t off = 0;
+  auto take = [&](uint64_t bytes) -> uint8_t* {
+    uint8_t* r = base ? base + off : nullptr;
+    off += align16(bytes);
+    return r;
+  };
+  const bool avg = method == METHOD_POISSON_AVERAGE;
+  L->X = (int32_t*)take(4 * n);
+  L->Y = (int32_t*)take(4 * n);
+  L->Z = (int32_t*)take(4 * n);
+  L->add = (double*)take(8 * n);
+  L->keyA = (uint32_t*)take(4 * n);
+  L->idxA = (uint32_t*)take(4 * n);
+  L->keyB = (uint32_t*)take(4 * n);
+  L->idxB = (uint32_t*)take(4 * n);
+  L->sadd = (double*)take(8 * n);
+  L->acc = (uint8_t*)take(n);
+  L->cell16 = (uint32_t*)take(4 * n);
+  L->H = hash_cap(n, method);
+  L->hkey = (uint32_t*)take(4 * (uint64_t)L->H);
+  L->hstart = (uint32_t*)take(4 * (uint64_t)L->H);
+  L->hend = (uint32_t*)take(4 * (uint64_t)L->H);
+  L->hhead = (uint32_t*)take(4 * (uint64_t)L->H);
+  L->next = (uint32_t*)take(4 * n);
+  L->eP = (uint32_t*)take(4 * n);
+  L->eX = (int32_t*)take(4 * n);
+  L->eY = (int32_t*)take(4 * n);
+  L->eZ = (int32_t*)take(4 * n);
+  L->eAdd = avg ? nullptr : (double*)take(8 * n);
+  L->eC16 = avg ? (uint32_t*)take(4 * n) : nullptr;
+  L->sums = avg ? (long long*)take(64 * n) : nullptr;
+  L->rgb = avg ? (uint16_t*)take(6 * n) : nullptr;
+  L->nbr = (uint32_t*)take(4 * kMaxNbr * n);
+  L->nbrCount = (uint8_t*)take(n);
+  L->sidx = L->idxA;
+  return off + 64;
+}
+
 __host__ __device__ inline uint64_t layout_bytes(int64_t n, int method) {
-  uint64_t H = hash_cap(n, method);
-  uint64_t b = 3 * align16(4 * n) + align16(8 * n) + 4 * align16(4 * n) + align16(8 * n) + align16(n) +
-               align16(4 * n) + 2 * align16(4 * H) + align16(4 * n);
-  if (method == METHOD_POISSON_AVERAGE) b += align16(64 * n) + align16(6 * n);
-  b += align16(4 * kMaxNbr * n) + align16(n);
-  return b + 64;
+  Layout L;
+  return layout_carve(nullptr, n, method, &L);
 }
 
 __device__ inline Layout make_layout(uint8_t* base, int64_t n, int method) {
   Layout L;
-  uint8_t* p = base;
-  auto take = [&](uint64_t bytes) { uint8_t* r = p; p += align16(bytes); return r; };
-  L.X = (int32_t*)take(4 * n);
-  L.Y = (int32_t*)take(4 * n);
-  L.Z = (int32_t*)take(4 * n);
-  L.add = (double*)take(8 * n);
-  L.keyA = (uint32_t*)take(4 * n);
-  L.idxA = (uint32_t*)take(4 * n);
-  L.keyB = (uint32_t*)take(4 * n);
-  L.idxB = (uint32_t*)take(4 * n);
-  L.sadd = (double*)take(8 * n);
-  L.acc = (uint8_t*)take(n);
-  L.cell16 = (uint32_t*)take(4 * n);
-  L.H = hash_cap(n, method);
-  L.hkey = (uint32_t*)take(4 * (uint64_t)L.H);
-  L.hhead = (uint32_t*)take(4 * (uint64_t)L.H);
-  L.next = (uint32_t*)take(4 * n);
-  L.sums = method == METHOD_POISSON_AVERAGE ? (long long*)take(64 * n) : nullptr;
-  L.rgb = method == METHOD_POISSON_AVERAGE ? (uint16_t*)take(6 * n) : nullptr;
-  L.nbr = (uint32_t*)take(4 * kMaxNbr * n);
-  L.nbrCount = (uint8_t*)take(n);
-  L.sidx = L.idxA;
+  layout_carve(base, n, method, &L);
   return L;
 }
 
@@ -175,6 +198,70 @@ __device__ inline uint64_t wave_match8(uint32_t key, uint64_t active) {
   return peers;
 }
 
+// ------------------------------------------------------ block radix pass
+struct RadixShared {
+  uint32_t hist[256];
+  uint32_t base[256];
+  uint32_t wc[kWaves][256];
+  int skip;
+};
+
+// One stable counting pass of a block-wide LSD radix sort over the digit
+// (key >> shift) & mask (mask < 256) of n (key, value) pairs.  Returns false
+// and writes nothing when every key has the same digit.
+__device__ bool block_radix_pass(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                 uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n, int shift,
+                                 uint32_t mask, RadixShared& sh) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  sh.hist[tid] = 0;
+  __syncthreads();
+  for (int64_t j = tid; j < n; j += kBlock) atomicAdd(&sh.hist[(kin[j] >> shift) & mask], 1u);
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t h0 = sh.hist[4 * lane], h1 = sh.hist[4 * lane + 1], h2 = sh.hist[4 * lane + 2],
+             h3 = sh.hist[4 * lane + 3];
+    uint32_t s = h0 + h1 + h2 + h3, incl = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      uint32_t t = __shfl_up(incl, d);
+      if (lane >= d) incl += t;
+    }
+    uint32_t ex = incl - s;
+    sh.base[4 * lane] = ex;
+    sh.base[4 * lane + 1] = ex + h0;
+    sh.base[4 * lane + 2] = ex + h0 + h1;
+    sh.base[4 * lane + 3] = ex + h0 + h1 + h2;
+    const uint32_t un = (uint32_t)n;
+    const bool full = h0 == un || h1 == un || h2 == un || h3 == un;
+    const uint64_t any = __ballot(full);
+    if (lane == 0) sh.skip = any != 0;
+  }
+  __syncthreads();
+  if (sh.skip) return false;
+  for (int64_t b0 = 0; b0 < n; b0 += kBlock) {
+    for (int w = 0; w < kWaves; w++) sh.wc[w][tid] = 0;
+    __syncthreads();
+    const int64_t j = b0 + tid;
+    const bool valid = j < n;
+    const uint32_t key = valid ? kin[j] : 0;
+    const uint32_t d = (key >> shift) & mask;
+    const uint64_t peers = wave_match8(d, __ballot(valid));
+    if (valid && (peers & lt) == 0) sh.wc[wave][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = sh.base[d] + (uint32_t)__popcll(peers & lt);
+      for (int w = 0; w < wave; w++) pos += sh.wc[w][d];
+      kout[pos] = key;
+      vout[pos] = vin[j];
+    }
+    __syncthreads();
+    sh.base[tid] += sh.wc[0][tid] + sh.wc[1][tid] + sh.wc[2][tid] + sh.wc[3][tid];
+    __syncthreads();
+  }
+  return true;
+}
+
 // ---------------------------------------------------------------- prepare
 __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
                                                      const NodeState* __restrict__ states, NodeWork* __restrict__ work,
@@ -182,10 +269,7 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
   __shared__ int64_t s_cs[9];
   __shared__ const uint8_t* s_cptr[8];
   __shared__ uint8_t* s_scr;
-  __shared__ uint32_t s_hist[256];
-  __shared__ uint32_t s_base[256];
-  __shared__ uint32_t s_wc[kWaves][256];
-  __shared__ int s_skip;
+  __shared__ RadixShared s_rad;
   const int k = blockIdx.x;
   const NodeInfo info = infos[nodes[k]];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -250,44 +334,8 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
   __syncthreads();
   // stable LSD radix sort on float(add) bits, 4 x 8 bits
   uint32_t *kc = L.keyA, *ic = L.idxA, *ko = L.keyB, *io = L.idxB;
-  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   for (int pass = 0; pass < 4; pass++) {
-    const int shift = 8 * pass;
-    s_hist[tid] = 0;
-    __syncthreads();
-    for (int64_t j = tid; j < n; j += kBlock) atomicAdd(&s_hist[(kc[j] >> shift) & 255], 1u);
-    __syncthreads();
-    if (tid == 0) {
-      uint32_t run = 0;
-      s_skip = 0;
-      for (int b = 0; b < 256; b++) {
-        if ((int64_t)s_hist[b] == n) s_skip = 1;
-        s_base[b] = run;
-        run += s_hist[b];
-      }
-    }
-    __syncthreads();
-    if (s_skip) continue;
-    for (int64_t b0 = 0; b0 < n; b0 += kBlock) {
-      for (int w = 0; w < kWaves; w++) s_wc[w][tid] = 0;
-      __syncthreads();
-      int64_t j = b0 + tid;
-      bool valid = j < n;
-      uint32_t key = valid ? kc[j] : 0;
-      uint32_t d = (key >> shift) & 255;
-      uint64_t peers = wave_match8(d, __ballot(valid));
-      if (valid && (peers & lt) == 0) s_wc[wave][d] = (uint32_t)__popcll(peers);
-      __syncthreads();
-      if (valid) {
-        uint32_t pos = s_base[d] + (uint32_t)__popcll(peers & lt);
-        for (int w = 0; w < wave; w++) pos += s_wc[w][d];
-        ko[pos] = key;
-        io[pos] = ic[j];
-      }
-      __syncthreads();
-      s_base[tid] += s_wc[0][tid] + s_wc[1][tid] + s_wc[2][tid] + s_wc[3][tid];
-      __syncthreads();
-    }
+    if (!block_radix_pass(kc, ic, ko, io, n, 8 * pass, 255u, s_rad)) continue;
     uint32_t* t = kc; kc = ko; ko = t;
     t = ic; ic = io; io = t;
   }
@@ -656,136 +704,273 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
 }
 
 // ------------------------------------------------------------- acceptance
-__device__ inline double dd64_pair(const Layout& L, uint32_t a, uint32_t b, const SampleParams& P) {
-  double dx = coord(L.X[a], P.scale[0], P.offset[0]) - coord(L.X[b], P.scale[0], P.offset[0]);
-  double dy = coord(L.Y[a], P.scale[1], P.offset[1]) - coord(L.Y[b], P.scale[1], P.offset[1]);
-  double dz = coord(L.Z[a], P.scale[2], P.offset[2]) - coord(L.Z[b], P.scale[2], P.offset[2]);
+__device__ inline double dd64_xyz(int32_t ax, int32_t ay, int32_t az, int32_t bx, int32_t by, int32_t bz,
+                                  const SampleParams& P) {
+  double dx = coord(ax, P.scale[0], P.offset[0]) - coord(bx, P.scale[0], P.offset[0]);
+  double dy = coord(ay, P.scale[1], P.offset[1]) - coord(by, P.scale[1], P.offset[1]);
+  double dz = coord(az, P.scale[2], P.offset[2]) - coord(bz, P.scale[2], P.offset[2]);
   return dx * dx + dy * dy + dz * dz;
+}
+
+__device__ inline double dd64_pair(const Layout& L, uint32_t a, uint32_t b, const SampleParams& P) {
+  return dd64_xyz(L.X[a], L.Y[a], L.Z[a], L.X[b], L.Y[b], L.Z[b], P);
 }
 
 // exact "dd < spacing^2" through an FP32 filter on integer deltas; the
 // FP64 reference expression is evaluated only inside a +-1e-3 band
-__device__ inline bool within_spacing(const Layout& L, uint32_t a, uint32_t b, const SampleParams& P, float sc[3],
-                                      float s2f, double s2, unsigned long long& rechecks, float& maxerr) {
-  float dx = (float)((int64_t)L.X[a] - (int64_t)L.X[b]) * sc[0];
-  float dy = (float)((int64_t)L.Y[a] - (int64_t)L.Y[b]) * sc[1];
-  float dz = (float)((int64_t)L.Z[a] - (int64_t)L.Z[b]) * sc[2];
+struct Filter {
+  float sc[3];
+  float s2f;
+  double s2;
+  unsigned long long rechecks = 0, tests = 0;
+  float maxerr = 0.f;
+};
+
+__device__ inline bool within_xyz(int32_t ax, int32_t ay, int32_t az, int32_t bx, int32_t by, int32_t bz,
+                                  const SampleParams& P, Filter& F) {
+  F.tests++;
+  float dx = (float)((int64_t)ax - (int64_t)bx) * F.sc[0];
+  float dy = (float)((int64_t)ay - (int64_t)by) * F.sc[1];
+  float dz = (float)((int64_t)az - (int64_t)bz) * F.sc[2];
   float d32 = dx * dx + dy * dy + dz * dz;
-  if (d32 > s2f * 1.001f) return false;
-  if (d32 < s2f * 0.999f) return true;
-  rechecks++;
-  double d64 = dd64_pair(L, a, b, P);
-  float e = (float)(__builtin_fabs((double)d32 - d64) / s2);
-  maxerr = e > maxerr ? e : maxerr;
-  return d64 < s2;
+  if (d32 > F.s2f * 1.001f) return false;
+  if (d32 < F.s2f * 0.999f) return true;
+  F.rechecks++;
+  double d64 = dd64_xyz(ax, ay, az, bx, by, bz, P);
+  float e = (float)(__builtin_fabs((double)d32 - d64) / F.s2);
+  F.maxerr = e > F.maxerr ? e : F.maxerr;
+  return d64 < F.s2;
 }
 
-// Hash grid of all candidates (entries are sorted positions) and, per
-// candidate, the earlier candidates that can reject it: within spacing,
-// and (Poisson) not behind the scan's early exit (sampler_poisson.h:160) or
-// (Poisson-average) inside its 16^3 search range.  Fully parallel.
-__global__ __launch_bounds__(kBlock) void k_neighbors(const int32_t* __restrict__ nodes,
-                                                       const NodeInfo* __restrict__ infos,
-                                                       NodeWork* __restrict__ work, SampleParams P) {
+__device__ inline Filter make_filter(const NodeGeom& g, const SampleParams& P) {
+  Filter F;
+  for (int a = 0; a < 3; a++) F.sc[a] = (float)P.scale[a];
+  F.s2f = (float)g.s2;
+  F.s2 = g.s2;
+  return F;
+}
+
+// one atomic per wave for the filter statistics
+__device__ inline void flush_filter(const Filter& F, const SampleParams& P) {
+  unsigned long long r = F.rechecks, t = F.tests;
+  float m = F.maxerr;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    r += __shfl_xor(r, d);
+    t += __shfl_xor(t, d);
+    float o = __shfl_xor(m, d);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (r) atomicAdd(&P.counters[0], r);
+    if (t) atomicAdd(&P.counters[1], t);
+    if (m > 0.f) atomicMax((unsigned int*)&P.counters[2], __float_as_uint(m));
+  }
+}
+
+// slot of a cell in the cell table, kNil when the cell holds no candidate
+__device__ inline uint32_t table_find(const Layout& L, uint32_t key) {
+  uint32_t slot = ghash(key, L.H);
+  for (uint32_t probe = 0; probe < L.H; probe++) {
+    uint32_t k = L.hkey[slot];
+    if (k == key) return slot;
+    if (k == kNil) return kNil;
+    slot = (slot + 1) & (L.H - 1);
+  }
+  return kNil;
+}
+
+struct CellBox {
+  int lx, hx, ly, hy, lz, hz;
+};
+
+__device__ inline CellBox cell_box(const NodeGeom& g, double x, double y, double z) {
+  const double cs = 2.0 * g.spacing, sq = g.spacing * (1.0 + 1e-9);
+  CellBox b;
+  b.lx = gcell(x - sq, g.min[0], cs); b.hx = gcell(x + sq, g.min[0], cs);
+  b.ly = gcell(y - sq, g.min[1], cs); b.hy = gcell(y + sq, g.min[1], cs);
+  b.lz = gcell(z - sq, g.min[2], cs); b.hz = gcell(z + sq, g.min[2], cs);
+  return b;
+}
+
+// Cell table of a node: candidates sorted by their 2*spacing cell (stable,
+// so ascending sorted position inside a cell) with their coordinates
+// copied alongside, and a hash from cell to its range.
+__global__ __launch_bounds__(kBlock) void k_cells(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
+                                                   NodeWork* __restrict__ work, SampleParams P) {
+  __shared__ RadixShared s_rad;
   const int k = blockIdx.x;
-  NodeWork w = work[k];
+  const NodeWork w = work[k];
   if (w.flags & NW_OVERFLOW) return;
   const int64_t n = w.n;
   const int tid = threadIdx.x;
   Layout L = make_layout(w.scratch, n, P.method);
-  NodeGeom g = node_geom(infos[nodes[k]], P);
+  const NodeGeom g = node_geom(infos[nodes[k]], P);
   const bool avg = P.method == METHOD_POISSON_AVERAGE;
   const double cs = 2.0 * g.spacing;
-  const double sq = g.spacing * (1.0 + 1e-9);
-  float sc[3] = {(float)P.scale[0], (float)P.scale[1], (float)P.scale[2]};
-  const float s2f = (float)g.s2;
-  unsigned long long rechecks = 0, tests = 0;
-  float maxerr = 0.f;
-  for (uint32_t i = tid; i < L.H; i += kBlock) { L.hkey[i] = kNil; L.hhead[i] = kNil; }
-  __syncthreads();
   for (int64_t p = tid; p < n; p += kBlock) {
     uint32_t j = L.idxA[p];
     double x = coord(L.X[j], P.scale[0], P.offset[0]), y = coord(L.Y[j], P.scale[1], P.offset[1]),
            z = coord(L.Z[j], P.scale[2], P.offset[2]);
-    uint32_t key = gkey(gcell(x, g.min[0], cs), gcell(y, g.min[1], cs), gcell(z, g.min[2], cs));
-    uint32_t slot = ghash(key, L.H);
-    while (true) {
-      uint32_t prev = atomicCAS(&L.hkey[slot], kNil, key);
-      if (prev == kNil || prev == key) break;
-      slot = (slot + 1) & (L.H - 1);
+    L.keyA[p] = gkey(gcell(x, g.min[0], cs), gcell(y, g.min[1], cs), gcell(z, g.min[2], cs));
+    L.idxB[p] = (uint32_t)p;
+  }
+  for (uint32_t i = tid; i < L.H; i += kBlock) L.hkey[i] = kNil;
+  __syncthreads();
+  uint32_t *kc = L.keyA, *vc = L.idxB, *ko = L.keyB, *vo = L.next;
+  for (int pass = 0; pass < 3; pass++) {
+    if (!block_radix_pass(kc, vc, ko, vo, n, 7 * pass, 127u, s_rad)) continue;
+    uint32_t* t = kc; kc = ko; ko = t;
+    t = vc; vc = vo; vo = t;
+  }
+  for (int64_t e = tid; e < n; e += kBlock) {
+    const uint32_t p = vc[e];
+    const uint32_t j = L.idxA[p];
+    L.eP[e] = p;
+    L.eX[e] = L.X[j];
+    L.eY[e] = L.Y[j];
+    L.eZ[e] = L.Z[j];
+    if (avg) L.eC16[e] = L.cell16[j];
+    else L.eAdd[e] = L.sadd[p];
+    const uint32_t key = kc[e];
+    if (e == 0 || kc[e - 1] != key) {
+      uint32_t slot = ghash(key, L.H);
+      while (atomicCAS(&L.hkey[slot], kNil, key) != kNil) slot = (slot + 1) & (L.H - 1);
+      L.hstart[slot] = (uint32_t)e;
     }
-    L.next[p] = atomicExch(&L.hhead[slot], (uint32_t)p);
-    L.acc[p] = 0;
   }
   __syncthreads();
-  bool many = false;
-  for (int64_t p = tid; p < n; p += kBlock) {
-    uint32_t j = L.idxA[p];
-    double x = coord(L.X[j], P.scale[0], P.offset[0]), y = coord(L.Y[j], P.scale[1], P.offset[1]),
-           z = coord(L.Z[j], P.scale[2], P.offset[2]);
+  for (int64_t e = tid; e < n; e += kBlock) {
+    const uint32_t key = kc[e];
+    if (e + 1 == n || kc[e + 1] != key) L.hend[table_find(L, key)] = (uint32_t)(e + 1);
+  }
+}
+
+// Work items of a level: 256 candidates of one node each.  item_start[k]
+// is the first item of node k, item_start[num_nodes] the total.
+__global__ __launch_bounds__(1024) void k_plan(const NodeWork* __restrict__ work, int32_t num_nodes,
+                                                uint32_t* __restrict__ item_start) {
+  __shared__ uint32_t s_w[16];
+  __shared__ uint32_t s_run;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) s_run = 0;
+  __syncthreads();
+  for (int b = 0; b < num_nodes; b += 1024) {
+    const int k = b + tid;
+    uint32_t v = 0;
+    if (k < num_nodes && !(work[k].flags & NW_OVERFLOW)) v = (uint32_t)((work[k].n + kBlock - 1) / kBlock);
+    uint32_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      uint32_t t = __shfl_up(incl, d);
+      if (lane >= d) incl += t;
+    }
+    if (lane == 63) s_w[wave] = incl;
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+    for (int q = 0; q < 16; q++) {
+      if (q < wave) woff += s_w[q];
+      tot += s_w[q];
+    }
+    const uint32_t run = s_run;
+    if (k < num_nodes) item_start[k] = run + woff + incl - v;
+    __syncthreads();
+    if (tid == 0) s_run = run + tot;
+    __syncthreads();
+  }
+  if (tid == 0) item_start[num_nodes] = s_run;
+}
+
+// Per candidate, the earlier candidates that can reject it: within spacing
+// and (Poisson) not behind the scan's early exit (sampler_poisson.h:160) or
+// (Poisson-average) inside its 16^3 search range.  Candidates are taken in
+// cell order so that the lanes of a wave read the same cell ranges; all
+// nodes of the level share one grid.
+__global__ __launch_bounds__(kBlock) void k_neighbors(const int32_t* __restrict__ nodes,
+                                                       const NodeInfo* __restrict__ infos,
+                                                       NodeWork* __restrict__ work, int32_t num_nodes,
+                                                       const uint32_t* __restrict__ item_start, SampleParams P) {
+  const uint32_t total = item_start[num_nodes];
+  const int tid = threadIdx.x;
+  const bool avg = P.method == METHOD_POISSON_AVERAGE;
+  Filter stats;
+  for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
+    int lo = 0, hi = num_nodes - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (item_start[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    const int k = __builtin_amdgcn_readfirstlane(lo);
+    const NodeWork& w = work[k];
+    const int64_t n = w.n;
+    const int64_t e = (int64_t)(item - item_start[k]) * kBlock + tid;
+    if (e >= n) continue;
+    const Layout L = make_layout(w.scratch, n, P.method);
+    const NodeGeom g = node_geom(infos[nodes[k]], P);
+    Filter F = make_filter(g, P);
+    const uint32_t p = L.eP[e];
+    const int32_t X = L.eX[e], Y = L.eY[e], Z = L.eZ[e];
+    const double x = coord(X, P.scale[0], P.offset[0]), y = coord(Y, P.scale[1], P.offset[1]),
+                 z = coord(Z, P.scale[2], P.offset[2]);
     int lo16[3], hi16[3];
     double limSq = 0.0;
     if (!avg) {
-      double cd = __builtin_sqrt(L.sadd[p]);
+      double cd = __builtin_sqrt(L.eAdd[e]);
       double lim = cd - g.spacing;
       limSq = lim * lim;
     } else {
       range16(g, x, y, z, lo16, hi16);
     }
+    const CellBox cb = cell_box(g, x, y, z);
     int cnt = 0;
     uint32_t* out = L.nbr + (uint64_t)p * kMaxNbr;
-    int lx = gcell(x - sq, g.min[0], cs), hx = gcell(x + sq, g.min[0], cs);
-    int ly = gcell(y - sq, g.min[1], cs), hy = gcell(y + sq, g.min[1], cs);
-    int lz = gcell(z - sq, g.min[2], cs), hz = gcell(z + sq, g.min[2], cs);
-    for (int cz = lz; cz <= hz; cz++)
-      for (int cy = ly; cy <= hy; cy++)
-        for (int cx = lx; cx <= hx; cx++)
-          for (uint32_t q = grid_find(L, gkey(cx, cy, cz)); q != kNil; q = L.next[q]) {
-            if ((int64_t)q >= p) continue;
-            uint32_t jq = L.idxA[q];
+    for (int cz = cb.lz; cz <= cb.hz; cz++)
+      for (int cy = cb.ly; cy <= cb.hy; cy++)
+        for (int cx = cb.lx; cx <= cb.hx; cx++) {
+          const uint32_t slot = table_find(L, gkey(cx, cy, cz));
+          if (slot == kNil) continue;
+          const uint32_t f1 = L.hend[slot];
+          for (uint32_t f = L.hstart[slot]; f < f1; f++) {
+            const uint32_t q = L.eP[f];
+            if (q >= p) break;
             if (!avg) {
-              if (L.sadd[q] < limSq) continue;
+              if (L.eAdd[f] < limSq) continue;
             } else {
-              if (!in_range16(L.cell16[jq], lo16, hi16)) continue;
+              if (!in_range16(L.eC16[f], lo16, hi16)) continue;
             }
-            tests++;
-            if (!within_spacing(L, jq, j, P, sc, s2f, g.s2, rechecks, maxerr)) continue;
+            if (!within_xyz(L.eX[f], L.eY[f], L.eZ[f], X, Y, Z, P, F)) continue;
             if (cnt < kMaxNbr) out[cnt] = q;
             cnt++;
           }
-    if (cnt > kMaxNbr) many = true;
-    L.nbrCount[p] = (uint8_t)(cnt < kMaxNbr ? cnt : kMaxNbr);
+        }
+    L.nbrCount[p] = cnt <= kMaxNbr ? (uint8_t)cnt : kNbrOverflow;
+    if (cnt > kMaxNbr) atomicOr(&work[k].flags, (uint32_t)NW_MANY_NBR);
+    stats.rechecks += F.rechecks;
+    stats.tests += F.tests;
+    stats.maxerr = F.maxerr > stats.maxerr ? F.maxerr : stats.maxerr;
   }
-  if (many) atomicOr(&work[k].flags, (uint32_t)NW_MANY_NBR);
-  atomicAdd(&P.counters[0], rechecks);
-  atomicAdd(&P.counters[1], tests);
-  if (maxerr > 0.f) atomicMax((unsigned int*)&P.counters[2], __float_as_uint(maxerr));
+  flush_filter(stats, P);
 }
 
 // Greedy acceptance in sorted order, 256 candidates per window: a candidate
 // is rejected by an accepted neighbour of an earlier window; inside the
-// window the dependencies are resolved in rounds.
+// window the dependencies are resolved in rounds.  Candidates whose
+// neighbour list overflowed search the cell table for accepted neighbours.
 __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
                                                     NodeWork* __restrict__ work, SampleParams P) {
   __shared__ unsigned long long s_dec[4], s_acc[4];
   __shared__ int s_done;
   __shared__ double s_minlim;
   const int k = blockIdx.x;
-  NodeWork w = work[k];
+  const NodeWork w = work[k];
   if (w.flags & NW_OVERFLOW) return;
   const int64_t n = w.n;
   const int tid = threadIdx.x;
   Layout L = make_layout(w.scratch, n, P.method);
   const NodeInfo info = infos[nodes[k]];
-  NodeGeom g = node_geom(info, P);
+  const NodeGeom g = node_geom(info, P);
   const bool avg = P.method == METHOD_POISSON_AVERAGE;
-  const bool inline_search = (w.flags & NW_MANY_NBR) != 0;
-  const double cs = 2.0 * g.spacing;
-  const double sq = g.spacing * (1.0 + 1e-9);
-  float sc[3] = {(float)P.scale[0], (float)P.scale[1], (float)P.scale[2]};
-  const float s2f = (float)g.s2;
-  unsigned long long rechecks = 0, tests = 0;
-  float maxerr = 0.f;
+  Filter F = make_filter(g, P);
   for (int64_t w0 = 0; w0 < n; w0 += kBlock) {
     const int64_t p = w0 + tid;
     const bool valid = p < n;
@@ -799,8 +984,8 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
         double lim = cd - g.spacing;
         limSq = lim * lim;
       }
-      if (!inline_search) {
-        const int cnt = L.nbrCount[p];
+      const int cnt = L.nbrCount[p];
+      if (cnt != kNbrOverflow) {
         const uint32_t* nb = L.nbr + (uint64_t)p * kMaxNbr;
         for (int i = 0; i < cnt; i++) {
           uint32_t q = nb[i];
@@ -808,31 +993,33 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
           else in_mask[(q - w0) >> 6] |= 1ull << ((q - w0) & 63);
         }
       } else {
-        uint32_t j = L.idxA[p];
-        double x = coord(L.X[j], P.scale[0], P.offset[0]), y = coord(L.Y[j], P.scale[1], P.offset[1]),
-               z = coord(L.Z[j], P.scale[2], P.offset[2]);
+        const uint32_t j = L.idxA[p];
+        const int32_t X = L.X[j], Y = L.Y[j], Z = L.Z[j];
+        const double x = coord(X, P.scale[0], P.offset[0]), y = coord(Y, P.scale[1], P.offset[1]),
+                     z = coord(Z, P.scale[2], P.offset[2]);
         int lo16[3], hi16[3];
         if (avg) range16(g, x, y, z, lo16, hi16);
-        int lx = gcell(x - sq, g.min[0], cs), hx = gcell(x + sq, g.min[0], cs);
-        int ly = gcell(y - sq, g.min[1], cs), hy = gcell(y + sq, g.min[1], cs);
-        int lz = gcell(z - sq, g.min[2], cs), hz = gcell(z + sq, g.min[2], cs);
-        for (int cz = lz; cz <= hz; cz++)
-          for (int cy = ly; cy <= hy; cy++)
-            for (int cx = lx; cx <= hx; cx++)
-              for (uint32_t q = grid_find(L, gkey(cx, cy, cz)); q != kNil; q = L.next[q]) {
-                if ((int64_t)q >= p) continue;
+        const CellBox cb = cell_box(g, x, y, z);
+        for (int cz = cb.lz; cz <= cb.hz && !rej; cz++)
+          for (int cy = cb.ly; cy <= cb.hy && !rej; cy++)
+            for (int cx = cb.lx; cx <= cb.hx && !rej; cx++) {
+              const uint32_t slot = table_find(L, gkey(cx, cy, cz));
+              if (slot == kNil) continue;
+              const uint32_t f1 = L.hend[slot];
+              for (uint32_t f = L.hstart[slot]; f < f1; f++) {
+                const uint32_t q = L.eP[f];
+                if ((int64_t)q >= p) break;
                 if ((int64_t)q < w0 && !L.acc[q]) continue;
-                uint32_t jq = L.idxA[q];
                 if (!avg) {
-                  if (L.sadd[q] < limSq) continue;
+                  if (L.eAdd[f] < limSq) continue;
                 } else {
-                  if (!in_range16(L.cell16[jq], lo16, hi16)) continue;
+                  if (!in_range16(L.eC16[f], lo16, hi16)) continue;
                 }
-                tests++;
-                if (!within_spacing(L, jq, j, P, sc, s2f, g.s2, rechecks, maxerr)) continue;
-                if ((int64_t)q < w0) rej = true;
-                else in_mask[(q - w0) >> 6] |= 1ull << ((q - w0) & 63);
+                if (!within_xyz(L.eX[f], L.eY[f], L.eZ[f], X, Y, Z, P, F)) continue;
+                if ((int64_t)q < w0) { rej = true; break; }
+                in_mask[(q - w0) >> 6] |= 1ull << ((q - w0) & 63);
               }
+            }
       }
     }
     if (!avg && tid == 0) s_minlim = 1e308;
@@ -844,13 +1031,14 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
     }
     bool decided = !valid;
     bool accepted = false;
+    if (valid && rej) { decided = true; accepted = false; }
     while (true) {
       if (!decided) {
         bool ready = true;
         for (int b = 0; b < 4; b++) if (in_mask[b] & ~s_dec[b]) ready = false;
-        if (ready) {
-          bool blocked = rej;
-          for (int b = 0; b < 4; b++) if (in_mask[b] & s_acc[b]) blocked = true;
+        bool blocked = false;
+        for (int b = 0; b < 4; b++) if (in_mask[b] & s_acc[b]) blocked = true;
+        if (blocked || ready) {
           accepted = !blocked;
           decided = true;
         }
@@ -879,11 +1067,7 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
     }
     __syncthreads();
   }
-  if (inline_search) {
-    atomicAdd(&P.counters[0], rechecks);
-    atomicAdd(&P.counters[1], tests);
-    if (maxerr > 0.f) atomicMax((unsigned int*)&P.counters[2], __float_as_uint(maxerr));
-  }
+  flush_filter(F, P);
 }
 
 // Poisson-average colour sums: every rejected candidate C adds its own colour
@@ -896,39 +1080,42 @@ __global__ __launch_bounds__(kBlock) void k_avg_colors(const int32_t* __restrict
                                                         const NodeInfo* __restrict__ infos,
                                                         NodeWork* __restrict__ work, SampleParams P) {
   const int k = blockIdx.x;
-  NodeWork w = work[k];
+  const NodeWork w = work[k];
   if (w.flags & NW_OVERFLOW) return;
   const int64_t n = w.n;
-  Layout L = make_layout(w.scratch, n, P.method);
-  NodeGeom g = node_geom(infos[nodes[k]], P);
-  const double cs = 2.0 * g.spacing, sq = g.spacing * (1.0 + 1e-9);
+  const Layout L = make_layout(w.scratch, n, P.method);
+  const NodeGeom g = node_geom(infos[nodes[k]], P);
   bool violation = false;
-  for (int64_t p = threadIdx.x; p < n; p += kBlock) {
-    uint32_t j = L.idxA[p];
-    double x = coord(L.X[j], P.scale[0], P.offset[0]), y = coord(L.Y[j], P.scale[1], P.offset[1]),
-           z = coord(L.Z[j], P.scale[2], P.offset[2]);
+  for (int64_t e = threadIdx.x; e < n; e += kBlock) {
+    const uint32_t p = L.eP[e];
+    const int32_t X = L.eX[e], Y = L.eY[e], Z = L.eZ[e];
+    const double x = coord(X, P.scale[0], P.offset[0]), y = coord(Y, P.scale[1], P.offset[1]),
+                 z = coord(Z, P.scale[2], P.offset[2]);
     int lo16[3], hi16[3];
     range16(g, x, y, z, lo16, hi16);
-    int lx = gcell(x - sq, g.min[0], cs), hx = gcell(x + sq, g.min[0], cs);
-    int ly = gcell(y - sq, g.min[1], cs), hy = gcell(y + sq, g.min[1], cs);
-    int lz = gcell(z - sq, g.min[2], cs), hz = gcell(z + sq, g.min[2], cs);
+    const CellBox cb = cell_box(g, x, y, z);
     const bool self_acc = L.acc[p];
+    const uint32_t j = L.idxA[p];
     const long long r = L.rgb[3 * j], gg = L.rgb[3 * j + 1], b = L.rgb[3 * j + 2];
-    for (int cz = lz; cz <= hz; cz++)
-      for (int cy = ly; cy <= hy; cy++)
-        for (int cx = lx; cx <= hx; cx++)
-          for (uint32_t q = grid_find(L, gkey(cx, cy, cz)); q != kNil; q = L.next[q]) {
-            if ((int64_t)q == p || !L.acc[q]) continue;
-            uint32_t jq = L.idxA[q];
-            if (!in_range16(L.cell16[jq], lo16, hi16)) continue;
-            if (!(dd64_pair(L, jq, j, P) < g.s2)) continue;
+    for (int cz = cb.lz; cz <= cb.hz; cz++)
+      for (int cy = cb.ly; cy <= cb.hy; cy++)
+        for (int cx = cb.lx; cx <= cb.hx; cx++) {
+          const uint32_t slot = table_find(L, gkey(cx, cy, cz));
+          if (slot == kNil) continue;
+          const uint32_t f1 = L.hend[slot];
+          for (uint32_t f = L.hstart[slot]; f < f1; f++) {
+            const uint32_t q = L.eP[f];
+            if (q == p || !L.acc[q]) continue;
+            if (!in_range16(L.eC16[f], lo16, hi16)) continue;
+            if (!(dd64_xyz(L.eX[f], L.eY[f], L.eZ[f], X, Y, Z, P) < g.s2)) continue;
             if (self_acc) { violation = true; continue; }
-            unsigned long long* d = (unsigned long long*)(L.sums + 8 * (uint64_t)q + (p < (int64_t)q ? 0 : 4));
+            unsigned long long* d = (unsigned long long*)(L.sums + 8 * (uint64_t)q + (p < q ? 0 : 4));
             atomicAdd(&d[0], (unsigned long long)r);
             atomicAdd(&d[1], (unsigned long long)gg);
             atomicAdd(&d[2], (unsigned long long)b);
             atomicAdd(&d[3], 1ull);
           }
+        }
   }
   if (violation) atomicOr(&work[k].flags, (uint32_t)NW_SEQ_AVERAGE);
 }
@@ -1180,7 +1367,12 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   PCG_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_exact_sort, grid, dim3(kBlock), 0, s, d_nodes, d_work, p, p.counters);
   PCG_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_neighbors, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p);
+  hipLaunchKernelGGL(k_cells, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p);
+  PCG_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, s, d_work, num_nodes, p.item_start);
+  PCG_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_neighbors, dim3(kNeighborBlocks), dim3(kBlock), 0, s, d_nodes, d_info, d_work, num_nodes,
+                     p.item_start, p);
   PCG_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_accept, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p);
   PCG_CHECK(hipGetLastError());
@@ -1196,6 +1388,15 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
 }
 
 uint64_t sample_scratch_bytes(int64_t n, int method) { return layout_bytes(n, method); }
+
+uint64_t sample_scratch_bound(int64_t candidates, int64_t max_nodes, int method) {
+  // linear bytes per candidate, measured where the hash holds exactly 2n
+  // slots (four uint32 arrays of H slots); a node's hash never exceeds
+  // max(minimum, 4n) slots, and every array adds at most 15 alignment bytes
+  const int64_t m = int64_t(1) << 20;
+  const uint64_t lin = (layout_bytes(m, method) - 64 - 16ull * (uint64_t)hash_cap(m, method)) / (uint64_t)m;
+  return (uint64_t)candidates * (lin + 64) + (uint64_t)max_nodes * (layout_bytes(1, method) + 512);
+}
 
 namespace {
 // byte ranges are arbitrary (records of any size): copy 4-byte words when

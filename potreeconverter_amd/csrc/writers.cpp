@@ -255,7 +255,7 @@ int write_index_files(const IndexResult& r, const std::string& targetDir) {
   fs::create_directories(targetDir);
   {
     std::vector<uint8_t> host(r.octreeBytes);
-    if (r.octreeBytes) PCG_CHECK(hipMemcpy(host.data(), r.d_octree, r.octreeBytes, hipMemcpyDeviceToHost));
+    if (r.octreeBytes) PCG_CHECK(stream_copy(host.data(), r.d_octree, r.octreeBytes, hipMemcpyDeviceToHost));
     std::ofstream f(targetDir + "/octree.bin", std::ios::binary);
     f.write((const char*)host.data(), (std::streamsize)host.size());
   }
