@@ -115,9 +115,26 @@ __device__ inline double stat_value(const uint8_t* rec, int kind) {
   return 0.0;
 }
 
-// K1: per-cell counts with an LDS hash that merges repeated cells before the
-// global atomics, plus attribute min/max (ordered uint64 bits) and the
-// classification histogram.  Each wave walks 64-point steps.
+// peers of this lane with the same key in the wave (match-any by key bits)
+__device__ inline uint64_t wave_match(uint32_t key, int bits, uint64_t active) {
+  uint64_t peers = active;
+  for (int b = 0; b < bits; b++) {
+    uint64_t m = __ballot((key >> b) & 1);
+    peers &= ((key >> b) & 1) ? m : ~m;
+  }
+  return peers;
+}
+
+// bytes of one wave's record staging area in LDS
+__host__ __device__ inline int record_area_bytes(int bpp) { return (64 * bpp + 32 + 15) & ~15; }
+
+// K1: per-cell counts, attribute min/max (ordered uint64 bits) and the
+// classification histogram.  Each wave walks a contiguous range of 64-point
+// steps (consecutive LAS points are spatially coherent): runs of equal
+// cells inside a step collapse to one LDS hash update, the LDS hash merges
+// repeats across the block's steps before the global atomics, and the
+// attribute statistics are spread over the lanes (lane l folds stat l % S
+// of every (64 / S)-th record of the step).
 __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ las, int64_t n,
                                                    int32_t* __restrict__ grid, uint64_t* __restrict__ stat_min,
                                                    uint64_t* __restrict__ stat_max,
@@ -127,63 +144,77 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ la
   __shared__ uint32_t s_key[kHashSize];
   __shared__ uint32_t s_cnt[kHashSize];
   __shared__ uint32_t s_hist[256];
-  __shared__ uint64_t s_smin[kWavesPerBlock][kMaxStats];
-  __shared__ uint64_t s_smax[kWavesPerBlock][kMaxStats];
-  __shared__ __attribute__((aligned(16))) uint8_t s_rec[kWavesPerBlock][64 * kMaxBppIn + 32];
+  __shared__ uint64_t s_smin[kBlock];
+  __shared__ uint64_t s_smax[kBlock];
+  extern __shared__ __attribute__((aligned(16))) uint8_t s_dyn[];  // record staging, one area per wave
   for (int i = threadIdx.x; i < kHashSize; i += kBlock) { s_key[i] = kEmpty; s_cnt[i] = 0; }
   for (int i = threadIdx.x; i < 256; i += kBlock) s_hist[i] = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int bpp = P.bpp_in;
   const uint64_t total = (uint64_t)n * bpp;
-  uint64_t my_min = ~0ull, my_max = 0ull;  // stat `lane` of this wave
-  uint8_t* rec_area = s_rec[wave];
+  uint8_t* rec_area = s_dyn + wave * record_area_bytes(bpp);
+  const int S = P.num_stats;
+  const int R = S > 0 ? 64 / S : 0;  // lanes per statistic
+  const int my_stat = (S > 0 && lane < R * S) ? lane % S : -1;
+  const int my_r0 = S > 0 ? lane / S : 0;
+  const StatOp op = my_stat >= 0 ? P.stats[my_stat] : StatOp{0, ST_NONE};
+  uint64_t my_min = ~0ull, my_max = 0ull;
   const int64_t steps_total = (n + 63) / 64;
   const int64_t wave_global = (int64_t)blockIdx.x * kWavesPerBlock + wave;
   const int64_t waves_total = (int64_t)gridDim.x * kWavesPerBlock;
-  // contiguous step ranges: consecutive points of a LAS file are spatially
-  // coherent, so the block's hash absorbs most repeats of a cell
   const int64_t per_wave = (steps_total + waves_total - 1) / waves_total;
   const int64_t step0 = wave_global * per_wave;
   const int64_t step1 = (step0 + per_wave) < steps_total ? (step0 + per_wave) : steps_total;
+  const uint64_t low_incl = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
   bool bad = false;
   for (int64_t step = step0; step < step1; step++) {
-    int64_t first = step * 64;
-    int cnt = (int)((n - first) < 64 ? (n - first) : 64);
-    uint32_t shift = wave_load_records(las, total, first, cnt, bpp, rec_area);
+    const int64_t first = step * 64;
+    const int cnt = (int)((n - first) < 64 ? (n - first) : 64);
+    const uint32_t shift = wave_load_records(las, total, first, cnt, bpp, rec_area);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    if (lane < cnt) {
-      const uint8_t* rec = rec_area + shift + lane * bpp;
+    const bool valid = lane < cnt;
+    const uint8_t* rec = rec_area + shift + lane * bpp;
+    uint32_t ekey = kEmpty;  // cell, kEmpty for lanes without an in-box point
+    if (valid) {
       bool inBox;
       int32_t q[3];
       uint32_t key = (uint32_t)cell_of_record(rec, P, inBox, q);
-      if (!inBox) bad = true;
-      else {
-        uint32_t h = (key * 2654435761u) >> 21;  // 11 bits
-        bool done = false;
-        for (int probe = 0; probe < 8 && !done; probe++) {
-          uint32_t slot = (h + probe) & (kHashSize - 1);
-          uint32_t cur = s_key[slot];
-          if (cur == kEmpty) {
-            cur = atomicCAS(&s_key[slot], kEmpty, key);
-            if (cur == kEmpty) cur = key;
-          }
-          if (cur == key) {
-            atomicAdd(&s_cnt[slot], 1u);
-            done = true;
-          }
-        }
-        if (!done) atomicAdd(&grid[key], 1);
-      }
-      if (P.class_src >= 0) atomicAdd(&s_hist[rec[P.class_src] & 31], 1u);
+      if (inBox) ekey = key; else bad = true;
     }
-    // attribute statistics: lane s folds stat s over the step's records
-    if (lane < P.num_stats) {
-      const StatOp op = P.stats[lane];
-      for (int r = 0; r < cnt; r++) {
-        const uint8_t* rec = rec_area + shift + r * bpp;
-        uint64_t b = ordered_bits(stat_value(rec + op.src, op.kind));
+    // runs of equal cells: the first lane of a run adds the run length
+    const uint32_t prev = __shfl_up(ekey, 1);
+    const bool head = lane == 0 || prev != ekey;
+    const uint64_t heads = __ballot(head);
+    if (head && ekey != kEmpty) {
+      const uint64_t after = heads & ~low_incl;
+      const int end = after ? __builtin_ctzll(after) : 64;
+      const uint32_t run = (uint32_t)(end - lane);
+      uint32_t h = (ekey * 2654435761u) >> 21;  // 11 bits
+      bool done = false;
+      for (int probe = 0; probe < 8 && !done; probe++) {
+        uint32_t slot = (h + probe) & (kHashSize - 1);
+        uint32_t cur = s_key[slot];
+        if (cur == kEmpty) {
+          cur = atomicCAS(&s_key[slot], kEmpty, ekey);
+          if (cur == kEmpty) cur = ekey;
+        }
+        if (cur == ekey) {
+          atomicAdd(&s_cnt[slot], run);
+          done = true;
+        }
+      }
+      if (!done) atomicAdd(&grid[ekey], (int32_t)run);
+    }
+    if (P.class_src >= 0) {
+      const uint32_t c = valid ? (uint32_t)(rec[P.class_src] & 31) : 32u;
+      const uint64_t peers = wave_match(c, 6, ~0ull);
+      if (valid && (peers & (low_incl >> 1)) == 0) atomicAdd(&s_hist[c], (uint32_t)__popcll(peers));
+    }
+    if (my_stat >= 0) {
+      for (int r = my_r0; r < cnt; r += R) {
+        uint64_t b = ordered_bits(stat_value(rec_area + shift + r * bpp + op.src, op.kind));
         my_min = b < my_min ? b : my_min;
         my_max = b > my_max ? b : my_max;
       }
@@ -192,7 +223,8 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ la
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   }
   if (bad) atomicOr(err, 1);
-  if (lane < kMaxStats) { s_smin[wave][lane] = my_min; s_smax[wave][lane] = my_max; }
+  s_smin[threadIdx.x] = my_min;
+  s_smax[threadIdx.x] = my_max;
   __syncthreads();
   for (int i = threadIdx.x; i < kHashSize; i += kBlock) {
     uint32_t k = s_key[i];
@@ -200,11 +232,13 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ la
   }
   for (int i = threadIdx.x; i < 256; i += kBlock)
     if (s_hist[i]) atomicAdd(&class_hist[i], (unsigned long long)s_hist[i]);
-  if (threadIdx.x < P.num_stats) {
+  if ((int)threadIdx.x < S) {
     uint64_t mn = ~0ull, mx = 0;
-    for (int w = 0; w < kWavesPerBlock; w++) {
-      mn = s_smin[w][threadIdx.x] < mn ? s_smin[w][threadIdx.x] : mn;
-      mx = s_smax[w][threadIdx.x] > mx ? s_smax[w][threadIdx.x] : mx;
+    for (int t = 0; t < kBlock; t++) {
+      const int l = t & 63;
+      if (l >= R * S || l % S != (int)threadIdx.x) continue;
+      mn = s_smin[t] < mn ? s_smin[t] : mn;
+      mx = s_smax[t] > mx ? s_smax[t] : mx;
     }
     atomicMin((unsigned long long*)&stat_min[threadIdx.x], (unsigned long long)mn);
     atomicMax((unsigned long long*)&stat_max[threadIdx.x], (unsigned long long)mx);
@@ -307,15 +341,6 @@ __global__ __launch_bounds__(kBlock) void k_tile_hist(const uint8_t* __restrict_
     for (int i = threadIdx.x; i < num_chunks; i += kBlock) M[(int64_t)i * tiles + t] = s_hist[i];
 }
 
-// peers of this lane with the same key in the wave (match-any by key bits)
-__device__ inline uint64_t wave_match(uint32_t key, int bits, uint64_t active) {
-  uint64_t peers = active;
-  for (int b = 0; b < bits; b++) {
-    uint64_t m = __ballot((key >> b) & 1);
-    peers &= ((key >> b) & 1) ? m : ~m;
-  }
-  return peers;
-}
 
 // K3b: transcode + stable scatter.  Each wave owns a contiguous quarter of
 // the tile; pass 1 counts per chunk per wave, pass 2 places each point at
@@ -448,6 +473,8 @@ __global__ void k_lasgen(LasGenParams p, const LasGenBuilding* __restrict__ bld,
   }
 }
 
+int g_bpp_in = 0;  // host copy of the record length in c_params
+
 }  // namespace
 
 int chunker_set_params(const ChunkerParams& p, hipStream_t stream) {
@@ -455,6 +482,7 @@ int chunker_set_params(const ChunkerParams& p, hipStream_t stream) {
     set_error("LAS record length above 128 bytes is not supported");
     return 1;
   }
+  g_bpp_in = p.bpp_in;
   PCG_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_params), &p, sizeof(p), 0, hipMemcpyHostToDevice, stream));
   return 0;
 }
@@ -466,7 +494,8 @@ int chunker_count(const uint8_t* d_las, int64_t n, int32_t* d_grid, uint64_t* d_
   int64_t max_blocks = (int64_t)num_cus * 8;
   if (blocks > max_blocks) blocks = max_blocks;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_count, dim3((unsigned)blocks), dim3(kBlock), 0, stream, d_las, n, d_grid, d_stat_min,
+  size_t lds = (size_t)kWavesPerBlock * record_area_bytes(g_bpp_in);
+  hipLaunchKernelGGL(k_count, dim3((unsigned)blocks), dim3(kBlock), lds, stream, d_las, n, d_grid, d_stat_min,
                      d_stat_max, d_class_hist, d_err);
   PCG_CHECK(hipGetLastError());
   return 0;

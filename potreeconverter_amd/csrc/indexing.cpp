@@ -287,7 +287,10 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     levelOffset[l] = allLevels.size();
     allLevels.insert(allLevels.end(), levels[l].begin(), levels[l].end());
   }
-  DevMem ditems;
+  DevMem ditems, pool;
+  // neighbour lists longer than a row (dense clusters): 4 entries per point
+  const uint64_t poolCap = (uint64_t)N * 4 + 1024;
+  if (pool.alloc(sizeof(uint32_t) * poolCap)) return 1;
   if (dinfo.alloc(sizeof(NodeInfo) * numNodes) || dstate.alloc(sizeof(NodeState) * numNodes) ||
       dwork.alloc(sizeof(NodeWork) * maxLevelNodes) || dlist.alloc(sizeof(int32_t) * (allLevels.size() + 1)) ||
       ditems.alloc(sizeof(uint32_t) * (maxLevelNodes + 1)) || arena.alloc(arenaCap) || scratch.alloc(scratchCap) ||
@@ -315,16 +318,38 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   sp.scratch_top = tops.as<uint64_t>() + 1;
   sp.counters = counters.as<unsigned long long>();
   sp.item_start = ditems.as<uint32_t>();
-  std::vector<uint32_t> workFlags;
+  sp.nbr_pool = pool.as<uint32_t>();
+  sp.nbr_pool_cap = poolCap;
+  sp.nbr_pool_top = (unsigned long long*)(tops.as<uint64_t>() + 2);
+  const char* envStats = getenv("PCG_LEVEL_STATS");
+  const bool levelStats = envStats && envStats[0] == '1';
   int64_t sampled = 0;
   for (int l = maxLevel; l >= 0; l--) {
     const auto& lst = levels[l];
     if (lst.empty()) continue;
     sampled += (int64_t)lst.size();
-    PCG_CHECK(hipMemsetAsync(tops.as<uint64_t>() + 1, 0, 8, s));
+    PCG_CHECK(hipMemsetAsync(tops.as<uint64_t>() + 1, 0, 16, s));  // scratch and pool tops
     if (sample_level(dlist.as<int32_t>() + levelOffset[l], (int32_t)lst.size(), dinfo.as<NodeInfo>(),
                      dstate.as<NodeState>(), dwork.as<NodeWork>(), sp, s))
       return 1;
+    if (levelStats) {
+      // diagnostic summary of the level (PCG_LEVEL_STATS=1), costs a sync
+      std::vector<NodeWork> wk(lst.size());
+      PCG_CHECK(stream_copy(wk.data(), dwork.p, sizeof(NodeWork) * wk.size(), hipMemcpyDeviceToHost));
+      int64_t cand = 0, maxn = 0, flagged[5] = {0, 0, 0, 0, 0};
+      for (const NodeWork& w : wk) {
+        cand += w.n;
+        maxn = std::max<int64_t>(maxn, w.n);
+        for (int b = 0; b < 5; b++) flagged[b] += (w.flags >> b) & 1;
+      }
+      unsigned long long hc7 = 0;
+      PCG_CHECK(stream_copy(&hc7, counters.as<unsigned long long>() + 7, 8, hipMemcpyDeviceToHost));
+      fprintf(stderr,
+              "[level %d] nodes %zu candidates %lld max %lld exact-sort %lld seq-accept %lld seq-average %lld "
+              "overflow %lld many-neighbours %lld (candidates so far %llu)\n",
+              l, lst.size(), (long long)cand, (long long)maxn, (long long)flagged[0], (long long)flagged[1],
+              (long long)flagged[2], (long long)flagged[3], (long long)flagged[4], hc7);
+    }
   }
   out.numSampled = sampled;
   PCG_CHECK(stream_copy(state.data(), dstate.p, sizeof(NodeState) * numNodes, hipMemcpyDeviceToHost));

@@ -62,9 +62,13 @@ struct SampleParams {
   uint64_t scratch_cap;
   uint64_t* scratch_top;
   uint32_t* item_start;    // num_nodes + 1 entries: work items of the level
+  uint32_t* nbr_pool;      // neighbour lists longer than a row
+  unsigned long long nbr_pool_cap;   // entries
+  unsigned long long* nbr_pool_top;  // entries used in the level (device counter)
   unsigned long long* counters;  // [0] fp64 re-checks, [1] pair tests, [2] max |dd32-dd64|/s^2 bits,
                                  // [3] exact-sort nodes, [4] sequential nodes, [5] candidates,
-                                 // [6] nodes that ran out of scratch or arena
+                                 // [6] nodes that ran out of scratch or arena,
+                                 // [7] candidates with more than 16 earlier neighbours
 };
 
 // Samples every node of `nodes` (all of one level, children already done).

@@ -31,8 +31,10 @@ constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kNil = 0xFFFFFFFFu;
 constexpr int kMaxNbr = 16;  // earlier neighbours kept per candidate
-constexpr uint8_t kNbrOverflow = 255;
+constexpr uint8_t kNbrOverflow = 255;  // list too long for the pool: search again
+constexpr uint8_t kNbrPooled = 254;    // list in the level's neighbour pool
 constexpr int kNeighborBlocks = 4096;  // grid of the level-wide neighbour search
+constexpr int kAccWords = 2048;         // LDS accepted bits per wave in k_accept (65536 candidates)
 
 __device__ inline int32_t ld_i32(const uint8_t* p) {
   return (int32_t)((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24));
@@ -922,29 +924,60 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(const int32_t* __restrict_
       range16(g, x, y, z, lo16, hi16);
     }
     const CellBox cb = cell_box(g, x, y, z);
-    int cnt = 0;
-    uint32_t* out = L.nbr + (uint64_t)p * kMaxNbr;
-    for (int cz = cb.lz; cz <= cb.hz; cz++)
-      for (int cy = cb.ly; cy <= cb.hy; cy++)
-        for (int cx = cb.lx; cx <= cb.hx; cx++) {
-          const uint32_t slot = table_find(L, gkey(cx, cy, cz));
-          if (slot == kNil) continue;
-          const uint32_t f1 = L.hend[slot];
-          for (uint32_t f = L.hstart[slot]; f < f1; f++) {
-            const uint32_t q = L.eP[f];
-            if (q >= p) break;
-            if (!avg) {
-              if (L.eAdd[f] < limSq) continue;
-            } else {
-              if (!in_range16(L.eC16[f], lo16, hi16)) continue;
+    // every earlier candidate within spacing that passes the filter, in cell
+    // order; emit(q, i) receives the i-th one
+    auto scan = [&](auto&& emit) {
+      int cnt = 0;
+      for (int cz = cb.lz; cz <= cb.hz; cz++)
+        for (int cy = cb.ly; cy <= cb.hy; cy++)
+          for (int cx = cb.lx; cx <= cb.hx; cx++) {
+            const uint32_t slot = table_find(L, gkey(cx, cy, cz));
+            if (slot == kNil) continue;
+            const uint32_t f1 = L.hend[slot];
+            for (uint32_t f = L.hstart[slot]; f < f1; f++) {
+              const uint32_t q = L.eP[f];
+              if (q >= p) break;
+              if (!avg) {
+                if (L.eAdd[f] < limSq) continue;
+              } else {
+                if (!in_range16(L.eC16[f], lo16, hi16)) continue;
+              }
+              if (!within_xyz(L.eX[f], L.eY[f], L.eZ[f], X, Y, Z, P, F)) continue;
+              emit(q, cnt);
+              cnt++;
             }
-            if (!within_xyz(L.eX[f], L.eY[f], L.eZ[f], X, Y, Z, P, F)) continue;
-            if (cnt < kMaxNbr) out[cnt] = q;
-            cnt++;
           }
-        }
-    L.nbrCount[p] = cnt <= kMaxNbr ? (uint8_t)cnt : kNbrOverflow;
-    if (cnt > kMaxNbr) atomicOr(&work[k].flags, (uint32_t)NW_MANY_NBR);
+      return cnt;
+    };
+    uint32_t* out = L.nbr + (uint64_t)p * kMaxNbr;
+    const int cnt = scan([&](uint32_t q, int i) {
+      if (i < kMaxNbr) out[i] = q;
+    });
+    if (cnt <= kMaxNbr) {
+      L.nbrCount[p] = (uint8_t)cnt;
+    } else {
+      // the full list goes to the level's pool (a second identical scan);
+      // the row keeps its offset and length
+      atomicOr(&work[k].flags, (uint32_t)NW_MANY_NBR);
+      atomicAdd(&P.counters[7], 1ull);
+      const unsigned long long off = atomicAdd(P.nbr_pool_top, (unsigned long long)cnt);
+      if (off + (unsigned long long)cnt <= P.nbr_pool_cap) {
+        uint32_t* dst = P.nbr_pool + off;
+        Filter F2 = make_filter(g, P);
+        Filter keep = F;
+        F = F2;
+        scan([&](uint32_t q, int i) {
+          if (i < cnt) dst[i] = q;
+        });
+        F = keep;
+        out[0] = (uint32_t)off;
+        out[1] = (uint32_t)(off >> 32);
+        out[2] = (uint32_t)cnt;
+        L.nbrCount[p] = kNbrPooled;
+      } else {
+        L.nbrCount[p] = kNbrOverflow;  // searched again by k_accept
+      }
+    }
     stats.rechecks += F.rechecks;
     stats.tests += F.tests;
     stats.maxerr = F.maxerr > stats.maxerr ? F.maxerr : stats.maxerr;
@@ -952,32 +985,56 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(const int32_t* __restrict_
   flush_filter(stats, P);
 }
 
-// Greedy acceptance in sorted order, 256 candidates per window: a candidate
-// is rejected by an accepted neighbour of an earlier window; inside the
-// window the dependencies are resolved in rounds.  Candidates whose
-// neighbour list overflowed search the cell table for accepted neighbours.
+// first i in [lo, hi) with v <= a[i] (a ascending), hi if none; one
+// probe per lane and step, so about log64 of the range dependent loads
+__device__ inline int64_t wave_lower_bound(const double* a, int64_t lo, int64_t hi, double v) {
+  const int lane = threadIdx.x & 63;
+  while (hi - lo > 64) {
+    const int64_t step = (hi - lo + 63) / 64;
+    const int64_t x = lo + lane * step;
+    const bool ge = x >= hi || !(a[x] < v);
+    const uint64_t m = __ballot(ge);
+    if (m & 1ull) return lo;
+    const int c = m ? __builtin_ctzll(m) : 64;
+    const int64_t nhi = lo + c * step;
+    lo = lo + (int64_t)(c - 1) * step + 1;
+    hi = nhi < hi ? nhi : hi;
+  }
+  const int64_t x = lo + lane;
+  const bool ge = x >= hi || !(a[x] < v);
+  const uint64_t m = __ballot(ge);
+  return m ? lo + __builtin_ctzll(m) : hi;
+}
+
+// Greedy acceptance in sorted order, one wavefront per node, 64 candidates
+// per window: a candidate is rejected by an accepted neighbour of an earlier
+// window; inside the window the dependencies are resolved by ballots.  The
+// accepted flags of the node live in LDS (global memory past 65536
+// candidates).
 __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
-                                                    NodeWork* __restrict__ work, SampleParams P) {
-  __shared__ unsigned long long s_dec[4], s_acc[4];
-  __shared__ int s_done;
-  __shared__ double s_minlim;
-  const int k = blockIdx.x;
+                                                    NodeWork* __restrict__ work, int32_t num_nodes, SampleParams P) {
+  __shared__ uint32_t s_bits[kWaves][kAccWords];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int k = __builtin_amdgcn_readfirstlane(blockIdx.x * kWaves + wave);
+  if (k >= num_nodes) return;
   const NodeWork w = work[k];
   if (w.flags & NW_OVERFLOW) return;
   const int64_t n = w.n;
-  const int tid = threadIdx.x;
-  Layout L = make_layout(w.scratch, n, P.method);
-  const NodeInfo info = infos[nodes[k]];
-  const NodeGeom g = node_geom(info, P);
+  const Layout L = make_layout(w.scratch, n, P.method);
+  const NodeGeom g = node_geom(infos[nodes[k]], P);
   const bool avg = P.method == METHOD_POISSON_AVERAGE;
   Filter F = make_filter(g, P);
-  for (int64_t w0 = 0; w0 < n; w0 += kBlock) {
-    const int64_t p = w0 + tid;
+  const bool lds = n <= (int64_t)kAccWords * 32;
+  uint32_t* bits = lds ? s_bits[wave] : L.keyA;
+  const int64_t words = (n + 31) / 32;
+  for (int64_t i = lane; i < words; i += 64) bits[i] = 0;
+  if (!lds) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+  for (int64_t w0 = 0; w0 < n; w0 += 64) {
+    const int64_t p = w0 + lane;
     const bool valid = p < n;
-    unsigned long long in_mask[4] = {0, 0, 0, 0};
+    uint64_t in_mask = 0;
     bool rej = false;
-    double limSq = 0.0;
-    if (tid < 4) { s_dec[tid] = 0; s_acc[tid] = 0; }
+    double limSq = 1e308;
     if (valid) {
       if (!avg) {
         double cd = __builtin_sqrt(L.sadd[p]);
@@ -985,12 +1042,22 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
         limSq = lim * lim;
       }
       const int cnt = L.nbrCount[p];
+      const uint32_t* row = L.nbr + (uint64_t)p * kMaxNbr;
       if (cnt != kNbrOverflow) {
-        const uint32_t* nb = L.nbr + (uint64_t)p * kMaxNbr;
-        for (int i = 0; i < cnt; i++) {
-          uint32_t q = nb[i];
-          if ((int64_t)q < w0) rej = rej || L.acc[q];
-          else in_mask[(q - w0) >> 6] |= 1ull << ((q - w0) & 63);
+        const uint32_t* lst = row;
+        int len = cnt;
+        if (cnt == kNbrPooled) {
+          lst = P.nbr_pool + ((uint64_t)row[0] | ((uint64_t)row[1] << 32));
+          len = (int)row[2];
+        }
+        for (int i = 0; i < len; i++) {
+          const uint32_t q = lst[i];
+          if ((int64_t)q >= w0) {
+            in_mask |= 1ull << (q - w0);
+          } else if ((bits[q >> 5] >> (q & 31)) & 1u) {
+            rej = true;
+            break;
+          }
         }
       } else {
         const uint32_t j = L.idxA[p];
@@ -1009,7 +1076,7 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
               for (uint32_t f = L.hstart[slot]; f < f1; f++) {
                 const uint32_t q = L.eP[f];
                 if ((int64_t)q >= p) break;
-                if ((int64_t)q < w0 && !L.acc[q]) continue;
+                if ((int64_t)q < w0 && !((bits[q >> 5] >> (q & 31)) & 1u)) continue;
                 if (!avg) {
                   if (L.eAdd[f] < limSq) continue;
                 } else {
@@ -1017,55 +1084,46 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
                 }
                 if (!within_xyz(L.eX[f], L.eY[f], L.eZ[f], X, Y, Z, P, F)) continue;
                 if ((int64_t)q < w0) { rej = true; break; }
-                in_mask[(q - w0) >> 6] |= 1ull << ((q - w0) & 63);
+                in_mask |= 1ull << (q - w0);
               }
             }
       }
     }
-    if (!avg && tid == 0) s_minlim = 1e308;
-    __syncthreads();
-    if (!avg && valid) {
-      // conservative bound for the 10 001-check window: the candidates with
-      // distance in [limSq, add) bound the accepted points the scan visits
-      atomicMin((unsigned long long*)&s_minlim, (unsigned long long)__double_as_longlong(limSq));
-    }
-    bool decided = !valid;
+    // in-window dependencies: the lowest undecided lane always resolves
+    bool decided = !valid || rej;
     bool accepted = false;
-    if (valid && rej) { decided = true; accepted = false; }
-    while (true) {
+    uint64_t dec = __ballot(decided), accm = 0;
+    while (dec != ~0ull) {
       if (!decided) {
-        bool ready = true;
-        for (int b = 0; b < 4; b++) if (in_mask[b] & ~s_dec[b]) ready = false;
-        bool blocked = false;
-        for (int b = 0; b < 4; b++) if (in_mask[b] & s_acc[b]) blocked = true;
-        if (blocked || ready) {
-          accepted = !blocked;
+        if (in_mask & accm) {
           decided = true;
+        } else if ((in_mask & ~dec) == 0) {
+          decided = true;
+          accepted = true;
         }
       }
-      __syncthreads();
-      if (decided && valid) {
-        atomicOr(&s_dec[tid >> 6], 1ull << (tid & 63));
-        if (accepted) atomicOr(&s_acc[tid >> 6], 1ull << (tid & 63));
-      }
-      if (tid == 0) s_done = 1;
-      __syncthreads();
-      if (!decided) s_done = 0;
-      __syncthreads();
-      if (s_done) break;
+      dec = __ballot(decided);
+      accm = __ballot(accepted);
     }
     if (valid) L.acc[p] = accepted ? 1 : 0;
-    if (!avg && tid == 0 && w0 + kBlock > 10001) {
-      double minlim = s_minlim;
-      int64_t lo = 0, hi = (w0 + kBlock < n ? w0 + kBlock : n);
-      int64_t last = hi - 1;
-      while (lo < hi) {
-        int64_t mid = (lo + hi) / 2;
-        if (L.sadd[mid] < minlim) lo = mid + 1; else hi = mid;
-      }
-      if (last - lo > 10000) atomicOr(&work[k].flags, (uint32_t)NW_SEQ_ACCEPT);
+    if (lane == 0) {
+      bits[w0 >> 5] = (uint32_t)accm;
+      if ((w0 >> 5) + 1 < words) bits[(w0 >> 5) + 1] = (uint32_t)(accm >> 32);
     }
-    __syncthreads();
+    if (!lds) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+    if (!avg && w0 + 64 > 10001) {
+      // the 10 001-check window of the reference scan may bind when more
+      // than 10 000 candidates lie in [min limSq of the window, add)
+      double ml = limSq;
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) {
+        double o = __shfl_xor(ml, d);
+        ml = o < ml ? o : ml;
+      }
+      const int64_t hi = (w0 + 64 < n ? w0 + 64 : n);
+      const int64_t lo = wave_lower_bound(L.sadd, 0, hi, ml);
+      if (hi - 1 - lo > 10000 && lane == 0) atomicOr(&work[k].flags, (uint32_t)NW_SEQ_ACCEPT);
+    }
   }
   flush_filter(F, P);
 }
@@ -1374,7 +1432,8 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   hipLaunchKernelGGL(k_neighbors, dim3(kNeighborBlocks), dim3(kBlock), 0, s, d_nodes, d_info, d_work, num_nodes,
                      p.item_start, p);
   PCG_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_accept, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p);
+  hipLaunchKernelGGL(k_accept, dim3((unsigned)((num_nodes + kWaves - 1) / kWaves)), dim3(kBlock), 0, s, d_nodes,
+                     d_info, d_work, num_nodes, p);
   PCG_CHECK(hipGetLastError());
   if (p.method == METHOD_POISSON_AVERAGE) {
     hipLaunchKernelGGL(k_avg_colors, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p);
