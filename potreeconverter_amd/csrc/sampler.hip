@@ -201,22 +201,27 @@ __device__ inline uint64_t wave_match8(uint32_t key, uint64_t active) {
 }
 
 // ------------------------------------------------------ block radix pass
+constexpr int kRadixE = 4;  // keys per thread per tile
+
 struct RadixShared {
   uint32_t hist[256];
   uint32_t base[256];
-  uint32_t wc[kWaves][256];
+  uint32_t wc[kRadixE][kWaves][256];  // per sub-tile and wave digit counts
   int skip;
 };
 
 // One stable counting pass of a block-wide LSD radix sort over the digit
-// (key >> shift) & mask (mask < 256) of n (key, value) pairs.  Returns false
-// and writes nothing when every key has the same digit.
+// (key >> shift) & mask (mask < 256) of n (key, value) pairs.  Tiles of
+// kRadixE * kBlock keys; sub-tile e of a tile precedes sub-tile e + 1.
+// Returns false and writes nothing when every key has the same digit.
 __device__ bool block_radix_pass(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                  uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n, int shift,
                                  uint32_t mask, RadixShared& sh) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   sh.hist[tid] = 0;
+  for (int e = 0; e < kRadixE; e++)
+    for (int w = 0; w < kWaves; w++) sh.wc[e][w][tid] = 0;
   __syncthreads();
   for (int64_t j = tid; j < n; j += kBlock) atomicAdd(&sh.hist[(kin[j] >> shift) & mask], 1u);
   __syncthreads();
@@ -241,27 +246,76 @@ __device__ bool block_radix_pass(const uint32_t* __restrict__ kin, const uint32_
   }
   __syncthreads();
   if (sh.skip) return false;
-  for (int64_t b0 = 0; b0 < n; b0 += kBlock) {
-    for (int w = 0; w < kWaves; w++) sh.wc[w][tid] = 0;
-    __syncthreads();
-    const int64_t j = b0 + tid;
-    const bool valid = j < n;
-    const uint32_t key = valid ? kin[j] : 0;
-    const uint32_t d = (key >> shift) & mask;
-    const uint64_t peers = wave_match8(d, __ballot(valid));
-    if (valid && (peers & lt) == 0) sh.wc[wave][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = sh.base[d] + (uint32_t)__popcll(peers & lt);
-      for (int w = 0; w < wave; w++) pos += sh.wc[w][d];
-      kout[pos] = key;
-      vout[pos] = vin[j];
+  for (int64_t b0 = 0; b0 < n; b0 += (int64_t)kRadixE * kBlock) {
+    uint32_t key[kRadixE], val[kRadixE], d[kRadixE];
+    uint64_t peers[kRadixE];
+    bool valid[kRadixE];
+#pragma unroll
+    for (int e = 0; e < kRadixE; e++) {
+      const int64_t j = b0 + e * kBlock + tid;
+      valid[e] = j < n;
+      key[e] = valid[e] ? kin[j] : 0;
+      val[e] = valid[e] ? vin[j] : 0;
+    }
+#pragma unroll
+    for (int e = 0; e < kRadixE; e++) {
+      d[e] = (key[e] >> shift) & mask;
+      peers[e] = wave_match8(d[e], __ballot(valid[e]));
+      if (valid[e] && (peers[e] & lt) == 0) sh.wc[e][wave][d[e]] = (uint32_t)__popcll(peers[e]);
     }
     __syncthreads();
-    sh.base[tid] += sh.wc[0][tid] + sh.wc[1][tid] + sh.wc[2][tid] + sh.wc[3][tid];
+#pragma unroll
+    for (int e = 0; e < kRadixE; e++) {
+      if (!valid[e]) continue;
+      uint32_t pos = sh.base[d[e]] + (uint32_t)__popcll(peers[e] & lt);
+      for (int e2 = 0; e2 < e; e2++)
+        for (int w = 0; w < kWaves; w++) pos += sh.wc[e2][w][d[e]];
+      for (int w = 0; w < wave; w++) pos += sh.wc[e][w][d[e]];
+      kout[pos] = key[e];
+      vout[pos] = val[e];
+    }
+    __syncthreads();
+    uint32_t add = 0;
+    for (int e = 0; e < kRadixE; e++)
+      for (int w = 0; w < kWaves; w++) {
+        add += sh.wc[e][w][tid];
+        sh.wc[e][w][tid] = 0;
+      }
+    sh.base[tid] += add;
     __syncthreads();
   }
   return true;
+}
+
+// Can two members of the tie group [p, e) (sorted positions of equal
+// distance) exclude each other, so that their order changes a decision or
+// a colour sum?  Groups above 64 members are taken as conflicting.
+__device__ bool tie_group_conflicts(const Layout& L, const NodeGeom& g, const SampleParams& P, int64_t p, int64_t e) {
+  if (e - p > 64) return true;
+  for (int64_t a = p; a < e; a++) {
+    for (int64_t b = a + 1; b < e; b++) {
+      uint32_t ia = L.idxA[a], ib = L.idxA[b];
+      double xa = coord(L.X[ia], P.scale[0], P.offset[0]), ya = coord(L.Y[ia], P.scale[1], P.offset[1]),
+             za = coord(L.Z[ia], P.scale[2], P.offset[2]);
+      double xb = coord(L.X[ib], P.scale[0], P.offset[0]), yb = coord(L.Y[ib], P.scale[1], P.offset[1]),
+             zb = coord(L.Z[ib], P.scale[2], P.offset[2]);
+      double dx = xa - xb, dy = ya - yb, dz = za - zb;
+      double dd = dx * dx + dy * dy + dz * dz;
+      if (!(dd < g.s2)) continue;
+      if (P.method == METHOD_POISSON) {
+        double cd = __builtin_sqrt(L.sadd[p]);
+        double lim = cd - g.spacing;
+        if (L.sadd[p] >= lim * lim) return true;
+      } else {
+        int lo[3], hi[3];
+        range16(g, xa, ya, za, lo, hi);
+        if (in_range16(L.cell16[ib], lo, hi)) return true;
+        range16(g, xb, yb, zb, lo, hi);
+        if (in_range16(L.cell16[ia], lo, hi)) return true;
+      }
+    }
+  }
+  return false;
 }
 
 // ---------------------------------------------------------------- prepare
@@ -367,34 +421,11 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
   // equal distances: the reference's unstable sort decides their order;
   // it matters only if two of them can exclude each other
   bool conflict = false;
-  for (int64_t p = tid; p + 1 < n; p += kBlock) {
+  for (int64_t p = tid; p + 1 < n && !conflict; p += kBlock) {
     if (!(L.sadd[p] == L.sadd[p + 1]) || (p > 0 && L.sadd[p - 1] == L.sadd[p])) continue;
     int64_t e = p + 1;
     while (e < n && L.sadd[e] == L.sadd[p]) e++;
-    if (e - p > 64) { conflict = true; continue; }
-    for (int64_t a = p; a < e && !conflict; a++) {
-      for (int64_t b = a + 1; b < e && !conflict; b++) {
-        uint32_t ia = L.idxA[a], ib = L.idxA[b];
-        double xa = coord(L.X[ia], P.scale[0], P.offset[0]), ya = coord(L.Y[ia], P.scale[1], P.offset[1]),
-               za = coord(L.Z[ia], P.scale[2], P.offset[2]);
-        double xb = coord(L.X[ib], P.scale[0], P.offset[0]), yb = coord(L.Y[ib], P.scale[1], P.offset[1]),
-               zb = coord(L.Z[ib], P.scale[2], P.offset[2]);
-        double dx = xa - xb, dy = ya - yb, dz = za - zb;
-        double dd = dx * dx + dy * dy + dz * dz;
-        if (!(dd < g.s2)) continue;
-        if (P.method == METHOD_POISSON) {
-          double cd = __builtin_sqrt(L.sadd[p]);
-          double lim = cd - g.spacing;
-          if (L.sadd[p] >= lim * lim) conflict = true;
-        } else {
-          int lo[3], hi[3];
-          range16(g, xa, ya, za, lo, hi);
-          if (in_range16(L.cell16[ib], lo, hi)) conflict = true;
-          range16(g, xb, yb, zb, lo, hi);
-          if (in_range16(L.cell16[ia], lo, hi)) conflict = true;
-        }
-      }
-    }
+    conflict = tie_group_conflicts(L, g, P, p, e);
   }
   if (conflict) atomicOr(&work[k].flags, (uint32_t)NW_EXACT_SORT);
 }
@@ -517,6 +548,7 @@ struct PartShared {
   unsigned long long total;
   long long fail;
   unsigned long long kc;
+  unsigned long long markL, markR;  // marked elements left / right of the cut
 };
 
 __device__ int64_t block_partition(PairArr a, uint32_t* posLe, uint32_t* posGe, int64_t f, int64_t l, PartShared& sh) {
@@ -524,7 +556,7 @@ __device__ int64_t block_partition(PairArr a, uint32_t* posLe, uint32_t* posGe, 
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   const uint64_t me = 1ull << lane;
   const double pv = a.K[f];
-  if (tid == 0) { sh.total = 0; sh.fail = INT64_MAX; sh.kc = 0; }
+  if (tid == 0) { sh.total = 0; sh.fail = INT64_MAX; sh.kc = 0; sh.markL = 0; sh.markR = 0; }
   __syncthreads();
   unsigned long long myLe = 0;
   for (int64_t i = f + 1 + tid; i < l; i += kBlock) myLe += !(pv < a.K[i]);
@@ -580,7 +612,8 @@ __device__ int64_t block_partition(PairArr a, uint32_t* posLe, uint32_t* posGe, 
   return cut;
 }
 
-__global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict__ nodes, NodeWork* __restrict__ work,
+__global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict__ nodes,
+                                                        const NodeInfo* __restrict__ infos, NodeWork* __restrict__ work,
                                                         SampleParams P, unsigned long long* counters) {
   constexpr int kSmall = 128;     // segments up to this size go to one lane each
   constexpr int kList = 1024;     // segments per level
@@ -588,24 +621,44 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
   __shared__ int32_t s_cf[kList], s_cl[kList], s_cd[kList];
   __shared__ int32_t s_nf[kList], s_nl[kList], s_nd[kList];
   __shared__ int32_t s_df[kDeferred], s_dl[kDeferred], s_dd[kDeferred];
-  __shared__ int s_nc, s_nn, s_nd_count, s_bad;
+  __shared__ int s_nc, s_nn, s_nd_count, s_bad, s_any;
   __shared__ PartShared sh;
   const int k = blockIdx.x;
   NodeWork w = work[k];
   if (!(w.flags & NW_EXACT_SORT) || (w.flags & NW_OVERFLOW)) return;
   const int64_t n = w.n;
   Layout L = make_layout(w.scratch, n, P.method);
+  const NodeGeom g = node_geom(infos[nodes[k]], P);
+  const int tid = threadIdx.x;
+  // Only the order of marked elements is needed: the members of tie groups
+  // that conflict (or of every tie group when the 10 001-check window of
+  // the Poisson scan could bind).  Segments of the replay without a marked
+  // element are not followed further.
+  uint8_t* mark = L.acc;  // by main index
+  for (int64_t i = tid; i < n; i += kBlock) mark[i] = 0;
+  if (tid == 0) s_any = 0;
+  __syncthreads();
+  const bool all_ties = P.method == METHOD_POISSON && n > 10001;
+  for (int64_t p = tid; p + 1 < n; p += kBlock) {
+    if (!(L.sadd[p] == L.sadd[p + 1]) || (p > 0 && L.sadd[p - 1] == L.sadd[p])) continue;
+    int64_t e = p + 1;
+    while (e < n && L.sadd[e] == L.sadd[p]) e++;
+    if (all_ties || tie_group_conflicts(L, g, P, p, e)) {
+      for (int64_t q = p; q < e; q++) mark[L.idxA[q]] = 1;
+      s_any = 1;
+    }
+  }
+  __syncthreads();
   PairArr a{L.sadd, L.idxB};
   uint32_t* posLe = L.keyB;
   uint32_t* posGe = L.keyA;
-  const int tid = threadIdx.x;
   for (int64_t i = tid; i < n; i += kBlock) { a.K[i] = L.add[i]; a.H[i] = (uint32_t)i; }
   if (tid == 0) {
     atomicAdd(&counters[3], 1ull);
     s_nc = 0; s_nd_count = 0; s_bad = 0;
     int64_t d = 0;
     for (int64_t m = n; m > 1; m >>= 1) d++;
-    if (n > 16) {
+    if (n > 16 && s_any) {
       if (n > kSmall) { s_cf[0] = 0; s_cl[0] = (int32_t)n; s_cd[0] = (int32_t)(2 * d); s_nc = 1; }
       else { s_df[0] = 0; s_dl[0] = (int32_t)n; s_dd[0] = (int32_t)(2 * d); s_nd_count = 1; }
     }
@@ -638,14 +691,23 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
       }
       __syncthreads();
       int64_t cut = block_partition(a, posLe, posGe, f, l, sh);
+      {
+        unsigned long long mL = 0, mR = 0;
+        for (int64_t i = f + tid; i < l; i += kBlock)
+          if (mark[a.H[i]]) { if (i < cut) mL++; else mR++; }
+        if (mL) atomicAdd(&sh.markL, mL);
+        if (mR) atomicAdd(&sh.markR, mR);
+      }
+      __syncthreads();
       if (tid == 0) {
         if (cut <= f || cut >= l) {
           s_bad = 1;  // cannot happen for a median-of-three pivot; stay bounded
         } else {
           int64_t pf[2] = {f, cut}, pl[2] = {cut, l};
+          const unsigned long long marks[2] = {sh.markL, sh.markR};
           for (int side = 0; side < 2; side++) {
             int64_t sz = pl[side] - pf[side];
-            if (sz <= 16) continue;
+            if (sz <= 16 || marks[side] == 0) continue;
             if (sz > kSmall && s_nn < kList) {
               s_nf[s_nn] = (int32_t)pf[side]; s_nl[s_nn] = (int32_t)pl[side]; s_nd[s_nn] = depth - 1; s_nn++;
             } else if (sz <= kSmall && s_nd_count < kDeferred) {
@@ -1423,7 +1485,7 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   dim3 grid((unsigned)num_nodes);
   hipLaunchKernelGGL(k_prepare, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_state, d_work, p);
   PCG_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_exact_sort, grid, dim3(kBlock), 0, s, d_nodes, d_work, p, p.counters);
+  hipLaunchKernelGGL(k_exact_sort, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p, p.counters);
   PCG_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_cells, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p);
   PCG_CHECK(hipGetLastError());
