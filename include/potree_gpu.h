@@ -27,6 +27,12 @@ int pcg_init(void);
 /* Text of the last error (thread-unsafe, like the reference's logger). */
 const char* pcg_last_error(void);
 
+/* The library keeps its device buffers (chunk records, sampling scratch,
+ * octree image, ...) between calls so that a job reuses the previous job's
+ * memory.  pcg_release frees them; the next call allocates again.  No
+ * reference counterpart (the reference converter is a one-shot process). */
+int pcg_release(void);
+
 /* Phase 1 + phase 2 from a LAS file, the drop-in for the two calls at
  * Converter/src/main.cpp:353 (chunker_countsort_laszip::doChunking) and
  * Converter/src/main.cpp:380-390 (indexer::doIndexing with

@@ -67,6 +67,13 @@ static void fill_timings(double* t, const DeviceChunks& dc, const IndexResult& r
   t[7] = (double)r.counters[4];
 }
 
+int pcg_release(void) {
+  if (ensure_device()) return 1;
+  PCG_CHECK(hipStreamSynchronize(device().stream));
+  release_workspace();
+  return 0;
+}
+
 int pcg_convert(const char* las_path, const char* outdir, const char* method, int32_t flags, double* timings) {
   if (ensure_device()) return 1;
   lasfile::Header h;

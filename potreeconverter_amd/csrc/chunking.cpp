@@ -50,8 +50,45 @@ int ensure_device() {
 }
 
 void DeviceChunks::release() {
-  if (d_records) (void)hipFree(d_records);
+  if (d_records && owned) (void)hipFree(d_records);
   d_records = nullptr;
+  owned = false;
+}
+
+namespace {
+struct WsEntry {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+WsEntry g_ws[WS_COUNT];
+}  // namespace
+
+void* workspace(int slot, size_t bytes) {
+  if (slot < 0 || slot >= WS_COUNT) {
+    set_error("bad workspace slot");
+    return nullptr;
+  }
+  WsEntry& e = g_ws[slot];
+  if (e.p && e.bytes >= bytes) return e.p;
+  if (e.p) (void)hipFree(e.p);
+  e.p = nullptr;
+  e.bytes = 0;
+  const size_t want = (bytes ? bytes : 16) + bytes / 8;
+  hipError_t err = hipMalloc(&e.p, want);
+  if (err != hipSuccess) {
+    e.p = nullptr;
+    set_error(std::string("hipMalloc of ") + std::to_string(want) + " bytes failed: " + hipGetErrorString(err));
+    return nullptr;
+  }
+  e.bytes = want;
+  return e.p;
+}
+
+void release_workspace() {
+  for (WsEntry& e : g_ws) {
+    if (e.p) (void)hipFree(e.p);
+    e = WsEntry();
+  }
 }
 
 void child_box(const Vec3& min, const Vec3& max, int index, Vec3& cmin, Vec3& cmax) {
@@ -89,9 +126,13 @@ static std::string node_id(int level, int64_t gridSize, int64_t x, int64_t y, in
   return id;
 }
 
+// a workspace buffer of the job
 struct DevBuf {
   void* p = nullptr;
-  ~DevBuf() { if (p) (void)hipFree(p); }
+  int alloc(int slot, size_t bytes) {
+    p = workspace(slot, bytes);
+    return p ? 0 : 1;
+  }
   template <class T>
   T* as() { return (T*)p; }
 };
@@ -144,10 +185,9 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
 
   const int64_t cells = (int64_t)G * G * G;
   DevBuf grid, stats, hist, err;
-  PCG_CHECK(hipMalloc(&grid.p, sizeof(int32_t) * cells));
-  PCG_CHECK(hipMalloc(&stats.p, sizeof(uint64_t) * 2 * kMaxStats));
-  PCG_CHECK(hipMalloc(&hist.p, sizeof(unsigned long long) * 256));
-  PCG_CHECK(hipMalloc(&err.p, sizeof(int32_t) * 4));
+  if (grid.alloc(WS_GRID, sizeof(int32_t) * cells) || stats.alloc(WS_STATS, sizeof(uint64_t) * 2 * kMaxStats) ||
+      hist.alloc(WS_HIST, sizeof(unsigned long long) * 256) || err.alloc(WS_ERR, sizeof(int32_t) * 4))
+    return 1;
   PCG_CHECK(hipMemsetAsync(grid.p, 0, sizeof(int32_t) * cells, s));
   PCG_CHECK(hipMemsetAsync(stats.p, 0xFF, sizeof(uint64_t) * kMaxStats, s));
   PCG_CHECK(hipMemsetAsync(stats.as<uint64_t>() + kMaxStats, 0, sizeof(uint64_t) * kMaxStats, s));
@@ -205,7 +245,7 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
   std::vector<int32_t*> lv(Lmax + 1);
   for (int l = 0; l < Lmax; l++) {
     int64_t g = (int64_t)1 << l;
-    PCG_CHECK(hipMalloc(&levels[l].p, sizeof(int32_t) * g * g * g));
+    if (levels[l].alloc(WS_LEVEL0 + l, sizeof(int32_t) * g * g * g)) return 1;
     lv[l] = levels[l].as<int32_t>();
   }
   lv[Lmax] = grid.as<int32_t>();
@@ -215,8 +255,8 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
   uint32_t numChunks = 0;
   for (int attempt = 0; attempt < 2; attempt++) {
     DevBuf list, counter;
-    PCG_CHECK(hipMalloc(&list.p, sizeof(int64_t) * 5 * maxOut));
-    PCG_CHECK(hipMalloc(&counter.p, sizeof(uint32_t)));
+    if (list.alloc(WS_CHUNK_LIST, sizeof(int64_t) * 5 * maxOut) || counter.alloc(WS_CHUNK_COUNTER, sizeof(uint32_t)))
+      return 1;
     PCG_CHECK(hipMemsetAsync(counter.p, 0, sizeof(uint32_t), s));
     if (chunker_chunk_cells(lv.data(), Lmax, list.as<int64_t>(), counter.as<uint32_t>(), maxOut, s)) return 1;
     PCG_CHECK(hipMemcpyAsync(&numChunks, counter.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -252,9 +292,9 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
     return 1;
   }
   DevBuf dchunks, lut;
-  PCG_CHECK(hipMalloc(&dchunks.p, sizeof(int64_t) * (sorted.size() + 4)));
+  if (dchunks.alloc(WS_CHUNKS, sizeof(int64_t) * (sorted.size() + 4))) return 1;
   PCG_CHECK(hipMemcpyAsync(dchunks.p, sorted.data(), sizeof(int64_t) * sorted.size(), hipMemcpyHostToDevice, s));
-  PCG_CHECK(hipMalloc(&lut.p, sizeof(int32_t) * cells));
+  if (lut.alloc(WS_LUT, sizeof(int32_t) * cells)) return 1;
   if (chunker_lut_fill(dchunks.as<int64_t>(), (int64_t)out.chunks.size(), Lmax, lut.as<int32_t>(), s)) return 1;
   levels.clear();
 
@@ -263,9 +303,12 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
   int64_t tiles = (N + tile - 1) / tile;
   int32_t C = (int32_t)out.chunks.size();
   DevBuf M, ws;
-  PCG_CHECK(hipMalloc(&M.p, sizeof(uint32_t) * std::max<int64_t>(1, tiles * C)));
-  PCG_CHECK(hipMalloc(&ws.p, scan_workspace_bytes(tiles * C)));
-  PCG_CHECK(hipMalloc(&out.d_records, (size_t)N * out.bpp + 64));
+  if (M.alloc(WS_TILE_M, sizeof(uint32_t) * std::max<int64_t>(1, tiles * C)) ||
+      ws.alloc(WS_SCAN, scan_workspace_bytes(tiles * C)))
+    return 1;
+  out.d_records = (uint8_t*)workspace(WS_RECORDS, (size_t)N * out.bpp + 64);
+  out.owned = false;
+  if (!out.d_records) return 1;
   int covered = 12;
   for (int o = 0; o < P.num_ops; o++) covered += P.ops[o].len;
   if (covered < out.bpp) PCG_CHECK(hipMemsetAsync(out.d_records, 0, (size_t)N * out.bpp, s));
@@ -340,7 +383,9 @@ int read_chunk_files(const std::string& targetDir, DeviceChunks& out) {
     std::ifstream f(dir + "/" + d.id + ".bin", std::ios::binary);
     f.read((char*)host.data() + d.offset * out.bpp, (std::streamsize)(d.count * out.bpp));
   }
-  PCG_CHECK(hipMalloc(&out.d_records, host.size() + 64));
+  out.d_records = (uint8_t*)workspace(WS_RECORDS, host.size() + 64);
+  out.owned = false;
+  if (!out.d_records) return 1;
   if (!host.empty()) PCG_CHECK(stream_copy(out.d_records, host.data(), host.size(), hipMemcpyHostToDevice));
   return 0;
 }

@@ -20,8 +20,27 @@ struct ChunkDesc {
   Vec3 min, max;       // childBoundingBoxOf chain from the cubic box
 };
 
+// Device buffers that stay allocated across calls.  The library runs one
+// job at a time on its stream, so a job reuses the memory of the previous
+// one instead of allocating and freeing tens of GB (hipMalloc / hipFree of
+// that size cost tens of milliseconds per job).  workspace() returns a
+// buffer of at least `bytes` for `slot`, grown (with 1/8 headroom) when
+// smaller, or nullptr with the error set.  The memory is held until
+// release_workspace() or the end of the process.
+enum WorkspaceSlot : int {
+  WS_GRID, WS_STATS, WS_HIST, WS_ERR, WS_CHUNK_LIST, WS_CHUNK_COUNTER, WS_CHUNKS, WS_LUT, WS_TILE_M, WS_SCAN,
+  WS_RECORDS, WS_OCT_B, WS_OCT_KEYS_A, WS_OCT_KEYS_B, WS_OCT_M, WS_OCT_SCAN, WS_OCT_SEGS, WS_OCT_TILES,
+  WS_OCT_NODES, WS_OCT_COUNTER, WS_INFO, WS_STATE, WS_WORK, WS_LIST, WS_ITEMS, WS_POOL, WS_OUTSEGS, WS_SEGSTART,
+  WS_ARENA, WS_SCRATCH, WS_TOPS, WS_COUNTERS, WS_OCTREE, WS_COPYDESC,
+  WS_LEVEL0,  // pyramid levels: WS_LEVEL0 + level
+  WS_COUNT = WS_LEVEL0 + 16
+};
+void* workspace(int slot, size_t bytes);
+void release_workspace();
+
 struct DeviceChunks {
   uint8_t* d_records = nullptr;  // numPoints * bpp bytes, grouped by chunk (ascending ID)
+  bool owned = false;            // d_records is the caller's to free (else a workspace buffer)
   int64_t numPoints = 0;
   int32_t bpp = 0;
   std::vector<ChunkDesc> chunks;

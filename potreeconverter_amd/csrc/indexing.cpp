@@ -72,18 +72,14 @@ struct Tree {
   }
 };
 
+// a workspace buffer of the job
 struct DevMem {
   void* p = nullptr;
-  size_t bytes = 0;
-  ~DevMem() { if (p) (void)hipFree(p); }
+  int slot;
+  explicit DevMem(int s) : slot(s) {}
   int alloc(size_t b) {
-    if (p && bytes >= b) return 0;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-    PCG_CHECK(hipMalloc(&p, b ? b : 16));
-    bytes = b;
-    return 0;
+    p = workspace(slot, b);
+    return p ? 0 : 1;
   }
   template <class T>
   T* as() const { return (T*)p; }
@@ -92,8 +88,7 @@ struct DevMem {
 }  // namespace
 
 void IndexResult::release() {
-  if (d_octree) (void)hipFree(d_octree);
-  d_octree = nullptr;
+  d_octree = nullptr;  // a workspace buffer
 }
 
 int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string& name, int64_t pointsTotal,
@@ -152,7 +147,8 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
 
   // ------------------------------------------------------ octree build
   const double tb0 = now_s();
-  DevMem bufB, keysA, keysB, dM, ws, dsegs, dtiles, dnodes, dcounter;
+  DevMem bufB(WS_OCT_B), keysA(WS_OCT_KEYS_A), keysB(WS_OCT_KEYS_B), dM(WS_OCT_M), ws(WS_OCT_SCAN), dsegs(WS_OCT_SEGS),
+      dtiles(WS_OCT_TILES), dnodes(WS_OCT_NODES), dcounter(WS_OCT_COUNTER);
   struct Seg { int32_t node; int64_t start, count; };
   std::vector<Seg> segs;
   for (int32_t id : chunkRoots)
@@ -275,9 +271,11 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     if (tree.nodes[i].hasChildren() && !(singleChunk && tree.nodes[i].upper)) levels[info[i].level].push_back(i);
   int64_t maxLevelNodes = 1;
   for (auto& l : levels) maxLevelNodes = std::max<int64_t>(maxLevelNodes, (int64_t)l.size());
-  DevMem dinfo, dstate, dwork, dlist, arena, scratch, tops, counters;
+  DevMem dinfo(WS_INFO), dstate(WS_STATE), dwork(WS_WORK), dlist(WS_LIST), arena(WS_ARENA), scratch(WS_SCRATCH),
+      tops(WS_TOPS), counters(WS_COUNTERS);
   const int methodId = avg ? METHOD_POISSON_AVERAGE : METHOD_POISSON;
-  uint64_t arenaCap = (uint64_t)N * bpp * (avg ? 4 : 3) + (1u << 20);
+  // every output range is rounded up to 16 bytes (9 ranges per node at most)
+  uint64_t arenaCap = (uint64_t)N * bpp * (avg ? 4 : 3) + (1u << 20) + (uint64_t)numNodes * 9 * 16;
   // every candidate of a level is a point of the chunk records
   uint64_t scratchCap = sample_scratch_bound(N, maxLevelNodes, methodId);
   // all level lists in one upload, deepest level first
@@ -287,10 +285,12 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     levelOffset[l] = allLevels.size();
     allLevels.insert(allLevels.end(), levels[l].begin(), levels[l].end());
   }
-  DevMem ditems, pool;
+  DevMem ditems(WS_ITEMS), pool(WS_POOL), doutsegs(WS_OUTSEGS), dsegstart(WS_SEGSTART);
   // neighbour lists longer than a row (dense clusters): 4 entries per point
   const uint64_t poolCap = (uint64_t)N * 4 + 1024;
-  if (pool.alloc(sizeof(uint32_t) * poolCap)) return 1;
+  if (pool.alloc(sizeof(uint32_t) * poolCap) || doutsegs.alloc(sizeof(OutSeg) * 9 * maxLevelNodes) ||
+      dsegstart.alloc(sizeof(uint32_t) * (9 * maxLevelNodes + 1)))
+    return 1;
   if (dinfo.alloc(sizeof(NodeInfo) * numNodes) || dstate.alloc(sizeof(NodeState) * numNodes) ||
       dwork.alloc(sizeof(NodeWork) * maxLevelNodes) || dlist.alloc(sizeof(int32_t) * (allLevels.size() + 1)) ||
       ditems.alloc(sizeof(uint32_t) * (maxLevelNodes + 1)) || arena.alloc(arenaCap) || scratch.alloc(scratchCap) ||
@@ -318,6 +318,8 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   sp.scratch_top = tops.as<uint64_t>() + 1;
   sp.counters = counters.as<unsigned long long>();
   sp.item_start = ditems.as<uint32_t>();
+  sp.segs = doutsegs.as<OutSeg>();
+  sp.seg_start = dsegstart.as<uint32_t>();
   sp.nbr_pool = pool.as<uint32_t>();
   sp.nbr_pool_cap = poolCap;
   sp.nbr_pool_top = (unsigned long long*)(tops.as<uint64_t>() + 2);
@@ -461,8 +463,9 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     out.completed.push_back(cn);
   }
   out.octreeBytes = offset;
-  PCG_CHECK(hipMalloc(&out.d_octree, offset + 64));
-  DevMem dcopies;
+  out.d_octree = (uint8_t*)workspace(WS_OCTREE, offset + 64);
+  if (!out.d_octree) return 1;
+  DevMem dcopies(WS_COPYDESC);
   if (dcopies.alloc(copies.size() * sizeof(CopyDesc) + 16)) return 1;
   if (!copies.empty()) {
     PCG_CHECK(hipMemcpyAsync(dcopies.p, copies.data(), copies.size() * sizeof(CopyDesc), hipMemcpyHostToDevice, s));

@@ -43,8 +43,21 @@ struct NodeWork {
   uint8_t* scratch;
   int64_t n;              // candidates
   int64_t child_start[9]; // candidate index of each child's first point
+  const uint8_t* csrc[8]; // records of each child before the node is sampled
   uint32_t flags;         // NW_*
   uint32_t pad;
+};
+
+// One output record range of a sampled node: its accepted records
+// (child == -1) or the remainder of child `child`.  `sel` lists the source
+// candidates in output order (main index for the node's range, index in the
+// child for a remainder); nullptr means every record of the child, in order.
+struct OutSeg {
+  uint8_t* dst;          // 16-byte aligned
+  const uint32_t* sel;
+  int64_t count;         // records
+  int32_t k;             // node index in the level
+  int32_t child;
 };
 
 struct SampleParams {
@@ -62,6 +75,8 @@ struct SampleParams {
   uint64_t scratch_cap;
   uint64_t* scratch_top;
   uint32_t* item_start;    // num_nodes + 1 entries: work items of the level
+  OutSeg* segs;            // 9 per node of the level
+  uint32_t* seg_start;     // 9 * num_nodes + 1 entries: copy items per segment
   uint32_t* nbr_pool;      // neighbour lists longer than a row
   unsigned long long nbr_pool_cap;   // entries
   unsigned long long* nbr_pool_top;  // entries used in the level (device counter)

@@ -1,5 +1,6 @@
-// Poisson and Poisson-average node sampling on gfx950, one workgroup per
-// node, all nodes of one octree level per launch.
+// Poisson and Poisson-average node sampling on gfx950: all nodes of one
+// octree level per launch (a workgroup or a wavefront per node, or a
+// level-wide grid over the candidates / output bytes).
 //
 // Restates Converter/include/sampler_poisson.h:14-304 and
 // sampler_poisson_average.h:14-492 for one inner node:
@@ -9,16 +10,19 @@
 //                  exact (double, index) fix-up, detection of equal
 //                  distances that can change a decision
 //   k_exact_sort   libstdc++ std::sort order for those nodes (:182-203)
+//   k_cells        cell table: candidates sorted by 2*spacing cell
+//   k_neighbors    per candidate, the earlier candidates within spacing
+//                  (FP32 filter, FP64 re-check near the threshold)
 //   k_accept       greedy acceptance (:133-180 / :184-281) in windows of
-//                  256 sorted candidates against a hash grid of 2*spacing
-//                  cells, FP32 distance filter with FP64 re-check near the
-//                  threshold; Poisson-average colour sums (:357-416) with
-//                  int64 atomics
+//                  64 sorted candidates, one wavefront per node
+//   k_avg_colors   Poisson-average colour sums (:357-416), int64 atomics
 //   k_sequential   exact scans for nodes where the windowed form could
 //                  differ (10 001-check cap, accepted points in each
 //                  other's colour range)
-//   k_partition    accepted records to the node, the rest back to the
-//                  children in their order (:247-291 / :418-482)
+//   k_route        accepted records to the node, the rest back to the
+//                  children in their order (:247-291 / :418-482): ranks,
+//                  selection lists and output ranges
+//   k_copy_out     the record bytes of every output range of the level
 #include "lstd_sort.h"
 #include "pcg_common.h"
 #include "sampler.h"
@@ -34,6 +38,7 @@ constexpr int kMaxNbr = 16;  // earlier neighbours kept per candidate
 constexpr uint8_t kNbrOverflow = 255;  // list too long for the pool: search again
 constexpr uint8_t kNbrPooled = 254;    // list in the level's neighbour pool
 constexpr int kNeighborBlocks = 4096;  // grid of the level-wide neighbour search
+constexpr int kCopyBlocks = 8192;      // grid of the level-wide record copy
 constexpr int kAccWords = 2048;         // LDS accepted bits per wave in k_accept (65536 candidates)
 
 __device__ inline int32_t ld_i32(const uint8_t* p) {
@@ -60,10 +65,9 @@ struct Layout {
   uint32_t* cell16;  // main order, packed 16^3 cell (average)
   uint16_t* rgb;     // main order, own colour (average)
   uint32_t H;
-  uint32_t *hkey, *hstart, *hend;  // cell table: 2*spacing cell -> [start, end) in cell order
-  uint32_t *hhead, *next;          // lists of the sequential forms
-  uint32_t* eP;                    // cell order -> sorted position (ascending inside a cell)
-  int32_t *eX, *eY, *eZ;           // cell order coordinates
+  uint4* htab;                     // cell table by hash slot: (2*spacing cell key, start, end, 0) in cell order
+  uint32_t *hkey, *hhead, *next;   // lists of the sequential forms
+  int4* ent;                       // cell order: (X, Y, Z, sorted position), positions ascending inside a cell
   double* eAdd;                    // cell order squared distance to the centre (Poisson)
   uint32_t* eC16;                  // cell order 16^3 cell (average)
   long long* sums;   // sorted order, 8 per candidate (average)
@@ -93,15 +97,11 @@ __host__ __device__ inline uint64_t layout_carve(uint8_t* base, int64_t n, int m
   L->acc = (uint8_t*)take(n);
   L->cell16 = (uint32_t*)take(4 * n);
   L->H = hash_cap(n, method);
+  L->htab = (uint4*)take(16 * (uint64_t)L->H);
   L->hkey = (uint32_t*)take(4 * (uint64_t)L->H);
-  L->hstart = (uint32_t*)take(4 * (uint64_t)L->H);
-  L->hend = (uint32_t*)take(4 * (uint64_t)L->H);
   L->hhead = (uint32_t*)take(4 * (uint64_t)L->H);
   L->next = (uint32_t*)take(4 * n);
-  L->eP = (uint32_t*)take(4 * n);
-  L->eX = (int32_t*)take(4 * n);
-  L->eY = (int32_t*)take(4 * n);
-  L->eZ = (int32_t*)take(4 * n);
+  L->ent = (int4*)take(16 * n);
   L->eAdd = avg ? nullptr : (double*)take(8 * n);
   L->eC16 = avg ? (uint32_t*)take(4 * n) : nullptr;
   L->sums = avg ? (long long*)take(64 * n) : nullptr;
@@ -178,17 +178,6 @@ __device__ inline int gcell(double v, double mn, double cs) {
 }
 __device__ inline uint32_t gkey(int x, int y, int z) { return (uint32_t)(x | (y << 7) | (z << 14)); }
 __device__ inline uint32_t ghash(uint32_t k, uint32_t H) { return (k * 2654435761u) & (H - 1); }
-
-__device__ inline uint32_t grid_find(const Layout& L, uint32_t key) {
-  uint32_t slot = ghash(key, L.H);
-  for (uint32_t probe = 0; probe < L.H; probe++) {
-    uint32_t k = L.hkey[slot];
-    if (k == key) return L.hhead[slot];
-    if (k == kNil) return kNil;
-    slot = (slot + 1) & (L.H - 1);
-  }
-  return kNil;
-}
 
 __device__ inline uint64_t wave_match8(uint32_t key, uint64_t active) {
   uint64_t peers = active;
@@ -343,6 +332,7 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
     NodeWork w;
     w.n = n;
     for (int c = 0; c < 9; c++) w.child_start[c] = s_cs[c];
+    for (int c = 0; c < 8; c++) w.csrc[c] = s_cptr[c];
     w.flags = 0;
     w.pad = 0;
     if (off + bytes > P.scratch_cap) {
@@ -836,7 +826,7 @@ __device__ inline void flush_filter(const Filter& F, const SampleParams& P) {
 __device__ inline uint32_t table_find(const Layout& L, uint32_t key) {
   uint32_t slot = ghash(key, L.H);
   for (uint32_t probe = 0; probe < L.H; probe++) {
-    uint32_t k = L.hkey[slot];
+    uint32_t k = L.htab[slot].x;
     if (k == key) return slot;
     if (k == kNil) return kNil;
     slot = (slot + 1) & (L.H - 1);
@@ -879,7 +869,7 @@ __global__ __launch_bounds__(kBlock) void k_cells(const int32_t* __restrict__ no
     L.keyA[p] = gkey(gcell(x, g.min[0], cs), gcell(y, g.min[1], cs), gcell(z, g.min[2], cs));
     L.idxB[p] = (uint32_t)p;
   }
-  for (uint32_t i = tid; i < L.H; i += kBlock) L.hkey[i] = kNil;
+  for (uint32_t i = tid; i < L.H; i += kBlock) L.htab[i] = make_uint4(kNil, 0u, 0u, 0u);
   __syncthreads();
   uint32_t *kc = L.keyA, *vc = L.idxB, *ko = L.keyB, *vo = L.next;
   for (int pass = 0; pass < 3; pass++) {
@@ -890,23 +880,20 @@ __global__ __launch_bounds__(kBlock) void k_cells(const int32_t* __restrict__ no
   for (int64_t e = tid; e < n; e += kBlock) {
     const uint32_t p = vc[e];
     const uint32_t j = L.idxA[p];
-    L.eP[e] = p;
-    L.eX[e] = L.X[j];
-    L.eY[e] = L.Y[j];
-    L.eZ[e] = L.Z[j];
+    L.ent[e] = make_int4(L.X[j], L.Y[j], L.Z[j], (int32_t)p);
     if (avg) L.eC16[e] = L.cell16[j];
     else L.eAdd[e] = L.sadd[p];
     const uint32_t key = kc[e];
     if (e == 0 || kc[e - 1] != key) {
       uint32_t slot = ghash(key, L.H);
-      while (atomicCAS(&L.hkey[slot], kNil, key) != kNil) slot = (slot + 1) & (L.H - 1);
-      L.hstart[slot] = (uint32_t)e;
+      while (atomicCAS(&L.htab[slot].x, kNil, key) != kNil) slot = (slot + 1) & (L.H - 1);
+      L.htab[slot].y = (uint32_t)e;
     }
   }
   __syncthreads();
   for (int64_t e = tid; e < n; e += kBlock) {
     const uint32_t key = kc[e];
-    if (e + 1 == n || kc[e + 1] != key) L.hend[table_find(L, key)] = (uint32_t)(e + 1);
+    if (e + 1 == n || kc[e + 1] != key) L.htab[table_find(L, key)].z = (uint32_t)(e + 1);
   }
 }
 
@@ -972,8 +959,9 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(const int32_t* __restrict_
     const Layout L = make_layout(w.scratch, n, P.method);
     const NodeGeom g = node_geom(infos[nodes[k]], P);
     Filter F = make_filter(g, P);
-    const uint32_t p = L.eP[e];
-    const int32_t X = L.eX[e], Y = L.eY[e], Z = L.eZ[e];
+    const int4 me = L.ent[e];
+    const uint32_t p = (uint32_t)me.w;
+    const int32_t X = me.x, Y = me.y, Z = me.z;
     const double x = coord(X, P.scale[0], P.offset[0]), y = coord(Y, P.scale[1], P.offset[1]),
                  z = coord(Z, P.scale[2], P.offset[2]);
     int lo16[3], hi16[3];
@@ -995,16 +983,17 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(const int32_t* __restrict_
           for (int cx = cb.lx; cx <= cb.hx; cx++) {
             const uint32_t slot = table_find(L, gkey(cx, cy, cz));
             if (slot == kNil) continue;
-            const uint32_t f1 = L.hend[slot];
-            for (uint32_t f = L.hstart[slot]; f < f1; f++) {
-              const uint32_t q = L.eP[f];
+            const uint4 t = L.htab[slot];
+            for (uint32_t f = t.y; f < t.z; f++) {
+              const int4 o = L.ent[f];
+              const uint32_t q = (uint32_t)o.w;
               if (q >= p) break;
+              if (!within_xyz(o.x, o.y, o.z, X, Y, Z, P, F)) continue;
               if (!avg) {
                 if (L.eAdd[f] < limSq) continue;
               } else {
                 if (!in_range16(L.eC16[f], lo16, hi16)) continue;
               }
-              if (!within_xyz(L.eX[f], L.eY[f], L.eZ[f], X, Y, Z, P, F)) continue;
               emit(q, cnt);
               cnt++;
             }
@@ -1134,9 +1123,10 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
             for (int cx = cb.lx; cx <= cb.hx && !rej; cx++) {
               const uint32_t slot = table_find(L, gkey(cx, cy, cz));
               if (slot == kNil) continue;
-              const uint32_t f1 = L.hend[slot];
-              for (uint32_t f = L.hstart[slot]; f < f1; f++) {
-                const uint32_t q = L.eP[f];
+              const uint4 t = L.htab[slot];
+              for (uint32_t f = t.y; f < t.z; f++) {
+                const int4 o = L.ent[f];
+                const uint32_t q = (uint32_t)o.w;
                 if ((int64_t)q >= p) break;
                 if ((int64_t)q < w0 && !((bits[q >> 5] >> (q & 31)) & 1u)) continue;
                 if (!avg) {
@@ -1144,7 +1134,7 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
                 } else {
                   if (!in_range16(L.eC16[f], lo16, hi16)) continue;
                 }
-                if (!within_xyz(L.eX[f], L.eY[f], L.eZ[f], X, Y, Z, P, F)) continue;
+                if (!within_xyz(o.x, o.y, o.z, X, Y, Z, P, F)) continue;
                 if ((int64_t)q < w0) { rej = true; break; }
                 in_mask |= 1ull << (q - w0);
               }
@@ -1207,8 +1197,9 @@ __global__ __launch_bounds__(kBlock) void k_avg_colors(const int32_t* __restrict
   const NodeGeom g = node_geom(infos[nodes[k]], P);
   bool violation = false;
   for (int64_t e = threadIdx.x; e < n; e += kBlock) {
-    const uint32_t p = L.eP[e];
-    const int32_t X = L.eX[e], Y = L.eY[e], Z = L.eZ[e];
+    const int4 me = L.ent[e];
+    const uint32_t p = (uint32_t)me.w;
+    const int32_t X = me.x, Y = me.y, Z = me.z;
     const double x = coord(X, P.scale[0], P.offset[0]), y = coord(Y, P.scale[1], P.offset[1]),
                  z = coord(Z, P.scale[2], P.offset[2]);
     int lo16[3], hi16[3];
@@ -1222,12 +1213,13 @@ __global__ __launch_bounds__(kBlock) void k_avg_colors(const int32_t* __restrict
         for (int cx = cb.lx; cx <= cb.hx; cx++) {
           const uint32_t slot = table_find(L, gkey(cx, cy, cz));
           if (slot == kNil) continue;
-          const uint32_t f1 = L.hend[slot];
-          for (uint32_t f = L.hstart[slot]; f < f1; f++) {
-            const uint32_t q = L.eP[f];
+          const uint4 t = L.htab[slot];
+          for (uint32_t f = t.y; f < t.z; f++) {
+            const int4 o = L.ent[f];
+            const uint32_t q = (uint32_t)o.w;
             if (q == p || !L.acc[q]) continue;
             if (!in_range16(L.eC16[f], lo16, hi16)) continue;
-            if (!(dd64_xyz(L.eX[f], L.eY[f], L.eZ[f], X, Y, Z, P) < g.s2)) continue;
+            if (!(dd64_xyz(o.x, o.y, o.z, X, Y, Z, P) < g.s2)) continue;
             if (self_acc) { violation = true; continue; }
             unsigned long long* d = (unsigned long long*)(L.sums + 8 * (uint64_t)q + (p < q ? 0 : 4));
             atomicAdd(&d[0], (unsigned long long)r);
@@ -1317,35 +1309,39 @@ __global__ void k_sequential(const int32_t* __restrict__ nodes, const NodeInfo* 
   }
 }
 
-// Writes the node's accepted records and the children's remainders and
-// updates the node states (sampler_poisson.h:247-294,
-// sampler_poisson_average.h:418-482).
-__global__ __launch_bounds__(kBlock) void k_partition(const int32_t* __restrict__ nodes,
-                                                       const NodeInfo* __restrict__ infos,
-                                                       NodeState* __restrict__ states, NodeWork* __restrict__ work,
-                                                       SampleParams P) {
+// Routes the node's records (sampler_poisson.h:247-294,
+// sampler_poisson_average.h:418-482): the accepted records, in
+// child-then-index order, become the node's points; the rejected ones stay
+// in their child in order (Poisson-average children keep every record, the
+// accepted ones with the averaged colour).  Allocates the output ranges,
+// writes the selection lists and the output segment table, and updates the
+// node states; k_copy_out moves the bytes.
+__global__ __launch_bounds__(kBlock) void k_route(const int32_t* __restrict__ nodes,
+                                                   const NodeInfo* __restrict__ infos,
+                                                   NodeState* __restrict__ states, NodeWork* __restrict__ work,
+                                                   SampleParams P) {
   __shared__ int64_t s_acc[8];
   __shared__ uint8_t* s_out[9];
-  __shared__ const uint8_t* s_src[8];
   __shared__ int s_fail;
   __shared__ uint32_t s_wsum[kWaves];
   const int k = blockIdx.x;
   const int node = nodes[k];
-  NodeWork w = work[k];
-  if (w.flags & NW_OVERFLOW) return;
+  const NodeWork& w = work[k];
+  OutSeg* segs = P.segs + 9 * (int64_t)k;
+  if (w.flags & NW_OVERFLOW) {
+    if (threadIdx.x < 9) segs[threadIdx.x].count = 0;
+    return;
+  }
   const NodeInfo info = infos[node];
   const int64_t n = w.n;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const bool avg = P.method == METHOD_POISSON_AVERAGE;
   const bool seq_avg = (w.flags & NW_SEQ_AVERAGE) != 0;
   Layout L = make_layout(w.scratch, n, P.method);
-  uint32_t* inv = L.keyB;
+  uint32_t* inv = L.keyB;  // main index -> sorted position
+  uint32_t* sel = L.next;  // [accepted (main index) | remainder of child 0 | ... ]
   for (int64_t p = tid; p < n; p += kBlock) inv[L.idxA[p]] = (uint32_t)p;
-  if (tid < 8) {
-    s_acc[tid] = 0;
-    int ch = info.children[tid];
-    s_src[tid] = ch >= 0 ? states[ch].ptr : nullptr;
-  }
+  if (tid < 8) s_acc[tid] = 0;
   __syncthreads();
   for (int c = 0; c < 8; c++) {
     unsigned long long cnt = 0;
@@ -1353,98 +1349,99 @@ __global__ __launch_bounds__(kBlock) void k_partition(const int32_t* __restrict_
     if (cnt) atomicAdd((unsigned long long*)&s_acc[c], cnt);
   }
   __syncthreads();
+  int64_t total = 0;
+  for (int c = 0; c < 8; c++) total += s_acc[c];
   if (tid == 0) {
-    int64_t total = 0;
-    for (int c = 0; c < 8; c++) total += s_acc[c];
-    uint64_t bytes = (uint64_t)total * P.bpp;
+    const int bpp = P.bpp;
+    auto seg_bytes = [&](int64_t recs) { return ((uint64_t)recs * bpp + 15) & ~15ull; };
+    uint64_t bytes = seg_bytes(total);
     for (int c = 0; c < 8; c++) {
       int64_t cnt = w.child_start[c + 1] - w.child_start[c];
       int64_t rej = cnt - s_acc[c];
-      if (info.children[c] >= 0 && rej > 0) bytes += (uint64_t)(avg ? cnt : rej) * P.bpp;
+      if (info.children[c] >= 0 && rej > 0) bytes += seg_bytes(avg ? cnt : rej);
     }
     uint64_t off = atomicAdd((unsigned long long*)P.arena_top, (unsigned long long)bytes);
     s_fail = off + bytes > P.arena_cap;
     if (s_fail) {
       atomicOr(&work[k].flags, (uint32_t)NW_OVERFLOW);
       atomicAdd(&P.counters[6], 1ull);
+      for (int q = 0; q < 9; q++) segs[q].count = 0;
     } else {
       uint8_t* o = P.arena + off;
       s_out[8] = o;
-      o += (uint64_t)total * P.bpp;
+      segs[0] = OutSeg{o, sel, total, k, -1};
+      o += seg_bytes(total);
+      int64_t selOff = total;
       for (int c = 0; c < 8; c++) {
         int64_t cnt = w.child_start[c + 1] - w.child_start[c];
         int64_t rej = cnt - s_acc[c];
         s_out[c] = nullptr;
+        segs[1 + c] = OutSeg{nullptr, nullptr, 0, k, c};
         if (info.children[c] >= 0 && rej > 0) {
           s_out[c] = o;
-          o += (uint64_t)(avg ? cnt : rej) * P.bpp;
+          segs[1 + c] = OutSeg{o, avg ? nullptr : sel + selOff, avg ? cnt : rej, k, c};
+          o += seg_bytes(avg ? cnt : rej);
+          if (!avg) selOff += rej;
         }
       }
     }
   }
   __syncthreads();
   if (s_fail) return;
-  // child accepted-prefix at each child start
-  int64_t acc_start[8];
-  {
-    int64_t r = 0;
-    for (int c = 0; c < 8; c++) { acc_start[c] = r; r += s_acc[c]; }
+  // ranks in main order: accepted -> sel[rank]; rejected (Poisson) -> the
+  // child's remainder list; Poisson-average colours of accepted records
+  __shared__ int64_t acc_start[8], rem_start[8];
+  if (tid == 0) {
+    int64_t r = 0, q = total;
+    for (int c = 0; c < 8; c++) {
+      acc_start[c] = r;
+      rem_start[c] = q;
+      r += s_acc[c];
+      int64_t cnt = w.child_start[c + 1] - w.child_start[c];
+      if (info.children[c] >= 0 && cnt - s_acc[c] > 0) q += cnt - s_acc[c];
+    }
   }
-  const int bpp = P.bpp;
-  int64_t run = 0;  // accepted before this chunk
+  __syncthreads();
+  int64_t run = 0;  // accepted before this tile
   for (int64_t b0 = 0; b0 < n; b0 += kBlock) {
-    int64_t j = b0 + tid;
-    bool valid = j < n;
-    uint32_t q = valid ? inv[j] : 0;
-    bool a = valid && L.acc[q];
-    uint64_t bal = __ballot(a);
-    uint32_t before_in_wave = (uint32_t)__popcll(bal & (lane == 0 ? 0ull : (~0ull >> (64 - lane))));
+    const int64_t j = b0 + tid;
+    const bool valid = j < n;
+    const uint32_t q = valid ? inv[j] : 0;
+    const bool a = valid && L.acc[q];
+    const uint64_t bal = __ballot(a);
+    const uint32_t before_in_wave = (uint32_t)__popcll(bal & (lane == 0 ? 0ull : (~0ull >> (64 - lane))));
     if (lane == 0) s_wsum[wave] = (uint32_t)__popcll(bal);
     __syncthreads();
     uint32_t woff = 0, tot = 0;
     for (int v = 0; v < kWaves; v++) { if (v < wave) woff += s_wsum[v]; tot += s_wsum[v]; }
     if (valid) {
       int c = 0;
-      while (c < 7 && j >= w.child_start[c + 1]) c++;
-      int64_t local = j - w.child_start[c];
-      int64_t apos = run + woff + before_in_wave;
-      const uint8_t* src = s_src[c] + local * bpp;
-      uint16_t col[3];
-      if (avg && a) {
-        long long rs[3], ws;
-        if (seq_avg) {
-          for (int ch = 0; ch < 3; ch++) rs[ch] = L.sums[8 * q + ch];
-          ws = L.sums[8 * q + 3];
-        } else {
-          for (int ch = 0; ch < 3; ch++)
-            rs[ch] = 2 * ((long long)L.rgb[3 * j + ch] + L.sums[8 * q + ch]) + L.sums[8 * q + 4 + ch];
-          ws = 2 * (1 + L.sums[8 * q + 3]) + L.sums[8 * q + 7];
-        }
-        for (int ch = 0; ch < 3; ch++) col[ch] = (uint16_t)(rs[ch] / ws);
-      }
-      auto put = [&](uint8_t* dst) {
-        for (int b = 0; b < bpp; b++) dst[b] = src[b];
-        if (avg && a) {
-          for (int ch = 0; ch < 3; ch++) {
-            dst[P.rgb_offset + 2 * ch] = (uint8_t)col[ch];
-            dst[P.rgb_offset + 2 * ch + 1] = (uint8_t)(col[ch] >> 8);
+#pragma unroll
+      for (int x = 1; x < 8; x++) c += j >= w.child_start[x] ? 1 : 0;
+      const int64_t local = j - w.child_start[c];
+      const int64_t apos = run + woff + before_in_wave;
+      if (a) {
+        sel[apos] = (uint32_t)j;
+        if (avg) {
+          long long rs[3], ws;
+          if (seq_avg) {
+            for (int ch = 0; ch < 3; ch++) rs[ch] = L.sums[8 * q + ch];
+            ws = L.sums[8 * q + 3];
+          } else {
+            for (int ch = 0; ch < 3; ch++)
+              rs[ch] = 2 * ((long long)L.rgb[3 * j + ch] + L.sums[8 * q + ch]) + L.sums[8 * q + 4 + ch];
+            ws = 2 * (1 + L.sums[8 * q + 3]) + L.sums[8 * q + 7];
           }
+          for (int ch = 0; ch < 3; ch++) L.rgb[3 * j + ch] = (uint16_t)(rs[ch] / ws);
         }
-      };
-      if (a) put(s_out[8] + apos * bpp);
-      if (avg) {
-        if (s_out[c]) put(s_out[c] + local * bpp);
-      } else if (!a) {
-        int64_t rpos = local - (apos - acc_start[c]);
-        put(s_out[c] + rpos * bpp);
+      } else if (!avg) {
+        sel[rem_start[c] + local - (apos - acc_start[c])] = (uint32_t)local;
       }
     }
     run += tot;
     __syncthreads();
   }
   if (tid == 0) {
-    int64_t total = 0;
-    for (int c = 0; c < 8; c++) total += s_acc[c];
     NodeState& me = states[node];
     me.ptr = s_out[8];
     me.count = total;
@@ -1470,6 +1467,124 @@ __global__ __launch_bounds__(kBlock) void k_partition(const int32_t* __restrict_
       }
     }
     atomicAdd(&P.counters[5], (unsigned long long)n);
+  }
+}
+
+constexpr int kCopyDwords = 4;  // output dwords per thread per copy item
+
+// Copy items of the level's output segments (kBlock * kCopyDwords dwords
+// each): seg_start[s] is the first item of segment s.
+__global__ __launch_bounds__(1024) void k_seg_plan(const OutSeg* __restrict__ segs, int32_t num_segs, int32_t bpp,
+                                                    uint32_t* __restrict__ seg_start) {
+  __shared__ uint32_t s_w[16];
+  __shared__ uint32_t s_run;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) s_run = 0;
+  __syncthreads();
+  constexpr uint64_t per_item = (uint64_t)kBlock * kCopyDwords * 4;
+  for (int b = 0; b < num_segs; b += 1024) {
+    const int q = b + tid;
+    uint32_t v = 0;
+    if (q < num_segs) v = (uint32_t)(((uint64_t)segs[q].count * bpp + per_item - 1) / per_item);
+    uint32_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      uint32_t t = __shfl_up(incl, d);
+      if (lane >= d) incl += t;
+    }
+    if (lane == 63) s_w[wave] = incl;
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+    for (int x = 0; x < 16; x++) {
+      if (x < wave) woff += s_w[x];
+      tot += s_w[x];
+    }
+    const uint32_t run = s_run;
+    if (q < num_segs) seg_start[q] = run + woff + incl - v;
+    __syncthreads();
+    if (tid == 0) s_run = run + tot;
+    __syncthreads();
+  }
+  if (tid == 0) seg_start[num_segs] = s_run;
+}
+
+// Writes the output segments of the level with aligned dword stores: each
+// dword takes its four bytes from the source record(s) it covers, the
+// averaged colour replacing RGB in accepted Poisson-average records.
+__global__ __launch_bounds__(kBlock) void k_copy_out(const OutSeg* __restrict__ segs, int32_t num_segs,
+                                                      const uint32_t* __restrict__ seg_start,
+                                                      const NodeWork* __restrict__ work, SampleParams P) {
+  const uint32_t total = seg_start[num_segs];
+  const int tid = threadIdx.x;
+  const int bpp = P.bpp;
+  const bool avg = P.method == METHOD_POISSON_AVERAGE;
+  const int rgb0 = P.rgb_offset, rgb1 = P.rgb_offset + 6;
+  for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
+    int lo = 0, hi = num_segs - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (seg_start[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    const int si = __builtin_amdgcn_readfirstlane(lo);
+    const OutSeg sg = segs[si];
+    const NodeWork& w = work[sg.k];
+    int64_t cs[9];
+    const uint8_t* csrc[8];
+#pragma unroll
+    for (int c = 0; c < 9; c++) cs[c] = w.child_start[c];
+#pragma unroll
+    for (int c = 0; c < 8; c++) csrc[c] = w.csrc[c];
+    const Layout L = make_layout(w.scratch, w.n, P.method);
+    const uint64_t seg_bytes = (uint64_t)sg.count * bpp;
+    const uint64_t dw0 = (uint64_t)(item - seg_start[si]) * (kBlock * kCopyDwords);
+    // source record of output record r: pointer and main index
+    auto resolve = [&](uint64_t r, int64_t& j) -> const uint8_t* {
+      int c;
+      int64_t local;
+      if (sg.child < 0) {
+        j = sg.sel[r];
+        c = 0;
+#pragma unroll
+        for (int q = 1; q < 8; q++) c += j >= cs[q] ? 1 : 0;
+        local = j - cs[c];
+      } else {
+        c = sg.child;
+        local = sg.sel ? (int64_t)sg.sel[r] : (int64_t)r;
+        j = cs[c] + local;
+      }
+      const uint8_t* base = csrc[0];
+#pragma unroll
+      for (int q = 1; q < 8; q++) base = c == q ? csrc[q] : base;
+      return base + local * bpp;
+    };
+#pragma unroll
+    for (int i = 0; i < kCopyDwords; i++) {
+      const uint64_t o = (dw0 + (uint64_t)i * kBlock + tid) * 4;
+      if (o >= seg_bytes) break;
+      uint64_t rr = o < 0xFFFFFFFFull ? (uint64_t)((uint32_t)o / (uint32_t)bpp) : o / (uint64_t)bpp;
+      int bb = (int)(o - rr * (uint64_t)bpp);
+      int64_t j;
+      const uint8_t* src = resolve(rr, j);
+      bool patch = avg && L.acc[L.keyB[j]];
+      uint32_t word = 0;
+      for (int t = 0; t < 4; t++) {
+        if (bb == bpp) {
+          rr++;
+          bb = 0;
+          if (rr >= (uint64_t)sg.count) break;
+          src = resolve(rr, j);
+          patch = avg && L.acc[L.keyB[j]];
+        }
+        uint32_t byte = src[bb];
+        if (patch && bb >= rgb0 && bb < rgb1) {
+          const uint16_t v = L.rgb[3 * j + (bb - rgb0) / 2];
+          byte = ((bb - rgb0) & 1) ? (v >> 8) : (v & 0xFF);
+        }
+        word |= byte << (8 * t);
+        bb++;
+      }
+      *(uint32_t*)(sg.dst + o) = word;
+    }
   }
 }
 
@@ -1503,7 +1618,12 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   }
   hipLaunchKernelGGL(k_sequential, grid, dim3(64), 0, s, d_nodes, d_info, d_work, p);
   PCG_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_partition, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_state, d_work, p);
+  hipLaunchKernelGGL(k_route, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_state, d_work, p);
+  PCG_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_seg_plan, dim3(1), dim3(1024), 0, s, p.segs, 9 * num_nodes, p.bpp, p.seg_start);
+  PCG_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_copy_out, dim3(kCopyBlocks), dim3(kBlock), 0, s, p.segs, 9 * num_nodes, p.seg_start, d_work,
+                     p);
   PCG_CHECK(hipGetLastError());
   return 0;
 }
