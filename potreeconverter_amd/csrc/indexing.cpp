@@ -338,19 +338,19 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
       // diagnostic summary of the level (PCG_LEVEL_STATS=1), costs a sync
       std::vector<NodeWork> wk(lst.size());
       PCG_CHECK(stream_copy(wk.data(), dwork.p, sizeof(NodeWork) * wk.size(), hipMemcpyDeviceToHost));
-      int64_t cand = 0, maxn = 0, flagged[5] = {0, 0, 0, 0, 0};
+      int64_t cand = 0, maxn = 0, flagged[6] = {0, 0, 0, 0, 0, 0};
       for (const NodeWork& w : wk) {
         cand += w.n;
         maxn = std::max<int64_t>(maxn, w.n);
-        for (int b = 0; b < 5; b++) flagged[b] += (w.flags >> b) & 1;
+        for (int b = 0; b < 6; b++) flagged[b] += (w.flags >> b) & 1;
       }
       unsigned long long hc7 = 0;
       PCG_CHECK(stream_copy(&hc7, counters.as<unsigned long long>() + 7, 8, hipMemcpyDeviceToHost));
       fprintf(stderr,
               "[level %d] nodes %zu candidates %lld max %lld exact-sort %lld seq-accept %lld seq-average %lld "
-              "overflow %lld many-neighbours %lld (candidates so far %llu)\n",
+              "overflow %lld many-neighbours %lld cell-path %lld (long lists so far %llu)\n",
               l, lst.size(), (long long)cand, (long long)maxn, (long long)flagged[0], (long long)flagged[1],
-              (long long)flagged[2], (long long)flagged[3], (long long)flagged[4], hc7);
+              (long long)flagged[2], (long long)flagged[3], (long long)flagged[4], (long long)flagged[5], hc7);
     }
   }
   out.numSampled = sampled;

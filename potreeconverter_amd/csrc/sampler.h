@@ -21,6 +21,8 @@ enum : uint32_t {
   NW_SEQ_AVERAGE = 4,  // two accepted points inside each other's colour range
   NW_OVERFLOW = 8,     // scratch or arena exhausted
   NW_MANY_NBR = 16,    // a candidate has more earlier neighbours than the list holds
+  NW_CELL_PATH = 32,   // decided by the cell table + neighbour lists (Poisson-average, or the
+                       // accepted shell outgrew the LDS ring of k_ring)
 };
 
 struct NodeInfo {

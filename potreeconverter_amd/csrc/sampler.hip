@@ -40,6 +40,9 @@ constexpr uint8_t kNbrPooled = 254;    // list in the level's neighbour pool
 constexpr int kNeighborBlocks = 4096;  // grid of the level-wide neighbour search
 constexpr int kCopyBlocks = 8192;      // grid of the level-wide record copy
 constexpr int kAccWords = 2048;         // LDS accepted bits per wave in k_accept (65536 candidates)
+// levels with fewer nodes than this decide acceptance on the cell-table path:
+// one wavefront per node cannot hide k_ring's latency without other nodes
+constexpr int kRingMinNodes = 512;
 
 __device__ inline int32_t ld_i32(const uint8_t* p) {
   return (int32_t)((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24));
@@ -333,7 +336,7 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
     w.n = n;
     for (int c = 0; c < 9; c++) w.child_start[c] = s_cs[c];
     for (int c = 0; c < 8; c++) w.csrc[c] = s_cptr[c];
-    w.flags = 0;
+    w.flags = (P.method == METHOD_POISSON_AVERAGE || (int)gridDim.x < kRingMinNodes) ? NW_CELL_PATH : 0;
     w.pad = 0;
     if (off + bytes > P.scratch_cap) {
       w.flags = NW_OVERFLOW;
@@ -855,7 +858,7 @@ __global__ __launch_bounds__(kBlock) void k_cells(const int32_t* __restrict__ no
   __shared__ RadixShared s_rad;
   const int k = blockIdx.x;
   const NodeWork w = work[k];
-  if (w.flags & NW_OVERFLOW) return;
+  if ((w.flags & NW_OVERFLOW) || !(w.flags & NW_CELL_PATH)) return;
   const int64_t n = w.n;
   const int tid = threadIdx.x;
   Layout L = make_layout(w.scratch, n, P.method);
@@ -909,7 +912,8 @@ __global__ __launch_bounds__(1024) void k_plan(const NodeWork* __restrict__ work
   for (int b = 0; b < num_nodes; b += 1024) {
     const int k = b + tid;
     uint32_t v = 0;
-    if (k < num_nodes && !(work[k].flags & NW_OVERFLOW)) v = (uint32_t)((work[k].n + kBlock - 1) / kBlock);
+    if (k < num_nodes && (work[k].flags & (NW_OVERFLOW | NW_CELL_PATH)) == NW_CELL_PATH)
+      v = (uint32_t)((work[k].n + kBlock - 1) / kBlock);
     uint32_t incl = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -1069,7 +1073,7 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
   const int k = __builtin_amdgcn_readfirstlane(blockIdx.x * kWaves + wave);
   if (k >= num_nodes) return;
   const NodeWork w = work[k];
-  if (w.flags & NW_OVERFLOW) return;
+  if ((w.flags & NW_OVERFLOW) || !(w.flags & NW_CELL_PATH)) return;
   const int64_t n = w.n;
   const Layout L = make_layout(w.scratch, n, P.method);
   const NodeGeom g = node_geom(infos[nodes[k]], P);
@@ -1177,6 +1181,244 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
       if (hi - 1 - lo > 10000 && lane == 0) atomicOr(&work[k].flags, (uint32_t)NW_SEQ_ACCEPT);
     }
   }
+  flush_filter(F, P);
+}
+
+// ------------------------------------------------------ accepted shell ring
+// Poisson acceptance in one pass, one wavefront per node, 64 sorted
+// candidates per window.  A candidate c can only be rejected by an accepted
+// point q with add(q) >= limSq(c) = (sqrt(add(c)) - spacing)^2
+// (sampler_poisson.h:147-170).  Candidates come in ascending distance, so
+// once the window's first candidate lies beyond spacing, accepted points
+// below that window's (cd - spacing)^2 can never reject again: the live
+// accepted points form a shell of width ~spacing that a ring buffer in LDS
+// holds, chained per 2*spacing cell (newest first).  The 10 001-check limit
+// of the reference scan cannot bind while the shell fits the ring.  A node
+// whose shell outgrows the ring is handed to the cell-table path.
+// Distances are filtered in FP32 on coordinates relative to the node
+// centre (error below 1e-4 * spacing^2); pairs within the +-1e-3 band are
+// re-checked in FP64 with the reference expression on the integer
+// coordinates.
+constexpr int kRingCap = 512;     // accepted points in the shell (power of two)
+constexpr int kRingSlots = 512;   // cell chains (direct-mapped, power of two)
+constexpr int kRingWaves = 2;     // nodes per k_ring block
+
+struct RingLds {
+  float x[kRingCap], y[kRingCap], z[kRingCap];
+  double add[kRingCap];
+  int32_t next[kRingCap];
+  int32_t pos[kRingCap];  // sorted position (FP64 re-check)
+  int32_t head[kRingSlots];
+};
+
+struct RingGeom {
+  float off, inv_cs;  // cell = floor((v + off) * inv_cs), v relative to the centre
+  float s2f, band_lo, band_hi;
+  double s2;
+};
+
+__device__ inline int ring_cell(float v, const RingGeom& G) {
+  int c = (int)__builtin_floorf((v + G.off) * G.inv_cs);
+  return c < 0 ? 0 : (c > 127 ? 127 : c);
+}
+
+__device__ inline uint32_t ring_slot(int cx, int cy, int cz) {
+  return ((uint32_t)(cx | (cy << 7) | (cz << 14)) * 2654435761u) >> (32 - 9);
+}
+
+__global__ __launch_bounds__(64 * kRingWaves) void k_ring(const int32_t* __restrict__ nodes,
+                                                           const NodeInfo* __restrict__ infos,
+                                                           NodeWork* __restrict__ work, int32_t num_nodes,
+                                                           SampleParams P) {
+  __shared__ RingLds s_ring[kRingWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int k = __builtin_amdgcn_readfirstlane(blockIdx.x * kRingWaves + wave);
+  if (k >= num_nodes) return;
+  const NodeWork& w = work[k];
+  if (w.flags & (NW_OVERFLOW | NW_CELL_PATH)) return;
+  const int64_t n = w.n;
+  const Layout L = make_layout(w.scratch, n, P.method);
+  const NodeGeom g = node_geom(infos[nodes[k]], P);
+  RingGeom G;
+  G.inv_cs = (float)(1.0 / (2.0 * g.spacing));
+  G.off = (float)(66.0 * g.spacing);
+  G.s2 = g.s2;
+  G.s2f = (float)g.s2;
+  G.band_lo = G.s2f * 0.999f;
+  G.band_hi = G.s2f * 1.001f;
+  const float sq = (float)(g.spacing * (1.0 + 1e-4));
+  unsigned long long rechecks = 0, tests = 0;
+  float maxerr = 0.f;
+  RingLds& R = s_ring[wave];
+  for (int i = lane; i < kRingSlots; i += 64) R.head[i] = -1;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  int32_t total = 0, tail = 0;  // accepted points appended / first live one (wave-uniform)
+  bool overflow = false;
+  // exact "dd < spacing^2" for a pair within the FP32 band
+  auto exact = [&](int32_t qX, int32_t qY, int32_t qZ, int32_t X, int32_t Y, int32_t Z, float d32) {
+    rechecks++;
+    const double d64 = dd64_xyz(qX, qY, qZ, X, Y, Z, P);
+    const float e = (float)(__builtin_fabs((double)d32 - d64) / G.s2);
+    maxerr = e > maxerr ? e : maxerr;
+    return d64 < G.s2;
+  };
+  int32_t X = 0, Y = 0, Z = 0;
+  double a = 0.0;
+  if (lane < n) {
+    const uint32_t j = L.idxA[lane];
+    X = L.X[j]; Y = L.Y[j]; Z = L.Z[j];
+    a = L.sadd[lane];
+  }
+  for (int64_t w0 = 0; w0 < n; w0 += 64) {
+    const int64_t p = w0 + lane;
+    const bool valid = p < n;
+    int32_t nX = 0, nY = 0, nZ = 0;
+    double na = 0.0;
+    if (p + 64 < n) {
+      const uint32_t j = L.idxA[p + 64];
+      nX = L.X[j]; nY = L.Y[j]; nZ = L.Z[j];
+      na = L.sadd[p + 64];
+    }
+    const double cd = __builtin_sqrt(a);
+    const double lim = cd - g.spacing;
+    const double limSq = lim * lim;
+    const float xf = (float)(coord(X, P.scale[0], P.offset[0]) - g.cx);
+    const float yf = (float)(coord(Y, P.scale[1], P.offset[1]) - g.cy);
+    const float zf = (float)(coord(Z, P.scale[2], P.offset[2]) - g.cz);
+    // evict accepted points that no current or later candidate can see
+    const double lim0 = __shfl(lim, 0);
+    const double thr = lim0 >= 0.0 ? lim0 * lim0 : 0.0;
+    while (tail < total) {
+      const int32_t idx = tail + lane;
+      const bool dead = idx < total && R.add[idx & (kRingCap - 1)] < thr;
+      const int d = __popcll(__ballot(dead));
+      tail += d;
+      if (d < 64) break;
+    }
+    bool rej = false;
+    uint64_t in_mask = 0;
+    if (valid) {
+      // the cells around the candidate, their chains walked side by side
+      const int lx = ring_cell(xf - sq, G), hx = ring_cell(xf + sq, G);
+      const int ly = ring_cell(yf - sq, G), hy = ring_cell(yf + sq, G);
+      const int lz = ring_cell(zf - sq, G), hz = ring_cell(zf + sq, G);
+      const int nx = hx - lx + 1, nxy = nx * (hy - ly + 1);
+      const int ncell = nxy * (hz - lz + 1);
+      for (int base = 0; base < ncell && !rej; base += 8) {
+        int32_t cur[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+          const int ci = base + t;
+          cur[t] = -1;
+          if (ci < ncell) {
+            const int cz = lz + ci / nxy, rem = ci % nxy;
+            cur[t] = R.head[ring_slot(lx + rem % nx, ly + rem / nx, cz)];
+          }
+        }
+        bool any = true;
+        while (any && !rej) {
+          any = false;
+#pragma unroll
+          for (int t = 0; t < 8; t++) {
+            const int32_t idx = cur[t];
+            if (idx < 0 || idx < tail - 64) continue;
+            any = true;
+            const int e = idx & (kRingCap - 1);
+            cur[t] = R.next[e];
+            // chains are newest first up to reordering inside one window,
+            // so nothing live follows an entry older than tail - 64
+            if (idx < tail || R.add[e] < limSq) continue;
+            tests++;
+            const float dx = R.x[e] - xf, dy = R.y[e] - yf, dz = R.z[e] - zf;
+            const float d32 = dx * dx + dy * dy + dz * dz;
+            if (d32 > G.band_hi) continue;
+            bool within = d32 < G.band_lo;
+            if (!within) {
+              const uint32_t jq = L.idxA[R.pos[e]];
+              within = exact(L.X[jq], L.Y[jq], L.Z[jq], X, Y, Z, d32);
+            }
+            if (within) rej = true;
+          }
+        }
+      }
+    }
+    // earlier candidates of the window, broadcast from their lanes: a
+    // branch-free pass sorts the pairs into clearly within spacing and
+    // inside the FP32 band; the band pairs get the FP64 check afterwards
+    uint64_t m_band = 0;
+#pragma unroll 8
+    for (int q = 0; q < 63; q++) {
+      const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xf), q));
+      const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(yf), q));
+      const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(zf), q));
+      const double qa = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(a), q),
+                                         __builtin_amdgcn_readlane(__double2loint(a), q));
+      const float dx = qx - xf, dy = qy - yf, dz = qz - zf;
+      const float d32 = dx * dx + dy * dy + dz * dz;
+      const bool cand = lane > q && !(qa < limSq);
+      const uint64_t bit = 1ull << q;
+      in_mask |= (cand && d32 < G.band_lo) ? bit : 0ull;
+      m_band |= (cand && !(d32 < G.band_lo) && !(d32 > G.band_hi)) ? bit : 0ull;
+    }
+    if (!valid || rej) { in_mask = 0; m_band = 0; }
+    tests += (uint64_t)lane;
+    uint64_t band_any = 0;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      const uint64_t o = (uint64_t)__shfl_xor((unsigned long long)(m_band | band_any), d);
+      band_any |= o | m_band;
+    }
+    while (band_any) {
+      const int q = __builtin_ctzll(band_any);
+      band_any &= band_any - 1;
+      const int32_t qX = __builtin_amdgcn_readlane(X, q), qY = __builtin_amdgcn_readlane(Y, q),
+                    qZ = __builtin_amdgcn_readlane(Z, q);
+      const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xf), q));
+      const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(yf), q));
+      const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(zf), q));
+      if ((m_band >> q) & 1ull) {
+        const float dx = qx - xf, dy = qy - yf, dz = qz - zf;
+        if (exact(qX, qY, qZ, X, Y, Z, dx * dx + dy * dy + dz * dz)) in_mask |= 1ull << q;
+      }
+    }
+    bool decided = !valid || rej;
+    bool accepted = false;
+    uint64_t dec = __ballot(decided), accm = 0;
+    while (dec != ~0ull) {
+      if (!decided) {
+        if (in_mask & accm) {
+          decided = true;
+        } else if ((in_mask & ~dec) == 0) {
+          decided = true;
+          accepted = true;
+        }
+      }
+      dec = __ballot(decided);
+      accm = __ballot(accepted);
+    }
+    if (valid) L.acc[p] = accepted ? 1 : 0;
+    // append the window's accepted points; entries back to tail - 64 must
+    // stay intact for the chain walks
+    const int32_t kacc = __popcll(accm);
+    const int32_t keep = tail - 64 > 0 ? tail - 64 : 0;
+    if (total + kacc - keep > kRingCap) {
+      overflow = true;
+      break;
+    }
+    if (accepted) {
+      const int32_t idx = total + __popcll(accm & lt);
+      const int e = idx & (kRingCap - 1);
+      R.x[e] = xf; R.y[e] = yf; R.z[e] = zf; R.add[e] = a; R.pos[e] = (int32_t)p;
+      R.next[e] = atomicExch(&R.head[ring_slot(ring_cell(xf, G), ring_cell(yf, G), ring_cell(zf, G))], idx);
+    }
+    total += kacc;
+    X = nX; Y = nY; Z = nZ; a = na;
+  }
+  if (overflow && lane == 0) atomicOr(&work[k].flags, (uint32_t)NW_CELL_PATH);
+  Filter F;
+  F.rechecks = rechecks;
+  F.tests = tests;
+  F.maxerr = maxerr;
   flush_filter(F, P);
 }
 
@@ -1602,6 +1844,11 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   PCG_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_exact_sort, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p, p.counters);
   PCG_CHECK(hipGetLastError());
+  if (p.method == METHOD_POISSON && num_nodes >= kRingMinNodes) {
+    hipLaunchKernelGGL(k_ring, dim3((unsigned)((num_nodes + kRingWaves - 1) / kRingWaves)), dim3(64 * kRingWaves),
+                       0, s, d_nodes, d_info, d_work, num_nodes, p);
+    PCG_CHECK(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_cells, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p);
   PCG_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, s, d_work, num_nodes, p.item_start);

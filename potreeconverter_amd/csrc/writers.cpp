@@ -152,7 +152,7 @@ std::vector<uint8_t> serialize_batch(HBatch& b, int64_t bytesWritten) {
 
 }  // namespace
 
-std::vector<uint8_t> build_hierarchy_bin(const std::vector<CompletedNode>& recs, int64_t& firstChunkSize) {
+std::vector<uint8_t> build_hierarchy_bin_maps(const std::vector<CompletedNode>& recs, int64_t& firstChunkSize) {
   std::map<std::string, std::vector<const CompletedNode*>> groups;
   for (const auto& r : recs) {
     std::string key = r.name.size() <= (size_t)kStep + 1 ? std::string("r") : r.name.substr(0, kStep + 1);
@@ -184,6 +184,179 @@ std::vector<uint8_t> build_hierarchy_bin(const std::vector<CompletedNode>& recs,
   auto rb = serialize_batch(*root, 0);
   memcpy(file.data(), rb.data(), rb.size());
   firstChunkSize = root->byteSize;
+  return file;
+}
+
+// The same layout from integer node keys.  A name "r" + d digits becomes
+// the marker key (1 << 3d) | digits (base 8), which orders breadth-first
+// (shorter names first, then by name); one sort orders every record the way
+// the batches, chunks and breadth-first node lists above do.  As in the
+// record loop above (HierarchyBuilder.h:148-162) a node's type is reset to
+// LEAF at its own record and turned NORMAL by children recorded after it, so
+// the type depends on the record order while the child mask does not.
+// Byte-identical to build_hierarchy_bin_maps; names deeper than 20 levels
+// take that path.
+namespace {
+inline uint64_t marker_key(const std::string& nm) {
+  uint64_t k = 1;
+  for (size_t c = 1; c < nm.size(); c++) k = (k << 3) | (uint64_t)(nm[c] - '0');
+  return k;
+}
+inline int key_depth(uint64_t k) { return (63 - __builtin_clzll(k)) / 3; }
+inline uint64_t key_prefix(uint64_t k, int depth) { return k >> (3 * (key_depth(k) - depth)); }
+
+struct HEntry {
+  uint64_t batch, chunk, node;  // marker keys
+  int32_t idx;                  // record index
+};
+
+void put_record(uint8_t* d, uint8_t type, uint8_t mask, uint32_t np, uint64_t offset, uint64_t size) {
+  d[0] = type;
+  d[1] = mask;
+  memcpy(d + 2, &np, 4);
+  memcpy(d + 6, &offset, 8);
+  memcpy(d + 14, &size, 8);
+}
+
+// open-addressing map marker key -> record index
+struct KeyIndex {
+  std::vector<uint64_t> keys;
+  std::vector<int32_t> vals;
+  uint64_t mask = 0;
+  explicit KeyIndex(size_t n) {
+    size_t cap = 16;
+    while (cap < 2 * n) cap <<= 1;
+    keys.assign(cap, 0);
+    vals.assign(cap, -1);
+    mask = cap - 1;
+  }
+  static uint64_t h(uint64_t k) { return (k * 0x9E3779B97F4A7C15ull) >> 17; }
+  void put(uint64_t k, int32_t v) {
+    uint64_t s = h(k) & mask;
+    while (keys[s] != 0 && keys[s] != k) s = (s + 1) & mask;
+    keys[s] = k;
+    vals[s] = v;
+  }
+  int32_t get(uint64_t k) const {
+    uint64_t s = h(k) & mask;
+    while (keys[s] != 0) {
+      if (keys[s] == k) return vals[s];
+      s = (s + 1) & mask;
+    }
+    return -1;
+  }
+};
+}  // namespace
+
+std::vector<uint8_t> build_hierarchy_bin(const std::vector<CompletedNode>& recs, int64_t& firstChunkSize) {
+  const int32_t n = (int32_t)recs.size();
+  for (const auto& r : recs)
+    if (r.name.size() > 21 || r.name.empty()) return build_hierarchy_bin_maps(recs, firstChunkSize);
+  std::vector<uint64_t> key(n);
+  KeyIndex index((size_t)n);
+  for (int32_t i = 0; i < n; i++) {
+    key[i] = marker_key(recs[i].name);
+    index.put(key[i], i);
+  }
+  // child masks; NORMAL needs a child recorded after the node
+  std::vector<uint8_t> mask(n, 0), later(n, 0);
+  for (int32_t i = 0; i < n; i++) {
+    if (key[i] == 1) continue;
+    const int32_t p = index.get(key[i] >> 3);
+    if (p < 0) continue;
+    mask[p] |= (uint8_t)(1u << (uint32_t)(key[i] & 7));
+    if (i > p) later[p] = 1;
+  }
+  auto plain_type = [&](int32_t i) -> uint8_t { return (mask[i] && later[i]) ? T_NORMAL : T_LEAF; };
+  // root batch: levels 0-4 breadth-first, one chunk "r"
+  std::vector<int32_t> rootNodes;
+  for (int32_t i = 0; i < n; i++)
+    if (key_depth(key[i]) <= kStep) rootNodes.push_back(i);
+  std::sort(rootNodes.begin(), rootNodes.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  // sub-batches: every node of level >= 4 under its level-4 prefix, in the
+  // chunk of its 4-level group (the batch root in its own chunk); chunk roots
+  // below the batch root are also in their own chunk
+  std::vector<HEntry> ent;
+  ent.reserve((size_t)n + n / 4);
+  for (int32_t i = 0; i < n; i++) {
+    const int d = key_depth(key[i]);
+    if (d < kStep) continue;
+    const uint64_t batch = key_prefix(key[i], kStep);
+    const int cd = d == kStep ? kStep : kStep * ((d - 1) / kStep);
+    ent.push_back(HEntry{batch, key_prefix(key[i], cd), key[i], i});
+    if (d > kStep && d % kStep == 0) ent.push_back(HEntry{batch, key[i], key[i], i});
+  }
+  std::sort(ent.begin(), ent.end(), [](const HEntry& a, const HEntry& b) {
+    if (a.batch != b.batch) return a.batch < b.batch;
+    if (a.chunk != b.chunk) return a.chunk < b.chunk;
+    return a.node < b.node;
+  });
+  std::vector<uint8_t> file(22 * rootNodes.size() + 22 * ent.size(), 0);
+  int64_t written = 22 * (int64_t)rootNodes.size();
+  struct SubBatch { int64_t start = -1, firstChunk = 0, entries = 0; };
+  std::vector<SubBatch> sub(n);
+  std::vector<size_t> cstart;
+  std::vector<int64_t> coff;
+  size_t e0 = 0;
+  while (e0 < ent.size()) {
+    size_t e1 = e0;
+    while (e1 < ent.size() && ent[e1].batch == ent[e0].batch) e1++;
+    cstart.clear();
+    for (size_t e = e0; e < e1; e++)
+      if (e == e0 || ent[e].chunk != ent[e - 1].chunk) cstart.push_back(e);
+    cstart.push_back(e1);
+    const size_t nchunks = cstart.size() - 1;
+    coff.assign(nchunks, 0);
+    for (size_t c = 1; c < nchunks; c++) coff[c] = coff[c - 1] + 22 * (int64_t)(cstart[c] - cstart[c - 1]);
+    auto chunk_of = [&](uint64_t k) -> size_t {
+      size_t lo = 0, hi = nchunks;
+      while (lo < hi) {
+        size_t mid = (lo + hi) / 2;
+        if (ent[cstart[mid]].chunk < k) lo = mid + 1; else hi = mid;
+      }
+      return lo;
+    };
+    for (size_t e = e0; e < e1; e++) {
+      const HEntry& x = ent[e];
+      const CompletedNode& r = recs[x.idx];
+      const bool chunkRoot = key_depth(x.node) % kStep == 0;  // every chunk key of a sub-batch
+      const bool isProxy = chunkRoot && x.chunk != x.node;
+      uint8_t type = plain_type(x.idx);
+      if (chunkRoot) type = isProxy ? T_PROXY : T_NORMAL;
+      uint64_t offset = (uint64_t)r.byteOffset, size = (uint64_t)(int64_t)(int32_t)r.byteSize;
+      if (isProxy) {
+        const size_t own = chunk_of(x.node);
+        offset = (uint64_t)written + (uint64_t)coff[own];
+        size = 22ull * (uint64_t)(cstart[own + 1] - cstart[own]);
+      }
+      put_record(file.data() + written + 22 * (e - e0), type, mask[x.idx], (uint32_t)(int32_t)(uint32_t)r.numPoints,
+                 offset, size);
+    }
+    SubBatch& sb = sub[ent[cstart[0]].idx];  // the batch root leads the first chunk
+    sb.start = written;
+    sb.firstChunk = 22 * (int64_t)(cstart[1] - cstart[0]);
+    sb.entries = (int64_t)(e1 - e0);
+    written += 22 * (int64_t)(e1 - e0);
+    e0 = e1;
+  }
+  for (size_t q = 0; q < rootNodes.size(); q++) {
+    const int32_t i = rootNodes[q];
+    const CompletedNode& r = recs[i];
+    uint8_t type = plain_type(i), m = mask[i];
+    uint64_t offset = (uint64_t)r.byteOffset, size = (uint64_t)(int64_t)(int32_t)r.byteSize;
+    if (key[i] == 1) type = T_NORMAL;  // the chunk root "r" in its own chunk
+    if (key_depth(key[i]) == kStep) {
+      m = 0;  // its children are in its sub-batch
+      type = T_LEAF;
+      if (sub[i].start >= 0 && sub[i].entries > 1) {
+        type = T_PROXY;
+        offset = (uint64_t)sub[i].start;
+        size = (uint64_t)sub[i].firstChunk;
+      }
+    }
+    put_record(file.data() + 22 * q, type, m, (uint32_t)(int32_t)(uint32_t)r.numPoints, offset, size);
+  }
+  firstChunkSize = 22 * (int64_t)rootNodes.size();
   return file;
 }
 
