@@ -55,6 +55,81 @@ __device__ inline int32_t ld_i32(const uint8_t* p) {
   return (int32_t)((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24));
 }
 
+// 4 bytes at any address inside an LDS staging area (two aligned dword
+// reads and a byte align); the area is padded so the second read stays in it
+__device__ inline uint32_t lds_u32(const uint8_t* p) {
+  const uintptr_t a = (uintptr_t)p;
+  const uint32_t* w = (const uint32_t*)(a & ~(uintptr_t)3);
+  return __builtin_amdgcn_alignbyte(w[1], w[0], (uint32_t)(a & 3));
+}
+__device__ inline uint64_t lds_u64(const uint8_t* p) {
+  const uintptr_t a = (uintptr_t)p;
+  const uint32_t* w = (const uint32_t*)(a & ~(uintptr_t)3);
+  const uint32_t s = (uint32_t)(a & 3);
+  return (uint64_t)__builtin_amdgcn_alignbyte(w[1], w[0], s) |
+         ((uint64_t)__builtin_amdgcn_alignbyte(w[2], w[1], s) << 32);
+}
+
+// Attribute statistics as order-preserving integer keys: unsigned values as
+// they are, signed ones with the sign bit flipped, floating point through
+// ordered_bits.  The key order equals the order of the values converted to
+// double, so min/max over keys converted once give the double min/max.
+struct StatLane {
+  int src;
+  uint32_t shift;
+  uint64_t mask;
+  int sign_bits;  // width of a signed field, 0 unsigned
+  int fp;         // 0 integer, 4 float, 8 double
+};
+
+__device__ inline StatLane stat_lane(const StatOp& op) {
+  StatLane s{op.src, 0, 0xFFull, 0, 0};
+  switch (op.kind) {
+    case ST_U8: break;
+    case ST_I8: s.sign_bits = 8; break;
+    case ST_U16: s.mask = 0xFFFFull; break;
+    case ST_I16: s.mask = 0xFFFFull; s.sign_bits = 16; break;
+    case ST_U32: s.mask = 0xFFFFFFFFull; break;
+    case ST_I32: s.mask = 0xFFFFFFFFull; s.sign_bits = 32; break;
+    case ST_U64: s.mask = ~0ull; break;
+    case ST_I64: s.mask = ~0ull; s.sign_bits = 64; break;
+    case ST_F32: s.mask = 0xFFFFFFFFull; s.fp = 4; break;
+    case ST_F64: s.mask = ~0ull; s.fp = 8; break;
+    case ST_LOW3: s.mask = 7; break;
+    case ST_MID3: s.mask = 7; s.shift = 3; break;
+    case ST_LOW5: s.mask = 31; break;
+    default: s.mask = 0; break;
+  }
+  return s;
+}
+
+__device__ inline uint64_t stat_key(const StatLane& s, uint64_t raw) {
+  uint64_t v = (raw >> s.shift) & s.mask;
+  if (s.fp == 4) {
+    float f;
+    uint32_t b = (uint32_t)v;
+    __builtin_memcpy(&f, &b, 4);
+    return ordered_bits((double)f);
+  }
+  if (s.fp == 8) {
+    double d;
+    __builtin_memcpy(&d, &v, 8);
+    return ordered_bits(d);
+  }
+  if (s.sign_bits) {
+    const int up = 64 - s.sign_bits;
+    v = (uint64_t)(((int64_t)(v << up)) >> up) ^ 0x8000000000000000ull;
+  }
+  return v;
+}
+
+// a key back to the ordered bits of the double value
+__device__ inline uint64_t stat_key_bits(const StatLane& s, uint64_t key) {
+  if (s.fp) return key;
+  if (s.sign_bits) return ordered_bits((double)(int64_t)(key ^ 0x8000000000000000ull));
+  return ordered_bits((double)key);
+}
+
 // laszip coordinate (scale * X + offset), requantization and the cell of
 // one point: chunker_countsort_laszip.cpp:224-257
 __device__ inline int64_t cell_of_record(const uint8_t* rec, const ChunkerParams& P, bool& inBox, int32_t q[3]) {
@@ -65,7 +140,7 @@ __device__ inline int64_t cell_of_record(const uint8_t* rec, const ChunkerParams
   int64_t mul = 1;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    int32_t X = ld_i32(rec + 4 * a);
+    int32_t X = (int32_t)lds_u32(rec + 4 * a);
     double x = P.las_scale[a] * X + P.las_offset[a];
     int32_t Xq = (int32_t)((x - P.pos_offset[a]) / P.pos_scale[a]);
     double u = ((double)Xq * P.pos_scale[a] + P.pos_offset[a] - P.min[a]) / P.cube;
@@ -80,40 +155,6 @@ __device__ inline int64_t cell_of_record(const uint8_t* rec, const ChunkerParams
   return idx;
 }
 
-__device__ inline double stat_value(const uint8_t* rec, int kind) {
-  switch (kind) {
-    case ST_U8: return (double)rec[0];
-    case ST_I8: return (double)(int8_t)rec[0];
-    case ST_U16: return (double)(uint16_t)(rec[0] | (rec[1] << 8));
-    case ST_I16: return (double)(int16_t)(rec[0] | (rec[1] << 8));
-    case ST_U32: return (double)(uint32_t)ld_i32(rec);
-    case ST_I32: return (double)ld_i32(rec);
-    case ST_U64: {
-      uint64_t v = (uint64_t)(uint32_t)ld_i32(rec) | ((uint64_t)(uint32_t)ld_i32(rec + 4) << 32);
-      return (double)v;
-    }
-    case ST_I64: {
-      uint64_t v = (uint64_t)(uint32_t)ld_i32(rec) | ((uint64_t)(uint32_t)ld_i32(rec + 4) << 32);
-      return (double)(int64_t)v;
-    }
-    case ST_F32: {
-      uint32_t b = (uint32_t)ld_i32(rec);
-      float f;
-      __builtin_memcpy(&f, &b, 4);
-      return (double)f;
-    }
-    case ST_F64: {
-      uint64_t v = (uint64_t)(uint32_t)ld_i32(rec) | ((uint64_t)(uint32_t)ld_i32(rec + 4) << 32);
-      double d;
-      __builtin_memcpy(&d, &v, 8);
-      return d;
-    }
-    case ST_LOW3: return (double)(rec[0] & 7);
-    case ST_MID3: return (double)((rec[0] >> 3) & 7);
-    case ST_LOW5: return (double)(rec[0] & 31);
-  }
-  return 0.0;
-}
 
 // peers of this lane with the same key in the wave (match-any by key bits)
 __device__ inline uint64_t wave_match(uint32_t key, int bits, uint64_t active) {
@@ -126,7 +167,7 @@ __device__ inline uint64_t wave_match(uint32_t key, int bits, uint64_t active) {
 }
 
 // bytes of one wave's record staging area in LDS
-__host__ __device__ inline int record_area_bytes(int bpp) { return (64 * bpp + 32 + 15) & ~15; }
+__host__ __device__ inline int record_area_bytes(int bpp) { return (64 * bpp + 48 + 15) & ~15; }
 
 // K1: per-cell counts, attribute min/max (ordered uint64 bits) and the
 // classification histogram.  Each wave walks a contiguous range of 64-point
@@ -158,8 +199,9 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ la
   const int R = S > 0 ? 64 / S : 0;  // lanes per statistic
   const int my_stat = (S > 0 && lane < R * S) ? lane % S : -1;
   const int my_r0 = S > 0 ? lane / S : 0;
-  const StatOp op = my_stat >= 0 ? P.stats[my_stat] : StatOp{0, ST_NONE};
-  uint64_t my_min = ~0ull, my_max = 0ull;
+  const StatLane sl = stat_lane(my_stat >= 0 ? P.stats[my_stat] : StatOp{0, ST_NONE});
+  uint64_t my_min = ~0ull, my_max = 0ull;  // keys
+  bool seen = false;
   const int64_t steps_total = (n + 63) / 64;
   const int64_t wave_global = (int64_t)blockIdx.x * kWavesPerBlock + wave;
   const int64_t waves_total = (int64_t)gridDim.x * kWavesPerBlock;
@@ -214,17 +256,20 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ la
     }
     if (my_stat >= 0) {
       for (int r = my_r0; r < cnt; r += R) {
-        uint64_t b = ordered_bits(stat_value(rec_area + shift + r * bpp + op.src, op.kind));
-        my_min = b < my_min ? b : my_min;
-        my_max = b > my_max ? b : my_max;
+        const uint64_t kk = stat_key(sl, lds_u64(rec_area + shift + r * bpp + sl.src));
+        my_min = kk < my_min ? kk : my_min;
+        my_max = kk > my_max ? kk : my_max;
+        seen = true;
       }
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   }
   if (bad) atomicOr(err, 1);
-  s_smin[threadIdx.x] = my_min;
-  s_smax[threadIdx.x] = my_max;
+  // keys back to ordered double bits (a lane that saw no record keeps the
+  // neutral ~0 / 0)
+  s_smin[threadIdx.x] = seen ? stat_key_bits(sl, my_min) : ~0ull;
+  s_smax[threadIdx.x] = seen ? stat_key_bits(sl, my_max) : 0ull;
   __syncthreads();
   for (int i = threadIdx.x; i < kHashSize; i += kBlock) {
     uint32_t k = s_key[i];
@@ -342,21 +387,37 @@ __global__ __launch_bounds__(kBlock) void k_tile_hist(const uint8_t* __restrict_
 }
 
 
+// bytes of one wave's output staging area in LDS
+__host__ __device__ inline int out_area_bytes(int bpo) { return (64 * bpo + 16 + 15) & ~15; }
+
+// dynamic LDS of k_scatter: per-wave chunk counters, record staging, output
+// staging
+__host__ __device__ inline size_t scatter_lds_bytes(int num_chunks, int bpp, int bpo) {
+  return (((size_t)kWavesPerBlock * num_chunks * 4 + 15) & ~(size_t)15) +
+         (size_t)kWavesPerBlock * (record_area_bytes(bpp) + out_area_bytes(bpo));
+}
+
 // K3b: transcode + stable scatter.  Each wave owns a contiguous quarter of
 // the tile; pass 1 counts per chunk per wave, pass 2 places each point at
-// M[c][t] + (earlier waves' count) + (running count) + (rank in step).
+// M[c][t] + (earlier waves' count) + (running count) + (rank in step).  The
+// 64 output records of a step are assembled in LDS grouped into runs of one
+// chunk (a run is contiguous in the output), and every run is written with
+// aligned dword stores, bytes only where a run starts or ends inside a word.
 __global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ las, int64_t n, int32_t tile_size,
                                                      const int32_t* __restrict__ lut, int32_t num_chunks,
                                                      int32_t key_bits, int64_t tiles,
                                                      const uint32_t* __restrict__ M, uint8_t* __restrict__ out) {
-  extern __shared__ uint32_t s_cnt[];  // [kWavesPerBlock][num_chunks]
-  __shared__ __attribute__((aligned(16))) uint8_t s_rec[kWavesPerBlock][64 * kMaxBppIn + 32];
+  extern __shared__ __attribute__((aligned(16))) uint8_t s_dyn[];
   const ChunkerParams& P = c_params;
+  const int bpp = P.bpp_in, bpo = P.bpp_out;
+  uint32_t* s_cnt = (uint32_t*)s_dyn;  // [kWavesPerBlock][num_chunks]
+  uint8_t* s_stage = s_dyn + (((size_t)kWavesPerBlock * num_chunks * 4 + 15) & ~(size_t)15);
   const int64_t t = blockIdx.x;
   for (int i = threadIdx.x; i < kWavesPerBlock * num_chunks; i += kBlock) s_cnt[i] = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int bpp = P.bpp_in, bpo = P.bpp_out;
+  uint8_t* rec_area = s_stage + (size_t)wave * (record_area_bytes(bpp) + out_area_bytes(bpo));
+  uint8_t* out_area = rec_area + record_area_bytes(bpp);
   const uint64_t total = (uint64_t)n * bpp;
   const int64_t t0 = t * tile_size;
   const int64_t t1 = (t0 + tile_size) < n ? (t0 + tile_size) : n;
@@ -368,7 +429,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ 
   // pass 1: counts
   for (int64_t first = w0; first < w1; first += 64) {
     int cnt = (int)((w1 - first) < 64 ? (w1 - first) : 64);
-    uint32_t shift = wave_load_records(las, total, first, cnt, bpp, s_rec[wave]);
+    uint32_t shift = wave_load_records(las, total, first, cnt, bpp, rec_area);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     uint32_t c = 0;
@@ -376,7 +437,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ 
     if (valid) {
       bool inBox;
       int32_t q[3];
-      int64_t cell = cell_of_record(s_rec[wave] + shift + lane * bpp, P, inBox, q);
+      int64_t cell = cell_of_record(rec_area + shift + lane * bpp, P, inBox, q);
       int32_t cc = inBox ? lut[cell] : -1;
       valid = cc >= 0;
       c = valid ? (uint32_t)cc : 0u;
@@ -399,13 +460,15 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ 
   }
   __syncthreads();
   // pass 2: transcode + scatter
+  const int out_words = (64 * bpo + 3) / 4;
   for (int64_t first = w0; first < w1; first += 64) {
     int cnt = (int)((w1 - first) < 64 ? (w1 - first) : 64);
-    uint32_t shift = wave_load_records(las, total, first, cnt, bpp, s_rec[wave]);
+    uint32_t shift = wave_load_records(las, total, first, cnt, bpp, rec_area);
+    for (int i = lane; i < out_words; i += 64) ((uint32_t*)out_area)[i] = 0u;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     bool valid = lane < cnt;
-    const uint8_t* rec = s_rec[wave] + shift + lane * bpp;
+    const uint8_t* rec = rec_area + shift + lane * bpp;
     uint32_t c = 0;
     int32_t q[3] = {0, 0, 0};
     if (valid) {
@@ -415,18 +478,29 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ 
       valid = cc >= 0;
       c = valid ? (uint32_t)cc : 0u;
     }
-    uint64_t active = __ballot(valid);
-    uint64_t peers = wave_match(c, key_bits, active);
-    uint32_t base = valid ? my_cnt[c] : 0;
+    const uint64_t active = __ballot(valid);
+    const uint64_t peers = wave_match(c, key_bits, active);
+    const uint32_t base = valid ? my_cnt[c] : 0;
     __builtin_amdgcn_wave_barrier();
-    if (valid && (peers & lt_mask) == 0) my_cnt[c] = base + (uint32_t)__popcll(peers);
+    const bool leader = valid && (peers & lt_mask) == 0;
+    const uint32_t len = leader ? (uint32_t)__popcll(peers) : 0u;
+    if (leader) my_cnt[c] = base + len;
+    // runs in the staging area in leader-lane order
+    uint32_t incl = len;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t o = __shfl_up(incl, d);
+      if (lane >= d) incl += o;
+    }
+    const uint32_t run_start = incl - len;  // valid on leader lanes
+    const int my_leader = valid ? __builtin_ctzll(peers) : lane;
+    const uint32_t my_run = __shfl(run_start, my_leader);
     if (valid) {
-      uint64_t pos = (uint64_t)base + (uint64_t)__popcll(peers & lt_mask);
-      uint8_t* dst = out + pos * bpo;
-      uint8_t tmp[4];
+      uint8_t* dst = out_area + (size_t)(my_run + (uint32_t)__popcll(peers & lt_mask)) * bpo;
       for (int a = 0; a < 3; a++) {
-        __builtin_memcpy(tmp, &q[a], 4);
-        dst[4 * a + 0] = tmp[0]; dst[4 * a + 1] = tmp[1]; dst[4 * a + 2] = tmp[2]; dst[4 * a + 3] = tmp[3];
+        const uint32_t v = (uint32_t)q[a];
+        dst[4 * a + 0] = (uint8_t)v; dst[4 * a + 1] = (uint8_t)(v >> 8);
+        dst[4 * a + 2] = (uint8_t)(v >> 16); dst[4 * a + 3] = (uint8_t)(v >> 24);
       }
       for (int o = 0; o < P.num_ops; o++) {
         const TranscodeOp op = P.ops[o];
@@ -435,6 +509,27 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ 
         } else {
           uint8_t v = rec[op.src];
           dst[op.dst] = op.mode == TC_LOW3 ? (v & 7) : (op.mode == TC_MID3 ? ((v >> 3) & 7) : (v & 31));
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    // write each run: aligned dwords inside, bytes at partial edge words
+    uint64_t leaders = __ballot(leader);
+    while (leaders) {
+      const int L = __builtin_ctzll(leaders);
+      leaders &= leaders - 1;
+      const uint64_t g0 = (uint64_t)__builtin_amdgcn_readlane(base, L) * (uint64_t)bpo;
+      const uint32_t nbytes = (uint32_t)__builtin_amdgcn_readlane(len, L) * (uint32_t)bpo;
+      const uint8_t* src = out_area + (size_t)__builtin_amdgcn_readlane(run_start, L) * bpo;
+      const uint64_t wfirst = g0 >> 2, wlast = (g0 + nbytes - 1) >> 2;
+      for (uint64_t wi = wfirst + lane; wi <= wlast; wi += 64) {
+        const uint64_t b0 = wi << 2;
+        if (b0 >= g0 && b0 + 4 <= g0 + nbytes) {
+          *(uint32_t*)(out + b0) = lds_u32(src + (b0 - g0));
+        } else {
+          for (uint64_t b = b0; b < b0 + 4; b++)
+            if (b >= g0 && b < g0 + nbytes) out[b] = src[b - g0];
         }
       }
     }
@@ -473,7 +568,8 @@ __global__ void k_lasgen(LasGenParams p, const LasGenBuilding* __restrict__ bld,
   }
 }
 
-int g_bpp_in = 0;  // host copy of the record length in c_params
+int g_bpp_in = 0;   // host copies of the record lengths in c_params
+int g_bpp_out = 0;
 
 }  // namespace
 
@@ -483,6 +579,7 @@ int chunker_set_params(const ChunkerParams& p, hipStream_t stream) {
     return 1;
   }
   g_bpp_in = p.bpp_in;
+  g_bpp_out = p.bpp_out;
   PCG_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_params), &p, sizeof(p), 0, hipMemcpyHostToDevice, stream));
   return 0;
 }
@@ -552,8 +649,8 @@ int chunker_distribute(const uint8_t* d_las, int64_t n, const int32_t* d_lut, in
   if (exclusive_scan_u32(d_M, tiles * num_chunks, d_scan_ws, nullptr, stream)) return 1;
   int key_bits = 1;
   while ((1 << key_bits) < num_chunks) key_bits++;
-  size_t cnt_lds = sizeof(uint32_t) * kWavesPerBlock * num_chunks;
-  if (cnt_lds > 96 * 1024) {
+  size_t cnt_lds = scatter_lds_bytes(num_chunks, g_bpp_in, g_bpp_out);
+  if (cnt_lds > 150 * 1024) {
     set_error("too many chunks for the LDS scatter counters");
     return 1;
   }

@@ -309,9 +309,7 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
   out.d_records = (uint8_t*)workspace(WS_RECORDS, (size_t)N * out.bpp + 64);
   out.owned = false;
   if (!out.d_records) return 1;
-  int covered = 12;
-  for (int o = 0; o < P.num_ops; o++) covered += P.ops[o].len;
-  if (covered < out.bpp) PCG_CHECK(hipMemsetAsync(out.d_records, 0, (size_t)N * out.bpp, s));
+  // k_scatter writes every byte of every output record (unset bytes 0)
   PCG_CHECK(hipEventRecord(ev[2], s));
   if (chunker_distribute(d_las, N, lut.as<int32_t>(), C, tile, M.as<uint32_t>(), ws.p, out.d_records,
                          err.as<int32_t>(), s))
