@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--cpu-points", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="one device-to-device copy of this many MiB before the steps (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -99,6 +101,16 @@ def main():
     pc._check(L.pcg_lasgen_header(g.buf, qmm, header))
     stats = (ctypes.c_double * STATS)()
 
+    if args.copy_calibration_mib > 0:
+        cb = args.copy_calibration_mib << 20
+        src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+        pc._check(L.pcg_device_alloc(ctypes.byref(src), cb))
+        pc._check(L.pcg_device_alloc(ctypes.byref(dst), cb))
+        pc._check(L.pcg_copy(dst, src, cb, 2))
+        pc._check(L.pcg_synchronize())
+        L.pcg_device_free(src)
+        L.pcg_device_free(dst)
+
     def step():
         pc._check(L.pcg_convert_device(d_las, n, header, args.method.encode(), None, stats))
 
@@ -128,6 +140,15 @@ def main():
     kern_s = (kern["count_ms"] + kern["distribute_ms"]) / 1e3
     achieved = chunk_bytes / kern_s / 1e9 if kern_s > 0 else 0.0
     last = per[-1]
+    # HBM bytes per launch of the chunking pass from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE runs of this workload (copy-calibrated)
+    traffic = None
+    tpath = os.path.join(HERE, "profiles", "chunking_200m_hbm_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        if tr.get("points") == n and args.seed == 0:
+            traffic = tr["traffic_bytes"]
     cpu = None if args.no_cpu else cpu_baseline(min(args.cpu_points, n), args.seed, args.method)
     line = {
         "metric": "Mpts/s count-sort+octree+Poisson at 200M pts, batched vs plain; % HBM peak",
@@ -168,7 +189,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "algorithmic_bytes": chunk_bytes,
             "kernel_ms": round(kern_s * 1e3, 4),
         },
