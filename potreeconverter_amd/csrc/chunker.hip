@@ -57,15 +57,16 @@ __device__ inline int32_t ld_i32(const uint8_t* p) {
 
 // 4 bytes at any address inside an LDS staging area (two aligned dword
 // reads and a byte align); the area is padded so the second read stays in it
+// (the aligned pointer is derived from p by pointer arithmetic so that the
+// compiler keeps the LDS address space and emits ds_read, not flat loads)
 __device__ inline uint32_t lds_u32(const uint8_t* p) {
-  const uintptr_t a = (uintptr_t)p;
-  const uint32_t* w = (const uint32_t*)(a & ~(uintptr_t)3);
-  return __builtin_amdgcn_alignbyte(w[1], w[0], (uint32_t)(a & 3));
+  const uint32_t s = (uint32_t)((uintptr_t)p & 3);
+  const uint32_t* w = (const uint32_t*)(p - s);
+  return __builtin_amdgcn_alignbyte(w[1], w[0], s);
 }
 __device__ inline uint64_t lds_u64(const uint8_t* p) {
-  const uintptr_t a = (uintptr_t)p;
-  const uint32_t* w = (const uint32_t*)(a & ~(uintptr_t)3);
-  const uint32_t s = (uint32_t)(a & 3);
+  const uint32_t s = (uint32_t)((uintptr_t)p & 3);
+  const uint32_t* w = (const uint32_t*)(p - s);
   return (uint64_t)__builtin_amdgcn_alignbyte(w[1], w[0], s) |
          ((uint64_t)__builtin_amdgcn_alignbyte(w[2], w[1], s) << 32);
 }
@@ -143,11 +144,24 @@ __device__ inline int64_t cell_of_record(const uint8_t* rec, const ChunkerParams
     int32_t X = (int32_t)lds_u32(rec + 4 * a);
     double x = P.las_scale[a] * X + P.las_offset[a];
     int32_t Xq = (int32_t)((x - P.pos_offset[a]) / P.pos_scale[a]);
-    double u = ((double)Xq * P.pos_scale[a] + P.pos_offset[a] - P.min[a]) / P.cube;
-    inBox = inBox && u >= 0.0 && u <= 1.0;
-    double gu = dG * u;
-    double lim = dG - 1.0;
-    int64_t ic = (int64_t)(lim < gu ? lim : gu);
+    const double v = (double)Xq * P.pos_scale[a] + P.pos_offset[a] - P.min[a];
+    // G * (v / cube) without the division: v * (G / cube) differs from it
+    // by a few 1e-13 at most, so its floor is the cell unless it lies within
+    // 1e-9 of a cell boundary or of the box faces; those take the reference
+    // expression
+    const double t = v * P.g_over_cube;
+    const double ft = __builtin_floor(t);
+    const double fr = t - ft;
+    int64_t ic;
+    if (fr > 1e-9 && fr < 1.0 - 1e-9 && t > 1e-9 && t < dG - 1e-9) {
+      ic = (int64_t)ft;
+    } else {
+      double u = v / P.cube;
+      inBox = inBox && u >= 0.0 && u <= 1.0;
+      double gu = dG * u;
+      double lim = dG - 1.0;
+      ic = (int64_t)(lim < gu ? lim : gu);
+    }
     idx += ic * mul;
     mul *= G;
     q[a] = Xq;
@@ -580,7 +594,11 @@ int chunker_set_params(const ChunkerParams& p, hipStream_t stream) {
   }
   g_bpp_in = p.bpp_in;
   g_bpp_out = p.bpp_out;
-  PCG_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_params), &p, sizeof(p), 0, hipMemcpyHostToDevice, stream));
+  static ChunkerParams staged;  // the async copy reads it after this call returns
+  PCG_CHECK(hipStreamSynchronize(stream));
+  staged = p;
+  staged.g_over_cube = (double)p.grid_size / p.cube;
+  PCG_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_params), &staged, sizeof(staged), 0, hipMemcpyHostToDevice, stream));
   return 0;
 }
 

@@ -59,6 +59,7 @@ struct ChunkerParams {
   double pos_offset[3];
   double min[3];
   double cube;
+  double g_over_cube;  // grid_size / cube (set by chunker_set_params)
   int32_t grid_size;
   int32_t bpp_in;
   int32_t bpp_out;
