@@ -39,12 +39,22 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_u32(uint32_t* __restrict__ 
   const int64_t base = (int64_t)tile * kScanTile + (int64_t)threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
   uint64_t local = 0;
+  if (base + kScanItems <= n) {
+    const uint4* src = (const uint4*)(data + base);  // 64-byte aligned
 #pragma unroll
-  for (int k = 0; k < kScanItems; k++) {
-    int64_t i = base + k;
-    v[k] = i < n ? data[i] : 0u;
-    local += v[k];
+    for (int k = 0; k < kScanItems / 4; k++) {
+      const uint4 q = src[k];
+      v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+      int64_t i = base + k;
+      v[k] = i < n ? data[i] : 0u;
+    }
   }
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) local += v[k];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint64_t incl = wave_incl_scan(local);
   if (lane == 63) s_wave[wave] = incl;
@@ -54,24 +64,41 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_u32(uint32_t* __restrict__ 
     if (w < wave) wave_off += s_wave[w];
     agg += s_wave[w];
   }
-  if (threadIdx.x == 0) {
+  // publish the aggregate, then wave 0 looks back over 64 predecessors at a
+  // time: the nearest one with an inclusive prefix ends the walk, the ones
+  // after it contribute their aggregates; a predecessor that has published
+  // nothing yet (tiles are taken in order, so it is running) is re-read
+  if (wave == 0) {
     if (tile == 0) {
-      __hip_atomic_store(&status[0], kFlagPre | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      s_prefix = 0;
-    } else {
-      __hip_atomic_store(&status[tile], kFlagAgg | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      uint64_t prefix = 0;
-      int64_t j = (int64_t)tile - 1;
-      while (true) {
-        uint64_t s = __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        uint64_t flag = s & ~kValMask;
-        if (flag == 0) continue;  // predecessor not published yet; tiles start in order, so it will
-        prefix += s & kValMask;
-        if (flag == kFlagPre) break;
-        j--;
+      if (lane == 0) {
+        __hip_atomic_store(&status[0], kFlagPre | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        s_prefix = 0;
       }
-      __hip_atomic_store(&status[tile], kFlagPre | (prefix + agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      s_prefix = prefix;
+    } else {
+      if (lane == 0) __hip_atomic_store(&status[tile], kFlagAgg | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      uint64_t prefix = 0;
+      int64_t j = (int64_t)tile - 1;  // newest predecessor of the window
+      while (true) {
+        const int64_t idx = j - lane;
+        uint64_t st = idx >= 0 ? __hip_atomic_load(&status[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+                               : kFlagPre;  // before tile 0: an empty inclusive prefix
+        const uint64_t flag = st & ~kValMask;
+        const uint64_t pre = __ballot(flag == kFlagPre);
+        const uint64_t none = __ballot(flag == 0);
+        const int stop = pre ? __builtin_ctzll(pre) : 64;  // nearest inclusive prefix
+        const uint64_t upto = stop == 63 ? ~0ull : ((stop == 64) ? ~0ull : ((1ull << (stop + 1)) - 1));
+        if (none & upto) continue;  // a predecessor in the window is not published yet
+        uint64_t v = ((uint64_t)lane <= (uint64_t)stop) ? (st & kValMask) : 0ull;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+        prefix += v;
+        if (stop < 64) break;
+        j -= 64;
+      }
+      if (lane == 0) {
+        __hip_atomic_store(&status[tile], kFlagPre | (prefix + agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        s_prefix = prefix;
+      }
     }
   }
   __syncthreads();
