@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--cpu-points", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--plain-points", type=int, default=20_000_000,
+                    help="size of the batched-vs-plain sampling comparison (0 skips it)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="one device-to-device copy of this many MiB before the steps (HBM counter calibration)")
     return ap.parse_args()
@@ -74,6 +76,50 @@ def cpu_baseline(points, seed, method):
         "kind": "port",
         "sample": f"first {points} points of the same generated stream; chunking {t[2]:.2f}s + indexing {t[3]:.2f}s "
                   f"(wall {wall:.1f}s incl. file read/write)",
+    }
+
+
+def plain_vs_batched(points, seed, method):
+    """Node sampling of the same input with every node of a level in one
+    launch per kernel (batched) and with one launch per node (plain)."""
+    import potreeconverter_amd as pc
+
+    L = pc.lib()
+    g = pc.LasGen(1, 200_000_000, seed=seed)  # the prefix of the benchmark's point stream
+    ctypes.c_int64.from_buffer(g.buf, 8).value = points
+    rl = g.record_length
+    d_las, d_q = ctypes.c_void_p(), ctypes.c_void_p()
+    pc._check(L.pcg_device_alloc(ctypes.byref(d_las), points * rl + 64))
+    pc._check(L.pcg_device_alloc(ctypes.byref(d_q), 64))
+    qmm = (ctypes.c_int32 * 6)(2**31 - 1, 2**31 - 1, 2**31 - 1, -2**31, -2**31, -2**31)
+    pc._check(L.pcg_copy(d_q, qmm, 24, 0))
+    pc._check(L.pcg_lasgen_device(g.buf, 0, points, d_las, d_q))
+    pc._check(L.pcg_copy(qmm, d_q, 24, 1))
+    header = ctypes.create_string_buffer(375)
+    pc._check(L.pcg_lasgen_header(g.buf, qmm, header))
+    st = {}
+    try:
+        for mode in ("batched", "batched", "plain"):
+            pc._check(L.pcg_set_option(b"plain_sampling", 1 if mode == "plain" else 0))
+            stats = (ctypes.c_double * STATS)()
+            pc._check(L.pcg_convert_device(d_las, points, header, method.encode(), None, stats))
+            st[mode] = list(stats)
+    finally:
+        L.pcg_set_option(b"plain_sampling", 0)
+        L.pcg_device_free(d_las)
+        L.pcg_device_free(d_q)
+    b, p = st["batched"], st["plain"]
+    cand = b[10]
+    return {
+        "points": points,
+        "candidates": int(cand),
+        "sampled_nodes": int(b[9]),
+        "batched_sampling_s": round(b[5], 4),
+        "plain_sampling_s": round(p[5], 4),
+        "batched_Mcand_per_s": round(cand / b[5] / 1e6, 1),
+        "plain_Mcand_per_s": round(cand / p[5] / 1e6, 1),
+        "speedup": round(p[5] / b[5], 2),
+        "same_output_bytes": b[7] == p[7],
     }
 
 
@@ -149,6 +195,9 @@ def main():
             tr = json.load(f)
         if tr.get("points") == n and args.seed == 0:
             traffic = tr["traffic_bytes"]
+    plain = None
+    if args.plain_points > 0:
+        plain = plain_vs_batched(min(args.plain_points, n), args.seed, args.method)
     cpu = None if args.no_cpu else cpu_baseline(min(args.cpu_points, n), args.seed, args.method)
     line = {
         "metric": "Mpts/s count-sort+octree+Poisson at 200M pts, batched vs plain; % HBM peak",
@@ -193,6 +242,7 @@ def main():
             "algorithmic_bytes": chunk_bytes,
             "kernel_ms": round(kern_s * 1e3, 4),
         },
+        "plain_vs_batched": plain,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -33,6 +33,15 @@ const char* pcg_last_error(void);
  * reference counterpart (the reference converter is a one-shot process). */
 int pcg_release(void);
 
+/* Library options.  "plain_sampling" (0/1): 1 launches the sampling kernels
+ * once per octree node instead of once per level for all its nodes (same
+ * results; the unbatched baseline the batched path is measured against).
+ * "ring_min_nodes" (default 512): Poisson octree levels with at least this
+ * many nodes decide acceptance with the accepted-shell ring kernel, the
+ * others with the cell-table kernels (same results).
+ * No reference counterpart. */
+int pcg_set_option(const char* key, int64_t value);
+
 /* Phase 1 + phase 2 from a LAS file, the drop-in for the two calls at
  * Converter/src/main.cpp:353 (chunker_countsort_laszip::doChunking) and
  * Converter/src/main.cpp:380-390 (indexer::doIndexing with

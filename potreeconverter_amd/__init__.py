@@ -47,7 +47,8 @@ def lib():
         L.pcg_lasgen_device.argtypes = [c.c_void_p, c.c_int64, c.c_int64, c.c_void_p, c.c_void_p]
         L.pcg_lasgen_header.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p]
         L.pcg_release.argtypes = []
-        for name in ("pcg_release", "pcg_convert", "pcg_do_chunking", "pcg_do_indexing", "pcg_count_cells", "pcg_convert_device",
+        L.pcg_set_option.argtypes = [c.c_char_p, c.c_int64]
+        for name in ("pcg_set_option", "pcg_release", "pcg_convert", "pcg_do_chunking", "pcg_do_indexing", "pcg_count_cells", "pcg_convert_device",
                      "pcg_device_alloc", "pcg_device_free", "pcg_copy", "pcg_synchronize", "pcg_lasgen_preset",
                      "pcg_lasgen_write", "pcg_lasgen_device", "pcg_lasgen_header", "pcg_init"):
             getattr(L, name).restype = c.c_int

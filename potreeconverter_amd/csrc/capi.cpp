@@ -74,6 +74,24 @@ int pcg_release(void) {
   return 0;
 }
 
+int pcg_set_option(const char* key, int64_t value) {
+  const std::string k = key ? key : "";
+  if (k == "plain_sampling") {
+    g_plain_sampling = value != 0;
+    return 0;
+  }
+  if (k == "ring_min_nodes") {
+    if (value < 1 || value > (1 << 30)) {
+      set_error("ring_min_nodes must be in [1, 2^30]");
+      return 1;
+    }
+    g_ring_min_nodes = (int32_t)value;
+    return 0;
+  }
+  set_error("unknown option: " + k);
+  return 1;
+}
+
 int pcg_convert(const char* las_path, const char* outdir, const char* method, int32_t flags, double* timings) {
   if (ensure_device()) return 1;
   lasfile::Header h;

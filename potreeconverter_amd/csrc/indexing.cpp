@@ -87,6 +87,9 @@ struct DevMem {
 
 }  // namespace
 
+bool g_plain_sampling = false;
+int32_t g_ring_min_nodes = 512;
+
 void IndexResult::release() {
   d_octree = nullptr;  // a workspace buffer
 }
@@ -318,6 +321,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   sp.scratch_top = tops.as<uint64_t>() + 1;
   sp.counters = counters.as<unsigned long long>();
   sp.item_start = ditems.as<uint32_t>();
+  sp.ring_min_nodes = g_ring_min_nodes;
   sp.segs = doutsegs.as<OutSeg>();
   sp.seg_start = dsegstart.as<uint32_t>();
   sp.nbr_pool = pool.as<uint32_t>();
@@ -331,9 +335,17 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     if (lst.empty()) continue;
     sampled += (int64_t)lst.size();
     PCG_CHECK(hipMemsetAsync(tops.as<uint64_t>() + 1, 0, 16, s));  // scratch and pool tops
-    if (sample_level(dlist.as<int32_t>() + levelOffset[l], (int32_t)lst.size(), dinfo.as<NodeInfo>(),
-                     dstate.as<NodeState>(), dwork.as<NodeWork>(), sp, s))
+    if (g_plain_sampling) {
+      // one launch sequence per node (the scratch of every node of the level
+      // stays allocated until the level is done, as in the batched mode)
+      for (size_t q = 0; q < lst.size(); q++)
+        if (sample_level(dlist.as<int32_t>() + levelOffset[l] + q, 1, dinfo.as<NodeInfo>(), dstate.as<NodeState>(),
+                         dwork.as<NodeWork>() + q, sp, s))
+          return 1;
+    } else if (sample_level(dlist.as<int32_t>() + levelOffset[l], (int32_t)lst.size(), dinfo.as<NodeInfo>(),
+                            dstate.as<NodeState>(), dwork.as<NodeWork>(), sp, s)) {
       return 1;
+    }
     if (levelStats) {
       // diagnostic summary of the level (PCG_LEVEL_STATS=1), costs a sync
       std::vector<NodeWork> wk(lst.size());

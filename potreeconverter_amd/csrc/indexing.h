@@ -33,6 +33,14 @@ struct IndexResult {
 
 // method: "poisson" | "poisson_average".  `name`/`pointsTotal` go to
 // metadata.json as in createMetadata (indexer.cpp:354-524).
+// Sampling launch mode: false = every node of an octree level in one launch
+// per kernel (batched), true = one launch per node (plain).
+extern bool g_plain_sampling;
+// Poisson levels with at least this many nodes decide acceptance with the
+// LDS shell ring (one wavefront per node); narrower levels, where one
+// wavefront per node cannot hide latency, use the cell-table path.
+extern int32_t g_ring_min_nodes;
+
 int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string& name, int64_t pointsTotal,
                  IndexResult& out);
 

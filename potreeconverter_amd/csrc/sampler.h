@@ -76,6 +76,8 @@ struct SampleParams {
   uint8_t* scratch;        // per-level scratch
   uint64_t scratch_cap;
   uint64_t* scratch_top;
+  int32_t ring_min_nodes;  // levels with fewer nodes decide acceptance on the cell-table path
+  int32_t pad2;
   uint32_t* item_start;    // num_nodes + 1 entries: work items of the level
   OutSeg* segs;            // 9 per node of the level
   uint32_t* seg_start;     // 9 * num_nodes + 1 entries: copy items per segment

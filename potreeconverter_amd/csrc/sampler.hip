@@ -40,9 +40,6 @@ constexpr uint8_t kNbrPooled = 254;    // list in the level's neighbour pool
 constexpr int kNeighborBlocks = 4096;  // grid of the level-wide neighbour search
 constexpr int kCopyBlocks = 8192;      // grid of the level-wide record copy
 constexpr int kAccWords = 2048;         // LDS accepted bits per wave in k_accept (65536 candidates)
-// levels with fewer nodes than this decide acceptance on the cell-table path:
-// one wavefront per node cannot hide k_ring's latency without other nodes
-constexpr int kRingMinNodes = 512;
 
 __device__ inline int32_t ld_i32(const uint8_t* p) {
   return (int32_t)((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24));
@@ -336,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
     w.n = n;
     for (int c = 0; c < 9; c++) w.child_start[c] = s_cs[c];
     for (int c = 0; c < 8; c++) w.csrc[c] = s_cptr[c];
-    w.flags = (P.method == METHOD_POISSON_AVERAGE || (int)gridDim.x < kRingMinNodes) ? NW_CELL_PATH : 0;
+    w.flags = (P.method == METHOD_POISSON_AVERAGE || (int)gridDim.x < P.ring_min_nodes) ? NW_CELL_PATH : 0;
     w.pad = 0;
     if (off + bytes > P.scratch_cap) {
       w.flags = NW_OVERFLOW;
@@ -1844,7 +1841,7 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   PCG_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_exact_sort, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p, p.counters);
   PCG_CHECK(hipGetLastError());
-  if (p.method == METHOD_POISSON && num_nodes >= kRingMinNodes) {
+  if (p.method == METHOD_POISSON && num_nodes >= p.ring_min_nodes) {
     hipLaunchKernelGGL(k_ring, dim3((unsigned)((num_nodes + kRingWaves - 1) / kRingWaves)), dim3(64 * kRingWaves),
                        0, s, d_nodes, d_info, d_work, num_nodes, p);
     PCG_CHECK(hipGetLastError());

@@ -29,3 +29,12 @@ def test_package_refuses_to_run_without_library(tmp_path, monkeypatch):
     except pc.PotreeGpuError:
         raised = True
     assert raised
+
+
+def test_unknown_option_is_an_error():
+    import potreeconverter_amd as pc
+
+    L = pc.lib()
+    assert L.pcg_set_option(b"no_such_option", 1) != 0
+    assert b"unknown option" in L.pcg_last_error()
+    assert L.pcg_set_option(b"ring_min_nodes", 0) != 0

@@ -173,3 +173,80 @@ def test_large_run_is_deterministic_and_conserves_points(tmp_path):
     assert os.path.getsize(outs[0] / "octree.bin") == 27 * n
     L.pcg_device_free(d)
     L.pcg_device_free(dq)
+
+
+def _set(key, value):
+    rc = pc.lib().pcg_set_option(key.encode(), value)
+    assert rc == 0, pc.lib().pcg_last_error()
+
+
+RING_CASES = [(0, 150_000, 2, False), (0, 2_000_000, 7, False), (0, 400_000, 5, True), (1, 1_000_000, 1, False)]
+
+
+@pytest.mark.parametrize("preset,n,seed,eb", RING_CASES, ids=[f"p{c[0]}_{c[1]}_s{c[2]}" for c in RING_CASES])
+def test_ring_path_matches_oracle(preset, n, seed, eb, tmp_path):
+    """Every Poisson level through the accepted-shell ring kernel."""
+    print(f"ring case preset={preset} n={n} seed={seed} extra={eb}", flush=True)
+    las = tmp_path / "in.las"
+    make_las(las, preset, n, seed, eb)
+    cpu, gpu = tmp_path / "cpu", tmp_path / "gpu"
+    oracle_convert(las, cpu, "poisson")
+    _set("ring_min_nodes", 1)
+    try:
+        pc.convert(las, gpu, "poisson")
+    finally:
+        _set("ring_min_nodes", 512)
+    res = same_files(cpu, gpu, ["octree.bin", "hierarchy.bin", "metadata.json"])
+    assert all(res.values()), res
+
+
+@pytest.mark.parametrize("method", ["poisson", "poisson_average"])
+def test_plain_sampling_matches_oracle(method, tmp_path):
+    """One launch sequence per node gives the batched path's bytes."""
+    print(f"plain case method={method}", flush=True)
+    las = tmp_path / "in.las"
+    make_las(las, 0, 300_000, 4, False)
+    cpu, gpu = tmp_path / "cpu", tmp_path / "gpu"
+    oracle_convert(las, cpu, method)
+    _set("plain_sampling", 1)
+    try:
+        pc.convert(las, gpu, method)
+    finally:
+        _set("plain_sampling", 0)
+    res = same_files(cpu, gpu, ["octree.bin", "hierarchy.bin", "metadata.json"])
+    assert all(res.values()), res
+
+
+def test_ring_and_cell_paths_agree_at_20m(tmp_path):
+    """20M points, where the widest levels use the ring by default: the same
+    bytes as with every level on the cell-table path and every level on the
+    ring."""
+    L = pc.lib()
+    n = 20_000_000
+    g = pc.LasGen(1, n, seed=3)
+    d = ctypes.c_void_p()
+    assert L.pcg_device_alloc(ctypes.byref(d), n * 26 + 64) == 0
+    q = (ctypes.c_int32 * 6)(2**31 - 1, 2**31 - 1, 2**31 - 1, -2**31, -2**31, -2**31)
+    dq = ctypes.c_void_p()
+    assert L.pcg_device_alloc(ctypes.byref(dq), 64) == 0
+    assert L.pcg_copy(dq, q, 24, 0) == 0
+    assert L.pcg_lasgen_device(g.buf, 0, n, d, dq) == 0
+    assert L.pcg_copy(q, dq, 24, 1) == 0
+    hdr = ctypes.create_string_buffer(375)
+    assert L.pcg_lasgen_header(g.buf, q, hdr) == 0
+    stats = (ctypes.c_double * 24)()
+    outs = []
+    try:
+        for k, ring_min in enumerate((512, 1 << 30, 1)):
+            print(f"20M run ring_min_nodes={ring_min}", flush=True)
+            _set("ring_min_nodes", ring_min)
+            o = tmp_path / f"run{k}"
+            rc = L.pcg_convert_device(d, n, hdr, b"poisson", str(o).encode(), stats)
+            assert rc == 0, L.pcg_last_error()
+            outs.append(o)
+    finally:
+        _set("ring_min_nodes", 512)
+        L.pcg_device_free(d)
+        L.pcg_device_free(dq)
+    for o in outs[1:]:
+        assert all(same_files(outs[0], o, ["octree.bin", "hierarchy.bin", "metadata.json"]).values())
