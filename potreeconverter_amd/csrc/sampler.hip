@@ -22,7 +22,8 @@
 //   k_route        accepted records to the node, the rest back to the
 //                  children in their order (:247-291 / :418-482): ranks,
 //                  selection lists and output ranges
-//   k_copy_out     the record bytes of every output range of the level
+//   k_copy_tiles   the record bytes: 64-record tiles of the children, each
+//                  written as one accepted run and one remainder run
 #include "lstd_sort.h"
 #include "pcg_common.h"
 #include "sampler.h"
@@ -73,6 +74,7 @@ struct Layout {
   long long* sums;   // sorted order, 8 per candidate (average)
   uint32_t* nbr;     // sorted order, kMaxNbr earlier neighbours per candidate
   uint8_t* nbrCount; // kNbrOverflow: more than kMaxNbr
+  uint32_t* tileAcc; // per 64-record tile of each child: accepted candidates of the node before it
 };
 
 // Carves the per-node scratch into the arrays above; with base == nullptr it
@@ -108,6 +110,7 @@ __host__ __device__ inline uint64_t layout_carve(uint8_t* base, int64_t n, int m
   L->rgb = avg ? (uint16_t*)take(6 * n) : nullptr;
   L->nbr = (uint32_t*)take(4 * kMaxNbr * n);
   L->nbrCount = (uint8_t*)take(n);
+  L->tileAcc = (uint32_t*)take(4 * (n / 64 + 16));
   L->sidx = L->idxA;
   return off + 64;
 }
@@ -1554,7 +1557,7 @@ __global__ void k_sequential(const int32_t* __restrict__ nodes, const NodeInfo* 
 // in their child in order (Poisson-average children keep every record, the
 // accepted ones with the averaged colour).  Allocates the output ranges,
 // writes the selection lists and the output segment table, and updates the
-// node states; k_copy_out moves the bytes.
+// node states; k_copy_tiles moves the bytes.
 __global__ __launch_bounds__(kBlock) void k_route(const int32_t* __restrict__ nodes,
                                                    const NodeInfo* __restrict__ infos,
                                                    NodeState* __restrict__ states, NodeWork* __restrict__ work,
@@ -1578,7 +1581,6 @@ __global__ __launch_bounds__(kBlock) void k_route(const int32_t* __restrict__ no
   const bool seq_avg = (w.flags & NW_SEQ_AVERAGE) != 0;
   Layout L = make_layout(w.scratch, n, P.method);
   uint32_t* inv = L.keyB;  // main index -> sorted position
-  uint32_t* sel = L.next;  // [accepted (main index) | remainder of child 0 | ... ]
   for (int64_t p = tid; p < n; p += kBlock) inv[L.idxA[p]] = (uint32_t)p;
   if (tid < 8) s_acc[tid] = 0;
   __syncthreads();
@@ -1608,9 +1610,8 @@ __global__ __launch_bounds__(kBlock) void k_route(const int32_t* __restrict__ no
     } else {
       uint8_t* o = P.arena + off;
       s_out[8] = o;
-      segs[0] = OutSeg{o, sel, total, k, -1};
+      segs[0] = OutSeg{o, nullptr, total, k, -1};
       o += seg_bytes(total);
-      int64_t selOff = total;
       for (int c = 0; c < 8; c++) {
         int64_t cnt = w.child_start[c + 1] - w.child_start[c];
         int64_t rej = cnt - s_acc[c];
@@ -1618,9 +1619,8 @@ __global__ __launch_bounds__(kBlock) void k_route(const int32_t* __restrict__ no
         segs[1 + c] = OutSeg{nullptr, nullptr, 0, k, c};
         if (info.children[c] >= 0 && rej > 0) {
           s_out[c] = o;
-          segs[1 + c] = OutSeg{o, avg ? nullptr : sel + selOff, avg ? cnt : rej, k, c};
+          segs[1 + c] = OutSeg{o, nullptr, avg ? cnt : rej, k, c};
           o += seg_bytes(avg ? cnt : rej);
-          if (!avg) selOff += rej;
         }
       }
     }
@@ -1629,15 +1629,12 @@ __global__ __launch_bounds__(kBlock) void k_route(const int32_t* __restrict__ no
   if (s_fail) return;
   // ranks in main order: accepted -> sel[rank]; rejected (Poisson) -> the
   // child's remainder list; Poisson-average colours of accepted records
-  __shared__ int64_t acc_start[8], rem_start[8];
+  __shared__ int64_t tile_base[8];  // first tile of each child in tileAcc
   if (tid == 0) {
-    int64_t r = 0, q = total;
+    int64_t tb = 0;
     for (int c = 0; c < 8; c++) {
-      acc_start[c] = r;
-      rem_start[c] = q;
-      r += s_acc[c];
-      int64_t cnt = w.child_start[c + 1] - w.child_start[c];
-      if (info.children[c] >= 0 && cnt - s_acc[c] > 0) q += cnt - s_acc[c];
+      tile_base[c] = tb;
+      tb += (w.child_start[c + 1] - w.child_start[c] + 63) / 64;
     }
   }
   __syncthreads();
@@ -1659,8 +1656,8 @@ __global__ __launch_bounds__(kBlock) void k_route(const int32_t* __restrict__ no
       for (int x = 1; x < 8; x++) c += j >= w.child_start[x] ? 1 : 0;
       const int64_t local = j - w.child_start[c];
       const int64_t apos = run + woff + before_in_wave;
+      if ((local & 63) == 0) L.tileAcc[tile_base[c] + (local >> 6)] = (uint32_t)apos;
       if (a) {
-        sel[apos] = (uint32_t)j;
         if (avg) {
           long long rs[3], ws;
           if (seq_avg) {
@@ -1673,8 +1670,6 @@ __global__ __launch_bounds__(kBlock) void k_route(const int32_t* __restrict__ no
           }
           for (int ch = 0; ch < 3; ch++) L.rgb[3 * j + ch] = (uint16_t)(rs[ch] / ws);
         }
-      } else if (!avg) {
-        sel[rem_start[c] + local - (apos - acc_start[c])] = (uint32_t)local;
       }
     }
     run += tot;
@@ -1709,22 +1704,23 @@ __global__ __launch_bounds__(kBlock) void k_route(const int32_t* __restrict__ no
   }
 }
 
-constexpr int kCopyDwords = 4;  // output dwords per thread per copy item
-
-// Copy items of the level's output segments (kBlock * kCopyDwords dwords
-// each): seg_start[s] is the first item of segment s.
-__global__ __launch_bounds__(1024) void k_seg_plan(const OutSeg* __restrict__ segs, int32_t num_segs, int32_t bpp,
-                                                    uint32_t* __restrict__ seg_start) {
+// Copy tiles of the level: 64 consecutive records of one child each.
+// tile_start[8 k + c] is the first tile of child c of node k.
+__global__ __launch_bounds__(1024) void k_tile_plan(const NodeWork* __restrict__ work, int32_t num_nodes,
+                                                     uint32_t* __restrict__ tile_start) {
   __shared__ uint32_t s_w[16];
   __shared__ uint32_t s_run;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (tid == 0) s_run = 0;
   __syncthreads();
-  constexpr uint64_t per_item = (uint64_t)kBlock * kCopyDwords * 4;
-  for (int b = 0; b < num_segs; b += 1024) {
+  const int num = 8 * num_nodes;
+  for (int b = 0; b < num; b += 1024) {
     const int q = b + tid;
     uint32_t v = 0;
-    if (q < num_segs) v = (uint32_t)(((uint64_t)segs[q].count * bpp + per_item - 1) / per_item);
+    if (q < num) {
+      const NodeWork& w = work[q >> 3];
+      if (!(w.flags & NW_OVERFLOW)) v = (uint32_t)((w.child_start[(q & 7) + 1] - w.child_start[q & 7] + 63) / 64);
+    }
     uint32_t incl = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -1739,91 +1735,127 @@ __global__ __launch_bounds__(1024) void k_seg_plan(const OutSeg* __restrict__ se
       tot += s_w[x];
     }
     const uint32_t run = s_run;
-    if (q < num_segs) seg_start[q] = run + woff + incl - v;
+    if (q < num) tile_start[q] = run + woff + incl - v;
     __syncthreads();
     if (tid == 0) s_run = run + tot;
     __syncthreads();
   }
-  if (tid == 0) seg_start[num_segs] = s_run;
+  if (tid == 0) tile_start[num] = s_run;
 }
 
-// Writes the output segments of the level with aligned dword stores: each
-// dword takes its four bytes from the source record(s) it covers, the
-// averaged colour replacing RGB in accepted Poisson-average records.
-__global__ __launch_bounds__(kBlock) void k_copy_out(const OutSeg* __restrict__ segs, int32_t num_segs,
-                                                      const uint32_t* __restrict__ seg_start,
-                                                      const NodeWork* __restrict__ work, SampleParams P) {
-  const uint32_t total = seg_start[num_segs];
-  const int tid = threadIdx.x;
+// Writes `count` records from the wave's staged tile (records `list[r]` of
+// the stage, in order) to dst + first * bpp: aligned dwords inside the range,
+// bytes at its edges; averaged colours replace RGB of accepted records when
+// `col` is set (Poisson-average).
+__device__ inline void write_run(uint8_t* __restrict__ dst, uint64_t first, uint32_t count, const uint8_t* stage,
+                                 const uint8_t* list, int bpp, const uint16_t* col, const int64_t* colj,
+                                 uint64_t colmask, int rgb0) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t g0 = first * (uint64_t)bpp;
+  const uint32_t nbytes = count * (uint32_t)bpp;
+  if (nbytes == 0) return;
+  const uint64_t wfirst = g0 >> 2, wlast = (g0 + nbytes - 1) >> 2;
+  for (uint64_t wi = wfirst + lane; wi <= wlast; wi += 64) {
+    const uint64_t b0 = wi << 2;
+    uint32_t word = 0;
+    bool full = true;
+    for (int t = 0; t < 4; t++) {
+      const uint64_t gb = b0 + t;
+      if (gb < g0 || gb >= g0 + nbytes) { full = false; continue; }
+      const uint32_t ob = (uint32_t)(gb - g0);
+      const uint32_t r = ob / (uint32_t)bpp;
+      const int bb = (int)(ob - r * (uint32_t)bpp);
+      const int l = list[r];
+      uint32_t byte = stage[l * bpp + bb];
+      if (col && ((colmask >> l) & 1ull) && bb >= rgb0 && bb < rgb0 + 6) {
+        const uint16_t v = col[3 * colj[l] + (bb - rgb0) / 2];
+        byte = ((bb - rgb0) & 1) ? (v >> 8) : (v & 0xFF);
+      }
+      word |= byte << (8 * t);
+    }
+    if (full) {
+      *(uint32_t*)(dst + b0) = word;
+    } else {
+      for (int t = 0; t < 4; t++) {
+        const uint64_t gb = b0 + t;
+        if (gb >= g0 && gb < g0 + nbytes) dst[gb] = (uint8_t)(word >> (8 * t));
+      }
+    }
+  }
+}
+
+// Moves the records of every sampled node of the level: each wavefront takes
+// a tile of 64 consecutive records of one child (16-byte loads into LDS) and
+// writes its accepted records as one run of the node's range and its
+// rejected records as one run of the child's remainder (Poisson-average: the
+// whole tile to the remainder, accepted records with the averaged colour,
+// and the accepted ones to the node).  Ranks come from the accepted counts
+// k_route recorded per tile.
+constexpr int kTileStage = 64 * 130 + 32;  // record bytes of a staged tile (bpp <= 130)
+
+__global__ __launch_bounds__(kBlock) void k_copy_tiles(const NodeWork* __restrict__ work, int32_t num_nodes,
+                                                        const uint32_t* __restrict__ tile_start, SampleParams P) {
+  __shared__ __attribute__((aligned(16))) uint8_t s_stage[kWaves][kTileStage];
+  __shared__ uint8_t s_list[kWaves][3][64];  // accepted lanes, rejected lanes, all lanes
+  __shared__ int64_t s_j[kWaves][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int num = 8 * num_nodes;
+  const uint32_t total = tile_start[num];
   const int bpp = P.bpp;
   const bool avg = P.method == METHOD_POISSON_AVERAGE;
-  const int rgb0 = P.rgb_offset, rgb1 = P.rgb_offset + 6;
-  for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
-    int lo = 0, hi = num_segs - 1;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  uint8_t* stage = s_stage[wave];
+  uint8_t* accList = s_list[wave][0];
+  uint8_t* rejList = s_list[wave][1];
+  uint8_t* allList = s_list[wave][2];
+  allList[lane] = (uint8_t)lane;
+  for (uint32_t item = blockIdx.x * kWaves + wave; item < total; item += gridDim.x * kWaves) {
+    int lo = 0, hi = num - 1;
     while (lo < hi) {
       int mid = (lo + hi + 1) >> 1;
-      if (seg_start[mid] <= item) lo = mid; else hi = mid - 1;
+      if (tile_start[mid] <= item) lo = mid; else hi = mid - 1;
     }
-    const int si = __builtin_amdgcn_readfirstlane(lo);
-    const OutSeg sg = segs[si];
-    const NodeWork& w = work[sg.k];
-    int64_t cs[9];
-    const uint8_t* csrc[8];
-#pragma unroll
-    for (int c = 0; c < 9; c++) cs[c] = w.child_start[c];
-#pragma unroll
-    for (int c = 0; c < 8; c++) csrc[c] = w.csrc[c];
+    const int q = __builtin_amdgcn_readfirstlane(lo);
+    const int k = q >> 3, c = q & 7;
+    const NodeWork& w = work[k];
     const Layout L = make_layout(w.scratch, w.n, P.method);
-    const uint64_t seg_bytes = (uint64_t)sg.count * bpp;
-    const uint64_t dw0 = (uint64_t)(item - seg_start[si]) * (kBlock * kCopyDwords);
-    // source record of output record r: pointer and main index
-    auto resolve = [&](uint64_t r, int64_t& j) -> const uint8_t* {
-      int c;
-      int64_t local;
-      if (sg.child < 0) {
-        j = sg.sel[r];
-        c = 0;
-#pragma unroll
-        for (int q = 1; q < 8; q++) c += j >= cs[q] ? 1 : 0;
-        local = j - cs[c];
-      } else {
-        c = sg.child;
-        local = sg.sel ? (int64_t)sg.sel[r] : (int64_t)r;
-        j = cs[c] + local;
-      }
-      const uint8_t* base = csrc[0];
-#pragma unroll
-      for (int q = 1; q < 8; q++) base = c == q ? csrc[q] : base;
-      return base + local * bpp;
-    };
-#pragma unroll
-    for (int i = 0; i < kCopyDwords; i++) {
-      const uint64_t o = (dw0 + (uint64_t)i * kBlock + tid) * 4;
-      if (o >= seg_bytes) break;
-      uint64_t rr = o < 0xFFFFFFFFull ? (uint64_t)((uint32_t)o / (uint32_t)bpp) : o / (uint64_t)bpp;
-      int bb = (int)(o - rr * (uint64_t)bpp);
-      int64_t j;
-      const uint8_t* src = resolve(rr, j);
-      bool patch = avg && L.acc[L.keyB[j]];
-      uint32_t word = 0;
-      for (int t = 0; t < 4; t++) {
-        if (bb == bpp) {
-          rr++;
-          bb = 0;
-          if (rr >= (uint64_t)sg.count) break;
-          src = resolve(rr, j);
-          patch = avg && L.acc[L.keyB[j]];
-        }
-        uint32_t byte = src[bb];
-        if (patch && bb >= rgb0 && bb < rgb1) {
-          const uint16_t v = L.rgb[3 * j + (bb - rgb0) / 2];
-          byte = ((bb - rgb0) & 1) ? (v >> 8) : (v & 0xFF);
-        }
-        word |= byte << (8 * t);
-        bb++;
-      }
-      *(uint32_t*)(sg.dst + o) = word;
+    const uint32_t t = item - tile_start[q];
+    const int64_t cnt_c = w.child_start[c + 1] - w.child_start[c];
+    const int cnt = (int)((cnt_c - (int64_t)t * 64) < 64 ? (cnt_c - (int64_t)t * 64) : 64);
+    // stage the tile's records with 16-byte loads
+    const uint8_t* src = w.csrc[c] + (uint64_t)t * 64 * bpp;
+    const uint32_t b0 = (uint32_t)((uintptr_t)src & 15);
+    const uint8_t* a0 = src - b0;
+    const int nvec = (int)((b0 + (uint32_t)cnt * bpp + 15) >> 4);
+    for (int v = lane; v < nvec; v += 64) *(uint4*)(stage + 16 * v) = *(const uint4*)(a0 + 16 * (uint64_t)v);
+    const bool valid = lane < cnt;
+    const int64_t j = w.child_start[c] + (int64_t)t * 64 + lane;
+    const bool a = valid && L.acc[L.keyB[j]];
+    const uint64_t accm = __ballot(a), rejm = __ballot(valid && !a);
+    if (a) accList[__popcll(accm & lt)] = (uint8_t)lane;
+    if (valid && !a) rejList[__popcll(rejm & lt)] = (uint8_t)lane;
+    s_j[wave][lane] = j;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    // the node's accepted records before this tile and before the child
+    int64_t tb = 0;
+    for (int x = 0; x < c; x++) tb += (w.child_start[x + 1] - w.child_start[x] + 63) / 64;
+    const uint32_t accBefore = L.tileAcc[tb + t];
+    const uint32_t accChild = L.tileAcc[tb];
+    const OutSeg* sg = P.segs + 9 * (int64_t)k;
+    const uint8_t* stg = stage + b0;
+    write_run(sg[0].dst, accBefore, (uint32_t)__popcll(accm), stg, accList, bpp, avg ? L.rgb : nullptr, s_j[wave],
+              accm, P.rgb_offset);
+    uint8_t* rd = sg[1 + c].dst;
+    if (rd) {
+      if (!avg)
+        write_run(rd, (uint64_t)t * 64 - (accBefore - accChild), (uint32_t)__popcll(rejm), stg, rejList, bpp, nullptr,
+                  nullptr, 0ull, 0);
+      else
+        write_run(rd, (uint64_t)t * 64, (uint32_t)cnt, stg, allList, bpp, L.rgb, s_j[wave], accm, P.rgb_offset);
     }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   }
 }
 
@@ -1836,6 +1868,10 @@ namespace pcg {
 int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_info, NodeState* d_state,
                  NodeWork* d_work, const SampleParams& p, hipStream_t s) {
   if (num_nodes == 0) return 0;
+  if (p.bpp > 130) {
+    set_error("record size above 130 bytes is not supported by the sampler");
+    return 1;
+  }
   dim3 grid((unsigned)num_nodes);
   hipLaunchKernelGGL(k_prepare, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_state, d_work, p);
   PCG_CHECK(hipGetLastError());
@@ -1864,10 +1900,9 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   PCG_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_route, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_state, d_work, p);
   PCG_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_seg_plan, dim3(1), dim3(1024), 0, s, p.segs, 9 * num_nodes, p.bpp, p.seg_start);
+  hipLaunchKernelGGL(k_tile_plan, dim3(1), dim3(1024), 0, s, d_work, num_nodes, p.seg_start);
   PCG_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_copy_out, dim3(kCopyBlocks), dim3(kBlock), 0, s, p.segs, 9 * num_nodes, p.seg_start, d_work,
-                     p);
+  hipLaunchKernelGGL(k_copy_tiles, dim3(kCopyBlocks), dim3(kBlock), 0, s, d_work, num_nodes, p.seg_start, p);
   PCG_CHECK(hipGetLastError());
   return 0;
 }
@@ -1884,25 +1919,31 @@ uint64_t sample_scratch_bound(int64_t candidates, int64_t max_nodes, int method)
 }
 
 namespace {
-// byte ranges are arbitrary (records of any size): copy 4-byte words when
-// source and destination share their alignment, bytes otherwise
+// Byte ranges are arbitrary (records of any size): aligned dword stores
+// into the destination, each assembled from the two aligned source dwords
+// that cover it (byte align by the source/destination offset difference);
+// bytes only at the edges of a range.  Sources are 16-byte aligned arena
+// ranges, so the dword reads around a range stay inside its allocation.
 __global__ __launch_bounds__(256) void k_gather_ranges(const CopyDesc* __restrict__ desc, uint8_t* __restrict__ dst) {
   const CopyDesc d = desc[blockIdx.x];
   const uint8_t* s = d.src;
   uint8_t* o = dst + d.dst_offset;
-  uint64_t n = d.bytes;
-  if ((((uintptr_t)s ^ (uintptr_t)o) & 3) == 0) {
-    uint64_t head = (4 - ((uintptr_t)s & 3)) & 3;
-    if (head > n) head = n;
-    if (threadIdx.x < head) o[threadIdx.x] = s[threadIdx.x];
-    uint64_t words = (n - head) / 4;
-    const uint32_t* s4 = (const uint32_t*)(s + head);
-    uint32_t* o4 = (uint32_t*)(o + head);
-    for (uint64_t i = threadIdx.x; i < words; i += blockDim.x) o4[i] = s4[i];
-    for (uint64_t i = head + words * 4 + threadIdx.x; i < n; i += blockDim.x) o[i] = s[i];
-  } else {
-    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) o[i] = s[i];
+  const uint64_t n = d.bytes;
+  if (n == 0) return;
+  const uint64_t head = (4 - ((uintptr_t)o & 3)) & 3;  // bytes before the first aligned destination word
+  const uint64_t h = head < n ? head : n;
+  if (threadIdx.x < h) o[threadIdx.x] = s[threadIdx.x];
+  const uint64_t words = (n - h) / 4;
+  const uint8_t* sw = s + h;  // source of the first aligned destination word
+  const uint32_t shift = (uint32_t)((uintptr_t)sw & 3);
+  const uint32_t* sa = (const uint32_t*)(sw - shift);
+  uint32_t* o4 = (uint32_t*)(o + h);
+  for (uint64_t i = threadIdx.x; i < words; i += blockDim.x) {
+    const uint32_t lo = sa[i];
+    const uint32_t hi = shift ? sa[i + 1] : 0u;
+    o4[i] = __builtin_amdgcn_alignbyte(hi, lo, shift);
   }
+  for (uint64_t i = h + words * 4 + threadIdx.x; i < n; i += blockDim.x) o[i] = s[i];
 }
 }  // namespace
 
