@@ -223,7 +223,8 @@ def main():
         "phases_s": {
             "chunking": round(last[2], 4), "count": round(last[0], 4), "lut+distribute": round(last[1], 4),
             "indexing": round(last[3], 4), "octree_build": round(last[4], 4), "sampling": round(last[5], 4),
-            "octree_assembly": round(last[6], 4),
+            "octree_assembly": round(last[6], 4), "host_tree_setup": round(last[21], 4),
+            "host_completion_order": round(last[22], 4), "host_hierarchy_metadata": round(last[23], 4),
         },
         "counters": {
             "chunks": int(last[16]), "nodes": int(last[8]), "sampled_nodes": int(last[9]),

@@ -82,7 +82,9 @@ int pcg_count_cells(const void* d_las, int64_t n, int32_t bpp, const double las_
  * [11] FP64 re-checks, [12] FP32 pair tests, [13] max FP32 relative error
  * of re-checked pairs, [14] nodes sorted in libstdc++ order, [15] nodes
  * resolved sequentially, [16] chunks, [17] octree rounds, [18] depth,
- * [19] counting-pass kernel ms, [20] distribute kernels ms (HIP events). */
+ * [19] counting-pass kernel ms, [20] distribute kernels ms (HIP events),
+ * [21] host tree set-up s, [22] host completion order s, [23] hierarchy.bin
+ * and metadata.json s. */
 int pcg_convert_device(const void* d_las, int64_t n, const uint8_t* las_header, const char* method,
                        const char* outdir, double* stats);
 

@@ -175,6 +175,9 @@ int pcg_convert_device(const void* d_las, int64_t n, const uint8_t* las_header, 
     stats[18] = (double)r.octreeDepth;
     stats[19] = (double)dc.ms_count_kernel;
     stats[20] = (double)dc.ms_distribute_kernels;
+    stats[21] = r.t_setup;
+    stats[22] = r.t_order;
+    stats[23] = r.t_hierarchy;
   }
   dc.release();
   r.release();

@@ -483,14 +483,18 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     PCG_CHECK(hipMemcpyAsync(dcopies.p, copies.data(), copies.size() * sizeof(CopyDesc), hipMemcpyHostToDevice, s));
     if (gather_ranges(dcopies.as<CopyDesc>(), (int64_t)copies.size(), out.d_octree, s)) return 1;
   }
+  const double ta0 = now_s();
   PCG_CHECK(hipStreamSynchronize(s));
   const double ta1 = now_s();
   out.t_assemble = ta1 - ts1;
+  out.t_order = ta0 - ts1;
 
   // root name is the chunk id when there is a single chunk: keep its name
   out.hierarchy = build_hierarchy_bin(out.completed, out.firstChunkSize);
   out.metadata = create_metadata_json(attributes, dc.min, dc.max, spacing, name, pointsTotal, out.firstChunkSize,
                                       octreeDepth);
+  out.t_hierarchy = now_s() - ta1;
+  out.t_setup = tb0 - t0;
   out.t_total = now_s() - t0;
   return 0;
 }

@@ -26,6 +26,7 @@ struct IndexResult {
   int64_t octreeDepth = 0;
   int64_t firstChunkSize = 0;
   double t_build = 0, t_sample = 0, t_assemble = 0, t_total = 0;
+  double t_setup = 0, t_order = 0, t_hierarchy = 0;  // host: tree, completion order + copy list, hierarchy + metadata
   uint64_t counters[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int64_t numNodes = 0, numSampled = 0, buildRounds = 0;
   void release();
