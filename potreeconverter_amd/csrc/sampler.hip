@@ -47,6 +47,17 @@ __device__ inline int32_t ld_i32(const uint8_t* p) {
 }
 __device__ inline uint16_t ld_u16(const uint8_t* p) { return (uint16_t)(p[0] | (p[1] << 8)); }
 
+// X, Y, Z (3 x int32 at any alignment) from four aligned dword loads; the
+// record buffers keep at least 3 bytes of their allocation after a record
+__device__ inline void ld_xyz(const uint8_t* p, int32_t& X, int32_t& Y, int32_t& Z) {
+  const uint32_t s = (uint32_t)((uintptr_t)p & 3);
+  const uint32_t* w = (const uint32_t*)(p - s);
+  const uint32_t w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
+  X = (int32_t)__builtin_amdgcn_alignbyte(w1, w0, s);
+  Y = (int32_t)__builtin_amdgcn_alignbyte(w2, w1, s);
+  Z = (int32_t)__builtin_amdgcn_alignbyte(w3, w2, s);
+}
+
 __host__ __device__ inline uint64_t align16(uint64_t v) { return (v + 15) & ~15ull; }
 
 __host__ __device__ inline uint32_t hash_cap(int64_t n, int method) {
@@ -359,7 +370,8 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
     int c = 0;
     while (c < 7 && j >= s_cs[c + 1]) c++;
     const uint8_t* r = s_cptr[c] + (j - s_cs[c]) * P.bpp;
-    int32_t X = ld_i32(r), Y = ld_i32(r + 4), Z = ld_i32(r + 8);
+    int32_t X, Y, Z;
+    ld_xyz(r, X, Y, Z);
     double x = coord(X, P.scale[0], P.offset[0]), y = coord(Y, P.scale[1], P.offset[1]),
            z = coord(Z, P.scale[2], P.offset[2]);
     double ax = x - g.cx, ay = y - g.cy, az = z - g.cz;
@@ -381,44 +393,92 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
     }
   }
   __syncthreads();
-  // stable LSD radix sort on float(add) bits, 4 x 8 bits
+  // stable LSD radix sort on float(add) bits
   uint32_t *kc = L.keyA, *ic = L.idxA, *ko = L.keyB, *io = L.idxB;
+  // the float key, 4 passes; runs of equal keys (distinct doubles that round
+  // to the same float, or equal distances) are put in (double add, main
+  // index) order below
   for (int pass = 0; pass < 4; pass++) {
     if (!block_radix_pass(kc, ic, ko, io, n, 8 * pass, 255u, s_rad)) continue;
     uint32_t* t = kc; kc = ko; ko = t;
     t = ic; ic = io; io = t;
   }
-  // runs of equal float keys: order by (double add, main index)
-  for (int64_t p = tid; p < n; p += kBlock) {
-    bool start = (p == 0 || kc[p - 1] != kc[p]) && p + 1 < n && kc[p + 1] == kc[p];
-    if (!start) continue;
-    int64_t e = p + 1;
-    while (e < n && kc[e] == kc[p]) e++;
-    for (int64_t i = p + 1; i < e; i++) {
-      uint32_t v = ic[i];
-      double dv = L.add[v];
-      int64_t q = i - 1;
-      while (q >= p && (L.add[ic[q]] > dv || (L.add[ic[q]] == dv && ic[q] > v))) {
-        ic[q + 1] = ic[q];
-        q--;
+  // runs of equal keys: order by (double add, main index); run starts are
+  // found four positions per thread at a time so that the key loads are in
+  // flight together
+  for (int64_t b0 = 0; b0 < n; b0 += 4 * kBlock) {
+    uint32_t kpv[4], kcur[4], knx[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = b0 + u * kBlock + tid;
+      kcur[u] = p < n ? kc[p] : 0u;
+      kpv[u] = (p > 0 && p < n) ? kc[p - 1] : ~0u;
+      knx[u] = p + 1 < n ? kc[p + 1] : ~0u;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = b0 + u * kBlock + tid;
+      if (p >= n || kpv[u] == kcur[u] || knx[u] != kcur[u]) continue;
+      const uint32_t kp = kcur[u];
+      int64_t e = p + 1;
+      while (e < n && kc[e] == kp) e++;
+      for (int64_t i = p + 1; i < e; i++) {
+        uint32_t v = ic[i];
+        double dv = L.add[v];
+        int64_t q = i - 1;
+        while (q >= p && (L.add[ic[q]] > dv || (L.add[ic[q]] == dv && ic[q] > v))) {
+          ic[q + 1] = ic[q];
+          q--;
+        }
+        ic[q + 1] = v;
       }
-      ic[q + 1] = v;
     }
   }
   __syncthreads();
-  if (ic != L.idxA)
-    for (int64_t p = tid; p < n; p += kBlock) L.idxA[p] = ic[p];
-  __syncthreads();
-  for (int64_t p = tid; p < n; p += kBlock) L.sadd[p] = L.add[L.idxA[p]];
+  // sorted positions and distances
+  for (int64_t b0 = 0; b0 < n; b0 += 4 * kBlock) {
+    uint32_t v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = b0 + u * kBlock + tid;
+      v[u] = p < n ? ic[p] : 0u;
+    }
+    double d[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = b0 + u * kBlock + tid;
+      d[u] = p < n ? L.add[v[u]] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = b0 + u * kBlock + tid;
+      if (p < n) {
+        L.sadd[p] = d[u];
+        if (ic != L.idxA) L.idxA[p] = v[u];
+      }
+    }
+  }
   __syncthreads();
   // equal distances: the reference's unstable sort decides their order;
   // it matters only if two of them can exclude each other
   bool conflict = false;
-  for (int64_t p = tid; p + 1 < n && !conflict; p += kBlock) {
-    if (!(L.sadd[p] == L.sadd[p + 1]) || (p > 0 && L.sadd[p - 1] == L.sadd[p])) continue;
-    int64_t e = p + 1;
-    while (e < n && L.sadd[e] == L.sadd[p]) e++;
-    conflict = tie_group_conflicts(L, g, P, p, e);
+  for (int64_t b0 = 0; b0 + 1 < n && !conflict; b0 += 4 * kBlock) {
+    double dp[4], dc[4], dn[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = b0 + u * kBlock + tid;
+      dc[u] = p < n ? L.sadd[p] : -1.0;
+      dn[u] = p + 1 < n ? L.sadd[p + 1] : -2.0;
+      dp[u] = (p > 0 && p < n) ? L.sadd[p - 1] : -3.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = b0 + u * kBlock + tid;
+      if (conflict || p + 1 >= n || !(dc[u] == dn[u]) || dp[u] == dc[u]) continue;
+      int64_t e = p + 1;
+      while (e < n && L.sadd[e] == dc[u]) e++;
+      conflict = tie_group_conflicts(L, g, P, p, e);
+    }
   }
   if (conflict) atomicOr(&work[k].flags, (uint32_t)NW_EXACT_SORT);
 }
@@ -1084,27 +1144,52 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
   const int64_t words = (n + 31) / 32;
   for (int64_t i = lane; i < words; i += 64) bits[i] = 0;
   if (!lds) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+  // the next window's neighbour rows are loaded while this one is decided
+  uint4 nr0 = make_uint4(0, 0, 0, 0), nr1 = nr0, nr2 = nr0, nr3 = nr0;
+  int ncnt = 0;
+  double nsadd = 0.0;
+  auto prefetch = [&](int64_t p) {
+    ncnt = 0;
+    nsadd = 0.0;
+    if (p < n) {
+      ncnt = L.nbrCount[p];
+      nsadd = L.sadd[p];
+      const uint4* rw = (const uint4*)(L.nbr + (uint64_t)p * kMaxNbr);
+      nr0 = rw[0]; nr1 = rw[1]; nr2 = rw[2]; nr3 = rw[3];
+    }
+  };
+  prefetch(lane);
   for (int64_t w0 = 0; w0 < n; w0 += 64) {
     const int64_t p = w0 + lane;
     const bool valid = p < n;
     uint64_t in_mask = 0;
     bool rej = false;
     double limSq = 1e308;
+    const int cnt = ncnt;
+    const double sa = nsadd;
+    const uint32_t row[16] = {nr0.x, nr0.y, nr0.z, nr0.w, nr1.x, nr1.y, nr1.z, nr1.w,
+                              nr2.x, nr2.y, nr2.z, nr2.w, nr3.x, nr3.y, nr3.z, nr3.w};
+    prefetch(p + 64);
     if (valid) {
       if (!avg) {
-        double cd = __builtin_sqrt(L.sadd[p]);
+        double cd = __builtin_sqrt(sa);
         double lim = cd - g.spacing;
         limSq = lim * lim;
       }
-      const int cnt = L.nbrCount[p];
-      const uint32_t* row = L.nbr + (uint64_t)p * kMaxNbr;
-      if (cnt != kNbrOverflow) {
-        const uint32_t* lst = row;
-        int len = cnt;
-        if (cnt == kNbrPooled) {
-          lst = P.nbr_pool + ((uint64_t)row[0] | ((uint64_t)row[1] << 32));
-          len = (int)row[2];
+      if (cnt != kNbrOverflow && cnt != kNbrPooled) {
+#pragma unroll
+        for (int i = 0; i < kMaxNbr; i++) {
+          if (i >= cnt || rej) continue;
+          const uint32_t q = row[i];
+          if ((int64_t)q >= w0) {
+            in_mask |= 1ull << (q - w0);
+          } else if ((bits[q >> 5] >> (q & 31)) & 1u) {
+            rej = true;
+          }
         }
+      } else if (cnt == kNbrPooled) {
+        const uint32_t* lst = P.nbr_pool + ((uint64_t)row[0] | ((uint64_t)row[1] << 32));
+        const int len = (int)row[2];
         for (int i = 0; i < len; i++) {
           const uint32_t q = lst[i];
           if ((int64_t)q >= w0) {
