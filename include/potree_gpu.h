@@ -96,7 +96,9 @@ int pcg_copy(void* dst, const void* src, uint64_t bytes, int32_t kind);
 int pcg_synchronize(void);
 
 /* Synthetic LAS generator (not part of the reference): preset 0 = 500 m
- * tile, 1 = 5 km tile in UTM-like coordinates.  `params` points at a
+ * tile, 1 = 5 km tile in UTM-like coordinates, 2 = precision scene (scale
+ * 0.0001, offset 1e6 m, near-duplicate clusters, a lattice whose pitch is a
+ * node spacing).  `params` points at a
  * caller-owned buffer of at least 256 bytes. */
 int pcg_lasgen_preset(int32_t preset, int64_t n, uint64_t seed, void* params);
 int32_t pcg_lasgen_record_length(const void* params);

@@ -63,7 +63,9 @@ def _check(rc):
 
 class LasGen:
     """Parameters of the seeded synthetic LAS generator (preset 0: 500 m tile,
-    preset 1: 5 km tile at UTM-like coordinates)."""
+    preset 1: 5 km tile at UTM-like coordinates, preset 2: precision scene with
+    scale 0.0001, offset 1e6 m, near-duplicate clusters and a lattice whose
+    pitch equals the level-4 node spacing)."""
 
     def __init__(self, preset, n, seed=0, extra_bytes=False):
         self.buf = ctypes.create_string_buffer(256)

@@ -15,6 +15,17 @@
 // (building).  The output record is a LAS point data record format 2 (26 B),
 // followed by 7 extra bytes (float32 N(0,1), uint16, int8) when
 // `extra_bytes` is set.
+//
+// Precision scenes (`span_units` > 0) add, in LAS integer units:
+//   - two anchor points (records 0 and 1) at the corners q0 and q0 + span of
+//     the X/Y range, so the cube edge is exactly `span_units` units;
+//   - near-duplicate clusters: a `dup_fraction` share of the points sits on
+//     one of `num_clusters` quantized centres with an integer jitter of
+//     -1, 0 or +1 unit per axis (exact duplicates are frequent);
+//   - a flat square lattice: a `lattice_fraction` share of the points sits on
+//     the nodes of a `lattice_side`^2 grid of pitch `lattice_pitch_units`,
+//     chosen equal to a node spacing so that neighbour distances tie with it
+//     up to binary64 rounding.
 #pragma once
 
 #include <stdint.h>
@@ -39,6 +50,13 @@ struct LasGenParams {
   int32_t tiles_per_side; // filled by lasgen_prepare
   int64_t num_tiles;      // filled by lasgen_prepare
   double wall_fraction;   // fraction of points on building walls
+  // precision scene (all zero for the plain scenes)
+  double dup_fraction;          // share of points in near-duplicate clusters
+  double lattice_fraction;      // share of points on the lattice
+  int64_t span_units;           // X/Y range in LAS units (0: no anchors)
+  int64_t num_clusters;         // filled by lasgen_prepare
+  int64_t lattice_pitch_units;  // lattice pitch in LAS units
+  int64_t lattice_side;         // filled by lasgen_prepare
 };
 
 struct LasGenBuilding {
@@ -132,6 +150,17 @@ inline void lasgen_prepare(LasGenParams& p) {
     if (mid * mid >= p.num_tiles) hi = mid; else lo = mid + 1;
   }
   p.tiles_per_side = (int32_t)lo;
+  p.num_clusters = 0;
+  p.lattice_side = 0;
+  if (p.span_units > 0) {
+    int64_t nd = (int64_t)((double)p.n * p.dup_fraction);
+    p.num_clusters = nd / 16 > 0 ? nd / 16 : 1;
+    int64_t nl = (int64_t)((double)p.n * p.lattice_fraction);
+    int64_t side = 1;
+    while (side * side < nl) side++;
+    int64_t cap = p.lattice_pitch_units > 0 ? (p.span_units / 2) / p.lattice_pitch_units : 1;
+    p.lattice_side = side < cap ? side : (cap > 0 ? cap : 1);
+  }
 }
 
 LASGEN_HD int32_t lasgen_quantize(double world, double offset, double inv_scale) {
@@ -156,7 +185,51 @@ LASGEN_HD void lasgen_point(const LasGenParams& p, const LasGenBuilding* bld, in
   uint8_t cls = 2;
   uint64_t ui = (uint64_t)i;
   int64_t tile = i / p.tile_points;
-  if (lasgen_u01(lasgen_hash(s, ui, 0)) < p.wall_fraction && p.num_buildings > 0) {
+  // precision scenes: 1 = anchor, 2 = near-duplicate cluster, 3 = lattice
+  int32_t mode = 0;
+  if (p.span_units > 0) {
+    double sel = lasgen_u01(lasgen_hash(s, ui, 6));
+    if (i < 2) mode = 1;
+    else if (sel < p.dup_fraction) mode = 2;
+    else if (sel < p.dup_fraction + p.lattice_fraction) mode = 3;
+  }
+  int32_t X, Y, Z;
+  int32_t qx0 = lasgen_quantize(p.origin[0], p.offset[0], p.inv_scale[0]);
+  int32_t qy0 = lasgen_quantize(p.origin[1], p.offset[1], p.inv_scale[1]);
+  if (mode == 1) {
+    int32_t d = (int32_t)(i ? p.span_units : 0);
+    x = i ? p.extent : 0.0;
+    y = x;
+    z = lasgen_terrain(p, x, y);
+    X = qx0 + d;
+    Y = qy0 + d;
+    Z = lasgen_quantize(p.origin[2] + z, p.offset[2], p.inv_scale[2]);
+  } else if (mode == 2) {
+    uint64_t c = (uint64_t)(lasgen_u01(lasgen_hash(s, ui, 7)) * (double)p.num_clusters);
+    x = lasgen_u01(lasgen_hash(s ^ 0xC1A5ull, c, 0)) * p.extent * 0.999;
+    y = lasgen_u01(lasgen_hash(s ^ 0xC1A5ull, c, 1)) * p.extent * 0.999;
+    z = lasgen_terrain(p, x, y);
+    uint64_t hj = lasgen_hash(s, ui, 8);
+    X = lasgen_quantize(p.origin[0] + x, p.offset[0], p.inv_scale[0]) + (int32_t)(hj % 3) - 1;
+    Y = lasgen_quantize(p.origin[1] + y, p.offset[1], p.inv_scale[1]) + (int32_t)((hj >> 8) % 3) - 1;
+    Z = lasgen_quantize(p.origin[2] + z, p.offset[2], p.inv_scale[2]) + (int32_t)((hj >> 16) % 3) - 1;
+    if (X < qx0 + 1) X = qx0 + 1;
+    if (Y < qy0 + 1) Y = qy0 + 1;
+  } else if (mode == 3) {
+    int64_t M = p.lattice_side;
+    int64_t k = (int64_t)(lasgen_u01(lasgen_hash(s, ui, 7)) * (double)(M * M));
+    int64_t a = k % M, b = k / M;
+    int64_t x0 = p.span_units / 4, y0 = p.span_units / 4;
+    x = (double)x0 / p.inv_scale[0];
+    y = (double)y0 / p.inv_scale[1];
+    z = lasgen_terrain(p, x, y);
+    X = qx0 + (int32_t)(x0 + a * p.lattice_pitch_units);
+    Y = qy0 + (int32_t)(y0 + b * p.lattice_pitch_units);
+    Z = lasgen_quantize(p.origin[2] + z, p.offset[2], p.inv_scale[2]);
+  }
+  if (mode != 0) {
+    // colours and attributes below follow z like the plain scene
+  } else if (lasgen_u01(lasgen_hash(s, ui, 0)) < p.wall_fraction && p.num_buildings > 0) {
     int32_t b = (int32_t)(lasgen_u01(lasgen_hash(s, ui, 1)) * (double)p.num_buildings);
     int32_t side = (int32_t)(lasgen_u01(lasgen_hash(s, ui, 2)) * 4.0);
     double t = lasgen_u01(lasgen_hash(s, ui, 3));
@@ -186,15 +259,16 @@ LASGEN_HD void lasgen_point(const LasGenParams& p, const LasGenBuilding* bld, in
       }
     }
   }
-  double lim = p.extent * 0.999999;
-  if (x < 0.0) x = 0.0;
-  if (x > lim) x = lim;
-  if (y < 0.0) y = 0.0;
-  if (y > lim) y = lim;
-
-  int32_t X = lasgen_quantize(p.origin[0] + x, p.offset[0], p.inv_scale[0]);
-  int32_t Y = lasgen_quantize(p.origin[1] + y, p.offset[1], p.inv_scale[1]);
-  int32_t Z = lasgen_quantize(p.origin[2] + z, p.offset[2], p.inv_scale[2]);
+  if (mode == 0) {
+    double lim = p.extent * 0.999999;
+    if (x < 0.0) x = 0.0;
+    if (x > lim) x = lim;
+    if (y < 0.0) y = 0.0;
+    if (y > lim) y = lim;
+    X = lasgen_quantize(p.origin[0] + x, p.offset[0], p.inv_scale[0]);
+    Y = lasgen_quantize(p.origin[1] + y, p.offset[1], p.inv_scale[1]);
+    Z = lasgen_quantize(p.origin[2] + z, p.offset[2], p.inv_scale[2]);
+  }
 
   uint64_t ha = lasgen_hash(s, ui, 5);
   uint16_t intensity = (uint16_t)(ha & 0xFFFF);

@@ -17,6 +17,9 @@ extern "C" {
 // Fills `params` for one of the named workloads; returns 0 on success.
 //   0: 500 m tile, offset 0, scale 0.001 (CPU-runnable config)
 //   1: 5 km tile in UTM-like coordinates (offset 500000, 5000000, 0)
+//   2: precision scene: 819.2 m tile, scale 0.0001, offset (1e6, 1e6, 0),
+//      10% of the points in near-duplicate clusters, 5% on a lattice whose
+//      pitch (0.4 m) is the node spacing at level 4 (819.2 / 128 / 2^4)
 int pcg_lasgen_preset(int32_t preset, int64_t n, uint64_t seed, LasGenParams* p) {
   memset(p, 0, sizeof(*p));
   p->seed = seed;
@@ -33,6 +36,15 @@ int pcg_lasgen_preset(int32_t preset, int64_t n, uint64_t seed, LasGenParams* p)
     p->extent = 5000.0;
     p->origin[0] = p->offset[0] = 500000.0;
     p->origin[1] = p->offset[1] = 5000000.0;
+  } else if (preset == 2) {
+    scale = 0.0001;
+    p->extent = 819.2;
+    p->origin[0] = p->offset[0] = 1000000.0;
+    p->origin[1] = p->offset[1] = 1000000.0;
+    p->span_units = 8192000;
+    p->dup_fraction = 0.10;
+    p->lattice_fraction = 0.05;
+    p->lattice_pitch_units = 4000;
   } else {
     return 1;
   }

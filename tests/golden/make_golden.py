@@ -31,6 +31,9 @@ CASES = [
     ("c0_400k_extra_poisson_average", 0, 400_000, 5, True, "poisson_average"),
     ("c1_1m_utm_poisson", 1, 1_000_000, 1, False, "poisson"),
     ("c0_150k_poisson", 0, 150_000, 2, False, "poisson"),
+    ("c3_1m_precision_poisson", 2, 1_000_000, 3, False, "poisson"),
+    ("c3_1m_precision_poisson_average", 2, 1_000_000, 3, False, "poisson_average"),
+    ("c3_300k_precision_extra_poisson", 2, 300_000, 11, True, "poisson"),
 ]
 
 

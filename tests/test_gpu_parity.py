@@ -26,6 +26,8 @@ CASES = [
     (0, 3_000_000, 1, False),
     (0, 400_000, 5, True),
     (1, 1_000_000, 1, False),
+    (2, 1_000_000, 3, False),
+    (2, 300_000, 11, True),
 ]
 
 # (method, preset, points, seed) cases kept off the GPU: this one hung with an
@@ -127,9 +129,10 @@ def test_out_of_box_point_is_rejected_like_the_reference(tmp_path):
     assert rc == 123
 
 
-def test_gpu_generator_equals_cpu_generator():
+@pytest.mark.parametrize("preset", [1, 2])
+def test_gpu_generator_equals_cpu_generator(preset):
     L = pc.lib()
-    g = pc.LasGen(1, 1_000_000, seed=7)
+    g = pc.LasGen(preset, 1_000_000, seed=7)
     cpu = g.records()
     d = ctypes.c_void_p()
     assert L.pcg_device_alloc(ctypes.byref(d), len(cpu) + 64) == 0
@@ -180,7 +183,8 @@ def _set(key, value):
     assert rc == 0, pc.lib().pcg_last_error()
 
 
-RING_CASES = [(0, 150_000, 2, False), (0, 2_000_000, 7, False), (0, 400_000, 5, True), (1, 1_000_000, 1, False)]
+RING_CASES = [(0, 150_000, 2, False), (0, 2_000_000, 7, False), (0, 400_000, 5, True), (1, 1_000_000, 1, False),
+              (2, 1_000_000, 3, False)]
 
 
 @pytest.mark.parametrize("preset,n,seed,eb", RING_CASES, ids=[f"p{c[0]}_{c[1]}_s{c[2]}" for c in RING_CASES])
