@@ -101,6 +101,9 @@ int pcg_synchronize(void);
  * node spacing).  `params` points at a
  * caller-owned buffer of at least 256 bytes. */
 int pcg_lasgen_preset(int32_t preset, int64_t n, uint64_t seed, void* params);
+/* Record format of the generated points (0-7, presets use 2) and whether 7
+ * extra bytes (float32, uint16, int8) follow each record. */
+int pcg_lasgen_configure(void* params, int32_t point_format, int32_t extra_bytes);
 int32_t pcg_lasgen_record_length(const void* params);
 void pcg_lasgen_fill(const void* params, int64_t first, int64_t count, uint8_t* out);
 int pcg_lasgen_write(const void* params, const char* path);

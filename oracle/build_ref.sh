@@ -1,7 +1,8 @@
 #!/bin/bash
 # Builds the reference indexer (Converter/src/indexer.cpp with its samplers,
 # HierarchyBuilder and vendored brotli encoder) straight from the read-only
-# reference checkout, plus our ref_driver.cpp, into oracle/_ref/.
+# reference checkout, plus our ref_driver.cpp, into oracle/_ref/, and the
+# vendored LASzip raw item readers behind ref_point_decode.cpp.
 # The reference chunker is not built: it includes laszip_api.h, which needs
 # the CMake-generated laszip_api_version.h that the checkout does not hold.
 # Without TBB headers libstdc++ uses its serial PSTL backend, so the
@@ -33,3 +34,7 @@ done
 wait
 g++ -o "$OUT/ref_indexer" "${objs[@]}" -lpthread
 echo "built $OUT/ref_indexer"
+# LASzip's raw point readers (header-only part of the vendored laszip that
+# needs no generated header) behind a small decode driver
+g++ -O2 -std=gnu++17 -w -I"$CONV/libs/laszip/src" -o "$OUT/ref_point_decode" "$HERE/ref_point_decode.cpp"
+echo "built $OUT/ref_point_decode"

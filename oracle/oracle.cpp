@@ -153,8 +153,16 @@ static bool attributes_for_format(const lasfile::Header& h, std::vector<Attr>& l
     case 1: list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId, gpsTime}; break;
     case 2: list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId, rgb}; break;
     case 3: list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId, gpsTime, rgb}; break;
+    case 4: list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId, gpsTime,
+                    wpdi, bowd, wps, rpwl, xyzt}; break;
+    case 5: list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId, gpsTime,
+                    rgb, wpdi, bowd, wps, rpwl, xyzt}; break;
+    case 6: list = {xyz, intensity, returnNumber, numberOfReturns, cflags, classification, userData, scanAngle, pointSourceId,
+                    gpsTime}; break;
+    case 7: list = {xyz, intensity, returnNumber, numberOfReturns, cflags, classification, userData, scanAngle, pointSourceId,
+                    gpsTime, rgb}; break;
     default:
-      g_error = "oracle handles LAS point formats 0-3";
+      g_error = "currently unsupported LAS format: " + std::to_string(h.point_format);
       return false;
   }
   for (auto& e : h.extra) {
@@ -331,33 +339,106 @@ extern "C" int64_t orc_create_lut(const int32_t* grid, int32_t grid_size, int64_
 
 namespace orc {
 
-// Record transcode for point formats 0-3 + extra bytes, with per-attribute
+// The fields of laszip's point struct that the chunker's handlers read
+// (laszip_api.h:189-226), filled the way laszip's raw item readers fill
+// them from an uncompressed record (lasreaditemraw.hpp: POINT10 copies 20
+// bytes, GPSTIME11 8, RGB12 / RGB14 6, WAVEPACKET13 29; POINT14 maps the
+// 30-byte point of formats 6-10 onto the legacy fields, :213-258).  The
+// struct starts zeroed (laszip_dll.cpp:166), so formats 0-5 leave the
+// extended fields 0.
+struct LzPoint {
+  uint16_t intensity = 0;
+  uint8_t return_number = 0, number_of_returns = 0;  // 3-bit fields
+  uint8_t classification = 0;                       // 5-bit field
+  int8_t scan_angle_rank = 0;
+  uint8_t user_data = 0;
+  uint16_t point_source_ID = 0;
+  int16_t extended_scan_angle = 0;
+  uint8_t extended_classification_flags = 0, extended_classification = 0;
+  double gps_time = 0;
+  uint16_t rgb[3] = {0, 0, 0};
+};
+
+static LzPoint laszip_decode(int fmt, const uint8_t* r) {
+  LzPoint p;
+  memcpy(&p.intensity, r + 12, 2);
+  if (fmt <= 5) {
+    p.return_number = r[14] & 7;
+    p.number_of_returns = (r[14] >> 3) & 7;
+    p.classification = r[15] & 31;
+    p.scan_angle_rank = (int8_t)r[16];
+    p.user_data = r[17];
+    memcpy(&p.point_source_ID, r + 18, 2);
+    int o = 20;
+    if (fmt == 1 || fmt == 3 || fmt == 4 || fmt == 5) { memcpy(&p.gps_time, r + o, 8); o += 8; }
+    if (fmt == 2 || fmt == 3 || fmt == 5) memcpy(p.rgb, r + o, 6);
+    return p;
+  }
+  // point 14: 4-bit return number / number of returns, flags nibble,
+  // classification byte, user data, scan angle, point source id, gps time
+  const int rn = r[14] & 15, nr = r[14] >> 4;
+  if (nr > 7) {
+    p.return_number = rn > 6 ? (rn >= nr ? 7 : 6) : rn;
+    p.number_of_returns = 7;
+  } else {
+    p.return_number = rn & 7;  // a 3-bit field
+    p.number_of_returns = nr;
+  }
+  const uint8_t cls = r[16];
+  // legacy byte = flags << 5 | class (if < 32); the 5-bit field is its low bits
+  p.classification = cls < 32 ? cls : 0;
+  p.extended_classification_flags = r[15] & 15;
+  p.extended_classification = cls;
+  p.user_data = r[17];
+  memcpy(&p.extended_scan_angle, r + 18, 2);
+  // legacy rank: clamp to int8 of the scan angle * 0.006 (float) rounded half
+  // away from zero (no attribute of formats 6-7 reads it)
+  const float deg = 0.006f * (float)p.extended_scan_angle;
+  const int16_t qd = deg >= 0 ? (int16_t)(deg + 0.5) : (int16_t)(deg - 0.5);
+  p.scan_angle_rank = (int8_t)(qd <= -128 ? -128 : (qd >= 127 ? 127 : qd));
+  memcpy(&p.point_source_ID, r + 20, 2);
+  memcpy(&p.gps_time, r + 22, 8);
+  if (fmt == 7) memcpy(p.rgb, r + 30, 6);
+  return p;
+}
+
+// Record transcode for point formats 0-7 + extra bytes, with per-attribute
 // min/max and the classification histogram:
 // chunker_countsort_laszip.cpp:380-658 (handlers), :770-803 (per point).
 struct Transcoder {
   int fmt = 2;
   int inBpp = 26;
   Attrs* out = nullptr;
-  int oIntensity = -1, oRet = -1, oNret = -1, oCls = -1, oSar = -1, oUd = -1, oPsid = -1, oGps = -1, oRgb = -1;
   int firstExtraIndex = 9, extraSrc = 0, extraDst = 0, extraBytes = 0;
+  // handler of each standard attribute (name lookup of :532-544 done once)
+  enum H { NONE, INTENSITY, RET, NRET, CLASS, SAR, SA, UD, PSID, GPS, CFLAGS, RGB };
+  std::vector<int> handler, offset;
 
   void init(const lasfile::Header& h, Attrs& o) {
     fmt = h.point_format;
     inBpp = h.record_length;
     out = &o;
-    oIntensity = o.offsetOf("intensity"); oRet = o.offsetOf("return number"); oNret = o.offsetOf("number of returns");
-    oCls = o.offsetOf("classification"); oSar = o.offsetOf("scan angle rank"); oUd = o.offsetOf("user data");
-    oPsid = o.offsetOf("point source id"); oGps = o.offsetOf("gps-time"); oRgb = o.offsetOf("rgb");
-    static const int core[] = {20, 28, 26, 34};  // legacy record size without extra bytes
+    static const int core[] = {20, 28, 26, 34, 57, 63, 30, 36};  // record size without extra bytes
     extraSrc = core[fmt];
-    // :571-577 the destination offset of the first extra attribute is the
-    // summed size of the standard attributes of the input format
-    static const int firstIdx[] = {8, 9, 9, 10};
+    // :561-577 the first extra attribute of each format and its destination
+    // offset: the summed size of the attributes before it
+    static const int firstIdx[] = {8, 9, 9, 10, 14, 15, 10, 11};
     firstExtraIndex = firstIdx[fmt];
     extraDst = 0;
     for (int i = 0; i < firstExtraIndex; i++) extraDst += o.list[i].size;
     extraBytes = 0;
     for (size_t i = firstExtraIndex; i < o.list.size(); i++) extraBytes += o.list[i].size;
+    static const std::pair<const char*, int> names[] = {
+        {"intensity", INTENSITY}, {"return number", RET}, {"number of returns", NRET}, {"classification", CLASS},
+        {"scan angle rank", SAR}, {"scan angle", SA}, {"user data", UD}, {"point source id", PSID},
+        {"gps-time", GPS}, {"classification flags", CFLAGS}, {"rgb", RGB}};
+    handler.assign(firstExtraIndex, NONE);
+    offset.assign(firstExtraIndex, 0);
+    for (int i = 1; i < firstExtraIndex; i++) {
+      for (auto& nm : names)
+        if (o.list[i].name == nm.first) handler[i] = nm.second;
+      offset[i] = o.offsetOf(o.list[i].name);
+    }
   }
 
   static void mm(Attr* a, double v) { a->min.x = std::min(a->min.x, v); a->max.x = std::max(a->max.x, v); }
@@ -381,50 +462,50 @@ struct Transcoder {
     Attr* pos = &o.list[0];
     pos->min.x = std::min(pos->min.x, xyz[0]); pos->min.y = std::min(pos->min.y, xyz[1]); pos->min.z = std::min(pos->min.z, xyz[2]);
     pos->max.x = std::max(pos->max.x, xyz[0]); pos->max.y = std::max(pos->max.y, xyz[1]); pos->max.z = std::max(pos->max.z, xyz[2]);
-    // handler order follows the input attribute list (:531-544)
-    uint16_t intensity; memcpy(&intensity, src + 12, 2);
-    uint8_t flags = src[14], cbyte = src[15];
-    int8_t sar = (int8_t)src[16];
-    uint8_t ud = src[17];
-    uint16_t psid; memcpy(&psid, src + 18, 2);
-    memcpy(dst + oIntensity, &intensity, 2); mm(&o.list[1], double(intensity));
-    uint8_t rn = flags & 7, nr = (flags >> 3) & 7;
-    dst[oRet] = rn; mm(o.get("return number"), double(rn));
-    dst[oNret] = nr; mm(o.get("number of returns"), double(nr));
-    // :472-486 legacy formats carry the 5-bit class; extended class is 0
-    uint8_t cls = cbyte & 0x1F;
-    Attr* ac = o.get("classification");
-    dst[oCls] = cls; ac->histogram[cls]++; mm(ac, double(cls));
-    dst[oSar] = (uint8_t)sar; mm(o.get("scan angle rank"), double(sar));
-    dst[oUd] = ud; mm(o.get("user data"), double(ud));
-    memcpy(dst + oPsid, &psid, 2); mm(o.get("point source id"), double(psid));
-    if (oGps >= 0) {
-      double g; memcpy(&g, src + 20, 8);
-      memcpy(dst + oGps, &g, 8); mm(o.get("gps-time"), g);
-    }
-    if (oRgb >= 0) {
-      int srcRgb = (fmt == 2) ? 20 : 28;
-      uint16_t c[3]; memcpy(c, src + srcRgb, 6);
-      memcpy(dst + oRgb, c, 6);
-      Attr* a = o.get("rgb");
-      a->min.x = std::min(a->min.x, double(c[0])); a->min.y = std::min(a->min.y, double(c[1])); a->min.z = std::min(a->min.z, double(c[2]));
-      a->max.x = std::max(a->max.x, double(c[0])); a->max.y = std::max(a->max.y, double(c[1])); a->max.z = std::max(a->max.z, double(c[2]));
+    const LzPoint p = laszip_decode(fmt, src);
+    // the standard handlers that exist for this format's attributes
+    // (:391-546); attributes without one (the wave packet) stay zero
+    for (int i = 1; i < firstExtraIndex; i++) {
+      Attr* a = &o.list[i];
+      const int off = offset[i];
+      switch (handler[i]) {
+        case INTENSITY: memcpy(dst + off, &p.intensity, 2); mm(a, double(p.intensity)); break;
+        case RET: dst[off] = p.return_number; mm(a, double(p.return_number)); break;
+        case NRET: dst[off] = p.number_of_returns; mm(a, double(p.number_of_returns)); break;
+        case CLASS: {  // :467-479
+          const uint8_t v = p.extended_classification > 31 ? p.extended_classification : p.classification;
+          dst[off] = v; a->histogram[v]++; mm(a, double(v));
+          break;
+        }
+        case SAR: memcpy(dst + off, &p.scan_angle_rank, 1); mm(a, double(p.scan_angle_rank)); break;
+        case SA: memcpy(dst + off, &p.extended_scan_angle, 2); mm(a, double(p.extended_scan_angle)); break;
+        case UD: dst[off] = p.user_data; mm(a, double(p.user_data)); break;
+        case PSID: memcpy(dst + off, &p.point_source_ID, 2); mm(a, double(p.point_source_ID)); break;
+        case GPS: memcpy(dst + off, &p.gps_time, 8); mm(a, p.gps_time); break;
+        case CFLAGS: dst[off] = p.extended_classification_flags; mm(a, double(p.extended_classification_flags)); break;
+        case RGB:
+          memcpy(dst + off, p.rgb, 6);
+          a->min.x = std::min(a->min.x, double(p.rgb[0])); a->min.y = std::min(a->min.y, double(p.rgb[1])); a->min.z = std::min(a->min.z, double(p.rgb[2]));
+          a->max.x = std::max(a->max.x, double(p.rgb[0])); a->max.y = std::max(a->max.y, double(p.rgb[1])); a->max.z = std::max(a->max.z, double(p.rgb[2]));
+          break;
+        default: break;  // no handler (:534-546): bytes stay 0, min/max stay +-inf
+      }
     }
     // extra attributes :579-651
     int so = 0, ao = extraDst;
     for (size_t i = firstExtraIndex; i < o.list.size(); i++) {
       Attr* a = &o.list[i];
       memcpy(dst + ao, src + extraSrc + so, a->size);
-      const uint8_t* p = src + extraSrc + so;
+      const uint8_t* ptr = src + extraSrc + so;
       if (a->numElements == 1) {
-        double x = elem(a->type, p);
+        double x = elem(a->type, ptr);
         a->min.x = std::min(a->min.x, x); a->max.x = std::max(a->max.x, x);
       } else if (a->numElements == 2) {
-        double x = elem(a->type, p), y = elem(a->type, p + a->elementSize);
+        double x = elem(a->type, ptr), y = elem(a->type, ptr + a->elementSize);
         a->min.x = std::min(a->min.x, x); a->min.y = std::min(a->min.y, y);
         a->max.x = std::max(a->max.x, x); a->max.y = std::max(a->max.y, y);
       } else if (a->numElements == 3) {
-        double x = elem(a->type, p), y = elem(a->type, p + a->elementSize), z = elem(a->type, p + 2 * a->elementSize);
+        double x = elem(a->type, ptr), y = elem(a->type, ptr + a->elementSize), z = elem(a->type, ptr + 2 * a->elementSize);
         a->min.x = std::min(a->min.x, x); a->min.y = std::min(a->min.y, y); a->min.z = std::min(a->min.z, z);
         a->max.x = std::max(a->max.x, x); a->max.y = std::max(a->max.y, y); a->max.z = std::max(a->max.z, z);
       }
@@ -1626,6 +1707,31 @@ extern "C" int orc_sample_node(const char* method, const uint8_t* records, const
       memcpy(o, kids[c]->points->data(), kids[c]->points->size());
       o += kids[c]->points->size();
     }
+  }
+  return 0;
+}
+
+// The laszip point fields of n records (32 bytes each, the layout of
+// oracle/ref_point_decode.cpp) as the oracle's laszip_decode restates them.
+extern "C" int orc_decode_points(int32_t fmt, int32_t record_length, const uint8_t* records, int64_t n,
+                                 uint8_t* out) {
+  if (fmt < 0 || fmt > 7) return 1;
+  for (int64_t i = 0; i < n; i++) {
+    const orc::LzPoint p = orc::laszip_decode(fmt, records + i * record_length);
+    uint8_t* o = out + i * 32;
+    memset(o, 0, 32);
+    memcpy(o + 0, &p.intensity, 2);
+    o[2] = p.return_number;
+    o[3] = p.number_of_returns;
+    o[4] = p.classification;
+    memcpy(o + 5, &p.scan_angle_rank, 1);
+    o[6] = p.user_data;
+    o[7] = p.extended_classification_flags;
+    memcpy(o + 8, &p.point_source_ID, 2);
+    memcpy(o + 10, &p.extended_scan_angle, 2);
+    o[12] = p.extended_classification;
+    memcpy(o + 14, p.rgb, 6);
+    memcpy(o + 24, &p.gps_time, 8);
   }
   return 0;
 }

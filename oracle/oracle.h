@@ -55,6 +55,8 @@ int orc_sample_node(const char* method, const uint8_t* records, const int64_t ch
                     double base_spacing, const double pos_scale[3], const double pos_offset[3],
                     uint8_t* accepted_flags, uint8_t* out_records);
 
+// laszip point fields of n records (32-byte entries, see ref_point_decode.cpp)
+int orc_decode_points(int32_t fmt, int32_t record_length, const uint8_t* records, int64_t n, uint8_t* out);
 const char* orc_last_error(void);
 
 #ifdef __cplusplus

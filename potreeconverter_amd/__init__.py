@@ -39,6 +39,7 @@ def lib():
         L.pcg_copy.argtypes = [c.c_void_p, c.c_void_p, c.c_uint64, c.c_int32]
         L.pcg_synchronize.argtypes = []
         L.pcg_lasgen_preset.argtypes = [c.c_int32, c.c_int64, c.c_uint64, c.c_void_p]
+        L.pcg_lasgen_configure.argtypes = [c.c_void_p, c.c_int32, c.c_int32]
         L.pcg_lasgen_record_length.argtypes = [c.c_void_p]
         L.pcg_lasgen_record_length.restype = c.c_int32
         L.pcg_lasgen_fill.argtypes = [c.c_void_p, c.c_int64, c.c_int64, c.c_void_p]
@@ -50,6 +51,7 @@ def lib():
         L.pcg_set_option.argtypes = [c.c_char_p, c.c_int64]
         for name in ("pcg_set_option", "pcg_release", "pcg_convert", "pcg_do_chunking", "pcg_do_indexing", "pcg_count_cells", "pcg_convert_device",
                      "pcg_device_alloc", "pcg_device_free", "pcg_copy", "pcg_synchronize", "pcg_lasgen_preset",
+                     "pcg_lasgen_configure",
                      "pcg_lasgen_write", "pcg_lasgen_device", "pcg_lasgen_header", "pcg_init"):
             getattr(L, name).restype = c.c_int
         _lib = L
@@ -67,15 +69,12 @@ class LasGen:
     scale 0.0001, offset 1e6 m, near-duplicate clusters and a lattice whose
     pitch equals the level-4 node spacing)."""
 
-    def __init__(self, preset, n, seed=0, extra_bytes=False):
+    def __init__(self, preset, n, seed=0, extra_bytes=False, point_format=2):
         self.buf = ctypes.create_string_buffer(256)
         _check(lib().pcg_lasgen_preset(preset, n, seed, self.buf))
-        if extra_bytes:
-            # LasGenParams.extra_bytes sits after seed, n, extent, origin[3],
-            # offset[3], inv_scale[3], amplitude, num_buildings, tile_points
-            off = 8 + 8 + 8 + 24 * 3 + 8 + 4 + 4
-            ctypes.c_int32.from_buffer(self.buf, off).value = 1
+        _check(lib().pcg_lasgen_configure(self.buf, point_format, int(bool(extra_bytes))))
         self.n = n
+        self.point_format = point_format
 
     @property
     def record_length(self):

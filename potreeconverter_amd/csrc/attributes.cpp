@@ -62,6 +62,17 @@ static StatKind stat_kind_of(AttributeType t) {
   }
 }
 
+bool has_statistics(const Attribute& a) {
+  if (a.type == AttributeType::UNDEFINED || a.size == 0) return false;
+  // the wave packet of formats 4-5 has no handler in
+  // chunker_countsort_laszip.cpp:534-546
+  static const char* const no_handler[] = {"wave packet descriptor index", "byte offset to waveform data",
+                                           "waveform packet size", "return point waveform location", "XYZ(t)"};
+  for (const char* n : no_handler)
+    if (a.name == n) return false;
+  return true;
+}
+
 bool output_attributes_for(const lasfile::Header& h, Attributes& out, ChunkerParams& P) {
   using T = AttributeType;
   Attribute xyz("position", 12, 3, 4, T::INT32), intensity("intensity", 2, 1, 2, T::UINT16);
@@ -69,6 +80,12 @@ bool output_attributes_for(const lasfile::Header& h, Attributes& out, ChunkerPar
   Attribute classification("classification", 1, 1, 1, T::UINT8), scanAngleRank("scan angle rank", 1, 1, 1, T::UINT8);
   Attribute userData("user data", 1, 1, 1, T::UINT8), pointSourceId("point source id", 2, 1, 2, T::UINT16);
   Attribute gpsTime("gps-time", 8, 1, 8, T::DOUBLE), rgb("rgb", 6, 3, 2, T::UINT16);
+  Attribute wavePacketDescriptorIndex("wave packet descriptor index", 1, 1, 1, T::UINT8);
+  Attribute byteOffsetToWaveformData("byte offset to waveform data", 8, 1, 8, T::UINT64);
+  Attribute waveformPacketSize("waveform packet size", 4, 1, 4, T::UINT32);
+  Attribute returnPointWaveformLocation("return point waveform location", 4, 1, 4, T::FLOAT);
+  Attribute XYZt("XYZ(t)", 12, 3, 4, T::FLOAT), classificationFlags("classification flags", 1, 1, 1, T::UINT8);
+  Attribute scanAngle("scan angle", 2, 1, 2, T::INT16);
   std::vector<Attribute> list;
   int fmt = h.point_format;
   switch (fmt) {
@@ -76,8 +93,26 @@ bool output_attributes_for(const lasfile::Header& h, Attributes& out, ChunkerPar
     case 1: list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId, gpsTime}; break;
     case 2: list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId, rgb}; break;
     case 3: list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId, gpsTime, rgb}; break;
+    case 4:
+      list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId, gpsTime,
+              wavePacketDescriptorIndex, byteOffsetToWaveformData, waveformPacketSize, returnPointWaveformLocation, XYZt};
+      break;
+    case 5:
+      list = {xyz, intensity, returnNumber, numberOfReturns, classification, scanAngleRank, userData, pointSourceId, gpsTime,
+              rgb, wavePacketDescriptorIndex, byteOffsetToWaveformData, waveformPacketSize, returnPointWaveformLocation,
+              XYZt};
+      break;
+    case 6:
+      list = {xyz, intensity, returnNumber, numberOfReturns, classificationFlags, classification, userData, scanAngle,
+              pointSourceId, gpsTime};
+      break;
+    case 7:
+      list = {xyz, intensity, returnNumber, numberOfReturns, classificationFlags, classification, userData, scanAngle,
+              pointSourceId, gpsTime, rgb};
+      break;
     default:
-      set_error("LAS point data format " + std::to_string(fmt) + " is not supported by the GPU chunker (formats 0-3)");
+      // PotreeConverter.h:181-185 (formats above 7 end the conversion)
+      set_error("currently unsupported LAS format: " + std::to_string(fmt));
       return false;
   }
   const size_t numStandard = list.size();
@@ -116,22 +151,45 @@ bool output_attributes_for(const lasfile::Header& h, Attributes& out, ChunkerPar
     P.ops[P.num_ops++] = TranscodeOp{src, dst, len, mode};
   };
   auto st = [&](int src, int kind) { P.stats[P.num_stats++] = StatOp{src, kind}; };
-  st(0, ST_I32); st(4, ST_I32); st(8, ST_I32);
-  op(12, out.getOffset("intensity"), 2, TC_COPY); st(12, ST_U16);
-  op(14, out.getOffset("return number"), 1, TC_LOW3); st(14, ST_LOW3);
-  op(14, out.getOffset("number of returns"), 1, TC_MID3); st(14, ST_MID3);
-  op(15, out.getOffset("classification"), 1, TC_LOW5); st(15, ST_LOW5);
-  op(16, out.getOffset("scan angle rank"), 1, TC_COPY); st(16, ST_I8);
-  op(17, out.getOffset("user data"), 1, TC_COPY); st(17, ST_U8);
-  op(18, out.getOffset("point source id"), 2, TC_COPY); st(18, ST_U16);
-  if (fmt == 1 || fmt == 3) { op(20, out.getOffset("gps-time"), 8, TC_COPY); st(20, ST_F64); }
-  if (fmt == 2 || fmt == 3) {
-    int src = fmt == 2 ? 20 : 28;
-    op(src, out.getOffset("rgb"), 6, TC_COPY);
-    st(src, ST_U16); st(src + 2, ST_U16); st(src + 4, ST_U16);
+  // the statistics start with std_stat(fmt, 0 .. std_stat_count(fmt)), the
+  // order the count kernel folds them in; the attributes without a handler
+  // in the reference (the wave packet of formats 4-5) get neither an op nor
+  // a statistic, so their bytes stay 0 and their min/max stay +-inf
+  P.point_format = fmt;
+  for (int i = 0; i < std_stat_count(fmt); i++) {
+    const StdStat ss = std_stat(fmt, i);
+    st(ss.src, ss.kind);
   }
-  P.class_src = 15;
-  static const int core[] = {20, 28, 26, 34};
+  op(12, out.getOffset("intensity"), 2, TC_COPY);
+  if (fmt <= 5) {
+    op(14, out.getOffset("return number"), 1, TC_LOW3);
+    op(14, out.getOffset("number of returns"), 1, TC_MID3);
+    op(15, out.getOffset("classification"), 1, TC_LOW5);
+    op(16, out.getOffset("scan angle rank"), 1, TC_COPY);
+    op(17, out.getOffset("user data"), 1, TC_COPY);
+    op(18, out.getOffset("point source id"), 2, TC_COPY);
+    if (fmt == 1 || fmt == 3 || fmt == 4 || fmt == 5) op(20, out.getOffset("gps-time"), 8, TC_COPY);
+    if (fmt == 2 || fmt == 3 || fmt == 5) op(fmt == 2 ? 20 : 28, out.getOffset("rgb"), 6, TC_COPY);
+    P.class_src = 15;
+    P.class_mask = 31;
+  } else {
+    // laszip's point-14 reader: legacy return fields from the 4-bit ones,
+    // classification flags, the full classification byte (chunker :467-479
+    // takes extended_classification when it is above 31, the 5-bit legacy
+    // field - the same byte - otherwise), scan angle as stored
+    op(14, out.getOffset("return number"), 1, TC_RET14);
+    op(14, out.getOffset("number of returns"), 1, TC_NRET14);
+    op(15, out.getOffset("classification flags"), 1, TC_LOW4);
+    op(16, out.getOffset("classification"), 1, TC_COPY);
+    op(17, out.getOffset("user data"), 1, TC_COPY);
+    op(18, out.getOffset("scan angle"), 2, TC_COPY);
+    op(20, out.getOffset("point source id"), 2, TC_COPY);
+    op(22, out.getOffset("gps-time"), 8, TC_COPY);
+    if (fmt == 7) op(30, out.getOffset("rgb"), 6, TC_COPY);
+    P.class_src = 16;
+    P.class_mask = 255;
+  }
+  static const int core[] = {20, 28, 26, 34, 57, 63, 30, 36};
   int srcOff = core[fmt];
   int dstOff = 0;
   for (size_t i = 0; i < numStandard; i++) dstOff += out.list[i].size;

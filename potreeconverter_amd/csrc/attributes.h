@@ -82,6 +82,9 @@ struct Attributes {
 bool output_attributes_for(const lasfile::Header& h, Attributes& out, ChunkerParams& params);
 
 // chunks/metadata.json text (chunker_countsort_laszip.cpp:1005-1068).
+// whether the chunker folds min/max for the attribute (the reference has a
+// handler for it)
+bool has_statistics(const Attribute& a);
 std::string chunk_metadata_json(const Vec3& min, const Vec3& max, const Attributes& a);
 bool parse_chunk_metadata(const std::string& text, Vec3& min, Vec3& max, Attributes& a);
 

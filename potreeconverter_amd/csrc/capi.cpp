@@ -225,6 +225,7 @@ int pcg_count_cells(const void* d_las, int64_t n, int32_t bpp, const double las_
   P.grid_size = grid_size;
   P.bpp_in = bpp;
   P.class_src = -1;
+  P.point_format = -1;  // counts only: no attribute statistics
   if (chunker_set_params(P, s)) return 1;
   int64_t cells = (int64_t)grid_size * grid_size * grid_size;
   void *stats = nullptr, *hist = nullptr, *err = nullptr;
@@ -233,7 +234,7 @@ int pcg_count_cells(const void* d_las, int64_t n, int32_t bpp, const double las_
   PCG_CHECK(hipMalloc(&err, 16));
   PCG_CHECK(hipMemsetAsync(d_grid, 0, sizeof(int32_t) * cells, s));
   PCG_CHECK(hipMemsetAsync(err, 0, 16, s));
-  int rc = chunker_count((const uint8_t*)d_las, n, d_grid, (uint64_t*)stats, (uint64_t*)stats + kMaxStats,
+  int rc = chunker_count((const uint8_t*)d_las, n, d_grid, nullptr, (uint64_t*)stats, (uint64_t*)stats + kMaxStats,
                          (unsigned long long*)hist, (int32_t*)err, device().num_cus, s);
   int32_t herr = 0;
   if (!rc) {
@@ -259,8 +260,8 @@ void pcg_lasgen_bounds(const LasGenParams* p, const int32_t qmin[3], const int32
 int pcg_lasgen_header(const void* params, const int32_t qminmax[6], uint8_t* header375) {
   const LasGenParams* p = (const LasGenParams*)params;
   lasfile::Header h;
-  h.point_format = 2;
-  h.record_length = (uint16_t)(p->extra_bytes ? 33 : 26);
+  h.point_format = (uint8_t)p->point_format;
+  h.record_length = (uint16_t)lasgen_record_length(*p);
   h.num_points = (uint64_t)p->n;
   for (int i = 0; i < 3; i++) {
     h.scale[i] = 1.0 / p->inv_scale[i];
@@ -275,7 +276,9 @@ int pcg_lasgen_header(const void* params, const int32_t qminmax[6], uint8_t* hea
   lasfile::wr<uint32_t>(header375 + 96, 375);
   header375[104] = h.point_format;
   lasfile::wr<uint16_t>(header375 + 105, h.record_length);
-  lasfile::wr<uint32_t>(header375 + 107, h.num_points <= 0xFFFFFFFFull ? (uint32_t)h.num_points : 0u);
+  // legacy point count: 0 for formats 6-10 and above 2^32 - 1 points (LAS 1.4)
+  lasfile::wr<uint32_t>(header375 + 107,
+                        h.num_points <= 0xFFFFFFFFull && h.point_format < 6 ? (uint32_t)h.num_points : 0u);
   for (int i = 0; i < 3; i++) lasfile::wr<double>(header375 + 131 + 8 * i, h.scale[i]);
   for (int i = 0; i < 3; i++) lasfile::wr<double>(header375 + 155 + 8 * i, h.offset[i]);
   lasfile::wr<double>(header375 + 179, h.max[0]); lasfile::wr<double>(header375 + 187, h.min[0]);

@@ -9,16 +9,22 @@
 namespace pcg {
 
 int chunker_set_params(const ChunkerParams& p, hipStream_t stream);
-int chunker_count(const uint8_t* d_las, int64_t n, int32_t* d_grid, uint64_t* d_stat_min, uint64_t* d_stat_max,
-                  unsigned long long* d_class_hist, int32_t* d_err, int num_cus, hipStream_t stream);
+// d_cells (n entries, may be null) receives the grid cell of every point
+// (0xFFFFFFFF outside the box)
+int chunker_count(const uint8_t* d_las, int64_t n, int32_t* d_grid, uint32_t* d_cells, uint64_t* d_stat_min,
+                  uint64_t* d_stat_max, unsigned long long* d_class_hist, int32_t* d_err, int num_cus,
+                  hipStream_t stream);
 // d_levels[l] holds (2^l)^3 int32 values; d_levels[level_max] is the count grid.
 int chunker_pyramid(int32_t* const* d_levels, int32_t level_max, int64_t cap, hipStream_t stream);
 int chunker_chunk_cells(int32_t* const* d_levels, int32_t level_max, int64_t* d_out, uint32_t* d_counter,
                         uint32_t max_out, hipStream_t stream);
 int chunker_lut_fill(const int64_t* d_chunks, int64_t num_chunks, int32_t level_max, int32_t* d_lut,
                      hipStream_t stream);
-int chunker_distribute(const uint8_t* d_las, int64_t n, const int32_t* d_lut, int32_t num_chunks, int32_t tile_size,
-                       uint32_t* d_M, void* d_scan_ws, uint8_t* d_out, int32_t* d_err, hipStream_t stream);
+// d_ids holds the cells written by chunker_count and is overwritten with
+// the chunk index of every point
+int chunker_distribute(const uint8_t* d_las, uint32_t* d_ids, int64_t n, const int32_t* d_lut, int32_t num_chunks,
+                       int32_t tile_size, uint32_t* d_M, void* d_scan_ws, uint8_t* d_out, int32_t* d_err,
+                       hipStream_t stream);
 int lasgen_device(const LasGenParams& p, int64_t first, int64_t count, uint8_t* d_out, int32_t* d_qminmax,
                   hipStream_t stream);
 

@@ -8,7 +8,7 @@
 //   k_pyramid_level   :1101-1183 image-pyramid merge
 //   k_chunk_cells     :1144-1172 cells that become chunks
 //   k_lut_fill        :1187-1199 LUT fill
-//   k_tile_hist       :838-876  per-tile chunk counts
+//   k_tile_hist       :817-876  chunk of every point (LUT), per-tile counts
 //   k_scatter         :770-904  transcode + stable scatter
 // All geometry is IEEE binary64 with the reference's evaluation order; the
 // TU is compiled with -ffp-contract=off.
@@ -27,6 +27,7 @@ constexpr int kCountSteps = 16;             // 64-point steps per wave in k_coun
 constexpr int kHashSize = 2048;             // LDS hash slots in k_count
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int kMaxBppIn = 128;
+constexpr int kMaxStdStats = 15;  // std_stat_count of point format 7
 
 __constant__ ChunkerParams c_params;
 
@@ -81,10 +82,11 @@ struct StatLane {
   uint64_t mask;
   int sign_bits;  // width of a signed field, 0 unsigned
   int fp;         // 0 integer, 4 float, 8 double
+  int legacy;     // 1: legacy return number, 2: legacy number of returns (formats 6-10)
 };
 
 __device__ inline StatLane stat_lane(const StatOp& op) {
-  StatLane s{op.src, 0, 0xFFull, 0, 0};
+  StatLane s{op.src, 0, 0xFFull, 0, 0, 0};
   switch (op.kind) {
     case ST_U8: break;
     case ST_I8: s.sign_bits = 8; break;
@@ -99,6 +101,9 @@ __device__ inline StatLane stat_lane(const StatOp& op) {
     case ST_LOW3: s.mask = 7; break;
     case ST_MID3: s.mask = 7; s.shift = 3; break;
     case ST_LOW5: s.mask = 31; break;
+    case ST_RET14: s.legacy = 1; break;
+    case ST_NRET14: s.legacy = 2; break;
+    case ST_LOW4: s.mask = 15; break;
     default: s.mask = 0; break;
   }
   return s;
@@ -106,6 +111,7 @@ __device__ inline StatLane stat_lane(const StatOp& op) {
 
 __device__ inline uint64_t stat_key(const StatLane& s, uint64_t raw) {
   uint64_t v = (raw >> s.shift) & s.mask;
+  if (s.legacy) return s.legacy == 1 ? legacy_return_number((uint32_t)v) : legacy_number_of_returns((uint32_t)v);
   if (s.fp == 4) {
     float f;
     uint32_t b = (uint32_t)v;
@@ -170,6 +176,16 @@ __device__ inline int64_t cell_of_record(const uint8_t* rec, const ChunkerParams
 }
 
 
+// requantized position of one point (the q[] of cell_of_record)
+__device__ inline void quantize_record(const uint8_t* rec, const ChunkerParams& P, int32_t q[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    int32_t X = (int32_t)lds_u32(rec + 4 * a);
+    double x = P.las_scale[a] * X + P.las_offset[a];
+    q[a] = (int32_t)((x - P.pos_offset[a]) / P.pos_scale[a]);
+  }
+}
+
 // peers of this lane with the same key in the wave (match-any by key bits)
 __device__ inline uint64_t wave_match(uint32_t key, int bits, uint64_t active) {
   uint64_t peers = active;
@@ -183,24 +199,127 @@ __device__ inline uint64_t wave_match(uint32_t key, int bits, uint64_t active) {
 // bytes of one wave's record staging area in LDS
 __host__ __device__ inline int record_area_bytes(int bpp) { return (64 * bpp + 48 + 15) & ~15; }
 
-// K1: per-cell counts, attribute min/max (ordered uint64 bits) and the
-// classification histogram.  Each wave walks a contiguous range of 64-point
-// steps (consecutive LAS points are spatially coherent): runs of equal
-// cells inside a step collapse to one LDS hash update, the LDS hash merges
-// repeats across the block's steps before the global atomics, and the
-// attribute statistics are spread over the lanes (lane l folds stat l % S
-// of every (64 / S)-th record of the step).
+// 4 bytes of a record at compile-time byte offset o from its dwords
+// (w[k] = bytes 4k .. 4k+3 of the record)
+template <int O>
+__device__ inline uint32_t rec_u32(const uint32_t* w) {
+  if constexpr ((O & 3) == 0) return w[O >> 2];
+  else return __builtin_amdgcn_alignbyte(w[(O >> 2) + 1], w[O >> 2], O & 3);
+}
+
+// order-preserving key of one standard statistic (compile-time kind)
+template <int FMT, int I>
+__device__ inline uint64_t std_key(const uint32_t* w) {
+  constexpr StdStat st = std_stat(FMT, I);
+  const uint32_t v = rec_u32<st.src>(w);
+  if constexpr (st.kind == ST_F64) {
+    return ordered_bits_from_raw((uint64_t)v | ((uint64_t)rec_u32<st.src + 4>(w) << 32));
+  } else if constexpr (st.kind == ST_I32) {
+    return v ^ 0x80000000u;
+  } else if constexpr (st.kind == ST_I16) {
+    return (uint32_t)(int32_t)(int16_t)(v & 0xFFFFu) ^ 0x80000000u;
+  } else if constexpr (st.kind == ST_I8) {
+    return (uint32_t)(int32_t)(int8_t)(v & 0xFFu) ^ 0x80000000u;
+  } else if constexpr (st.kind == ST_U16) {
+    return v & 0xFFFFu;
+  } else if constexpr (st.kind == ST_LOW3) {
+    return v & 7u;
+  } else if constexpr (st.kind == ST_MID3) {
+    return (v >> 3) & 7u;
+  } else if constexpr (st.kind == ST_LOW5) {
+    return v & 31u;
+  } else if constexpr (st.kind == ST_LOW4) {
+    return v & 15u;
+  } else if constexpr (st.kind == ST_RET14) {
+    return legacy_return_number(v & 0xFFu);
+  } else if constexpr (st.kind == ST_NRET14) {
+    return legacy_number_of_returns(v & 0xFFu);
+  } else {
+    return v & 0xFFu;  // ST_U8
+  }
+}
+
+// a standard statistic's key back to the ordered bits of the double value
+template <int FMT, int I>
+__device__ inline uint64_t std_key_bits(uint64_t key) {
+  constexpr StdStat st = std_stat(FMT, I);
+  if constexpr (st.kind == ST_F64) return key;
+  else if constexpr (st.kind == ST_I32 || st.kind == ST_I16 || st.kind == ST_I8)
+    return ordered_bits((double)(int32_t)((uint32_t)key ^ 0x80000000u));
+  else return ordered_bits((double)(uint32_t)key);
+}
+
+// per-lane running keys of the standard statistics: 32-bit keys, and the one
+// 64-bit (gps time) key of the formats that have it
+struct StdAcc {
+  uint32_t mn[kMaxStdStats], mx[kMaxStdStats];
+  uint64_t mn64, mx64;
+};
+
+template <int FMT, int I>
+__device__ inline void fold_std(const uint32_t* w, StdAcc& a) {
+  if constexpr (I < std_stat_count(FMT)) {
+    const uint64_t k = std_key<FMT, I>(w);
+    if constexpr (std_stat(FMT, I).kind == ST_F64) {
+      a.mn64 = k < a.mn64 ? k : a.mn64;
+      a.mx64 = k > a.mx64 ? k : a.mx64;
+    } else {
+      const uint32_t k32 = (uint32_t)k;
+      a.mn[I] = k32 < a.mn[I] ? k32 : a.mn[I];
+      a.mx[I] = k32 > a.mx[I] ? k32 : a.mx[I];
+    }
+    fold_std<FMT, I + 1>(w, a);
+  }
+}
+
+// wave minimum / maximum of every standard key, as ordered double bits, into
+// this wave's row (neutral ~0 / 0 for a wave without records)
+template <int FMT, int I>
+__device__ inline void publish_std(const StdAcc& acc, bool wave_seen, int lane, uint64_t* s_min_row,
+                                   uint64_t* s_max_row) {
+  if constexpr (I < std_stat_count(FMT)) {
+    constexpr bool wide = std_stat(FMT, I).kind == ST_F64;
+    uint64_t a = wide ? acc.mn64 : acc.mn[I], b = wide ? acc.mx64 : acc.mx[I];
+    for (int d = 32; d >= 1; d >>= 1) {
+      const uint64_t oa = __shfl_xor(a, d), ob = __shfl_xor(b, d);
+      a = oa < a ? oa : a;
+      b = ob > b ? ob : b;
+    }
+    if (lane == 0) {
+      s_min_row[I] = wave_seen ? std_key_bits<FMT, I>(a) : ~0ull;
+      s_max_row[I] = wave_seen ? std_key_bits<FMT, I>(b) : 0ull;
+    }
+    publish_std<FMT, I + 1>(acc, wave_seen, lane, s_min_row, s_max_row);
+  }
+}
+
+// K1: per-cell counts, the cell of every point, attribute min/max (ordered
+// uint64 bits) and the classification histogram.  Each wave walks a
+// contiguous range of 64-point steps (consecutive LAS points are spatially
+// coherent): runs of equal cells inside a step collapse to one LDS hash
+// update, the LDS hash merges repeats across the block's steps before the
+// global atomics.  The standard attributes of point format FMT are folded
+// by every lane for its own record with compile-time field offsets (the
+// record's leading dwords are read from LDS once); the extra-byte statistics
+// are spread over the lanes (lane l folds extra statistic l % E of every
+// (64 / E)-th record of the step).  FMT 8: no standard statistics.
+template <int FMT>
 __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ las, int64_t n,
-                                                   int32_t* __restrict__ grid, uint64_t* __restrict__ stat_min,
+                                                   int32_t* __restrict__ grid, uint32_t* __restrict__ cells,
+                                                   uint64_t* __restrict__ stat_min,
                                                    uint64_t* __restrict__ stat_max,
                                                    unsigned long long* __restrict__ class_hist,
                                                    int32_t* __restrict__ err) {
+  constexpr int NS = std_stat_count(FMT);
+  constexpr int NW = (std_record_bytes(FMT) + 3) / 4 + 1;  // dwords read per record (+1 for alignbyte)
   const ChunkerParams& P = c_params;
   __shared__ uint32_t s_key[kHashSize];
   __shared__ uint32_t s_cnt[kHashSize];
   __shared__ uint32_t s_hist[256];
   __shared__ uint64_t s_smin[kBlock];
   __shared__ uint64_t s_smax[kBlock];
+  __shared__ uint64_t s_rmin[kWavesPerBlock][kMaxStdStats];
+  __shared__ uint64_t s_rmax[kWavesPerBlock][kMaxStdStats];
   extern __shared__ __attribute__((aligned(16))) uint8_t s_dyn[];  // record staging, one area per wave
   for (int i = threadIdx.x; i < kHashSize; i += kBlock) { s_key[i] = kEmpty; s_cnt[i] = 0; }
   for (int i = threadIdx.x; i < 256; i += kBlock) s_hist[i] = 0;
@@ -209,13 +328,18 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ la
   const int bpp = P.bpp_in;
   const uint64_t total = (uint64_t)n * bpp;
   uint8_t* rec_area = s_dyn + wave * record_area_bytes(bpp);
-  const int S = P.num_stats;
-  const int R = S > 0 ? 64 / S : 0;  // lanes per statistic
-  const int my_stat = (S > 0 && lane < R * S) ? lane % S : -1;
-  const int my_r0 = S > 0 ? lane / S : 0;
+  const int E = P.num_stats - NS;  // extra-byte statistics
+  const int R = E > 0 ? 64 / E : 0;  // lanes per extra statistic
+  const int my_stat = (E > 0 && lane < R * E) ? NS + lane % E : -1;
+  const int my_r0 = E > 0 ? lane / E : 0;
   const StatLane sl = stat_lane(my_stat >= 0 ? P.stats[my_stat] : StatOp{0, ST_NONE});
   uint64_t my_min = ~0ull, my_max = 0ull;  // keys
   bool seen = false;
+  StdAcc acc;
+#pragma unroll
+  for (int k = 0; k < kMaxStdStats; k++) { acc.mn[k] = ~0u; acc.mx[k] = 0u; }
+  acc.mn64 = ~0ull;
+  acc.mx64 = 0ull;
   const int64_t steps_total = (n + 63) / 64;
   const int64_t wave_global = (int64_t)blockIdx.x * kWavesPerBlock + wave;
   const int64_t waves_total = (int64_t)gridDim.x * kWavesPerBlock;
@@ -238,6 +362,18 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ la
       int32_t q[3];
       uint32_t key = (uint32_t)cell_of_record(rec, P, inBox, q);
       if (inBox) ekey = key; else bad = true;
+      if (cells) cells[first + lane] = ekey;
+      if constexpr (NS > 0) {
+        // the record's leading dwords: aligned LDS reads and byte aligns
+        const uint32_t sh = (uint32_t)((uintptr_t)rec & 3);
+        const uint32_t* a = (const uint32_t*)(rec - sh);
+        uint32_t raw[NW + 1], w[NW];
+#pragma unroll
+        for (int k = 0; k <= NW; k++) raw[k] = a[k];
+#pragma unroll
+        for (int k = 0; k < NW; k++) w[k] = __builtin_amdgcn_alignbyte(raw[k + 1], raw[k], sh);
+        fold_std<FMT, 0>(w, acc);
+      }
     }
     // runs of equal cells: the first lane of a run adds the run length
     const uint32_t prev = __shfl_up(ekey, 1);
@@ -264,8 +400,9 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ la
       if (!done) atomicAdd(&grid[ekey], (int32_t)run);
     }
     if (P.class_src >= 0) {
-      const uint32_t c = valid ? (uint32_t)(rec[P.class_src] & 31) : 32u;
-      const uint64_t peers = wave_match(c, 6, ~0ull);
+      const uint32_t cm = (uint32_t)P.class_mask;
+      const uint32_t c = valid ? ((uint32_t)rec[P.class_src] & cm) : cm + 1;
+      const uint64_t peers = wave_match(c, cm == 31 ? 6 : 9, ~0ull);
       if (valid && (peers & (low_incl >> 1)) == 0) atomicAdd(&s_hist[c], (uint32_t)__popcll(peers));
     }
     if (my_stat >= 0) {
@@ -284,6 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ la
   // neutral ~0 / 0)
   s_smin[threadIdx.x] = seen ? stat_key_bits(sl, my_min) : ~0ull;
   s_smax[threadIdx.x] = seen ? stat_key_bits(sl, my_max) : 0ull;
+  if constexpr (NS > 0) publish_std<FMT, 0>(acc, step0 < step1, lane, s_rmin[wave], s_rmax[wave]);
   __syncthreads();
   for (int i = threadIdx.x; i < kHashSize; i += kBlock) {
     uint32_t k = s_key[i];
@@ -291,16 +429,25 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ la
   }
   for (int i = threadIdx.x; i < 256; i += kBlock)
     if (s_hist[i]) atomicAdd(&class_hist[i], (unsigned long long)s_hist[i]);
-  if ((int)threadIdx.x < S) {
+  if ((int)threadIdx.x < NS) {
     uint64_t mn = ~0ull, mx = 0;
-    for (int t = 0; t < kBlock; t++) {
-      const int l = t & 63;
-      if (l >= R * S || l % S != (int)threadIdx.x) continue;
-      mn = s_smin[t] < mn ? s_smin[t] : mn;
-      mx = s_smax[t] > mx ? s_smax[t] : mx;
+    for (int w = 0; w < kWavesPerBlock; w++) {
+      mn = s_rmin[w][threadIdx.x] < mn ? s_rmin[w][threadIdx.x] : mn;
+      mx = s_rmax[w][threadIdx.x] > mx ? s_rmax[w][threadIdx.x] : mx;
     }
     atomicMin((unsigned long long*)&stat_min[threadIdx.x], (unsigned long long)mn);
     atomicMax((unsigned long long*)&stat_max[threadIdx.x], (unsigned long long)mx);
+  }
+  if ((int)threadIdx.x < E) {
+    uint64_t mn = ~0ull, mx = 0;
+    for (int t = 0; t < kBlock; t++) {
+      const int l = t & 63;
+      if (l >= R * E || l % E != (int)threadIdx.x) continue;
+      mn = s_smin[t] < mn ? s_smin[t] : mn;
+      mx = s_smax[t] > mx ? s_smax[t] : mx;
+    }
+    atomicMin((unsigned long long*)&stat_min[NS + threadIdx.x], (unsigned long long)mn);
+    atomicMax((unsigned long long*)&stat_max[NS + threadIdx.x], (unsigned long long)mx);
   }
 }
 
@@ -358,41 +505,55 @@ __global__ void k_lut_fill(const int64_t* __restrict__ chunks, int64_t num_chunk
   }
 }
 
-// K3a: per-tile chunk histogram, written chunk-major: M[c * tiles + t]
-__global__ __launch_bounds__(kBlock) void k_tile_hist(const uint8_t* __restrict__ las, int64_t n, int32_t tile_size,
+// K3a: the chunk of every point (LUT entry of its cell, written over the
+// cell; kEmpty where there is none) and the per-tile chunk histogram,
+// chunk-major: M[c * tiles + t].  Each thread takes 4 consecutive points per
+// pass (tile_size is a multiple of 4 * kBlock, so tiles start 16-byte
+// aligned) and adds runs of equal chunks with one LDS atomic.
+__global__ __launch_bounds__(kBlock) void k_tile_hist(uint32_t* __restrict__ ids, int64_t n, int32_t tile_size,
                                                        const int32_t* __restrict__ lut, int32_t num_chunks,
                                                        int64_t tiles, uint32_t* __restrict__ M,
                                                        int32_t* __restrict__ err) {
   extern __shared__ uint32_t s_hist[];  // num_chunks entries (or none: then global atomics)
-  __shared__ __attribute__((aligned(16))) uint8_t s_rec[kWavesPerBlock][64 * kMaxBppIn + 32];
-  const ChunkerParams& P = c_params;
   const bool lds_hist = num_chunks <= 8192;
   const int64_t t = blockIdx.x;
   if (lds_hist)
     for (int i = threadIdx.x; i < num_chunks; i += kBlock) s_hist[i] = 0;
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int bpp = P.bpp_in;
-  const uint64_t total = (uint64_t)n * bpp;
   const int64_t t0 = t * tile_size;
   const int64_t t1 = (t0 + tile_size) < n ? (t0 + tile_size) : n;
   bool bad = false;
-  for (int64_t first = t0 + (int64_t)wave * 64; first < t1; first += kBlock) {
-    int cnt = (int)((t1 - first) < 64 ? (t1 - first) : 64);
-    uint32_t shift = wave_load_records(las, total, first, cnt, bpp, s_rec[wave]);
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    if (lane < cnt) {
-      bool inBox;
-      int32_t q[3];
-      int64_t cell = cell_of_record(s_rec[wave] + shift + lane * bpp, P, inBox, q);
-      int32_t c = inBox ? lut[cell] : -1;
-      if (c < 0) bad = true;
-      else if (lds_hist) atomicAdd(&s_hist[c], 1u);
-      else atomicAdd(&M[(int64_t)c * tiles + t], 1u);
+  for (int64_t i = t0 + 4 * (int64_t)threadIdx.x; i < t1; i += 4 * kBlock) {
+    const int m = (t1 - i) < 4 ? (int)(t1 - i) : 4;
+    uint32_t v[4];
+    if (m == 4) {
+      const uint4 q = *(const uint4*)(ids + i);
+      v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
+      for (int j = 0; j < 4; j++) v[j] = j < m ? ids[i + j] : kEmpty;
     }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      int32_t c = -1;
+      if (j < m && v[j] != kEmpty) c = lut[v[j]];
+      if (j < m && c < 0) bad = true;
+      v[j] = c < 0 ? kEmpty : (uint32_t)c;
+    }
+    if (m == 4) {
+      *(uint4*)(ids + i) = make_uint4(v[0], v[1], v[2], v[3]);
+    } else {
+      for (int j = 0; j < m; j++) ids[i + j] = v[j];
+    }
+    uint32_t run = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (j >= m || v[j] == kEmpty) continue;
+      run++;
+      if (j + 1 < m && v[j + 1] == v[j]) continue;
+      if (lds_hist) atomicAdd(&s_hist[v[j]], run);
+      else atomicAdd(&M[(int64_t)v[j] * tiles + t], run);
+      run = 0;
+    }
   }
   if (bad) atomicOr(err, 2);
   __syncthreads();
@@ -417,10 +578,11 @@ __host__ __device__ inline size_t scatter_lds_bytes(int num_chunks, int bpp, int
 // 64 output records of a step are assembled in LDS grouped into runs of one
 // chunk (a run is contiguous in the output), and every run is written with
 // aligned dword stores, bytes only where a run starts or ends inside a word.
-__global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ las, int64_t n, int32_t tile_size,
-                                                     const int32_t* __restrict__ lut, int32_t num_chunks,
-                                                     int32_t key_bits, int64_t tiles,
-                                                     const uint32_t* __restrict__ M, uint8_t* __restrict__ out) {
+__global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ las,
+                                                     const uint32_t* __restrict__ ids, int64_t n,
+                                                     int32_t tile_size, int32_t num_chunks, int32_t key_bits,
+                                                     int64_t tiles, const uint32_t* __restrict__ M,
+                                                     uint8_t* __restrict__ out) {
   extern __shared__ __attribute__((aligned(16))) uint8_t s_dyn[];
   const ChunkerParams& P = c_params;
   const int bpp = P.bpp_in, bpo = P.bpp_out;
@@ -440,27 +602,19 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ 
   const int64_t w1 = (w0 + per_wave) < t1 ? (w0 + per_wave) : t1;
   uint32_t* my_cnt = s_cnt + wave * num_chunks;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  // pass 1: counts
+  // pass 1: counts from the chunk ids
   for (int64_t first = w0; first < w1; first += 64) {
     int cnt = (int)((w1 - first) < 64 ? (w1 - first) : 64);
-    uint32_t shift = wave_load_records(las, total, first, cnt, bpp, rec_area);
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     uint32_t c = 0;
     bool valid = lane < cnt;
     if (valid) {
-      bool inBox;
-      int32_t q[3];
-      int64_t cell = cell_of_record(rec_area + shift + lane * bpp, P, inBox, q);
-      int32_t cc = inBox ? lut[cell] : -1;
-      valid = cc >= 0;
-      c = valid ? (uint32_t)cc : 0u;
+      const uint32_t cc = ids[first + lane];
+      valid = cc < (uint32_t)num_chunks;
+      c = valid ? cc : 0u;
     }
     uint64_t active = __ballot(valid);
     uint64_t peers = wave_match(c, key_bits, active);
     if (valid && (peers & lt_mask) == 0) my_cnt[c] += (uint32_t)__popcll(peers);
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   }
   __syncthreads();
   // exclusive prefix over waves, seeded with the global tile offset
@@ -486,11 +640,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ 
     uint32_t c = 0;
     int32_t q[3] = {0, 0, 0};
     if (valid) {
-      bool inBox;
-      int64_t cell = cell_of_record(rec, P, inBox, q);
-      int32_t cc = inBox ? lut[cell] : -1;
-      valid = cc >= 0;
-      c = valid ? (uint32_t)cc : 0u;
+      const uint32_t cc = ids[first + lane];
+      valid = cc < (uint32_t)num_chunks;
+      c = valid ? cc : 0u;
+      quantize_record(rec, P, q);
     }
     const uint64_t active = __ballot(valid);
     const uint64_t peers = wave_match(c, key_bits, active);
@@ -521,8 +674,17 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ 
         if (op.mode == TC_COPY) {
           for (int b = 0; b < op.len; b++) dst[op.dst + b] = rec[op.src + b];
         } else {
-          uint8_t v = rec[op.src];
-          dst[op.dst] = op.mode == TC_LOW3 ? (v & 7) : (op.mode == TC_MID3 ? ((v >> 3) & 7) : (v & 31));
+          const uint32_t v = rec[op.src];
+          uint32_t o;
+          switch (op.mode) {
+            case TC_LOW3: o = v & 7; break;
+            case TC_MID3: o = (v >> 3) & 7; break;
+            case TC_RET14: o = legacy_return_number(v); break;
+            case TC_NRET14: o = legacy_number_of_returns(v); break;
+            case TC_LOW4: o = v & 15; break;
+            default: o = v & 31; break;
+          }
+          dst[op.dst] = (uint8_t)o;
         }
       }
     }
@@ -557,9 +719,9 @@ __global__ void k_lasgen(LasGenParams p, const LasGenBuilding* __restrict__ bld,
   __shared__ LasGenBuilding s_b[64];
   for (int i = threadIdx.x; i < p.num_buildings && i < 64; i += blockDim.x) s_b[i] = bld[i];
   __syncthreads();
-  const int rl = p.extra_bytes ? 33 : 26;
+  const int rl = lasgen_record_length(p);
   int32_t mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
-  uint8_t rec[40];
+  uint8_t rec[72];
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
     lasgen_point(p, s_b, first + i, rec);
     uint8_t* d = out + i * rl;
@@ -584,6 +746,7 @@ __global__ void k_lasgen(LasGenParams p, const LasGenBuilding* __restrict__ bld,
 
 int g_bpp_in = 0;   // host copies of the record lengths in c_params
 int g_bpp_out = 0;
+int g_point_format = 8;  // k_count instantiation (8: no standard statistics)
 
 }  // namespace
 
@@ -598,20 +761,48 @@ int chunker_set_params(const ChunkerParams& p, hipStream_t stream) {
   PCG_CHECK(hipStreamSynchronize(stream));
   staged = p;
   staged.g_over_cube = (double)p.grid_size / p.cube;
+  if (staged.class_mask == 0) staged.class_mask = 31;
+  g_point_format = (p.point_format >= 0 && p.point_format <= 7) ? p.point_format : 8;
+  const int ns = std_stat_count(g_point_format);
+  if (p.num_stats < ns || p.num_stats - ns > 64) {
+    set_error("statistics table does not start with the point format's standard statistics");
+    return 1;
+  }
+  for (int i = 0; i < ns; i++) {
+    const StdStat st = std_stat(g_point_format, i);
+    if (p.stats[i].src != st.src || p.stats[i].kind != st.kind) {
+      set_error("statistics table does not start with the point format's standard statistics");
+      return 1;
+    }
+  }
   PCG_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_params), &staged, sizeof(staged), 0, hipMemcpyHostToDevice, stream));
   return 0;
 }
 
-int chunker_count(const uint8_t* d_las, int64_t n, int32_t* d_grid, uint64_t* d_stat_min, uint64_t* d_stat_max,
-                  unsigned long long* d_class_hist, int32_t* d_err, int num_cus, hipStream_t stream) {
+int chunker_count(const uint8_t* d_las, int64_t n, int32_t* d_grid, uint32_t* d_cells, uint64_t* d_stat_min,
+                  uint64_t* d_stat_max, unsigned long long* d_class_hist, int32_t* d_err, int num_cus,
+                  hipStream_t stream) {
   int64_t steps = (n + 63) / 64;
   int64_t blocks = (steps + kWavesPerBlock * kCountSteps - 1) / (kWavesPerBlock * kCountSteps);
   int64_t max_blocks = (int64_t)num_cus * 8;
   if (blocks > max_blocks) blocks = max_blocks;
   if (blocks < 1) blocks = 1;
   size_t lds = (size_t)kWavesPerBlock * record_area_bytes(g_bpp_in);
-  hipLaunchKernelGGL(k_count, dim3((unsigned)blocks), dim3(kBlock), lds, stream, d_las, n, d_grid, d_stat_min,
-                     d_stat_max, d_class_hist, d_err);
+#define PCG_COUNT(F)                                                                                      \
+  hipLaunchKernelGGL(k_count<F>, dim3((unsigned)blocks), dim3(kBlock), lds, stream, d_las, n, d_grid, d_cells, \
+                     d_stat_min, d_stat_max, d_class_hist, d_err)
+  switch (g_point_format) {
+    case 0: PCG_COUNT(0); break;
+    case 1: PCG_COUNT(1); break;
+    case 2: PCG_COUNT(2); break;
+    case 3: PCG_COUNT(3); break;
+    case 4: PCG_COUNT(4); break;
+    case 5: PCG_COUNT(5); break;
+    case 6: PCG_COUNT(6); break;
+    case 7: PCG_COUNT(7); break;
+    default: PCG_COUNT(8); break;
+  }
+#undef PCG_COUNT
   PCG_CHECK(hipGetLastError());
   return 0;
 }
@@ -655,13 +846,18 @@ int chunker_lut_fill(const int64_t* d_chunks, int64_t num_chunks, int32_t level_
   return 0;
 }
 
-int chunker_distribute(const uint8_t* d_las, int64_t n, const int32_t* d_lut, int32_t num_chunks, int32_t tile_size,
-                       uint32_t* d_M, void* d_scan_ws, uint8_t* d_out, int32_t* d_err, hipStream_t stream) {
+int chunker_distribute(const uint8_t* d_las, uint32_t* d_ids, int64_t n, const int32_t* d_lut, int32_t num_chunks,
+                       int32_t tile_size, uint32_t* d_M, void* d_scan_ws, uint8_t* d_out, int32_t* d_err,
+                       hipStream_t stream) {
   int64_t tiles = (n + tile_size - 1) / tile_size;
   if (tiles == 0) return 0;
+  if (tile_size % (4 * kBlock) != 0 || ((uintptr_t)d_ids & 15) != 0) {
+    set_error("chunk id tiles must be multiples of 1024 points on a 16-byte aligned array");
+    return 1;
+  }
   size_t hist_lds = num_chunks <= 8192 ? sizeof(uint32_t) * num_chunks : 0;
   if (num_chunks > 8192) PCG_CHECK(hipMemsetAsync(d_M, 0, sizeof(uint32_t) * tiles * num_chunks, stream));
-  hipLaunchKernelGGL(k_tile_hist, dim3((unsigned)tiles), dim3(kBlock), hist_lds, stream, d_las, n, tile_size, d_lut,
+  hipLaunchKernelGGL(k_tile_hist, dim3((unsigned)tiles), dim3(kBlock), hist_lds, stream, d_ids, n, tile_size, d_lut,
                      num_chunks, tiles, d_M, d_err);
   PCG_CHECK(hipGetLastError());
   if (exclusive_scan_u32(d_M, tiles * num_chunks, d_scan_ws, nullptr, stream)) return 1;
@@ -672,7 +868,7 @@ int chunker_distribute(const uint8_t* d_las, int64_t n, const int32_t* d_lut, in
     set_error("too many chunks for the LDS scatter counters");
     return 1;
   }
-  hipLaunchKernelGGL(k_scatter, dim3((unsigned)tiles), dim3(kBlock), cnt_lds, stream, d_las, n, tile_size, d_lut,
+  hipLaunchKernelGGL(k_scatter, dim3((unsigned)tiles), dim3(kBlock), cnt_lds, stream, d_las, d_ids, n, tile_size,
                      num_chunks, key_bits, tiles, d_M, d_out);
   PCG_CHECK(hipGetLastError());
   return 0;

@@ -184,8 +184,9 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
   if (chunker_set_params(P, s)) return 1;
 
   const int64_t cells = (int64_t)G * G * G;
-  DevBuf grid, stats, hist, err;
+  DevBuf grid, stats, hist, err, ids;
   if (grid.alloc(WS_GRID, sizeof(int32_t) * cells) || stats.alloc(WS_STATS, sizeof(uint64_t) * 2 * kMaxStats) ||
+      ids.alloc(WS_POINT_IDS, sizeof(uint32_t) * std::max<int64_t>(N, 1) + 64) ||
       hist.alloc(WS_HIST, sizeof(unsigned long long) * 256) || err.alloc(WS_ERR, sizeof(int32_t) * 4))
     return 1;
   PCG_CHECK(hipMemsetAsync(grid.p, 0, sizeof(int32_t) * cells, s));
@@ -197,7 +198,7 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
   if (!ev[0])
     for (int i = 0; i < 4; i++) PCG_CHECK(hipEventCreate(&ev[i]));
   PCG_CHECK(hipEventRecord(ev[0], s));
-  if (chunker_count(d_las, N, grid.as<int32_t>(), stats.as<uint64_t>(), stats.as<uint64_t>() + kMaxStats,
+  if (chunker_count(d_las, N, grid.as<int32_t>(), ids.as<uint32_t>(), stats.as<uint64_t>(), stats.as<uint64_t>() + kMaxStats,
                     hist.as<unsigned long long>(), err.as<int32_t>(), dev.num_cus, s))
     return 1;
   PCG_CHECK(hipEventRecord(ev[1], s));
@@ -229,7 +230,7 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
     }
     for (size_t i = 1; i < A.list.size(); i++) {
       Attribute& at = A.list[i];
-      if (at.type == AttributeType::UNDEFINED || at.size == 0) continue;
+      if (!has_statistics(at)) continue;
       int comps = at.numElements < 3 ? at.numElements : 3;
       for (int e = 0; e < comps; e++) {
         if (N > 0) { at.min[e] = mn(si); at.max[e] = mx(si); }
@@ -311,8 +312,8 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
   if (!out.d_records) return 1;
   // k_scatter writes every byte of every output record (unset bytes 0)
   PCG_CHECK(hipEventRecord(ev[2], s));
-  if (chunker_distribute(d_las, N, lut.as<int32_t>(), C, tile, M.as<uint32_t>(), ws.p, out.d_records,
-                         err.as<int32_t>(), s))
+  if (chunker_distribute(d_las, ids.as<uint32_t>(), N, lut.as<int32_t>(), C, tile, M.as<uint32_t>(), ws.p,
+                         out.d_records, err.as<int32_t>(), s))
     return 1;
   PCG_CHECK(hipEventRecord(ev[3], s));
   PCG_CHECK(hipMemcpyAsync(herr, err.p, sizeof(herr), hipMemcpyDeviceToHost, s));

@@ -164,7 +164,8 @@ inline bool write_file(const std::string& path, const Header& h, const uint8_t* 
   wr<uint32_t>(b + 100, h.extra.empty() ? 0u : 1u);
   b[104] = h.point_format;
   wr<uint16_t>(b + 105, h.record_length);
-  wr<uint32_t>(b + 107, h.num_points <= 0xFFFFFFFFull ? (uint32_t)h.num_points : 0u);
+  // legacy point count: 0 for formats 6-10 and above 2^32 - 1 points (LAS 1.4)
+  wr<uint32_t>(b + 107, h.num_points <= 0xFFFFFFFFull && h.point_format < 6 ? (uint32_t)h.num_points : 0u);
   for (int i = 0; i < 3; i++) wr<double>(b + 131 + 8 * i, h.scale[i]);
   for (int i = 0; i < 3; i++) wr<double>(b + 155 + 8 * i, h.offset[i]);
   wr<double>(b + 179, h.max[0]); wr<double>(b + 187, h.min[0]);

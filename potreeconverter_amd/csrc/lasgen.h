@@ -12,9 +12,13 @@
 // the given amplitude plus `num_buildings` axis-aligned boxes with points on
 // roofs and walls; intensity uniform u16; RGB from a height colour map plus
 // Irwin-Hall(12) noise with sigma 2000; classification 2 (ground) or 6
-// (building).  The output record is a LAS point data record format 2 (26 B),
-// followed by 7 extra bytes (float32 N(0,1), uint16, int8) when
-// `extra_bytes` is set.
+// (building).  The output record is a LAS point data record of format
+// `point_format` (0-7; 2 unless changed: 26 B), followed by 7 extra bytes
+// (float32 N(0,1), uint16, int8) when `extra_bytes` is set.  Format 2
+// records carry no user data, no classification flags and no scan
+// direction / edge bits; the other formats set them from the point hash,
+// plus gps time, a wave packet (4-5) and the 4-bit return fields, scan
+// angle and full classification byte of formats 6-7.
 //
 // Precision scenes (`span_units` > 0) add, in LAS integer units:
 //   - two anchor points (records 0 and 1) at the corners q0 and q0 + span of
@@ -57,6 +61,8 @@ struct LasGenParams {
   int64_t num_clusters;         // filled by lasgen_prepare
   int64_t lattice_pitch_units;  // lattice pitch in LAS units
   int64_t lattice_side;         // filled by lasgen_prepare
+  int32_t point_format;         // LAS point data record format 0-7 (presets: 2)
+  int32_t pad;
 };
 
 struct LasGenBuilding {
@@ -178,7 +184,8 @@ LASGEN_HD void lasgen_store32(uint8_t* d, uint32_t v) {
   d[0] = (uint8_t)v; d[1] = (uint8_t)(v >> 8); d[2] = (uint8_t)(v >> 16); d[3] = (uint8_t)(v >> 24);
 }
 
-// Writes record `i` (26 or 33 bytes) to `out`.  `bld` holds num_buildings entries.
+// Writes record `i` (lasgen_record_length bytes) to `out`.  `bld` holds
+// num_buildings entries.
 LASGEN_HD void lasgen_point(const LasGenParams& p, const LasGenBuilding* bld, int64_t i, uint8_t* out) {
   uint64_t s = p.seed;
   double x, y, z;
@@ -289,27 +296,92 @@ LASGEN_HD void lasgen_point(const LasGenParams& p, const LasGenBuilding* bld, in
     nb = nb + lasgen_u01(lasgen_hash(s, ui, 48 + k));
   }
 
+  const int32_t fmt = p.point_format;
+  const uint16_t red = lasgen_clamp_u16(cr + 2000.0 * nr);
+  const uint16_t green = lasgen_clamp_u16(cg + 2000.0 * ng);
+  const uint16_t blue = lasgen_clamp_u16(cb + 2000.0 * nb);
+  // fields of the other point formats (format 2 uses none of them)
+  const uint64_t hf = lasgen_hash(s, ui, 96);
+  const double gps = 400000000.0 + (double)i * 0.0001 + lasgen_u01(lasgen_hash(s, ui, 97)) * 0.00005;
   lasgen_store32(out + 0, (uint32_t)X);
   lasgen_store32(out + 4, (uint32_t)Y);
   lasgen_store32(out + 8, (uint32_t)Z);
   lasgen_store16(out + 12, intensity);
-  out[14] = (uint8_t)((ret & 7) | ((nret & 7) << 3));
-  out[15] = cls;
-  out[16] = (uint8_t)(int8_t)sar;
-  out[17] = 0;  // user data
-  lasgen_store16(out + 18, (uint16_t)(tile & 0xFFFF));
-  lasgen_store16(out + 20, lasgen_clamp_u16(cr + 2000.0 * nr));
-  lasgen_store16(out + 22, lasgen_clamp_u16(cg + 2000.0 * ng));
-  lasgen_store16(out + 24, lasgen_clamp_u16(cb + 2000.0 * nb));
+  int32_t core = 26;
+  if (fmt <= 5) {
+    out[14] = (uint8_t)((ret & 7) | ((nret & 7) << 3) | (fmt == 2 ? 0 : (int32_t)((hf >> 40) & 3) << 6));
+    // formats other than 2 also set the synthetic / keypoint / withheld bits
+    out[15] = (uint8_t)(cls | (fmt == 2 ? 0 : (int32_t)((hf >> 8) & 7) << 5));
+    out[16] = (uint8_t)(int8_t)sar;
+    out[17] = fmt == 2 ? 0 : (uint8_t)(hf >> 16);  // user data
+    lasgen_store16(out + 18, (uint16_t)(tile & 0xFFFF));
+    int32_t o = 20;
+    if (fmt == 1 || fmt == 3 || fmt == 4 || fmt == 5) {
+      uint64_t gb;
+      __builtin_memcpy(&gb, &gps, 8);
+      lasgen_store32(out + o, (uint32_t)gb);
+      lasgen_store32(out + o + 4, (uint32_t)(gb >> 32));
+      o += 8;
+    }
+    if (fmt == 2 || fmt == 3 || fmt == 5) {
+      lasgen_store16(out + o, red);
+      lasgen_store16(out + o + 2, green);
+      lasgen_store16(out + o + 4, blue);
+      o += 6;
+    }
+    if (fmt == 4 || fmt == 5) {
+      // wave packet: descriptor index, byte offset, size, location, xyz(t)
+      const uint64_t hw = lasgen_hash(s, ui, 98);
+      out[o] = (uint8_t)hw;
+      lasgen_store32(out + o + 1, (uint32_t)(hw >> 8));
+      lasgen_store32(out + o + 5, (uint32_t)(hw >> 40));
+      lasgen_store32(out + o + 9, (uint32_t)(hw >> 3));
+      for (int32_t k = 0; k < 4; k++) lasgen_store32(out + o + 13 + 4 * k, (uint32_t)(hw >> (7 * k)) & 0x3F7FFFFFu);
+      o += 29;
+    }
+    core = o;
+  } else {
+    // formats 6-7: 4-bit return number / number of returns (any pair, so
+    // that laszip's legacy mapping sees every case), classification flags,
+    // scanner channel, scan direction, edge of flight line; a full
+    // classification byte that is sometimes above 31
+    const int32_t nret4 = 1 + (int32_t)((hf >> 16) % 15);
+    const int32_t ret4 = 1 + (int32_t)((hf >> 24) % 15);
+    out[14] = (uint8_t)(ret4 | (nret4 << 4));
+    out[15] = (uint8_t)((hf >> 32) & 0xFF);
+    const uint32_t pick = (uint32_t)((hf >> 40) & 63);
+    out[16] = (uint8_t)(pick == 0 ? 32 + (int32_t)((hf >> 48) % 224) : (pick == 1 ? (int32_t)((hf >> 48) & 31) : cls));
+    out[17] = (uint8_t)(hf >> 56);
+    const int32_t ang = (int32_t)((hf >> 4) % 30001) - 15000;
+    lasgen_store16(out + 18, (uint16_t)(int16_t)ang);
+    lasgen_store16(out + 20, (uint16_t)(tile & 0xFFFF));
+    uint64_t gb;
+    __builtin_memcpy(&gb, &gps, 8);
+    lasgen_store32(out + 22, (uint32_t)gb);
+    lasgen_store32(out + 26, (uint32_t)(gb >> 32));
+    core = 30;
+    if (fmt == 7) {
+      lasgen_store16(out + 30, red);
+      lasgen_store16(out + 32, green);
+      lasgen_store16(out + 34, blue);
+      core = 36;
+    }
+  }
   if (p.extra_bytes) {
     double g = -6.0;
     for (int32_t k = 0; k < 12; k++) g = g + lasgen_u01(lasgen_hash(s, ui, 64 + k));
     float f = (float)g;
     uint32_t fb;
     __builtin_memcpy(&fb, &f, 4);
-    lasgen_store32(out + 26, fb);
+    lasgen_store32(out + core, fb);
     uint64_t hb = lasgen_hash(s, ui, 80);
-    lasgen_store16(out + 30, (uint16_t)(hb & 0xFFFF));
-    out[32] = (uint8_t)((hb >> 16) & 0xFF);
+    lasgen_store16(out + core + 4, (uint16_t)(hb & 0xFFFF));
+    out[core + 6] = (uint8_t)((hb >> 16) & 0xFF);
   }
+}
+
+// record length of a generated point (core record of the format + 7 extra bytes)
+LASGEN_HD int32_t lasgen_record_length(const LasGenParams& p) {
+  const int32_t core[8] = {20, 28, 26, 34, 57, 63, 30, 36};
+  return core[p.point_format] + (p.extra_bytes ? 7 : 0);
 }

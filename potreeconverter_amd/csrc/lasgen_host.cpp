@@ -29,6 +29,7 @@ int pcg_lasgen_preset(int32_t preset, int64_t n, uint64_t seed, LasGenParams* p)
   p->tile_points = 4096;
   p->wall_fraction = 0.05;
   p->extra_bytes = 0;
+  p->point_format = 2;
   double scale = 0.001;
   if (preset == 0) {
     p->extent = 500.0;
@@ -53,14 +54,23 @@ int pcg_lasgen_preset(int32_t preset, int64_t n, uint64_t seed, LasGenParams* p)
   return 0;
 }
 
-int32_t pcg_lasgen_record_length(const LasGenParams* p) { return p->extra_bytes ? 33 : 26; }
+// Sets the record format of a preset's points (0-7) and whether 7 extra
+// bytes follow; returns 0 on success.
+int pcg_lasgen_configure(LasGenParams* p, int32_t point_format, int32_t extra_bytes) {
+  if (point_format < 0 || point_format > 7) return 1;
+  p->point_format = point_format;
+  p->extra_bytes = extra_bytes ? 1 : 0;
+  return 0;
+}
+
+int32_t pcg_lasgen_record_length(const LasGenParams* p) { return lasgen_record_length(*p); }
 
 // Generates records [first, first + count) into `out`.
 void pcg_lasgen_fill(const LasGenParams* p, int64_t first, int64_t count, uint8_t* out) {
   LasGenBuilding bld[64];
   int nb = p->num_buildings < 64 ? p->num_buildings : 64;
   for (int b = 0; b < nb; b++) bld[b] = lasgen_building(*p, b);
-  int rl = p->extra_bytes ? 33 : 26;
+  int rl = lasgen_record_length(*p);
   for (int64_t i = 0; i < count; i++) lasgen_point(*p, bld, first + i, out + i * rl);
 }
 
@@ -89,7 +99,7 @@ int pcg_lasgen_write(const LasGenParams* p, const char* path) {
     }
   }
   lasfile::Header h;
-  h.point_format = 2;
+  h.point_format = (uint8_t)p->point_format;
   h.record_length = (uint16_t)rl;
   h.num_points = (uint64_t)p->n;
   for (int i = 0; i < 3; i++) {

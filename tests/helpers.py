@@ -8,6 +8,7 @@ from potree_format import canonical_nodes
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
 REF_INDEXER = os.path.join(ROOT, "oracle", "_ref", "ref_indexer")
+REF_POINT_DECODE = os.path.join(ROOT, "oracle", "_ref", "ref_point_decode")
 _oracle = None
 
 
@@ -19,6 +20,8 @@ def oracle():
                                                           ctypes.POINTER(ctypes.c_double)]
         O.orc_index.argtypes = [ctypes.c_char_p] * 3 + [ctypes.POINTER(ctypes.c_double)]
         O.orc_last_error.restype = ctypes.c_char_p
+        O.orc_decode_points.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64,
+                                        ctypes.c_void_p]
         O.orc_count_cells.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32] + \
             [ctypes.POINTER(ctypes.c_double)] * 6 + [ctypes.c_int32, ctypes.c_void_p]
         O.orc_create_lut.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p,
@@ -36,9 +39,9 @@ def oracle_convert(las, outdir, method, keep_chunks=False, chunk_only=False):
     return list(t)
 
 
-def make_las(path, preset, n, seed, extra_bytes=False):
+def make_las(path, preset, n, seed, extra_bytes=False, point_format=2):
     import potreeconverter_amd as pc
-    g = pc.LasGen(preset, n, seed=seed, extra_bytes=extra_bytes)
+    g = pc.LasGen(preset, n, seed=seed, extra_bytes=extra_bytes, point_format=point_format)
     g.write(str(path))
     return g
 

@@ -55,15 +55,40 @@ def test_convert_matches_oracle(preset, n, seed, eb, method, tmp_path):
     assert all(res.values()), res
 
 
-def test_chunking_matches_oracle(tmp_path):
+FORMAT_CASES = [(f, eb) for f in range(8) for eb in (False, True)]
+
+
+@pytest.mark.parametrize("fmt,eb", FORMAT_CASES, ids=[f"pdrf{f}{'_eb' if e else ''}" for f, e in FORMAT_CASES])
+def test_chunking_matches_oracle(fmt, eb, tmp_path):
+    """Chunk files and chunks/metadata.json of every point format 0-7, with
+    and without extra bytes (the statistics, transcode and classification
+    histogram of each format's attribute handlers)."""
+    print(f"chunking case pdrf={fmt} extra={eb}", flush=True)
     las = tmp_path / "in.las"
-    make_las(las, 1, 800_000, 4)
+    make_las(las, 1, 800_000, 4 + fmt, eb, fmt)
     cpu, gpu = tmp_path / "cpu", tmp_path / "gpu"
     oracle_convert(las, cpu, "poisson", chunk_only=True)
     pc.do_chunking(las, gpu)
     a, b = sorted(os.listdir(cpu / "chunks")), sorted(os.listdir(gpu / "chunks"))
     assert a == b
     assert all(same_files(cpu / "chunks", gpu / "chunks", a).values())
+
+
+FORMAT_CONVERT_CASES = [(1, "poisson", False), (3, "poisson_average", True), (4, "poisson", False),
+                        (5, "poisson_average", False), (6, "poisson", True), (7, "poisson_average", False)]
+
+
+@pytest.mark.parametrize("fmt,method,eb", FORMAT_CONVERT_CASES,
+                         ids=[f"pdrf{c[0]}_{c[1]}{'_eb' if c[2] else ''}" for c in FORMAT_CONVERT_CASES])
+def test_convert_point_formats_matches_oracle(fmt, method, eb, tmp_path):
+    print(f"format case pdrf={fmt} method={method} extra={eb}", flush=True)
+    las = tmp_path / "in.las"
+    make_las(las, 0, 600_000, 20 + fmt, eb, fmt)
+    cpu, gpu = tmp_path / "cpu", tmp_path / "gpu"
+    oracle_convert(las, cpu, method)
+    pc.convert(las, gpu, method)
+    res = same_files(cpu, gpu, ["octree.bin", "hierarchy.bin", "metadata.json"])
+    assert all(res.values()), res
 
 
 def test_indexing_of_reference_format_chunks(tmp_path):
