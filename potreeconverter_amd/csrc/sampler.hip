@@ -86,6 +86,7 @@ struct Layout {
   uint32_t* nbr;     // sorted order, kMaxNbr earlier neighbours per candidate
   uint8_t* nbrCount; // kNbrOverflow: more than kMaxNbr
   uint32_t* tileAcc; // per 64-record tile of each child: accepted candidates of the node before it
+  uint8_t* accMain;  // main order: accepted (written by k_route for the record copy)
 };
 
 // Carves the per-node scratch into the arrays above; with base == nullptr it
@@ -122,6 +123,7 @@ __host__ __device__ inline uint64_t layout_carve(uint8_t* base, int64_t n, int m
   L->nbr = (uint32_t*)take(4 * kMaxNbr * n);
   L->nbrCount = (uint8_t*)take(n);
   L->tileAcc = (uint32_t*)take(4 * (n / 64 + 16));
+  L->accMain = (uint8_t*)take(n);
   L->sidx = L->idxA;
   return off + 64;
 }
@@ -216,18 +218,32 @@ struct RadixShared {
 // One stable counting pass of a block-wide LSD radix sort over the digit
 // (key >> shift) & mask (mask < 256) of n (key, value) pairs.  Tiles of
 // kRadixE * kBlock keys; sub-tile e of a tile precedes sub-tile e + 1.
-// Returns false and writes nothing when every key has the same digit.
+// `pre` (LDS, 256 counts) is the digit's histogram when the caller counted
+// it already; otherwise the pass counts.  Returns false and writes nothing
+// when every key has the same digit.
 __device__ bool block_radix_pass(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                  uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n, int shift,
-                                 uint32_t mask, RadixShared& sh) {
+                                 uint32_t mask, RadixShared& sh, const uint32_t* pre = nullptr) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  sh.hist[tid] = 0;
+  sh.hist[tid] = pre ? pre[tid] : 0u;
   for (int e = 0; e < kRadixE; e++)
     for (int w = 0; w < kWaves; w++) sh.wc[e][w][tid] = 0;
   __syncthreads();
-  for (int64_t j = tid; j < n; j += kBlock) atomicAdd(&sh.hist[(kin[j] >> shift) & mask], 1u);
-  __syncthreads();
+  if (!pre) {
+    for (int64_t j0 = 0; j0 < n; j0 += 4 * kBlock) {
+      uint32_t kk[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t j = j0 + u * kBlock + tid;
+        kk[u] = j < n ? kin[j] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (j0 + u * kBlock + tid < n) atomicAdd(&sh.hist[(kk[u] >> shift) & mask], 1u);
+    }
+    __syncthreads();
+  }
   if (wave == 0) {
     uint32_t h0 = sh.hist[4 * lane], h1 = sh.hist[4 * lane + 1], h2 = sh.hist[4 * lane + 2],
              h3 = sh.hist[4 * lane + 3];
@@ -329,9 +345,11 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
   __shared__ const uint8_t* s_cptr[8];
   __shared__ uint8_t* s_scr;
   __shared__ RadixShared s_rad;
+  __shared__ uint32_t s_h4[4][256];  // histograms of the four key bytes, counted while gathering
   const int k = blockIdx.x;
   const NodeInfo info = infos[nodes[k]];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int i = tid; i < 4 * 256; i += kBlock) (&s_h4[0][0])[i] = 0;
   if (tid == 0) {
     s_cs[0] = 0;
     for (int c = 0; c < 8; c++) {
@@ -365,31 +383,52 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
   const int64_t n = s_cs[8];
   Layout L = make_layout(s_scr, n, P.method);
   NodeGeom g = node_geom(info, P);
-  // gather (sampler_poisson.h:82-108)
-  for (int64_t j = tid; j < n; j += kBlock) {
-    int c = 0;
-    while (c < 7 && j >= s_cs[c + 1]) c++;
-    const uint8_t* r = s_cptr[c] + (j - s_cs[c]) * P.bpp;
-    int32_t X, Y, Z;
-    ld_xyz(r, X, Y, Z);
-    double x = coord(X, P.scale[0], P.offset[0]), y = coord(Y, P.scale[1], P.offset[1]),
-           z = coord(Z, P.scale[2], P.offset[2]);
-    double ax = x - g.cx, ay = y - g.cy, az = z - g.cz;
-    double add = ax * ax + ay * ay + az * az;
-    L.X[j] = X; L.Y[j] = Y; L.Z[j] = Z;
-    L.add[j] = add;
-    float f = (float)add;
-    uint32_t fb;
-    __builtin_memcpy(&fb, &f, 4);
-    L.keyA[j] = fb;
-    L.idxA[j] = (uint32_t)j;
-    if (P.method == METHOD_POISSON_AVERAGE) {
-      L.cell16[j] = pack16(cell16_axis(x, g.min[0], g.size[0]), cell16_axis(y, g.min[1], g.size[1]),
-                           cell16_axis(z, g.min[2], g.size[2]));
-      L.rgb[3 * j + 0] = ld_u16(r + P.rgb_offset);
-      L.rgb[3 * j + 1] = ld_u16(r + P.rgb_offset + 2);
-      L.rgb[3 * j + 2] = ld_u16(r + P.rgb_offset + 4);
-      for (int q = 0; q < 8; q++) L.sums[8 * j + q] = 0;
+  // gather (sampler_poisson.h:82-108), four records per thread in flight;
+  // the byte histograms of the sort keys are counted on the way
+  for (int64_t j0 = 0; j0 < n; j0 += 4 * kBlock) {
+    int32_t X4[4], Y4[4], Z4[4];
+    const uint8_t* r4[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t j = j0 + u * kBlock + tid;
+      r4[u] = nullptr;
+      X4[u] = Y4[u] = Z4[u] = 0;
+      if (j < n) {
+        int c = 0;
+        while (c < 7 && j >= s_cs[c + 1]) c++;
+        r4[u] = s_cptr[c] + (j - s_cs[c]) * P.bpp;
+        ld_xyz(r4[u], X4[u], Y4[u], Z4[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t j = j0 + u * kBlock + tid;
+      if (j >= n) continue;
+      const uint8_t* r = r4[u];
+      const int32_t X = X4[u], Y = Y4[u], Z = Z4[u];
+      double x = coord(X, P.scale[0], P.offset[0]), y = coord(Y, P.scale[1], P.offset[1]),
+             z = coord(Z, P.scale[2], P.offset[2]);
+      double ax = x - g.cx, ay = y - g.cy, az = z - g.cz;
+      double add = ax * ax + ay * ay + az * az;
+      L.X[j] = X; L.Y[j] = Y; L.Z[j] = Z;
+      L.add[j] = add;
+      float f = (float)add;
+      uint32_t fb;
+      __builtin_memcpy(&fb, &f, 4);
+      L.keyA[j] = fb;
+      L.idxA[j] = (uint32_t)j;
+      atomicAdd(&s_h4[0][fb & 255u], 1u);
+      atomicAdd(&s_h4[1][(fb >> 8) & 255u], 1u);
+      atomicAdd(&s_h4[2][(fb >> 16) & 255u], 1u);
+      atomicAdd(&s_h4[3][fb >> 24], 1u);
+      if (P.method == METHOD_POISSON_AVERAGE) {
+        L.cell16[j] = pack16(cell16_axis(x, g.min[0], g.size[0]), cell16_axis(y, g.min[1], g.size[1]),
+                             cell16_axis(z, g.min[2], g.size[2]));
+        L.rgb[3 * j + 0] = ld_u16(r + P.rgb_offset);
+        L.rgb[3 * j + 1] = ld_u16(r + P.rgb_offset + 2);
+        L.rgb[3 * j + 2] = ld_u16(r + P.rgb_offset + 4);
+        for (int q = 0; q < 8; q++) L.sums[8 * j + q] = 0;
+      }
     }
   }
   __syncthreads();
@@ -399,7 +438,7 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
   // to the same float, or equal distances) are put in (double add, main
   // index) order below
   for (int pass = 0; pass < 4; pass++) {
-    if (!block_radix_pass(kc, ic, ko, io, n, 8 * pass, 255u, s_rad)) continue;
+    if (!block_radix_pass(kc, ic, ko, io, n, 8 * pass, 255u, s_rad, s_h4[pass])) continue;
     uint32_t* t = kc; kc = ko; ko = t;
     t = ic; ic = io; io = t;
   }
@@ -1427,12 +1466,15 @@ __global__ __launch_bounds__(64 * kRingWaves) void k_ring(const int32_t* __restr
         }
       }
     }
-    // earlier candidates of the window, broadcast from their lanes: a
-    // branch-free pass sorts the pairs into clearly within spacing and
-    // inside the FP32 band; the band pairs get the FP64 check afterwards
+    // earlier candidates of the window, broadcast from their lanes: a pass
+    // sorts the pairs into clearly within spacing and inside the FP32 band
+    // (the band pairs get the FP64 check afterwards).  Only candidates that
+    // the ring did not reject are broadcast: a rejected one is never
+    // accepted, so it cannot exclude a later one.
     uint64_t m_band = 0;
-#pragma unroll 8
-    for (int q = 0; q < 63; q++) {
+    const uint64_t surv = __ballot(valid && !rej) & ~(1ull << 63);
+    for (uint64_t mm = surv; mm; mm &= mm - 1) {
+      const int q = __builtin_amdgcn_readfirstlane(__builtin_ctzll(mm));
       const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xf), q));
       const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(yf), q));
       const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(zf), q));
@@ -1446,7 +1488,7 @@ __global__ __launch_bounds__(64 * kRingWaves) void k_ring(const int32_t* __restr
       m_band |= (cand && !(d32 < G.band_lo) && !(d32 > G.band_hi)) ? bit : 0ull;
     }
     if (!valid || rej) { in_mask = 0; m_band = 0; }
-    tests += (uint64_t)lane;
+    tests += (uint64_t)__popcll(surv & lt);
     uint64_t band_any = 0;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
@@ -1742,6 +1784,7 @@ __global__ __launch_bounds__(kBlock) void k_route(const int32_t* __restrict__ no
       const int64_t local = j - w.child_start[c];
       const int64_t apos = run + woff + before_in_wave;
       if ((local & 63) == 0) L.tileAcc[tile_base[c] + (local >> 6)] = (uint32_t)apos;
+      L.accMain[j] = a ? 1 : 0;
       if (a) {
         if (avg) {
           long long rs[3], ws;
@@ -1828,13 +1871,26 @@ __global__ __launch_bounds__(1024) void k_tile_plan(const NodeWork* __restrict__
   if (tid == 0) tile_start[num] = s_run;
 }
 
+// 4 bytes at any address inside an LDS staging area (two aligned dword
+// reads and a byte align; the area is padded so the second read stays in
+// it; the aligned pointer comes from pointer arithmetic so that the compiler
+// keeps the LDS address space)
+__device__ inline uint32_t lds_u32(const uint8_t* p) {
+  const uint32_t sh = (uint32_t)((uintptr_t)p & 3);
+  const uint32_t* w = (const uint32_t*)(p - sh);
+  return __builtin_amdgcn_alignbyte(w[1], w[0], sh);
+}
+
 // Writes `count` records from the wave's staged tile (records `list[r]` of
 // the stage, in order) to dst + first * bpp: aligned dwords inside the range,
 // bytes at its edges; averaged colours replace RGB of accepted records when
-// `col` is set (Poisson-average).
+// `col` is set (Poisson-average).  An inner dword holds bytes of at most two
+// consecutive records (bpp >= 12): one or two unaligned 4-byte LDS reads;
+// the record of a byte offset comes from a multiply by `rmul` =
+// 2^21 / bpp + 1 (exact for offsets below 64 * 130 + 64 and 12 <= bpp <= 130).
 __device__ inline void write_run(uint8_t* __restrict__ dst, uint64_t first, uint32_t count, const uint8_t* stage,
-                                 const uint8_t* list, int bpp, const uint16_t* col, const int64_t* colj,
-                                 uint64_t colmask, int rgb0) {
+                                 const uint8_t* list, int bpp, uint32_t rmul, const uint16_t* col,
+                                 const int64_t* colj, uint64_t colmask, int rgb0) {
   const int lane = threadIdx.x & 63;
   const uint64_t g0 = first * (uint64_t)bpp;
   const uint32_t nbytes = count * (uint32_t)bpp;
@@ -1842,21 +1898,39 @@ __device__ inline void write_run(uint8_t* __restrict__ dst, uint64_t first, uint
   const uint64_t wfirst = g0 >> 2, wlast = (g0 + nbytes - 1) >> 2;
   for (uint64_t wi = wfirst + lane; wi <= wlast; wi += 64) {
     const uint64_t b0 = wi << 2;
+    const bool full = b0 >= g0 && b0 + 4 <= g0 + nbytes;
+    bool bytewise = !full;
     uint32_t word = 0;
-    bool full = true;
-    for (int t = 0; t < 4; t++) {
-      const uint64_t gb = b0 + t;
-      if (gb < g0 || gb >= g0 + nbytes) { full = false; continue; }
-      const uint32_t ob = (uint32_t)(gb - g0);
-      const uint32_t r = ob / (uint32_t)bpp;
-      const int bb = (int)(ob - r * (uint32_t)bpp);
+    if (full) {
+      const uint32_t ob = (uint32_t)(b0 - g0);
+      const uint32_t r = (ob * rmul) >> 21;
+      const uint32_t bb = ob - r * (uint32_t)bpp;
       const int l = list[r];
-      uint32_t byte = stage[l * bpp + bb];
-      if (col && ((colmask >> l) & 1ull) && bb >= rgb0 && bb < rgb0 + 6) {
-        const uint16_t v = col[3 * colj[l] + (bb - rgb0) / 2];
-        byte = ((bb - rgb0) & 1) ? (v >> 8) : (v & 0xFF);
+      word = lds_u32(stage + l * bpp + bb);
+      if (bb + 4 > (uint32_t)bpp) {
+        const uint32_t kb = 8 * ((uint32_t)bpp - bb);  // bits from record r
+        const uint32_t w2 = lds_u32(stage + list[r + 1] * bpp);
+        word = (word & ((1u << kb) - 1u)) | (w2 << kb);
       }
-      word |= byte << (8 * t);
+      // an accepted record's RGB bytes in this dword take the slow path
+      if (col && ((colmask >> l) & 1ull) && bb < (uint32_t)rgb0 + 6 && bb + 4 > (uint32_t)rgb0) bytewise = true;
+    }
+    if (bytewise) {
+      word = 0;
+      for (int t = 0; t < 4; t++) {
+        const uint64_t gb = b0 + t;
+        if (gb < g0 || gb >= g0 + nbytes) continue;
+        const uint32_t ob = (uint32_t)(gb - g0);
+        const uint32_t r = (ob * rmul) >> 21;
+        const int bb = (int)(ob - r * (uint32_t)bpp);
+        const int l = list[r];
+        uint32_t byte = stage[l * bpp + bb];
+        if (col && ((colmask >> l) & 1ull) && bb >= rgb0 && bb < rgb0 + 6) {
+          const uint16_t v = col[3 * colj[l] + (bb - rgb0) / 2];
+          byte = ((bb - rgb0) & 1) ? (v >> 8) : (v & 0xFF);
+        }
+        word |= byte << (8 * t);
+      }
     }
     if (full) {
       *(uint32_t*)(dst + b0) = word;
@@ -1887,6 +1961,7 @@ __global__ __launch_bounds__(kBlock) void k_copy_tiles(const NodeWork* __restric
   const int num = 8 * num_nodes;
   const uint32_t total = tile_start[num];
   const int bpp = P.bpp;
+  const uint32_t rmul = (1u << 21) / (uint32_t)bpp + 1u;
   const bool avg = P.method == METHOD_POISSON_AVERAGE;
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   uint8_t* stage = s_stage[wave];
@@ -1894,13 +1969,23 @@ __global__ __launch_bounds__(kBlock) void k_copy_tiles(const NodeWork* __restric
   uint8_t* rejList = s_list[wave][1];
   uint8_t* allList = s_list[wave][2];
   allList[lane] = (uint8_t)lane;
-  for (uint32_t item = blockIdx.x * kWaves + wave; item < total; item += gridDim.x * kWaves) {
+  // each wave takes a contiguous range of tiles: one search for its first
+  // tile's (node, child), then a forward walk
+  const uint32_t waves_total = gridDim.x * kWaves;
+  const uint32_t per = (total + waves_total - 1) / waves_total;
+  const uint32_t i0 = (blockIdx.x * kWaves + wave) * per;
+  const uint32_t i1 = (i0 + per) < total ? (i0 + per) : total;
+  int q = 0;
+  if (i0 < i1) {
     int lo = 0, hi = num - 1;
     while (lo < hi) {
       int mid = (lo + hi + 1) >> 1;
-      if (tile_start[mid] <= item) lo = mid; else hi = mid - 1;
+      if (tile_start[mid] <= i0) lo = mid; else hi = mid - 1;
     }
-    const int q = __builtin_amdgcn_readfirstlane(lo);
+    q = __builtin_amdgcn_readfirstlane(lo);
+  }
+  for (uint32_t item = i0; item < i1; item++) {
+    while (tile_start[q + 1] <= item) q++;  // tile_start[num] == total ends the walk
     const int k = q >> 3, c = q & 7;
     const NodeWork& w = work[k];
     const Layout L = make_layout(w.scratch, w.n, P.method);
@@ -1915,7 +2000,7 @@ __global__ __launch_bounds__(kBlock) void k_copy_tiles(const NodeWork* __restric
     for (int v = lane; v < nvec; v += 64) *(uint4*)(stage + 16 * v) = *(const uint4*)(a0 + 16 * (uint64_t)v);
     const bool valid = lane < cnt;
     const int64_t j = w.child_start[c] + (int64_t)t * 64 + lane;
-    const bool a = valid && L.acc[L.keyB[j]];
+    const bool a = valid && L.accMain[j];
     const uint64_t accm = __ballot(a), rejm = __ballot(valid && !a);
     if (a) accList[__popcll(accm & lt)] = (uint8_t)lane;
     if (valid && !a) rejList[__popcll(rejm & lt)] = (uint8_t)lane;
@@ -1929,15 +2014,16 @@ __global__ __launch_bounds__(kBlock) void k_copy_tiles(const NodeWork* __restric
     const uint32_t accChild = L.tileAcc[tb];
     const OutSeg* sg = P.segs + 9 * (int64_t)k;
     const uint8_t* stg = stage + b0;
-    write_run(sg[0].dst, accBefore, (uint32_t)__popcll(accm), stg, accList, bpp, avg ? L.rgb : nullptr, s_j[wave],
-              accm, P.rgb_offset);
+    write_run(sg[0].dst, accBefore, (uint32_t)__popcll(accm), stg, accList, bpp, rmul, avg ? L.rgb : nullptr,
+              s_j[wave], accm, P.rgb_offset);
     uint8_t* rd = sg[1 + c].dst;
     if (rd) {
       if (!avg)
-        write_run(rd, (uint64_t)t * 64 - (accBefore - accChild), (uint32_t)__popcll(rejm), stg, rejList, bpp, nullptr,
-                  nullptr, 0ull, 0);
+        write_run(rd, (uint64_t)t * 64 - (accBefore - accChild), (uint32_t)__popcll(rejm), stg, rejList, bpp, rmul,
+                  nullptr, nullptr, 0ull, 0);
       else
-        write_run(rd, (uint64_t)t * 64, (uint32_t)cnt, stg, allList, bpp, L.rgb, s_j[wave], accm, P.rgb_offset);
+        write_run(rd, (uint64_t)t * 64, (uint32_t)cnt, stg, allList, bpp, rmul, L.rgb, s_j[wave], accm,
+                  P.rgb_offset);
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -1953,8 +2039,8 @@ namespace pcg {
 int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_info, NodeState* d_state,
                  NodeWork* d_work, const SampleParams& p, hipStream_t s) {
   if (num_nodes == 0) return 0;
-  if (p.bpp > 130) {
-    set_error("record size above 130 bytes is not supported by the sampler");
+  if (p.bpp > 130 || p.bpp < 12) {
+    set_error("record size outside 12..130 bytes is not supported by the sampler");
     return 1;
   }
   dim3 grid((unsigned)num_nodes);
