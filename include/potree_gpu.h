@@ -38,7 +38,11 @@ int pcg_release(void);
  * results; the unbatched baseline the batched path is measured against).
  * "ring_min_nodes" (default 512): Poisson octree levels with at least this
  * many nodes decide acceptance with the accepted-shell ring kernel, the
- * others with the cell-table kernels (same results).
+ * others with the cell-table kernels (same results).  "sample_groups"
+ * (0-8, default 0): the chunk subtrees are sampled in this many sets, each
+ * on a stream of its own, before the nodes above the chunk roots (same
+ * results; 1 = one level-synchronous pass over the whole tree; 0 = one set
+ * per hardware queue besides the library stream's, GPU_MAX_HW_QUEUES - 1).
  * No reference counterpart. */
 int pcg_set_option(const char* key, int64_t value);
 

@@ -88,6 +88,14 @@ int pcg_set_option(const char* key, int64_t value) {
     g_ring_min_nodes = (int32_t)value;
     return 0;
   }
+  if (k == "sample_groups") {
+    if (value < 0 || value > kMaxSampleGroups) {
+      set_error("sample_groups must be in [0, " + std::to_string(kMaxSampleGroups) + "]");
+      return 1;
+    }
+    g_sample_groups = (int32_t)value;
+    return 0;
+  }
   set_error("unknown option: " + k);
   return 1;
 }

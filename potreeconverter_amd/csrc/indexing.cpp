@@ -85,10 +85,36 @@ struct DevMem {
   T* as() const { return (T*)p; }
 };
 
+// streams and events of the sampling groups (default priority, created once)
+int group_streams(int G, std::vector<hipStream_t>& out) {
+  static std::vector<hipStream_t> streams;
+  while ((int)streams.size() < G) {
+    hipStream_t st = nullptr;
+    PCG_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    streams.push_back(st);
+  }
+  out.assign(streams.begin(), streams.begin() + G);
+  return 0;
+}
+
+hipEvent_t group_event(int i) {
+  static std::vector<hipEvent_t> events;
+  while ((int)events.size() <= i) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      set_error("hipEventCreateWithFlags failed");
+      return nullptr;
+    }
+    events.push_back(e);
+  }
+  return events[i];
+}
+
 }  // namespace
 
 bool g_plain_sampling = false;
 int32_t g_ring_min_nodes = 512;
+int32_t g_sample_groups = 0;
 
 void IndexResult::release() {
   d_octree = nullptr;  // a workspace buffer
@@ -126,6 +152,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   tree.add("r", dc.min, dc.max, -1);
   tree.nodes[0].upper = true;
   std::vector<int32_t> chunkRoots;
+  std::vector<int64_t> chunkRootPoints;
   for (const auto& c : dc.chunks) {
     int32_t id;
     if (c.id == "r") {
@@ -146,6 +173,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     tree.nodes[id].off = c.offset;
     tree.nodes[id].count = c.count;
     chunkRoots.push_back(id);
+    chunkRootPoints.push_back(c.count);
   }
 
   // ------------------------------------------------------ octree build
@@ -268,43 +296,87 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     st.flags = 0;
     st.pad = 0;
   }
-  std::vector<std::vector<int32_t>> levels(maxLevel + 1);
   const bool singleChunk = chunkRoots.size() == 1;
-  for (int32_t i = 0; i < numNodes; i++)
-    if (tree.nodes[i].hasChildren() && !(singleChunk && tree.nodes[i].upper)) levels[info[i].level].push_back(i);
-  int64_t maxLevelNodes = 1;
-  for (auto& l : levels) maxLevelNodes = std::max<int64_t>(maxLevelNodes, (int64_t)l.size());
-  DevMem dinfo(WS_INFO), dstate(WS_STATE), dwork(WS_WORK), dlist(WS_LIST), arena(WS_ARENA), scratch(WS_SCRATCH),
-      tops(WS_TOPS), counters(WS_COUNTERS);
+  const char* envStats = getenv("PCG_LEVEL_STATS");
+  const bool levelStats = envStats && envStats[0] == '1';
+  // Sampling groups: the chunk subtrees are split into G sets (longest
+  // processing time first on their point counts); each set samples its own
+  // levels on a stream of its own, so that the narrow, latency-bound levels
+  // of one set overlap the wide levels of another.  The nodes above the
+  // chunk roots follow on the library stream once every set is done.  With
+  // one group every sampled node is in the level lists of that group.
+  int G = 1;
+  if (!g_plain_sampling && !levelStats && chunkRoots.size() > 1) {
+    int want = g_sample_groups;
+    if (want <= 0) {
+      // one group per hardware queue besides the library stream's: streams
+      // that share a queue run one after the other
+      const char* q = getenv("GPU_MAX_HW_QUEUES");
+      const int queues = q && atoi(q) > 0 ? atoi(q) : 4;
+      want = std::max(1, std::min(queues - 1, kMaxSampleGroups));
+    }
+    G = (int)std::min<size_t>((size_t)want, chunkRoots.size());
+  }
+  std::vector<int32_t> groupOf(numNodes, G > 1 ? -1 : 0);  // -1: above the chunk roots
+  std::vector<int64_t> groupPoints(G, 0);
+  if (G > 1) {
+    std::vector<size_t> crs(chunkRoots.size());
+    for (size_t i = 0; i < crs.size(); i++) crs[i] = i;
+    std::stable_sort(crs.begin(), crs.end(), [&](size_t x, size_t y) { return chunkRootPoints[x] > chunkRootPoints[y]; });
+    for (size_t ci : crs) {
+      int g = (int)(std::min_element(groupPoints.begin(), groupPoints.end()) - groupPoints.begin());
+      groupPoints[g] += chunkRootPoints[ci];
+      std::vector<int32_t> stack{chunkRoots[ci]};
+      while (!stack.empty()) {
+        int32_t id = stack.back();
+        stack.pop_back();
+        groupOf[id] = g;
+        for (int c = 0; c < 8; c++)
+          if (tree.nodes[id].children[c] >= 0) stack.push_back(tree.nodes[id].children[c]);
+      }
+    }
+  } else {
+    groupPoints[0] = N;
+  }
+  // level lists: group g in levels[g], the nodes above the chunk roots in
+  // levels[G] (empty with one group)
+  std::vector<std::vector<std::vector<int32_t>>> levels(G + 1, std::vector<std::vector<int32_t>>(maxLevel + 1));
+  for (int32_t i = 0; i < numNodes; i++) {
+    if (!tree.nodes[i].hasChildren() || (singleChunk && tree.nodes[i].upper)) continue;
+    const int g = groupOf[i] >= 0 ? groupOf[i] : G;
+    levels[g][info[i].level].push_back(i);
+  }
+  std::vector<int64_t> maxLevelNodes(G + 1, 1);
+  for (int g = 0; g <= G; g++)
+    for (auto& l : levels[g]) maxLevelNodes[g] = std::max<int64_t>(maxLevelNodes[g], (int64_t)l.size());
+  DevMem dinfo(WS_INFO), dstate(WS_STATE), dlist(WS_LIST), arena(WS_ARENA), counters(WS_COUNTERS),
+      gtop(WS_TOPS);
   const int methodId = avg ? METHOD_POISSON_AVERAGE : METHOD_POISSON;
   // every output range is rounded up to 16 bytes (9 ranges per node at most)
   uint64_t arenaCap = (uint64_t)N * bpp * (avg ? 4 : 3) + (1u << 20) + (uint64_t)numNodes * 9 * 16;
-  // every candidate of a level is a point of the chunk records
-  uint64_t scratchCap = sample_scratch_bound(N, maxLevelNodes, methodId);
-  // all level lists in one upload, deepest level first
+  // all level lists in one upload: per group, deepest level first
   std::vector<int32_t> allLevels;
-  std::vector<size_t> levelOffset(maxLevel + 1, 0);
-  for (int l = maxLevel; l >= 0; l--) {
-    levelOffset[l] = allLevels.size();
-    allLevels.insert(allLevels.end(), levels[l].begin(), levels[l].end());
-  }
-  DevMem ditems(WS_ITEMS), pool(WS_POOL), doutsegs(WS_OUTSEGS), dsegstart(WS_SEGSTART);
-  // neighbour lists longer than a row (dense clusters): 4 entries per point
-  const uint64_t poolCap = (uint64_t)N * 4 + 1024;
-  if (pool.alloc(sizeof(uint32_t) * poolCap) || doutsegs.alloc(sizeof(OutSeg) * 9 * maxLevelNodes) ||
-      dsegstart.alloc(sizeof(uint32_t) * (9 * maxLevelNodes + 1)))
-    return 1;
+  std::vector<std::vector<size_t>> levelOffset(G + 1, std::vector<size_t>(maxLevel + 1, 0));
+  for (int g = 0; g <= G; g++)
+    for (int l = maxLevel; l >= 0; l--) {
+      levelOffset[g][l] = allLevels.size();
+      allLevels.insert(allLevels.end(), levels[g][l].begin(), levels[g][l].end());
+    }
   if (dinfo.alloc(sizeof(NodeInfo) * numNodes) || dstate.alloc(sizeof(NodeState) * numNodes) ||
-      dwork.alloc(sizeof(NodeWork) * maxLevelNodes) || dlist.alloc(sizeof(int32_t) * (allLevels.size() + 1)) ||
-      ditems.alloc(sizeof(uint32_t) * (maxLevelNodes + 1)) || arena.alloc(arenaCap) || scratch.alloc(scratchCap) ||
-      tops.alloc(64) || counters.alloc(64))
+      dlist.alloc(sizeof(int32_t) * (allLevels.size() + 1)) || arena.alloc(arenaCap) || counters.alloc(64) ||
+      gtop.alloc(64))
     return 1;
-  if (!allLevels.empty())
-    PCG_CHECK(hipMemcpyAsync(dlist.p, allLevels.data(), sizeof(int32_t) * allLevels.size(), hipMemcpyHostToDevice, s));
-  PCG_CHECK(hipMemcpyAsync(dinfo.p, info.data(), sizeof(NodeInfo) * numNodes, hipMemcpyHostToDevice, s));
-  PCG_CHECK(hipMemcpyAsync(dstate.p, state.data(), sizeof(NodeState) * numNodes, hipMemcpyHostToDevice, s));
-  PCG_CHECK(hipMemsetAsync(tops.p, 0, 64, s));
-  PCG_CHECK(hipMemsetAsync(counters.p, 0, 64, s));
+  // per-group buffers: node work, item and segment tables, neighbour pool,
+  // scratch, and the group's scratch / pool tops
+  struct GroupMem {
+    DevMem work, items, outsegs, segstart, pool, scratch, tops;
+    explicit GroupMem(int g)
+        : work(ws_group_slot(g, 0)), items(ws_group_slot(g, 1)), outsegs(ws_group_slot(g, 2)),
+          segstart(ws_group_slot(g, 3)), pool(ws_group_slot(g, 4)), scratch(ws_group_slot(g, 5)),
+          tops(ws_group_slot(g, 6)) {}
+  };
+  std::vector<GroupMem> gm;
+  for (int g = 0; g < G; g++) gm.emplace_back(g);
   SampleParams sp;
   memset(&sp, 0, sizeof(sp));
   for (int a = 0; a < 3; a++) { sp.scale[a] = attributes.posScale[a]; sp.offset[a] = attributes.posOffset[a]; }
@@ -315,55 +387,123 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   sp.method = methodId;
   sp.arena = arena.as<uint8_t>();
   sp.arena_cap = arenaCap;
-  sp.arena_top = tops.as<uint64_t>();
-  sp.scratch = scratch.as<uint8_t>();
-  sp.scratch_cap = scratchCap;
-  sp.scratch_top = tops.as<uint64_t>() + 1;
+  sp.arena_top = gtop.as<uint64_t>();
   sp.counters = counters.as<unsigned long long>();
-  sp.item_start = ditems.as<uint32_t>();
-  sp.ring_min_nodes = g_ring_min_nodes;
-  sp.segs = doutsegs.as<OutSeg>();
-  sp.seg_start = dsegstart.as<uint32_t>();
-  sp.nbr_pool = pool.as<uint32_t>();
-  sp.nbr_pool_cap = poolCap;
-  sp.nbr_pool_top = (unsigned long long*)(tops.as<uint64_t>() + 2);
-  const char* envStats = getenv("PCG_LEVEL_STATS");
-  const bool levelStats = envStats && envStats[0] == '1';
-  int64_t sampled = 0;
-  for (int l = maxLevel; l >= 0; l--) {
-    const auto& lst = levels[l];
-    if (lst.empty()) continue;
-    sampled += (int64_t)lst.size();
-    PCG_CHECK(hipMemsetAsync(tops.as<uint64_t>() + 1, 0, 16, s));  // scratch and pool tops
-    if (g_plain_sampling) {
-      // one launch sequence per node (the scratch of every node of the level
-      // stays allocated until the level is done, as in the batched mode)
-      for (size_t q = 0; q < lst.size(); q++)
-        if (sample_level(dlist.as<int32_t>() + levelOffset[l] + q, 1, dinfo.as<NodeInfo>(), dstate.as<NodeState>(),
-                         dwork.as<NodeWork>() + q, sp, s))
-          return 1;
-    } else if (sample_level(dlist.as<int32_t>() + levelOffset[l], (int32_t)lst.size(), dinfo.as<NodeInfo>(),
-                            dstate.as<NodeState>(), dwork.as<NodeWork>(), sp, s)) {
+  // a group's buffers for levels of at most `maxNodes` nodes and `cand`
+  // candidates in all
+  auto group_params = [&](GroupMem& m, int64_t maxNodes, int64_t cand, SampleParams& p) -> int {
+    // every candidate of a level is a point of the group's chunk records;
+    // neighbour lists longer than a row (dense clusters): 4 entries per point
+    const uint64_t scratchCap = sample_scratch_bound(cand, maxNodes, methodId);
+    const uint64_t poolCap = (uint64_t)cand * 4 + 1024;
+    if (m.work.alloc(sizeof(NodeWork) * maxNodes) || m.items.alloc(sizeof(uint32_t) * (maxNodes + 1)) ||
+        m.outsegs.alloc(sizeof(OutSeg) * 9 * maxNodes) || m.segstart.alloc(sizeof(uint32_t) * (9 * maxNodes + 1)) ||
+        m.pool.alloc(sizeof(uint32_t) * poolCap) || m.scratch.alloc(scratchCap) || m.tops.alloc(64))
       return 1;
-    }
-    if (levelStats) {
-      // diagnostic summary of the level (PCG_LEVEL_STATS=1), costs a sync
-      std::vector<NodeWork> wk(lst.size());
-      PCG_CHECK(stream_copy(wk.data(), dwork.p, sizeof(NodeWork) * wk.size(), hipMemcpyDeviceToHost));
-      int64_t cand = 0, maxn = 0, flagged[6] = {0, 0, 0, 0, 0, 0};
-      for (const NodeWork& w : wk) {
-        cand += w.n;
-        maxn = std::max<int64_t>(maxn, w.n);
-        for (int b = 0; b < 6; b++) flagged[b] += (w.flags >> b) & 1;
+    p = sp;
+    p.scratch = m.scratch.as<uint8_t>();
+    p.scratch_cap = scratchCap;
+    p.scratch_top = m.tops.as<uint64_t>() + 1;
+    p.item_start = m.items.as<uint32_t>();
+    p.segs = m.outsegs.as<OutSeg>();
+    p.seg_start = m.segstart.as<uint32_t>();
+    p.nbr_pool = m.pool.as<uint32_t>();
+    p.nbr_pool_cap = poolCap;
+    p.nbr_pool_top = (unsigned long long*)(m.tops.as<uint64_t>() + 2);
+    // the ring needs enough nodes per launch to fill the GPU; concurrent
+    // groups share it
+    p.ring_min_nodes = std::max(1, g_ring_min_nodes / G);
+    return 0;
+  };
+  std::vector<SampleParams> gp(G);
+  for (int g = 0; g < G; g++)
+    if (group_params(gm[g], maxLevelNodes[g], groupPoints[g], gp[g])) return 1;
+  if (!allLevels.empty())
+    PCG_CHECK(hipMemcpyAsync(dlist.p, allLevels.data(), sizeof(int32_t) * allLevels.size(), hipMemcpyHostToDevice, s));
+  PCG_CHECK(hipMemcpyAsync(dinfo.p, info.data(), sizeof(NodeInfo) * numNodes, hipMemcpyHostToDevice, s));
+  PCG_CHECK(hipMemcpyAsync(dstate.p, state.data(), sizeof(NodeState) * numNodes, hipMemcpyHostToDevice, s));
+  PCG_CHECK(hipMemsetAsync(gtop.p, 0, 64, s));
+  PCG_CHECK(hipMemsetAsync(counters.p, 0, 64, s));
+  int64_t sampled = 0;
+  // one level of one list set on one stream
+  auto run_level = [&](int lg, int l, GroupMem& m, const SampleParams& p, hipStream_t st) -> int {
+    {
+      const auto& lst = levels[lg][l];
+      if (lst.empty()) return 0;
+      sampled += (int64_t)lst.size();
+      PCG_CHECK(hipMemsetAsync(m.tops.as<uint64_t>() + 1, 0, 16, st));  // scratch and pool tops
+      if (g_plain_sampling) {
+        // one launch sequence per node (the scratch of every node of the level
+        // stays allocated until the level is done, as in the batched mode)
+        for (size_t q = 0; q < lst.size(); q++)
+          if (sample_level(dlist.as<int32_t>() + levelOffset[lg][l] + q, 1, dinfo.as<NodeInfo>(),
+                           dstate.as<NodeState>(), m.work.as<NodeWork>() + q, p, st))
+            return 1;
+      } else if (sample_level(dlist.as<int32_t>() + levelOffset[lg][l], (int32_t)lst.size(), dinfo.as<NodeInfo>(),
+                              dstate.as<NodeState>(), m.work.as<NodeWork>(), p, st)) {
+        return 1;
       }
-      unsigned long long hc7 = 0;
-      PCG_CHECK(stream_copy(&hc7, counters.as<unsigned long long>() + 7, 8, hipMemcpyDeviceToHost));
-      fprintf(stderr,
-              "[level %d] nodes %zu candidates %lld max %lld exact-sort %lld seq-accept %lld seq-average %lld "
-              "overflow %lld many-neighbours %lld cell-path %lld (long lists so far %llu)\n",
-              l, lst.size(), (long long)cand, (long long)maxn, (long long)flagged[0], (long long)flagged[1],
-              (long long)flagged[2], (long long)flagged[3], (long long)flagged[4], (long long)flagged[5], hc7);
+      if (levelStats) {
+        // diagnostic summary of the level (PCG_LEVEL_STATS=1), costs a sync
+        std::vector<NodeWork> wk(lst.size());
+        PCG_CHECK(stream_copy(wk.data(), m.work.p, sizeof(NodeWork) * wk.size(), hipMemcpyDeviceToHost));
+        int64_t cand = 0, maxn = 0, flagged[6] = {0, 0, 0, 0, 0, 0};
+        for (const NodeWork& w : wk) {
+          cand += w.n;
+          maxn = std::max<int64_t>(maxn, w.n);
+          for (int b = 0; b < 6; b++) flagged[b] += (w.flags >> b) & 1;
+        }
+        unsigned long long hc7 = 0;
+        PCG_CHECK(stream_copy(&hc7, counters.as<unsigned long long>() + 7, 8, hipMemcpyDeviceToHost));
+        fprintf(stderr,
+                "[level %d] nodes %zu candidates %lld max %lld exact-sort %lld seq-accept %lld seq-average %lld "
+                "overflow %lld many-neighbours %lld cell-path %lld (long lists so far %llu)\n",
+                l, lst.size(), (long long)cand, (long long)maxn, (long long)flagged[0], (long long)flagged[1],
+                (long long)flagged[2], (long long)flagged[3], (long long)flagged[4], (long long)flagged[5], hc7);
+      }
     }
+    return 0;
+  };
+  auto run_levels = [&](int lg, GroupMem& m, const SampleParams& p, hipStream_t st) -> int {
+    for (int l = maxLevel; l >= 0; l--)
+      if (run_level(lg, l, m, p, st)) return 1;
+    return 0;
+  };
+  if (G == 1) {
+    if (run_levels(0, gm[0], gp[0], s)) return 1;
+  } else {
+    std::vector<hipStream_t> gs;
+    if (group_streams(G, gs)) return 1;
+    hipEvent_t ev_start = group_event(0);
+    if (!ev_start) return 1;
+    PCG_CHECK(hipEventRecord(ev_start, s));
+    // every group's work is enqueued before the library stream's waits, so
+    // that a stream sharing a hardware queue with the library stream is
+    // never queued behind a wait for another group
+    // levels are enqueued round-robin over the groups so that every group
+    // starts at once
+    for (int g = 0; g < G; g++) PCG_CHECK(hipStreamWaitEvent(gs[g], ev_start, 0));
+    for (int l = maxLevel; l >= 0; l--)
+      for (int g = 0; g < G; g++)
+        if (run_level(g, l, gm[g], gp[g], gs[g])) return 1;
+    for (int g = 0; g < G; g++) {
+      hipEvent_t ev_done = group_event(1 + g);
+      if (!ev_done) return 1;
+      PCG_CHECK(hipEventRecord(ev_done, gs[g]));
+    }
+    for (int g = 0; g < G; g++) PCG_CHECK(hipStreamWaitEvent(s, group_event(1 + g), 0));
+    // the nodes above the chunk roots: their candidates are points the
+    // chunk roots kept
+    int64_t upperCand = 0;
+    {
+      std::vector<NodeState> st2(numNodes);
+      PCG_CHECK(stream_copy(st2.data(), dstate.p, sizeof(NodeState) * numNodes, hipMemcpyDeviceToHost));
+      for (int32_t cr : chunkRoots) upperCand += st2[cr].count;
+    }
+    SampleParams up;
+    if (group_params(gm[0], maxLevelNodes[G], upperCand, up)) return 1;
+    up.ring_min_nodes = g_ring_min_nodes;
+    if (run_levels(G, gm[0], up, s)) return 1;
   }
   out.numSampled = sampled;
   PCG_CHECK(stream_copy(state.data(), dstate.p, sizeof(NodeState) * numNodes, hipMemcpyDeviceToHost));

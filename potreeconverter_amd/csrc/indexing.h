@@ -41,6 +41,10 @@ extern bool g_plain_sampling;
 // LDS shell ring (one wavefront per node); narrower levels, where one
 // wavefront per node cannot hide latency, use the cell-table path.
 extern int32_t g_ring_min_nodes;
+// Chunk subtrees are sampled in this many sets, each on a stream of its own
+// (1 .. kMaxSampleGroups; 1: one level-synchronous pass over the whole tree;
+// 0: one set per hardware queue besides the library stream's).
+extern int32_t g_sample_groups;
 
 int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string& name, int64_t pointsTotal,
                  IndexResult& out);

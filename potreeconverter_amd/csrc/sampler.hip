@@ -2081,12 +2081,16 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
 uint64_t sample_scratch_bytes(int64_t n, int method) { return layout_bytes(n, method); }
 
 uint64_t sample_scratch_bound(int64_t candidates, int64_t max_nodes, int method) {
-  // linear bytes per candidate, measured where the hash holds exactly 2n
-  // slots (four uint32 arrays of H slots); a node's hash never exceeds
-  // max(minimum, 4n) slots, and every array adds at most 15 alignment bytes
+  // A node of n candidates takes its per-candidate arrays (A bytes per
+  // candidate, each array rounded up to 16 bytes), 24 bytes per hash slot
+  // (htab, hkey, hhead) for H(n) <= max(minimum, 4n) slots, and 64 bytes.
+  // A is measured at n = 2^20 with the hash taken out; layout_bytes(1)
+  // covers the minimum hash, the 64 bytes and 16 bytes per array, which
+  // bounds the rounding of every array.  So layout_bytes(n) <= (A + 96) n +
+  // layout_bytes(1) for every n.
   const int64_t m = int64_t(1) << 20;
-  const uint64_t lin = (layout_bytes(m, method) - 64 - 16ull * (uint64_t)hash_cap(m, method)) / (uint64_t)m;
-  return (uint64_t)candidates * (lin + 64) + (uint64_t)max_nodes * (layout_bytes(1, method) + 512);
+  const uint64_t lin = (layout_bytes(m, method) - 64 - 24ull * (uint64_t)hash_cap(m, method)) / (uint64_t)m + 1;
+  return (uint64_t)candidates * (lin + 96) + (uint64_t)max_nodes * (layout_bytes(1, method) + 512);
 }
 
 namespace {

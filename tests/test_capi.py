@@ -38,3 +38,5 @@ def test_unknown_option_is_an_error():
     assert L.pcg_set_option(b"no_such_option", 1) != 0
     assert b"unknown option" in L.pcg_last_error()
     assert L.pcg_set_option(b"ring_min_nodes", 0) != 0
+    assert L.pcg_set_option(b"sample_groups", -1) != 0
+    assert L.pcg_set_option(b"sample_groups", 9) != 0

@@ -279,3 +279,22 @@ def test_ring_and_cell_paths_agree_at_20m(tmp_path):
         L.pcg_device_free(dq)
     for o in outs[1:]:
         assert all(same_files(outs[0], o, ["octree.bin", "hierarchy.bin", "metadata.json"]).values())
+
+
+def test_sample_groups_agree(tmp_path):
+    """Chunk subtrees sampled in 1, 3 and 8 concurrent groups: the same bytes
+    (and the oracle's)."""
+    las = tmp_path / "in.las"
+    make_las(las, 0, 2_000_000, 7, False)
+    cpu = tmp_path / "cpu"
+    oracle_convert(las, cpu, "poisson_average")
+    try:
+        for g in (1, 3, 8):
+            print(f"groups case {g}", flush=True)
+            _set("sample_groups", g)
+            out = tmp_path / f"g{g}"
+            pc.convert(las, out, "poisson_average")
+            res = same_files(cpu, out, ["octree.bin", "hierarchy.bin", "metadata.json"])
+            assert all(res.values()), (g, res)
+    finally:
+        _set("sample_groups", 0)
