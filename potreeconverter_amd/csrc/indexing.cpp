@@ -469,8 +469,50 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
       if (run_level(lg, l, m, p, st)) return 1;
     return 0;
   };
+  // ------------------------- completion order, phase 1 (host, meanwhile)
+  // The walk of phase 1 depends on the tree only; which children it
+  // completes depends on the sampling results, so the walk records every
+  // child it would complete and the filter runs once the results are back.
+  std::vector<uint8_t> sampledFlag(numNodes, 0);
+  std::vector<int32_t> order;  // node ids in onNodeCompleted order
+  std::vector<int32_t> phase1Cand;
+  bool collectPhase1 = true;
+  auto complete_children = [&](int32_t id) {
+    if (!collectPhase1) {
+      for (int c = 0; c < 8; c++) {
+        int32_t ch = tree.nodes[id].children[c];
+        if (ch >= 0 && (state[ch].flags & NS_COMPLETED)) order.push_back(ch);
+      }
+      return;
+    }
+    for (int c = 0; c < 8; c++) {
+      int32_t ch = tree.nodes[id].children[c];
+      if (ch >= 0) phase1Cand.push_back(ch);
+    }
+  };
+  // sample(): post-order over not-yet-sampled children (sampler_poisson.h:27-36)
+  // phase 1 samples each chunk subtree; afterwards the chunk roots are
+  // leaves of the upper tree (their children were cleared, indexer.cpp:1675)
+  std::function<void(int32_t, bool)> sample_rec = [&](int32_t id, bool phase1) {
+    if (!phase1 && tree.nodes[id].chunkRoot) {
+      sampledFlag[id] = 1;
+      return;
+    }
+    for (int c = 0; c < 8; c++) {
+      int32_t ch = tree.nodes[id].children[c];
+      if (ch < 0 || sampledFlag[ch]) continue;
+      sample_rec(ch, phase1);
+    }
+    sampledFlag[id] = 1;
+    if (tree.nodes[id].hasChildren()) complete_children(id);
+  };
+  auto walk_phase1 = [&]() {
+    for (int32_t id : chunkRoots) sample_rec(id, true);  // ascending chunk ID
+    collectPhase1 = false;
+  };
   if (G == 1) {
     if (run_levels(0, gm[0], gp[0], s)) return 1;
+    walk_phase1();
   } else {
     std::vector<hipStream_t> gs;
     if (group_streams(G, gs)) return 1;
@@ -492,6 +534,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
       PCG_CHECK(hipEventRecord(ev_done, gs[g]));
     }
     for (int g = 0; g < G; g++) PCG_CHECK(hipStreamWaitEvent(s, group_event(1 + g), 0));
+    walk_phase1();
     // the nodes above the chunk roots: their candidates are points the
     // chunk roots kept
     int64_t upperCand = 0;
@@ -518,31 +561,10 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   out.t_sample = ts1 - tb1;
 
   // --------------------------------------------- canonical completion order
-  std::vector<uint8_t> sampledFlag(numNodes, 0);
-  std::vector<int32_t> order;  // node ids in onNodeCompleted order
-  auto complete_children = [&](int32_t id) {
-    for (int c = 0; c < 8; c++) {
-      int32_t ch = tree.nodes[id].children[c];
-      if (ch >= 0 && (state[ch].flags & NS_COMPLETED)) order.push_back(ch);
-    }
-  };
-  // sample(): post-order over not-yet-sampled children (sampler_poisson.h:27-36)
-  // phase 1 samples each chunk subtree; afterwards the chunk roots are
-  // leaves of the upper tree (their children were cleared, indexer.cpp:1675)
-  std::function<void(int32_t, bool)> sample_rec = [&](int32_t id, bool phase1) {
-    if (!phase1 && tree.nodes[id].chunkRoot) {
-      sampledFlag[id] = 1;
-      return;
-    }
-    for (int c = 0; c < 8; c++) {
-      int32_t ch = tree.nodes[id].children[c];
-      if (ch < 0 || sampledFlag[ch]) continue;
-      sample_rec(ch, phase1);
-    }
-    sampledFlag[id] = 1;
-    if (tree.nodes[id].hasChildren()) complete_children(id);
-  };
-  for (int32_t id : chunkRoots) sample_rec(id, true);  // ascending chunk ID
+  // phase 1 was walked while the GPU sampled; keep the children that
+  // completed
+  for (int32_t id : phase1Cand)
+    if (state[id].flags & NS_COMPLETED) order.push_back(id);
   // processChunkRoots (indexer.cpp:202-281): subtrees of the upper tree with
   // fewer than 5M chunk-root points become one task; tasks in pre-order
   {
@@ -624,16 +646,16 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     if (gather_ranges(dcopies.as<CopyDesc>(), (int64_t)copies.size(), out.d_octree, s)) return 1;
   }
   const double ta0 = now_s();
-  PCG_CHECK(hipStreamSynchronize(s));
-  const double ta1 = now_s();
-  out.t_assemble = ta1 - ts1;
   out.t_order = ta0 - ts1;
-
-  // root name is the chunk id when there is a single chunk: keep its name
+  // hierarchy.bin and metadata.json on the host while the GPU gathers
+  // octree.bin (root name is the chunk id when there is a single chunk)
   out.hierarchy = build_hierarchy_bin(out.completed, out.firstChunkSize);
   out.metadata = create_metadata_json(attributes, dc.min, dc.max, spacing, name, pointsTotal, out.firstChunkSize,
                                       octreeDepth);
-  out.t_hierarchy = now_s() - ta1;
+  const double ta1 = now_s();
+  out.t_hierarchy = ta1 - ta0;
+  PCG_CHECK(hipStreamSynchronize(s));
+  out.t_assemble = now_s() - ts1;
   out.t_setup = tb0 - t0;
   out.t_total = now_s() - t0;
   return 0;

@@ -190,6 +190,22 @@ __device__ inline uint64_t wave_match_bits(uint32_t key, int bits, uint64_t acti
 
 // stable scatter of records + keys for one digit; each wave owns a
 // contiguous quarter of the tile (same scheme as the chunker scatter)
+// bytes of one wave's record staging area and output staging area in LDS
+__host__ __device__ inline int seg_area_bytes(int bpp) { return (64 * bpp + 48 + 15) & ~15; }
+
+// 4 bytes at any address inside an LDS staging area (two aligned dword
+// reads and a byte align; the areas are padded so the second read stays in
+// them)
+__device__ inline uint32_t lds_u32(const uint8_t* p) {
+  const uint32_t sh = (uint32_t)((uintptr_t)p & 3);
+  const uint32_t* w = (const uint32_t*)(p - sh);
+  return __builtin_amdgcn_alignbyte(w[1], w[0], sh);
+}
+
+// The 64 records of a step are assembled in LDS grouped into runs of one
+// digit (a run is contiguous in the output: the stable order of equal
+// digits is the lane order), and every run is written with aligned dword
+// stores, bytes only where a run starts or ends inside a word.
 __global__ __launch_bounds__(kBlock) void k_seg_scatter(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
                                                          const uint16_t* __restrict__ ksrc,
                                                          uint16_t* __restrict__ kdst, int32_t bpp,
@@ -197,10 +213,12 @@ __global__ __launch_bounds__(kBlock) void k_seg_scatter(const uint8_t* __restric
                                                          const SegDesc* __restrict__ segs, int shift, int bins,
                                                          const uint32_t* __restrict__ M) {
   __shared__ uint32_t s_cnt[kWaves][256];
-  __shared__ __attribute__((aligned(16))) uint8_t s_rec[kWaves][64 * kMaxBpp + 32];
+  extern __shared__ __attribute__((aligned(16))) uint8_t s_dyn[];  // per wave: record area, output area
   const TileDesc t = tiles[blockIdx.x];
   const int64_t delta = segs[t.seg].delta;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint8_t* rec_area = s_dyn + (size_t)wave * 2 * seg_area_bytes(bpp);
+  uint8_t* out_area = rec_area + seg_area_bytes(bpp);
   const int bits = bins == 256 ? 8 : 7;
   for (int i = threadIdx.x; i < kWaves * 256; i += kBlock) (&s_cnt[0][0])[i] = 0;
   __syncthreads();
@@ -234,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_scatter(const uint8_t* __restric
       uint64_t a0 = b0 & ~15ull;
       uint64_t end = b0 + (uint64_t)cnt * bpp;
       int nvec = (int)((end - a0 + 15) >> 4);
-      for (int v = lane; v < nvec; v += 64) *(uint4*)(s_rec[wave] + 16 * v) = *(const uint4*)(src + a0 + 16 * (uint64_t)v);
+      for (int v = lane; v < nvec; v += 64) *(uint4*)(rec_area + 16 * v) = *(const uint4*)(src + a0 + 16 * (uint64_t)v);
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -246,13 +264,44 @@ __global__ __launch_bounds__(kBlock) void k_seg_scatter(const uint8_t* __restric
     uint64_t peers = wave_match_bits(d, bits, __ballot(valid));
     uint32_t base = valid ? s_cnt[wave][d] : 0;
     __builtin_amdgcn_wave_barrier();
-    if (valid && (peers & lt_mask) == 0) s_cnt[wave][d] = base + (uint32_t)__popcll(peers);
+    const bool leader = valid && (peers & lt_mask) == 0;
+    const uint32_t len = leader ? (uint32_t)__popcll(peers) : 0u;
+    if (leader) s_cnt[wave][d] = base + len;
+    // runs in the output area in leader-lane order
+    uint32_t incl = len;
+#pragma unroll
+    for (int dd = 1; dd < 64; dd <<= 1) {
+      const uint32_t o = __shfl_up(incl, dd);
+      if (lane >= dd) incl += o;
+    }
+    const uint32_t run_start = incl - len;  // valid on leader lanes
+    const int my_leader = valid ? __builtin_ctzll(peers) : lane;
+    const uint32_t my_run = __shfl(run_start, my_leader);
     if (valid) {
-      int64_t pos = (int64_t)base + __popcll(peers & lt_mask) + delta;
-      const uint8_t* r = s_rec[wave] + shift_b + lane * bpp;
-      uint8_t* o = dst + pos * bpp;
+      const uint8_t* r = rec_area + shift_b + lane * bpp;
+      uint8_t* o = out_area + (size_t)(my_run + (uint32_t)__popcll(peers & lt_mask)) * bpp;
       for (int b = 0; b < bpp; b++) o[b] = r[b];
-      kdst[pos] = key;
+      kdst[(int64_t)base + __popcll(peers & lt_mask) + delta] = key;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    uint64_t leaders = __ballot(leader);
+    while (leaders) {
+      const int L = __builtin_ctzll(leaders);
+      leaders &= leaders - 1;
+      const uint64_t g0 = ((uint64_t)__builtin_amdgcn_readlane(base, L) + (uint64_t)delta) * (uint64_t)bpp;
+      const uint32_t nbytes = (uint32_t)__builtin_amdgcn_readlane(len, L) * (uint32_t)bpp;
+      const uint8_t* sr = out_area + (size_t)__builtin_amdgcn_readlane(run_start, L) * bpp;
+      const uint64_t wfirst = g0 >> 2, wlast = (g0 + nbytes - 1) >> 2;
+      for (uint64_t wi = wfirst + lane; wi <= wlast; wi += 64) {
+        const uint64_t b0 = wi << 2;
+        if (b0 >= g0 && b0 + 4 <= g0 + nbytes) {
+          *(uint32_t*)(dst + b0) = lds_u32(sr + (b0 - g0));
+        } else {
+          for (uint64_t bb = b0; bb < b0 + 4; bb++)
+            if (bb >= g0 && bb < g0 + nbytes) dst[bb] = sr[bb - g0];
+        }
+      }
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -286,8 +335,15 @@ int octree_round(uint8_t* d_a, uint8_t* d_b, uint16_t* d_ka, uint16_t* d_kb, int
     hipLaunchKernelGGL(k_seg_hist, dim3((unsigned)num_tiles), dim3(kBlock), 0, s, ks, d_tiles, shift, bins, d_M);
     PCG_CHECK(hipGetLastError());
     if (exclusive_scan_u32(d_M, m_total, d_scan_ws, nullptr, s)) return 1;
-    hipLaunchKernelGGL(k_seg_scatter, dim3((unsigned)num_tiles), dim3(kBlock), 0, s, src, dst, ks, kd, bpp, d_tiles,
-                       d_segs, shift, bins, d_M);
+    static bool lds_set = false;  // staging above 64 KB for records near kMaxBpp
+    if (!lds_set) {
+      PCG_CHECK(hipFuncSetAttribute((const void*)k_seg_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    kWaves * 2 * seg_area_bytes(kMaxBpp)));
+      lds_set = true;
+    }
+    hipLaunchKernelGGL(k_seg_scatter, dim3((unsigned)num_tiles), dim3(kBlock),
+                       (size_t)kWaves * 2 * seg_area_bytes(bpp), s, src, dst, ks, kd, bpp, d_tiles, d_segs, shift,
+                       bins, d_M);
     PCG_CHECK(hipGetLastError());
   }
   return 0;

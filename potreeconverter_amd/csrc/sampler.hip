@@ -728,9 +728,34 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
   // element are not followed further.
   uint8_t* mark = L.acc;  // by main index
   for (int64_t i = tid; i < n; i += kBlock) mark[i] = 0;
-  if (tid == 0) s_any = 0;
+  __shared__ int s_cap;
+  if (tid == 0) { s_any = 0; s_cap = 0; }
   __syncthreads();
-  const bool all_ties = P.method == METHOD_POISSON && n > 10001;
+  // Can the 10 001-check window bind?  The test of k_accept, per 64-candidate
+  // window: more than 10 000 candidates in [min limSq of the window, add).
+  // Only then does the order of ties that cannot exclude each other matter
+  // (the sequential form then scans the accepted points in sorted order).
+  if (P.method == METHOD_POISSON && n > 10001) {
+    for (int64_t w0 = 64 * (int64_t)tid; w0 < n; w0 += 64 * (int64_t)kBlock) {
+      if (w0 + 64 <= 10001) continue;
+      const int64_t hi = w0 + 64 < n ? w0 + 64 : n;
+      double ml = 1e308;
+      for (int64_t p = w0; p < hi; p++) {
+        const double cd = __builtin_sqrt(L.sadd[p]);
+        const double lim = cd - g.spacing;
+        const double ls = lim * lim;
+        ml = ls < ml ? ls : ml;
+      }
+      int64_t lo = 0, h = hi;  // lower bound of ml in sadd[0, hi)
+      while (lo < h) {
+        const int64_t mid = lo + (h - lo) / 2;
+        if (L.sadd[mid] < ml) lo = mid + 1; else h = mid;
+      }
+      if (hi - 1 - lo > 10000) s_cap = 1;
+    }
+  }
+  __syncthreads();
+  const bool all_ties = s_cap != 0;
   for (int64_t p = tid; p + 1 < n; p += kBlock) {
     if (!(L.sadd[p] == L.sadd[p + 1]) || (p > 0 && L.sadd[p - 1] == L.sadd[p])) continue;
     int64_t e = p + 1;
@@ -1165,12 +1190,12 @@ __device__ inline int64_t wave_lower_bound(const double* a, int64_t lo, int64_t 
 // window; inside the window the dependencies are resolved by ballots.  The
 // accepted flags of the node live in LDS (global memory past 65536
 // candidates).
-__global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
-                                                    NodeWork* __restrict__ work, int32_t num_nodes, SampleParams P) {
-  __shared__ uint32_t s_bits[kWaves][kAccWords];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int k = __builtin_amdgcn_readfirstlane(blockIdx.x * kWaves + wave);
-  if (k >= num_nodes) return;
+// Greedy acceptance of node k by one wavefront; the accepted bits live in
+// `lds_bits` (lds_words words) when they fit, in global memory otherwise.
+__device__ void accept_node(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
+                            NodeWork* __restrict__ work, int k, const SampleParams& P, uint32_t* lds_bits,
+                            int64_t lds_words) {
+  const int lane = threadIdx.x & 63;
   const NodeWork w = work[k];
   if ((w.flags & NW_OVERFLOW) || !(w.flags & NW_CELL_PATH)) return;
   const int64_t n = w.n;
@@ -1178,8 +1203,8 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
   const NodeGeom g = node_geom(infos[nodes[k]], P);
   const bool avg = P.method == METHOD_POISSON_AVERAGE;
   Filter F = make_filter(g, P);
-  const bool lds = n <= (int64_t)kAccWords * 32;
-  uint32_t* bits = lds ? s_bits[wave] : L.keyA;
+  const bool lds = n <= lds_words * 32;
+  uint32_t* bits = lds ? lds_bits : L.keyA;
   const int64_t words = (n + 31) / 32;
   for (int64_t i = lane; i < words; i += 64) bits[i] = 0;
   if (!lds) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
@@ -1227,15 +1252,22 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
           }
         }
       } else if (cnt == kNbrPooled) {
+        // a long list: four entries in flight per step
         const uint32_t* lst = P.nbr_pool + ((uint64_t)row[0] | ((uint64_t)row[1] << 32));
         const int len = (int)row[2];
-        for (int i = 0; i < len; i++) {
-          const uint32_t q = lst[i];
-          if ((int64_t)q >= w0) {
-            in_mask |= 1ull << (q - w0);
-          } else if ((bits[q >> 5] >> (q & 31)) & 1u) {
-            rej = true;
-            break;
+        for (int i0 = 0; i0 < len && !rej; i0 += 4) {
+          uint32_t q4[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) q4[u] = i0 + u < len ? lst[i0 + u] : 0xFFFFFFFFu;
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const uint32_t q = q4[u];
+            if (q == 0xFFFFFFFFu || rej) continue;
+            if ((int64_t)q >= w0) {
+              in_mask |= 1ull << (q - w0);
+            } else if ((bits[q >> 5] >> (q & 31)) & 1u) {
+              rej = true;
+            }
           }
         }
       } else {
@@ -1306,6 +1338,18 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
     }
   }
   flush_filter(F, P);
+}
+
+// one wavefront per node (accepted bits past kAccWords * 32 candidates in
+// global memory: a kernel holding a large LDS share would wait for CUs that
+// the other sampling streams keep busy)
+__global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
+                                                    NodeWork* __restrict__ work, int32_t num_nodes, SampleParams P) {
+  __shared__ uint32_t s_bits[kWaves][kAccWords];
+  const int wave = threadIdx.x >> 6;
+  const int k = __builtin_amdgcn_readfirstlane(blockIdx.x * kWaves + wave);
+  if (k >= num_nodes) return;
+  accept_node(nodes, infos, work, k, P, s_bits[wave], kAccWords);
 }
 
 // ------------------------------------------------------ accepted shell ring
@@ -2063,6 +2107,7 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   hipLaunchKernelGGL(k_accept, dim3((unsigned)((num_nodes + kWaves - 1) / kWaves)), dim3(kBlock), 0, s, d_nodes,
                      d_info, d_work, num_nodes, p);
   PCG_CHECK(hipGetLastError());
+
   if (p.method == METHOD_POISSON_AVERAGE) {
     hipLaunchKernelGGL(k_avg_colors, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p);
     PCG_CHECK(hipGetLastError());
