@@ -211,7 +211,8 @@ constexpr int kRadixE = 4;  // keys per thread per tile
 struct RadixShared {
   uint32_t hist[256];
   uint32_t base[256];
-  uint32_t wc[kRadixE][kWaves][256];  // per sub-tile and wave digit counts
+  uint32_t wc[kRadixE][kWaves][256];  // per sub-tile and wave digit counts (zero between tiles)
+  uint32_t wp[kRadixE][kWaves][256];  // their exclusive prefix per digit, from the digit's base
   int skip;
 };
 
@@ -283,26 +284,31 @@ __device__ bool block_radix_pass(const uint32_t* __restrict__ kin, const uint32_
       if (valid[e] && (peers[e] & lt) == 0) sh.wc[e][wave][d[e]] = (uint32_t)__popcll(peers[e]);
     }
     __syncthreads();
+    // per digit (one per thread): exclusive prefix over (sub-tile, wave)
+    // from the digit's base; counts back to zero for the next tile
+    {
+      uint32_t run = sh.base[tid];
+#pragma unroll
+      for (int e = 0; e < kRadixE; e++)
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) {
+          const uint32_t c = sh.wc[e][w][tid];
+          sh.wc[e][w][tid] = 0;
+          sh.wp[e][w][tid] = run;
+          run += c;
+        }
+      sh.base[tid] = run;
+    }
+    __syncthreads();
 #pragma unroll
     for (int e = 0; e < kRadixE; e++) {
       if (!valid[e]) continue;
-      uint32_t pos = sh.base[d[e]] + (uint32_t)__popcll(peers[e] & lt);
-      for (int e2 = 0; e2 < e; e2++)
-        for (int w = 0; w < kWaves; w++) pos += sh.wc[e2][w][d[e]];
-      for (int w = 0; w < wave; w++) pos += sh.wc[e][w][d[e]];
+      const uint32_t pos = sh.wp[e][wave][d[e]] + (uint32_t)__popcll(peers[e] & lt);
       kout[pos] = key[e];
       vout[pos] = val[e];
     }
-    __syncthreads();
-    uint32_t add = 0;
-    for (int e = 0; e < kRadixE; e++)
-      for (int w = 0; w < kWaves; w++) {
-        add += sh.wc[e][w][tid];
-        sh.wc[e][w][tid] = 0;
-      }
-    sh.base[tid] += add;
-    __syncthreads();
   }
+  __syncthreads();
   return true;
 }
 
@@ -980,6 +986,7 @@ __device__ inline CellBox cell_box(const NodeGeom& g, double x, double y, double
 __global__ __launch_bounds__(kBlock) void k_cells(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
                                                    NodeWork* __restrict__ work, SampleParams P) {
   __shared__ RadixShared s_rad;
+  __shared__ uint32_t s_h3[3][256];  // histograms of the three 7-bit key digits, counted with the keys
   const int k = blockIdx.x;
   const NodeWork w = work[k];
   if ((w.flags & NW_OVERFLOW) || !(w.flags & NW_CELL_PATH)) return;
@@ -989,18 +996,42 @@ __global__ __launch_bounds__(kBlock) void k_cells(const int32_t* __restrict__ no
   const NodeGeom g = node_geom(infos[nodes[k]], P);
   const bool avg = P.method == METHOD_POISSON_AVERAGE;
   const double cs = 2.0 * g.spacing;
-  for (int64_t p = tid; p < n; p += kBlock) {
-    uint32_t j = L.idxA[p];
-    double x = coord(L.X[j], P.scale[0], P.offset[0]), y = coord(L.Y[j], P.scale[1], P.offset[1]),
-           z = coord(L.Z[j], P.scale[2], P.offset[2]);
-    L.keyA[p] = gkey(gcell(x, g.min[0], cs), gcell(y, g.min[1], cs), gcell(z, g.min[2], cs));
-    L.idxB[p] = (uint32_t)p;
+  for (int i = tid; i < 3 * 256; i += kBlock) (&s_h3[0][0])[i] = 0;
+  __syncthreads();
+  // four candidates per thread in flight
+  for (int64_t p0 = 0; p0 < n; p0 += 4 * kBlock) {
+    uint32_t j4[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = p0 + u * kBlock + tid;
+      j4[u] = p < n ? L.idxA[p] : 0u;
+    }
+    int32_t X4[4], Y4[4], Z4[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = p0 + u * kBlock + tid;
+      X4[u] = Y4[u] = Z4[u] = 0;
+      if (p < n) { X4[u] = L.X[j4[u]]; Y4[u] = L.Y[j4[u]]; Z4[u] = L.Z[j4[u]]; }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = p0 + u * kBlock + tid;
+      if (p >= n) continue;
+      double x = coord(X4[u], P.scale[0], P.offset[0]), y = coord(Y4[u], P.scale[1], P.offset[1]),
+             z = coord(Z4[u], P.scale[2], P.offset[2]);
+      const uint32_t key = gkey(gcell(x, g.min[0], cs), gcell(y, g.min[1], cs), gcell(z, g.min[2], cs));
+      L.keyA[p] = key;
+      L.idxB[p] = (uint32_t)p;
+      atomicAdd(&s_h3[0][key & 127u], 1u);
+      atomicAdd(&s_h3[1][(key >> 7) & 127u], 1u);
+      atomicAdd(&s_h3[2][(key >> 14) & 127u], 1u);
+    }
   }
   for (uint32_t i = tid; i < L.H; i += kBlock) L.htab[i] = make_uint4(kNil, 0u, 0u, 0u);
   __syncthreads();
   uint32_t *kc = L.keyA, *vc = L.idxB, *ko = L.keyB, *vo = L.next;
   for (int pass = 0; pass < 3; pass++) {
-    if (!block_radix_pass(kc, vc, ko, vo, n, 7 * pass, 127u, s_rad)) continue;
+    if (!block_radix_pass(kc, vc, ko, vo, n, 7 * pass, 127u, s_rad, s_h3[pass])) continue;
     uint32_t* t = kc; kc = ko; ko = t;
     t = vc; vc = vo; vo = t;
   }
