@@ -9,6 +9,7 @@
 // the canonical order described in oracle.h.
 #include "oracle.h"
 
+#include <dlfcn.h>
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -682,6 +683,7 @@ static const int64_t kLeafCap = 10000;  // indexer.h:47
 
 struct Indexer {
   Attrs attributes;
+  std::string encoding = "DEFAULT";
   std::shared_ptr<Node> root;
   double spacing = 1.0;
   int64_t octreeDepth = 0;
@@ -1359,7 +1361,7 @@ static std::string create_metadata(const Indexer& ix, const std::string& name, i
   ss << t(2) << s("min") << ": " << toJson(ix.root->min) << ", " << "\n";
   ss << t(2) << s("max") << ": " << toJson(ix.root->max) << "\n";
   ss << t(1) << "}" << "," << "\n";
-  ss << t(1) << s("encoding") << ": " << s("DEFAULT") << "," << "\n";
+  ss << t(1) << s("encoding") << ": " << s(ix.encoding) << "," << "\n";
   ss << t(1) << s("attributes") << ": " << "[" << "\n";
   const auto& L = ix.attributes.list;
   for (size_t i = 0; i < L.size(); i++) {
@@ -1422,24 +1424,125 @@ struct ChunkIn {
 };
 
 // doIndexing (indexer.cpp:1581-1795), canonical schedule.
+// BROTLI node encoding (indexer.cpp:1110-1401): attributes as separate
+// arrays (position as two Morton codes, rgb as one), points ordered by the
+// position codes with std::sort, then brotli quality 6, default window and
+// mode.  Brotli comes from the system's libbrotlienc.so.1 (1.0.9, the version
+// vendored by the reference), bound at run time.
+typedef int (*BrotliCompressFn)(int quality, int lgwin, int mode, size_t input_size, const uint8_t* input,
+                                size_t* encoded_size, uint8_t* encoded);
+static BrotliCompressFn brotli_compress() {
+  static BrotliCompressFn fn = nullptr;
+  static bool tried = false;
+  if (!tried) {
+    tried = true;
+    void* h = dlopen("libbrotlienc.so.1", RTLD_NOW | RTLD_LOCAL);
+    if (h) fn = (BrotliCompressFn)dlsym(h, "BrotliEncoderCompress");
+  }
+  return fn;
+}
+
+static bool compress_node(const std::vector<uint8_t>& pts, int64_t n, const Attrs& attrs, std::vector<uint8_t>& out) {
+  struct MC { uint64_t lower, upper, whatever, index; };
+  const int bpp = attrs.bytes;
+  std::unordered_map<std::string, std::vector<uint8_t>> buffers;
+  std::vector<MC> mcs;
+  for (const Attr& a : attrs.list) {
+    const int aoff = attrs.offsetOf(a.name);
+    if (a.name == "rgb") {
+      std::vector<uint8_t> b(8 * n);
+      for (int64_t i = 0; i < n; i++) {
+        uint16_t c[3];
+        memcpy(c, &pts[i * bpp + aoff], 6);
+        uint64_t mc = morton(c[0], c[1], c[2]);
+        memcpy(&b[8 * i], &mc, 8);
+      }
+      buffers["rgb_morton"] = std::move(b);
+    } else if (a.name == "position") {
+      // the minimum starts at int64 max converted to int32 (-1)
+      int32_t mn[3] = {(int32_t)std::numeric_limits<int64_t>::max(), (int32_t)std::numeric_limits<int64_t>::max(),
+                       (int32_t)std::numeric_limits<int64_t>::max()};
+      std::vector<int32_t> xyz(3 * n);
+      for (int64_t i = 0; i < n; i++) {
+        memcpy(&xyz[3 * i], &pts[i * bpp + aoff], 12);
+        for (int k = 0; k < 3; k++) mn[k] = std::min(mn[k], xyz[3 * i + k]);
+      }
+      for (int64_t i = 0; i < n; i++) {
+        uint32_t m[3];
+        for (int k = 0; k < 3; k++) m[k] = (uint32_t)(xyz[3 * i + k] - mn[k]);
+        MC mc;
+        mc.lower = morton(m[0] & 0xffff, m[1] & 0xffff, m[2] & 0xffff);
+        mc.upper = morton(m[0] >> 16, m[1] >> 16, m[2] >> 16);
+        mc.whatever = morton(m[0], m[1], m[2]);
+        mc.index = (uint64_t)i;
+        mcs.push_back(mc);
+      }
+      std::vector<uint8_t> b(16 * n);
+      for (int64_t i = 0; i < n; i++) {
+        memcpy(&b[16 * i], &mcs[i].upper, 8);
+        memcpy(&b[16 * i + 8], &mcs[i].lower, 8);
+      }
+      buffers["position_morton"] = std::move(b);
+    }
+    std::vector<uint8_t> b(a.size * n);
+    for (int64_t i = 0; i < n; i++) memcpy(&b[i * a.size], &pts[i * bpp + aoff], a.size);
+    buffers[a.name] = std::move(b);
+  }
+  std::sort(mcs.begin(), mcs.end(), [](MC& x, MC& y) { return x.upper == y.upper ? x.lower < y.lower : x.upper < y.upper; });
+  std::vector<uint8_t> merged;
+  for (const Attr& a : attrs.list) {
+    const std::string name = a.name == "position" ? "position_morton" : (a.name == "rgb" ? "rgb_morton" : a.name);
+    const std::vector<uint8_t>& b = buffers[name];
+    const int64_t as = (int64_t)b.size() / n;
+    for (int64_t i = 0; i < n; i++) {
+      const int64_t src = (int64_t)mcs[i].index;
+      merged.insert(merged.end(), b.begin() + src * as, b.begin() + (src + 1) * as);
+    }
+  }
+  BrotliCompressFn fn = brotli_compress();
+  if (!fn) { g_error = "libbrotlienc.so.1 is not available"; return false; }
+  size_t cap = (size_t)(merged.size() * 1.5 + 1000);
+  for (int attempt = 0; attempt < 5; attempt++) {
+    out.assign(cap, 0);
+    size_t es = cap;
+    if (fn(6, 22, 0, merged.size(), merged.data(), &es, out.data())) {
+      out.resize(es);
+      return true;
+    }
+    cap = (size_t)((cap + 1024) * 1.5);
+  }
+  g_error = "brotli compression failed";
+  return false;
+}
+
 static int do_indexing(std::vector<ChunkIn>& chunks, Attrs attributes, V3 min, V3 max, const std::string& method,
-                       const std::string& name, int64_t pointsTotal, const std::string& outdir, double* tIndex) {
+                       const std::string& name, int64_t pointsTotal, const std::string& outdir, double* tIndex,
+                       const std::string& encoding = "DEFAULT") {
   double t0 = now_s();
   Indexer ix;
   json_roundtrip(attributes);
   ix.attributes = attributes;
+  ix.encoding = encoding;
+  if (encoding != "DEFAULT" && encoding != "BROTLI") { g_error = "unknown encoding " + encoding; return 1; }
+  bool compressFailed = false;
   ix.root = std::make_shared<Node>("r", min, max);
   ix.spacing = (max - min).x / 128.0;
   bool avg = method == "poisson_average";
   if (!avg && method != "poisson") { g_error = "unknown method " + method; return 1; }
   NodeCB onCompleted = [&](Node* n) {
-    // Writer::writeAndUnload (indexer.cpp:1453-1516), DEFAULT encoding
+    // Writer::writeAndUnload (indexer.cpp:1453-1516)
     if (n->numPoints != 0) {
-      int64_t bs = (int64_t)n->points->size();
+      std::vector<uint8_t> enc;
+      const std::vector<uint8_t>* src = n->points.get();
+      if (ix.encoding == "BROTLI") {
+        if (!compress_node(*n->points, n->numPoints, ix.attributes, enc)) compressFailed = true;
+        src = &enc;
+      }
+      int64_t bs = (int64_t)src->size();
       n->byteSize = bs;
       n->byteOffset = ix.byteOffset;
       ix.byteOffset += bs;
-      ix.octree.insert(ix.octree.end(), n->points->begin(), n->points->end());
+      ix.octree.insert(ix.octree.end(), src->begin(), src->end());
       n->points = nullptr;
     }
     ix.hrecords.push_back({n->name, n->byteOffset, n->byteSize, n->numPoints});
@@ -1473,6 +1576,7 @@ static int do_indexing(std::vector<ChunkIn>& chunks, Attrs attributes, V3 min, V
   else if (!ix.root->sampled) sample(ix.root.get());
   onCompleted(ix.root.get());
 
+  if (compressFailed) return 1;
   int64_t firstChunkSize = 0;
   auto hier = build_hierarchy_file(ix.hrecords, firstChunkSize);
   std::string meta = create_metadata(ix, name, pointsTotal, firstChunkSize);
@@ -1594,8 +1698,8 @@ static bool read_chunks(const std::string& outdir, std::vector<ChunkIn>& chunks,
 
 }  // namespace orc
 
-extern "C" int orc_convert(const char* las_path, const char* outdir, const char* method, int32_t keep_chunks,
-                           int32_t chunk_only, double* timings) {
+extern "C" int orc_convert_encoded(const char* las_path, const char* outdir, const char* method,
+                                   const char* encoding, int32_t keep_chunks, int32_t chunk_only, double* timings) {
   ChunkerOut co;
   int rc = do_chunking(las_path, co);
   if (rc) return rc;
@@ -1627,9 +1731,14 @@ extern "C" int orc_convert(const char* las_path, const char* outdir, const char*
   lasfile::read_header(las_path, h, nullptr);
   std::string name = fs::path(las_path).stem().string();
   double tIndex = 0;
-  rc = do_indexing(chunks, co.attrs, co.min, co.max, method, name, (int64_t)h.num_points, od, &tIndex);
+  rc = do_indexing(chunks, co.attrs, co.min, co.max, method, name, (int64_t)h.num_points, od, &tIndex, encoding);
   if (timings) timings[3] = tIndex;
   return rc;
+}
+
+extern "C" int orc_convert(const char* las_path, const char* outdir, const char* method, int32_t keep_chunks,
+                           int32_t chunk_only, double* timings) {
+  return orc_convert_encoded(las_path, outdir, method, "DEFAULT", keep_chunks, chunk_only, timings);
 }
 
 extern "C" int orc_index(const char* outdir, const char* method, const char* name, double* timings) {

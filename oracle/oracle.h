@@ -44,6 +44,10 @@ int orc_convert(const char* las_path, const char* outdir, const char* method, in
 // doIndexing alone on an existing outdir/chunks directory written by the
 // chunker (chunks are read in ascending ID order).
 int orc_index(const char* outdir, const char* method, const char* name, double* timings);
+// orc_convert with the node encoding of the reference's --encoding
+// ("DEFAULT" or "BROTLI")
+int orc_convert_encoded(const char* las_path, const char* outdir, const char* method, const char* encoding,
+                        int32_t keep_chunks, int32_t chunk_only, double* timings);
 
 // Poisson node sampling on one synthetic node (sampler_poisson.h:14-304 /
 // sampler_poisson_average.h:14-492) for unit tests: children point buffers

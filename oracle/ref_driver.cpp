@@ -2,7 +2,7 @@
 // SamplerPoisson / SamplerPoissonAverage) on an existing <outdir>/chunks
 // directory.  It is compiled against the reference sources in place by
 // oracle/build_ref.sh and lives only in oracle/_ref/ (never shipped).
-//   ref_indexer <outdir> <poisson|poisson_average> <name> <pointsTotal>
+//   ref_indexer <outdir> <poisson|poisson_average> <name> <pointsTotal> [DEFAULT|BROTLI]
 #include <stdlib.h>
 
 #include <iostream>
@@ -24,7 +24,7 @@ int main(int argc, char** argv) {
   Options options;
   options.name = argv[3];
   options.method = method;
-  options.encoding = "DEFAULT";
+  options.encoding = argc > 5 ? argv[5] : "DEFAULT";
   options.keepChunks = true;
   logger::addOutputFile(outdir + "/log.txt");
   if (method == "poisson") {

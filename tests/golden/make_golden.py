@@ -34,6 +34,10 @@ CASES = [
     ("c3_1m_precision_poisson", 2, 1_000_000, 3, False, "poisson"),
     ("c3_1m_precision_poisson_average", 2, 1_000_000, 3, False, "poisson_average"),
     ("c3_300k_precision_extra_poisson", 2, 300_000, 11, True, "poisson"),
+    # BROTLI node encoding (--encoding BROTLI)
+    ("c0_150k_poisson_brotli", 0, 150_000, 2, False, "poisson", "BROTLI"),
+    ("c0_400k_extra_poisson_average_brotli", 0, 400_000, 5, True, "poisson_average", "BROTLI"),
+    ("c3_300k_precision_extra_poisson_brotli", 2, 300_000, 11, True, "poisson", "BROTLI"),
 ]
 
 
@@ -41,17 +45,19 @@ def main():
     assert os.path.exists(REF_INDEXER), "build the reference indexer first: bash oracle/build_ref.sh"
     td = tempfile.mkdtemp()
     try:
-        for name, preset, n, seed, eb, method in CASES:
+        for name, preset, n, seed, eb, method, *enc in CASES:
+            encoding = enc[0] if enc else "DEFAULT"
             las = os.path.join(td, name + ".las")
             make_las(las, preset, n, seed, eb)
             work = os.path.join(td, name)
             oracle_convert(las, work, method, keep_chunks=True, chunk_only=True)
-            subprocess.run([REF_INDEXER, work, method, name, str(n)], check=True, stdout=subprocess.DEVNULL,
+            subprocess.run([REF_INDEXER, work, method, name, str(n), encoding], check=True, stdout=subprocess.DEVNULL,
                            stderr=subprocess.DEVNULL)
             with open(os.path.join(work, "metadata.json")) as f:
                 meta = f.read()
             fixture = {
-                "case": {"preset": preset, "points": n, "seed": seed, "extra_bytes": eb, "method": method},
+                "case": {"preset": preset, "points": n, "seed": seed, "extra_bytes": eb, "method": method,
+                         "encoding": encoding},
                 "producer": "reference indexer (Converter/src/indexer.cpp) on oracle-chunked input",
                 "metadata_json": meta,
                 "nodes": digest_output(work),

@@ -22,6 +22,8 @@ def oracle():
         O.orc_last_error.restype = ctypes.c_char_p
         O.orc_decode_points.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64,
                                         ctypes.c_void_p]
+        O.orc_convert_encoded.argtypes = [ctypes.c_char_p] * 4 + [ctypes.c_int32, ctypes.c_int32,
+                                                                  ctypes.POINTER(ctypes.c_double)]
         O.orc_count_cells.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32] + \
             [ctypes.POINTER(ctypes.c_double)] * 6 + [ctypes.c_int32, ctypes.c_void_p]
         O.orc_create_lut.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p,
@@ -31,10 +33,10 @@ def oracle():
     return _oracle
 
 
-def oracle_convert(las, outdir, method, keep_chunks=False, chunk_only=False):
+def oracle_convert(las, outdir, method, keep_chunks=False, chunk_only=False, encoding="DEFAULT"):
     t = (ctypes.c_double * 8)()
-    rc = oracle().orc_convert(str(las).encode(), str(outdir).encode(), method.encode(), int(keep_chunks),
-                              int(chunk_only), t)
+    rc = oracle().orc_convert_encoded(str(las).encode(), str(outdir).encode(), method.encode(), encoding.encode(),
+                                      int(keep_chunks), int(chunk_only), t)
     assert rc == 0, oracle().orc_last_error()
     return list(t)
 

@@ -27,7 +27,7 @@ def test_oracle_matches_reference_fixture(fixture, tmp_path):
     las = tmp_path / (name + ".las")
     make_las(las, case["preset"], case["points"], case["seed"], case["extra_bytes"])
     out = tmp_path / "out"
-    oracle_convert(las, out, case["method"])
+    oracle_convert(las, out, case["method"], encoding=case.get("encoding", "DEFAULT"))
     with open(out / "metadata.json") as f:
         assert f.read() == fx["metadata_json"]
     got = digest_output(str(out))
