@@ -57,6 +57,17 @@ int pcg_set_option(const char* key, int64_t value);
  * in libstdc++ order, nodes resolved sequentially. */
 int pcg_convert(const char* las_path, const char* outdir, const char* method, int32_t flags, double* timings);
 
+/* pcg_convert with the converter's --encoding option
+ * (Converter/src/main.cpp:27,60,125, handed to indexer::doIndexing through
+ * Options::encoding).  "BROTLI" writes every node as the reference's
+ * compress() does (Converter/src/indexer.cpp:1110-1401: attribute arrays,
+ * position and rgb as Morton codes, points in position-code order, brotli
+ * quality 6); any other string writes the records unchanged
+ * (indexer.cpp:1462) and is echoed in metadata.json (indexer.cpp:516).
+ * NULL means "DEFAULT", the reference's default. */
+int pcg_convert_encoded(const char* las_path, const char* outdir, const char* method, const char* encoding,
+                        int32_t flags, double* timings);
+
 /* Phase 1 only (chunker_countsort_laszip::doChunking,
  * Converter/include/chunker_countsort_laszip.h:19): writes outdir/chunks/
  * (<id>.bin + metadata.json) exactly as the reference chunker's contract. */
@@ -66,6 +77,11 @@ int pcg_do_chunking(const char* las_path, const char* outdir, double* timings);
  * existing outdir/chunks/ written by either chunker. */
 int pcg_do_indexing(const char* outdir, const char* method, const char* name, int64_t points_total,
                     double* timings);
+
+/* pcg_do_indexing with Options::encoding (as pcg_convert_encoded;
+ * Converter/src/indexer.cpp:1453-1516). */
+int pcg_do_indexing_encoded(const char* outdir, const char* method, const char* encoding, const char* name,
+                            int64_t points_total, double* timings);
 
 /* Per-cell counts of the chunker's counting pass
  * (countPointsInCells, Converter/src/chunker_countsort_laszip.cpp:131-361)
@@ -91,6 +107,12 @@ int pcg_count_cells(const void* d_las, int64_t n, int32_t bpp, const double las_
  * and metadata.json s. */
 int pcg_convert_device(const void* d_las, int64_t n, const uint8_t* las_header, const char* method,
                        const char* outdir, double* stats);
+
+/* pcg_convert_device with Options::encoding (as pcg_convert_encoded).  With
+ * "BROTLI" the octree image is assembled in host memory and stats[7] is its
+ * compressed size. */
+int pcg_convert_device_encoded(const void* d_las, int64_t n, const uint8_t* las_header, const char* method,
+                               const char* encoding, const char* outdir, double* stats);
 
 /* Device memory helpers for callers without a HIP runtime binding.
  * kind: 0 host->device, 1 device->host, 2 device->device. */

@@ -1523,7 +1523,8 @@ static int do_indexing(std::vector<ChunkIn>& chunks, Attrs attributes, V3 min, V
   json_roundtrip(attributes);
   ix.attributes = attributes;
   ix.encoding = encoding;
-  if (encoding != "DEFAULT" && encoding != "BROTLI") { g_error = "unknown encoding " + encoding; return 1; }
+  // any encoding other than BROTLI writes the records unchanged and is only
+  // echoed in metadata.json (indexer.cpp:516, 1462)
   bool compressFailed = false;
   ix.root = std::make_shared<Node>("r", min, max);
   ix.spacing = (max - min).x / 128.0;

@@ -45,7 +45,8 @@ int orc_convert(const char* las_path, const char* outdir, const char* method, in
 // chunker (chunks are read in ascending ID order).
 int orc_index(const char* outdir, const char* method, const char* name, double* timings);
 // orc_convert with the node encoding of the reference's --encoding
-// ("DEFAULT" or "BROTLI")
+// ("BROTLI" compresses the nodes; any other name writes the records
+// unchanged and is echoed in metadata.json)
 int orc_convert_encoded(const char* las_path, const char* outdir, const char* method, const char* encoding,
                         int32_t keep_chunks, int32_t chunk_only, double* timings);
 

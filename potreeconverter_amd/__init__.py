@@ -34,6 +34,12 @@ def lib():
             c.c_int32, c.c_void_p, c.c_void_p]
         L.pcg_convert_device.argtypes = [c.c_void_p, c.c_int64, c.c_void_p, c.c_char_p, c.c_char_p,
                                          c.POINTER(c.c_double)]
+        L.pcg_convert_encoded.argtypes = [c.c_char_p, c.c_char_p, c.c_char_p, c.c_char_p, c.c_int32,
+                                          c.POINTER(c.c_double)]
+        L.pcg_do_indexing_encoded.argtypes = [c.c_char_p, c.c_char_p, c.c_char_p, c.c_char_p, c.c_int64,
+                                              c.POINTER(c.c_double)]
+        L.pcg_convert_device_encoded.argtypes = [c.c_void_p, c.c_int64, c.c_void_p, c.c_char_p, c.c_char_p,
+                                                 c.c_char_p, c.POINTER(c.c_double)]
         L.pcg_device_alloc.argtypes = [c.POINTER(c.c_void_p), c.c_uint64]
         L.pcg_device_free.argtypes = [c.c_void_p]
         L.pcg_copy.argtypes = [c.c_void_p, c.c_void_p, c.c_uint64, c.c_int32]
@@ -50,6 +56,7 @@ def lib():
         L.pcg_release.argtypes = []
         L.pcg_set_option.argtypes = [c.c_char_p, c.c_int64]
         for name in ("pcg_set_option", "pcg_release", "pcg_convert", "pcg_do_chunking", "pcg_do_indexing", "pcg_count_cells", "pcg_convert_device",
+                     "pcg_convert_encoded", "pcg_do_indexing_encoded", "pcg_convert_device_encoded",
                      "pcg_device_alloc", "pcg_device_free", "pcg_copy", "pcg_synchronize", "pcg_lasgen_preset",
                      "pcg_lasgen_configure",
                      "pcg_lasgen_write", "pcg_lasgen_device", "pcg_lasgen_header", "pcg_init"):
@@ -90,9 +97,15 @@ class LasGen:
         _check(lib().pcg_lasgen_write(self.buf, str(path).encode()))
 
 
-def convert(las_path, outdir, method="poisson", keep_chunks=False):
+def convert(las_path, outdir, method="poisson", keep_chunks=False, encoding=None):
+    """PotreeConverter <las> -o <outdir> -m <method> [--keep-chunks] [--encoding <encoding>]."""
     t = (ctypes.c_double * 8)()
-    _check(lib().pcg_convert(str(las_path).encode(), str(outdir).encode(), method.encode(), 1 if keep_chunks else 0, t))
+    flags = 1 if keep_chunks else 0
+    if encoding is None:
+        _check(lib().pcg_convert(str(las_path).encode(), str(outdir).encode(), method.encode(), flags, t))
+    else:
+        _check(lib().pcg_convert_encoded(str(las_path).encode(), str(outdir).encode(), method.encode(),
+                                         encoding.encode(), flags, t))
     return list(t)
 
 
@@ -102,7 +115,11 @@ def do_chunking(las_path, outdir):
     return list(t)
 
 
-def do_indexing(outdir, method="poisson", name="", points_total=0):
+def do_indexing(outdir, method="poisson", name="", points_total=0, encoding=None):
     t = (ctypes.c_double * 8)()
-    _check(lib().pcg_do_indexing(str(outdir).encode(), method.encode(), name.encode(), points_total, t))
+    if encoding is None:
+        _check(lib().pcg_do_indexing(str(outdir).encode(), method.encode(), name.encode(), points_total, t))
+    else:
+        _check(lib().pcg_do_indexing_encoded(str(outdir).encode(), method.encode(), encoding.encode(),
+                                             name.encode(), points_total, t))
     return list(t)

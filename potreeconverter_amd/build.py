@@ -23,6 +23,7 @@ SOURCES = [
     "chunking.cpp",
     "indexing.cpp",
     "writers.cpp",
+    "encoding.cpp",
     "attributes.cpp",
     "lasgen_host.cpp",
     "capi.cpp",
@@ -59,7 +60,7 @@ def build(verbose=False):
             if m.strip():
                 print(m)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-ldl", "-pthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed\n" + r.stdout + r.stderr)

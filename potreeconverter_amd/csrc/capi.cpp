@@ -100,7 +100,14 @@ int pcg_set_option(const char* key, int64_t value) {
   return 1;
 }
 
+static std::string encoding_or_default(const char* encoding) { return encoding ? encoding : "DEFAULT"; }
+
 int pcg_convert(const char* las_path, const char* outdir, const char* method, int32_t flags, double* timings) {
+  return pcg_convert_encoded(las_path, outdir, method, nullptr, flags, timings);
+}
+
+int pcg_convert_encoded(const char* las_path, const char* outdir, const char* method, const char* encoding,
+                        int32_t flags, double* timings) {
   if (ensure_device()) return 1;
   lasfile::Header h;
   uint8_t* d_las = nullptr;
@@ -117,7 +124,7 @@ int pcg_convert(const char* las_path, const char* outdir, const char* method, in
   }
   IndexResult r;
   std::string name = fs::path(las_path).stem().string();
-  rc = gpu_indexing(dc, method, name, (int64_t)h.num_points, r);
+  rc = gpu_indexing(dc, method, name, (int64_t)h.num_points, r, encoding_or_default(encoding));
   if (!rc) rc = write_index_files(r, outdir);
   fill_timings(timings, dc, r, 0.0);
   dc.release();
@@ -127,12 +134,18 @@ int pcg_convert(const char* las_path, const char* outdir, const char* method, in
 
 int pcg_do_indexing(const char* outdir, const char* method, const char* name, int64_t points_total,
                     double* timings) {
+  return pcg_do_indexing_encoded(outdir, method, nullptr, name, points_total, timings);
+}
+
+int pcg_do_indexing_encoded(const char* outdir, const char* method, const char* encoding, const char* name,
+                            int64_t points_total, double* timings) {
   if (ensure_device()) return 1;
   DeviceChunks dc;
   int rc = read_chunk_files(outdir, dc);
   if (rc) { dc.release(); return rc; }
   IndexResult r;
-  rc = gpu_indexing(dc, method, name ? name : "", points_total > 0 ? points_total : dc.numPoints, r);
+  rc = gpu_indexing(dc, method, name ? name : "", points_total > 0 ? points_total : dc.numPoints, r,
+                    encoding_or_default(encoding));
   if (!rc) rc = write_index_files(r, outdir);
   fill_timings(timings, dc, r, 0.0);
   dc.release();
@@ -142,6 +155,11 @@ int pcg_do_indexing(const char* outdir, const char* method, const char* name, in
 
 int pcg_convert_device(const void* d_las, int64_t n, const uint8_t* las_header, const char* method,
                        const char* outdir, double* stats) {
+  return pcg_convert_device_encoded(d_las, n, las_header, method, nullptr, outdir, stats);
+}
+
+int pcg_convert_device_encoded(const void* d_las, int64_t n, const uint8_t* las_header, const char* method,
+                               const char* encoding, const char* outdir, double* stats) {
   if (ensure_device()) return 1;
   lasfile::Header h;
   h.point_format = las_header[104] & 0x3F;
@@ -156,7 +174,7 @@ int pcg_convert_device(const void* d_las, int64_t n, const uint8_t* las_header, 
   int rc = gpu_chunking((const uint8_t*)d_las, h, dc);
   if (rc) { dc.release(); return rc; }
   IndexResult r;
-  rc = gpu_indexing(dc, method, "device", n, r);
+  rc = gpu_indexing(dc, method, "device", n, r, encoding_or_default(encoding));
   if (!rc && outdir) rc = write_index_files(r, outdir);
   if (stats) {
     stats[0] = dc.t_count;

@@ -20,6 +20,7 @@
 #include <map>
 #include <unordered_map>
 
+#include "encoding.h"
 #include "indexing.h"
 #include "octree.h"
 #include "sampler.h"
@@ -118,10 +119,11 @@ int32_t g_sample_groups = 0;
 
 void IndexResult::release() {
   d_octree = nullptr;  // a workspace buffer
+  std::vector<uint8_t>().swap(h_octree);
 }
 
 int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string& name, int64_t pointsTotal,
-                 IndexResult& out) {
+                 IndexResult& out, const std::string& encoding) {
   if (ensure_device()) return 1;
   hipStream_t s = device().stream;
   const double t0 = now_s();
@@ -132,6 +134,9 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     set_error("unknown sampling method: " + method);
     return 1;
   }
+  // any other encoding name writes the records unchanged (indexer.cpp:1462)
+  const bool brotli = encoding == "BROTLI";
+  if (brotli && !brotli_available()) return 1;
   Attributes attributes = dc.attributes;
   // chunks/metadata.json round trip: non-finite min/max read back as +inf
   for (auto& a : attributes.list)
@@ -619,31 +624,58 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   order.push_back(rootId);
 
   // byte offsets (Writer::writeAndUnload, indexer.cpp:1453-1516)
-  std::vector<CopyDesc> copies;
-  uint64_t offset = 0;
   out.completed.clear();
-  for (int32_t id : order) {
-    CompletedNode cn;
-    cn.name = tree.nodes[id].name;
-    const NodeState& st = state[id];
-    cn.numPoints = st.count;
-    if (st.count != 0) {
-      uint64_t bytes = (uint64_t)st.count * bpp;
-      cn.byteOffset = (int64_t)offset;
-      cn.byteSize = (int64_t)bytes;
-      copies.push_back(CopyDesc{st.ptr, offset, bytes});
-      offset += bytes;
+  if (brotli) {
+    std::vector<EncodeInput> encIn;
+    std::vector<size_t> encNode;
+    for (int32_t id : order) {
+      CompletedNode cn;
+      cn.name = tree.nodes[id].name;
+      const NodeState& st = state[id];
+      cn.numPoints = st.count;
+      if (st.count != 0) {
+        encIn.push_back(EncodeInput{st.ptr, st.count});
+        encNode.push_back(out.completed.size());
+      }
+      out.completed.push_back(cn);
     }
-    out.completed.push_back(cn);
-  }
-  out.octreeBytes = offset;
-  out.d_octree = (uint8_t*)workspace(WS_OCTREE, offset + 64);
-  if (!out.d_octree) return 1;
-  DevMem dcopies(WS_COPYDESC);
-  if (dcopies.alloc(copies.size() * sizeof(CopyDesc) + 16)) return 1;
-  if (!copies.empty()) {
-    PCG_CHECK(hipMemcpyAsync(dcopies.p, copies.data(), copies.size() * sizeof(CopyDesc), hipMemcpyHostToDevice, s));
-    if (gather_ranges(dcopies.as<CopyDesc>(), (int64_t)copies.size(), out.d_octree, s)) return 1;
+    std::vector<int64_t> sizes;
+    if (encode_brotli(attributes, bpp, encIn, out.h_octree, sizes, s)) return 1;
+    int64_t offset = 0;
+    for (size_t i = 0; i < encNode.size(); i++) {
+      CompletedNode& cn = out.completed[encNode[i]];
+      cn.byteOffset = offset;
+      cn.byteSize = sizes[i];
+      offset += sizes[i];
+    }
+    out.octreeBytes = (uint64_t)offset;
+    out.octreeOnHost = true;
+  } else {
+    std::vector<CopyDesc> copies;
+    uint64_t offset = 0;
+    for (int32_t id : order) {
+      CompletedNode cn;
+      cn.name = tree.nodes[id].name;
+      const NodeState& st = state[id];
+      cn.numPoints = st.count;
+      if (st.count != 0) {
+        uint64_t bytes = (uint64_t)st.count * bpp;
+        cn.byteOffset = (int64_t)offset;
+        cn.byteSize = (int64_t)bytes;
+        copies.push_back(CopyDesc{st.ptr, offset, bytes});
+        offset += bytes;
+      }
+      out.completed.push_back(cn);
+    }
+    out.octreeBytes = offset;
+    out.d_octree = (uint8_t*)workspace(WS_OCTREE, offset + 64);
+    if (!out.d_octree) return 1;
+    DevMem dcopies(WS_COPYDESC);
+    if (dcopies.alloc(copies.size() * sizeof(CopyDesc) + 16)) return 1;
+    if (!copies.empty()) {
+      PCG_CHECK(hipMemcpyAsync(dcopies.p, copies.data(), copies.size() * sizeof(CopyDesc), hipMemcpyHostToDevice, s));
+      if (gather_ranges(dcopies.as<CopyDesc>(), (int64_t)copies.size(), out.d_octree, s)) return 1;
+    }
   }
   const double ta0 = now_s();
   out.t_order = ta0 - ts1;
@@ -651,7 +683,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   // octree.bin (root name is the chunk id when there is a single chunk)
   out.hierarchy = build_hierarchy_bin(out.completed, out.firstChunkSize);
   out.metadata = create_metadata_json(attributes, dc.min, dc.max, spacing, name, pointsTotal, out.firstChunkSize,
-                                      octreeDepth);
+                                      octreeDepth, encoding);
   const double ta1 = now_s();
   out.t_hierarchy = ta1 - ta0;
   PCG_CHECK(hipStreamSynchronize(s));

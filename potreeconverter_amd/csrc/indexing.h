@@ -18,7 +18,9 @@ struct CompletedNode {
 };
 
 struct IndexResult {
-  uint8_t* d_octree = nullptr;   // octree.bin image in HBM
+  uint8_t* d_octree = nullptr;   // octree.bin image in HBM (encoding DEFAULT)
+  std::vector<uint8_t> h_octree; // octree.bin image in host memory (encoding BROTLI)
+  bool octreeOnHost = false;
   uint64_t octreeBytes = 0;
   std::vector<CompletedNode> completed;
   std::vector<uint8_t> hierarchy;  // hierarchy.bin
@@ -46,8 +48,11 @@ extern int32_t g_ring_min_nodes;
 // 0: one set per hardware queue besides the library stream's).
 extern int32_t g_sample_groups;
 
+// encoding: "BROTLI" compresses every node (indexer.cpp:1110-1401); any
+// other value writes the records unchanged and is echoed in metadata.json
+// (indexer.cpp:516, 1462).
 int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string& name, int64_t pointsTotal,
-                 IndexResult& out);
+                 IndexResult& out, const std::string& encoding = "DEFAULT");
 
 int write_index_files(const IndexResult& r, const std::string& targetDir);
 
@@ -57,6 +62,6 @@ std::vector<uint8_t> build_hierarchy_bin(const std::vector<CompletedNode>& recs,
 
 std::string create_metadata_json(const Attributes& attributes, const Vec3& min, const Vec3& max, double spacing,
                                  const std::string& name, int64_t pointsTotal, int64_t firstChunkSize,
-                                 int64_t octreeDepth);
+                                 int64_t octreeDepth, const std::string& encoding = "DEFAULT");
 
 }  // namespace pcg

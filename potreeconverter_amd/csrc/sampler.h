@@ -108,4 +108,34 @@ struct CopyDesc {
 };
 int gather_ranges(const CopyDesc* d_desc, int64_t num, uint8_t* d_dst, hipStream_t s);
 
+// BROTLI node layout (indexer.cpp:1123-1354): every node's records as one
+// array per attribute (position as two 48-bit Morton codes, rgb as one),
+// points in ascending (upper, lower) position code, std::sort order for equal
+// codes.  One EncNode per node with points; `out_off` / `scr_off` are byte
+// offsets into the output and scratch buffers (scratch: enc_scratch_bytes).
+struct EncNode {
+  const uint8_t* src;
+  int64_t count;
+  uint64_t out_off;
+  uint64_t scr_off;
+};
+enum EncColumnKind : int32_t { EC_RAW = 0, EC_POSITION_MORTON = 1, EC_RGB_MORTON = 2 };
+struct EncColumn {
+  int32_t kind;
+  int32_t src;   // byte offset of the attribute in the record
+  int32_t size;  // bytes per point in the layout
+  int32_t pad;
+};
+constexpr int kMaxEncColumns = 40;
+struct EncParams {
+  int32_t bpp;
+  int32_t num_cols;
+  int32_t pos_off;
+  int32_t pad;
+  EncColumn cols[kMaxEncColumns];
+};
+uint64_t enc_scratch_bytes(int64_t count);
+int encode_layout(const EncNode* d_nodes, int64_t num_nodes, const EncParams& p, uint8_t* d_scratch, uint8_t* d_out,
+                  hipStream_t s);
+
 }  // namespace pcg

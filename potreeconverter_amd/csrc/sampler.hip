@@ -2105,6 +2105,145 @@ __global__ __launch_bounds__(kBlock) void k_copy_tiles(const NodeWork* __restric
   }
 }
 
+// ------------------------------------------------------ BROTLI layout
+// converter_utils.h:89-104: the low 21 bits of each coordinate interleaved,
+// x at bit 0 of each triple
+__device__ inline uint64_t spread21(uint32_t v) {
+  uint64_t r = 0;
+#pragma unroll
+  for (int i = 0; i < 21; i++) r |= (uint64_t)((v >> i) & 1u) << (3 * i);
+  return r;
+}
+__device__ inline uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  return spread21(x) | spread21(y) << 1 | spread21(z) << 2;
+}
+
+// (upper, lower) code order through the codes' ranks
+struct RankLess {
+  const uint32_t* rk;
+  __device__ bool operator()(uint32_t a, uint32_t b) const { return rk[a] < rk[b]; }
+};
+
+// One block per node: position codes relative to the node's minimum
+// coordinate (whose start value is int64 max converted to int32, i.e. -1:
+// indexer.cpp:1164-1167), a stable LSD radix sort of the 96-bit (upper,
+// lower) code in three 32-bit words, then libstdc++'s std::sort on one lane
+// when two points share a code (their order is the sort's), and the arrays.
+__global__ __launch_bounds__(kBlock) void k_soa_layout(const EncNode* __restrict__ nodes, EncParams E,
+                                                        uint8_t* __restrict__ scratch, uint8_t* __restrict__ out) {
+  __shared__ RadixShared s_rad;
+  __shared__ int32_t s_min[3];
+  __shared__ int s_dup;
+  __shared__ int64_t s_start[kBlock];
+  const EncNode nd = nodes[blockIdx.x];
+  const int64_t n = nd.count;
+  const int tid = threadIdx.x;
+  if (n <= 0) return;
+  const int bpp = E.bpp;
+  uint8_t* base = scratch + nd.scr_off;
+  uint64_t* up = (uint64_t*)base;
+  uint64_t* lo = (uint64_t*)(base + align16(8 * (uint64_t)n));
+  uint32_t* kA = (uint32_t*)(base + 2 * align16(8 * (uint64_t)n));
+  uint32_t* kB = (uint32_t*)((uint8_t*)kA + align16(4 * (uint64_t)n));
+  uint32_t* iA = (uint32_t*)((uint8_t*)kB + align16(4 * (uint64_t)n));
+  uint32_t* iB = (uint32_t*)((uint8_t*)iA + align16(4 * (uint64_t)n));
+  if (tid < 3) s_min[tid] = (int32_t)-1;
+  if (tid == 0) s_dup = 0;
+  __syncthreads();
+  int32_t mn[3] = {-1, -1, -1};
+  for (int64_t i = tid; i < n; i += kBlock) {
+    const uint8_t* r = nd.src + i * bpp + E.pos_off;
+    for (int a = 0; a < 3; a++) {
+      const int32_t v = ld_i32(r + 4 * a);
+      mn[a] = v < mn[a] ? v : mn[a];
+    }
+  }
+  for (int a = 0; a < 3; a++) atomicMin(&s_min[a], mn[a]);
+  __syncthreads();
+  for (int64_t i = tid; i < n; i += kBlock) {
+    const uint8_t* r = nd.src + i * bpp + E.pos_off;
+    uint32_t m[3];
+    for (int a = 0; a < 3; a++) m[a] = (uint32_t)(ld_i32(r + 4 * a) - s_min[a]);
+    lo[i] = morton3(m[0] & 0xffffu, m[1] & 0xffffu, m[2] & 0xffffu);
+    up[i] = morton3(m[0] >> 16, m[1] >> 16, m[2] >> 16);
+    kA[i] = (uint32_t)lo[i];
+    iA[i] = (uint32_t)i;
+  }
+  __syncthreads();
+  uint32_t *kc = kA, *ic = iA, *ko = kB, *io = iB;
+  for (int word = 0; word < 3; word++) {
+    if (word > 0) {
+      for (int64_t p = tid; p < n; p += kBlock) {
+        const uint32_t j = ic[p];
+        kc[p] = word == 1 ? ((uint32_t)(lo[j] >> 32) | ((uint32_t)up[j] << 16)) : (uint32_t)(up[j] >> 16);
+      }
+      __syncthreads();
+    }
+    for (int pass = 0; pass < 4; pass++) {
+      if (!block_radix_pass(kc, ic, ko, io, n, 8 * pass, 255u, s_rad)) continue;
+      uint32_t* t = kc; kc = ko; ko = t;
+      t = ic; ic = io; io = t;
+    }
+  }
+  for (int64_t p = 1 + tid; p < n; p += kBlock)
+    if (up[ic[p]] == up[ic[p - 1]] && lo[ic[p]] == lo[ic[p - 1]]) s_dup = 1;
+  __syncthreads();
+  if (s_dup) {
+    // rank of every point: the sorted position of the first point with its
+    // code (a run-start max-scan over one contiguous slice per thread)
+    uint32_t* rk = ko;
+    const int64_t per = (n + kBlock - 1) / kBlock;
+    const int64_t b = (int64_t)tid * per, e = b + per < n ? b + per : n;
+    int64_t last = -1;
+    for (int64_t p = b; p < e; p++)
+      if (p == 0 || up[ic[p]] != up[ic[p - 1]] || lo[ic[p]] != lo[ic[p - 1]]) last = p;
+    s_start[tid] = last;
+    __syncthreads();
+    if (tid == 0) {
+      int64_t run = 0;
+      for (int t = 0; t < kBlock; t++) {
+        const int64_t v = s_start[t];
+        s_start[t] = run;
+        if (v >= 0) run = v;
+      }
+    }
+    __syncthreads();
+    int64_t cur = s_start[tid];
+    for (int64_t p = b; p < e; p++) {
+      if (p == 0 || up[ic[p]] != up[ic[p - 1]] || lo[ic[p]] != lo[ic[p - 1]]) cur = p;
+      rk[ic[p]] = (uint32_t)cur;
+    }
+    for (int64_t i = tid; i < n; i += kBlock) io[i] = (uint32_t)i;
+    __syncthreads();
+    if (tid == 0) lstd::sort(io, n, RankLess{rk});
+    __syncthreads();
+    ic = io;
+  }
+  uint8_t* o = out + nd.out_off;
+  uint64_t colbase = 0;
+  for (int c = 0; c < E.num_cols; c++) {
+    const EncColumn col = E.cols[c];
+    for (int64_t p = tid; p < n; p += kBlock) {
+      const uint32_t j = ic[p];
+      uint8_t* d = o + colbase + (uint64_t)p * col.size;
+      const uint8_t* r = nd.src + (uint64_t)j * bpp;
+      if (col.kind == EC_POSITION_MORTON) {
+        const uint64_t u = up[j], l = lo[j];
+        for (int b = 0; b < 8; b++) { d[b] = (uint8_t)(u >> (8 * b)); d[8 + b] = (uint8_t)(l >> (8 * b)); }
+      } else if (col.kind == EC_RGB_MORTON) {
+        const uint32_t cr = (uint32_t)r[col.src] | ((uint32_t)r[col.src + 1] << 8);
+        const uint32_t cg = (uint32_t)r[col.src + 2] | ((uint32_t)r[col.src + 3] << 8);
+        const uint32_t cb = (uint32_t)r[col.src + 4] | ((uint32_t)r[col.src + 5] << 8);
+        const uint64_t mc = morton3(cr, cg, cb);
+        for (int b = 0; b < 8; b++) d[b] = (uint8_t)(mc >> (8 * b));
+      } else {
+        for (int b = 0; b < col.size; b++) d[b] = r[col.src + b];
+      }
+    }
+    colbase += (uint64_t)n * col.size;
+  }
+}
+
 }  // namespace
 
 }  // namespace pcg
@@ -2201,6 +2340,18 @@ __global__ __launch_bounds__(256) void k_gather_ranges(const CopyDesc* __restric
 int gather_ranges(const CopyDesc* d_desc, int64_t num, uint8_t* d_dst, hipStream_t s) {
   if (num == 0) return 0;
   hipLaunchKernelGGL(k_gather_ranges, dim3((unsigned)num), dim3(256), 0, s, d_desc, d_dst);
+  PCG_CHECK(hipGetLastError());
+  return 0;
+}
+
+uint64_t enc_scratch_bytes(int64_t count) {
+  return 2 * align16(8 * (uint64_t)count) + 4 * align16(4 * (uint64_t)count);
+}
+
+int encode_layout(const EncNode* d_nodes, int64_t num_nodes, const EncParams& p, uint8_t* d_scratch, uint8_t* d_out,
+                  hipStream_t s) {
+  if (num_nodes == 0) return 0;
+  hipLaunchKernelGGL(k_soa_layout, dim3((unsigned)num_nodes), dim3(kBlock), 0, s, d_nodes, p, d_scratch, d_out);
   PCG_CHECK(hipGetLastError());
   return 0;
 }

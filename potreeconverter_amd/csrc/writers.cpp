@@ -362,7 +362,7 @@ std::vector<uint8_t> build_hierarchy_bin(const std::vector<CompletedNode>& recs,
 
 std::string create_metadata_json(const Attributes& attributes, const Vec3& min, const Vec3& max, double spacing,
                                  const std::string& name, int64_t pointsTotal, int64_t firstChunkSize,
-                                 int64_t octreeDepth) {
+                                 int64_t octreeDepth, const std::string& encoding) {
   // indexer.cpp:354-524: tabs, key order and %.17g numbers
   auto d = [](double v) { return fmt17(v); };
   auto vec3 = [&](const Vec3& v) { return "[" + d(v.x) + ", " + d(v.y) + ", " + d(v.z) + "]"; };
@@ -387,7 +387,7 @@ std::string create_metadata_json(const Attributes& attributes, const Vec3& min, 
   ss << t(2) << s("min") << ": " << vec3(min) << ", \n";
   ss << t(2) << s("max") << ": " << vec3(max) << "\n";
   ss << t(1) << "},\n";
-  ss << t(1) << s("encoding") << ": " << s("DEFAULT") << ",\n";
+  ss << t(1) << s("encoding") << ": " << s(encoding) << ",\n";
   ss << t(1) << s("attributes") << ": [\n";
   const auto& L = attributes.list;
   for (size_t i = 0; i < L.size(); i++) {
@@ -427,10 +427,14 @@ std::string create_metadata_json(const Attributes& attributes, const Vec3& min, 
 int write_index_files(const IndexResult& r, const std::string& targetDir) {
   fs::create_directories(targetDir);
   {
-    std::vector<uint8_t> host(r.octreeBytes);
-    if (r.octreeBytes) PCG_CHECK(stream_copy(host.data(), r.d_octree, r.octreeBytes, hipMemcpyDeviceToHost));
     std::ofstream f(targetDir + "/octree.bin", std::ios::binary);
-    f.write((const char*)host.data(), (std::streamsize)host.size());
+    if (r.octreeOnHost) {
+      f.write((const char*)r.h_octree.data(), (std::streamsize)r.h_octree.size());
+    } else {
+      std::vector<uint8_t> host(r.octreeBytes);
+      if (r.octreeBytes) PCG_CHECK(stream_copy(host.data(), r.d_octree, r.octreeBytes, hipMemcpyDeviceToHost));
+      f.write((const char*)host.data(), (std::streamsize)host.size());
+    }
   }
   {
     std::ofstream f(targetDir + "/hierarchy.bin", std::ios::binary);
