@@ -43,14 +43,17 @@ int pcg_release(void);
  * on a stream of its own, before the nodes above the chunk roots (same
  * results; 1 = one level-synchronous pass over the whole tree; 0 = one set
  * per hardware queue besides the library stream's, GPU_MAX_HW_QUEUES - 1).
- * No reference counterpart. */
+ * "random_seed" (-1 or 0 .. 2^32-1, default -1): seed of every leaf shuffle
+ * of -m random (SamplerRandom, Converter/include/sampler_random.h:102-104,
+ * where it is system_clock::now()); -1 takes the clock as the reference does.
+ * The other options have no reference counterpart. */
 int pcg_set_option(const char* key, int64_t value);
 
 /* Phase 1 + phase 2 from a LAS file, the drop-in for the two calls at
  * Converter/src/main.cpp:353 (chunker_countsort_laszip::doChunking) and
  * Converter/src/main.cpp:380-390 (indexer::doIndexing with
- * SamplerPoisson / SamplerPoissonAverage).  method: "poisson" or
- * "poisson_average".  Writes outdir/{octree.bin, hierarchy.bin,
+ * SamplerPoisson / SamplerPoissonAverage / SamplerRandom).  method:
+ * "poisson", "poisson_average" or "random".  Writes outdir/{octree.bin, hierarchy.bin,
  * metadata.json}; flags bit 0 keeps outdir/chunks/ (--keep-chunks).
  * timings (nullable, 8 doubles): chunking count, chunking distribute,
  * chunking total, indexing, octree build, sampling (seconds), nodes sorted

@@ -23,6 +23,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <random>
 #include <sstream>
 #include <string>
 #include <unordered_map>
@@ -1423,6 +1424,94 @@ struct ChunkIn {
   std::shared_ptr<std::vector<uint8_t>> points;
 };
 
+// SamplerRandom::sample (sampler_random.h:14-234), seeded: the reference
+// seeds every leaf shuffle with system_clock::now() (:102-104); here the seed
+// is a parameter (orc_set_random_seed), the value the clock would return.
+static uint32_t g_random_seed = 0;
+
+static void sample_random(Node* node, Attrs& attributes, const NodeCB& onCompleted, const NodeCB& onDiscarded) {
+  post_order_unsampled<PPoint>(node, [&](Node* n) {
+    n->sampled = true;
+    const int64_t bpp = attributes.bytes;
+    const V3 scale = attributes.posScale, offset = attributes.posOffset;
+    if (n->isLeaf()) {
+      // :97-117: point i moves to slot indices[i] of a shuffled identity
+      std::vector<int> indices(n->numPoints);
+      for (int i = 0; i < n->numPoints; i++) indices[i] = i;
+      std::shuffle(indices.begin(), indices.end(), std::default_random_engine(g_random_seed));
+      auto buf = std::make_shared<std::vector<uint8_t>>(n->points->size());
+      for (int i = 0; i < n->numPoints; i++)
+        memcpy(buf->data() + (int64_t)indices[i] * bpp, n->points->data() + (int64_t)i * bpp, bpp);
+      n->points = buf;
+      return;
+    }
+    // :48-87: a 128^3 grid over the node box; a cell holds the first point
+    // taken from it in child-then-index order
+    const int64_t gridSize = 128;
+    std::unordered_map<int64_t, bool> taken;
+    const V3 mn = n->min, size = n->max - n->min;
+    static const double all = sqrt(3.0);
+    std::vector<std::vector<int8_t>> flags(8);
+    std::vector<int64_t> numRejected(8, 0);
+    int64_t numAccepted = 0;
+    for (int ci = 0; ci < 8; ci++) {
+      auto c = n->children[ci];
+      if (!c) continue;
+      flags[ci].assign(c->numPoints, 0);
+      for (int64_t i = 0; i < c->numPoints; i++) {
+        int32_t X, Y, Z;
+        const uint8_t* p = c->points->data() + i * bpp;
+        memcpy(&X, p, 4); memcpy(&Y, p + 4, 4); memcpy(&Z, p + 8, 4);
+        const double x = (X * scale.x) + offset.x, y = (Y * scale.y) + offset.y, z = (Z * scale.z) + offset.z;
+        const double nx = (x - mn.x) / size.x, ny = (y - mn.y) / size.y, nz = (z - mn.z) / size.z;
+        const double lx = 2.0 * fmod(double(gridSize) * nx, 1.0) - 1.0;
+        const double ly = 2.0 * fmod(double(gridSize) * ny, 1.0) - 1.0;
+        const double lz = 2.0 * fmod(double(gridSize) * nz, 1.0) - 1.0;
+        const double distance = sqrt(lx * lx + ly * ly + lz * lz);
+        int64_t ix = (int64_t)(double(gridSize) * nx), iy = (int64_t)(double(gridSize) * ny),
+                iz = (int64_t)(double(gridSize) * nz);
+        ix = std::max(int64_t(0), std::min(ix, gridSize - 1));
+        iy = std::max(int64_t(0), std::min(iy, gridSize - 1));
+        iz = std::max(int64_t(0), std::min(iz, gridSize - 1));
+        const int64_t index = ix + iy * gridSize + iz * gridSize * gridSize;
+        bool isAccepted;
+        if (c->numPoints < 100) isAccepted = true;
+        else isAccepted = distance < 0.7 * all && !taken.count(index);
+        if (isAccepted) { taken[index] = true; numAccepted++; } else { numRejected[ci]++; }
+        flags[ci][i] = isAccepted ? 1 : 0;
+      }
+    }
+    // :184-228, the same child hand-back as SamplerPoisson
+    auto acc = std::make_shared<std::vector<uint8_t>>();
+    acc->reserve(numAccepted * bpp);
+    for (int ci = 0; ci < 8; ci++) {
+      auto c = n->children[ci];
+      if (!c) continue;
+      auto rej = std::make_shared<std::vector<uint8_t>>();
+      rej->reserve(numRejected[ci] * bpp);
+      for (int64_t i = 0; i < c->numPoints; i++) {
+        const uint8_t* p = c->points->data() + i * bpp;
+        if (flags[ci][i]) acc->insert(acc->end(), p, p + bpp); else rej->insert(rej->end(), p, p + bpp);
+      }
+      if (numRejected[ci] == 0 && c->isLeaf()) {
+        onDiscarded(c.get());
+        n->children[ci] = nullptr;
+      }
+      if (numRejected[ci] > 0) {
+        c->points = rej;
+        c->numPoints = numRejected[ci];
+        onCompleted(c.get());
+      } else if (numRejected[ci] == 0) {
+        c->points = nullptr;
+        c->numPoints = 0;
+        onCompleted(c.get());
+      }
+    }
+    n->points = acc;
+    n->numPoints = numAccepted;
+  });
+}
+
 // doIndexing (indexer.cpp:1581-1795), canonical schedule.
 // BROTLI node encoding (indexer.cpp:1110-1401): attributes as separate
 // arrays (position as two Morton codes, rgb as one), points ordered by the
@@ -1529,7 +1618,8 @@ static int do_indexing(std::vector<ChunkIn>& chunks, Attrs attributes, V3 min, V
   ix.root = std::make_shared<Node>("r", min, max);
   ix.spacing = (max - min).x / 128.0;
   bool avg = method == "poisson_average";
-  if (!avg && method != "poisson") { g_error = "unknown method " + method; return 1; }
+  const bool rnd = method == "random";
+  if (!avg && !rnd && method != "poisson") { g_error = "unknown method " + method; return 1; }
   NodeCB onCompleted = [&](Node* n) {
     // Writer::writeAndUnload (indexer.cpp:1453-1516)
     if (n->numPoints != 0) {
@@ -1551,6 +1641,7 @@ static int do_indexing(std::vector<ChunkIn>& chunks, Attrs attributes, V3 min, V
   NodeCB onDiscarded = [](Node*) {};
   auto sample = [&](Node* n) {
     if (avg) sample_poisson_average(n, ix.attributes, ix.spacing, onCompleted, onDiscarded);
+    else if (rnd) sample_random(n, ix.attributes, onCompleted, onDiscarded);
     else sample_poisson(n, ix.attributes, ix.spacing, onCompleted, onDiscarded);
   };
   std::sort(chunks.begin(), chunks.end(), [](const ChunkIn& a, const ChunkIn& b) { return a.id < b.id; });
@@ -1847,3 +1938,8 @@ extern "C" int orc_decode_points(int32_t fmt, int32_t record_length, const uint8
 }
 
 extern "C" const char* orc_last_error(void) { return g_error.c_str(); }
+
+extern "C" int orc_set_random_seed(uint32_t seed) {
+  g_random_seed = seed;
+  return 0;
+}

@@ -33,7 +33,7 @@ int64_t orc_create_lut(const int32_t* grid, int32_t grid_size, int64_t max_point
                        int64_t* chunk_desc /* 5 per chunk */, int64_t max_chunks, int32_t* lut);
 
 // Whole conversion of one LAS file: doChunking + doIndexing with the given
-// method ("poisson" | "poisson_average"), canonical order.  Writes
+// method ("poisson" | "poisson_average" | "random"), canonical order.  Writes
 // outdir/{octree.bin, hierarchy.bin, metadata.json}; with keep_chunks also
 // outdir/chunks/{<id>.bin, metadata.json}.  chunk_only stops after the
 // chunker.  timings (may be null) receives
@@ -63,6 +63,11 @@ int orc_sample_node(const char* method, const uint8_t* records, const int64_t ch
 // laszip point fields of n records (32-byte entries, see ref_point_decode.cpp)
 int orc_decode_points(int32_t fmt, int32_t record_length, const uint8_t* records, int64_t n, uint8_t* out);
 const char* orc_last_error(void);
+
+// Seed of the method "random" (SamplerRandom, sampler_random.h): every leaf
+// shuffle uses std::default_random_engine(seed), the value the reference
+// takes from system_clock::now() (sampler_random.h:102-104).
+int orc_set_random_seed(uint32_t seed);
 
 #ifdef __cplusplus
 }

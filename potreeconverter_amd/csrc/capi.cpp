@@ -88,6 +88,14 @@ int pcg_set_option(const char* key, int64_t value) {
     g_ring_min_nodes = (int32_t)value;
     return 0;
   }
+  if (k == "random_seed") {
+    if (value < -1 || value > 0xFFFFFFFFll) {
+      set_error("random_seed must be -1 (clock) or in [0, 2^32 - 1]");
+      return 1;
+    }
+    g_random_seed = value;
+    return 0;
+  }
   if (k == "sample_groups") {
     if (value < 0 || value > kMaxSampleGroups) {
       set_error("sample_groups must be in [0, " + std::to_string(kMaxSampleGroups) + "]");

@@ -114,6 +114,7 @@ hipEvent_t group_event(int i) {
 }  // namespace
 
 bool g_plain_sampling = false;
+int64_t g_random_seed = -1;
 int32_t g_ring_min_nodes = 512;
 int32_t g_sample_groups = 0;
 
@@ -130,7 +131,8 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   out.release();
   out = IndexResult();
   const bool avg = method == "poisson_average";
-  if (!avg && method != "poisson") {
+  const bool rnd = method == "random";
+  if (!avg && !rnd && method != "poisson") {
     set_error("unknown sampling method: " + method);
     return 1;
   }
@@ -356,7 +358,41 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     for (auto& l : levels[g]) maxLevelNodes[g] = std::max<int64_t>(maxLevelNodes[g], (int64_t)l.size());
   DevMem dinfo(WS_INFO), dstate(WS_STATE), dlist(WS_LIST), arena(WS_ARENA), counters(WS_COUNTERS),
       gtop(WS_TOPS);
-  const int methodId = avg ? METHOD_POISSON_AVERAGE : METHOD_POISSON;
+  const int methodId = avg ? METHOD_POISSON_AVERAGE : (rnd ? METHOD_RANDOM : METHOD_POISSON);
+  // SamplerRandom shuffles every leaf before its parent samples it
+  // (sampler_random.h:89-121): the shuffled copies replace the leaves'
+  // records, so the state table points at them from the start
+  std::vector<ShuffleLeaf> shuffle;
+  DevMem shufDesc(WS_SHUF_DESC), shufOut(WS_SHUF_OUT), shufIdx(WS_SHUF_IDX);
+  if (rnd) {
+    uint64_t outBytes = 0, idxBytes = 0;
+    for (int32_t i = 0; i < numNodes; i++) {
+      if (tree.nodes[i].hasChildren() || state[i].count <= 0) continue;
+      const int64_t c = state[i].count;
+      shuffle.push_back(ShuffleLeaf{state[i].ptr, (uint8_t*)(uintptr_t)outBytes,
+                                    (uint32_t*)(uintptr_t)(c > kShuffleLdsPoints ? idxBytes + 1 : 0), c});
+      outBytes += ((uint64_t)c * bpp + 15) & ~15ull;
+      if (c > kShuffleLdsPoints) idxBytes += (4 * (uint64_t)c + 15) & ~15ull;
+    }
+    if (shufOut.alloc(outBytes + 64) || shufIdx.alloc(idxBytes + 64) ||
+        shufDesc.alloc(sizeof(ShuffleLeaf) * shuffle.size() + 16))
+      return 1;
+    size_t q = 0;
+    for (int32_t i = 0; i < numNodes; i++) {
+      if (tree.nodes[i].hasChildren() || state[i].count <= 0) continue;
+      ShuffleLeaf& sl = shuffle[q++];
+      sl.dst = shufOut.as<uint8_t>() + (uintptr_t)sl.dst;
+      sl.idx = sl.idx ? (uint32_t*)(shufIdx.as<uint8_t>() + ((uintptr_t)sl.idx - 1)) : nullptr;
+      state[i].ptr = sl.dst;
+    }
+    if (!shuffle.empty())
+      PCG_CHECK(hipMemcpyAsync(shufDesc.p, shuffle.data(), sizeof(ShuffleLeaf) * shuffle.size(),
+                               hipMemcpyHostToDevice, s));
+    uint32_t seed = (uint32_t)g_random_seed;
+    if (g_random_seed < 0)  // the reference's seed: the clock (sampler_random.h:102)
+      seed = (unsigned)std::chrono::system_clock::now().time_since_epoch().count();
+    if (shuffle_leaves(shufDesc.as<ShuffleLeaf>(), (int64_t)shuffle.size(), bpp, seed, s)) return 1;
+  }
   // every output range is rounded up to 16 bytes (9 ranges per node at most)
   uint64_t arenaCap = (uint64_t)N * bpp * (avg ? 4 : 3) + (1u << 20) + (uint64_t)numNodes * 9 * 16;
   // all level lists in one upload: per group, deepest level first
@@ -390,6 +426,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   sp.bpp = bpp;
   sp.rgb_offset = rgbOffset;
   sp.method = methodId;
+  sp.random_limit = 0.7 * std::sqrt(3.0);  // sampler_random.h:159-164
   sp.arena = arena.as<uint8_t>();
   sp.arena_cap = arenaCap;
   sp.arena_top = gtop.as<uint64_t>();

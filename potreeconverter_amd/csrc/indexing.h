@@ -34,7 +34,7 @@ struct IndexResult {
   void release();
 };
 
-// method: "poisson" | "poisson_average".  `name`/`pointsTotal` go to
+// method: "poisson" | "poisson_average" | "random".  `name`/`pointsTotal` go to
 // metadata.json as in createMetadata (indexer.cpp:354-524).
 // Sampling launch mode: false = every node of an octree level in one launch
 // per kernel (batched), true = one launch per node (plain).
@@ -47,6 +47,9 @@ extern int32_t g_ring_min_nodes;
 // (1 .. kMaxSampleGroups; 1: one level-synchronous pass over the whole tree;
 // 0: one set per hardware queue besides the library stream's).
 extern int32_t g_sample_groups;
+// Seed of every SamplerRandom leaf shuffle (0 .. 2^32-1), or -1 for the
+// reference's choice, the system clock at the start of indexing.
+extern int64_t g_random_seed;
 
 // encoding: "BROTLI" compresses every node (indexer.cpp:1110-1401); any
 // other value writes the records unchanged and is echoed in metadata.json

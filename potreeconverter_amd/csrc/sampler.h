@@ -1,11 +1,11 @@
-// Bottom-up Poisson / Poisson-average node sampling on gfx950 (sampler.hip).
+// Bottom-up Poisson / Poisson-average / random node sampling on gfx950 (sampler.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace pcg {
 
-enum SampleMethod : int32_t { METHOD_POISSON = 0, METHOD_POISSON_AVERAGE = 1 };
+enum SampleMethod : int32_t { METHOD_POISSON = 0, METHOD_POISSON_AVERAGE = 1, METHOD_RANDOM = 2 };
 
 // node state flags
 enum : int32_t {
@@ -88,6 +88,7 @@ struct SampleParams {
                                  // [3] exact-sort nodes, [4] sequential nodes, [5] candidates,
                                  // [6] nodes that ran out of scratch or arena,
                                  // [7] candidates with more than 16 earlier neighbours
+  double random_limit;     // SamplerRandom: 0.7 * sqrt(3), the cell-centre distance bound
 };
 
 // Samples every node of `nodes` (all of one level, children already done).
@@ -99,6 +100,19 @@ uint64_t sample_scratch_bytes(int64_t n, int method);
 // Scratch bytes that cover any level of `max_nodes` nodes holding at most
 // `candidates` candidates in total.
 uint64_t sample_scratch_bound(int64_t candidates, int64_t max_nodes, int method);
+
+// SamplerRandom's leaf shuffle (sampler_random.h:89-121): record i of a
+// leaf moves to slot perm[i] of std::shuffle over the identity with
+// std::default_random_engine(seed).  `idx` is 4 * count bytes of scratch
+// for leaves too large for the kernel's LDS table (may be null otherwise).
+struct ShuffleLeaf {
+  const uint8_t* src;
+  uint8_t* dst;
+  uint32_t* idx;
+  int64_t count;
+};
+constexpr int64_t kShuffleLdsPoints = 10240;
+int shuffle_leaves(const ShuffleLeaf* d_leaves, int64_t num, int32_t bpp, uint32_t seed, hipStream_t s);
 
 // Copies each (src, bytes) range to dst + offset (one workgroup per range).
 struct CopyDesc {

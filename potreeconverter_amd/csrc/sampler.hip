@@ -1,4 +1,4 @@
-// Poisson and Poisson-average node sampling on gfx950: all nodes of one
+// Poisson, Poisson-average and random node sampling on gfx950: all nodes of one
 // octree level per launch (a workgroup or a wavefront per node, or a
 // level-wide grid over the candidates / output bytes).
 //
@@ -24,6 +24,10 @@
 //                  selection lists and output ranges
 //   k_copy_tiles   the record bytes: 64-record tiles of the children, each
 //                  written as one accepted run and one remainder run
+// SamplerRandom (sampler_random.h:14-234) shares k_route / k_copy_tiles:
+//   k_shuffle_leaves  std::shuffle of every leaf (seeded variant)
+//   k_random_prepare  first taker of each 128^3 grid cell
+#include "lstd_shuffle.h"
 #include "lstd_sort.h"
 #include "pcg_common.h"
 #include "sampler.h"
@@ -98,6 +102,21 @@ __host__ __device__ inline uint64_t layout_carve(uint8_t* base, int64_t n, int m
     off += align16(bytes);
     return r;
   };
+  if (method == METHOD_RANDOM) {
+    // candidates stay in main order; the cell table maps a grid cell to the
+    // first candidate that may take it (hkey: cell, hhead: candidate)
+    *L = Layout();
+    L->idxA = (uint32_t*)take(4 * n);
+    L->keyB = (uint32_t*)take(4 * n);
+    L->acc = (uint8_t*)take(n);
+    L->H = hash_cap(n, method);
+    L->hkey = (uint32_t*)take(4 * (uint64_t)L->H);
+    L->hhead = (uint32_t*)take(4 * (uint64_t)L->H);
+    L->tileAcc = (uint32_t*)take(4 * (n / 64 + 16));
+    L->accMain = (uint8_t*)take(n);
+    L->sidx = L->idxA;
+    return off + 64;
+  }
   const bool avg = method == METHOD_POISSON_AVERAGE;
   L->X = (int32_t*)take(4 * n);
   L->Y = (int32_t*)take(4 * n);
@@ -2105,6 +2124,146 @@ __global__ __launch_bounds__(kBlock) void k_copy_tiles(const NodeWork* __restric
   }
 }
 
+// ------------------------------------------------------ SamplerRandom
+// Restates Converter/include/sampler_random.h:14-234 for one inner node.
+// Candidates are the children's points in child-then-index order (:133-146);
+// each falls into a cell of a 128^3 grid over the node box and is taken
+// when its child has fewer than 100 points, or when it lies within
+// 0.7 * sqrt(3) (in half-cell units) of its cell centre and the cell is
+// still free (:147-178).  The grid is filled in candidate order, so the
+// free-cell test passes exactly for the first candidate of its cell among
+// those that take a cell (all points of small children, and the points
+// near their cell centre).  That first candidate is an atomicMin per cell;
+// the hand-back to the children (:184-228) is k_route + k_copy_tiles, as
+// for SamplerPoisson.
+
+// fmod(v, 1.0) up to the sign of a zero result: v - trunc(v) is exact (the
+// fractional part of v is representable in v's precision), and the
+// reference only uses the value as 2 * f - 1
+__device__ inline double frac_toward_zero(double v) { return v - __builtin_trunc(v); }
+
+__device__ inline uint32_t random_slot(const Layout& L, uint32_t cell) {
+  uint32_t h = (cell * 2654435761u) & (L.H - 1u);
+  while (true) {
+    const uint32_t kk = L.hkey[h];
+    if (kk == cell || kk == kNil) return h;
+    h = (h + 1u) & (L.H - 1u);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_random_prepare(const int32_t* __restrict__ nodes,
+                                                            const NodeInfo* __restrict__ infos,
+                                                            const NodeState* __restrict__ states,
+                                                            NodeWork* __restrict__ work, SampleParams P) {
+  __shared__ int64_t s_cs[9];
+  __shared__ const uint8_t* s_cptr[8];
+  __shared__ uint8_t* s_scr;
+  const int k = blockIdx.x;
+  const NodeInfo info = infos[nodes[k]];
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    s_cs[0] = 0;
+    for (int c = 0; c < 8; c++) {
+      const int ch = info.children[c];
+      s_cptr[c] = ch >= 0 ? states[ch].ptr : nullptr;
+      s_cs[c + 1] = s_cs[c] + (ch >= 0 ? states[ch].count : 0);
+    }
+    const int64_t n = s_cs[8];
+    const uint64_t bytes = layout_bytes(n, P.method);
+    const uint64_t off = atomicAdd((unsigned long long*)P.scratch_top, (unsigned long long)bytes);
+    NodeWork w;
+    w.n = n;
+    for (int c = 0; c < 9; c++) w.child_start[c] = s_cs[c];
+    for (int c = 0; c < 8; c++) w.csrc[c] = s_cptr[c];
+    w.flags = 0;
+    w.pad = 0;
+    if (off + bytes > P.scratch_cap) {
+      w.flags = NW_OVERFLOW;
+      w.scratch = nullptr;
+      atomicAdd(&P.counters[6], 1ull);
+    } else {
+      w.scratch = P.scratch + off;
+    }
+    s_scr = w.scratch;
+    work[k] = w;
+  }
+  __syncthreads();
+  if (!s_scr) return;
+  const int64_t n = s_cs[8];
+  Layout L = make_layout(s_scr, n, P.method);
+  for (uint32_t h = tid; h < L.H; h += kBlock) {
+    L.hkey[h] = kNil;
+    L.hhead[h] = kNil;
+  }
+  __syncthreads();
+  // toCellIndex (:64-87) on the node box, FP64 in the reference's order
+  const double mnx = info.min[0], mny = info.min[1], mnz = info.min[2];
+  const double sx = info.max[0] - info.min[0], sy = info.max[1] - info.min[1], sz = info.max[2] - info.min[2];
+  for (int64_t j = tid; j < n; j += kBlock) {
+    int c = 0;
+    while (c < 7 && j >= s_cs[c + 1]) c++;
+    int32_t X, Y, Z;
+    ld_xyz(s_cptr[c] + (j - s_cs[c]) * P.bpp, X, Y, Z);
+    const double x = coord(X, P.scale[0], P.offset[0]), y = coord(Y, P.scale[1], P.offset[1]),
+                 z = coord(Z, P.scale[2], P.offset[2]);
+    const double nx = (x - mnx) / sx, ny = (y - mny) / sy, nz = (z - mnz) / sz;
+    const double lx = 2.0 * frac_toward_zero(128.0 * nx) - 1.0;
+    const double ly = 2.0 * frac_toward_zero(128.0 * ny) - 1.0;
+    const double lz = 2.0 * frac_toward_zero(128.0 * nz) - 1.0;
+    const double dist = __builtin_sqrt(lx * lx + ly * ly + lz * lz);
+    int64_t ix = (int64_t)(128.0 * nx), iy = (int64_t)(128.0 * ny), iz = (int64_t)(128.0 * nz);
+    ix = ix < 0 ? 0 : (ix > 127 ? 127 : ix);
+    iy = iy < 0 ? 0 : (iy > 127 ? 127 : iy);
+    iz = iz < 0 ? 0 : (iz > 127 ? 127 : iz);
+    const uint32_t cell = (uint32_t)(ix + iy * 128 + iz * 128 * 128);
+    const bool small = s_cs[c + 1] - s_cs[c] < 100;
+    const bool near = dist < P.random_limit;
+    L.idxA[j] = (uint32_t)j;
+    L.keyB[j] = cell;
+    L.acc[j] = small ? 2 : (near ? 1 : 0);
+    if (small || near) {
+      uint32_t h = (cell * 2654435761u) & (L.H - 1u);
+      while (true) {
+        const uint32_t prev = atomicCAS(&L.hkey[h], kNil, cell);
+        if (prev == kNil || prev == cell) break;
+        h = (h + 1u) & (L.H - 1u);
+      }
+      atomicMin(&L.hhead[h], (uint32_t)j);
+    }
+  }
+  __syncthreads();
+  for (int64_t j = tid; j < n; j += kBlock) {
+    const uint8_t a = L.acc[j];
+    if (a == 1) L.acc[j] = L.hhead[random_slot(L, L.keyB[j])] == (uint32_t)j ? 1 : 0;
+    else if (a == 2) L.acc[j] = 1;
+  }
+}
+
+// One wavefront per leaf: lane 0 replays std::shuffle's swaps on the slot
+// table (LDS, or global scratch for large leaves), then the wave moves each
+// record to its slot.
+__global__ __launch_bounds__(64) void k_shuffle_leaves(const ShuffleLeaf* __restrict__ leaves, int32_t bpp,
+                                                       uint32_t seed) {
+  __shared__ uint32_t s_idx[kShuffleLdsPoints];
+  const ShuffleLeaf lf = leaves[blockIdx.x];
+  const int64_t n = lf.count;
+  uint32_t* idx = n <= kShuffleLdsPoints ? s_idx : lf.idx;
+  for (int64_t i = threadIdx.x; i < n; i += 64) idx[i] = (uint32_t)i;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    lstd::shuffle_swaps(n, seed, [idx](int64_t i, int64_t j) {
+      const uint32_t t = idx[i];
+      idx[i] = idx[j];
+      idx[j] = t;
+    });
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < n; i += 64) {
+    const uint8_t* src = lf.src + i * bpp;
+    uint8_t* dst = lf.dst + (int64_t)idx[i] * bpp;
+    for (int b = 0; b < bpp; b++) dst[b] = src[b];
+  }
+}
+
 // ------------------------------------------------------ BROTLI layout
 // converter_utils.h:89-104: the low 21 bits of each coordinate interleaved,
 // x at bit 0 of each triple
@@ -2258,6 +2417,17 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
     return 1;
   }
   dim3 grid((unsigned)num_nodes);
+  if (p.method == METHOD_RANDOM) {
+    hipLaunchKernelGGL(k_random_prepare, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_state, d_work, p);
+    PCG_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_route, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_state, d_work, p);
+    PCG_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_tile_plan, dim3(1), dim3(1024), 0, s, d_work, num_nodes, p.seg_start);
+    PCG_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_copy_tiles, dim3(kCopyBlocks), dim3(kBlock), 0, s, d_work, num_nodes, p.seg_start, p);
+    PCG_CHECK(hipGetLastError());
+    return 0;
+  }
   hipLaunchKernelGGL(k_prepare, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_state, d_work, p);
   PCG_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_exact_sort, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p, p.counters);
@@ -2297,14 +2467,17 @@ uint64_t sample_scratch_bytes(int64_t n, int method) { return layout_bytes(n, me
 
 uint64_t sample_scratch_bound(int64_t candidates, int64_t max_nodes, int method) {
   // A node of n candidates takes its per-candidate arrays (A bytes per
-  // candidate, each array rounded up to 16 bytes), 24 bytes per hash slot
-  // (htab, hkey, hhead) for H(n) <= max(minimum, 4n) slots, and 64 bytes.
-  // A is measured at n = 2^20 with the hash taken out; layout_bytes(1)
-  // covers the minimum hash, the 64 bytes and 16 bytes per array, which
-  // bounds the rounding of every array.  So layout_bytes(n) <= (A + 96) n +
-  // layout_bytes(1) for every n.
+  // candidate, each array rounded up to 16 bytes), at most 24 bytes per hash
+  // slot (htab, hkey, hhead; the random method keeps hkey and hhead only)
+  // for H(n) <= max(minimum, 4n) slots, and 64 bytes.  A is measured at
+  // n = 2^20 with the hash taken out; layout_bytes(1) covers the minimum
+  // hash, the 64 bytes and 16 bytes per array, which bounds the rounding of
+  // every array.  So layout_bytes(n) <= (A + 96) n + layout_bytes(1) for
+  // every n.
   const int64_t m = int64_t(1) << 20;
-  const uint64_t lin = (layout_bytes(m, method) - 64 - 24ull * (uint64_t)hash_cap(m, method)) / (uint64_t)m + 1;
+  const uint64_t slotBytes = method == METHOD_RANDOM ? 8 : 24;
+  const uint64_t lin =
+      (layout_bytes(m, method) - 64 - slotBytes * (uint64_t)hash_cap(m, method)) / (uint64_t)m + 1;
   return (uint64_t)candidates * (lin + 96) + (uint64_t)max_nodes * (layout_bytes(1, method) + 512);
 }
 
@@ -2340,6 +2513,13 @@ __global__ __launch_bounds__(256) void k_gather_ranges(const CopyDesc* __restric
 int gather_ranges(const CopyDesc* d_desc, int64_t num, uint8_t* d_dst, hipStream_t s) {
   if (num == 0) return 0;
   hipLaunchKernelGGL(k_gather_ranges, dim3((unsigned)num), dim3(256), 0, s, d_desc, d_dst);
+  PCG_CHECK(hipGetLastError());
+  return 0;
+}
+
+int shuffle_leaves(const ShuffleLeaf* d_leaves, int64_t num, int32_t bpp, uint32_t seed, hipStream_t s) {
+  if (num == 0) return 0;
+  hipLaunchKernelGGL(k_shuffle_leaves, dim3((unsigned)num), dim3(64), 0, s, d_leaves, bpp, seed);
   PCG_CHECK(hipGetLastError());
   return 0;
 }

@@ -9,7 +9,11 @@ its metadata.json plus a per-node digest of hierarchy.bin / octree.bin
 The reference writes nodes in thread-completion order, so the digest is
 keyed by node name rather than by file offset.
 
-Run from the repository root:  python tests/golden/make_golden.py
+SamplerRandom seeds its leaf shuffles from the clock; the driver fixes the
+clock at the case's seed (REF_CLOCK_NS), the seeded variant the oracle and
+the GPU path implement.
+
+Run from the repository root:  python tests/golden/make_golden.py [case names]
 """
 import json
 import os
@@ -38,6 +42,10 @@ CASES = [
     ("c0_150k_poisson_brotli", 0, 150_000, 2, False, "poisson", "BROTLI"),
     ("c0_400k_extra_poisson_average_brotli", 0, 400_000, 5, True, "poisson_average", "BROTLI"),
     ("c3_300k_precision_extra_poisson_brotli", 2, 300_000, 11, True, "poisson", "BROTLI"),
+    # SamplerRandom with the clock fixed at the seed (ref_driver.cpp)
+    ("c0_150k_random", 0, 150_000, 2, False, "random", "DEFAULT", 7),
+    ("c3_300k_precision_extra_random", 2, 300_000, 11, True, "random", "DEFAULT", 123456789),
+    ("c1_1m_utm_random_brotli", 1, 1_000_000, 1, False, "random", "BROTLI", 4000000000),
 ]
 
 
@@ -45,19 +53,24 @@ def main():
     assert os.path.exists(REF_INDEXER), "build the reference indexer first: bash oracle/build_ref.sh"
     td = tempfile.mkdtemp()
     try:
-        for name, preset, n, seed, eb, method, *enc in CASES:
-            encoding = enc[0] if enc else "DEFAULT"
+        only = set(sys.argv[1:])
+        for name, preset, n, seed, eb, method, *rest in CASES:
+            if only and name not in only:
+                continue
+            encoding = rest[0] if rest else "DEFAULT"
+            random_seed = rest[1] if len(rest) > 1 else None
             las = os.path.join(td, name + ".las")
             make_las(las, preset, n, seed, eb)
             work = os.path.join(td, name)
             oracle_convert(las, work, method, keep_chunks=True, chunk_only=True)
+            env = dict(os.environ, REF_CLOCK_NS=str(random_seed or 0))
             subprocess.run([REF_INDEXER, work, method, name, str(n), encoding], check=True, stdout=subprocess.DEVNULL,
-                           stderr=subprocess.DEVNULL)
+                           stderr=subprocess.DEVNULL, env=env)
             with open(os.path.join(work, "metadata.json")) as f:
                 meta = f.read()
             fixture = {
                 "case": {"preset": preset, "points": n, "seed": seed, "extra_bytes": eb, "method": method,
-                         "encoding": encoding},
+                         "encoding": encoding, **({"random_seed": random_seed} if random_seed is not None else {})},
                 "producer": "reference indexer (Converter/src/indexer.cpp) on oracle-chunked input",
                 "metadata_json": meta,
                 "nodes": digest_output(work),

@@ -22,6 +22,7 @@ def oracle():
         O.orc_last_error.restype = ctypes.c_char_p
         O.orc_decode_points.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64,
                                         ctypes.c_void_p]
+        O.orc_set_random_seed.argtypes = [ctypes.c_uint32]
         O.orc_convert_encoded.argtypes = [ctypes.c_char_p] * 4 + [ctypes.c_int32, ctypes.c_int32,
                                                                   ctypes.POINTER(ctypes.c_double)]
         O.orc_count_cells.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32] + \
@@ -33,8 +34,9 @@ def oracle():
     return _oracle
 
 
-def oracle_convert(las, outdir, method, keep_chunks=False, chunk_only=False, encoding="DEFAULT"):
+def oracle_convert(las, outdir, method, keep_chunks=False, chunk_only=False, encoding="DEFAULT", random_seed=0):
     t = (ctypes.c_double * 8)()
+    oracle().orc_set_random_seed(ctypes.c_uint32(random_seed))
     rc = oracle().orc_convert_encoded(str(las).encode(), str(outdir).encode(), method.encode(), encoding.encode(),
                                       int(keep_chunks), int(chunk_only), t)
     assert rc == 0, oracle().orc_last_error()

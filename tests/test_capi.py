@@ -40,3 +40,7 @@ def test_unknown_option_is_an_error():
     assert L.pcg_set_option(b"ring_min_nodes", 0) != 0
     assert L.pcg_set_option(b"sample_groups", -1) != 0
     assert L.pcg_set_option(b"sample_groups", 9) != 0
+    assert L.pcg_set_option(b"random_seed", -2) != 0
+    assert L.pcg_set_option(b"random_seed", 1 << 32) != 0
+    assert L.pcg_set_option(b"random_seed", (1 << 32) - 1) == 0
+    assert L.pcg_set_option(b"random_seed", -1) == 0

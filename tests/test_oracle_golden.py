@@ -27,7 +27,8 @@ def test_oracle_matches_reference_fixture(fixture, tmp_path):
     las = tmp_path / (name + ".las")
     make_las(las, case["preset"], case["points"], case["seed"], case["extra_bytes"])
     out = tmp_path / "out"
-    oracle_convert(las, out, case["method"], encoding=case.get("encoding", "DEFAULT"))
+    oracle_convert(las, out, case["method"], encoding=case.get("encoding", "DEFAULT"),
+                   random_seed=case.get("random_seed", 0))
     with open(out / "metadata.json") as f:
         assert f.read() == fx["metadata_json"]
     got = digest_output(str(out))
@@ -45,15 +46,31 @@ def test_fixtures_discriminate_methods():
 
 
 @pytest.mark.skipif(not os.path.exists(REF_INDEXER), reason="reference indexer not built (oracle/build_ref.sh)")
-@pytest.mark.parametrize("method", ["poisson", "poisson_average"])
+@pytest.mark.parametrize("method", ["poisson", "poisson_average", "random"])
 def test_oracle_matches_live_reference(method, tmp_path):
     las = tmp_path / "live.las"
     make_las(las, 0, 700_000, 11)
     ref = tmp_path / "ref"
     oracle_convert(las, ref, method, keep_chunks=True, chunk_only=True)
+    # the reference's clock, which seeds SamplerRandom, fixed at the seed
+    env = dict(os.environ, REF_CLOCK_NS="2718281828")
     subprocess.run([REF_INDEXER, str(ref), method, "live", "700000"], check=True, stdout=subprocess.DEVNULL,
-                   stderr=subprocess.DEVNULL)
+                   stderr=subprocess.DEVNULL, env=env)
     orc = tmp_path / "orc"
-    oracle_convert(las, orc, method)
+    oracle_convert(las, orc, method, random_seed=2718281828)
     assert open(ref / "metadata.json").read() == open(orc / "metadata.json").read()
     assert digest_output(str(ref)) == digest_output(str(orc))
+
+
+def test_random_fixture_depends_on_the_seed(tmp_path):
+    """The random fixtures pin the seed: a neighbouring seed changes most
+    leaves (their shuffle) and so the sampled nodes above them."""
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "c0_150k_random.json")))
+    case = fx["case"]
+    las = tmp_path / "in.las"
+    make_las(las, case["preset"], case["points"], case["seed"], case["extra_bytes"])
+    out = tmp_path / "out"
+    oracle_convert(las, out, "random", random_seed=case["random_seed"] + 1)
+    got = digest_output(str(out))
+    differ = sum(got.get(k) != v for k, v in fx["nodes"].items())
+    assert differ > len(fx["nodes"]) // 2
