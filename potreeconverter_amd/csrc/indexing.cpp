@@ -38,6 +38,7 @@ double now_s() {
 
 struct TNode {
   std::string name;
+  uint64_t key = 0;      // node_key(name), 0 below depth 20
   Vec3 min, max;
   int32_t children[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
   int32_t parent = -1;
@@ -56,6 +57,7 @@ struct Tree {
   int32_t add(const std::string& name, const Vec3& mn, const Vec3& mx, int32_t parent) {
     TNode t;
     t.name = name;
+    t.key = name.size() <= 21 ? node_key(name) : 0;
     t.min = mn;
     t.max = mx;
     t.parent = parent;
@@ -548,9 +550,29 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     sampledFlag[id] = 1;
     if (tree.nodes[id].hasChildren()) complete_children(id);
   };
+  // While the GPU samples, the host also lays out hierarchy.bin for the
+  // case that every node of the tree completes (SamplerPoisson and
+  // SamplerRandom complete every child they sample; SamplerPoissonAverage
+  // discards some): the layout then depends on the tree only, and after
+  // sampling only the record order and the records' values are filled in.
+  HierarchyPlan plan;
+  bool planned = false;
+  std::vector<int32_t> rank;
+  std::vector<int64_t> recNp, recOff, recSize;
+  std::vector<CopyDesc> copies;
   auto walk_phase1 = [&]() {
     for (int32_t id : chunkRoots) sample_rec(id, true);  // ascending chunk ID
     collectPhase1 = false;
+    std::vector<uint64_t> keys(numNodes);
+    for (int32_t i = 0; i < numNodes; i++) keys[i] = tree.nodes[i].key;
+    planned = plan_hierarchy(keys, plan);
+    // the per-node arrays of the assembly, touched now rather than after
+    // sampling
+    rank.assign(numNodes, -1);
+    recNp.assign(numNodes, 0);
+    recOff.assign(numNodes, 0);
+    recSize.assign(numNodes, 0);
+    if (!brotli) copies.assign(numNodes, CopyDesc{nullptr, 0, 0});
   };
   if (G == 1) {
     if (run_levels(0, gm[0], gp[0], s)) return 1;
@@ -660,65 +682,81 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   int32_t rootId = chunkRoots.size() == 1 ? chunkRoots[0] : 0;
   order.push_back(rootId);
 
-  // byte offsets (Writer::writeAndUnload, indexer.cpp:1453-1516)
-  out.completed.clear();
+  // byte offsets (Writer::writeAndUnload, indexer.cpp:1453-1516), per node
+  for (int32_t id : order) recNp[id] = state[id].count;
   if (brotli) {
     std::vector<EncodeInput> encIn;
-    std::vector<size_t> encNode;
+    std::vector<int32_t> encId;
     for (int32_t id : order) {
-      CompletedNode cn;
-      cn.name = tree.nodes[id].name;
       const NodeState& st = state[id];
-      cn.numPoints = st.count;
       if (st.count != 0) {
         encIn.push_back(EncodeInput{st.ptr, st.count});
-        encNode.push_back(out.completed.size());
+        encId.push_back(id);
       }
-      out.completed.push_back(cn);
     }
     std::vector<int64_t> sizes;
     if (encode_brotli(attributes, bpp, encIn, out.h_octree, sizes, s)) return 1;
     int64_t offset = 0;
-    for (size_t i = 0; i < encNode.size(); i++) {
-      CompletedNode& cn = out.completed[encNode[i]];
-      cn.byteOffset = offset;
-      cn.byteSize = sizes[i];
+    for (size_t i = 0; i < encId.size(); i++) {
+      recOff[encId[i]] = offset;
+      recSize[encId[i]] = sizes[i];
       offset += sizes[i];
     }
     out.octreeBytes = (uint64_t)offset;
     out.octreeOnHost = true;
   } else {
-    std::vector<CopyDesc> copies;
+    size_t nc = 0;
     uint64_t offset = 0;
+    if (copies.size() < order.size()) copies.resize(order.size());
     for (int32_t id : order) {
-      CompletedNode cn;
-      cn.name = tree.nodes[id].name;
       const NodeState& st = state[id];
-      cn.numPoints = st.count;
       if (st.count != 0) {
-        uint64_t bytes = (uint64_t)st.count * bpp;
-        cn.byteOffset = (int64_t)offset;
-        cn.byteSize = (int64_t)bytes;
-        copies.push_back(CopyDesc{st.ptr, offset, bytes});
+        const uint64_t bytes = (uint64_t)st.count * bpp;
+        recOff[id] = (int64_t)offset;
+        recSize[id] = (int64_t)bytes;
+        copies[nc++] = CopyDesc{st.ptr, offset, bytes};
         offset += bytes;
       }
-      out.completed.push_back(cn);
     }
     out.octreeBytes = offset;
     out.d_octree = (uint8_t*)workspace(WS_OCTREE, offset + 64);
     if (!out.d_octree) return 1;
     DevMem dcopies(WS_COPYDESC);
-    if (dcopies.alloc(copies.size() * sizeof(CopyDesc) + 16)) return 1;
-    if (!copies.empty()) {
-      PCG_CHECK(hipMemcpyAsync(dcopies.p, copies.data(), copies.size() * sizeof(CopyDesc), hipMemcpyHostToDevice, s));
-      if (gather_ranges(dcopies.as<CopyDesc>(), (int64_t)copies.size(), out.d_octree, s)) return 1;
+    if (dcopies.alloc(nc * sizeof(CopyDesc) + 16)) return 1;
+    if (nc) {
+      PCG_CHECK(hipMemcpyAsync(dcopies.p, copies.data(), nc * sizeof(CopyDesc), hipMemcpyHostToDevice, s));
+      if (gather_ranges(dcopies.as<CopyDesc>(), (int64_t)nc, out.d_octree, s)) return 1;
     }
   }
   const double ta0 = now_s();
   out.t_order = ta0 - ts1;
   // hierarchy.bin and metadata.json on the host while the GPU gathers
   // octree.bin (root name is the chunk id when there is a single chunk)
-  out.hierarchy = build_hierarchy_bin(out.completed, out.firstChunkSize);
+  int32_t ranked = 0;
+  for (size_t q = 0; q < order.size(); q++)
+    if (rank[order[q]] < 0) {
+      rank[order[q]] = (int32_t)q;
+      ranked++;
+    }
+  out.completed.clear();
+  if (planned && ranked == numNodes && order.size() == (size_t)numNodes) {
+    out.hierarchy = fill_hierarchy(plan, rank.data(), recNp.data(), recOff.data(), recSize.data());
+    out.firstChunkSize = plan.firstChunkSize;
+  } else {
+    // completion records in order (some nodes were discarded, or the tree
+    // is deeper than node keys reach)
+    out.completed.reserve(order.size());
+    for (int32_t id : order) {
+      CompletedNode cn;
+      cn.key = tree.nodes[id].key;
+      if (!cn.key) cn.name = tree.nodes[id].name;
+      cn.numPoints = recNp[id];
+      cn.byteOffset = recOff[id];
+      cn.byteSize = recSize[id];
+      out.completed.push_back(cn);
+    }
+    out.hierarchy = build_hierarchy_bin(out.completed, out.firstChunkSize);
+  }
   out.metadata = create_metadata_json(attributes, dc.min, dc.max, spacing, name, pointsTotal, out.firstChunkSize,
                                       octreeDepth, encoding);
   const double ta1 = now_s();

@@ -11,6 +11,9 @@ namespace pcg {
 
 // One onNodeCompleted call (indexer.cpp:1605-1608), in completion order.
 struct CompletedNode {
+  // "r" + d child digits as the key (1 << 3d) | digits (base 8) for d <= 20;
+  // key 0: the node is named by `name` instead
+  uint64_t key = 0;
   std::string name;
   int64_t numPoints = 0;
   int64_t byteOffset = 0;
@@ -22,7 +25,7 @@ struct IndexResult {
   std::vector<uint8_t> h_octree; // octree.bin image in host memory (encoding BROTLI)
   bool octreeOnHost = false;
   uint64_t octreeBytes = 0;
-  std::vector<CompletedNode> completed;
+  std::vector<CompletedNode> completed;  // the completion records when hierarchy.bin was built from them
   std::vector<uint8_t> hierarchy;  // hierarchy.bin
   std::string metadata;            // metadata.json
   int64_t octreeDepth = 0;
@@ -62,6 +65,33 @@ int write_index_files(const IndexResult& r, const std::string& targetDir);
 // HierarchyFlusher + HierarchyBuilder over the completion records
 // (indexer.h:120-245, HierarchyBuilder.h:26-354); batches in ascending name.
 std::vector<uint8_t> build_hierarchy_bin(const std::vector<CompletedNode>& recs, int64_t& firstChunkSize);
+// The layout of hierarchy.bin depends on the set of records only, except
+// for each node's NORMAL / LEAF type (whether a child is recorded after it)
+// and the records' point counts and byte ranges.  plan_hierarchy fixes the
+// layout for the records' node keys (items 0..n-1, any order; false when a
+// key is missing or deeper than 20 levels); fill_hierarchy writes the file
+// for one record order (rank[item]) and the records' values.
+struct HierarchyPlan {
+  enum : uint8_t { S_PLAIN = 0, S_NORMAL = 1, S_LEAF = 2, S_PROXY = 3 };
+  struct Slot {
+    int32_t item;
+    uint8_t kind;
+    uint8_t mask;
+    uint64_t offset, size;  // S_PROXY
+  };
+  std::vector<Slot> slots;      // one per 22-byte record, in file order
+  std::vector<int32_t> parent;  // parent item or -1
+  int64_t firstChunkSize = 0;
+  int32_t items = 0;
+};
+bool plan_hierarchy(const std::vector<uint64_t>& keys, HierarchyPlan& plan);
+std::vector<uint8_t> fill_hierarchy(const HierarchyPlan& plan, const int32_t* rank, const int64_t* numPoints,
+                                    const int64_t* byteOffset, const int64_t* byteSize);
+// the same from node names only (the batch / chunk maps of HierarchyBuilder)
+std::vector<uint8_t> build_hierarchy_bin_maps(const std::vector<CompletedNode>& recs, int64_t& firstChunkSize);
+// node key <-> name ("r" + digits), keys for names of at most 21 characters
+uint64_t node_key(const std::string& name);
+std::string node_name(uint64_t key);
 
 std::string create_metadata_json(const Attributes& attributes, const Vec3& min, const Vec3& max, double spacing,
                                  const std::string& name, int64_t pointsTotal, int64_t firstChunkSize,

@@ -2045,6 +2045,7 @@ __device__ inline void write_run(uint8_t* __restrict__ dst, uint64_t first, uint
 // and the accepted ones to the node).  Ranks come from the accepted counts
 // k_route recorded per tile.
 constexpr int kTileStage = 64 * 130 + 32;  // record bytes of a staged tile (bpp <= 130)
+constexpr int kTileVecs = (64 * 130 + 15 + 15) / 16 / 64 + 1;  // 16-byte loads per lane for one tile
 
 __global__ __launch_bounds__(kBlock) void k_copy_tiles(const NodeWork* __restrict__ work, int32_t num_nodes,
                                                         const uint32_t* __restrict__ tile_start, SampleParams P) {
@@ -2078,45 +2079,79 @@ __global__ __launch_bounds__(kBlock) void k_copy_tiles(const NodeWork* __restric
     }
     q = __builtin_amdgcn_readfirstlane(lo);
   }
+  // values of the current (node, child), reloaded when the walk moves on;
+  // a wave's tiles are consecutive, so most tiles reuse them
+  int cq = -1;
+  uint32_t qbeg = 0, qend = 0;
+  const uint8_t* csrc = nullptr;
+  int64_t cstart = 0, ccnt = 0;
+  const uint8_t* accMain = nullptr;
+  const uint32_t* tacc = nullptr;
+  const uint16_t* rgb = nullptr;
+  uint32_t accChild = 0;
+  uint8_t* dAcc = nullptr;
+  uint8_t* dRej = nullptr;
   for (uint32_t item = i0; item < i1; item++) {
-    while (tile_start[q + 1] <= item) q++;  // tile_start[num] == total ends the walk
-    const int k = q >> 3, c = q & 7;
-    const NodeWork& w = work[k];
-    const Layout L = make_layout(w.scratch, w.n, P.method);
-    const uint32_t t = item - tile_start[q];
-    const int64_t cnt_c = w.child_start[c + 1] - w.child_start[c];
-    const int cnt = (int)((cnt_c - (int64_t)t * 64) < 64 ? (cnt_c - (int64_t)t * 64) : 64);
-    // stage the tile's records with 16-byte loads
-    const uint8_t* src = w.csrc[c] + (uint64_t)t * 64 * bpp;
+    if (cq < 0 || item >= qend) {
+      while (tile_start[q + 1] <= item) q++;  // tile_start[num] == total ends the walk
+      cq = q;
+      qbeg = tile_start[q];
+      qend = tile_start[q + 1];
+      const int k = q >> 3, c = q & 7;
+      const NodeWork& w = work[k];
+      const Layout L = make_layout(w.scratch, w.n, P.method);
+      csrc = w.csrc[c];
+      cstart = w.child_start[c];
+      ccnt = w.child_start[c + 1] - cstart;
+      accMain = L.accMain;
+      rgb = L.rgb;
+      // the node's accepted records before the child
+      int64_t tb = 0;
+      for (int x = 0; x < c; x++) tb += (w.child_start[x + 1] - w.child_start[x] + 63) / 64;
+      tacc = L.tileAcc + tb;
+      accChild = tacc[0];
+      const OutSeg* sg = P.segs + 9 * (int64_t)k;
+      dAcc = sg[0].dst;
+      dRej = sg[1 + c].dst;
+    }
+    const uint32_t t = item - qbeg;
+    const int cnt = (int)((ccnt - (int64_t)t * 64) < 64 ? (ccnt - (int64_t)t * 64) : 64);
+    // the tile's records (16-byte loads), its flags and the accepted count
+    // before it, all in flight together
+    const uint8_t* src = csrc + (uint64_t)t * 64 * bpp;
     const uint32_t b0 = (uint32_t)((uintptr_t)src & 15);
     const uint8_t* a0 = src - b0;
     const int nvec = (int)((b0 + (uint32_t)cnt * bpp + 15) >> 4);
-    for (int v = lane; v < nvec; v += 64) *(uint4*)(stage + 16 * v) = *(const uint4*)(a0 + 16 * (uint64_t)v);
+    uint4 rv[kTileVecs];
+#pragma unroll
+    for (int u = 0; u < kTileVecs; u++) {
+      const int v = lane + 64 * u;
+      if (v < nvec) rv[u] = *(const uint4*)(a0 + 16 * (uint64_t)v);
+    }
     const bool valid = lane < cnt;
-    const int64_t j = w.child_start[c] + (int64_t)t * 64 + lane;
-    const bool a = valid && L.accMain[j];
+    const int64_t j = cstart + (int64_t)t * 64 + lane;
+    const bool a = valid && accMain[j];
+    const uint32_t accBefore = tacc[t];
+#pragma unroll
+    for (int u = 0; u < kTileVecs; u++) {
+      const int v = lane + 64 * u;
+      if (v < nvec) *(uint4*)(stage + 16 * v) = rv[u];
+    }
     const uint64_t accm = __ballot(a), rejm = __ballot(valid && !a);
     if (a) accList[__popcll(accm & lt)] = (uint8_t)lane;
     if (valid && !a) rejList[__popcll(rejm & lt)] = (uint8_t)lane;
     s_j[wave][lane] = j;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    // the node's accepted records before this tile and before the child
-    int64_t tb = 0;
-    for (int x = 0; x < c; x++) tb += (w.child_start[x + 1] - w.child_start[x] + 63) / 64;
-    const uint32_t accBefore = L.tileAcc[tb + t];
-    const uint32_t accChild = L.tileAcc[tb];
-    const OutSeg* sg = P.segs + 9 * (int64_t)k;
     const uint8_t* stg = stage + b0;
-    write_run(sg[0].dst, accBefore, (uint32_t)__popcll(accm), stg, accList, bpp, rmul, avg ? L.rgb : nullptr,
-              s_j[wave], accm, P.rgb_offset);
-    uint8_t* rd = sg[1 + c].dst;
-    if (rd) {
+    write_run(dAcc, accBefore, (uint32_t)__popcll(accm), stg, accList, bpp, rmul, avg ? rgb : nullptr, s_j[wave],
+              accm, P.rgb_offset);
+    if (dRej) {
       if (!avg)
-        write_run(rd, (uint64_t)t * 64 - (accBefore - accChild), (uint32_t)__popcll(rejm), stg, rejList, bpp, rmul,
+        write_run(dRej, (uint64_t)t * 64 - (accBefore - accChild), (uint32_t)__popcll(rejm), stg, rejList, bpp, rmul,
                   nullptr, nullptr, 0ull, 0);
       else
-        write_run(rd, (uint64_t)t * 64, (uint32_t)cnt, stg, allList, bpp, rmul, L.rgb, s_j[wave], accm,
+        write_run(dRej, (uint64_t)t * 64, (uint32_t)cnt, stg, allList, bpp, rmul, rgb, s_j[wave], accm,
                   P.rgb_offset);
     }
     __builtin_amdgcn_wave_barrier();

@@ -152,7 +152,30 @@ std::vector<uint8_t> serialize_batch(HBatch& b, int64_t bytesWritten) {
 
 }  // namespace
 
-std::vector<uint8_t> build_hierarchy_bin_maps(const std::vector<CompletedNode>& recs, int64_t& firstChunkSize) {
+uint64_t node_key(const std::string& nm) {
+  uint64_t k = 1;
+  for (size_t c = 1; c < nm.size(); c++) k = (k << 3) | (uint64_t)(nm[c] - '0');
+  return k;
+}
+
+std::string node_name(uint64_t key) {
+  std::string s;
+  for (; key > 1; key >>= 3) s.push_back((char)('0' + (key & 7)));
+  s.push_back('r');
+  return std::string(s.rbegin(), s.rend());
+}
+
+std::vector<uint8_t> build_hierarchy_bin_maps(const std::vector<CompletedNode>& recsIn, int64_t& firstChunkSize) {
+  // records named by key get their names
+  std::vector<CompletedNode> named;
+  bool needNames = false;
+  for (const auto& r : recsIn) needNames |= r.name.empty();
+  if (needNames) {
+    named = recsIn;
+    for (auto& r : named)
+      if (r.name.empty()) r.name = node_name(r.key);
+  }
+  const std::vector<CompletedNode>& recs = needNames ? named : recsIn;
   std::map<std::string, std::vector<const CompletedNode*>> groups;
   for (const auto& r : recs) {
     std::string key = r.name.size() <= (size_t)kStep + 1 ? std::string("r") : r.name.substr(0, kStep + 1);
@@ -197,11 +220,6 @@ std::vector<uint8_t> build_hierarchy_bin_maps(const std::vector<CompletedNode>& 
 // Byte-identical to build_hierarchy_bin_maps; names deeper than 20 levels
 // take that path.
 namespace {
-inline uint64_t marker_key(const std::string& nm) {
-  uint64_t k = 1;
-  for (size_t c = 1; c < nm.size(); c++) k = (k << 3) | (uint64_t)(nm[c] - '0');
-  return k;
-}
 inline int key_depth(uint64_t k) { return (63 - __builtin_clzll(k)) / 3; }
 inline uint64_t key_prefix(uint64_t k, int depth) { return k >> (3 * (key_depth(k) - depth)); }
 
@@ -248,26 +266,24 @@ struct KeyIndex {
 };
 }  // namespace
 
-std::vector<uint8_t> build_hierarchy_bin(const std::vector<CompletedNode>& recs, int64_t& firstChunkSize) {
-  const int32_t n = (int32_t)recs.size();
-  for (const auto& r : recs)
-    if (r.name.size() > 21 || r.name.empty()) return build_hierarchy_bin_maps(recs, firstChunkSize);
-  std::vector<uint64_t> key(n);
+bool plan_hierarchy(const std::vector<uint64_t>& key, HierarchyPlan& plan) {
+  const int32_t n = (int32_t)key.size();
+  plan.items = n;
+  plan.slots.clear();
+  plan.parent.assign(n, -1);
+  for (int32_t i = 0; i < n; i++)
+    if (key[i] == 0 || key_depth(key[i]) > 20) return false;
   KeyIndex index((size_t)n);
-  for (int32_t i = 0; i < n; i++) {
-    key[i] = marker_key(recs[i].name);
-    index.put(key[i], i);
-  }
-  // child masks; NORMAL needs a child recorded after the node
-  std::vector<uint8_t> mask(n, 0), later(n, 0);
+  for (int32_t i = 0; i < n; i++) index.put(key[i], i);
+  // child masks (present children), parents for the NORMAL / LEAF rule
+  std::vector<uint8_t> mask(n, 0);
   for (int32_t i = 0; i < n; i++) {
     if (key[i] == 1) continue;
     const int32_t p = index.get(key[i] >> 3);
     if (p < 0) continue;
     mask[p] |= (uint8_t)(1u << (uint32_t)(key[i] & 7));
-    if (i > p) later[p] = 1;
+    plan.parent[i] = p;
   }
-  auto plain_type = [&](int32_t i) -> uint8_t { return (mask[i] && later[i]) ? T_NORMAL : T_LEAF; };
   // root batch: levels 0-4 breadth-first, one chunk "r"
   std::vector<int32_t> rootNodes;
   for (int32_t i = 0; i < n; i++)
@@ -275,26 +291,41 @@ std::vector<uint8_t> build_hierarchy_bin(const std::vector<CompletedNode>& recs,
   std::sort(rootNodes.begin(), rootNodes.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
   // sub-batches: every node of level >= 4 under its level-4 prefix, in the
   // chunk of its 4-level group (the batch root in its own chunk); chunk roots
-  // below the batch root are also in their own chunk
-  std::vector<HEntry> ent;
-  ent.reserve((size_t)n + n / 4);
+  // below the batch root are also in their own chunk.  Entries are bucketed
+  // by batch (a level-4 key: 8^4 marker keys from 8^4 on) with a counting
+  // sort, then each batch is sorted by (chunk, node).
+  constexpr uint64_t kBatch0 = 1ull << (3 * kStep), kBatches = 1ull << (3 * kStep);
+  std::vector<HEntry> tmp;
+  tmp.reserve((size_t)n + n / 4);
+  std::vector<uint32_t> bstart(kBatches + 1, 0);
   for (int32_t i = 0; i < n; i++) {
     const int d = key_depth(key[i]);
     if (d < kStep) continue;
     const uint64_t batch = key_prefix(key[i], kStep);
     const int cd = d == kStep ? kStep : kStep * ((d - 1) / kStep);
-    ent.push_back(HEntry{batch, key_prefix(key[i], cd), key[i], i});
-    if (d > kStep && d % kStep == 0) ent.push_back(HEntry{batch, key[i], key[i], i});
+    tmp.push_back(HEntry{batch, key_prefix(key[i], cd), key[i], i});
+    bstart[batch - kBatch0 + 1]++;
+    if (d > kStep && d % kStep == 0) {
+      tmp.push_back(HEntry{batch, key[i], key[i], i});
+      bstart[batch - kBatch0 + 1]++;
+    }
   }
-  std::sort(ent.begin(), ent.end(), [](const HEntry& a, const HEntry& b) {
-    if (a.batch != b.batch) return a.batch < b.batch;
-    if (a.chunk != b.chunk) return a.chunk < b.chunk;
-    return a.node < b.node;
-  });
-  std::vector<uint8_t> file(22 * rootNodes.size() + 22 * ent.size(), 0);
+  for (uint64_t b = 0; b < kBatches; b++) bstart[b + 1] += bstart[b];
+  std::vector<HEntry> ent(tmp.size());
+  {
+    std::vector<uint32_t> fill(bstart.begin(), bstart.end() - 1);
+    for (const HEntry& e : tmp) ent[fill[e.batch - kBatch0]++] = e;
+  }
+  for (uint64_t b = 0; b < kBatches; b++)
+    if (bstart[b + 1] - bstart[b] > 1)
+      std::sort(ent.begin() + bstart[b], ent.begin() + bstart[b + 1], [](const HEntry& x, const HEntry& y) {
+        if (x.chunk != y.chunk) return x.chunk < y.chunk;
+        return x.node < y.node;
+      });
+  plan.slots.resize(rootNodes.size() + ent.size());
   int64_t written = 22 * (int64_t)rootNodes.size();
   struct SubBatch { int64_t start = -1, firstChunk = 0, entries = 0; };
-  std::vector<SubBatch> sub(n);
+  std::unordered_map<int32_t, SubBatch> sub;
   std::vector<size_t> cstart;
   std::vector<int64_t> coff;
   size_t e0 = 0;
@@ -318,19 +349,18 @@ std::vector<uint8_t> build_hierarchy_bin(const std::vector<CompletedNode>& recs,
     };
     for (size_t e = e0; e < e1; e++) {
       const HEntry& x = ent[e];
-      const CompletedNode& r = recs[x.idx];
       const bool chunkRoot = key_depth(x.node) % kStep == 0;  // every chunk key of a sub-batch
       const bool isProxy = chunkRoot && x.chunk != x.node;
-      uint8_t type = plain_type(x.idx);
-      if (chunkRoot) type = isProxy ? T_PROXY : T_NORMAL;
-      uint64_t offset = (uint64_t)r.byteOffset, size = (uint64_t)(int64_t)(int32_t)r.byteSize;
+      HierarchyPlan::Slot& sl = plan.slots[rootNodes.size() + e];
+      sl.item = x.idx;
+      sl.mask = mask[x.idx];
+      sl.kind = chunkRoot ? (isProxy ? HierarchyPlan::S_PROXY : HierarchyPlan::S_NORMAL) : HierarchyPlan::S_PLAIN;
+      sl.offset = sl.size = 0;
       if (isProxy) {
         const size_t own = chunk_of(x.node);
-        offset = (uint64_t)written + (uint64_t)coff[own];
-        size = 22ull * (uint64_t)(cstart[own + 1] - cstart[own]);
+        sl.offset = (uint64_t)written + (uint64_t)coff[own];
+        sl.size = 22ull * (uint64_t)(cstart[own + 1] - cstart[own]);
       }
-      put_record(file.data() + written + 22 * (e - e0), type, mask[x.idx], (uint32_t)(int32_t)(uint32_t)r.numPoints,
-                 offset, size);
     }
     SubBatch& sb = sub[ent[cstart[0]].idx];  // the batch root leads the first chunk
     sb.start = written;
@@ -341,23 +371,73 @@ std::vector<uint8_t> build_hierarchy_bin(const std::vector<CompletedNode>& recs,
   }
   for (size_t q = 0; q < rootNodes.size(); q++) {
     const int32_t i = rootNodes[q];
-    const CompletedNode& r = recs[i];
-    uint8_t type = plain_type(i), m = mask[i];
-    uint64_t offset = (uint64_t)r.byteOffset, size = (uint64_t)(int64_t)(int32_t)r.byteSize;
-    if (key[i] == 1) type = T_NORMAL;  // the chunk root "r" in its own chunk
+    HierarchyPlan::Slot& sl = plan.slots[q];
+    sl.item = i;
+    sl.mask = mask[i];
+    sl.kind = HierarchyPlan::S_PLAIN;
+    sl.offset = sl.size = 0;
+    if (key[i] == 1) sl.kind = HierarchyPlan::S_NORMAL;  // the chunk root "r" in its own chunk
     if (key_depth(key[i]) == kStep) {
-      m = 0;  // its children are in its sub-batch
-      type = T_LEAF;
-      if (sub[i].start >= 0 && sub[i].entries > 1) {
-        type = T_PROXY;
-        offset = (uint64_t)sub[i].start;
-        size = (uint64_t)sub[i].firstChunk;
+      sl.mask = 0;  // its children are in its sub-batch
+      sl.kind = HierarchyPlan::S_LEAF;
+      auto it = sub.find(i);
+      if (it != sub.end() && it->second.start >= 0 && it->second.entries > 1) {
+        sl.kind = HierarchyPlan::S_PROXY;
+        sl.offset = (uint64_t)it->second.start;
+        sl.size = (uint64_t)it->second.firstChunk;
       }
     }
-    put_record(file.data() + 22 * q, type, m, (uint32_t)(int32_t)(uint32_t)r.numPoints, offset, size);
   }
-  firstChunkSize = 22 * (int64_t)rootNodes.size();
+  plan.firstChunkSize = 22 * (int64_t)rootNodes.size();
+  return true;
+}
+
+std::vector<uint8_t> fill_hierarchy(const HierarchyPlan& plan, const int32_t* rank, const int64_t* numPoints,
+                                    const int64_t* byteOffset, const int64_t* byteSize) {
+  const int32_t n = plan.items;
+  // NORMAL needs a child recorded after the node (HierarchyBuilder.h:148-162)
+  std::vector<uint8_t> later(n, 0);
+  for (int32_t i = 0; i < n; i++) {
+    const int32_t p = plan.parent[i];
+    if (p >= 0 && rank[i] > rank[p]) later[p] = 1;
+  }
+  std::vector<uint8_t> file(22 * plan.slots.size());
+  for (size_t q = 0; q < plan.slots.size(); q++) {
+    const HierarchyPlan::Slot& sl = plan.slots[q];
+    const int32_t i = sl.item;
+    uint8_t type;
+    uint64_t offset = (uint64_t)byteOffset[i], size = (uint64_t)(int64_t)(int32_t)byteSize[i];
+    switch (sl.kind) {
+      case HierarchyPlan::S_PLAIN: type = (sl.mask && later[i]) ? T_NORMAL : T_LEAF; break;
+      case HierarchyPlan::S_NORMAL: type = T_NORMAL; break;
+      case HierarchyPlan::S_LEAF: type = T_LEAF; break;
+      default: type = T_PROXY; offset = sl.offset; size = sl.size; break;
+    }
+    put_record(file.data() + 22 * q, type, sl.mask, (uint32_t)(int32_t)(uint32_t)numPoints[i], offset, size);
+  }
   return file;
+}
+
+std::vector<uint8_t> build_hierarchy_bin(const std::vector<CompletedNode>& recs, int64_t& firstChunkSize) {
+  const int32_t n = (int32_t)recs.size();
+  std::vector<uint64_t> key(n);
+  for (int32_t i = 0; i < n; i++) {
+    const CompletedNode& r = recs[i];
+    if (!r.key && (r.name.size() > 21 || r.name.empty())) return build_hierarchy_bin_maps(recs, firstChunkSize);
+    key[i] = r.key ? r.key : node_key(r.name);
+  }
+  HierarchyPlan plan;
+  if (!plan_hierarchy(key, plan)) return build_hierarchy_bin_maps(recs, firstChunkSize);
+  std::vector<int32_t> rank(n);
+  std::vector<int64_t> np(n), off(n), size(n);
+  for (int32_t i = 0; i < n; i++) {
+    rank[i] = i;
+    np[i] = recs[i].numPoints;
+    off[i] = recs[i].byteOffset;
+    size[i] = recs[i].byteSize;
+  }
+  firstChunkSize = plan.firstChunkSize;
+  return fill_hierarchy(plan, rank.data(), np.data(), off.data(), size.data());
 }
 
 std::string create_metadata_json(const Attributes& attributes, const Vec3& min, const Vec3& max, double spacing,
