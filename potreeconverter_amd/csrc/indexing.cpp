@@ -37,8 +37,9 @@ double now_s() {
 }
 
 struct TNode {
-  std::string name;
-  uint64_t key = 0;      // node_key(name), 0 below depth 20
+  uint64_t key = 0;      // node_key of the name for levels up to 20
+  std::string name;      // the name, kept only for levels above 20 (key 0)
+  int32_t level = 0;     // name.size() - 1
   Vec3 min, max;
   int32_t children[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
   int32_t parent = -1;
@@ -50,28 +51,57 @@ struct TNode {
     for (int c = 0; c < 8; c++) if (children[c] >= 0) return true;
     return false;
   }
+  std::string full_name() const { return key ? node_name(key) : name; }
 };
 
 struct Tree {
   std::vector<TNode> nodes;
-  int32_t add(const std::string& name, const Vec3& mn, const Vec3& mx, int32_t parent) {
-    TNode t;
-    t.name = name;
-    t.key = name.size() <= 21 ? node_key(name) : 0;
+  int32_t push(uint64_t key, std::string&& name, int32_t level, const Vec3& mn, const Vec3& mx, int32_t parent,
+               int digit) {
+    nodes.emplace_back();
+    TNode& t = nodes.back();
+    t.key = key;
+    t.name = std::move(name);
+    t.level = level;
     t.min = mn;
     t.max = mx;
     t.parent = parent;
-    nodes.push_back(t);
-    int32_t id = (int32_t)nodes.size() - 1;
-    if (parent >= 0) nodes[parent].children[name.back() - '0'] = id;
+    const int32_t id = (int32_t)nodes.size() - 1;
+    if (parent >= 0) nodes[parent].children[digit] = id;
     return id;
+  }
+  int32_t add(const std::string& name, const Vec3& mn, const Vec3& mx, int32_t parent) {
+    const bool keyed = name.size() <= 21;
+    return push(keyed ? node_key(name) : 0, keyed ? std::string() : std::string(name), (int32_t)name.size() - 1, mn,
+                mx, parent, name.back() - '0');
   }
   // child of `id` with digit c, created with childBoundingBoxOf if missing
   int32_t child(int32_t id, int c) {
     if (nodes[id].children[c] >= 0) return nodes[id].children[c];
     Vec3 a, b;
     child_box(nodes[id].min, nodes[id].max, c, a, b);
-    return add(nodes[id].name + char('0' + c), a, b, id);
+    const TNode& p = nodes[id];
+    const int32_t level = p.level + 1;
+    if (level <= 20) return push((p.key << 3) | (uint64_t)c, std::string(), level, a, b, id, c);
+    return push(0, p.full_name() + char('0' + c), level, a, b, id, c);
+  }
+};
+
+// Host arrays of a job keep their memory for the next job (the library runs
+// one job at a time): the vector takes over the storage of `keep` and hands
+// it back, emptied, when the job ends.  Tens of MB of fresh pages per job
+// would otherwise cost milliseconds of page faults on the critical path.
+template <class T>
+struct Reused {
+  std::vector<T>& keep;
+  std::vector<T> v;
+  explicit Reused(std::vector<T>& k) : keep(k) {
+    v.swap(keep);
+    v.clear();
+  }
+  ~Reused() {
+    v.clear();
+    v.swap(keep);
   }
 };
 
@@ -157,7 +187,15 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   const int64_t N = dc.numPoints;
 
   // ---------------------------------------------------------------- tree
+  static std::vector<TNode> keepNodes;
+  Reused<TNode> nodesMem(keepNodes);
   Tree tree;
+  tree.nodes.swap(nodesMem.v);
+  struct GiveBack {
+    Tree& t;
+    std::vector<TNode>& v;
+    ~GiveBack() { t.nodes.swap(v); }
+  } giveBack{tree, nodesMem.v};
   tree.add("r", dc.min, dc.max, -1);
   tree.nodes[0].upper = true;
   std::vector<int32_t> chunkRoots;
@@ -251,6 +289,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     std::vector<NodeOut> no(numOut);
     if (numOut) PCG_CHECK(stream_copy(no.data(), dnodes.p, numOut * sizeof(NodeOut), hipMemcpyDeviceToHost));
     std::vector<Seg> next;
+    tree.nodes.reserve(tree.nodes.size() + 2 * (size_t)numOut + 64);
     for (const NodeOut& o : no) {
       const Seg& sg = segs[o.seg];
       int32_t id = sg.node;
@@ -258,12 +297,12 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
       TNode& t = tree.nodes[id];
       t.off = sg.start + o.start;
       t.count = o.count;
-      octreeDepth = std::max<int64_t>(octreeDepth, (int64_t)t.name.size() - 1);
+      octreeDepth = std::max<int64_t>(octreeDepth, (int64_t)t.level);
       if ((int64_t)o.count > kLeafCap) {
         if ((int64_t)o.count == sg.count) {
           // indexer.cpp:1002-1098 (all points in one 1/32^5 cell): the
           // reference's duplicate-removal path is not implemented here
-          set_error("a node of " + std::to_string(o.count) + " points could not be split (" + t.name +
+          set_error("a node of " + std::to_string(o.count) + " points could not be split (" + t.full_name() +
                     "); the duplicate-point path of the reference is not supported");
           return 1;
         }
@@ -287,15 +326,21 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   // ------------------------------------------------------------ sampling
   const int32_t numNodes = (int32_t)tree.nodes.size();
   out.numNodes = numNodes;
-  std::vector<NodeInfo> info(numNodes);
-  std::vector<NodeState> state(numNodes);
+  static std::vector<NodeInfo> keepInfo;
+  static std::vector<NodeState> keepState;
+  Reused<NodeInfo> infoMem(keepInfo);
+  Reused<NodeState> stateMem(keepState);
+  std::vector<NodeInfo>& info = infoMem.v;
+  std::vector<NodeState>& state = stateMem.v;
+  info.resize(numNodes);
+  state.resize(numNodes);
   int maxLevel = 0;
   for (int32_t i = 0; i < numNodes; i++) {
     const TNode& t = tree.nodes[i];
     NodeInfo& f = info[i];
     for (int c = 0; c < 8; c++) f.children[c] = t.children[c];
     for (int a = 0; a < 3; a++) { f.min[a] = t.min[a]; f.max[a] = t.max[a]; }
-    f.level = (int32_t)t.name.size() - 1;
+    f.level = t.level;
     f.pad = 0;
     maxLevel = std::max(maxLevel, f.level);
     NodeState& st = state[i];
@@ -749,7 +794,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     for (int32_t id : order) {
       CompletedNode cn;
       cn.key = tree.nodes[id].key;
-      if (!cn.key) cn.name = tree.nodes[id].name;
+      if (!cn.key) cn.name = tree.nodes[id].full_name();
       cn.numPoints = recNp[id];
       cn.byteOffset = recOff[id];
       cn.byteSize = recSize[id];
