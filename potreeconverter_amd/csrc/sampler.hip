@@ -661,7 +661,7 @@ __device__ void pair_partition_loop(PairArr a, int64_t first, int64_t last, int6
 // (K+1)-th such left element or the K-th right element, whichever is first.
 // All control flow is uniform over the block.
 struct PartShared {
-  uint32_t wge[4], wle[4];
+  uint32_t wge[4][kWaves], wle[4][kWaves];  // per sub-chunk and wave
   unsigned long long total;
   long long fail;
   unsigned long long kc;
@@ -676,38 +676,62 @@ __device__ int64_t block_partition(PairArr a, uint32_t* posLe, uint32_t* posGe, 
   if (tid == 0) { sh.total = 0; sh.fail = INT64_MAX; sh.kc = 0; sh.markL = 0; sh.markR = 0; }
   __syncthreads();
   unsigned long long myLe = 0;
+#pragma unroll 4
   for (int64_t i = f + 1 + tid; i < l; i += kBlock) myLe += !(pv < a.K[i]);
   if (myLe) atomicAdd(&sh.total, myLe);
   __syncthreads();
   const int64_t leT = (int64_t)sh.total;
+  // four consecutive sub-chunks of kBlock elements per step (their loads in
+  // flight together, one barrier pair per step), each ranked after the
+  // previous ones exactly as one sub-chunk per step would be
+  constexpr int U = 4;
   int64_t cG = 0, cL = 0;
-  for (int64_t b = f + 1; b < l; b += kBlock) {
-    const int64_t i = b + tid;
-    const bool in = i < l;
-    const double v = in ? a.K[i] : pv;
-    const bool ge = in && !(v < pv), le = in && !(pv < v);
-    const uint64_t bg = __ballot(ge), bl = __ballot(le);
-    if (lane == 0) { sh.wge[wave] = (uint32_t)__popcll(bg); sh.wle[wave] = (uint32_t)__popcll(bl); }
+  for (int64_t b = f + 1; b < l; b += U * kBlock) {
+    bool ge[U], le[U];
+    uint64_t bg[U], bl[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = b + u * kBlock + tid;
+      const bool in = i < l;
+      const double v = in ? a.K[i] : pv;
+      ge[u] = in && !(v < pv);
+      le[u] = in && !(pv < v);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      bg[u] = __ballot(ge[u]);
+      bl[u] = __ballot(le[u]);
+      if (lane == 0) { sh.wge[u][wave] = (uint32_t)__popcll(bg[u]); sh.wle[u][wave] = (uint32_t)__popcll(bl[u]); }
+    }
     __syncthreads();
-    int64_t preG = cG, preL = cL, totG = 0, totL = 0;
-    for (int w = 0; w < 4; w++) {
-      if (w < wave) { preG += sh.wge[w]; preL += sh.wle[w]; }
-      totG += sh.wge[w];
-      totL += sh.wle[w];
-    }
-    if (le) posLe[f + preL + __popcll(bl & lt)] = (uint32_t)(i - f);
-    if (ge) {
-      int64_t k = preG + __popcll(bg & lt) + 1;
-      int64_t leIncl = preL + __popcll(bl & (lt | me));
-      if ((leT - leIncl) >= k) {
-        posGe[f + k - 1] = (uint32_t)(i - f);
-        atomicAdd(&sh.kc, 1ull);
-      } else {
-        atomicMin(&sh.fail, (long long)i);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = b + u * kBlock + tid;
+      int64_t preG = cG, preL = cL, totG = 0, totL = 0;
+      for (int w = 0; w < kWaves; w++) {
+        if (w < wave) { preG += sh.wge[u][w]; preL += sh.wle[u][w]; }
+        totG += sh.wge[u][w];
+        totL += sh.wle[u][w];
       }
+      if (le[u]) posLe[f + preL + __popcll(bl[u] & lt)] = (uint32_t)(i - f);
+      bool ok = false, bad = false;
+      if (ge[u]) {
+        const int64_t k = preG + __popcll(bg[u] & lt) + 1;
+        const int64_t leIncl = preL + __popcll(bl[u] & (lt | me));
+        ok = (leT - leIncl) >= k;
+        bad = !ok;
+        if (ok) posGe[f + k - 1] = (uint32_t)(i - f);
+      }
+      // one LDS atomic per wave: the pairs it formed, and its first element
+      // without a partner (elements rise with the lane)
+      const uint64_t okm = __ballot(ok), badm = __ballot(bad);
+      if (lane == 0) {
+        if (okm) atomicAdd(&sh.kc, (unsigned long long)__popcll(okm));
+        if (badm) atomicMin(&sh.fail, (long long)(b + u * kBlock + 64 * wave + __builtin_ctzll(badm)));
+      }
+      cG += totG;
+      cL += totL;
     }
-    cG += totG;
-    cL += totL;
     __syncthreads();
   }
   const int64_t kc = (int64_t)sh.kc;
