@@ -764,6 +764,8 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
   __shared__ int32_t s_df[kDeferred], s_dl[kDeferred], s_dd[kDeferred];
   __shared__ int s_nc, s_nn, s_nd_count, s_bad, s_any;
   __shared__ PartShared sh;
+  __shared__ double s_segK[kWaves][kSmall];
+  __shared__ uint32_t s_segH[kWaves][kSmall];
   const int k = blockIdx.x;
   NodeWork w = work[k];
   if (!(w.flags & NW_EXACT_SORT) || (w.flags & NW_OVERFLOW)) return;
@@ -906,8 +908,33 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
     }
     __syncthreads();
   } else {
-    // small segments: the rest of the partition loop, one lane each
-    for (int d = tid; d < s_nd_count; d += kBlock) pair_partition_loop(a, s_df[d], s_dl[d], s_dd[d]);
+    // small segments: the rest of the partition loop, one wave each with the
+    // segment staged in LDS and replayed by one lane (the replay is a chain
+    // of dependent element accesses: LDS latency instead of memory latency);
+    // segments longer than the stage (depth-limit ones) on one lane in place
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int d = wave; d < s_nd_count; d += kWaves) {
+      const int64_t f = s_df[d], l = s_dl[d];
+      const int len = (int)(l - f);
+      if (len > kSmall) {
+        if (lane == 0) pair_partition_loop(a, f, l, s_dd[d]);
+        continue;
+      }
+      for (int i = lane; i < len; i += 64) {
+        s_segK[wave][i] = a.K[f + i];
+        s_segH[wave][i] = a.H[f + i];
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      if (lane == 0) pair_partition_loop(PairArr{s_segK[wave], s_segH[wave]}, 0, len, s_dd[d]);
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      for (int i = lane; i < len; i += 64) {
+        a.K[f + i] = s_segK[wave][i];
+        a.H[f + i] = s_segH[wave][i];
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
     __syncthreads();
   }
   // final positions after the partition phase, then order each run of equal
