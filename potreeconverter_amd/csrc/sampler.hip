@@ -797,12 +797,10 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
         const double ls = lim * lim;
         ml = ls < ml ? ls : ml;
       }
-      int64_t lo = 0, h = hi;  // lower bound of ml in sadd[0, hi)
-      while (lo < h) {
-        const int64_t mid = lo + (h - lo) / 2;
-        if (L.sadd[mid] < ml) lo = mid + 1; else h = mid;
-      }
-      if (hi - 1 - lo > 10000) s_cap = 1;
+      // more than 10 000 in [ml, add(hi - 1)): the first sadd >= ml lies
+      // before hi - 10 001, i.e. sadd[hi - 10 002] >= ml (sadd ascending)
+      const int64_t m = hi - 10001;
+      if (m >= 1 && !(L.sadd[m - 1] < ml)) s_cap = 1;
     }
   }
   __syncthreads();
@@ -1265,27 +1263,6 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(const int32_t* __restrict_
   flush_filter(stats, P);
 }
 
-// first i in [lo, hi) with v <= a[i] (a ascending), hi if none; one
-// probe per lane and step, so about log64 of the range dependent loads
-__device__ inline int64_t wave_lower_bound(const double* a, int64_t lo, int64_t hi, double v) {
-  const int lane = threadIdx.x & 63;
-  while (hi - lo > 64) {
-    const int64_t step = (hi - lo + 63) / 64;
-    const int64_t x = lo + lane * step;
-    const bool ge = x >= hi || !(a[x] < v);
-    const uint64_t m = __ballot(ge);
-    if (m & 1ull) return lo;
-    const int c = m ? __builtin_ctzll(m) : 64;
-    const int64_t nhi = lo + c * step;
-    lo = lo + (int64_t)(c - 1) * step + 1;
-    hi = nhi < hi ? nhi : hi;
-  }
-  const int64_t x = lo + lane;
-  const bool ge = x >= hi || !(a[x] < v);
-  const uint64_t m = __ballot(ge);
-  return m ? lo + __builtin_ctzll(m) : hi;
-}
-
 // Greedy acceptance in sorted order, one wavefront per node, 64 candidates
 // per window: a candidate is rejected by an accepted neighbour of an earlier
 // window; inside the window the dependencies are resolved by ballots.  The
@@ -1433,9 +1410,11 @@ __device__ void accept_node(const int32_t* __restrict__ nodes, const NodeInfo* _
         double o = __shfl_xor(ml, d);
         ml = o < ml ? o : ml;
       }
+      // the first sadd >= ml lies more than 10 000 positions before hi - 1
+      // exactly when sadd[hi - 10 002] >= ml (sadd ascending): one load
       const int64_t hi = (w0 + 64 < n ? w0 + 64 : n);
-      const int64_t lo = wave_lower_bound(L.sadd, 0, hi, ml);
-      if (hi - 1 - lo > 10000 && lane == 0) atomicOr(&work[k].flags, (uint32_t)NW_SEQ_ACCEPT);
+      const int64_t m = hi - 10001;
+      if (m >= 1 && lane == 0 && !(L.sadd[m - 1] < ml)) atomicOr(&work[k].flags, (uint32_t)NW_SEQ_ACCEPT);
     }
   }
   flush_filter(F, P);
