@@ -1319,16 +1319,20 @@ __device__ void accept_node(const int32_t* __restrict__ nodes, const NodeInfo* _
         limSq = lim * lim;
       }
       if (cnt != kNbrOverflow && cnt != kNbrPooled) {
+        // every entry tested (no early exit), so that the accepted-bit loads
+        // are issued together; a rejected candidate's in-window mask is not
+        // used
+        uint32_t hit = 0;
 #pragma unroll
         for (int i = 0; i < kMaxNbr; i++) {
-          if (i >= cnt || rej) continue;
           const uint32_t q = row[i];
-          if ((int64_t)q >= w0) {
-            in_mask |= 1ull << (q - w0);
-          } else if ((bits[q >> 5] >> (q & 31)) & 1u) {
-            rej = true;
-          }
+          const bool use = i < cnt;
+          if (use && (int64_t)q >= w0) in_mask |= 1ull << (q - w0);
+          const bool earlier = use && (int64_t)q < w0;
+          const uint32_t word = earlier ? bits[q >> 5] : 0u;
+          hit |= (word >> (q & 31)) & 1u;
         }
+        rej = hit != 0;
       } else if (cnt == kNbrPooled) {
         // a long list: four entries in flight per step
         const uint32_t* lst = P.nbr_pool + ((uint64_t)row[0] | ((uint64_t)row[1] << 32));
