@@ -1521,16 +1521,20 @@ __global__ __launch_bounds__(64 * kRingWaves) void k_ring(const int32_t* __restr
     X = L.X[j]; Y = L.Y[j]; Z = L.Z[j];
     a = L.sadd[lane];
   }
+  // the next window's candidates are loaded while this one is decided; the
+  // window after it has its main indices loaded already, so no load waits
+  // on another load of the same window
+  uint32_t jn = lane + 64 < n ? L.idxA[lane + 64] : 0u;
   for (int64_t w0 = 0; w0 < n; w0 += 64) {
     const int64_t p = w0 + lane;
     const bool valid = p < n;
     int32_t nX = 0, nY = 0, nZ = 0;
     double na = 0.0;
     if (p + 64 < n) {
-      const uint32_t j = L.idxA[p + 64];
-      nX = L.X[j]; nY = L.Y[j]; nZ = L.Z[j];
+      nX = L.X[jn]; nY = L.Y[jn]; nZ = L.Z[jn];
       na = L.sadd[p + 64];
     }
+    jn = p + 128 < n ? L.idxA[p + 128] : 0u;
     const double cd = __builtin_sqrt(a);
     const double lim = cd - g.spacing;
     const double limSq = lim * lim;
