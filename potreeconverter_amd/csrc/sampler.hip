@@ -1103,17 +1103,42 @@ __global__ __launch_bounds__(kBlock) void k_cells(const int32_t* __restrict__ no
     uint32_t* t = kc; kc = ko; ko = t;
     t = vc; vc = vo; vo = t;
   }
-  for (int64_t e = tid; e < n; e += kBlock) {
-    const uint32_t p = vc[e];
-    const uint32_t j = L.idxA[p];
-    L.ent[e] = make_int4(L.X[j], L.Y[j], L.Z[j], (int32_t)p);
-    if (avg) L.eC16[e] = L.cell16[j];
-    else L.eAdd[e] = L.sadd[p];
-    const uint32_t key = kc[e];
-    if (e == 0 || kc[e - 1] != key) {
-      uint32_t slot = ghash(key, L.H);
-      while (atomicCAS(&L.htab[slot].x, kNil, key) != kNil) slot = (slot + 1) & (L.H - 1);
-      L.htab[slot].y = (uint32_t)e;
+  // cell-ordered entries, four per thread with each level of the index
+  // chain (sorted position -> main index -> coordinates) loaded together
+  for (int64_t e0 = tid; e0 < n; e0 += 4 * kBlock) {
+    uint32_t p4[4], k4[4], kp4[4], j4[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t e = e0 + u * kBlock;
+      p4[u] = e < n ? vc[e] : 0u;
+      k4[u] = e < n ? kc[e] : 0u;
+      kp4[u] = (e > 0 && e < n) ? kc[e - 1] : ~0u;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) j4[u] = e0 + u * kBlock < n ? L.idxA[p4[u]] : 0u;
+    int4 v4[4];
+    double a4[4];
+    uint32_t c4[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (e0 + u * kBlock >= n) continue;
+      v4[u] = make_int4(L.X[j4[u]], L.Y[j4[u]], L.Z[j4[u]], (int32_t)p4[u]);
+      if (avg) c4[u] = L.cell16[j4[u]];
+      else a4[u] = L.sadd[p4[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t e = e0 + u * kBlock;
+      if (e >= n) continue;
+      L.ent[e] = v4[u];
+      if (avg) L.eC16[e] = c4[u];
+      else L.eAdd[e] = a4[u];
+      const uint32_t key = k4[u];
+      if (e == 0 || kp4[u] != key) {
+        uint32_t slot = ghash(key, L.H);
+        while (atomicCAS(&L.htab[slot].x, kNil, key) != kNil) slot = (slot + 1) & (L.H - 1);
+        L.htab[slot].y = (uint32_t)e;
+      }
     }
   }
   __syncthreads();
@@ -1824,7 +1849,7 @@ __global__ __launch_bounds__(kBlock) void k_route(const int32_t* __restrict__ no
   __shared__ int64_t s_acc[8];
   __shared__ uint8_t* s_out[9];
   __shared__ int s_fail;
-  __shared__ uint32_t s_wsum[kWaves];
+  __shared__ uint32_t s_wsum[4][kWaves];
   const int k = blockIdx.x;
   const int node = nodes[k];
   const NodeWork& w = work[k];
@@ -1897,42 +1922,59 @@ __global__ __launch_bounds__(kBlock) void k_route(const int32_t* __restrict__ no
     }
   }
   __syncthreads();
-  int64_t run = 0;  // accepted before this tile
-  for (int64_t b0 = 0; b0 < n; b0 += kBlock) {
-    const int64_t j = b0 + tid;
-    const bool valid = j < n;
-    const uint32_t q = valid ? inv[j] : 0;
-    const bool a = valid && L.acc[q];
-    const uint64_t bal = __ballot(a);
-    const uint32_t before_in_wave = (uint32_t)__popcll(bal & (lane == 0 ? 0ull : (~0ull >> (64 - lane))));
-    if (lane == 0) s_wsum[wave] = (uint32_t)__popcll(bal);
-    __syncthreads();
-    uint32_t woff = 0, tot = 0;
-    for (int v = 0; v < kWaves; v++) { if (v < wave) woff += s_wsum[v]; tot += s_wsum[v]; }
-    if (valid) {
-      int c = 0;
+  int64_t run = 0;  // accepted before this step
+  // four consecutive sub-chunks of kBlock candidates per step: their flag
+  // loads in flight together, one barrier pair per step
+  for (int64_t b0 = 0; b0 < n; b0 += 4 * kBlock) {
+    uint32_t q4[4];
+    bool a4[4];
 #pragma unroll
-      for (int x = 1; x < 8; x++) c += j >= w.child_start[x] ? 1 : 0;
-      const int64_t local = j - w.child_start[c];
-      const int64_t apos = run + woff + before_in_wave;
-      if ((local & 63) == 0) L.tileAcc[tile_base[c] + (local >> 6)] = (uint32_t)apos;
-      L.accMain[j] = a ? 1 : 0;
-      if (a) {
-        if (avg) {
-          long long rs[3], ws;
-          if (seq_avg) {
-            for (int ch = 0; ch < 3; ch++) rs[ch] = L.sums[8 * q + ch];
-            ws = L.sums[8 * q + 3];
-          } else {
-            for (int ch = 0; ch < 3; ch++)
-              rs[ch] = 2 * ((long long)L.rgb[3 * j + ch] + L.sums[8 * q + ch]) + L.sums[8 * q + 4 + ch];
-            ws = 2 * (1 + L.sums[8 * q + 3]) + L.sums[8 * q + 7];
+    for (int u = 0; u < 4; u++) {
+      const int64_t j = b0 + u * kBlock + tid;
+      q4[u] = j < n ? inv[j] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) a4[u] = b0 + u * kBlock + tid < n && L.acc[q4[u]];
+    uint64_t bal[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      bal[u] = __ballot(a4[u]);
+      if (lane == 0) s_wsum[u][wave] = (uint32_t)__popcll(bal[u]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t j = b0 + u * kBlock + tid;
+      const uint32_t q = q4[u];
+      const bool a = a4[u];
+      const uint32_t before_in_wave = (uint32_t)__popcll(bal[u] & (lane == 0 ? 0ull : (~0ull >> (64 - lane))));
+      uint32_t woff = 0, tot = 0;
+      for (int v = 0; v < kWaves; v++) { if (v < wave) woff += s_wsum[u][v]; tot += s_wsum[u][v]; }
+      if (j < n) {
+        int c = 0;
+#pragma unroll
+        for (int x = 1; x < 8; x++) c += j >= w.child_start[x] ? 1 : 0;
+        const int64_t local = j - w.child_start[c];
+        const int64_t apos = run + woff + before_in_wave;
+        if ((local & 63) == 0) L.tileAcc[tile_base[c] + (local >> 6)] = (uint32_t)apos;
+        L.accMain[j] = a ? 1 : 0;
+        if (a) {
+          if (avg) {
+            long long rs[3], ws;
+            if (seq_avg) {
+              for (int ch = 0; ch < 3; ch++) rs[ch] = L.sums[8 * q + ch];
+              ws = L.sums[8 * q + 3];
+            } else {
+              for (int ch = 0; ch < 3; ch++)
+                rs[ch] = 2 * ((long long)L.rgb[3 * j + ch] + L.sums[8 * q + ch]) + L.sums[8 * q + 4 + ch];
+              ws = 2 * (1 + L.sums[8 * q + 3]) + L.sums[8 * q + 7];
+            }
+            for (int ch = 0; ch < 3; ch++) L.rgb[3 * j + ch] = (uint16_t)(rs[ch] / ws);
           }
-          for (int ch = 0; ch < 3; ch++) L.rgb[3 * j + ch] = (uint16_t)(rs[ch] / ws);
         }
       }
+      run += tot;
     }
-    run += tot;
     __syncthreads();
   }
   if (tid == 0) {
