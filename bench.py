@@ -33,7 +33,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--points", type=int, default=200_000_000)
-    ap.add_argument("--method", default="poisson", choices=["poisson", "poisson_average"])
+    ap.add_argument("--method", default="poisson", choices=["poisson", "poisson_average", "random"])
+    ap.add_argument("--encoding", default="DEFAULT",
+                    help="--encoding of the converter: DEFAULT (octree image in HBM) or BROTLI (compressed nodes "
+                         "assembled in host memory)")
+    ap.add_argument("--random-seed", type=int, default=0, help="seed of -m random's leaf shuffles")
     ap.add_argument("--cpu-points", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
@@ -157,8 +161,15 @@ def main():
         L.pcg_device_free(src)
         L.pcg_device_free(dst)
 
+    if args.method == "random":
+        pc._check(L.pcg_set_option(b"random_seed", args.random_seed))
+
     def step():
-        pc._check(L.pcg_convert_device(d_las, n, header, args.method.encode(), None, stats))
+        if args.encoding == "DEFAULT":
+            pc._check(L.pcg_convert_device(d_las, n, header, args.method.encode(), None, stats))
+        else:
+            pc._check(L.pcg_convert_device_encoded(d_las, n, header, args.method.encode(), args.encoding.encode(),
+                                                   None, stats))
 
     for _ in range(args.warmup):
         step()
@@ -217,6 +228,7 @@ def main():
                         f"terrain fBm + 20 buildings, seed {args.seed}",
             "points": n,
             "method": args.method,
+            "encoding": args.encoding,
             "path": "batched (all nodes of a level per launch)",
             "grid": 128 if n < 100_000_000 else (256 if n < 500_000_000 else 512),
         },
