@@ -27,6 +27,8 @@
 // SamplerRandom (sampler_random.h:14-234) shares k_route / k_copy_tiles:
 //   k_shuffle_leaves  std::shuffle of every leaf (seeded variant)
 //   k_random_prepare  first taker of each 128^3 grid cell
+#include <algorithm>
+
 #include "lstd_shuffle.h"
 #include "lstd_sort.h"
 #include "pcg_common.h"
@@ -1722,7 +1724,11 @@ __global__ __launch_bounds__(kBlock) void k_avg_colors(const int32_t* __restrict
   const Layout L = make_layout(w.scratch, n, P.method);
   const NodeGeom g = node_geom(infos[nodes[k]], P);
   bool violation = false;
-  for (int64_t e = threadIdx.x; e < n; e += kBlock) {
+  // a node's candidates are split over gridDim.y workgroups (the sums are
+  // atomic), so that narrow levels fill the GPU
+  const int64_t per = (n + gridDim.y - 1) / gridDim.y;
+  const int64_t e0 = (int64_t)blockIdx.y * per, e1 = e0 + per < n ? e0 + per : n;
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
     const int4 me = L.ent[e];
     const uint32_t p = (uint32_t)me.w;
     const int32_t X = me.x, Y = me.y, Z = me.z;
@@ -2564,7 +2570,10 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   PCG_CHECK(hipGetLastError());
 
   if (p.method == METHOD_POISSON_AVERAGE) {
-    hipLaunchKernelGGL(k_avg_colors, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p);
+    // about 2048 workgroups in all, at most 64 per node
+    const unsigned slices = (unsigned)std::max(1, std::min(64, 2048 / std::max(1, (int)num_nodes)));
+    hipLaunchKernelGGL(k_avg_colors, dim3((unsigned)num_nodes, slices), dim3(kBlock), 0, s, d_nodes, d_info, d_work,
+                       p);
     PCG_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_sequential, grid, dim3(64), 0, s, d_nodes, d_info, d_work, p);
