@@ -1708,6 +1708,67 @@ __global__ __launch_bounds__(64 * kRingWaves) void k_ring(const int32_t* __restr
   flush_filter(F, P);
 }
 
+// Accepted entries of a Poisson-average node in cell order, with pre[e] =
+// accepted entries before entry e (n + 1 values), so that the colour pass
+// scans only the accepted points of a cell.  The arrays take the neighbour
+// list space, which nothing reads after k_accept.
+__device__ inline void avg_accepted_arrays(const Layout& L, int64_t n, int4*& aent, uint32_t*& aC16,
+                                           uint32_t*& pre) {
+  aent = (int4*)L.nbr;
+  aC16 = (uint32_t*)(aent + n);
+  pre = aC16 + n;
+}
+
+__global__ __launch_bounds__(kBlock) void k_avg_compact(NodeWork* __restrict__ work, SampleParams P) {
+  __shared__ uint32_t s_w[4][kWaves];
+  const int k = blockIdx.x;
+  const NodeWork w = work[k];
+  if (w.flags & NW_OVERFLOW) return;
+  const int64_t n = w.n;
+  const Layout L = make_layout(w.scratch, n, P.method);
+  int4* aent;
+  uint32_t *aC16, *pre;
+  avg_accepted_arrays(L, n, aent, aC16, pre);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  uint32_t run = 0;
+  for (int64_t b0 = 0; b0 < n; b0 += 4 * kBlock) {
+    int4 o[4];
+    bool a[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t e = b0 + u * kBlock + tid;
+      o[u] = e < n ? L.ent[e] : make_int4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) a[u] = b0 + u * kBlock + tid < n && L.acc[(uint32_t)o[u].w];
+    uint64_t bal[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      bal[u] = __ballot(a[u]);
+      if (lane == 0) s_w[u][wave] = (uint32_t)__popcll(bal[u]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t e = b0 + u * kBlock + tid;
+      uint32_t woff = 0, tot = 0;
+      for (int v = 0; v < kWaves; v++) { if (v < wave) woff += s_w[u][v]; tot += s_w[u][v]; }
+      const uint32_t off = run + woff + (uint32_t)__popcll(bal[u] & lt);
+      if (e < n) {
+        pre[e] = off;
+        if (a[u]) {
+          aent[off] = o[u];
+          aC16[off] = L.eC16[e];
+        }
+      }
+      run += tot;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) pre[n] = run;
+}
+
 // Poisson-average colour sums: every rejected candidate C adds its own colour
 // (weight 1) to each accepted P with P inside C's 16^3 range and dd < s^2,
 // into P's "before" or "after" slots by sorted rank.  With accepted points
@@ -1728,6 +1789,9 @@ __global__ __launch_bounds__(kBlock) void k_avg_colors(const int32_t* __restrict
   // atomic), so that narrow levels fill the GPU
   const int64_t per = (n + gridDim.y - 1) / gridDim.y;
   const int64_t e0 = (int64_t)blockIdx.y * per, e1 = e0 + per < n ? e0 + per : n;
+  int4* aent;
+  uint32_t *aC16, *pre;
+  avg_accepted_arrays(L, n, aent, aC16, pre);
   for (int64_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
     const int4 me = L.ent[e];
     const uint32_t p = (uint32_t)me.w;
@@ -1746,11 +1810,12 @@ __global__ __launch_bounds__(kBlock) void k_avg_colors(const int32_t* __restrict
           const uint32_t slot = table_find(L, gkey(cx, cy, cz));
           if (slot == kNil) continue;
           const uint4 t = L.htab[slot];
-          for (uint32_t f = t.y; f < t.z; f++) {
-            const int4 o = L.ent[f];
+          const uint32_t f0 = pre[t.y], f1 = pre[t.z];
+          for (uint32_t f = f0; f < f1; f++) {
+            const int4 o = aent[f];
             const uint32_t q = (uint32_t)o.w;
-            if (q == p || !L.acc[q]) continue;
-            if (!in_range16(L.eC16[f], lo16, hi16)) continue;
+            if (q == p) continue;
+            if (!in_range16(aC16[f], lo16, hi16)) continue;
             if (!(dd64_xyz(o.x, o.y, o.z, X, Y, Z, P) < g.s2)) continue;
             if (self_acc) { violation = true; continue; }
             unsigned long long* d = (unsigned long long*)(L.sums + 8 * (uint64_t)q + (p < q ? 0 : 4));
@@ -2572,6 +2637,8 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   if (p.method == METHOD_POISSON_AVERAGE) {
     // about 2048 workgroups in all, at most 64 per node
     const unsigned slices = (unsigned)std::max(1, std::min(64, 2048 / std::max(1, (int)num_nodes)));
+    hipLaunchKernelGGL(k_avg_compact, grid, dim3(kBlock), 0, s, d_work, p);
+    PCG_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_avg_colors, dim3((unsigned)num_nodes, slices), dim3(kBlock), 0, s, d_nodes, d_info, d_work,
                        p);
     PCG_CHECK(hipGetLastError());
