@@ -1238,18 +1238,35 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(const int32_t* __restrict_
             const uint32_t slot = table_find(L, gkey(cx, cy, cz));
             if (slot == kNil) continue;
             const uint4 t = L.htab[slot];
-            for (uint32_t f = t.y; f < t.z; f++) {
-              const int4 o = L.ent[f];
-              const uint32_t q = (uint32_t)o.w;
-              if (q >= p) break;
-              if (!within_xyz(o.x, o.y, o.z, X, Y, Z, P, F)) continue;
-              if (!avg) {
-                if (L.eAdd[f] < limSq) continue;
-              } else {
-                if (!in_range16(L.eC16[f], lo16, hi16)) continue;
+            // four entries loaded per step, then taken in order (entries of
+            // a cell rise in sorted position: the first later one ends it)
+            bool stop = false;
+            for (uint32_t f0 = t.y; f0 < t.z && !stop; f0 += 4) {
+              int4 o4[4];
+              double a4[4];
+              uint32_t c4[4];
+#pragma unroll
+              for (int u = 0; u < 4; u++) {
+                const uint32_t f = f0 + u;
+                o4[u] = f < t.z ? L.ent[f] : make_int4(0, 0, 0, -1);
+                if (!avg) a4[u] = f < t.z ? L.eAdd[f] : 0.0;
+                else c4[u] = f < t.z ? L.eC16[f] : 0u;
               }
-              emit(q, cnt);
-              cnt++;
+#pragma unroll
+              for (int u = 0; u < 4; u++) {
+                if (stop || f0 + u >= t.z) continue;
+                const int4 o = o4[u];
+                const uint32_t q = (uint32_t)o.w;
+                if (q >= p) { stop = true; continue; }
+                if (!within_xyz(o.x, o.y, o.z, X, Y, Z, P, F)) continue;
+                if (!avg) {
+                  if (a4[u] < limSq) continue;
+                } else {
+                  if (!in_range16(c4[u], lo16, hi16)) continue;
+                }
+                emit(q, cnt);
+                cnt++;
+              }
             }
           }
       return cnt;
