@@ -1828,18 +1828,28 @@ __global__ __launch_bounds__(kBlock) void k_avg_colors(const int32_t* __restrict
           if (slot == kNil) continue;
           const uint4 t = L.htab[slot];
           const uint32_t f0 = pre[t.y], f1 = pre[t.z];
-          for (uint32_t f = f0; f < f1; f++) {
-            const int4 o = aent[f];
-            const uint32_t q = (uint32_t)o.w;
-            if (q == p) continue;
-            if (!in_range16(aC16[f], lo16, hi16)) continue;
-            if (!(dd64_xyz(o.x, o.y, o.z, X, Y, Z, P) < g.s2)) continue;
-            if (self_acc) { violation = true; continue; }
-            unsigned long long* d = (unsigned long long*)(L.sums + 8 * (uint64_t)q + (p < q ? 0 : 4));
-            atomicAdd(&d[0], (unsigned long long)r);
-            atomicAdd(&d[1], (unsigned long long)gg);
-            atomicAdd(&d[2], (unsigned long long)b);
-            atomicAdd(&d[3], 1ull);
+          for (uint32_t fb = f0; fb < f1; fb += 4) {
+            int4 o4[4];
+            uint32_t c4[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              o4[u] = fb + u < f1 ? aent[fb + u] : make_int4(0, 0, 0, (int32_t)p);
+              c4[u] = fb + u < f1 ? aC16[fb + u] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int4 o = o4[u];
+              const uint32_t q = (uint32_t)o.w;
+              if (q == p) continue;
+              if (!in_range16(c4[u], lo16, hi16)) continue;
+              if (!(dd64_xyz(o.x, o.y, o.z, X, Y, Z, P) < g.s2)) continue;
+              if (self_acc) { violation = true; continue; }
+              unsigned long long* d = (unsigned long long*)(L.sums + 8 * (uint64_t)q + (p < q ? 0 : 4));
+              atomicAdd(&d[0], (unsigned long long)r);
+              atomicAdd(&d[1], (unsigned long long)gg);
+              atomicAdd(&d[2], (unsigned long long)b);
+              atomicAdd(&d[3], 1ull);
+            }
           }
         }
   }
