@@ -1497,7 +1497,7 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
 // coordinates.
 constexpr int kRingCap = 512;     // accepted points in the shell (power of two)
 constexpr int kRingSlots = 512;   // cell chains (direct-mapped, power of two)
-constexpr int kRingWaves = 2;     // nodes per k_ring block
+constexpr int kRingWaves = 4;     // nodes per k_ring block
 
 struct RingLds {
   float x[kRingCap], y[kRingCap], z[kRingCap];
