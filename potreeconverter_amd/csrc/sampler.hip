@@ -1496,7 +1496,7 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
 // re-checked in FP64 with the reference expression on the integer
 // coordinates.
 constexpr int kRingCap = 512;     // accepted points in the shell (power of two)
-constexpr int kRingSlots = 512;   // cell chains (direct-mapped, power of two)
+constexpr int kRingSlots = 1024;  // cell chains (direct-mapped, power of two)
 constexpr int kRingWaves = 4;     // nodes per k_ring block
 
 struct RingLds {
@@ -1519,8 +1519,9 @@ __device__ inline int ring_cell(float v, const RingGeom& G) {
 }
 
 __device__ inline uint32_t ring_slot(int cx, int cy, int cz) {
-  return ((uint32_t)(cx | (cy << 7) | (cz << 14)) * 2654435761u) >> (32 - 9);
+  return ((uint32_t)(cx | (cy << 7) | (cz << 14)) * 2654435761u) >> (32 - 10);  // 10 bits: kRingSlots
 }
+static_assert(kRingSlots == 1024, "ring_slot hashes cells to 10 bits");
 
 __global__ __launch_bounds__(64 * kRingWaves) void k_ring(const int32_t* __restrict__ nodes,
                                                            const NodeInfo* __restrict__ infos,
