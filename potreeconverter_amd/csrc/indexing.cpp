@@ -226,7 +226,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   // ------------------------------------------------------ octree build
   const double tb0 = now_s();
   DevMem bufB(WS_OCT_B), keysA(WS_OCT_KEYS_A), keysB(WS_OCT_KEYS_B), dM(WS_OCT_M), ws(WS_OCT_SCAN), dsegs(WS_OCT_SEGS),
-      dtiles(WS_OCT_TILES), dnodes(WS_OCT_NODES), dcounter(WS_OCT_COUNTER);
+      dtiles(WS_OCT_TILES), dnodes(WS_OCT_NODES), dcounter(WS_OCT_COUNTER), dh5(WS_OCT_H5);
   struct Seg { int32_t node; int64_t start, count; };
   std::vector<Seg> segs;
   for (int32_t id : chunkRoots)
@@ -270,14 +270,15 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     const uint32_t maxNodes = (uint32_t)std::min<int64_t>(segs.size() * 32768ll, 1ll << 26);
     if (dsegs.alloc(sd.size() * sizeof(SegDesc)) || dtiles.alloc(td.size() * sizeof(TileDesc)) ||
         dM.alloc((size_t)mTotal * 4 + 64) || ws.alloc(scan_workspace_bytes(mTotal)) ||
-        dnodes.alloc((size_t)maxNodes * sizeof(NodeOut)))
+        dnodes.alloc((size_t)maxNodes * sizeof(NodeOut)) || dh5.alloc(sd.size() * 32768 * sizeof(uint32_t)))
       return 1;
     PCG_CHECK(hipMemcpyAsync(dsegs.p, sd.data(), sd.size() * sizeof(SegDesc), hipMemcpyHostToDevice, s));
     PCG_CHECK(hipMemcpyAsync(dtiles.p, td.data(), td.size() * sizeof(TileDesc), hipMemcpyHostToDevice, s));
     PCG_CHECK(hipMemsetAsync(dcounter.p, 0, 16, s));
     if (octree_round(dc.d_records, bufB.as<uint8_t>(), keysA.as<uint16_t>(), keysB.as<uint16_t>(), bpp, op,
                      dsegs.as<SegDesc>(), (int32_t)sd.size(), dtiles.as<TileDesc>(), (int64_t)td.size(), mTotal,
-                     dM.as<uint32_t>(), ws.p, dnodes.as<NodeOut>(), dcounter.as<uint32_t>(), maxNodes, s))
+                     dM.as<uint32_t>(), ws.p, dnodes.as<NodeOut>(), dcounter.as<uint32_t>(), maxNodes,
+                     dh5.as<uint32_t>(), s))
       return 1;
     uint32_t numOut = 0;
     PCG_CHECK(hipMemcpyAsync(&numOut, dcounter.p, 4, hipMemcpyDeviceToHost, s));

@@ -40,6 +40,6 @@ struct NodeOut {
 int octree_round(uint8_t* d_a, uint8_t* d_b, uint16_t* d_ka, uint16_t* d_kb, int32_t bpp, const OctreeParams& p,
                  const SegDesc* d_segs, int32_t num_segs, const TileDesc* d_tiles, int64_t num_tiles,
                  int64_t m_total, uint32_t* d_M, void* d_scan_ws, NodeOut* d_nodes, uint32_t* d_node_counter,
-                 uint32_t max_nodes, hipStream_t s);
+                 uint32_t max_nodes, uint32_t* d_h5, hipStream_t s);
 
 }  // namespace pcg

@@ -10,6 +10,8 @@
 //   k_seg_scatter   stable LSD radix pass (8 then 7 bits) over the keys,
 //                   moving whole records; two passes equal the reference's
 //                   stable counting sort by the 15-bit key (:874-909)
+#include <algorithm>
+
 #include "octree.h"
 #include "pcg_common.h"
 #include "scan.h"
@@ -61,20 +63,22 @@ __global__ __launch_bounds__(kBlock) void k_seg_keys(const uint8_t* __restrict__
 // One workgroup per segment: histogram (32768 bins in LDS), pyramid, and
 // the list of realized nodes (level, morton index, start, count).
 constexpr int kPyrBlock = 1024;
-__global__ __launch_bounds__(kPyrBlock) void k_seg_pyramid(const uint16_t* __restrict__ keys,
-                                                            const SegDesc* __restrict__ segs,
-                                                            NodeOut* __restrict__ out, uint32_t* __restrict__ counter,
-                                                            uint32_t max_out) {
+// Per-segment 32^3 cell counts: gridDim.y workgroups per segment each count
+// a slice of its keys in LDS and add their non-zero bins to the segment's
+// global histogram (zeroed by the caller), so that every CU takes part.
+__global__ __launch_bounds__(kPyrBlock) void k_seg_count(const uint16_t* __restrict__ keys,
+                                                          const SegDesc* __restrict__ segs,
+                                                          uint32_t* __restrict__ h5g) {
   __shared__ uint32_t h5[32768];
-  __shared__ uint32_t up[4096 + 512 + 64 + 8 + 1];
-  __shared__ uint32_t s_wsum[kPyrBlock / 64];
   const SegDesc s = segs[blockIdx.x];
   for (int i = threadIdx.x; i < 32768; i += kPyrBlock) h5[i] = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  for (int64_t k0 = 0; k0 < s.count; k0 += kPyrBlock) {
+  const int64_t per = ((s.count + gridDim.y - 1) / gridDim.y + kPyrBlock - 1) / kPyrBlock * kPyrBlock;
+  const int64_t a = (int64_t)blockIdx.y * per, b = a + per < s.count ? a + per : s.count;
+  for (int64_t k0 = a; k0 < b; k0 += kPyrBlock) {
     int64_t k = k0 + threadIdx.x;
-    bool valid = k < s.count;
+    bool valid = k < b;
     uint32_t key = valid ? keys[s.start + k] : 0xFFFFu;
     // merge runs of equal keys inside the wave before the LDS atomics
     uint32_t prev = __shfl_up(key, 1, 64);
@@ -90,6 +94,23 @@ __global__ __launch_bounds__(kPyrBlock) void k_seg_pyramid(const uint16_t* __res
     }
   }
   __syncthreads();
+  uint32_t* g = h5g + (size_t)blockIdx.x * 32768;
+  for (int i = threadIdx.x; i < 32768; i += kPyrBlock)
+    if (h5[i]) atomicAdd(&g[i], h5[i]);
+}
+
+__global__ __launch_bounds__(kPyrBlock) void k_seg_pyramid(const uint32_t* __restrict__ h5g,
+                                                            const SegDesc* __restrict__ segs,
+                                                            NodeOut* __restrict__ out, uint32_t* __restrict__ counter,
+                                                            uint32_t max_out) {
+  __shared__ uint32_t h5[32768];
+  __shared__ uint32_t up[4096 + 512 + 64 + 8 + 1];
+  __shared__ uint32_t s_wsum[kPyrBlock / 64];
+  const SegDesc s = segs[blockIdx.x];
+  const uint32_t* g = h5g + (size_t)blockIdx.x * 32768;
+  for (int i = threadIdx.x; i < 32768; i += kPyrBlock) h5[i] = g[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
   // pyramid in morton layout: level l cell m = sum of level l+1 cells 8m..8m+7
   uint32_t* lv4 = up;
   uint32_t* lv3 = up + 4096;
@@ -313,7 +334,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_scatter(const uint8_t* __restric
 int octree_round(uint8_t* d_a, uint8_t* d_b, uint16_t* d_ka, uint16_t* d_kb, int32_t bpp, const OctreeParams& p,
                  const SegDesc* d_segs, int32_t num_segs, const TileDesc* d_tiles, int64_t num_tiles,
                  int64_t m_total, uint32_t* d_M, void* d_scan_ws, NodeOut* d_nodes, uint32_t* d_node_counter,
-                 uint32_t max_nodes, hipStream_t s) {
+                 uint32_t max_nodes, uint32_t* d_h5, hipStream_t s) {
   if (num_segs == 0) return 0;
   if (bpp > kMaxBpp) {
     set_error("record size above 128 bytes is not supported by the octree build");
@@ -321,7 +342,12 @@ int octree_round(uint8_t* d_a, uint8_t* d_b, uint16_t* d_ka, uint16_t* d_kb, int
   }
   hipLaunchKernelGGL(k_seg_keys, dim3((unsigned)num_tiles), dim3(kBlock), 0, s, d_a, bpp, d_segs, d_tiles, p, d_ka);
   PCG_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_seg_pyramid, dim3((unsigned)num_segs), dim3(kPyrBlock), 0, s, d_ka, d_segs, d_nodes,
+  // about 512 counting workgroups in all, at most 16 per segment
+  const unsigned slices = (unsigned)std::max(1, std::min(16, 512 / std::max(1, (int)num_segs)));
+  PCG_CHECK(hipMemsetAsync(d_h5, 0, (size_t)num_segs * 32768 * sizeof(uint32_t), s));
+  hipLaunchKernelGGL(k_seg_count, dim3((unsigned)num_segs, slices), dim3(kPyrBlock), 0, s, d_ka, d_segs, d_h5);
+  PCG_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_seg_pyramid, dim3((unsigned)num_segs), dim3(kPyrBlock), 0, s, d_h5, d_segs, d_nodes,
                      d_node_counter, max_nodes);
   PCG_CHECK(hipGetLastError());
   // pass 1: low 8 bits A -> B, pass 2: high 7 bits B -> A
