@@ -227,10 +227,11 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   const double tb0 = now_s();
   DevMem bufB(WS_OCT_B), keysA(WS_OCT_KEYS_A), keysB(WS_OCT_KEYS_B), dM(WS_OCT_M), ws(WS_OCT_SCAN), dsegs(WS_OCT_SEGS),
       dtiles(WS_OCT_TILES), dnodes(WS_OCT_NODES), dcounter(WS_OCT_COUNTER), dh5(WS_OCT_H5);
-  struct Seg { int32_t node; int64_t start, count; };
+  struct Seg { int32_t node; int64_t start, count; Vec3 mn, mx; };
   std::vector<Seg> segs;
   for (int32_t id : chunkRoots)
-    if (tree.nodes[id].count >= kLeafCap) segs.push_back({id, tree.nodes[id].off, tree.nodes[id].count});
+    if (tree.nodes[id].count >= kLeafCap)
+      segs.push_back({id, tree.nodes[id].off, tree.nodes[id].count, tree.nodes[id].min, tree.nodes[id].max});
   if (!segs.empty()) {
     if (bufB.alloc((size_t)N * bpp + 64) || keysA.alloc((size_t)N * 2 + 64) || keysB.alloc((size_t)N * 2 + 64) ||
         dcounter.alloc(16))
@@ -240,17 +241,22 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   for (int a = 0; a < 3; a++) { op.scale[a] = attributes.posScale[a]; op.offset[a] = attributes.posOffset[a]; }
   const int64_t tileSize = 4096;
   int64_t octreeDepth = 0;
-  while (!segs.empty()) {
-    out.buildRounds++;
-    std::vector<SegDesc> sd(segs.size());
-    std::vector<TileDesc> td;
+  // One round per level of refinement.  The next round's segments (nodes
+  // above the leaf cap, with their boxes from the childBoundingBoxOf chain)
+  // follow from a round's output alone, so that round is launched before
+  // the host expands the tree from the output.
+  std::vector<SegDesc> sd;  // descriptors of the round in flight (kept until it is done)
+  std::vector<TileDesc> td;
+  uint32_t maxNodes = 0;
+  auto launch_round = [&](const std::vector<Seg>& sg) -> int {
+    sd.assign(sg.size(), SegDesc());
+    td.clear();
     int64_t before = 0, tilesBefore = 0;
-    for (size_t i = 0; i < segs.size(); i++) {
-      const TNode& t = tree.nodes[segs[i].node];
+    for (size_t i = 0; i < sg.size(); i++) {
       SegDesc& d = sd[i];
-      d.start = segs[i].start;
-      d.count = segs[i].count;
-      for (int a = 0; a < 3; a++) { d.min[a] = t.min[a]; d.size[a] = t.max[a] - t.min[a]; }
+      d.start = sg[i].start;
+      d.count = sg[i].count;
+      for (int a = 0; a < 3; a++) { d.min[a] = sg[i].mn[a]; d.size[a] = sg[i].mx[a] - sg[i].mn[a]; }
       d.delta = d.start - before;
       int64_t nt = (d.count + tileSize - 1) / tileSize;
       for (int64_t k = 0; k < nt; k++) {
@@ -267,7 +273,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
       before += d.count;
     }
     const int64_t mTotal = 256 * tilesBefore;
-    const uint32_t maxNodes = (uint32_t)std::min<int64_t>(segs.size() * 32768ll, 1ll << 26);
+    maxNodes = (uint32_t)std::min<int64_t>(sg.size() * 32768ll, 1ll << 26);
     if (dsegs.alloc(sd.size() * sizeof(SegDesc)) || dtiles.alloc(td.size() * sizeof(TileDesc)) ||
         dM.alloc((size_t)mTotal * 4 + 64) || ws.alloc(scan_workspace_bytes(mTotal)) ||
         dnodes.alloc((size_t)maxNodes * sizeof(NodeOut)) || dh5.alloc(sd.size() * 32768 * sizeof(uint32_t)))
@@ -275,11 +281,14 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     PCG_CHECK(hipMemcpyAsync(dsegs.p, sd.data(), sd.size() * sizeof(SegDesc), hipMemcpyHostToDevice, s));
     PCG_CHECK(hipMemcpyAsync(dtiles.p, td.data(), td.size() * sizeof(TileDesc), hipMemcpyHostToDevice, s));
     PCG_CHECK(hipMemsetAsync(dcounter.p, 0, 16, s));
-    if (octree_round(dc.d_records, bufB.as<uint8_t>(), keysA.as<uint16_t>(), keysB.as<uint16_t>(), bpp, op,
-                     dsegs.as<SegDesc>(), (int32_t)sd.size(), dtiles.as<TileDesc>(), (int64_t)td.size(), mTotal,
-                     dM.as<uint32_t>(), ws.p, dnodes.as<NodeOut>(), dcounter.as<uint32_t>(), maxNodes,
-                     dh5.as<uint32_t>(), s))
-      return 1;
+    return octree_round(dc.d_records, bufB.as<uint8_t>(), keysA.as<uint16_t>(), keysB.as<uint16_t>(), bpp, op,
+                        dsegs.as<SegDesc>(), (int32_t)sd.size(), dtiles.as<TileDesc>(), (int64_t)td.size(), mTotal,
+                        dM.as<uint32_t>(), ws.p, dnodes.as<NodeOut>(), dcounter.as<uint32_t>(), maxNodes,
+                        dh5.as<uint32_t>(), s);
+  };
+  if (!segs.empty() && launch_round(segs)) return 1;
+  while (!segs.empty()) {
+    out.buildRounds++;
     uint32_t numOut = 0;
     PCG_CHECK(hipMemcpyAsync(&numOut, dcounter.p, 4, hipMemcpyDeviceToHost, s));
     PCG_CHECK(hipStreamSynchronize(s));
@@ -290,8 +299,36 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     std::vector<NodeOut> no(numOut);
     if (numOut) PCG_CHECK(stream_copy(no.data(), dnodes.p, numOut * sizeof(NodeOut), hipMemcpyDeviceToHost));
     std::vector<Seg> next;
-    tree.nodes.reserve(tree.nodes.size() + 2 * (size_t)numOut + 64);
-    for (const NodeOut& o : no) {
+    std::vector<size_t> nextFrom;  // the output record of each next segment
+    for (size_t k = 0; k < no.size(); k++) {
+      const NodeOut& o = no[k];
+      if ((int64_t)o.count <= kLeafCap) continue;
+      const Seg& sg = segs[o.seg];
+      if ((int64_t)o.count == sg.count) {
+        // indexer.cpp:1002-1098 (all points in one 1/32^5 cell): the
+        // reference's duplicate-removal path is not implemented here
+        int32_t id = sg.node;
+        for (int i = 0; i < o.level; i++) id = tree.child(id, (o.morton >> (3 * (o.level - 1 - i))) & 7);
+        set_error("a node of " + std::to_string(o.count) + " points could not be split (" +
+                  tree.nodes[id].full_name() + "); the duplicate-point path of the reference is not supported");
+        return 1;
+      }
+      Vec3 mn = sg.mn, mx = sg.mx;
+      for (int i = 0; i < o.level; i++) {
+        Vec3 a, b;
+        child_box(mn, mx, (int)((o.morton >> (3 * (o.level - 1 - i))) & 7), a, b);
+        mn = a;
+        mx = b;
+      }
+      next.push_back({-1, sg.start + (int64_t)o.start, (int64_t)o.count, mn, mx});
+      nextFrom.push_back(k);
+    }
+    if (!next.empty() && launch_round(next)) return 1;
+    // the tree from this round's output, while the GPU refines the next
+    std::vector<int32_t> outId(no.size());
+    tree.nodes.reserve(tree.nodes.size() + 2 * no.size() + 64);
+    for (size_t k = 0; k < no.size(); k++) {
+      const NodeOut& o = no[k];
       const Seg& sg = segs[o.seg];
       int32_t id = sg.node;
       for (int i = 0; i < o.level; i++) id = tree.child(id, (o.morton >> (3 * (o.level - 1 - i))) & 7);
@@ -299,19 +336,14 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
       t.off = sg.start + o.start;
       t.count = o.count;
       octreeDepth = std::max<int64_t>(octreeDepth, (int64_t)t.level);
-      if ((int64_t)o.count > kLeafCap) {
-        if ((int64_t)o.count == sg.count) {
-          // indexer.cpp:1002-1098 (all points in one 1/32^5 cell): the
-          // reference's duplicate-removal path is not implemented here
-          set_error("a node of " + std::to_string(o.count) + " points could not be split (" + t.full_name() +
-                    "); the duplicate-point path of the reference is not supported");
-          return 1;
-        }
-        next.push_back({id, t.off, t.count});
-      }
+      outId[k] = id;
     }
     // refined nodes hold no records themselves
-    for (const Seg& n2 : next) { tree.nodes[n2.node].off = -1; tree.nodes[n2.node].count = 0; }
+    for (size_t j = 0; j < next.size(); j++) {
+      next[j].node = outId[nextFrom[j]];
+      tree.nodes[next[j].node].off = -1;
+      tree.nodes[next[j].node].count = 0;
+    }
     segs.swap(next);
   }
   // segment roots that were split keep no records of their own
