@@ -196,6 +196,51 @@ __device__ inline uint64_t wave_match(uint32_t key, int bits, uint64_t active) {
   return peers;
 }
 
+// The distinct keys of one 64-point step, in the order of their lowest
+// lane.  For every valid lane: `peers`, the valid lanes with its key, and
+// `run`, the start of its key's run when the runs are laid out in that
+// order.  The first kMatchRounds keys are taken one per round (spatially
+// coherent input has one or two chunks per step); lanes still left after
+// that take the bitwise match and a prefix sum over their leaders.
+constexpr int kMatchRounds = 6;
+__device__ inline void step_runs(uint32_t key, bool valid, int key_bits, uint64_t lt_mask, uint64_t& peers,
+                                 uint32_t& run) {
+  const int lane = threadIdx.x & 63;
+  uint64_t left = __ballot(valid);
+  uint32_t start = 0;
+  peers = 0;
+  run = 0;
+  for (int r = 0; r < kMatchRounds && left; r++) {
+    const int L = __builtin_ctzll(left);
+    const uint32_t v = __builtin_amdgcn_readlane(key, L);
+    const uint64_t m = __ballot(key == v) & left;
+    if ((m >> lane) & 1) {
+      peers = m;
+      run = start;
+    }
+    start += (uint32_t)__popcll(m);
+    left &= ~m;
+  }
+  if (left) {
+    const bool mine = (left >> lane) & 1;
+    const uint64_t p = wave_match(key, key_bits, left);
+    const bool lead = mine && (p & lt_mask) == 0;
+    const uint32_t len = lead ? (uint32_t)__popcll(p) : 0u;
+    uint32_t incl = len;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t o = __shfl_up(incl, d);
+      if (lane >= d) incl += o;
+    }
+    const uint32_t rs = start + incl - len;  // valid on leader lanes
+    const uint32_t r2 = __shfl(rs, mine ? __builtin_ctzll(p) : lane);
+    if (mine) {
+      peers = p;
+      run = r2;
+    }
+  }
+}
+
 // bytes of one wave's record staging area in LDS
 __host__ __device__ inline int record_area_bytes(int bpp) { return (64 * bpp + 48 + 15) & ~15; }
 
@@ -572,12 +617,42 @@ __host__ __device__ inline size_t scatter_lds_bytes(int num_chunks, int bpp, int
          (size_t)kWavesPerBlock * (record_area_bytes(bpp) + out_area_bytes(bpo));
 }
 
+// Writes the step's runs from the LDS staging area to their places in the
+// output: aligned dwords inside a run, bytes where a run starts or ends
+// inside a word.  Leader lanes hold the run's first output record (`base`),
+// its length and its start in the staging area.
+__device__ inline void write_runs(uint64_t leaders, uint32_t base, uint32_t len, uint32_t run_start, int bpo,
+                                  const uint8_t* out_area, uint8_t* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  while (leaders) {
+    const int L = __builtin_ctzll(leaders);
+    leaders &= leaders - 1;
+    const uint64_t g0 = (uint64_t)__builtin_amdgcn_readlane(base, L) * (uint64_t)bpo;
+    const uint32_t head = (uint32_t)(g0 & 3);
+    const uint32_t nb = (uint32_t)__builtin_amdgcn_readlane(len, L) * (uint32_t)bpo + head;
+    uint8_t* gb = out + (g0 - head);  // 4-byte aligned; byte i <-> staging byte i - head
+    const uint8_t* sb = out_area + (size_t)__builtin_amdgcn_readlane(run_start, L) * bpo;
+    const uint32_t nw = (nb + 3) >> 2;
+    for (uint32_t u = lane; u < nw; u += 64) {
+      const uint32_t b = u << 2;
+      if (b >= head && b + 4 <= nb) {
+        *(uint32_t*)(gb + b) = lds_u32(sb + (b - head));
+      } else {
+        for (uint32_t k = b; k < b + 4; k++)
+          if (k >= head && k < nb) gb[k] = sb[k - head];
+      }
+    }
+  }
+}
+
 // K3b: transcode + stable scatter.  Each wave owns a contiguous quarter of
 // the tile; pass 1 counts per chunk per wave, pass 2 places each point at
 // M[c][t] + (earlier waves' count) + (running count) + (rank in step).  The
 // 64 output records of a step are assembled in LDS grouped into runs of one
-// chunk (a run is contiguous in the output), and every run is written with
-// aligned dword stores, bytes only where a run starts or ends inside a word.
+// chunk (a run is contiguous in the output) and written by write_runs.
+// This kernel transcodes with the byte operations of ChunkerParams::ops (any
+// record layout); k_scatter_tail takes the layouts of point formats 0-3 and
+// 6-7.
 __global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ las,
                                                      const uint32_t* __restrict__ ids, int64_t n,
                                                      int32_t tile_size, int32_t num_chunks, int32_t key_bits,
@@ -612,8 +687,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ 
       valid = cc < (uint32_t)num_chunks;
       c = valid ? cc : 0u;
     }
-    uint64_t active = __ballot(valid);
-    uint64_t peers = wave_match(c, key_bits, active);
+    uint64_t peers;
+    uint32_t run;
+    step_runs(c, valid, key_bits, lt_mask, peers, run);
     if (valid && (peers & lt_mask) == 0) my_cnt[c] += (uint32_t)__popcll(peers);
   }
   __syncthreads();
@@ -645,23 +721,14 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ 
       c = valid ? cc : 0u;
       quantize_record(rec, P, q);
     }
-    const uint64_t active = __ballot(valid);
-    const uint64_t peers = wave_match(c, key_bits, active);
+    uint64_t peers;
+    uint32_t my_run;
+    step_runs(c, valid, key_bits, lt_mask, peers, my_run);
     const uint32_t base = valid ? my_cnt[c] : 0;
     __builtin_amdgcn_wave_barrier();
     const bool leader = valid && (peers & lt_mask) == 0;
     const uint32_t len = leader ? (uint32_t)__popcll(peers) : 0u;
     if (leader) my_cnt[c] = base + len;
-    // runs in the staging area in leader-lane order
-    uint32_t incl = len;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t o = __shfl_up(incl, d);
-      if (lane >= d) incl += o;
-    }
-    const uint32_t run_start = incl - len;  // valid on leader lanes
-    const int my_leader = valid ? __builtin_ctzll(peers) : lane;
-    const uint32_t my_run = __shfl(run_start, my_leader);
     if (valid) {
       uint8_t* dst = out_area + (size_t)(my_run + (uint32_t)__popcll(peers & lt_mask)) * bpo;
       for (int a = 0; a < 3; a++) {
@@ -690,25 +757,194 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ 
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    // write each run: aligned dwords inside, bytes at partial edge words
-    uint64_t leaders = __ballot(leader);
-    while (leaders) {
-      const int L = __builtin_ctzll(leaders);
-      leaders &= leaders - 1;
-      const uint64_t g0 = (uint64_t)__builtin_amdgcn_readlane(base, L) * (uint64_t)bpo;
-      const uint32_t nbytes = (uint32_t)__builtin_amdgcn_readlane(len, L) * (uint32_t)bpo;
-      const uint8_t* src = out_area + (size_t)__builtin_amdgcn_readlane(run_start, L) * bpo;
-      const uint64_t wfirst = g0 >> 2, wlast = (g0 + nbytes - 1) >> 2;
-      for (uint64_t wi = wfirst + lane; wi <= wlast; wi += 64) {
-        const uint64_t b0 = wi << 2;
-        if (b0 >= g0 && b0 + 4 <= g0 + nbytes) {
-          *(uint32_t*)(out + b0) = lds_u32(src + (b0 - g0));
-        } else {
-          for (uint64_t b = b0; b < b0 + 4; b++)
-            if (b >= g0 && b < g0 + nbytes) out[b] = src[b - g0];
-        }
-      }
+    write_runs(__ballot(leader), base, len, my_run, bpo, out_area, out);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  }
+}
+
+// K3b for the record layouts of point formats 0-3 and 6-7 (with or without
+// extra bytes), whose output record is: the requantized position (12
+// bytes), intensity (LAS bytes 12-13), three bytes derived from LAS bytes 14
+// and 15 (formats 0-3: return number, number of returns, 5-bit class;
+// formats 6-7, EXT: legacy return number, legacy number of returns,
+// classification flags), then LAS bytes 16 .. bpp - 1 one byte later
+// (chunker_countsort_laszip.cpp:391-651 writes them attribute by attribute;
+// every one of them is a plain copy).  Each lane builds its output record as
+// dwords: one byte align per dword from the staged LAS record, one byte
+// permute to the record's alignment in the staging area, plain LDS stores
+// for the dwords that lie inside the record and LDS ORs (the area is zeroed
+// per step) for the two shared with the neighbouring records.
+// 16 bytes of the LAS buffer at a 16-byte aligned offset; bytes at or past
+// `total` read as 0
+__device__ inline uint4 load_vec16(const uint8_t* __restrict__ src, uint64_t total, uint64_t addr) {
+  if (addr + 16 <= total) return *(const uint4*)(src + addr);
+  uint32_t w[4] = {0u, 0u, 0u, 0u};
+  for (int b = 0; b < 16; b++)
+    if (addr + b < total) w[b >> 2] |= (uint32_t)src[addr + b] << (8 * (b & 3));
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// 16-byte vectors per lane that hold one step's records of up to `bpp`
+// bytes each (a step starts anywhere in a 16-byte vector)
+__host__ __device__ constexpr int prefetch_vecs(int bpp) { return (64 * bpp + 30) / (16 * 64) + 1; }
+
+template <bool EXT, int PV>
+__global__ __launch_bounds__(kBlock) void k_scatter_tail(const uint8_t* __restrict__ las,
+                                                          const uint32_t* __restrict__ ids, int64_t n,
+                                                          int32_t tile_size, int32_t num_chunks, int32_t key_bits,
+                                                          int64_t tiles, const uint32_t* __restrict__ M,
+                                                          uint8_t* __restrict__ out) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t s_dyn[];
+  const ChunkerParams& P = c_params;
+  const int bpp = P.bpp_in, bpo = P.bpp_out;  // bpo = bpp + 1
+  uint32_t* s_cnt = (uint32_t*)s_dyn;  // [kWavesPerBlock][num_chunks]
+  uint8_t* s_stage = s_dyn + (((size_t)kWavesPerBlock * num_chunks * 4 + 15) & ~(size_t)15);
+  const int64_t t = blockIdx.x;
+  for (int i = threadIdx.x; i < kWavesPerBlock * num_chunks; i += kBlock) s_cnt[i] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint8_t* rec_area = s_stage + (size_t)wave * (record_area_bytes(bpp) + out_area_bytes(bpo));
+  uint8_t* out_area = rec_area + record_area_bytes(bpp);
+  const uint64_t total = (uint64_t)n * bpp;
+  const int64_t t0 = t * tile_size;
+  const int64_t t1 = (t0 + tile_size) < n ? (t0 + tile_size) : n;
+  const int64_t per_wave = (tile_size + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int64_t w0 = t0 + wave * per_wave;
+  const int64_t w1 = (w0 + per_wave) < t1 ? (w0 + per_wave) : t1;
+  uint32_t* my_cnt = s_cnt + wave * num_chunks;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int64_t first = w0; first < w1; first += 64) {
+    int cnt = (int)((w1 - first) < 64 ? (w1 - first) : 64);
+    uint32_t c = 0;
+    bool valid = lane < cnt;
+    if (valid) {
+      const uint32_t cc = ids[first + lane];
+      valid = cc < (uint32_t)num_chunks;
+      c = valid ? cc : 0u;
     }
+    uint64_t peers;
+    uint32_t run;
+    step_runs(c, valid, key_bits, lt_mask, peers, run);
+    if (valid && (peers & lt_mask) == 0) my_cnt[c] += (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < num_chunks; c += kBlock) {
+    uint32_t run = M[(int64_t)c * tiles + t];
+    for (int w = 0; w < kWavesPerBlock; w++) {
+      uint32_t v = s_cnt[w * num_chunks + c];
+      s_cnt[w * num_chunks + c] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  const int out_vecs = (64 * bpo + 15) / 16;
+  const int ow = (bpo + 3) / 4;        // output dwords of a record
+  const int inner = (bpo - 4) / 4;     // dwords 1 .. inner lie inside the record at any alignment
+  const uint32_t last_mask = (bpo & 3) ? ((1u << (8 * (bpo & 3))) - 1u) : 0xFFFFFFFFu;
+  // pass 2: the next step's records and chunk ids are loaded into
+  // registers while the current step is transcoded
+  uint4 pf[PV];
+  uint32_t id_next = kEmpty;
+  auto issue = [&](int64_t f) {
+    const int m = (int)((w1 - f) < 64 ? (w1 - f) : 64);
+    const uint64_t b0 = (uint64_t)f * bpp, a0 = b0 & ~15ull;
+    const int nvec = (int)((b0 + (uint64_t)m * bpp - a0 + 15) >> 4);
+#pragma unroll
+    for (int j = 0; j < PV; j++) {
+      const int v = lane + 64 * j;
+      if (v < nvec) pf[j] = load_vec16(las, total, a0 + 16ull * v);
+    }
+    id_next = lane < m ? ids[f + lane] : kEmpty;
+  };
+  if (w0 < w1) issue(w0);
+  for (int64_t first = w0; first < w1; first += 64) {
+    int cnt = (int)((w1 - first) < 64 ? (w1 - first) : 64);
+    uint32_t shift;
+    {
+      const uint64_t b0 = (uint64_t)first * bpp, a0 = b0 & ~15ull;
+      const int nvec = (int)((b0 + (uint64_t)cnt * bpp - a0 + 15) >> 4);
+#pragma unroll
+      for (int j = 0; j < PV; j++) {
+        const int v = lane + 64 * j;
+        if (v < nvec) *(uint4*)(rec_area + 16 * v) = pf[j];
+      }
+      shift = (uint32_t)(b0 - a0);
+    }
+    const uint32_t cc = id_next;
+    if (first + 64 < w1) issue(first + 64);
+    for (int i = lane; i < out_vecs; i += 64) ((uint4*)out_area)[i] = make_uint4(0u, 0u, 0u, 0u);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    bool valid = lane < cnt;
+    uint32_t c = 0;
+    if (valid) {
+      valid = cc < (uint32_t)num_chunks;
+      c = valid ? cc : 0u;
+    }
+    uint64_t peers;
+    uint32_t my_run;
+    step_runs(c, valid, key_bits, lt_mask, peers, my_run);
+    const uint32_t base = valid ? my_cnt[c] : 0;
+    __builtin_amdgcn_wave_barrier();
+    const bool leader = valid && (peers & lt_mask) == 0;
+    const uint32_t len = leader ? (uint32_t)__popcll(peers) : 0u;
+    if (leader) my_cnt[c] = base + len;
+    if (valid) {
+      // the LAS record's dwords from aligned LDS reads
+      const uint8_t* rec = rec_area + shift + lane * bpp;
+      const uint32_t sh = (uint32_t)((uintptr_t)rec & 3);
+      const uint32_t* ra = (const uint32_t*)(rec - sh);
+      const uint32_t r0 = ra[0], r1 = ra[1], r2 = ra[2], r3 = ra[3], r4 = ra[4];
+      const uint32_t w[4] = {__builtin_amdgcn_alignbyte(r1, r0, sh), __builtin_amdgcn_alignbyte(r2, r1, sh),
+                             __builtin_amdgcn_alignbyte(r3, r2, sh), __builtin_amdgcn_alignbyte(r4, r3, sh)};
+      uint32_t o[4];
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        const double x = P.las_scale[a] * (int32_t)w[a] + P.las_offset[a];
+        o[a] = (uint32_t)(int32_t)((x - P.pos_offset[a]) / P.pos_scale[a]);
+      }
+      const uint32_t b14 = (w[3] >> 16) & 0xFFu, b15 = w[3] >> 24;
+      uint32_t ret, nret, cls;
+      if constexpr (EXT) {
+        ret = legacy_return_number(b14);
+        nret = legacy_number_of_returns(b14);
+        cls = b15 & 15u;
+      } else {
+        ret = b14 & 7u;
+        nret = (b14 >> 3) & 7u;
+        cls = b15 & 31u;
+      }
+      o[3] = (w[3] & 0xFFFFu) | (ret << 16) | (nret << 24);
+      // output dword k >= 4 holds LAS bytes 4k-1 .. 4k+2 (byte 16 of the
+      // output is the class byte): LAS byte offset sh + 4k - 1 from ra
+      const uint32_t* rb = ra + ((sh + 3) >> 2) - 1;
+      const uint32_t s2 = (sh + 3) & 3;
+      const uint32_t off = (my_run + (uint32_t)__popcll(peers & lt_mask)) * (uint32_t)bpo;
+      const uint32_t sel = 0x07060504u - (off & 3) * 0x01010101u;
+      uint32_t* dw = (uint32_t*)out_area + (off >> 2);
+      atomicOr(dw, __builtin_amdgcn_perm(o[0], 0u, sel));
+      dw[1] = __builtin_amdgcn_perm(o[1], o[0], sel);
+      dw[2] = __builtin_amdgcn_perm(o[2], o[1], sel);
+      dw[3] = __builtin_amdgcn_perm(o[3], o[2], sel);
+      uint32_t prev = o[3];
+      uint32_t lo = rb[4];
+      for (int k = 4; k < ow; k++) {
+        const uint32_t hi = rb[k + 1];
+        uint32_t v = __builtin_amdgcn_alignbyte(hi, lo, s2);
+        lo = hi;
+        if (k == 4) v = (v & 0xFFFFFF00u) | cls;
+        if (k == ow - 1) v &= last_mask;
+        const uint32_t x = __builtin_amdgcn_perm(v, prev, sel);
+        if (k <= inner) dw[k] = x;
+        else atomicOr(dw + k, x);
+        prev = v;
+      }
+      atomicOr(dw + ow, __builtin_amdgcn_perm(0u, prev, sel));
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    write_runs(__ballot(leader), base, len, my_run, bpo, out_area, out);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   }
@@ -747,6 +983,34 @@ __global__ void k_lasgen(LasGenParams p, const LasGenBuilding* __restrict__ bld,
 int g_bpp_in = 0;   // host copies of the record lengths in c_params
 int g_bpp_out = 0;
 int g_point_format = 8;  // k_count instantiation (8: no standard statistics)
+int g_tail_layout = -1;  // k_scatter_tail<false> (0), <true> (1), or k_scatter (-1)
+
+// The layout k_scatter_tail writes: intensity copied, the three derived
+// bytes at 14-16, then LAS bytes 16 .. bpp - 1 copied one byte later.
+int tail_layout(const ChunkerParams& p) {
+  if (p.bpp_out != p.bpp_in + 1 || p.bpp_in < 17 || p.num_ops < 4) return -1;
+  const TranscodeOp& i0 = p.ops[0];
+  if (i0.src != 12 || i0.dst != 12 || i0.len != 2 || i0.mode != TC_COPY) return -1;
+  int layout = -1;
+  const int legacy[3] = {TC_LOW3, TC_MID3, TC_LOW5}, ext[3] = {TC_RET14, TC_NRET14, TC_LOW4};
+  for (int v = 0; v < 2; v++) {
+    const int* modes = v ? ext : legacy;
+    bool ok = true;
+    for (int k = 0; k < 3; k++) {
+      const TranscodeOp& o = p.ops[1 + k];
+      ok = ok && o.src == (k < 2 ? 14 : 15) && o.dst == 14 + k && o.len == 1 && o.mode == modes[k];
+    }
+    if (ok) layout = v;
+  }
+  if (layout < 0) return -1;
+  int next = 16;
+  for (int k = 4; k < p.num_ops; k++) {
+    const TranscodeOp& o = p.ops[k];
+    if (o.mode != TC_COPY || o.src != next || o.dst != o.src + 1 || o.len <= 0) return -1;
+    next += o.len;
+  }
+  return next == p.bpp_in ? layout : -1;
+}
 
 }  // namespace
 
@@ -757,6 +1021,7 @@ int chunker_set_params(const ChunkerParams& p, hipStream_t stream) {
   }
   g_bpp_in = p.bpp_in;
   g_bpp_out = p.bpp_out;
+  g_tail_layout = tail_layout(p);
   static ChunkerParams staged;  // the async copy reads it after this call returns
   PCG_CHECK(hipStreamSynchronize(stream));
   staged = p;
@@ -868,8 +1133,23 @@ int chunker_distribute(const uint8_t* d_las, uint32_t* d_ids, int64_t n, const i
     set_error("too many chunks for the LDS scatter counters");
     return 1;
   }
-  hipLaunchKernelGGL(k_scatter, dim3((unsigned)tiles), dim3(kBlock), cnt_lds, stream, d_las, d_ids, n, tile_size,
-                     num_chunks, key_bits, tiles, d_M, d_out);
+  // k_scatter_tail instantiations: prefetch registers for records of up to
+  // 32 and 64 bytes (longer records take k_scatter)
+#define PCG_TAIL(E, V)                                                                                   \
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_scatter_tail<E, V>), dim3((unsigned)tiles), dim3(kBlock), cnt_lds, stream, \
+                     d_las, d_ids, n, tile_size, num_chunks, key_bits, tiles, d_M, d_out)
+  const bool short_rec = g_bpp_in <= 32;
+  if (g_tail_layout == 0 && g_bpp_in <= 64) {
+    if (short_rec) PCG_TAIL(false, prefetch_vecs(32));
+    else PCG_TAIL(false, prefetch_vecs(64));
+  } else if (g_tail_layout == 1 && g_bpp_in <= 64) {
+    if (short_rec) PCG_TAIL(true, prefetch_vecs(32));
+    else PCG_TAIL(true, prefetch_vecs(64));
+  } else {
+    hipLaunchKernelGGL(k_scatter, dim3((unsigned)tiles), dim3(kBlock), cnt_lds, stream, d_las, d_ids, n, tile_size,
+                       num_chunks, key_bits, tiles, d_M, d_out);
+  }
+#undef PCG_TAIL
   PCG_CHECK(hipGetLastError());
   return 0;
 }
