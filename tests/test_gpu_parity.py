@@ -74,6 +74,35 @@ def test_chunking_matches_oracle(fmt, eb, tmp_path):
     assert all(same_files(cpu / "chunks", gpu / "chunks", a).values())
 
 
+SHUFFLED_CASES = [(2, False), (7, True), (5, False)]
+
+
+@pytest.mark.parametrize("fmt,eb", SHUFFLED_CASES, ids=[f"pdrf{f}{'_eb' if e else ''}" for f, e in SHUFFLED_CASES])
+def test_chunking_of_shuffled_records_matches_oracle(fmt, eb, tmp_path):
+    """Records in random order, so that every 64-point step of the scatter
+    holds points of many chunks (the per-step run matching past its first
+    rounds); formats 2 and 7 take the dword scatter, format 5 the byte-wise
+    one."""
+    print(f"shuffled chunking case pdrf={fmt} extra={eb}", flush=True)
+    las = tmp_path / "in.las"
+    make_las(las, 1, 400_000, 40 + fmt, eb, fmt)
+    data = bytearray(las.read_bytes())
+    assert data[24:26] == bytes([1, 4])
+    off = int.from_bytes(data[96:100], "little")
+    rl = int.from_bytes(data[105:107], "little")
+    n = int.from_bytes(data[247:255], "little")
+    assert n == 400_000 and off + n * rl <= len(data)
+    recs = np.frombuffer(bytes(data[off:off + n * rl]), np.uint8).reshape(n, rl)
+    data[off:off + n * rl] = recs[np.random.default_rng(fmt).permutation(n)].tobytes()
+    las.write_bytes(bytes(data))
+    cpu, gpu = tmp_path / "cpu", tmp_path / "gpu"
+    oracle_convert(las, cpu, "poisson", chunk_only=True)
+    pc.do_chunking(las, gpu)
+    a, b = sorted(os.listdir(cpu / "chunks")), sorted(os.listdir(gpu / "chunks"))
+    assert a == b and len(a) > 8
+    assert all(same_files(cpu / "chunks", gpu / "chunks", a).values())
+
+
 FORMAT_CONVERT_CASES = [(1, "poisson", False), (3, "poisson_average", True), (4, "poisson", False),
                         (5, "poisson_average", False), (6, "poisson", True), (7, "poisson_average", False)]
 
