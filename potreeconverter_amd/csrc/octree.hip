@@ -265,6 +265,10 @@ __global__ __launch_bounds__(kBlock) void k_seg_scatter(const uint8_t* __restric
     }
   }
   __syncthreads();
+  const int out_vecs = (64 * bpp + 15) / 16;
+  const int ow = (bpp + 3) / 4;     // dwords of a record
+  const int inner = (bpp - 4) / 4;  // dwords 1 .. inner lie inside the record at any alignment
+  const uint32_t last_mask = (bpp & 3) ? ((1u << (8 * (bpp & 3))) - 1u) : 0xFFFFFFFFu;
   for (int64_t f = w0; f < w1; f += 64) {
     int cnt = (int)((w1 - f) < 64 ? (w1 - f) : 64);
     // stage the wave's records with 16-byte loads
@@ -275,6 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_scatter(const uint8_t* __restric
       int nvec = (int)((end - a0 + 15) >> 4);
       for (int v = lane; v < nvec; v += 64) *(uint4*)(rec_area + 16 * v) = *(const uint4*)(src + a0 + 16 * (uint64_t)v);
     }
+    for (int i = lane; i < out_vecs; i += 64) ((uint4*)out_area)[i] = make_uint4(0u, 0u, 0u, 0u);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     const uint32_t shift_b = (uint32_t)(((uint64_t)(t.start + f) * bpp) & 15ull);
@@ -299,9 +304,28 @@ __global__ __launch_bounds__(kBlock) void k_seg_scatter(const uint8_t* __restric
     const int my_leader = valid ? __builtin_ctzll(peers) : lane;
     const uint32_t my_run = __shfl(run_start, my_leader);
     if (valid) {
+      // the record as dwords: one byte align per dword from the staged
+      // record, one byte permute to its place in the output area, plain
+      // stores inside it, ORs (the area is zeroed per step) for the two
+      // dwords shared with the neighbouring records
       const uint8_t* r = rec_area + shift_b + lane * bpp;
-      uint8_t* o = out_area + (size_t)(my_run + (uint32_t)__popcll(peers & lt_mask)) * bpp;
-      for (int b = 0; b < bpp; b++) o[b] = r[b];
+      const uint32_t sh = (uint32_t)((uintptr_t)r & 3);
+      const uint32_t* ra = (const uint32_t*)(r - sh);
+      const uint32_t off = (my_run + (uint32_t)__popcll(peers & lt_mask)) * (uint32_t)bpp;
+      const uint32_t sel = 0x07060504u - (off & 3) * 0x01010101u;
+      uint32_t* dw = (uint32_t*)out_area + (off >> 2);
+      uint32_t prev = 0u, lo = ra[0];
+      for (int q = 0; q < ow; q++) {
+        const uint32_t hi = ra[q + 1];
+        uint32_t v = __builtin_amdgcn_alignbyte(hi, lo, sh);
+        lo = hi;
+        if (q == ow - 1) v &= last_mask;
+        const uint32_t x = __builtin_amdgcn_perm(v, prev, sel);
+        if (q >= 1 && q <= inner) dw[q] = x;
+        else atomicOr(dw + q, x);
+        prev = v;
+      }
+      atomicOr(dw + ow, __builtin_amdgcn_perm(0u, prev, sel));
       kdst[(int64_t)base + __popcll(peers & lt_mask) + delta] = key;
     }
     __builtin_amdgcn_wave_barrier();
