@@ -25,7 +25,12 @@ def lib():
             raise PotreeGpuError(f"{LIB_PATH} is missing; run potreeconverter_amd/build.py")
         L = ctypes.CDLL(LIB_PATH)
         c = ctypes
+        # every entry point gets its full parameter list (tests/test_capi.py
+        # checks them against include/potree_gpu.h): without argtypes ctypes
+        # passes Python ints as 32-bit C ints, which truncates 64-bit sizes
+        L.pcg_init.argtypes = []
         L.pcg_init.restype = c.c_int
+        L.pcg_last_error.argtypes = []
         L.pcg_last_error.restype = c.c_char_p
         L.pcg_convert.argtypes = [c.c_char_p, c.c_char_p, c.c_char_p, c.c_int32, c.POINTER(c.c_double)]
         L.pcg_do_chunking.argtypes = [c.c_char_p, c.c_char_p, c.POINTER(c.c_double)]

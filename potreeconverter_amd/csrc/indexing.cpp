@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <functional>
 #include <map>
@@ -141,6 +142,19 @@ hipEvent_t group_event(int i) {
     events.push_back(e);
   }
   return events[i];
+}
+
+// Distinct (X, Y, Z) positions among n records of bpp bytes in HBM: the key
+// of the reference's duplicate test (indexer.cpp:1010-1027, "X, Y, Z"
+// strings).  Only nodes that a refinement round cannot split come here.
+int count_distinct_positions(const uint8_t* d, int64_t n, int32_t bpp, int64_t& distinct) {
+  std::vector<uint8_t> h((size_t)n * bpp);
+  PCG_CHECK(stream_copy(h.data(), d, h.size(), hipMemcpyDeviceToHost));
+  std::vector<std::array<int32_t, 3>> k((size_t)n);
+  for (int64_t i = 0; i < n; i++) memcpy(k[i].data(), &h[(size_t)i * bpp], 12);
+  std::sort(k.begin(), k.end());
+  distinct = std::unique(k.begin(), k.end()) - k.begin();
+  return 0;
 }
 
 }  // namespace
@@ -300,18 +314,31 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     if (numOut) PCG_CHECK(stream_copy(no.data(), dnodes.p, numOut * sizeof(NodeOut), hipMemcpyDeviceToHost));
     std::vector<Seg> next;
     std::vector<size_t> nextFrom;  // the output record of each next segment
+    std::vector<int64_t> kept(no.size(), -1);  // records kept by the duplicate path
     for (size_t k = 0; k < no.size(); k++) {
       const NodeOut& o = no[k];
       if ((int64_t)o.count <= kLeafCap) continue;
       const Seg& sg = segs[o.seg];
-      if ((int64_t)o.count == sg.count) {
-        // indexer.cpp:1002-1098 (all points in one 1/32^5 cell): the
-        // reference's duplicate-removal path is not implemented here
-        int32_t id = sg.node;
-        for (int i = 0; i < o.level; i++) id = tree.child(id, (o.morton >> (3 * (o.level - 1 - i))) & 7);
-        set_error("a node of " + std::to_string(o.count) + " points could not be split (" +
-                  tree.nodes[id].full_name() + "); the duplicate-point path of the reference is not supported");
-        return 1;
+      int64_t count = o.count;
+      if (count == sg.count) {
+        // indexer.cpp:1002-1098: every point of the segment in one cell five
+        // levels down.  With at least maxPointsPerChunk / 2 duplicate
+        // positions the reference keeps #distinct records of the node, and
+        // they are its FIRST #distinct records in the current order (the
+        // copy at :1088-1090 indexes the buffer by the loop counter, not by
+        // the distinct list); then it tries again (:1093-1106): a leaf
+        // below the leaf cap, otherwise another round.  With fewer
+        // duplicates it only warns and refines the node as it is.
+        int64_t distinct = 0;
+        if (count_distinct_positions(dc.d_records + (size_t)(sg.start + (int64_t)o.start) * bpp, count, bpp,
+                                     distinct))
+          return 1;
+        if (count - distinct >= kLeafCap / 2) {
+          count = distinct;
+          kept[k] = distinct;
+          out.duplicateNodes++;
+          if (count < kLeafCap) continue;
+        }
       }
       Vec3 mn = sg.mn, mx = sg.mx;
       for (int i = 0; i < o.level; i++) {
@@ -320,7 +347,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
         mn = a;
         mx = b;
       }
-      next.push_back({-1, sg.start + (int64_t)o.start, (int64_t)o.count, mn, mx});
+      next.push_back({-1, sg.start + (int64_t)o.start, count, mn, mx});
       nextFrom.push_back(k);
     }
     if (!next.empty() && launch_round(next)) return 1;
@@ -334,7 +361,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
       for (int i = 0; i < o.level; i++) id = tree.child(id, (o.morton >> (3 * (o.level - 1 - i))) & 7);
       TNode& t = tree.nodes[id];
       t.off = sg.start + o.start;
-      t.count = o.count;
+      t.count = kept[k] >= 0 ? kept[k] : (int64_t)o.count;
       octreeDepth = std::max<int64_t>(octreeDepth, (int64_t)t.level);
       outId[k] = id;
     }
@@ -484,7 +511,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
       allLevels.insert(allLevels.end(), levels[g][l].begin(), levels[g][l].end());
     }
   if (dinfo.alloc(sizeof(NodeInfo) * numNodes) || dstate.alloc(sizeof(NodeState) * numNodes) ||
-      dlist.alloc(sizeof(int32_t) * (allLevels.size() + 1)) || arena.alloc(arenaCap) || counters.alloc(64) ||
+      dlist.alloc(sizeof(int32_t) * (allLevels.size() + 1)) || arena.alloc(arenaCap) || counters.alloc(128) ||
       gtop.alloc(64))
     return 1;
   // per-group buffers: node work, item and segment tables, neighbour pool,
@@ -545,7 +572,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   PCG_CHECK(hipMemcpyAsync(dinfo.p, info.data(), sizeof(NodeInfo) * numNodes, hipMemcpyHostToDevice, s));
   PCG_CHECK(hipMemcpyAsync(dstate.p, state.data(), sizeof(NodeState) * numNodes, hipMemcpyHostToDevice, s));
   PCG_CHECK(hipMemsetAsync(gtop.p, 0, 64, s));
-  PCG_CHECK(hipMemsetAsync(counters.p, 0, 64, s));
+  PCG_CHECK(hipMemsetAsync(counters.p, 0, 128, s));
   int64_t sampled = 0;
   // one level of one list set on one stream
   auto run_level = [&](int lg, int l, GroupMem& m, const SampleParams& p, hipStream_t st) -> int {
@@ -692,9 +719,13 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   }
   out.numSampled = sampled;
   PCG_CHECK(stream_copy(state.data(), dstate.p, sizeof(NodeState) * numNodes, hipMemcpyDeviceToHost));
-  unsigned long long hc[8];
+  unsigned long long hc[16];
   PCG_CHECK(stream_copy(hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost));
   for (int i = 0; i < 8; i++) out.counters[i] = hc[i];
+  if (hc[8] != 0) {
+    set_error("equal-distance order check failed in " + std::to_string(hc[8]) + " runs");
+    return 1;
+  }
   if (hc[6] != 0) {
     set_error("sampling arena or scratch exhausted in " + std::to_string(hc[6]) + " nodes");
     return 1;

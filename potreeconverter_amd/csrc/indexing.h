@@ -34,6 +34,7 @@ struct IndexResult {
   double t_setup = 0, t_order = 0, t_hierarchy = 0;  // host: tree, completion order + copy list, hierarchy + metadata
   uint64_t counters[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int64_t numNodes = 0, numSampled = 0, buildRounds = 0;
+  int64_t duplicateNodes = 0;  // nodes whose duplicate positions were dropped (indexer.cpp:1002-1098)
   void release();
 };
 

@@ -87,7 +87,9 @@ struct SampleParams {
   unsigned long long* counters;  // [0] fp64 re-checks, [1] pair tests, [2] max |dd32-dd64|/s^2 bits,
                                  // [3] exact-sort nodes, [4] sequential nodes, [5] candidates,
                                  // [6] nodes that ran out of scratch or arena,
-                                 // [7] candidates with more than 16 earlier neighbours
+                                 // [7] candidates with more than 16 earlier neighbours,
+                                 // [8] runs of equal distances whose libstdc++ order failed
+                                 // its window check (an error); 16 entries
   double random_limit;     // SamplerRandom: 0.7 * sqrt(3), the cell-centre distance bound
 };
 

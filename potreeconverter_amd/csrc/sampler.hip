@@ -333,6 +333,32 @@ __device__ bool block_radix_pass(const uint32_t* __restrict__ kin, const uint32_
   return true;
 }
 
+// End of the run of keys equal to a[p] in the ascending array a[0, n):
+// galloping from p, then a binary search, so a run of R members costs
+// O(log R) dependent loads instead of R.
+template <class T>
+__device__ inline int64_t run_end(const T* a, int64_t p, int64_t n) {
+  const T v = a[p];
+  int64_t lo = p + 1;  // a[lo - 1] == v
+  int64_t hi = n;
+  for (int64_t step = 1;; step <<= 1) {
+    const int64_t probe = lo + step - 1;
+    if (probe >= n) break;
+    if (!(a[probe] == v)) { hi = probe; break; }
+    lo = probe + 1;
+  }
+  while (lo < hi) {  // a[lo - 1] == v, and hi == n or a[hi] != v
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (a[mid] == v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Runs of equal sort keys up to this length are ordered by one lane
+// (insertion, at most kShortRun^2 / 2 steps); longer ones by block-wide
+// passes, so that no node costs quadratic time in one lane.
+constexpr int kShortRun = 32;
+
 // Can two members of the tie group [p, e) (sorted positions of equal
 // distance) exclude each other, so that their order changes a decision or
 // a colour sum?  Groups above 64 members are taken as conflicting.
@@ -471,7 +497,11 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
   }
   // runs of equal keys: order by (double add, main index); run starts are
   // found four positions per thread at a time so that the key loads are in
-  // flight together
+  // flight together.  Runs longer than kShortRun (near-duplicate clusters)
+  // are left to the full-width passes below.
+  __shared__ int s_long;
+  if (tid == 0) s_long = 0;
+  __syncthreads();
   for (int64_t b0 = 0; b0 < n; b0 += 4 * kBlock) {
     uint32_t kpv[4], kcur[4], knx[4];
 #pragma unroll
@@ -485,9 +515,11 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
     for (int u = 0; u < 4; u++) {
       const int64_t p = b0 + u * kBlock + tid;
       if (p >= n || kpv[u] == kcur[u] || knx[u] != kcur[u]) continue;
-      const uint32_t kp = kcur[u];
-      int64_t e = p + 1;
-      while (e < n && kc[e] == kp) e++;
+      const int64_t e = run_end(kc, p, n);
+      if (e - p > kShortRun) {
+        s_long = 1;
+        continue;
+      }
       for (int64_t i = p + 1; i < e; i++) {
         uint32_t v = ic[i];
         double dv = L.add[v];
@@ -501,6 +533,25 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
     }
   }
   __syncthreads();
+  if (s_long) {
+    // a long run of equal float keys: order the whole node by the 64 bits of
+    // the double distance (non-negative, so its bit pattern ascends with
+    // it), low word then high word, eight stable passes from the current
+    // order, in which equal doubles are in main-index order
+    for (int half = 0; half < 2; half++) {
+      for (int64_t p = tid; p < n; p += kBlock) {
+        uint64_t b;
+        __builtin_memcpy(&b, &L.add[ic[p]], 8);
+        kc[p] = half == 0 ? (uint32_t)b : (uint32_t)(b >> 32);
+      }
+      __syncthreads();
+      for (int pass = 0; pass < 4; pass++) {
+        if (!block_radix_pass(kc, ic, ko, io, n, 8 * pass, 255u, s_rad)) continue;
+        uint32_t* t = kc; kc = ko; ko = t;
+        t = ic; ic = io; io = t;
+      }
+    }
+  }
   // sorted positions and distances
   for (int64_t b0 = 0; b0 < n; b0 += 4 * kBlock) {
     uint32_t v[4];
@@ -809,8 +860,7 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
   const bool all_ties = s_cap != 0;
   for (int64_t p = tid; p + 1 < n; p += kBlock) {
     if (!(L.sadd[p] == L.sadd[p + 1]) || (p > 0 && L.sadd[p - 1] == L.sadd[p])) continue;
-    int64_t e = p + 1;
-    while (e < n && L.sadd[e] == L.sadd[p]) e++;
+    const int64_t e = run_end(L.sadd, p, n);
     if (all_ties || tie_group_conflicts(L, g, P, p, e)) {
       for (int64_t q = p; q < e; q++) mark[L.idxA[q]] = 1;
       s_any = 1;
@@ -937,25 +987,62 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
     }
     __syncthreads();
   }
-  // final positions after the partition phase, then order each run of equal
-  // distances (from the radix order in idxA) by those positions
+  // Positions after the partition phase, then each marked run of equal
+  // distances [p, e) (radix order in idxA) is put in the order of those
+  // positions.  Every marked element lies in a final segment of at most 16
+  // elements (or a heap-sorted one, where it already holds its final
+  // place), and the final insertion sort keeps it inside that segment, so
+  // the run's members sit at post-partition positions [p - 15, e + 15).
+  // Short runs: one lane, insertion by position.  Long runs: one wave walks
+  // that window in order and compacts the run's members by ballot.
+  // Unmarked runs keep the radix order: no decision depends on it.
   for (int64_t i = tid; i < n; i += kBlock) posLe[a.H[i]] = (uint32_t)i;
   __syncthreads();
   for (int64_t p = tid; p < n; p += kBlock) L.sadd[p] = L.add[L.idxA[p]];
   __syncthreads();
-  for (int64_t p = tid; p < n; p += kBlock) {
-    if (p > 0 && L.sadd[p - 1] == L.sadd[p]) continue;
-    int64_t e = p + 1;
-    while (e < n && L.sadd[e] == L.sadd[p]) e++;
-    for (int64_t i = p + 1; i < e; i++) {
-      uint32_t v = L.idxA[i];
-      uint32_t pv = posLe[v];
-      int64_t q = i - 1;
-      while (q >= p && posLe[L.idxA[q]] > pv) {
-        L.idxA[q + 1] = L.idxA[q];
-        q--;
+  if (s_any) {
+    const int lane = tid & 63, wave = tid >> 6;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    for (int64_t base = 64 * (int64_t)wave; base < n; base += kBlock) {
+      const int64_t p = base + lane;
+      bool head = p + 1 < n && L.sadd[p] == L.sadd[p + 1] && !(p > 0 && L.sadd[p - 1] == L.sadd[p]);
+      if (head) head = mark[L.idxA[p]] != 0;
+      const int64_t e = head ? run_end(L.sadd, p, n) : p;
+      const bool longrun = head && e - p > kShortRun;
+      if (head && !longrun) {
+        for (int64_t i = p + 1; i < e; i++) {
+          uint32_t v = L.idxA[i];
+          uint32_t pv = posLe[v];
+          int64_t q = i - 1;
+          while (q >= p && posLe[L.idxA[q]] > pv) {
+            L.idxA[q + 1] = L.idxA[q];
+            q--;
+          }
+          L.idxA[q + 1] = v;
+        }
       }
-      L.idxA[q + 1] = v;
+      for (uint64_t lm = __ballot(longrun); lm; lm &= lm - 1) {
+        const int src = __builtin_ctzll(lm);
+        const int64_t rp = __shfl((int)p, src), re = __shfl((int)e, src);
+        const double key = L.sadd[rp];
+        const int64_t w0 = rp > 16 ? rp - 16 : 0, w1 = re + 16 < n ? re + 16 : n;
+        int64_t r = rp;
+        for (int64_t j0 = w0; j0 < w1; j0 += 64) {
+          const int64_t j = j0 + lane;
+          uint32_t h = 0;
+          bool hit = false;
+          if (j < w1) {
+            h = a.H[j];
+            hit = L.add[h] == key;
+          }
+          const uint64_t hm = __ballot(hit);
+          if (hit && r + __popcll(hm & lt) < re) L.idxA[r + __popcll(hm & lt)] = h;
+          r += __popcll(hm);
+        }
+        // the window held exactly the run: anything else is a broken
+        // invariant, reported instead of a wrong order
+        if (lane == 0 && r != re) atomicAdd(&counters[8], 1ull);
+      }
     }
   }
 }
@@ -1857,39 +1944,136 @@ __global__ __launch_bounds__(kBlock) void k_avg_colors(const int32_t* __restrict
   if (violation) atomicOr(&work[k].flags, (uint32_t)NW_SEQ_AVERAGE);
 }
 
-// Exact sequential forms for the nodes flagged above (single lane).
+// Poisson acceptance with the 10 001-check limit of the reference scan
+// (sampler_poisson.h:147-176), for nodes where it can bind; one wavefront.
+// The scan visits the accepted points newest first and stops at the first
+// one closer to the centre than (cd - spacing)^2.  Accepted points come in
+// ascending distance, so an accepted neighbour q within spacing (the
+// neighbour lists hold exactly those not behind that stop) rejects
+// candidate p iff the scan reaches it inside its 10 001 checks:
+//   na(p) - 1 - accIndex(q) <= 10 000,
+// where na(p) is the number accepted before p and accIndex(q) the number
+// accepted before q.  accIndex comes from the accepted bits and a prefix
+// count per 32-candidate word; in-window neighbours are always inside the
+// limit; a candidate whose latest earlier neighbour sits at the edge of
+// the limit waits until every lower lane of its window is decided.
+__device__ void accept_node_capped(const Layout& L, const NodeGeom& g, int64_t n, const SampleParams& P) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  uint32_t* bits = L.keyA;  // accepted flags by sorted position
+  uint32_t* pre = L.keyB;   // accepted candidates before each word of `bits`
+  const int64_t words = (n + 31) / 32;
+  for (int64_t i = lane; i < words; i += 64) { bits[i] = 0; pre[i] = 0; }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+  Filter F = make_filter(g, P);
+  int64_t A0 = 0;  // accepted before the window
+  for (int64_t w0 = 0; w0 < n; w0 += 64) {
+    const int64_t p = w0 + lane;
+    const bool valid = p < n;
+    uint64_t in_mask = 0;
+    int64_t amax = -1;  // largest accIndex of an accepted earlier-window neighbour
+    auto consider = [&](uint32_t q) {
+      if ((int64_t)q >= w0) {
+        in_mask |= 1ull << (q - w0);
+        return;
+      }
+      const uint32_t wd = bits[q >> 5];
+      if (!((wd >> (q & 31)) & 1u)) return;
+      const int64_t ai = (int64_t)pre[q >> 5] + __popc(wd & ((1u << (q & 31)) - 1u));
+      amax = ai > amax ? ai : amax;
+    };
+    if (valid) {
+      const int cnt = L.nbrCount[p];
+      if (cnt != kNbrOverflow && cnt != kNbrPooled) {
+        const uint32_t* row = L.nbr + (uint64_t)p * kMaxNbr;
+        for (int i = 0; i < cnt; i++) consider(row[i]);
+      } else if (cnt == kNbrPooled) {
+        const uint32_t* rw = L.nbr + (uint64_t)p * kMaxNbr;
+        const uint32_t* lst = P.nbr_pool + ((uint64_t)rw[0] | ((uint64_t)rw[1] << 32));
+        const uint32_t len = rw[2];
+        for (uint32_t i = 0; i < len; i++) consider(lst[i]);
+      } else {
+        const double cd = __builtin_sqrt(L.sadd[p]);
+        const double lim = cd - g.spacing;
+        const double limSq = lim * lim;
+        const uint32_t j = L.idxA[p];
+        const int32_t X = L.X[j], Y = L.Y[j], Z = L.Z[j];
+        const double x = coord(X, P.scale[0], P.offset[0]), y = coord(Y, P.scale[1], P.offset[1]),
+                     z = coord(Z, P.scale[2], P.offset[2]);
+        const CellBox cb = cell_box(g, x, y, z);
+        for (int cz = cb.lz; cz <= cb.hz; cz++)
+          for (int cy = cb.ly; cy <= cb.hy; cy++)
+            for (int cx = cb.lx; cx <= cb.hx; cx++) {
+              const uint32_t slot = table_find(L, gkey(cx, cy, cz));
+              if (slot == kNil) continue;
+              const uint4 t = L.htab[slot];
+              for (uint32_t f = t.y; f < t.z; f++) {
+                const int4 o = L.ent[f];
+                const uint32_t q = (uint32_t)o.w;
+                if ((int64_t)q >= p) break;
+                if (L.eAdd[f] < limSq) continue;
+                if (!within_xyz(o.x, o.y, o.z, X, Y, Z, P, F)) continue;
+                consider(q);
+              }
+            }
+      }
+    }
+    // na(p) lies in [A0, A0 + lane]
+    const bool rej = valid && amax >= 0 && amax >= A0 + lane - 10001;
+    const bool edge = valid && !rej && amax >= 0 && amax >= A0 - 10001;
+    bool decided = !valid || rej;
+    bool accepted = false;
+    uint64_t dec = __ballot(decided), accm = 0;
+    while (dec != ~0ull) {
+      if (!decided) {
+        if (in_mask & accm) {
+          decided = true;
+        } else if ((in_mask & ~dec) == 0) {
+          if (!edge) {
+            decided = true;
+            accepted = true;
+          } else if ((~dec & lt) == 0) {
+            const int64_t na = A0 + __popcll(accm & lt);
+            decided = true;
+            accepted = !(amax >= na - 10001);
+          }
+        }
+      }
+      dec = __ballot(decided);
+      accm = __ballot(accepted);
+    }
+    if (valid) L.acc[p] = accepted ? 1 : 0;
+    if (lane == 0) {
+      const int64_t wd = w0 >> 5;
+      bits[wd] = (uint32_t)accm;
+      pre[wd] = (uint32_t)A0;
+      if (wd + 1 < words) {
+        bits[wd + 1] = (uint32_t)(accm >> 32);
+        pre[wd + 1] = (uint32_t)(A0 + __popc((uint32_t)accm));
+      }
+    }
+    A0 += __popcll(accm);
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+  }
+  flush_filter(F, P);
+}
+
+// Exact forms for the nodes flagged above: the capped Poisson scan by one
+// wavefront, the Poisson-average colour pass by one lane.
 __global__ void k_sequential(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
                              NodeWork* __restrict__ work, SampleParams P) {
   const int k = blockIdx.x;
   NodeWork w = work[k];
   if ((w.flags & NW_OVERFLOW) || !(w.flags & (NW_SEQ_ACCEPT | NW_SEQ_AVERAGE))) return;
-  if (threadIdx.x != 0) return;
-  atomicAdd(&P.counters[4], 1ull);
+  if (threadIdx.x == 0) atomicAdd(&P.counters[4], 1ull);
   const int64_t n = w.n;
   Layout L = make_layout(w.scratch, n, P.method);
   NodeGeom g = node_geom(infos[nodes[k]], P);
   if (P.method == METHOD_POISSON) {
-    // sampler_poisson.h:133-180 over the accepted list kept in `next`
-    int64_t na = 0;
-    for (int64_t p = 0; p < n; p++) {
-      uint32_t j = L.idxA[p];
-      double cd = __builtin_sqrt(L.sadd[p]);
-      double limit = cd - g.spacing;
-      double limitSquared = limit * limit;
-      bool accept = true;
-      int64_t cnt = 0;
-      for (int64_t i = na - 1; i >= 0; i--) {
-        uint32_t q = L.next[i];
-        if (L.sadd[q] < limitSquared) break;
-        if (dd64_pair(L, L.idxA[q], j, P) < g.s2) { accept = false; break; }
-        cnt++;
-        if (cnt > 10000) break;
-      }
-      L.acc[p] = accept ? 1 : 0;
-      if (accept) L.next[na++] = (uint32_t)p;
-    }
+    accept_node_capped(L, g, n, P);
     return;
   }
+  if (threadIdx.x != 0) return;
   // sampler_poisson_average.h:184-230 and :357-416 with per-cell lists in
   // push order (head in hkey, tail in hhead)
   for (int c = 0; c < 4096; c++) { L.hkey[c] = kNil; L.hhead[c] = kNil; }

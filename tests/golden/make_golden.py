@@ -13,6 +13,10 @@ SamplerRandom seeds its leaf shuffles from the clock; the driver fixes the
 clock at the case's seed (REF_CLOCK_NS), the seeded variant the oracle and
 the GPU path implement.
 
+Edge-case scenes (tests/scenes.py: long equal-distance runs, duplicate
+stacks, exact-spacing lattices, the 10 001-check limit) are written the same
+way from their scene files.
+
 Run from the repository root:  python tests/golden/make_golden.py [case names]
 """
 import json
@@ -27,6 +31,7 @@ sys.path.insert(0, os.path.dirname(HERE))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 from helpers import REF_INDEXER, digest_output, make_las, oracle_convert  # noqa: E402
+from scenes import make_scene  # noqa: E402
 
 CASES = [
     # name, preset, points, seed, extra bytes, method
@@ -48,6 +53,37 @@ CASES = [
     ("c1_1m_utm_random_brotli", 1, 1_000_000, 1, False, "random", "BROTLI", 4000000000),
 ]
 
+# edge-case scenes of tests/scenes.py: name, scene, method
+SCENE_CASES = [
+    ("scene_stacks_poisson", "stacks", "poisson"),
+    ("scene_stacks_poisson_average", "stacks", "poisson_average"),
+    ("scene_dups_poisson", "dups", "poisson"),
+    ("scene_dups_poisson_average", "dups", "poisson_average"),
+    ("scene_spacing_poisson", "spacing", "poisson"),
+    ("scene_spacing_poisson_average", "spacing", "poisson_average"),
+    ("scene_sphere_poisson", "sphere", "poisson"),
+    ("scene_sphere_poisson_average", "sphere", "poisson_average"),
+]
+
+
+def write_fixture(td, name, case, las, n, method, encoding="DEFAULT", random_seed=None):
+    work = os.path.join(td, name)
+    oracle_convert(las, work, method, keep_chunks=True, chunk_only=True)
+    env = dict(os.environ, REF_CLOCK_NS=str(random_seed or 0))
+    subprocess.run([REF_INDEXER, work, method, name, str(n), encoding], check=True, stdout=subprocess.DEVNULL,
+                   stderr=subprocess.DEVNULL, env=env)
+    with open(os.path.join(work, "metadata.json")) as f:
+        meta = f.read()
+    fixture = {
+        "case": case,
+        "producer": "reference indexer (Converter/src/indexer.cpp) on oracle-chunked input",
+        "metadata_json": meta,
+        "nodes": digest_output(work),
+    }
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(fixture, f, indent=0, sort_keys=True)
+    print(name, len(fixture["nodes"]), "nodes")
+
 
 def main():
     assert os.path.exists(REF_INDEXER), "build the reference indexer first: bash oracle/build_ref.sh"
@@ -61,23 +97,16 @@ def main():
             random_seed = rest[1] if len(rest) > 1 else None
             las = os.path.join(td, name + ".las")
             make_las(las, preset, n, seed, eb)
-            work = os.path.join(td, name)
-            oracle_convert(las, work, method, keep_chunks=True, chunk_only=True)
-            env = dict(os.environ, REF_CLOCK_NS=str(random_seed or 0))
-            subprocess.run([REF_INDEXER, work, method, name, str(n), encoding], check=True, stdout=subprocess.DEVNULL,
-                           stderr=subprocess.DEVNULL, env=env)
-            with open(os.path.join(work, "metadata.json")) as f:
-                meta = f.read()
-            fixture = {
-                "case": {"preset": preset, "points": n, "seed": seed, "extra_bytes": eb, "method": method,
-                         "encoding": encoding, **({"random_seed": random_seed} if random_seed is not None else {})},
-                "producer": "reference indexer (Converter/src/indexer.cpp) on oracle-chunked input",
-                "metadata_json": meta,
-                "nodes": digest_output(work),
-            }
-            with open(os.path.join(HERE, name + ".json"), "w") as f:
-                json.dump(fixture, f, indent=0, sort_keys=True)
-            print(name, len(fixture["nodes"]), "nodes")
+            case = {"preset": preset, "points": n, "seed": seed, "extra_bytes": eb, "method": method,
+                    "encoding": encoding, **({"random_seed": random_seed} if random_seed is not None else {})}
+            write_fixture(td, name, case, las, n, method, encoding, random_seed)
+        for name, scene, method in SCENE_CASES:
+            if only and name not in only:
+                continue
+            las = os.path.join(td, name + ".las")
+            n = make_scene(scene, las)
+            case = {"scene": scene, "points": n, "method": method, "encoding": "DEFAULT"}
+            write_fixture(td, name, case, las, n, method)
     finally:
         shutil.rmtree(td, ignore_errors=True)
 

@@ -15,6 +15,7 @@ import pytest
 
 import potreeconverter_amd as pc
 from helpers import digest_output, make_las, oracle, oracle_convert, read
+from scenes import make_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -30,10 +31,6 @@ CASES = [
     (2, 300_000, 11, True),
 ]
 
-# (method, preset, points, seed) cases kept off the GPU: this one hung with an
-# earlier lock-based version of the tie-order kernel whose failure mechanism
-# was not established; that kernel was replaced
-OFF_GPU = {("poisson", 0, 2_000_000, 0)}
 
 
 def same_files(a, b, names):
@@ -43,8 +40,6 @@ def same_files(a, b, names):
 @pytest.mark.parametrize("preset,n,seed,eb", CASES, ids=[f"p{c[0]}_{c[1]}_s{c[2]}{'_eb' if c[3] else ''}" for c in CASES])
 @pytest.mark.parametrize("method", ["poisson", "poisson_average"])
 def test_convert_matches_oracle(preset, n, seed, eb, method, tmp_path):
-    if (method, preset, n, seed) in OFF_GPU:
-        pytest.skip("kept off the GPU: hung with a replaced kernel whose failure mechanism was not established")
     print(f"case preset={preset} n={n} seed={seed} extra={eb} method={method}", flush=True)
     las = tmp_path / "in.las"
     make_las(las, preset, n, seed, eb)
@@ -327,3 +322,30 @@ def test_sample_groups_agree(tmp_path):
             assert all(res.values()), (g, res)
     finally:
         _set("sample_groups", 0)
+
+
+SCENE_CASES = [(sc, m) for sc in ("stacks", "dups", "spacing", "sphere") for m in ("poisson", "poisson_average")]
+
+
+@pytest.mark.parametrize("scene,method", SCENE_CASES, ids=[f"{a}_{b}" for a, b in SCENE_CASES])
+def test_edge_scene_matches_oracle(scene, method, tmp_path):
+    """The edge-case scenes (pinned to the reference indexer by fixtures):
+    equal-distance runs longer than a wavefront, the duplicate path of
+    buildHierarchy, neighbours exactly one spacing apart, and the 10 001-check
+    limit of the Poisson scan, which must route root candidates through the
+    capped acceptance."""
+    print(f"scene case {scene} method={method}", flush=True)
+    las = tmp_path / "in.las"
+    n = make_scene(scene, las)
+    cpu, gpu = tmp_path / "cpu", tmp_path / "gpu"
+    oracle_convert(las, cpu, method)
+    t = pc.convert(las, gpu, method)
+    res = same_files(cpu, gpu, ["octree.bin", "hierarchy.bin", "metadata.json"])
+    assert all(res.values()), res
+    if scene == "stacks":
+        assert t[6] > 0  # nodes put in libstdc++ order
+    if scene == "sphere" and method == "poisson":
+        assert t[7] > 0  # nodes where the 10 001-check limit can bind
+    if scene == "dups" and method == "poisson":
+        # 14 999 + 10 000 duplicate records are dropped as the reference drops them
+        assert os.path.getsize(gpu / "octree.bin") == 27 * (n - 24_999)

@@ -14,6 +14,7 @@ import subprocess
 import pytest
 
 from helpers import REF_INDEXER, digest_output, make_las, oracle_convert
+from scenes import make_scene
 
 GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.json")))
 
@@ -25,7 +26,10 @@ def test_oracle_matches_reference_fixture(fixture, tmp_path):
     case = fx["case"]
     name = os.path.basename(fixture)[:-5]
     las = tmp_path / (name + ".las")
-    make_las(las, case["preset"], case["points"], case["seed"], case["extra_bytes"])
+    if "scene" in case:
+        make_scene(case["scene"], las)
+    else:
+        make_las(las, case["preset"], case["points"], case["seed"], case["extra_bytes"])
     out = tmp_path / "out"
     oracle_convert(las, out, case["method"], encoding=case.get("encoding", "DEFAULT"),
                    random_seed=case.get("random_seed", 0))
