@@ -29,6 +29,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "../potreeconverter_amd/csrc/json_dump.h"
 #include "../potreeconverter_amd/csrc/lasfile.h"
 
 namespace fs = std::filesystem;
@@ -523,53 +524,50 @@ static std::string d17(double v) {
   ss << std::setprecision(std::numeric_limits<double>::max_digits10) << v;
   return ss.str();
 }
-static std::string jnum(double v) {  // JSON value for chunks/metadata.json (inf -> null like nlohmann)
-  if (std::isinf(v) || std::isnan(v)) return "null";
-  return d17(v);
-}
-
-// chunks/metadata.json (chunker_countsort_laszip.cpp:1005-1068).  The key
-// set is the reference's; number formatting is %.17g (round-trip exact),
-// non-finite values become null as nlohmann::json does.
+// chunks/metadata.json (chunker_countsort_laszip.cpp:1005-1068): the
+// document writeMetadata builds, laid out as nlohmann::json's dump(4) (keys
+// in lexicographic order, one array element per line, the library's number
+// form from json_dump.h, non-finite values as null); min/max/scale/offset
+// only for attributes of 1-3 elements, the histogram only when non-empty
+// for one-byte attributes.
 static void write_chunk_metadata(const std::string& path, V3 min, V3 max, const Attrs& a) {
-  std::ostringstream s;
-  s << "{\n    \"attributes\": [\n";
-  for (size_t i = 0; i < a.list.size(); i++) {
-    const Attr& t = a.list[i];
-    auto vec = [&](V3 v) {
-      std::string r = "[" + jnum(v.x);
-      if (t.numElements >= 2) r += ", " + jnum(v.y);
-      if (t.numElements >= 3) r += ", " + jnum(v.z);
-      return r + "]";
-    };
-    s << "        {\n";
-    s << "            \"description\": \"" << t.description << "\",\n";
-    s << "            \"elementSize\": " << t.elementSize << ",\n";
+  namespace J = pcg::jsondump;
+  auto vec = [](V3 v, int n, int indent) {
+    std::vector<std::string> e{J::number(v.x)};
+    if (n >= 2) e.push_back(J::number(v.y));
+    if (n >= 3) e.push_back(J::number(v.z));
+    return J::array(e, indent);
+  };
+  const std::string p(12, ' ');
+  std::vector<std::string> items;
+  for (const Attr& t : a.list) {
+    std::string o = "{\n" + p + "\"description\": " + J::string(t.description) + ",\n";
+    o += p + "\"elementSize\": " + std::to_string(t.elementSize) + ",\n";
     bool empty = true;
     for (auto h : t.histogram) if (h != 0) empty = false;
     if (t.size == 1 && !empty) {
-      s << "            \"histogram\": [";
-      for (size_t k = 0; k < t.histogram.size(); k++) s << (k ? ", " : "") << t.histogram[k];
-      s << "],\n";
+      std::vector<std::string> h;
+      for (auto c : t.histogram) h.push_back(std::to_string(c));
+      o += p + "\"histogram\": " + J::array(h, 12) + ",\n";
     }
-    s << "            \"max\": " << vec(t.max) << ",\n";
-    s << "            \"min\": " << vec(t.min) << ",\n";
-    s << "            \"name\": \"" << t.name << "\",\n";
-    s << "            \"numElements\": " << t.numElements << ",\n";
-    s << "            \"offset\": " << vec(t.offset) << ",\n";
-    s << "            \"scale\": " << vec(t.scale) << ",\n";
-    s << "            \"size\": " << t.size << ",\n";
-    s << "            \"type\": \"" << type_name(t.type) << "\"\n";
-    s << "        }" << (i + 1 < a.list.size() ? "," : "") << "\n";
+    const bool ranged = t.numElements >= 1 && t.numElements <= 3;
+    if (ranged) o += p + "\"max\": " + vec(t.max, t.numElements, 12) + ",\n" + p + "\"min\": " +
+                     vec(t.min, t.numElements, 12) + ",\n";
+    o += p + "\"name\": " + J::string(t.name) + ",\n";
+    o += p + "\"numElements\": " + std::to_string(t.numElements) + ",\n";
+    if (ranged) o += p + "\"offset\": " + vec(t.offset, t.numElements, 12) + ",\n" + p + "\"scale\": " +
+                     vec(t.scale, t.numElements, 12) + ",\n";
+    o += p + "\"size\": " + std::to_string(t.size) + ",\n";
+    o += p + "\"type\": " + J::string(type_name(t.type)) + "\n        }";
+    items.push_back(o);
   }
-  s << "    ],\n";
-  s << "    \"max\": [" << d17(max.x) << ", " << d17(max.y) << ", " << d17(max.z) << "],\n";
-  s << "    \"min\": [" << d17(min.x) << ", " << d17(min.y) << ", " << d17(min.z) << "],\n";
-  s << "    \"offset\": [" << d17(a.posOffset.x) << ", " << d17(a.posOffset.y) << ", " << d17(a.posOffset.z) << "],\n";
-  s << "    \"scale\": [" << d17(a.posScale.x) << ", " << d17(a.posScale.y) << ", " << d17(a.posScale.z) << "]\n";
-  s << "}";
+  std::string s = "{\n    \"attributes\": " + J::array(items, 4) + ",\n";
+  s += "    \"max\": " + vec(max, 3, 4) + ",\n";
+  s += "    \"min\": " + vec(min, 3, 4) + ",\n";
+  s += "    \"offset\": " + vec(a.posOffset, 3, 4) + ",\n";
+  s += "    \"scale\": " + vec(a.posScale, 3, 4) + "\n}";
   std::ofstream f(path, std::ios::binary);
-  f << s.str();
+  f << s;
 }
 
 struct ChunkerOut {

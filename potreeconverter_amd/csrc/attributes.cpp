@@ -1,5 +1,6 @@
 // Attribute schema, GPU transcode tables and chunk metadata text.
 #include "attributes.h"
+#include "json_dump.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -219,45 +220,53 @@ std::string fmt17(double v) {
   return buf;
 }
 
-static std::string jnum(double v) { return std::isfinite(v) ? fmt17(v) : "null"; }
-
+// chunks/metadata.json as the reference's writeMetadata writes it
+// (chunker_countsort_laszip.cpp:1005-1068): an nlohmann::json object dumped
+// with dump(4), so keys in lexicographic order, one array element per line,
+// numbers in the library's form (json_dump.h).  min/max/scale/offset of an
+// attribute exist only for 1-3 elements; the histogram only for one-byte
+// attributes that counted something.
 std::string chunk_metadata_json(const Vec3& min, const Vec3& max, const Attributes& a) {
-  std::ostringstream s;
-  s << "{\n    \"attributes\": [\n";
-  for (size_t i = 0; i < a.list.size(); i++) {
-    const Attribute& t = a.list[i];
-    auto vec = [&](const Vec3& v) {
-      std::string r = "[" + jnum(v.x);
-      if (t.numElements >= 2) r += ", " + jnum(v.y);
-      if (t.numElements >= 3) r += ", " + jnum(v.z);
-      return r + "]";
-    };
-    s << "        {\n";
-    s << "            \"description\": \"" << t.description << "\",\n";
-    s << "            \"elementSize\": " << t.elementSize << ",\n";
-    bool empty = std::all_of(t.histogram.begin(), t.histogram.end(), [](int64_t h) { return h == 0; });
+  namespace J = jsondump;
+  auto vec = [](const Vec3& v, int n, int indent) {
+    std::vector<std::string> e{J::number(v.x)};
+    if (n >= 2) e.push_back(J::number(v.y));
+    if (n >= 3) e.push_back(J::number(v.z));
+    return J::array(e, indent);
+  };
+  const std::string p(12, ' ');
+  std::vector<std::string> attrs;
+  for (const Attribute& t : a.list) {
+    std::string o = "{\n";
+    o += p + "\"description\": " + J::string(t.description) + ",\n";
+    o += p + "\"elementSize\": " + std::to_string(t.elementSize) + ",\n";
+    const bool empty = std::all_of(t.histogram.begin(), t.histogram.end(), [](int64_t h) { return h == 0; });
     if (t.size == 1 && !empty) {
-      s << "            \"histogram\": [";
-      for (size_t k = 0; k < t.histogram.size(); k++) s << (k ? ", " : "") << t.histogram[k];
-      s << "],\n";
+      std::vector<std::string> h;
+      for (int64_t c : t.histogram) h.push_back(std::to_string(c));
+      o += p + "\"histogram\": " + J::array(h, 12) + ",\n";
     }
-    s << "            \"max\": " << vec(t.max) << ",\n";
-    s << "            \"min\": " << vec(t.min) << ",\n";
-    s << "            \"name\": \"" << t.name << "\",\n";
-    s << "            \"numElements\": " << t.numElements << ",\n";
-    s << "            \"offset\": " << vec(t.offset) << ",\n";
-    s << "            \"scale\": " << vec(t.scale) << ",\n";
-    s << "            \"size\": " << t.size << ",\n";
-    s << "            \"type\": \"" << attribute_type_name(t.type) << "\"\n";
-    s << "        }" << (i + 1 < a.list.size() ? "," : "") << "\n";
+    const bool ranged = t.numElements >= 1 && t.numElements <= 3;
+    if (ranged) {
+      o += p + "\"max\": " + vec(t.max, t.numElements, 12) + ",\n";
+      o += p + "\"min\": " + vec(t.min, t.numElements, 12) + ",\n";
+    }
+    o += p + "\"name\": " + J::string(t.name) + ",\n";
+    o += p + "\"numElements\": " + std::to_string(t.numElements) + ",\n";
+    if (ranged) {
+      o += p + "\"offset\": " + vec(t.offset, t.numElements, 12) + ",\n";
+      o += p + "\"scale\": " + vec(t.scale, t.numElements, 12) + ",\n";
+    }
+    o += p + "\"size\": " + std::to_string(t.size) + ",\n";
+    o += p + "\"type\": " + J::string(attribute_type_name(t.type)) + "\n        }";
+    attrs.push_back(o);
   }
-  s << "    ],\n";
-  s << "    \"max\": [" << fmt17(max.x) << ", " << fmt17(max.y) << ", " << fmt17(max.z) << "],\n";
-  s << "    \"min\": [" << fmt17(min.x) << ", " << fmt17(min.y) << ", " << fmt17(min.z) << "],\n";
-  s << "    \"offset\": [" << fmt17(a.posOffset.x) << ", " << fmt17(a.posOffset.y) << ", " << fmt17(a.posOffset.z) << "],\n";
-  s << "    \"scale\": [" << fmt17(a.posScale.x) << ", " << fmt17(a.posScale.y) << ", " << fmt17(a.posScale.z) << "]\n";
-  s << "}";
-  return s.str();
+  std::string s = "{\n    \"attributes\": " + J::array(attrs, 4) + ",\n";
+  s += "    \"max\": " + vec(max, 3, 4) + ",\n";
+  s += "    \"min\": " + vec(min, 3, 4) + ",\n";
+  s += "    \"offset\": " + vec(a.posOffset, 3, 4) + ",\n";
+  s += "    \"scale\": " + vec(a.posScale, 3, 4) + "\n}";
+  return s;
 }
 
 // ---- minimal JSON reader for chunks/metadata.json ----
