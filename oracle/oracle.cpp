@@ -578,6 +578,9 @@ struct ChunkerOut {
   double tCount = 0, tDistribute = 0, tTotal = 0;
 };
 
+// grid size and chunk cap forced by orc_set_chunking (0: the reference's rule)
+static int64_t g_force_grid = 0, g_force_cap = 0;
+
 // doChunking (chunker_countsort_laszip.cpp:1206-1261) for one source with
 // the whole file in memory.  State.pointsTotal = header point count.
 static int do_chunking(const std::string& lasPath, ChunkerOut& co) {
@@ -594,8 +597,8 @@ static int do_chunking(const std::string& lasPath, ChunkerOut& co) {
   V3 max = min + statsCube;
   double cube = (max - min).max();
   int64_t N = (int64_t)h.num_points;
-  int64_t maxPointsPerChunk = std::min(N / 20, int64_t(10000000));
-  int64_t G = N < 100000000 ? 128 : (N < 500000000 ? 256 : 512);
+  int64_t maxPointsPerChunk = g_force_cap > 0 ? g_force_cap : std::min(N / 20, int64_t(10000000));
+  int64_t G = g_force_grid > 0 ? g_force_grid : (N < 100000000 ? 128 : (N < 500000000 ? 256 : 512));
   std::vector<uint8_t> las((size_t)N * h.record_length);
   if (!lasfile::read_points(lasPath, h, 0, N, las.data())) { g_error = "short read"; return 1; }
   V3 cubeMax = min + cube;
@@ -1936,6 +1939,12 @@ extern "C" int orc_decode_points(int32_t fmt, int32_t record_length, const uint8
 }
 
 extern "C" const char* orc_last_error(void) { return g_error.c_str(); }
+
+extern "C" int orc_set_chunking(int64_t grid, int64_t cap) {
+  g_force_grid = grid;
+  g_force_cap = cap;
+  return 0;
+}
 
 extern "C" int orc_set_random_seed(uint32_t seed) {
   g_random_seed = seed;

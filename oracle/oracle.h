@@ -68,6 +68,9 @@ const char* orc_last_error(void);
 // shuffle uses std::default_random_engine(seed), the value the reference
 // takes from system_clock::now() (sampler_random.h:102-104).
 int orc_set_random_seed(uint32_t seed);
+/* Forces the chunk grid size and chunk cap of the next conversions (0: the
+ * reference's rule from the point count). */
+int orc_set_chunking(int64_t grid, int64_t cap);
 
 #ifdef __cplusplus
 }

@@ -104,6 +104,22 @@ int pcg_set_option(const char* key, int64_t value) {
     g_sample_groups = (int32_t)value;
     return 0;
   }
+  if (k == "chunk_grid") {
+    if (value != 0 && value != 128 && value != 256 && value != 512) {
+      set_error("chunk_grid must be 0 (the reference's rule), 128, 256 or 512");
+      return 1;
+    }
+    g_chunk_grid = (int32_t)value;
+    return 0;
+  }
+  if (k == "chunk_cap") {
+    if (value < 0) {
+      set_error("chunk_cap must be 0 (the reference's rule) or positive");
+      return 1;
+    }
+    g_chunk_cap = value;
+    return 0;
+  }
   set_error("unknown option: " + k);
   return 1;
 }

@@ -30,6 +30,9 @@ Device& device() {
   return d;
 }
 
+int32_t g_chunk_grid = 0;
+int64_t g_chunk_cap = 0;
+
 int ensure_device() {
   static bool ready = false;
   if (ready) return 0;
@@ -166,9 +169,9 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
   out.bpp = A.bytes;
   out.min = hmin;
   out.max = hmin + cube;
-  // :1210-1220
-  int64_t cap = std::min(N / 20, (int64_t)10000000);
-  int32_t G = N < 100000000 ? 128 : (N < 500000000 ? 256 : 512);
+  // :1210-1220 (a test option may force both, as on a larger input)
+  int64_t cap = g_chunk_cap > 0 ? g_chunk_cap : std::min(N / 20, (int64_t)10000000);
+  int32_t G = g_chunk_grid > 0 ? g_chunk_grid : (N < 100000000 ? 128 : (N < 500000000 ? 256 : 512));
   int32_t Lmax = G == 128 ? 7 : (G == 256 ? 8 : 9);
   out.gridSize = G;
   out.maxPointsPerChunk = cap;

@@ -76,6 +76,13 @@ inline hipError_t stream_copy(void* dst, const void* src, size_t bytes, hipMemcp
   return hipStreamSynchronize(s);
 }
 
+// Chunk grid size and chunk point cap forced by pcg_set_option ("chunk_grid",
+// "chunk_cap"; 0 = the reference's rule from the point count,
+// chunker_countsort_laszip.cpp:1210-1220): lets a small input take the
+// G = 256 / 512 paths of a large one.
+extern int32_t g_chunk_grid;
+extern int64_t g_chunk_cap;
+
 // doChunking on LAS records already resident in HBM (header describes them).
 int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& out);
 

@@ -23,6 +23,7 @@ def oracle():
         O.orc_decode_points.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64,
                                         ctypes.c_void_p]
         O.orc_set_random_seed.argtypes = [ctypes.c_uint32]
+        O.orc_set_chunking.argtypes = [ctypes.c_int64, ctypes.c_int64]
         O.orc_convert_encoded.argtypes = [ctypes.c_char_p] * 4 + [ctypes.c_int32, ctypes.c_int32,
                                                                   ctypes.POINTER(ctypes.c_double)]
         O.orc_count_cells.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32] + \

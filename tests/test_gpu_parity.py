@@ -349,3 +349,36 @@ def test_edge_scene_matches_oracle(scene, method, tmp_path):
     if scene == "dups" and method == "poisson":
         # 14 999 + 10 000 duplicate records are dropped as the reference drops them
         assert os.path.getsize(gpu / "octree.bin") == 27 * (n - 24_999)
+
+
+GRID_CASES = [(256, 100_000), (512, 100_000), (256, 1_000_000)]
+
+
+@pytest.mark.parametrize("grid,cap", GRID_CASES, ids=[f"g{g}_cap{c}" for g, c in GRID_CASES])
+def test_forced_chunk_grid_matches_oracle(grid, cap, tmp_path):
+    """The chunk grids the reference picks for 100M-500M points (256) and
+    above (512), forced on 2M points of the bench's generator in both the
+    oracle and the library: chunk files, chunks/metadata.json and the octree
+    bytes.  (A cap at or above the point count is no configuration of the
+    reference: its LUT then holds no chunk.)"""
+    print(f"forced grid case G={grid} cap={cap}", flush=True)
+    las = tmp_path / "in.las"
+    make_las(las, 1, 2_000_000, 0, False)
+    cpu, gpu = tmp_path / "cpu", tmp_path / "gpu"
+    oracle().orc_set_chunking(grid, cap)
+    try:
+        oracle_convert(las, cpu, "poisson", keep_chunks=True)
+    finally:
+        oracle().orc_set_chunking(0, 0)
+    _set("chunk_grid", grid)
+    _set("chunk_cap", cap)
+    try:
+        pc.convert(las, gpu, "poisson", keep_chunks=True)
+    finally:
+        _set("chunk_grid", 0)
+        _set("chunk_cap", 0)
+    res = same_files(cpu, gpu, ["octree.bin", "hierarchy.bin", "metadata.json"])
+    assert all(res.values()), res
+    a, b = sorted(os.listdir(cpu / "chunks")), sorted(os.listdir(gpu / "chunks"))
+    assert a == b
+    assert all(same_files(cpu / "chunks", gpu / "chunks", a).values())
