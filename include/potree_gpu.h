@@ -95,6 +95,19 @@ int pcg_count_cells(const void* d_las, int64_t n, int32_t bpp, const double las_
                     const double las_offset[3], const double pos_scale[3], const double pos_offset[3],
                     const double min[3], const double max[3], int32_t grid_size, int32_t* d_grid, void* stream);
 
+/* pcg_count_cells without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, hosts that forbid hipMalloc).  The pass's
+ * scratch (statistics, histogram, error flags) is a fixed 4 KiB device array
+ * per GPU, held until the process ends, whatever n is; d_grid is cleared by a
+ * kernel, not hipMemsetAsync.  Rejects bpp outside 12..128 and grid_size
+ * outside 1..512.  Calls that overlap on one GPU share the array and race.
+ * Same results and return codes as pcg_count_cells. */
+int pcg_count_cells_noalloc(const void* d_las, int64_t n, int32_t bpp, const double las_scale[3],
+                            const double las_offset[3], const double pos_scale[3], const double pos_offset[3],
+                            const double min[3], const double max[3], int32_t grid_size, int32_t* d_grid,
+                            void* stream);
+
 /* Device-resident conversion for benchmarks and servers: the LAS records
  * are already in HBM (d_las, n records, header fields in `las_header`, a
  * 375-byte LAS 1.4 header image).  Produces the octree in HBM and, when

@@ -37,6 +37,7 @@ def lib():
         L.pcg_do_indexing.argtypes = [c.c_char_p, c.c_char_p, c.c_char_p, c.c_int64, c.POINTER(c.c_double)]
         L.pcg_count_cells.argtypes = [c.c_void_p, c.c_int64, c.c_int32] + [c.POINTER(c.c_double)] * 6 + [
             c.c_int32, c.c_void_p, c.c_void_p]
+        L.pcg_count_cells_noalloc.argtypes = L.pcg_count_cells.argtypes
         L.pcg_convert_device.argtypes = [c.c_void_p, c.c_int64, c.c_void_p, c.c_char_p, c.c_char_p,
                                          c.POINTER(c.c_double)]
         L.pcg_convert_encoded.argtypes = [c.c_char_p, c.c_char_p, c.c_char_p, c.c_char_p, c.c_int32,
@@ -60,7 +61,7 @@ def lib():
         L.pcg_lasgen_header.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p]
         L.pcg_release.argtypes = []
         L.pcg_set_option.argtypes = [c.c_char_p, c.c_int64]
-        for name in ("pcg_set_option", "pcg_release", "pcg_convert", "pcg_do_chunking", "pcg_do_indexing", "pcg_count_cells", "pcg_convert_device",
+        for name in ("pcg_set_option", "pcg_release", "pcg_convert", "pcg_do_chunking", "pcg_do_indexing", "pcg_count_cells", "pcg_count_cells_noalloc", "pcg_convert_device",
                      "pcg_convert_encoded", "pcg_do_indexing_encoded", "pcg_convert_device_encoded",
                      "pcg_device_alloc", "pcg_device_free", "pcg_copy", "pcg_synchronize", "pcg_lasgen_preset",
                      "pcg_lasgen_configure",

@@ -18,7 +18,11 @@ static_assert(sizeof(LasGenParams) <= 256, "LasGenParams must fit the caller's 2
 
 extern "C" {
 
-int pcg_init(void) { return ensure_device(); }
+int pcg_init(void) {
+  if (ensure_device()) return 1;
+  uint8_t* scr = nullptr;  // the no-allocation scratch's address on this device
+  return noalloc_scratch(&scr);
+}
 
 const char* pcg_last_error(void) { return last_error().c_str(); }
 
@@ -255,6 +259,47 @@ int pcg_copy(void* dst, const void* src, uint64_t bytes, int32_t kind) {
 int pcg_synchronize(void) {
   if (ensure_device()) return 1;
   PCG_CHECK(hipDeviceSynchronize());
+  return 0;
+}
+
+// chunker parameters of a counting-only pass (no attribute statistics)
+static int count_params(int64_t n, int32_t bpp, const double las_scale[3], const double las_offset[3],
+                        const double pos_scale[3], const double pos_offset[3], const double min[3],
+                        const double max[3], int32_t grid_size, hipStream_t s) {
+  if (n < 0 || bpp < 12 || bpp > 128 || grid_size < 1 || grid_size > 512) {
+    set_error("count_cells: need n >= 0, 12 <= bpp <= 128 and 1 <= grid_size <= 512");
+    return 1;
+  }
+  ChunkerParams P;
+  memset(&P, 0, sizeof(P));
+  Vec3 mn(min[0], min[1], min[2]), mx(max[0], max[1], max[2]);
+  for (int a = 0; a < 3; a++) {
+    P.las_scale[a] = las_scale[a]; P.las_offset[a] = las_offset[a];
+    P.pos_scale[a] = pos_scale[a]; P.pos_offset[a] = pos_offset[a];
+    P.min[a] = min[a];
+  }
+  P.cube = (mx - mn).max();
+  P.grid_size = grid_size;
+  P.bpp_in = bpp;
+  P.class_src = -1;
+  P.point_format = -1;  // counts only: no attribute statistics
+  return chunker_set_params(P, s);
+}
+
+int pcg_count_cells_noalloc(const void* d_las, int64_t n, int32_t bpp, const double las_scale[3],
+                            const double las_offset[3], const double pos_scale[3], const double pos_offset[3],
+                            const double min[3], const double max[3], int32_t grid_size, int32_t* d_grid,
+                            void* stream) {
+  if (ensure_device()) return 1;
+  hipStream_t s = stream ? (hipStream_t)stream : device().stream;
+  if (count_params(n, bpp, las_scale, las_offset, pos_scale, pos_offset, min, max, grid_size, s)) return 1;
+  const int64_t cells = (int64_t)grid_size * grid_size * grid_size;
+  int32_t herr = 0;
+  if (chunker_count_noalloc((const uint8_t*)d_las, n, d_grid, cells, device().num_cus, s, &herr)) return 1;
+  if (herr & 1) {
+    set_error("encountered point outside bounding box.");
+    return 123;
+  }
   return 0;
 }
 

@@ -11,6 +11,15 @@ namespace pcg {
 int chunker_set_params(const ChunkerParams& p, hipStream_t stream);
 // d_cells (n entries, may be null) receives the grid cell of every point
 // (0xFFFFFFFF outside the box)
+// Counting pass without any allocation: k_count's statistics, histogram and
+// error flags live in a fixed device array (4 KiB per GPU, whatever n is),
+// cleared by fill kernels before the pass; *h_err receives the error flags.
+// Calls that overlap on one GPU share that array and race.
+int chunker_count_noalloc(const uint8_t* d_las, int64_t n, int32_t* d_grid, int64_t cells, int num_cus,
+                          hipStream_t stream, int32_t* h_err);
+// Device address of that array on the current device (looked up once per device).
+int noalloc_scratch(uint8_t** out);
+
 int chunker_count(const uint8_t* d_las, int64_t n, int32_t* d_grid, uint32_t* d_cells, uint64_t* d_stat_min,
                   uint64_t* d_stat_max, unsigned long long* d_class_hist, int32_t* d_err, int num_cus,
                   hipStream_t stream);

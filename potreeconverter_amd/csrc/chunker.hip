@@ -12,6 +12,8 @@
 //   k_scatter         :770-904  transcode + stable scatter
 // All geometry is IEEE binary64 with the reference's evaluation order; the
 // TU is compiled with -ffp-contract=off.
+#include <mutex>
+
 #include "chunker.h"
 #include "lasgen.h"
 #include "pcg_common.h"
@@ -1012,7 +1014,78 @@ int tail_layout(const ChunkerParams& p) {
   return next == p.bpp_in ? layout : -1;
 }
 
+// Fills n 32-bit words with v (a kernel rather than hipMemsetAsync, so that
+// callers may capture the no-allocation entry point in a graph).
+__global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Scratch of the no-allocation counting entry point: k_count's statistics
+// minima and maxima (kMaxStats each), the classification histogram (256)
+// and the error flags.  Every GPU the library runs on holds one copy until
+// the process ends.  Host code never passes this name to a kernel: it is a
+// host-side stand-in, and its device address comes from
+// hipGetSymbolAddress (noalloc_scratch below).
+constexpr size_t kNoAllocStatMin = 0;
+constexpr size_t kNoAllocStatMax = kNoAllocStatMin + 8 * kMaxStats;
+constexpr size_t kNoAllocHist = kNoAllocStatMax + 8 * kMaxStats;
+constexpr size_t kNoAllocErr = kNoAllocHist + 8 * 256;
+constexpr size_t kNoAllocBytes = 4096;
+static_assert(kNoAllocErr + 16 <= kNoAllocBytes, "no-allocation scratch too small");
+__device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+uint8_t* g_scratch_ptr[kMaxDevices];
+std::mutex g_scratch_mutex;
+
+void fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+}
+
 }  // namespace
+
+int noalloc_scratch(uint8_t** out) {
+  int dev = 0;
+  PCG_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device index outside the no-allocation scratch table");
+    return 1;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    PCG_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = (uint8_t*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+int chunker_count_noalloc(const uint8_t* d_las, int64_t n, int32_t* d_grid, int64_t cells, int num_cus,
+                          hipStream_t stream, int32_t* h_err) {
+  uint8_t* scr = nullptr;
+  if (noalloc_scratch(&scr)) return 1;
+  uint64_t* smin = (uint64_t*)(scr + kNoAllocStatMin);
+  uint64_t* smax = (uint64_t*)(scr + kNoAllocStatMax);
+  unsigned long long* hist = (unsigned long long*)(scr + kNoAllocHist);
+  int32_t* err = (int32_t*)(scr + kNoAllocErr);
+  // the previous call's contents stay in the scratch: reset what starts at
+  // a neutral value, and the caller's grid
+  fill_u32((uint32_t*)smin, 2 * kMaxStats, 0xFFFFFFFFu, stream);
+  fill_u32((uint32_t*)smax, 2 * kMaxStats, 0u, stream);
+  fill_u32((uint32_t*)hist, 2 * 256, 0u, stream);
+  fill_u32((uint32_t*)err, 4, 0u, stream);
+  fill_u32((uint32_t*)d_grid, cells, 0u, stream);
+  PCG_CHECK(hipGetLastError());
+  if (chunker_count(d_las, n, d_grid, nullptr, smin, smax, hist, err, num_cus, stream)) return 1;
+  PCG_CHECK(hipMemcpyAsync(h_err, err, 4, hipMemcpyDeviceToHost, stream));
+  PCG_CHECK(hipStreamSynchronize(stream));
+  return 0;
+}
 
 int chunker_set_params(const ChunkerParams& p, hipStream_t stream) {
   if (p.bpp_in > kMaxBppIn) {
