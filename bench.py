@@ -246,7 +246,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "chunking pass (k_count + k_tile_hist + k_scatter_tail)",
+            "kernel": "chunking pass (k_count + k_tile_hist + scan + k_distribute)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -108,6 +108,10 @@ int pcg_set_option(const char* key, int64_t value) {
     g_sample_groups = (int32_t)value;
     return 0;
   }
+  if (k == "distribute_tail") {
+    g_distribute_tail = value != 0;
+    return 0;
+  }
   if (k == "chunk_grid") {
     if (value != 0 && value != 128 && value != 256 && value != 512) {
       set_error("chunk_grid must be 0 (the reference's rule), 128, 256 or 512");

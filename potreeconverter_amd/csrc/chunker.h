@@ -34,6 +34,16 @@ int chunker_lut_fill(const int64_t* d_chunks, int64_t num_chunks, int32_t level_
 int chunker_distribute(const uint8_t* d_las, uint32_t* d_ids, int64_t n, const int32_t* d_lut, int32_t num_chunks,
                        int32_t tile_size, uint32_t* d_M, void* d_scan_ws, uint8_t* d_out, int32_t* d_err,
                        hipStream_t stream);
+// Distribute for the record layouts of point formats 0-3 and 6-7 (records
+// up to 64 bytes, fewer than 65535 chunks): the tile histogram writes every
+// record's chunk as 16 bits (d_ids, n entries) and the tile matrix, the scan
+// gives every (chunk, tile) its first record, and the tile transcode writes
+// the records with aligned 16-byte stores.  Tiles of 4096 records; d_M and
+// d_scan_ws as for chunker_distribute.
+bool distribute_tail_fits(int32_t num_chunks);
+int chunker_distribute_tail(const uint8_t* d_las, const uint32_t* d_cells, uint16_t* d_ids, int64_t n,
+                            const int32_t* d_lut, int32_t num_chunks, uint32_t* d_M, void* d_scan_ws,
+                            uint8_t* d_out, int32_t* d_err, hipStream_t stream);
 int lasgen_device(const LasGenParams& p, int64_t first, int64_t count, uint8_t* d_out, int32_t* d_qminmax,
                   hipStream_t stream);
 

@@ -9,7 +9,9 @@
 //   k_chunk_cells     :1144-1172 cells that become chunks
 //   k_lut_fill        :1187-1199 LUT fill
 //   k_tile_hist       :817-876  chunk of every point (LUT), per-tile counts
-//   k_scatter         :770-904  transcode + stable scatter
+//   k_distribute      :770-904  transcode + stable scatter (point formats 0-3,
+//                               6-7; runs as aligned 16-byte stores)
+//   k_scatter         :770-904  the same for any record layout, byte-wise
 // All geometry is IEEE binary64 with the reference's evaluation order; the
 // TU is compiled with -ffp-contract=off.
 #include <mutex>
@@ -371,7 +373,9 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ la
   for (int i = threadIdx.x; i < kHashSize; i += kBlock) { s_key[i] = kEmpty; s_cnt[i] = 0; }
   for (int i = threadIdx.x; i < 256; i += kBlock) s_hist[i] = 0;
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the wave index is uniform: readfirstlane keeps it (and every range and
+  // address derived from it) in scalar registers
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int bpp = P.bpp_in;
   const uint64_t total = (uint64_t)n * bpp;
   uint8_t* rec_area = s_dyn + wave * record_area_bytes(bpp);
@@ -557,7 +561,11 @@ __global__ void k_lut_fill(const int64_t* __restrict__ chunks, int64_t num_chunk
 // chunk-major: M[c * tiles + t].  Each thread takes 4 consecutive points per
 // pass (tile_size is a multiple of 4 * kBlock, so tiles start 16-byte
 // aligned) and adds runs of equal chunks with one LDS atomic.
-__global__ __launch_bounds__(kBlock) void k_tile_hist(uint32_t* __restrict__ ids, int64_t n, int32_t tile_size,
+// IdT = uint32_t writes the chunks over the cells (ids_out == cells);
+// uint16_t writes them to a separate array (kEmpty becomes 0xFFFF).
+template <typename IdT>
+__global__ __launch_bounds__(kBlock) void k_tile_hist(const uint32_t* cells, IdT* ids_out,
+                                                       int64_t n, int32_t tile_size,
                                                        const int32_t* __restrict__ lut, int32_t num_chunks,
                                                        int64_t tiles, uint32_t* __restrict__ M,
                                                        int32_t* __restrict__ err) {
@@ -574,10 +582,10 @@ __global__ __launch_bounds__(kBlock) void k_tile_hist(uint32_t* __restrict__ ids
     const int m = (t1 - i) < 4 ? (int)(t1 - i) : 4;
     uint32_t v[4];
     if (m == 4) {
-      const uint4 q = *(const uint4*)(ids + i);
+      const uint4 q = *(const uint4*)(cells + i);
       v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
     } else {
-      for (int j = 0; j < 4; j++) v[j] = j < m ? ids[i + j] : kEmpty;
+      for (int j = 0; j < 4; j++) v[j] = j < m ? cells[i + j] : kEmpty;
     }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -586,10 +594,18 @@ __global__ __launch_bounds__(kBlock) void k_tile_hist(uint32_t* __restrict__ ids
       if (j < m && c < 0) bad = true;
       v[j] = c < 0 ? kEmpty : (uint32_t)c;
     }
-    if (m == 4) {
-      *(uint4*)(ids + i) = make_uint4(v[0], v[1], v[2], v[3]);
+    if constexpr (sizeof(IdT) == 4) {
+      if (m == 4) {
+        *(uint4*)(ids_out + i) = make_uint4(v[0], v[1], v[2], v[3]);
+      } else {
+        for (int j = 0; j < m; j++) ids_out[i + j] = v[j];
+      }
     } else {
-      for (int j = 0; j < m; j++) ids[i + j] = v[j];
+      if (m == 4) {
+        *(uint2*)(ids_out + i) = make_uint2((v[0] & 0xFFFFu) | (v[1] << 16), (v[2] & 0xFFFFu) | (v[3] << 16));
+      } else {
+        for (int j = 0; j < m; j++) ids_out[i + j] = (IdT)v[j];
+      }
     }
     uint32_t run = 0;
 #pragma unroll
@@ -653,7 +669,7 @@ __device__ inline void write_runs(uint64_t leaders, uint32_t base, uint32_t len,
 // 64 output records of a step are assembled in LDS grouped into runs of one
 // chunk (a run is contiguous in the output) and written by write_runs.
 // This kernel transcodes with the byte operations of ChunkerParams::ops (any
-// record layout); k_scatter_tail takes the layouts of point formats 0-3 and
+// record layout); k_distribute takes the layouts of point formats 0-3 and
 // 6-7.
 __global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ las,
                                                      const uint32_t* __restrict__ ids, int64_t n,
@@ -791,122 +807,228 @@ __device__ inline uint4 load_vec16(const uint8_t* __restrict__ src, uint64_t tot
 // bytes each (a step starts anywhere in a 16-byte vector)
 __host__ __device__ constexpr int prefetch_vecs(int bpp) { return (64 * bpp + 30) / (16 * 64) + 1; }
 
+// ------------------------------------------------------ tile transcode
+// Transcode + scatter for the record layouts of point formats 0-3 and 6-7
+// (chunker_countsort_laszip.cpp:838-904: records grouped by chunk, input
+// order inside a chunk), after k_tile_hist<uint16_t> wrote every record's
+// chunk and the scan gave every (chunk, tile) its first output record.
+// Each wave counts its quarter of the tile per chunk, then transcodes it 64
+// records per step.  The records of one chunk in a step form one run of the
+// output; a run is staged in LDS at the 16-byte phase of its destination
+// and leaves as aligned 16-byte stores (aligned pieces of 8, 4, 2, 1 bytes
+// at its two ends).
+//
+// gfx950 counts loads and stores of a wave on one in-order counter (vmcnt),
+// so waiting for a load also waits for every store issued before it.  A
+// step's stores are therefore issued at the start of the next iteration,
+// ahead of that iteration's prefetch, and the wait for the prefetch comes
+// after a whole transcode: both have that long to complete.
+constexpr int kDistTile = 4096;
+constexpr int kDistPerWave = kDistTile / kWavesPerBlock;  // 1024 records
+constexpr int kDistSteps = kDistPerWave / 64;              // 16 steps
+constexpr uint32_t kNoChunk16 = 0xFFFFu;
+
+__host__ __device__ inline size_t align16z(size_t v) { return (v + 15) & ~(size_t)15; }
+
+// one wave's output staging: 64 records, each run shifted to the phase of
+// its destination (at most 15 bytes of padding per run), the dword after
+// the last record
+__host__ __device__ inline int stage_area_bytes(int bpo) { return (64 * bpo + 16 * 64 + 32 + 15) & ~15; }
+
+// dynamic LDS of k_distribute: per-wave chunk bases, per-wave run table,
+// the tile's chunk ids, per wave the record buffer and the output staging
+__host__ __device__ inline size_t distribute_lds_bytes(int num_chunks, int bpp, int bpo) {
+  return align16z((size_t)kWavesPerBlock * num_chunks * 4) + (size_t)kWavesPerBlock * 64 * 16 +
+         (size_t)kDistTile * 2 + (size_t)kWavesPerBlock * (record_area_bytes(bpp) + stage_area_bytes(bpo));
+}
+
 template <bool EXT, int PV>
-__global__ __launch_bounds__(kBlock) void k_scatter_tail(const uint8_t* __restrict__ las,
-                                                          const uint32_t* __restrict__ ids, int64_t n,
-                                                          int32_t tile_size, int32_t num_chunks, int32_t key_bits,
-                                                          int64_t tiles, const uint32_t* __restrict__ M,
-                                                          uint8_t* __restrict__ out) {
+__global__ __launch_bounds__(kBlock) void k_distribute(const uint8_t* __restrict__ las,
+                                                        const uint16_t* __restrict__ ids, int64_t n,
+                                                        int32_t num_chunks, int64_t tiles,
+                                                        const uint32_t* __restrict__ M,
+                                                        uint8_t* __restrict__ out) {
   extern __shared__ __attribute__((aligned(16))) uint8_t s_dyn[];
   const ChunkerParams& P = c_params;
   const int bpp = P.bpp_in, bpo = P.bpp_out;  // bpo = bpp + 1
-  uint32_t* s_cnt = (uint32_t*)s_dyn;  // [kWavesPerBlock][num_chunks]
-  uint8_t* s_stage = s_dyn + (((size_t)kWavesPerBlock * num_chunks * 4 + 15) & ~(size_t)15);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  uint32_t* s_base = (uint32_t*)s_dyn;  // [wave][num_chunks]: counts, then next output record
+  uint4* s_runs = (uint4*)(s_dyn + align16z((size_t)kWavesPerBlock * num_chunks * 4));  // [wave][64]
+  uint16_t* s_ids = (uint16_t*)(s_runs + kWavesPerBlock * 64);  // [kDistTile]: chunk of each record
+  uint8_t* s_stage = (uint8_t*)(s_ids + kDistTile);
+  const int rab = record_area_bytes(bpp);
+  uint8_t* rec_area = s_stage + (size_t)wave * (rab + stage_area_bytes(bpo));
+  uint8_t* out_area = rec_area + rab;
+  for (int i = tid; i < kWavesPerBlock * num_chunks; i += kBlock) s_base[i] = 0;
   const int64_t t = blockIdx.x;
-  for (int i = threadIdx.x; i < kWavesPerBlock * num_chunks; i += kBlock) s_cnt[i] = 0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint8_t* rec_area = s_stage + (size_t)wave * (record_area_bytes(bpp) + out_area_bytes(bpo));
-  uint8_t* out_area = rec_area + record_area_bytes(bpp);
-  const uint64_t total = (uint64_t)n * bpp;
-  const int64_t t0 = t * tile_size;
-  const int64_t t1 = (t0 + tile_size) < n ? (t0 + tile_size) : n;
-  const int64_t per_wave = (tile_size + kWavesPerBlock - 1) / kWavesPerBlock;
-  const int64_t w0 = t0 + wave * per_wave;
-  const int64_t w1 = (w0 + per_wave) < t1 ? (w0 + per_wave) : t1;
-  uint32_t* my_cnt = s_cnt + wave * num_chunks;
+  const int64_t t0 = t * kDistTile;
+  const int64_t t1 = (t0 + kDistTile) < n ? (t0 + kDistTile) : n;
+  const int64_t w0 = (t0 + wave * kDistPerWave) < t1 ? (t0 + wave * kDistPerWave) : t1;
+  const int64_t w1 = (w0 + kDistPerWave) < t1 ? (w0 + kDistPerWave) : t1;
+  const int steps = (int)((w1 - w0 + 63) / 64);
+  uint32_t* my_base = s_base + wave * num_chunks;
+  uint4* my_runs = s_runs + wave * 64;
+  uint16_t* my_ids = s_ids + wave * kDistPerWave;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (int64_t first = w0; first < w1; first += 64) {
-    int cnt = (int)((w1 - first) < 64 ? (w1 - first) : 64);
-    uint32_t c = 0;
-    bool valid = lane < cnt;
-    if (valid) {
-      const uint32_t cc = ids[first + lane];
-      valid = cc < (uint32_t)num_chunks;
-      c = valid ? cc : 0u;
-    }
-    uint64_t peers;
-    uint32_t run;
-    step_runs(c, valid, key_bits, lt_mask, peers, run);
-    if (valid && (peers & lt_mask) == 0) my_cnt[c] += (uint32_t)__popcll(peers);
-  }
-  __syncthreads();
-  for (int c = threadIdx.x; c < num_chunks; c += kBlock) {
-    uint32_t run = M[(int64_t)c * tiles + t];
-    for (int w = 0; w < kWavesPerBlock; w++) {
-      uint32_t v = s_cnt[w * num_chunks + c];
-      s_cnt[w * num_chunks + c] = run;
-      run += v;
-    }
-  }
-  __syncthreads();
-  const int out_vecs = (64 * bpo + 15) / 16;
-  const int ow = (bpo + 3) / 4;        // output dwords of a record
-  const int inner = (bpo - 4) / 4;     // dwords 1 .. inner lie inside the record at any alignment
-  const uint32_t last_mask = (bpo & 3) ? ((1u << (8 * (bpo & 3))) - 1u) : 0xFFFFFFFFu;
-  // pass 2: the next step's records and chunk ids are loaded into
-  // registers while the current step is transcoded
+  const uint64_t total = (uint64_t)n * bpp;
+
+  // a step's records as 16-byte vectors, without a branch per lane (lanes
+  // past the step reload its last vector) except in the one step that
+  // reaches the end of the input
   uint4 pf[PV];
-  uint32_t id_next = kEmpty;
+  int pnvec = 0;  // vectors of the step in pf
   auto issue = [&](int64_t f) {
     const int m = (int)((w1 - f) < 64 ? (w1 - f) : 64);
     const uint64_t b0 = (uint64_t)f * bpp, a0 = b0 & ~15ull;
     const int nvec = (int)((b0 + (uint64_t)m * bpp - a0 + 15) >> 4);
-#pragma unroll
-    for (int j = 0; j < PV; j++) {
-      const int v = lane + 64 * j;
-      if (v < nvec) pf[j] = load_vec16(las, total, a0 + 16ull * v);
-    }
-    id_next = lane < m ? ids[f + lane] : kEmpty;
-  };
-  if (w0 < w1) issue(w0);
-  for (int64_t first = w0; first < w1; first += 64) {
-    int cnt = (int)((w1 - first) < 64 ? (w1 - first) : 64);
-    uint32_t shift;
-    {
-      const uint64_t b0 = (uint64_t)first * bpp, a0 = b0 & ~15ull;
-      const int nvec = (int)((b0 + (uint64_t)cnt * bpp - a0 + 15) >> 4);
+    pnvec = nvec;
+    if (a0 + 16ull * (uint64_t)nvec <= total) {
 #pragma unroll
       for (int j = 0; j < PV; j++) {
         const int v = lane + 64 * j;
-        if (v < nvec) *(uint4*)(rec_area + 16 * v) = pf[j];
+        pf[j] = *(const uint4*)(las + a0 + 16ull * (uint64_t)(v < nvec ? v : nvec - 1));
       }
-      shift = (uint32_t)(b0 - a0);
+    } else {
+#pragma unroll
+      for (int j = 0; j < PV; j++) {
+        const int v = lane + 64 * j;
+        pf[j] = v < nvec ? load_vec16(las, total, a0 + 16ull * v) : make_uint4(0u, 0u, 0u, 0u);
+      }
     }
-    const uint32_t cc = id_next;
-    if (first + 64 < w1) issue(first + 64);
-    for (int i = lane; i < out_vecs; i += 64) ((uint4*)out_area)[i] = make_uint4(0u, 0u, 0u, 0u);
+  };
+  // the step's vectors only: the record area holds record_area_bytes(bpp)
+  auto put = [&]() {
+#pragma unroll
+    for (int j = 0; j < PV; j++) {
+      const int v = lane + 64 * j;
+      if (v < pnvec) *(uint4*)(rec_area + 16 * v) = pf[j];
+    }
+  };
+  // the step's runs (staged in out_area, listed in my_runs) to the output
+  auto write_runs = [&](int nr) {
+    for (int r = 0; r < nr; r++) {
+      const uint4 R = my_runs[r];
+      const uint32_t ph = R.z & 15u;  // == R.x & 15
+      const uint8_t* sv = out_area + (R.x - ph);
+      uint8_t* gv = out + ((((uint64_t)R.w << 32) | R.z) - ph);
+      const uint32_t span = ph + R.y;
+      const uint32_t nvec = (span + 15) >> 4;
+      for (uint32_t k = lane; k < nvec; k += 64) {
+        const uint32_t b0 = k << 4;
+        if (b0 >= ph && b0 + 16 <= span) {
+          *(uint4*)(gv + b0) = *(const uint4*)(sv + b0);
+        } else {
+          // the run's bytes [lo, hi) of an end vector in naturally aligned
+          // pieces: up to 8-byte alignment, then descending sizes
+          uint32_t lo = b0 < ph ? ph : b0;
+          const uint32_t hi = b0 + 16 < span ? b0 + 16 : span;
+          if ((lo & 1) && lo < hi) { gv[lo] = sv[lo]; lo += 1; }
+          if ((lo & 2) && lo + 2 <= hi) { *(uint16_t*)(gv + lo) = *(const uint16_t*)(sv + lo); lo += 2; }
+          if ((lo & 4) && lo + 4 <= hi) { *(uint32_t*)(gv + lo) = *(const uint32_t*)(sv + lo); lo += 4; }
+          if ((lo & 8) && lo + 8 <= hi) { *(uint2*)(gv + lo) = *(const uint2*)(sv + lo); lo += 8; }
+          if (lo + 8 <= hi) { *(uint2*)(gv + lo) = *(const uint2*)(sv + lo); lo += 8; }
+          if (lo + 4 <= hi) { *(uint32_t*)(gv + lo) = *(const uint32_t*)(sv + lo); lo += 4; }
+          if (lo + 2 <= hi) { *(uint16_t*)(gv + lo) = *(const uint16_t*)(sv + lo); lo += 2; }
+          if (lo < hi) gv[lo] = sv[lo];
+        }
+      }
+    }
+  };
+
+  // the wave's chunk ids into LDS (read there by the transcode: a global
+  // load inside a step would wait for the step's prefetch and stores), the
+  // first step's records, the counts per chunk (one round per chunk)
+  {
+    uint32_t cv[kDistSteps];
+#pragma unroll
+    for (int j = 0; j < kDistSteps; j++) {
+      const int64_t i = w0 + 64 * j + lane;
+      cv[j] = i < w1 ? ids[i] : kNoChunk16;
+    }
+#pragma unroll
+    for (int j = 0; j < kDistSteps; j++) my_ids[64 * j + lane] = (uint16_t)cv[j];
+  }
+  if (steps > 0) {
+    issue(w0);
+    put();
+  }
+  __syncthreads();
+  for (int st = 0; st < steps; st++) {
+    const uint32_t c = my_ids[64 * st + lane];
+    for (uint64_t left = __ballot(c != kNoChunk16); left;) {
+      const uint32_t v = __builtin_amdgcn_readlane(c, __builtin_ctzll(left));
+      const uint64_t m = __ballot(c == v) & left;
+      if (lane == 0) my_base[v] += (uint32_t)__popcll(m);
+      left &= ~m;
+    }
+  }
+  __syncthreads();
+  // first output record of every chunk for each wave: the tile's (scanned
+  // matrix) plus the earlier waves' counts
+  for (int c = tid; c < num_chunks; c += kBlock) {
+    uint32_t run = M[(int64_t)c * tiles + t];
+#pragma unroll
+    for (int q = 0; q < kWavesPerBlock; q++) {
+      const uint32_t v = s_base[q * num_chunks + c];
+      s_base[q * num_chunks + c] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+
+  const int ow = (bpo + 3) / 4;     // output dwords of a record
+  const int inner = (bpo - 4) / 4;  // dwords 1 .. inner lie inside the record at any alignment
+  const uint32_t last_mask = (bpo & 3) ? ((1u << (8 * (bpo & 3))) - 1u) : 0xFFFFFFFFu;
+  int nr_prev = 0;  // runs of the previous step, still to be written
+  for (int st = 0; st < steps; st++) {
+    const int64_t first = w0 + 64 * (int64_t)st;
+    const int cnt = (int)((w1 - first) < 64 ? (w1 - first) : 64);
+    // the previous step's stores, then the next step's records
+    write_runs(nr_prev);
+    if (st + 1 < steps) issue(first + 64);
+    const uint32_t shift = (uint32_t)(((uint64_t)first * bpp) & 15u);
+    const uint32_t c = lane < cnt ? (uint32_t)my_ids[64 * st + lane] : kNoChunk16;
+    const bool valid = c != kNoChunk16;
+    // runs in the order of their lowest lane, each staged at the 16-byte
+    // phase of its first output byte
+    uint32_t sofs = 0, my_so = 0;
+    int nr = 0;
+    for (uint64_t left = __ballot(valid); left; nr++) {
+      const uint32_t v = __builtin_amdgcn_readlane(c, __builtin_ctzll(left));
+      const uint64_t m = __ballot(c == v) & left;
+      const uint32_t len = (uint32_t)__popcll(m);
+      const uint32_t b = my_base[v];
+      const uint64_t g0 = (uint64_t)b * (uint32_t)bpo;
+      sofs += (uint32_t)(g0 - sofs) & 15u;
+      if ((m >> lane) & 1) my_so = sofs + (uint32_t)__popcll(m & lt_mask) * (uint32_t)bpo;
+      if (lane == 0) {
+        my_base[v] = b + len;
+        my_runs[nr] = make_uint4(sofs, len * (uint32_t)bpo, (uint32_t)g0, (uint32_t)(g0 >> 32));
+      }
+      sofs += len * (uint32_t)bpo;
+      left &= ~m;
+    }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    bool valid = lane < cnt;
-    uint32_t c = 0;
-    if (valid) {
-      valid = cc < (uint32_t)num_chunks;
-      c = valid ? cc : 0u;
-    }
-    uint64_t peers;
-    uint32_t my_run;
-    step_runs(c, valid, key_bits, lt_mask, peers, my_run);
-    const uint32_t base = valid ? my_cnt[c] : 0;
+    for (uint32_t i = lane; i < (sofs + 8 + 15) / 16; i += 64) ((uint4*)out_area)[i] = make_uint4(0u, 0u, 0u, 0u);
     __builtin_amdgcn_wave_barrier();
-    const bool leader = valid && (peers & lt_mask) == 0;
-    const uint32_t len = leader ? (uint32_t)__popcll(peers) : 0u;
-    if (leader) my_cnt[c] = base + len;
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     if (valid) {
       // the LAS record's dwords from aligned LDS reads
       const uint8_t* rec = rec_area + shift + lane * bpp;
       const uint32_t sh = (uint32_t)((uintptr_t)rec & 3);
       const uint32_t* ra = (const uint32_t*)(rec - sh);
       const uint32_t r0 = ra[0], r1 = ra[1], r2 = ra[2], r3 = ra[3], r4 = ra[4];
-      const uint32_t w[4] = {__builtin_amdgcn_alignbyte(r1, r0, sh), __builtin_amdgcn_alignbyte(r2, r1, sh),
-                             __builtin_amdgcn_alignbyte(r3, r2, sh), __builtin_amdgcn_alignbyte(r4, r3, sh)};
+      const uint32_t wd[4] = {__builtin_amdgcn_alignbyte(r1, r0, sh), __builtin_amdgcn_alignbyte(r2, r1, sh),
+                              __builtin_amdgcn_alignbyte(r3, r2, sh), __builtin_amdgcn_alignbyte(r4, r3, sh)};
       uint32_t o[4];
 #pragma unroll
       for (int a = 0; a < 3; a++) {
-        const double x = P.las_scale[a] * (int32_t)w[a] + P.las_offset[a];
+        const double x = P.las_scale[a] * (int32_t)wd[a] + P.las_offset[a];
         o[a] = (uint32_t)(int32_t)((x - P.pos_offset[a]) / P.pos_scale[a]);
       }
-      const uint32_t b14 = (w[3] >> 16) & 0xFFu, b15 = w[3] >> 24;
+      const uint32_t b14 = (wd[3] >> 16) & 0xFFu, b15 = wd[3] >> 24;
       uint32_t ret, nret, cls;
       if constexpr (EXT) {
         ret = legacy_return_number(b14);
@@ -917,12 +1039,12 @@ __global__ __launch_bounds__(kBlock) void k_scatter_tail(const uint8_t* __restri
         nret = (b14 >> 3) & 7u;
         cls = b15 & 31u;
       }
-      o[3] = (w[3] & 0xFFFFu) | (ret << 16) | (nret << 24);
+      o[3] = (wd[3] & 0xFFFFu) | (ret << 16) | (nret << 24);
       // output dword k >= 4 holds LAS bytes 4k-1 .. 4k+2 (byte 16 of the
       // output is the class byte): LAS byte offset sh + 4k - 1 from ra
       const uint32_t* rb = ra + ((sh + 3) >> 2) - 1;
       const uint32_t s2 = (sh + 3) & 3;
-      const uint32_t off = (my_run + (uint32_t)__popcll(peers & lt_mask)) * (uint32_t)bpo;
+      const uint32_t off = my_so;
       const uint32_t sel = 0x07060504u - (off & 3) * 0x01010101u;
       uint32_t* dw = (uint32_t*)out_area + (off >> 2);
       atomicOr(dw, __builtin_amdgcn_perm(o[0], 0u, sel));
@@ -946,10 +1068,14 @@ __global__ __launch_bounds__(kBlock) void k_scatter_tail(const uint8_t* __restri
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    write_runs(__ballot(leader), base, len, my_run, bpo, out_area, out);
+    // the next step's records into the record area (read by this step's
+    // transcode above), after a whole step of latency
+    if (st + 1 < steps) put();
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    nr_prev = nr;
   }
+  write_runs(nr_prev);
 }
 
 __global__ void k_lasgen(LasGenParams p, const LasGenBuilding* __restrict__ bld, int64_t first, int64_t count,
@@ -985,9 +1111,9 @@ __global__ void k_lasgen(LasGenParams p, const LasGenBuilding* __restrict__ bld,
 int g_bpp_in = 0;   // host copies of the record lengths in c_params
 int g_bpp_out = 0;
 int g_point_format = 8;  // k_count instantiation (8: no standard statistics)
-int g_tail_layout = -1;  // k_scatter_tail<false> (0), <true> (1), or k_scatter (-1)
+int g_tail_layout = -1;  // k_distribute<false> (0), <true> (1), or k_scatter (-1)
 
-// The layout k_scatter_tail writes: intensity copied, the three derived
+// The layout k_distribute writes: intensity copied, the three derived
 // bytes at 14-16, then LAS bytes 16 .. bpp - 1 copied one byte later.
 int tail_layout(const ChunkerParams& p) {
   if (p.bpp_out != p.bpp_in + 1 || p.bpp_in < 17 || p.num_ops < 4) return -1;
@@ -1195,8 +1321,8 @@ int chunker_distribute(const uint8_t* d_las, uint32_t* d_ids, int64_t n, const i
   }
   size_t hist_lds = num_chunks <= 8192 ? sizeof(uint32_t) * num_chunks : 0;
   if (num_chunks > 8192) PCG_CHECK(hipMemsetAsync(d_M, 0, sizeof(uint32_t) * tiles * num_chunks, stream));
-  hipLaunchKernelGGL(k_tile_hist, dim3((unsigned)tiles), dim3(kBlock), hist_lds, stream, d_ids, n, tile_size, d_lut,
-                     num_chunks, tiles, d_M, d_err);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_hist<uint32_t>), dim3((unsigned)tiles), dim3(kBlock), hist_lds, stream,
+                     d_ids, d_ids, n, tile_size, d_lut, num_chunks, tiles, d_M, d_err);
   PCG_CHECK(hipGetLastError());
   if (exclusive_scan_u32(d_M, tiles * num_chunks, d_scan_ws, nullptr, stream)) return 1;
   int key_bits = 1;
@@ -1206,23 +1332,59 @@ int chunker_distribute(const uint8_t* d_las, uint32_t* d_ids, int64_t n, const i
     set_error("too many chunks for the LDS scatter counters");
     return 1;
   }
-  // k_scatter_tail instantiations: prefetch registers for records of up to
-  // 32 and 64 bytes (longer records take k_scatter)
-#define PCG_TAIL(E, V)                                                                                   \
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_scatter_tail<E, V>), dim3((unsigned)tiles), dim3(kBlock), cnt_lds, stream, \
-                     d_las, d_ids, n, tile_size, num_chunks, key_bits, tiles, d_M, d_out)
-  const bool short_rec = g_bpp_in <= 32;
-  if (g_tail_layout == 0 && g_bpp_in <= 64) {
-    if (short_rec) PCG_TAIL(false, prefetch_vecs(32));
-    else PCG_TAIL(false, prefetch_vecs(64));
-  } else if (g_tail_layout == 1 && g_bpp_in <= 64) {
-    if (short_rec) PCG_TAIL(true, prefetch_vecs(32));
-    else PCG_TAIL(true, prefetch_vecs(64));
-  } else {
-    hipLaunchKernelGGL(k_scatter, dim3((unsigned)tiles), dim3(kBlock), cnt_lds, stream, d_las, d_ids, n, tile_size,
-                       num_chunks, key_bits, tiles, d_M, d_out);
+  // the byte-wise transcode of any record layout (the layouts of point
+  // formats 0-3 and 6-7 normally take chunker_distribute_tail)
+  hipLaunchKernelGGL(k_scatter, dim3((unsigned)tiles), dim3(kBlock), cnt_lds, stream, d_las, d_ids, n, tile_size,
+                     num_chunks, key_bits, tiles, d_M, d_out);
+  PCG_CHECK(hipGetLastError());
+  return 0;
+}
+
+bool distribute_tail_fits(int32_t num_chunks) {
+  return (g_tail_layout == 0 || g_tail_layout == 1) && g_bpp_in <= 64 && num_chunks < (int32_t)kNoChunk16 &&
+         distribute_lds_bytes(num_chunks, g_bpp_in, g_bpp_out) <= 150 * 1024;
+}
+
+int chunker_distribute_tail(const uint8_t* d_las, const uint32_t* d_cells, uint16_t* d_ids, int64_t n,
+                            const int32_t* d_lut, int32_t num_chunks, uint32_t* d_M, void* d_scan_ws,
+                            uint8_t* d_out, int32_t* d_err, hipStream_t stream) {
+  const int64_t tiles = (n + kDistTile - 1) / kDistTile;
+  if (tiles == 0) return 0;
+  if (!distribute_tail_fits(num_chunks)) {
+    set_error("record layout or chunk count outside the tile transcode");
+    return 1;
   }
-#undef PCG_TAIL
+  size_t hist_lds = num_chunks <= 8192 ? sizeof(uint32_t) * num_chunks : 0;
+  if (num_chunks > 8192) PCG_CHECK(hipMemsetAsync(d_M, 0, sizeof(uint32_t) * tiles * num_chunks, stream));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_hist<uint16_t>), dim3((unsigned)tiles), dim3(kBlock), hist_lds, stream,
+                     d_cells, d_ids, n, kDistTile, d_lut, num_chunks, tiles, d_M, d_err);
+  PCG_CHECK(hipGetLastError());
+  if (exclusive_scan_u32(d_M, tiles * num_chunks, d_scan_ws, nullptr, stream)) return 1;
+  const size_t lds = distribute_lds_bytes(num_chunks, g_bpp_in, g_bpp_out);
+  static bool attr = false;
+  if (!attr) {
+    PCG_CHECK(hipFuncSetAttribute((const void*)k_distribute<false, prefetch_vecs(32)>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    PCG_CHECK(hipFuncSetAttribute((const void*)k_distribute<false, prefetch_vecs(64)>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    PCG_CHECK(hipFuncSetAttribute((const void*)k_distribute<true, prefetch_vecs(32)>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    PCG_CHECK(hipFuncSetAttribute((const void*)k_distribute<true, prefetch_vecs(64)>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    attr = true;
+  }
+#define PCG_DIST(E, V)                                                                                    \
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_distribute<E, V>), dim3((unsigned)tiles), dim3(kBlock), lds, stream, d_las, \
+                     d_ids, n, num_chunks, tiles, d_M, d_out)
+  const bool short_rec = g_bpp_in <= 32;
+  if (g_tail_layout == 0) {
+    if (short_rec) PCG_DIST(false, prefetch_vecs(32));
+    else PCG_DIST(false, prefetch_vecs(64));
+  } else {
+    if (short_rec) PCG_DIST(true, prefetch_vecs(32));
+    else PCG_DIST(true, prefetch_vecs(64));
+  }
+#undef PCG_DIST
   PCG_CHECK(hipGetLastError());
   return 0;
 }

@@ -32,6 +32,7 @@ Device& device() {
 
 int32_t g_chunk_grid = 0;
 int64_t g_chunk_cap = 0;
+bool g_distribute_tail = true;
 
 int ensure_device() {
   static bool ready = false;
@@ -306,18 +307,27 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
   const int32_t tile = 4096;
   int64_t tiles = (N + tile - 1) / tile;
   int32_t C = (int32_t)out.chunks.size();
+  out.d_records = (uint8_t*)workspace(WS_RECORDS, (size_t)N * out.bpp + 64);
+  out.owned = false;
+  if (!out.d_records) return 1;
+  // the distribute kernels write every byte of every output record (unset bytes 0)
   DevBuf M, ws;
   if (M.alloc(WS_TILE_M, sizeof(uint32_t) * std::max<int64_t>(1, tiles * C)) ||
       ws.alloc(WS_SCAN, scan_workspace_bytes(tiles * C)))
     return 1;
-  out.d_records = (uint8_t*)workspace(WS_RECORDS, (size_t)N * out.bpp + 64);
-  out.owned = false;
-  if (!out.d_records) return 1;
-  // k_scatter writes every byte of every output record (unset bytes 0)
-  PCG_CHECK(hipEventRecord(ev[2], s));
-  if (chunker_distribute(d_las, ids.as<uint32_t>(), N, lut.as<int32_t>(), C, tile, M.as<uint32_t>(), ws.p,
-                         out.d_records, err.as<int32_t>(), s))
-    return 1;
+  if (g_distribute_tail && distribute_tail_fits(C)) {
+    DevBuf ids16;
+    if (ids16.alloc(WS_CHUNK_IDS16, sizeof(uint16_t) * std::max<int64_t>(N, 1) + 64)) return 1;
+    PCG_CHECK(hipEventRecord(ev[2], s));
+    if (chunker_distribute_tail(d_las, ids.as<uint32_t>(), ids16.as<uint16_t>(), N, lut.as<int32_t>(), C,
+                                M.as<uint32_t>(), ws.p, out.d_records, err.as<int32_t>(), s))
+      return 1;
+  } else {
+    PCG_CHECK(hipEventRecord(ev[2], s));
+    if (chunker_distribute(d_las, ids.as<uint32_t>(), N, lut.as<int32_t>(), C, tile, M.as<uint32_t>(), ws.p,
+                           out.d_records, err.as<int32_t>(), s))
+      return 1;
+  }
   PCG_CHECK(hipEventRecord(ev[3], s));
   PCG_CHECK(hipMemcpyAsync(herr, err.p, sizeof(herr), hipMemcpyDeviceToHost, s));
   PCG_CHECK(hipStreamSynchronize(s));

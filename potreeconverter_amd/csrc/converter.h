@@ -30,7 +30,7 @@ struct ChunkDesc {
 enum WorkspaceSlot : int {
   WS_GRID, WS_STATS, WS_HIST, WS_ERR, WS_CHUNK_LIST, WS_CHUNK_COUNTER, WS_CHUNKS, WS_LUT, WS_TILE_M, WS_SCAN,
   WS_RECORDS, WS_OCT_B, WS_OCT_KEYS_A, WS_OCT_KEYS_B, WS_OCT_M, WS_OCT_SCAN, WS_OCT_SEGS, WS_OCT_TILES,
-  WS_OCT_NODES, WS_OCT_COUNTER, WS_INFO, WS_STATE, WS_LIST, WS_ARENA, WS_TOPS, WS_COUNTERS, WS_OCTREE, WS_COPYDESC,
+  WS_OCT_NODES, WS_OCT_COUNTER, WS_INFO, WS_STATE, WS_LIST, WS_ARENA, WS_TOPS, WS_COUNTERS, WS_OCTREE, WS_COPYDESC, WS_CHUNK_IDS16,
   WS_POINT_IDS, WS_ENC_NODES, WS_ENC_SCRATCH, WS_SHUF_DESC, WS_SHUF_OUT, WS_SHUF_IDX, WS_OCT_H5,
   WS_LEVEL0,  // pyramid levels: WS_LEVEL0 + level
   WS_GROUP0 = WS_LEVEL0 + 16,  // sampling group g, buffer k: WS_GROUP0 + 8 * g + k
@@ -82,6 +82,9 @@ inline hipError_t stream_copy(void* dst, const void* src, size_t bytes, hipMemcp
 // G = 256 / 512 paths of a large one.
 extern int32_t g_chunk_grid;
 extern int64_t g_chunk_cap;
+// The tile transcode (16-bit chunk ids, aligned 16-byte run stores) where the
+// record layout allows, else the byte-wise scatter ("distribute_tail" option).
+extern bool g_distribute_tail;
 
 // doChunking on LAS records already resident in HBM (header describes them).
 int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& out);

@@ -382,3 +382,23 @@ def test_forced_chunk_grid_matches_oracle(grid, cap, tmp_path):
     a, b = sorted(os.listdir(cpu / "chunks")), sorted(os.listdir(gpu / "chunks"))
     assert a == b
     assert all(same_files(cpu / "chunks", gpu / "chunks", a).values())
+
+
+@pytest.mark.parametrize("fmt,eb", [(2, False), (7, True)], ids=["pdrf2", "pdrf7_eb"])
+def test_bytewise_distribute_matches_oracle(fmt, eb, tmp_path):
+    """The byte-wise scatter (taken by point formats 4-5, records over 64
+    bytes and very many chunks) forced on the layouts the tile transcode
+    normally takes: the same chunk files."""
+    print(f"byte-wise distribute case pdrf={fmt} extra={eb}", flush=True)
+    las = tmp_path / "in.las"
+    make_las(las, 1, 500_000, 60 + fmt, eb, fmt)
+    cpu, gpu = tmp_path / "cpu", tmp_path / "gpu"
+    oracle_convert(las, cpu, "poisson", chunk_only=True)
+    _set("distribute_tail", 0)
+    try:
+        pc.do_chunking(las, gpu)
+    finally:
+        _set("distribute_tail", 1)
+    a, b = sorted(os.listdir(cpu / "chunks")), sorted(os.listdir(gpu / "chunks"))
+    assert a == b and len(a) > 4
+    assert all(same_files(cpu / "chunks", gpu / "chunks", a).values())
