@@ -7,6 +7,11 @@ coordinates, LAS 1.4 point format 2, scale 0.001) up to the octree.bin image
 in HBM plus hierarchy.bin / metadata.json on the host.  No files are written
 inside the timed region.
 
+--config c2: one step = doIndexing (default -m poisson_average) of 4096
+pre-chunked level-4 chunks resident in HBM, log-normal point counts (median
+50 000, clipped to 1 000 .. 5 000 000; about 400M points), 34-byte records
+with three extra-bytes attributes (potreeconverter_amd.ChunkGen).
+
 Prints one JSON line (see README for the fields).  `cpu_baseline` times the
 single-threaded CPU oracle (a restatement of the reference algorithm) on a
 bounded prefix of the same point stream, on this machine's host.
@@ -30,10 +35,15 @@ STATS = 24
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--config", default="c1", choices=["c1", "c2"],
+                    help="c1: 200M-point LAS tile, chunking + indexing; c2: indexing of 4096 pre-chunked chunks")
+    ap.add_argument("--c2-plain-median", type=int, default=2_500,
+                    help="c2: median chunk size of the batched-vs-plain comparison (0 skips it)")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--points", type=int, default=200_000_000)
-    ap.add_argument("--method", default="poisson", choices=["poisson", "poisson_average", "random"])
+    ap.add_argument("--method", default=None, choices=["poisson", "poisson_average", "random"],
+                    help="default: poisson (c1), poisson_average (c2)")
     ap.add_argument("--encoding", default="DEFAULT",
                     help="--encoding of the converter: DEFAULT (octree image in HBM) or BROTLI (compressed nodes "
                          "assembled in host memory)")
@@ -127,12 +137,124 @@ def plain_vs_batched(points, seed, method):
     }
 
 
+C2 = dict(num_chunks=4096, median=50_000, sigma=1.16, min_count=1_000, max_count=5_000_000)
+
+
+def c2_indexing(args):
+    """Generates the pre-chunked set in HBM, then times doIndexing on it."""
+    import potreeconverter_amd as pc
+
+    L = pc.lib()
+
+    def setup(median):
+        g = pc.ChunkGen(**dict(C2, median=median), seed=args.seed)
+        d = ctypes.c_void_p()
+        pc._check(L.pcg_device_alloc(ctypes.byref(d), g.total * g.RECORD_BYTES + 64))
+        meta = g.device(d.value)
+        n = len(g.names)
+        names = (ctypes.c_char_p * n)(*[x.encode() for x in g.names])
+        counts = (ctypes.c_int64 * n)(*g.counts)
+        return g, d, meta.encode(), names, counts
+
+    stats = (ctypes.c_double * STATS)()
+
+    def index(job):
+        g, d, meta, names, counts = job
+        pc._check(L.pcg_do_indexing_device(d, len(g.names), names, counts, meta, args.method.encode(),
+                                           args.encoding.encode(), b"c2", None, stats))
+        return list(stats)
+
+    job = setup(C2["median"])
+    g = job[0]
+    for _ in range(args.warmup):
+        index(job)
+    pc._check(L.pcg_synchronize())
+    per = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        per.append(index(job))
+    pc._check(L.pcg_synchronize())
+    elapsed = time.perf_counter() - t0
+    L.pcg_device_free(job[1])
+    last = per[-1]
+    samp_s = sum(p[5] for p in per) / len(per)
+    bpp = g.RECORD_BYTES
+    cand = last[10]
+    # node sampling: 12 + 2 * bpp bytes per candidate (distance key + index,
+    # the record read once to gather and once to route)
+    samp_bytes = cand * (12 + 2 * bpp)
+    achieved = samp_bytes / samp_s / 1e9 if samp_s > 0 else 0.0
+    plain = None
+    if args.c2_plain_median > 0:
+        small = setup(args.c2_plain_median)
+        st = {}
+        try:
+            for mode in ("batched", "batched", "plain"):
+                pc._check(L.pcg_set_option(b"plain_sampling", 1 if mode == "plain" else 0))
+                st[mode] = index(small)
+        finally:
+            L.pcg_set_option(b"plain_sampling", 0)
+            L.pcg_device_free(small[1])
+        b, p = st["batched"], st["plain"]
+        plain = {
+            "points": small[0].total, "median": args.c2_plain_median, "candidates": int(b[10]),
+            "sampled_nodes": int(b[9]), "batched_sampling_s": round(b[5], 4), "plain_sampling_s": round(p[5], 4),
+            "batched_Mcand_per_s": round(b[10] / b[5] / 1e6, 1), "plain_Mcand_per_s": round(b[10] / p[5] / 1e6, 1),
+            "speedup": round(p[5] / b[5], 2), "same_output_bytes": b[7] == p[7],
+        }
+    line = {
+        "metric": "Mpts/s doIndexing of 4096 pre-chunked log-normal chunks (octree + sampling), batched vs plain",
+        "value": round(g.total / (elapsed / args.steps) / 1e6, 3),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"4096 level-4 chunks of a 2048 m cube, log-normal counts (median 50000, sigma 1.16, "
+                        f"clip 1000..5000000), {g.total} pts, PDRF2 + 3 extra bytes (34 B), seed {args.seed}",
+            "points": g.total, "chunks": len(g.names), "method": args.method, "encoding": args.encoding,
+            "path": "batched (all nodes of a level per launch)",
+        },
+        "phases_s": {
+            "indexing": round(last[3], 4), "octree_build": round(last[4], 4), "sampling": round(last[5], 4),
+            "octree_assembly": round(last[6], 4), "host_tree_setup": round(last[21], 4),
+            "host_completion_order": round(last[22], 4), "host_hierarchy_metadata": round(last[23], 4),
+        },
+        "counters": {
+            "nodes": int(last[8]), "sampled_nodes": int(last[9]), "candidates": int(cand),
+            "fp32_pair_tests": int(last[12]), "fp64_rechecks": int(last[11]), "max_fp32_rel_err": last[13],
+            "libstdcxx_order_nodes": int(last[14]), "sequential_nodes": int(last[15]),
+            "octree_rounds": int(last[17]), "depth": int(last[18]), "octree_bytes": int(last[7]),
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": "node sampling (all sampling kernels, sampling phase)",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": samp_bytes,
+            "kernel_ms": round(samp_s * 1e3, 3),
+        },
+        "plain_vs_batched": plain,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md, "
                                    "--gpus above 1 is not supported"}))
         sys.exit(2)
+    if args.method is None:
+        args.method = "poisson_average" if args.config == "c2" else "poisson"
+    if args.config == "c2":
+        c2_indexing(args)
+        return
     import potreeconverter_amd as pc
 
     L = pc.lib()

@@ -130,6 +130,21 @@ int pcg_convert_device(const void* d_las, int64_t n, const uint8_t* las_header, 
 int pcg_convert_device_encoded(const void* d_las, int64_t n, const uint8_t* las_header, const char* method,
                                const char* encoding, const char* outdir, double* stats);
 
+/* Phase 2 on chunk records already in HBM: indexer::doIndexing
+ * (Converter/include/indexer.h:379) over the chunks that getChunks
+ * (Converter/src/indexer.cpp:53-179) would read from a chunks/ directory.
+ * d_records holds the records of num_chunks chunks back to back, in
+ * ascending chunk name ("r" + octal digits, the names of the <name>.bin
+ * files), chunk_counts[i] records each; chunks_metadata is the text of
+ * chunks/metadata.json (cube min/max, attributes, scale, offset).
+ * d_records is only read (the library indexes a copy).  method, encoding and
+ * name as pcg_do_indexing_encoded; outdir (nullable) receives octree.bin,
+ * hierarchy.bin and metadata.json; stats (nullable) as pcg_convert_device,
+ * with the chunking entries 0. */
+int pcg_do_indexing_device(const void* d_records, int32_t num_chunks, const char* const* chunk_names,
+                           const int64_t* chunk_counts, const char* chunks_metadata, const char* method,
+                           const char* encoding, const char* name, const char* outdir, double* stats);
+
 /* Device memory helpers for callers without a HIP runtime binding.
  * kind: 0 host->device, 1 device->host, 2 device->device. */
 int pcg_device_alloc(void** ptr, uint64_t bytes);
@@ -155,6 +170,31 @@ int pcg_lasgen_device(const void* params, int64_t first, int64_t count, uint8_t*
 /* 375-byte LAS 1.4 header image for generated points with the given
  * integer extents. */
 int pcg_lasgen_header(const void* params, const int32_t qminmax[6], uint8_t* header375);
+
+/* Synthetic pre-chunked input of phase 2 (not part of the reference): the
+ * cube [0, 2048 m)^3 at scale 0.001, num_chunks (a power of two up to 4096)
+ * chunks "rABCD" of octree level 4 spread over its 16^3 cells, each a local
+ * terrain patch with box buildings, records in the output layout of LAS
+ * point format 2 plus three extra-bytes attributes (float32 N(0,1) "gauss",
+ * uint16 "uniform16", int8 "uniform8"; 34 bytes).  Chunk point counts are
+ * log-normal: round(median * exp(sigma * g)), g ~ N(0,1), clipped to
+ * [min_count, max_count].  `params` points at a caller-owned buffer of at
+ * least 128 bytes. */
+int pcg_chunkgen_params(int32_t num_chunks, double median, double sigma, int64_t min_count, int64_t max_count,
+                        uint64_t seed, void* params);
+/* counts (nullable, num_chunks values) and names (nullable, 6 bytes each:
+ * "rABCD" and a terminator) of the chunks in ascending name order. */
+int pcg_chunkgen_counts(const void* params, int64_t* counts, char* names);
+/* records [first, first+count) of chunk `chunk` into out (host) */
+void pcg_chunkgen_fill(const void* params, int32_t chunk, int64_t first, int64_t count, uint8_t* out);
+/* Writes outdir/chunks/<name>.bin and outdir/chunks/metadata.json as the
+ * reference chunker would have left them (host generation). */
+int pcg_chunkgen_write(const void* params, const char* outdir);
+/* Generates every chunk's records into d_records on the GPU (34 bytes per
+ * point, ascending name order) and writes the text of the matching
+ * chunks/metadata.json (attribute minima, maxima and the classification
+ * histogram of the generated records) into metadata. */
+int pcg_chunkgen_device(const void* params, void* d_records, char* metadata, int64_t metadata_capacity);
 
 #ifdef __cplusplus
 }

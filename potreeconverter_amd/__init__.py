@@ -59,13 +59,25 @@ def lib():
         L.pcg_lasgen_write.argtypes = [c.c_void_p, c.c_char_p]
         L.pcg_lasgen_device.argtypes = [c.c_void_p, c.c_int64, c.c_int64, c.c_void_p, c.c_void_p]
         L.pcg_lasgen_header.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p]
+        L.pcg_do_indexing_device.argtypes = [c.c_void_p, c.c_int32, c.POINTER(c.c_char_p), c.POINTER(c.c_int64),
+                                             c.c_char_p, c.c_char_p, c.c_char_p, c.c_char_p, c.c_char_p,
+                                             c.POINTER(c.c_double)]
+        L.pcg_chunkgen_params.argtypes = [c.c_int32, c.c_double, c.c_double, c.c_int64, c.c_int64, c.c_uint64,
+                                          c.c_void_p]
+        L.pcg_chunkgen_counts.argtypes = [c.c_void_p, c.POINTER(c.c_int64), c.c_void_p]
+        L.pcg_chunkgen_fill.argtypes = [c.c_void_p, c.c_int32, c.c_int64, c.c_int64, c.c_void_p]
+        L.pcg_chunkgen_fill.restype = None
+        L.pcg_chunkgen_write.argtypes = [c.c_void_p, c.c_char_p]
+        L.pcg_chunkgen_device.argtypes = [c.c_void_p, c.c_void_p, c.c_char_p, c.c_int64]
         L.pcg_release.argtypes = []
         L.pcg_set_option.argtypes = [c.c_char_p, c.c_int64]
         for name in ("pcg_set_option", "pcg_release", "pcg_convert", "pcg_do_chunking", "pcg_do_indexing", "pcg_count_cells", "pcg_count_cells_noalloc", "pcg_convert_device",
                      "pcg_convert_encoded", "pcg_do_indexing_encoded", "pcg_convert_device_encoded",
                      "pcg_device_alloc", "pcg_device_free", "pcg_copy", "pcg_synchronize", "pcg_lasgen_preset",
                      "pcg_lasgen_configure",
-                     "pcg_lasgen_write", "pcg_lasgen_device", "pcg_lasgen_header", "pcg_init"):
+                     "pcg_lasgen_write", "pcg_lasgen_device", "pcg_lasgen_header", "pcg_init",
+                     "pcg_do_indexing_device", "pcg_chunkgen_params", "pcg_chunkgen_counts", "pcg_chunkgen_write",
+                     "pcg_chunkgen_device"):
             getattr(L, name).restype = c.c_int
         _lib = L
     return _lib
@@ -101,6 +113,59 @@ class LasGen:
 
     def write(self, path):
         _check(lib().pcg_lasgen_write(self.buf, str(path).encode()))
+
+
+class ChunkGen:
+    """Parameters of the seeded synthetic pre-chunked input of phase 2:
+    num_chunks level-4 chunks "rABCD" of a 2048 m cube, log-normal point
+    counts round(median * exp(sigma * g)) clipped to [min_count, max_count],
+    34-byte records (LAS format 2 output layout + float32 / uint16 / int8
+    extra bytes).  C2 = ChunkGen(4096, 50_000, C2_SIGMA, 1_000, 5_000_000)."""
+
+    RECORD_BYTES = 34
+
+    def __init__(self, num_chunks, median, sigma, min_count, max_count, seed=0):
+        self.buf = ctypes.create_string_buffer(128)
+        _check(lib().pcg_chunkgen_params(num_chunks, median, sigma, min_count, max_count, seed, self.buf))
+        self.num_chunks = num_chunks
+        counts = (ctypes.c_int64 * num_chunks)()
+        names = ctypes.create_string_buffer(6 * num_chunks)
+        _check(lib().pcg_chunkgen_counts(self.buf, counts, names))
+        self.counts = list(counts)
+        self.names = [names.raw[6 * i:6 * i + 5].decode() for i in range(num_chunks)]
+        self.total = sum(self.counts)
+
+    def records(self, chunk, first=0, count=None):
+        count = self.counts[chunk] - first if count is None else count
+        out = ctypes.create_string_buffer(count * self.RECORD_BYTES)
+        lib().pcg_chunkgen_fill(self.buf, chunk, first, count, out)
+        return out.raw
+
+    def write(self, outdir):
+        """outdir/chunks/<name>.bin + metadata.json (host generation)."""
+        _check(lib().pcg_chunkgen_write(self.buf, str(outdir).encode()))
+
+    def device(self, d_records):
+        """Generates the records into d_records (a device pointer of at least
+        total * 34 bytes) on the GPU; returns the chunks/metadata.json text."""
+        meta = ctypes.create_string_buffer(1 << 20)
+        _check(lib().pcg_chunkgen_device(self.buf, ctypes.c_void_p(d_records), meta, len(meta)))
+        return meta.value.decode()
+
+
+def do_indexing_device(d_records, names, counts, metadata, method="poisson", encoding=None, name="",
+                       outdir=None):
+    """indexer::doIndexing on chunk records resident in HBM (ascending chunk
+    names, records back to back); returns the 24 statistics of
+    pcg_convert_device."""
+    n = len(names)
+    cn = (ctypes.c_char_p * n)(*[x.encode() for x in names])
+    cc = (ctypes.c_int64 * n)(*counts)
+    st = (ctypes.c_double * 24)()
+    _check(lib().pcg_do_indexing_device(ctypes.c_void_p(d_records), n, cn, cc, metadata.encode(), method.encode(),
+                                        encoding.encode() if encoding else None, name.encode(),
+                                        str(outdir).encode() if outdir else None, st))
+    return list(st)
 
 
 def convert(las_path, outdir, method="poisson", keep_chunks=False, encoding=None):

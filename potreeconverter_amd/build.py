@@ -20,12 +20,14 @@ SOURCES = [
     "chunker.hip",
     "octree.hip",
     "sampler.hip",
+    "chunkgen.hip",
     "chunking.cpp",
     "indexing.cpp",
     "writers.cpp",
     "encoding.cpp",
     "attributes.cpp",
     "lasgen_host.cpp",
+    "chunkgen_host.cpp",
     "capi.cpp",
 ]
 

@@ -185,6 +185,37 @@ int pcg_do_indexing_encoded(const char* outdir, const char* method, const char* 
   return rc;
 }
 
+static void fill_stats(double* stats, const DeviceChunks& dc, const IndexResult& r) {
+  if (!stats) return;
+  stats[0] = dc.t_count;
+  stats[1] = dc.t_distribute;
+  stats[2] = dc.t_total;
+  stats[3] = r.t_total;
+  stats[4] = r.t_build;
+  stats[5] = r.t_sample;
+  stats[6] = r.t_assemble;
+  stats[7] = (double)r.octreeBytes;
+  stats[8] = (double)r.numNodes;
+  stats[9] = (double)r.numSampled;
+  stats[10] = (double)r.counters[5];  // candidates
+  stats[11] = (double)r.counters[0];  // fp64 re-checks
+  stats[12] = (double)r.counters[1];  // fp32 pair tests
+  uint32_t eb = (uint32_t)r.counters[2];
+  float e;
+  memcpy(&e, &eb, 4);
+  stats[13] = (double)e;              // max |dd32 - dd64| / spacing^2 on re-checked pairs
+  stats[14] = (double)r.counters[3];  // nodes sorted in libstdc++ order
+  stats[15] = (double)r.counters[4];  // nodes resolved sequentially
+  stats[16] = (double)dc.chunks.size();
+  stats[17] = (double)r.buildRounds;
+  stats[18] = (double)r.octreeDepth;
+  stats[19] = (double)dc.ms_count_kernel;
+  stats[20] = (double)dc.ms_distribute_kernels;
+  stats[21] = r.t_setup;
+  stats[22] = r.t_order;
+  stats[23] = r.t_hierarchy;
+}
+
 int pcg_convert_device(const void* d_las, int64_t n, const uint8_t* las_header, const char* method,
                        const char* outdir, double* stats) {
   return pcg_convert_device_encoded(d_las, n, las_header, method, nullptr, outdir, stats);
@@ -208,35 +239,62 @@ int pcg_convert_device_encoded(const void* d_las, int64_t n, const uint8_t* las_
   IndexResult r;
   rc = gpu_indexing(dc, method, "device", n, r, encoding_or_default(encoding));
   if (!rc && outdir) rc = write_index_files(r, outdir);
-  if (stats) {
-    stats[0] = dc.t_count;
-    stats[1] = dc.t_distribute;
-    stats[2] = dc.t_total;
-    stats[3] = r.t_total;
-    stats[4] = r.t_build;
-    stats[5] = r.t_sample;
-    stats[6] = r.t_assemble;
-    stats[7] = (double)r.octreeBytes;
-    stats[8] = (double)r.numNodes;
-    stats[9] = (double)r.numSampled;
-    stats[10] = (double)r.counters[5];  // candidates
-    stats[11] = (double)r.counters[0];  // fp64 re-checks
-    stats[12] = (double)r.counters[1];  // fp32 pair tests
-    uint32_t eb = (uint32_t)r.counters[2];
-    float e;
-    memcpy(&e, &eb, 4);
-    stats[13] = (double)e;              // max |dd32 - dd64| / spacing^2 on re-checked pairs
-    stats[14] = (double)r.counters[3];  // nodes sorted in libstdc++ order
-    stats[15] = (double)r.counters[4];  // nodes resolved sequentially
-    stats[16] = (double)dc.chunks.size();
-    stats[17] = (double)r.buildRounds;
-    stats[18] = (double)r.octreeDepth;
-    stats[19] = (double)dc.ms_count_kernel;
-    stats[20] = (double)dc.ms_distribute_kernels;
-    stats[21] = r.t_setup;
-    stats[22] = r.t_order;
-    stats[23] = r.t_hierarchy;
+  fill_stats(stats, dc, r);
+  dc.release();
+  r.release();
+  return rc;
+}
+
+int pcg_do_indexing_device(const void* d_records, int32_t num_chunks, const char* const* chunk_names,
+                           const int64_t* chunk_counts, const char* chunks_metadata, const char* method,
+                           const char* encoding, const char* name, const char* outdir, double* stats) {
+  if (ensure_device()) return 1;
+  if (num_chunks < 0 || (num_chunks > 0 && (!chunk_names || !chunk_counts || !d_records)) || !chunks_metadata) {
+    set_error("pcg_do_indexing_device: missing chunk list, records or metadata");
+    return 1;
   }
+  DeviceChunks dc;
+  if (!parse_chunk_metadata(chunks_metadata, dc.min, dc.max, dc.attributes)) return 1;
+  dc.bpp = dc.attributes.bytes;
+  int64_t off = 0;
+  for (int32_t i = 0; i < num_chunks; i++) {
+    ChunkDesc d;
+    d.id = chunk_names[i] ? chunk_names[i] : "";
+    bool ok = d.id.size() >= 1 && d.id.size() <= 21 && d.id[0] == 'r' && chunk_counts[i] >= 0;
+    for (size_t c = 1; ok && c < d.id.size(); c++) ok = d.id[c] >= '0' && d.id[c] <= '7';
+    if (!ok) {
+      set_error("bad chunk " + std::to_string(i) + ": name \"" + d.id + "\", count " + std::to_string(chunk_counts[i]));
+      return 1;
+    }
+    // chunks in ascending name, the canonical order in which the library
+    // reads a chunks/ directory (getChunks, Converter/src/indexer.cpp:53-179,
+    // takes the directory listing's order)
+    if (i > 0 && !(dc.chunks.back().id < d.id)) {
+      set_error("chunk names must be distinct and in ascending order: " + dc.chunks.back().id + ", " + d.id);
+      return 1;
+    }
+    d.count = chunk_counts[i];
+    d.offset = off;
+    off += d.count;
+    d.level = (int32_t)d.id.size() - 1;
+    chunk_box(d.id, dc.min, dc.max, d.min, d.max);
+    dc.chunks.push_back(d);
+  }
+  dc.numPoints = off;
+  dc.pointsTotal = off;
+  // the octree build reorders the records in place: work on a copy
+  dc.d_records = (uint8_t*)workspace(WS_RECORDS, (size_t)off * dc.bpp + 64);
+  dc.owned = false;
+  if (!dc.d_records) return 1;
+  hipStream_t s = device().stream;
+  if (off > 0) PCG_CHECK(hipMemcpyAsync(dc.d_records, d_records, (size_t)off * dc.bpp, hipMemcpyDeviceToDevice, s));
+  IndexResult r;
+  int rc = gpu_indexing(dc, method, name ? name : "", off, r, encoding_or_default(encoding));
+  if (!rc && outdir) {
+    fs::create_directories(outdir);
+    rc = write_index_files(r, outdir);
+  }
+  fill_stats(stats, dc, r);
   dc.release();
   r.release();
   return rc;

@@ -32,6 +32,7 @@ enum WorkspaceSlot : int {
   WS_RECORDS, WS_OCT_B, WS_OCT_KEYS_A, WS_OCT_KEYS_B, WS_OCT_M, WS_OCT_SCAN, WS_OCT_SEGS, WS_OCT_TILES,
   WS_OCT_NODES, WS_OCT_COUNTER, WS_INFO, WS_STATE, WS_LIST, WS_ARENA, WS_TOPS, WS_COUNTERS, WS_OCTREE, WS_COPYDESC, WS_CHUNK_IDS16,
   WS_POINT_IDS, WS_ENC_NODES, WS_ENC_SCRATCH, WS_SHUF_DESC, WS_SHUF_OUT, WS_SHUF_IDX, WS_OCT_H5,
+  WS_GEN_TILES, WS_GEN_OFFSETS, WS_GEN_STATS,
   WS_LEVEL0,  // pyramid levels: WS_LEVEL0 + level
   WS_GROUP0 = WS_LEVEL0 + 16,  // sampling group g, buffer k: WS_GROUP0 + 8 * g + k
   WS_COUNT = WS_GROUP0 + 8 * 8

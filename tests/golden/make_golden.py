@@ -17,6 +17,11 @@ Edge-case scenes (tests/scenes.py: long equal-distance runs, duplicate
 stacks, exact-spacing lattices, the 10 001-check limit) are written the same
 way from their scene files.
 
+Pre-chunked sets of the batching workload (potreeconverter_amd.ChunkGen:
+level-4 chunks "rABCD" with log-normal point counts and three extra-bytes
+attributes) are written as a chunks/ directory by the library's host
+generator and indexed by the reference indexer directly.
+
 Run from the repository root:  python tests/golden/make_golden.py [case names]
 """
 import json
@@ -65,10 +70,23 @@ SCENE_CASES = [
     ("scene_sphere_poisson_average", "sphere", "poisson_average"),
 ]
 
+# pre-chunked sets: name, ChunkGen arguments, method
+CHUNKGEN_CASES = [
+    ("c2_64chunks_poisson_average", dict(num_chunks=64, median=4000, sigma=1.16, min_count=100, max_count=100_000,
+                                         seed=3), "poisson_average"),
+    ("c2_64chunks_poisson", dict(num_chunks=64, median=4000, sigma=1.16, min_count=100, max_count=100_000,
+                                 seed=3), "poisson"),
+    ("c2_512chunks_poisson_average", dict(num_chunks=512, median=2000, sigma=1.16, min_count=100,
+                                          max_count=200_000, seed=7), "poisson_average"),
+]
 
-def write_fixture(td, name, case, las, n, method, encoding="DEFAULT", random_seed=None):
+
+def write_fixture(td, name, case, las, n, method, encoding="DEFAULT", random_seed=None, chunkgen=None):
     work = os.path.join(td, name)
-    oracle_convert(las, work, method, keep_chunks=True, chunk_only=True)
+    if chunkgen is not None:
+        chunkgen.write(work)
+    else:
+        oracle_convert(las, work, method, keep_chunks=True, chunk_only=True)
     env = dict(os.environ, REF_CLOCK_NS=str(random_seed or 0))
     subprocess.run([REF_INDEXER, work, method, name, str(n), encoding], check=True, stdout=subprocess.DEVNULL,
                    stderr=subprocess.DEVNULL, env=env)
@@ -107,6 +125,13 @@ def main():
             n = make_scene(scene, las)
             case = {"scene": scene, "points": n, "method": method, "encoding": "DEFAULT"}
             write_fixture(td, name, case, las, n, method)
+        from potreeconverter_amd import ChunkGen
+        for name, args, method in CHUNKGEN_CASES:
+            if only and name not in only:
+                continue
+            g = ChunkGen(**args)
+            case = {"chunkgen": args, "points": g.total, "method": method, "encoding": "DEFAULT"}
+            write_fixture(td, name, case, None, g.total, method, chunkgen=g)
     finally:
         shutil.rmtree(td, ignore_errors=True)
 

@@ -44,6 +44,14 @@ def oracle_convert(las, outdir, method, keep_chunks=False, chunk_only=False, enc
     return list(t)
 
 
+def oracle_index(outdir, method, name=""):
+    """indexer::doIndexing of the oracle on outdir/chunks."""
+    t = (ctypes.c_double * 8)()
+    rc = oracle().orc_index(str(outdir).encode(), method.encode(), name.encode(), t)
+    assert rc == 0, oracle().orc_last_error()
+    return list(t)
+
+
 def make_las(path, preset, n, seed, extra_bytes=False, point_format=2):
     import potreeconverter_amd as pc
     g = pc.LasGen(preset, n, seed=seed, extra_bytes=extra_bytes, point_format=point_format)

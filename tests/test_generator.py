@@ -62,3 +62,46 @@ def test_precision_scene():
     cube = (8_192_000 * 1e-4 + 1e6) - (0 * 1e-4 + 1e6)
     sp = cube / 128 / 2**4
     assert abs(sp - 0.4) < 1e-12 and sp != 0.4
+
+
+C2_SIGMA = 1.16  # median 50 000, clip 1 000 .. 5 000 000: about 400M points over 4096 chunks
+
+
+def test_chunkgen_counts_and_names():
+    g = pc.ChunkGen(4096, 50_000, C2_SIGMA, 1_000, 5_000_000)
+    assert g.names == sorted(g.names) and len(set(g.names)) == 4096
+    assert g.names[0] == "r0000" and g.names[-1] == "r7777"
+    c = np.array(g.counts)
+    assert c.min() >= 1_000 and c.max() <= 5_000_000
+    assert abs(np.median(c) / 50_000 - 1) < 0.06
+    # log(count) ~ N(log 50000, sigma) away from the clip
+    s = np.log(c[(c > 1_000) & (c < 5_000_000)]).std()
+    assert abs(s / C2_SIGMA - 1) < 0.06
+    assert 3.5e8 < g.total < 4.5e8
+    # a subset of chunks spreads over the cube: stride 4096 / num_chunks cells
+    h = pc.ChunkGen(64, 1000, 0.5, 10, 5000)
+    assert h.names[1] == "r0100" and h.names[8] == "r1000"
+
+
+def test_chunkgen_records_stay_in_their_cell(tmp_path):
+    g = pc.ChunkGen(16, 3000, 0.5, 100, 20_000, seed=5)
+    g.write(tmp_path)
+    w = 2048.0 / 16
+    for i in (0, 5, 15):
+        raw = open(tmp_path / "chunks" / (g.names[i] + ".bin"), "rb").read()
+        assert raw == g.records(i)
+        assert g.records(i, 10, 7) == raw[10 * 34:17 * 34]
+        rec = np.frombuffer(raw, np.uint8).reshape(-1, 34)
+        xyz = rec[:, :12].copy().view(np.int32) * 0.001
+        # the cell from the name digits (x -> 4, y -> 2, z -> 1 per level)
+        cell = [0, 0, 0]
+        for d in g.names[i][1:]:
+            d = int(d)
+            cell = [2 * cell[0] + (d >> 2), 2 * cell[1] + ((d >> 1) & 1), 2 * cell[2] + (d & 1)]
+        lo = np.array(cell) * w
+        assert (xyz >= lo).all() and (xyz < lo + w).all()
+        assert set(np.unique(rec[:, 16])) <= {2, 6}
+        gauss = rec[:, 27:31].copy().view(np.float32).ravel()
+        assert abs(gauss.mean()) < 0.1 and abs(gauss.std() - 1) < 0.1
+    meta = open(tmp_path / "chunks" / "metadata.json").read()
+    assert '"name": "gauss"' in meta and '"name": "uniform8"' in meta

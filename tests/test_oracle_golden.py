@@ -13,7 +13,7 @@ import subprocess
 
 import pytest
 
-from helpers import REF_INDEXER, digest_output, make_las, oracle_convert
+from helpers import REF_INDEXER, digest_output, make_las, oracle_convert, oracle_index
 from scenes import make_scene
 
 # indexer fixtures (per-node digests); chunks_metadata_* and json_numbers pin
@@ -29,13 +29,19 @@ def test_oracle_matches_reference_fixture(fixture, tmp_path):
     case = fx["case"]
     name = os.path.basename(fixture)[:-5]
     las = tmp_path / (name + ".las")
-    if "scene" in case:
-        make_scene(case["scene"], las)
-    else:
-        make_las(las, case["preset"], case["points"], case["seed"], case["extra_bytes"])
     out = tmp_path / "out"
-    oracle_convert(las, out, case["method"], encoding=case.get("encoding", "DEFAULT"),
-                   random_seed=case.get("random_seed", 0))
+    if "chunkgen" in case:
+        # pre-chunked input: the generated chunks/ directory, indexed alone
+        from potreeconverter_amd import ChunkGen
+        ChunkGen(**case["chunkgen"]).write(out)
+        oracle_index(out, case["method"], name)
+    else:
+        if "scene" in case:
+            make_scene(case["scene"], las)
+        else:
+            make_las(las, case["preset"], case["points"], case["seed"], case["extra_bytes"])
+        oracle_convert(las, out, case["method"], encoding=case.get("encoding", "DEFAULT"),
+                       random_seed=case.get("random_seed", 0))
     with open(out / "metadata.json") as f:
         assert f.read() == fx["metadata_json"]
     got = digest_output(str(out))
