@@ -39,6 +39,8 @@ def parse():
                     help="c1: 200M-point LAS tile, chunking + indexing; c2: indexing of 4096 pre-chunked chunks")
     ap.add_argument("--c2-plain-median", type=int, default=2_500,
                     help="c2: median chunk size of the batched-vs-plain comparison (0 skips it)")
+    ap.add_argument("--sample-groups", type=int, default=None,
+                    help="library option sample_groups (streams of the chunk-subtree sampling; default: the library's)")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--points", type=int, default=200_000_000)
@@ -252,6 +254,9 @@ def main():
         sys.exit(2)
     if args.method is None:
         args.method = "poisson_average" if args.config == "c2" else "poisson"
+    if args.sample_groups is not None:
+        import potreeconverter_amd as pc
+        pc._check(pc.lib().pcg_set_option(b"sample_groups", args.sample_groups))
     if args.config == "c2":
         c2_indexing(args)
         return

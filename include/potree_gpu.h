@@ -145,6 +145,40 @@ int pcg_do_indexing_device(const void* d_records, int32_t num_chunks, const char
                            const int64_t* chunk_counts, const char* chunks_metadata, const char* method,
                            const char* encoding, const char* name, const char* outdir, double* stats);
 
+/* One Sampler::sample call (Converter/include/structures.h:152-164) of
+ * SamplerPoisson (sampler_poisson.h:14-304), SamplerPoissonAverage
+ * (sampler_poisson_average.h:14-492) or SamplerRandom (sampler_random.h),
+ * by method "poisson" / "poisson_average" / "random", on the GPU: the
+ * plugin a Sampler subclass forwards to (its glue is in INTEGRATION.md).
+ * The caller flattens the subtree below the node handed to sample() into
+ * num_nodes nodes, node 0 being that node: children (8 ids per node, -1 =
+ * none, every other node below node 0 exactly once), bounds (6 doubles per
+ * node: min x, y, z, max x, y, z), levels (name.size() - 1; a child one
+ * below its parent), sampled (nullable; 1 for a node sampled before, which
+ * the traversal does not enter: it only lends its records), counts and
+ * records: the records (bpp bytes each, positions as int32 X, Y, Z at
+ * byte 0) of the nodes without children, back to back in node order; nodes
+ * with children hold none.  rgb_offset locates the uint16 rgb triple
+ * (Poisson-average); scale / offset are Attributes::posScale / posOffset;
+ * base_spacing is the sampler's baseSpacing.  Random leaf shuffles take
+ * the "random_seed" option.
+ * Results: out_counts[i] and, back to back in node order in out_records
+ * (capacity out_capacity records), every node's records after the call;
+ * out_children (8 per node) with the children the sampler unlinked set to
+ * -1; events (2 int32 per callback: node, 0 = onNodeCompleted,
+ * 1 = onNodeDiscarded; at most 2 * num_nodes callbacks) in the order the
+ * reference calls them, and *num_events.  The records of a node unlinked
+ * without a callback (Poisson-average) are unspecified.  Poisson and
+ * random move records only (out_capacity >= sum(counts) suffices);
+ * Poisson-average adds each sampled node's accepted copies
+ * (sum(counts) * (levels spanned + 1) suffices); a smaller capacity fails
+ * with the needed size in the error text. */
+int pcg_sample_subtree(const char* method, int32_t num_nodes, const int32_t* children, const double* bounds,
+                       const int32_t* levels, const uint8_t* sampled, const int64_t* counts, const void* records,
+                       int32_t bpp, int32_t rgb_offset, const double scale[3], const double offset[3],
+                       double base_spacing, int64_t* out_counts, void* out_records, int64_t out_capacity,
+                       int32_t* out_children, int32_t* events, int32_t* num_events);
+
 /* Device memory helpers for callers without a HIP runtime binding.
  * kind: 0 host->device, 1 device->host, 2 device->device. */
 int pcg_device_alloc(void** ptr, uint64_t bytes);

@@ -38,3 +38,31 @@ echo "built $OUT/ref_indexer"
 # needs no generated header) behind a small decode driver
 g++ -O2 -std=gnu++17 -w -I"$CONV/libs/laszip/src" -o "$OUT/ref_point_decode" "$HERE/ref_point_decode.cpp"
 echo "built $OUT/ref_point_decode"
+# The samplers alone (header-only) behind a subtree driver: one
+# Sampler::sample call on a subtree read from a file, callbacks recorded
+g++ $CXXFLAGS -o "$OUT/ref_sampler" "$HERE/ref_sampler.cpp" "$OUT/obj/unsuck_platform_specific.o" \
+  "$OUT/obj/logger.o" -lpthread
+echo "built $OUT/ref_sampler"
+# The reference indexer with the GPU samplers of INTEGRATION.md plugged in
+# (its PotreeGpuSampler.h block, extracted here), linked to the library
+LIBDIR="$HERE/../potreeconverter_amd"
+awk '/^\/\/ PotreeGpuSampler.h/{on=1} on&&/^```/{on=0} on' "$HERE/../INTEGRATION.md" > "$OUT/PotreeGpuSampler.h"
+if [ -f "$LIBDIR/libpotree_gpu.so" ]; then
+  g++ $CXXFLAGS -DREF_GPU_SAMPLERS -I"$OUT" -I"$HERE/../include" -c "$HERE/ref_driver.cpp" -o "$OUT/obj/ref_driver_gpu.o"
+  gpu_objs=()
+  for o in "${objs[@]}"; do [ "$(basename "$o")" = ref_driver.o ] || gpu_objs+=("$o"); done
+  g++ -o "$OUT/ref_indexer_gpu" "${gpu_objs[@]}" "$OUT/obj/ref_driver_gpu.o" -L"$LIBDIR" -lpotree_gpu \
+    -Wl,-rpath,'$ORIGIN/../../potreeconverter_amd' -lpthread
+  echo "built $OUT/ref_indexer_gpu"
+fi
+# The same glue over a CPU stand-in of pcg_sample_subtree made of the
+# reference samplers (host code only, AddressSanitizer)
+g++ -O1 -g -std=gnu++20 -w -fsanitize=address $INC -DREF_GPU_SAMPLERS -I"$OUT" -I"$HERE/../include" \
+  -c "$HERE/ref_driver.cpp" -o "$OUT/obj/ref_driver_stub.o"
+g++ -O1 -g -std=gnu++20 -w -fsanitize=address $INC -I"$HERE/../include" -c "$HERE/ref_sampler_stub.cpp" \
+  -o "$OUT/obj/ref_sampler_stub.o"
+stub_objs=()
+for o in "${objs[@]}"; do [ "$(basename "$o")" = ref_driver.o ] || stub_objs+=("$o"); done
+g++ -fsanitize=address -o "$OUT/ref_indexer_stub" "${stub_objs[@]}" "$OUT/obj/ref_driver_stub.o" \
+  "$OUT/obj/ref_sampler_stub.o" -lpthread
+echo "built $OUT/ref_indexer_stub"

@@ -3,6 +3,9 @@
 // directory.  It is compiled against the reference sources in place by
 // oracle/build_ref.sh and lives only in oracle/_ref/ (never shipped).
 //   ref_indexer <outdir> <poisson|poisson_average|random> <name> <pointsTotal> [DEFAULT|BROTLI]
+// Built a second time with REF_GPU_SAMPLERS as ref_indexer_gpu: the same
+// reference indexer with the GPU samplers plugged in through the Sampler
+// interface (libpotree_gpu.so, pcg_sample_subtree).
 #include <stdlib.h>
 
 #include <iostream>
@@ -15,6 +18,14 @@
 #include "sampler_poisson.h"
 #include "sampler_poisson_average.h"
 #include "sampler_random.h"
+#ifdef REF_GPU_SAMPLERS
+// the GPU samplers of the integration guide (PotreeGpuSampler.h, taken
+// from INTEGRATION.md by oracle/build_ref.sh) in place of the reference's
+#include "PotreeGpuSampler.h"
+#define SamplerPoisson SamplerPoissonGPU
+#define SamplerPoissonAverage SamplerPoissonAverageGPU
+#define SamplerRandom SamplerRandomGPU
+#endif
 
 // SamplerRandom seeds every leaf shuffle with system_clock::now()
 // (sampler_random.h:102-104).  This driver fixes the clock at REF_CLOCK_NS
@@ -46,6 +57,11 @@ int main(int argc, char** argv) {
   options.encoding = argc > 5 ? argv[5] : "DEFAULT";
   options.keepChunks = true;
   logger::addOutputFile(outdir + "/log.txt");
+#ifdef REF_GPU_SAMPLERS
+  // the leaf shuffles of the GPU SamplerRandom take the library's seed
+  const char* clk = getenv("REF_CLOCK_NS");
+  if (pcg_set_option("random_seed", (int64_t)(uint32_t)(clk ? strtoll(clk, nullptr, 10) : 0)) != 0) return 3;
+#endif
   if (method == "poisson") {
     SamplerPoisson sampler;
     indexer::doIndexing(outdir, state, options, sampler);

@@ -69,6 +69,10 @@ def lib():
         L.pcg_chunkgen_fill.restype = None
         L.pcg_chunkgen_write.argtypes = [c.c_void_p, c.c_char_p]
         L.pcg_chunkgen_device.argtypes = [c.c_void_p, c.c_void_p, c.c_char_p, c.c_int64]
+        L.pcg_sample_subtree.argtypes = [c.c_char_p, c.c_int32, c.c_void_p, c.c_void_p, c.c_void_p, c.c_void_p,
+                                         c.c_void_p, c.c_void_p, c.c_int32, c.c_int32, c.POINTER(c.c_double),
+                                         c.POINTER(c.c_double), c.c_double, c.c_void_p, c.c_void_p, c.c_int64,
+                                         c.c_void_p, c.c_void_p, c.POINTER(c.c_int32)]
         L.pcg_release.argtypes = []
         L.pcg_set_option.argtypes = [c.c_char_p, c.c_int64]
         for name in ("pcg_set_option", "pcg_release", "pcg_convert", "pcg_do_chunking", "pcg_do_indexing", "pcg_count_cells", "pcg_count_cells_noalloc", "pcg_convert_device",
@@ -77,7 +81,7 @@ def lib():
                      "pcg_lasgen_configure",
                      "pcg_lasgen_write", "pcg_lasgen_device", "pcg_lasgen_header", "pcg_init",
                      "pcg_do_indexing_device", "pcg_chunkgen_params", "pcg_chunkgen_counts", "pcg_chunkgen_write",
-                     "pcg_chunkgen_device"):
+                     "pcg_chunkgen_device", "pcg_sample_subtree"):
             getattr(L, name).restype = c.c_int
         _lib = L
     return _lib
@@ -166,6 +170,42 @@ def do_indexing_device(d_records, names, counts, metadata, method="poisson", enc
                                         encoding.encode() if encoding else None, name.encode(),
                                         str(outdir).encode() if outdir else None, st))
     return list(st)
+
+
+def sample_subtree(method, children, bounds, levels, counts, records, bpp, rgb_offset, scale, offset,
+                   base_spacing, sampled=None):
+    """One Sampler::sample call on a flattened subtree (node 0 = the node
+    handed to sample()).  children: 8 ids per node (-1 none); bounds: min
+    xyz + max xyz per node; levels; counts and records (bytes, node order) of
+    the nodes without children; sampled: 1 for holders sampled before.
+    Returns (counts, records per node, children per node, events), events
+    as (node, 0 = completed | 1 = discarded) in callback order."""
+    n = len(levels)
+    I32, I64, F64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    ch = (I32 * (8 * n))(*[x for row in children for x in row])
+    bd = (F64 * (6 * n))(*[x for row in bounds for x in row])
+    lv = (I32 * n)(*levels)
+    sm = (ctypes.c_uint8 * n)(*(sampled or [0] * n))
+    cn = (I64 * n)(*counts)
+    total = sum(counts)
+    # Poisson / random only move records; Poisson-average copies the
+    # accepted ones into every sampled node
+    height = max(levels) - min(levels) + 1
+    cap = total * (height + 1) if method == "poisson_average" else total
+    out = ctypes.create_string_buffer(max(cap * bpp, 1))
+    oc = (I64 * n)()
+    och = (I32 * (8 * n))()
+    ev = (I32 * (4 * n))()
+    ne = I32()
+    _check(lib().pcg_sample_subtree(method.encode(), n, ch, bd, lv, sm, cn, records, bpp, rgb_offset,
+                                    (F64 * 3)(*scale), (F64 * 3)(*offset), base_spacing, oc, out, cap, och, ev,
+                                    ctypes.byref(ne)))
+    recs, o = [], 0
+    for i in range(n):
+        recs.append(out.raw[o * bpp:(o + oc[i]) * bpp])
+        o += oc[i]
+    return (list(oc), recs, [list(och[8 * i:8 * i + 8]) for i in range(n)],
+            [(ev[2 * e], ev[2 * e + 1]) for e in range(ne.value)])
 
 
 def convert(las_path, outdir, method="poisson", keep_chunks=False, encoding=None):

@@ -10,6 +10,7 @@
 #include "converter.h"
 #include "indexing.h"
 #include "lasgen.h"
+#include "sampler.h"
 
 namespace fs = std::filesystem;
 using namespace pcg;
@@ -298,6 +299,122 @@ int pcg_do_indexing_device(const void* d_records, int32_t num_chunks, const char
   dc.release();
   r.release();
   return rc;
+}
+
+int pcg_sample_subtree(const char* method, int32_t num_nodes, const int32_t* children, const double* bounds,
+                       const int32_t* levels, const uint8_t* sampled, const int64_t* counts, const void* records,
+                       int32_t bpp, int32_t rgb_offset, const double scale[3], const double offset[3],
+                       double base_spacing, int64_t* out_counts, void* out_records, int64_t out_capacity,
+                       int32_t* out_children, int32_t* events, int32_t* num_events) {
+  if (ensure_device()) return 1;
+  const std::string m = method ? method : "";
+  if (m != "poisson" && m != "poisson_average" && m != "random") {
+    set_error("unknown sampling method: " + m);
+    return 1;
+  }
+  if (num_nodes < 1 || !children || !bounds || !levels || !counts || !scale || !offset || !out_counts ||
+      !out_children || !events || !num_events) {
+    set_error("pcg_sample_subtree: missing argument");
+    return 1;
+  }
+  if (bpp < 12 || bpp > 4096 || (m == "poisson_average" && (rgb_offset < 12 || rgb_offset + 6 > bpp))) {
+    set_error("pcg_sample_subtree: bytes per point " + std::to_string(bpp) + " / rgb offset " +
+              std::to_string(rgb_offset) + " out of range");
+    return 1;
+  }
+  // a tree rooted at node 0: one parent per node, levels one apart, records
+  // only at the nodes without children
+  std::vector<int32_t> parent(num_nodes, -1);
+  std::vector<int64_t> offsets(num_nodes, 0);
+  int64_t total = 0;
+  for (int32_t i = 0; i < num_nodes; i++) {
+    bool inner = false;
+    for (int c = 0; c < 8; c++) {
+      const int32_t ch = children[8 * i + c];
+      if (ch == -1) continue;
+      if (ch <= 0 || ch >= num_nodes || parent[ch] != -1 || levels[ch] != levels[i] + 1) {
+        set_error("pcg_sample_subtree: bad child " + std::to_string(ch) + " of node " + std::to_string(i));
+        return 1;
+      }
+      parent[ch] = i;
+      inner = true;
+    }
+    if (counts[i] < 0 || (inner && counts[i] != 0) || levels[i] < 0 || levels[i] > 60) {
+      set_error("pcg_sample_subtree: node " + std::to_string(i) + " has " + std::to_string(counts[i]) +
+                " records at level " + std::to_string(levels[i]) + " (only nodes without children hold records)");
+      return 1;
+    }
+    offsets[i] = total;
+    total += counts[i];
+  }
+  for (int32_t i = 1; i < num_nodes; i++) {
+    int32_t a = i, steps = 0;
+    while (a > 0 && steps <= num_nodes) { a = parent[a]; steps++; }
+    if (a != 0) {
+      set_error("pcg_sample_subtree: node " + std::to_string(i) + " is not below node 0");
+      return 1;
+    }
+  }
+  if (total > 0 && !records) {
+    set_error("pcg_sample_subtree: missing records");
+    return 1;
+  }
+  DeviceChunks dc;
+  dc.bpp = bpp;
+  dc.numPoints = total;
+  dc.pointsTotal = total;
+  for (int a = 0; a < 3; a++) {
+    dc.attributes.posScale[a] = scale[a];
+    dc.attributes.posOffset[a] = offset[a];
+  }
+  dc.attributes.bytes = bpp;
+  dc.d_records = (uint8_t*)workspace(WS_RECORDS, (size_t)total * bpp + 64);
+  if (!dc.d_records) return 1;
+  if (total > 0) PCG_CHECK(stream_copy(dc.d_records, records, (size_t)total * bpp, hipMemcpyHostToDevice));
+  SubtreeSpec spec;
+  spec.numNodes = num_nodes;
+  spec.children = children;
+  spec.bounds = bounds;
+  spec.levels = levels;
+  spec.sampled = sampled;
+  spec.counts = counts;
+  spec.offsets = offsets.data();
+  spec.baseSpacing = base_spacing;
+  spec.rgbOffset = rgb_offset;
+  IndexResult r;
+  SubtreeResult res;
+  int rc = gpu_indexing(dc, m, "", total, r, "DEFAULT", &spec, &res);
+  if (rc) return rc;
+  // every node's records back to back in node order, gathered on the GPU
+  int64_t outTotal = 0;
+  std::vector<CopyDesc> copies;
+  for (int32_t i = 0; i < num_nodes; i++) {
+    if (res.count[i] > 0)
+      copies.push_back(CopyDesc{res.ptr[i], (uint64_t)outTotal * bpp, (uint64_t)res.count[i] * bpp});
+    outTotal += res.count[i];
+  }
+  if (outTotal > out_capacity || (outTotal > 0 && !out_records)) {
+    set_error("pcg_sample_subtree: the results hold " + std::to_string(outTotal) + " records, the output buffer " +
+              std::to_string(out_capacity));
+    return 1;
+  }
+  if (outTotal > 0) {
+    hipStream_t s = device().stream;
+    uint8_t* d_out = (uint8_t*)workspace(WS_OCTREE, (size_t)outTotal * bpp + 64);
+    CopyDesc* d_desc = (CopyDesc*)workspace(WS_COPYDESC, copies.size() * sizeof(CopyDesc) + 16);
+    if (!d_out || !d_desc) return 1;
+    PCG_CHECK(hipMemcpyAsync(d_desc, copies.data(), copies.size() * sizeof(CopyDesc), hipMemcpyHostToDevice, s));
+    if (gather_ranges(d_desc, (int64_t)copies.size(), d_out, s)) return 1;
+    PCG_CHECK(stream_copy(out_records, d_out, (size_t)outTotal * bpp, hipMemcpyDeviceToHost));
+  }
+  for (int32_t i = 0; i < num_nodes; i++) out_counts[i] = res.count[i];
+  memcpy(out_children, res.children.data(), sizeof(int32_t) * 8 * (size_t)num_nodes);
+  for (size_t e = 0; e < res.events.size(); e++) {
+    events[2 * e] = res.events[e].first;
+    events[2 * e + 1] = res.events[e].second;
+  }
+  *num_events = (int32_t)res.events.size();
+  return 0;
 }
 
 int pcg_device_alloc(void** ptr, uint64_t bytes) {

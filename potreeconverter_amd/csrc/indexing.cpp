@@ -48,6 +48,7 @@ struct TNode {
   int64_t count = 0;     // leaf records: count
   bool chunkRoot = false;
   bool upper = false;    // root or intermediate node above the chunk roots
+  bool presampled = false;  // holds records sampled before (a subtree's holder not visited)
   bool hasChildren() const {
     for (int c = 0; c < 8; c++) if (children[c] >= 0) return true;
     return false;
@@ -170,7 +171,7 @@ void IndexResult::release() {
 }
 
 int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string& name, int64_t pointsTotal,
-                 IndexResult& out, const std::string& encoding) {
+                 IndexResult& out, const std::string& encoding, const SubtreeSpec* sub, SubtreeResult* subOut) {
   if (ensure_device()) return 1;
   hipStream_t s = device().stream;
   const double t0 = now_s();
@@ -193,7 +194,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
       if (!std::isfinite(a.max[i])) a.max[i] = INFINITY;
     }
   const int32_t bpp = dc.bpp;
-  const int rgbOffset = attributes.getOffset("rgb");
+  const int rgbOffset = sub ? sub->rgbOffset : attributes.getOffset("rgb");
   if (avg && rgbOffset < 0) {
     set_error("poisson_average needs an rgb attribute");
     return 1;
@@ -210,11 +211,52 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     std::vector<TNode>& v;
     ~GiveBack() { t.nodes.swap(v); }
   } giveBack{tree, nodesMem.v};
-  tree.add("r", dc.min, dc.max, -1);
-  tree.nodes[0].upper = true;
   std::vector<int32_t> chunkRoots;
   std::vector<int64_t> chunkRootPoints;
-  for (const auto& c : dc.chunks) {
+  if (sub) {
+    // the caller's subtree: node 0 takes the place of the single chunk root
+    // (nothing above it), the nodes without children hold records
+    tree.nodes.resize(sub->numNodes);
+    for (int32_t i = 0; i < sub->numNodes; i++) {
+      TNode& t = tree.nodes[i];
+      t = TNode();
+      t.level = sub->levels[i];
+      for (int a = 0; a < 3; a++) {
+        t.min[a] = sub->bounds[6 * i + a];
+        t.max[a] = sub->bounds[6 * i + 3 + a];
+      }
+      for (int c = 0; c < 8; c++) t.children[c] = sub->children[8 * i + c];
+      if (!t.hasChildren()) {
+        t.off = sub->offsets[i];
+        t.count = sub->counts[i];
+        t.presampled = sub->sampled && sub->sampled[i];
+      }
+    }
+    // keys from the child digits (node 0 is "r"), for hierarchy planning
+    std::vector<int32_t> stack{0};
+    tree.nodes[0].key = 1;
+    while (!stack.empty()) {
+      const int32_t id = stack.back();
+      stack.pop_back();
+      for (int c = 0; c < 8; c++) {
+        const int32_t ch = tree.nodes[id].children[c];
+        if (ch < 0) continue;
+        tree.nodes[ch].parent = id;
+        const uint64_t k = tree.nodes[id].key;
+        tree.nodes[ch].key = (k && k < (1ull << 60)) ? ((k << 3) | (uint64_t)c) : 0;
+        if (!tree.nodes[ch].key) tree.nodes[ch].name = "r";
+        stack.push_back(ch);
+      }
+    }
+    tree.nodes[0].chunkRoot = true;
+    chunkRoots.push_back(0);
+    chunkRootPoints.push_back(N);
+  } else {
+    tree.add("r", dc.min, dc.max, -1);
+    tree.nodes[0].upper = true;
+  }
+  static const std::vector<ChunkDesc> kNoChunks;
+  for (const auto& c : sub ? kNoChunks : dc.chunks) {
     int32_t id;
     if (c.id == "r") {
       id = 0;
@@ -244,7 +286,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   struct Seg { int32_t node; int64_t start, count; Vec3 mn, mx; };
   std::vector<Seg> segs;
   for (int32_t id : chunkRoots)
-    if (tree.nodes[id].count >= kLeafCap)
+    if (!sub && tree.nodes[id].count >= kLeafCap)
       segs.push_back({id, tree.nodes[id].off, tree.nodes[id].count, tree.nodes[id].min, tree.nodes[id].max});
   if (!segs.empty()) {
     if (bufB.alloc((size_t)N * bpp + 64) || keysA.alloc((size_t)N * 2 + 64) || keysB.alloc((size_t)N * 2 + 64) ||
@@ -474,7 +516,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   if (rnd) {
     uint64_t outBytes = 0, idxBytes = 0;
     for (int32_t i = 0; i < numNodes; i++) {
-      if (tree.nodes[i].hasChildren() || state[i].count <= 0) continue;
+      if (tree.nodes[i].hasChildren() || tree.nodes[i].presampled || state[i].count <= 0) continue;
       const int64_t c = state[i].count;
       shuffle.push_back(ShuffleLeaf{state[i].ptr, (uint8_t*)(uintptr_t)outBytes,
                                     (uint32_t*)(uintptr_t)(c > kShuffleLdsPoints ? idxBytes + 1 : 0), c});
@@ -486,7 +528,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
       return 1;
     size_t q = 0;
     for (int32_t i = 0; i < numNodes; i++) {
-      if (tree.nodes[i].hasChildren() || state[i].count <= 0) continue;
+      if (tree.nodes[i].hasChildren() || tree.nodes[i].presampled || state[i].count <= 0) continue;
       ShuffleLeaf& sl = shuffle[q++];
       sl.dst = shufOut.as<uint8_t>() + (uintptr_t)sl.dst;
       sl.idx = sl.idx ? (uint32_t*)(shufIdx.as<uint8_t>() + ((uintptr_t)sl.idx - 1)) : nullptr;
@@ -528,7 +570,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   SampleParams sp;
   memset(&sp, 0, sizeof(sp));
   for (int a = 0; a < 3; a++) { sp.scale[a] = attributes.posScale[a]; sp.offset[a] = attributes.posOffset[a]; }
-  const double spacing = (dc.max - dc.min).x / 128.0;  // indexer.cpp:1603
+  const double spacing = sub ? sub->baseSpacing : (dc.max - dc.min).x / 128.0;  // indexer.cpp:1603
   sp.base_spacing = spacing;
   sp.bpp = bpp;
   sp.rgb_offset = rgbOffset;
@@ -681,7 +723,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   };
   if (G == 1) {
     if (run_levels(0, gm[0], gp[0], s)) return 1;
-    walk_phase1();
+    if (!sub) walk_phase1();
   } else {
     std::vector<hipStream_t> gs;
     if (group_streams(G, gs)) return 1;
@@ -732,6 +774,55 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   }
   const double ts1 = now_s();
   out.t_sample = ts1 - tb1;
+
+  if (sub) {
+    // Sampler::sample's callbacks (sampler_poisson.h:247-294,
+    // sampler_random.h:184-225, sampler_poisson_average.h:418-482): in
+    // post-order over the visited nodes, each inner node's children in
+    // index order.  Poisson / random: a child left without points that is a
+    // leaf at that moment (its own children may have been unlinked) is
+    // discarded and unlinked, then completed all the same (the first `if`
+    // at sampler_poisson.h:264 has no `else`); Poisson-average: a child
+    // without rejected points is unlinked without a callback.
+    SubtreeResult& R = *subOut;
+    R.ptr.assign(numNodes, nullptr);
+    R.count.assign(numNodes, 0);
+    R.children.assign((size_t)8 * numNodes, -1);
+    R.events.clear();
+    for (int32_t i = 0; i < numNodes; i++) {
+      R.ptr[i] = state[i].ptr;
+      R.count[i] = state[i].count;
+      for (int c = 0; c < 8; c++) R.children[8 * i + c] = tree.nodes[i].children[c];
+    }
+    std::function<void(int32_t)> visit = [&](int32_t id) {
+      const TNode& t = tree.nodes[id];
+      if (!t.hasChildren()) return;
+      for (int c = 0; c < 8; c++)
+        if (t.children[c] >= 0) visit(t.children[c]);
+      for (int c = 0; c < 8; c++) {
+        const int32_t ch = t.children[c];
+        if (ch < 0) continue;
+        const NodeState& cs = state[ch];
+        if (avg) {
+          if (cs.flags & NS_DISCARDED) R.children[8 * id + c] = -1;
+          else R.events.emplace_back(ch, 0);
+          continue;
+        }
+        bool leafNow = true;
+        for (int d = 0; d < 8; d++)
+          if (R.children[8 * ch + d] >= 0) leafNow = false;
+        if (cs.count == 0 && leafNow) {
+          R.events.emplace_back(ch, 1);
+          R.children[8 * id + c] = -1;
+        }
+        R.events.emplace_back(ch, 0);
+      }
+    };
+    visit(0);
+    out.numSampled = sampled;
+    out.t_total = now_s() - t0;
+    return 0;
+  }
 
   // --------------------------------------------- canonical completion order
   // phase 1 was walked while the GPU sampled; keep the children that

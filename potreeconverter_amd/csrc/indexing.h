@@ -55,11 +55,41 @@ extern int32_t g_sample_groups;
 // reference's choice, the system clock at the start of indexing.
 extern int64_t g_random_seed;
 
+// One Sampler::sample call (Converter/include/structures.h:152-164) on a
+// caller-built subtree instead of a tree built from chunks: node 0 is the
+// node handed to sample(); nodes without children hold records (leaves, and
+// nodes sampled before, which the traversal of sampler_poisson.h:27-36 does
+// not enter).  Node i holds counts[i] records from dc.d_records +
+// offsets[i] * bpp.
+struct SubtreeSpec {
+  int32_t numNodes = 0;
+  const int32_t* children = nullptr;  // 8 per node, -1 = none
+  const double* bounds = nullptr;     // 6 per node: min x, y, z, max x, y, z
+  const int32_t* levels = nullptr;    // name.size() - 1 of every node
+  const uint8_t* sampled = nullptr;   // nullable: 1 = sampled before (not visited)
+  const int64_t* counts = nullptr;
+  const int64_t* offsets = nullptr;
+  double baseSpacing = 0;
+  int32_t rgbOffset = -1;
+};
+// Results: every node's records after the call (device pointers into the
+// library's buffers, valid until the next call), the children that stay
+// linked, and the callbacks in the reference's order: kind 0 =
+// onNodeCompleted, 1 = onNodeDiscarded.
+struct SubtreeResult {
+  std::vector<const uint8_t*> ptr;
+  std::vector<int64_t> count;
+  std::vector<int32_t> children;
+  std::vector<std::pair<int32_t, int32_t>> events;
+};
+
 // encoding: "BROTLI" compresses every node (indexer.cpp:1110-1401); any
 // other value writes the records unchanged and is echoed in metadata.json
-// (indexer.cpp:516, 1462).
+// (indexer.cpp:516, 1462).  With `sub` the tree is the given subtree and the
+// call ends after sampling with `subOut` filled (no octree image).
 int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string& name, int64_t pointsTotal,
-                 IndexResult& out, const std::string& encoding = "DEFAULT");
+                 IndexResult& out, const std::string& encoding = "DEFAULT", const SubtreeSpec* sub = nullptr,
+                 SubtreeResult* subOut = nullptr);
 
 int write_index_files(const IndexResult& r, const std::string& targetDir);
 

@@ -22,6 +22,10 @@ level-4 chunks "rABCD" with log-normal point counts and three extra-bytes
 attributes) are written as a chunks/ directory by the library's host
 generator and indexed by the reference indexer directly.
 
+Subtree cases (tests/subtrees.py) pin one Sampler::sample call: the
+reference samplers run on a flattened subtree by oracle/_ref/ref_sampler,
+which records the callbacks.
+
 Run from the repository root:  python tests/golden/make_golden.py [case names]
 """
 import json
@@ -35,7 +39,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
-from helpers import REF_INDEXER, digest_output, make_las, oracle_convert  # noqa: E402
+from helpers import REF_INDEXER, REF_SAMPLER, digest_output, make_las, oracle_convert  # noqa: E402
 from scenes import make_scene  # noqa: E402
 
 CASES = [
@@ -79,6 +83,38 @@ CHUNKGEN_CASES = [
     ("c2_512chunks_poisson_average", dict(num_chunks=512, median=2000, sigma=1.16, min_count=100,
                                           max_count=200_000, seed=7), "poisson_average"),
 ]
+
+# one Sampler::sample call on a subtree of tests/subtrees.py: name, subtree,
+# method, clock of the random leaf shuffles
+SUBTREE_CASES = [
+    ("subtree_chunk_poisson", "chunk", "poisson", 0),
+    ("subtree_chunk_poisson_average", "chunk", "poisson_average", 0),
+    ("subtree_chunk_random", "chunk", "random", 4242),
+    ("subtree_upper_poisson", "upper", "poisson", 0),
+    ("subtree_upper_poisson_average", "upper", "poisson_average", 0),
+    ("subtree_upper_random", "upper", "random", 77),
+    ("subtree_sparse_poisson", "sparse", "poisson", 0),
+    ("subtree_sparse_poisson_average", "sparse", "poisson_average", 0),
+]
+
+
+def write_subtree_fixture(td, name, subtree, method, clock):
+    import hashlib
+    from subtrees import make_subtree, parse_reference_output
+    t = make_subtree(subtree)
+    src, dst = os.path.join(td, name + ".in"), os.path.join(td, name + ".out")
+    t.input_file(src)
+    subprocess.run([REF_SAMPLER, method, src, dst], check=True, env=dict(os.environ, REF_CLOCK_NS=str(clock)))
+    nodes, events = parse_reference_output(dst, len(t.levels))
+    fixture = {
+        "case": {"subtree": subtree, "method": method, "random_seed": clock},
+        "producer": "reference samplers (Converter/include/sampler_*.h), one Sampler::sample call",
+        "nodes": [[c, ch, hashlib.sha256(r).hexdigest()] for c, ch, sm, r in nodes],
+        "events": events,
+    }
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(fixture, f, indent=0, sort_keys=True)
+    print(name, len(nodes), "nodes", len(events), "events", sum(e[1] for e in events), "discarded")
 
 
 def write_fixture(td, name, case, las, n, method, encoding="DEFAULT", random_seed=None, chunkgen=None):
@@ -132,6 +168,10 @@ def main():
             g = ChunkGen(**args)
             case = {"chunkgen": args, "points": g.total, "method": method, "encoding": "DEFAULT"}
             write_fixture(td, name, case, None, g.total, method, chunkgen=g)
+        for name, subtree, method, clock in SUBTREE_CASES:
+            if only and name not in only:
+                continue
+            write_subtree_fixture(td, name, subtree, method, clock)
     finally:
         shutil.rmtree(td, ignore_errors=True)
 

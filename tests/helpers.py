@@ -8,6 +8,7 @@ from potree_format import canonical_nodes
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
 REF_INDEXER = os.path.join(ROOT, "oracle", "_ref", "ref_indexer")
+REF_SAMPLER = os.path.join(ROOT, "oracle", "_ref", "ref_sampler")
 REF_POINT_DECODE = os.path.join(ROOT, "oracle", "_ref", "ref_point_decode")
 _oracle = None
 
