@@ -35,15 +35,17 @@ STATS = 24
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--config", default="c1", choices=["c1", "c2"],
-                    help="c1: 200M-point LAS tile, chunking + indexing; c2: indexing of 4096 pre-chunked chunks")
+    ap.add_argument("--config", default="c1", choices=["c1", "c2", "c3"],
+                    help="c1: 200M-point LAS tile, chunking + indexing; c2: indexing of 4096 pre-chunked chunks; "
+                         "c3: the precision scene (offset 1e6 m, scale 0.0001, near-duplicate clusters, lattices "
+                         "at the node spacing) at 1B points, chunking + indexing")
     ap.add_argument("--c2-plain-median", type=int, default=2_500,
                     help="c2: median chunk size of the batched-vs-plain comparison (0 skips it)")
     ap.add_argument("--sample-groups", type=int, default=None,
                     help="library option sample_groups (streams of the chunk-subtree sampling; default: the library's)")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--points", type=int, default=200_000_000)
+    ap.add_argument("--points", type=int, default=None, help="default: 200M (c1), 1B (c3)")
     ap.add_argument("--method", default=None, choices=["poisson", "poisson_average", "random"],
                     help="default: poisson (c1), poisson_average (c2)")
     ap.add_argument("--encoding", default="DEFAULT",
@@ -60,7 +62,7 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(points, seed, method):
+def cpu_baseline(points, seed, method, preset=1, total=200_000_000):
     """Times the CPU oracle on the first `points` records of the same stream."""
     import potreeconverter_amd as pc
 
@@ -69,13 +71,13 @@ def cpu_baseline(points, seed, method):
         return None
     O = ctypes.CDLL(ocl)
     O.orc_convert.argtypes = [ctypes.c_char_p] * 3 + [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
-    g = pc.LasGen(1, 200_000_000, seed=seed)
+    g = pc.LasGen(preset, total, seed=seed)
     g.n = points
-    # generate the prefix of the 200M-point stream and write it as its own LAS file
+    # generate the prefix of the full stream and write it as its own LAS file
     with tempfile.TemporaryDirectory() as td:
         las = os.path.join(td, "sample.las")
-        ctypes.c_int64.from_buffer(g.buf, 8).value = points  # LasGenParams.n (tiling stays that of 200M)
-        # keep the 200M tiling: tiles_per_side / num_tiles were fixed by the preset call
+        ctypes.c_int64.from_buffer(g.buf, 8).value = points  # LasGenParams.n (tiling stays that of the full set)
+        # keep the full set's tiling: tiles_per_side / num_tiles were fixed by the preset call
         pc.lib().pcg_lasgen_write(g.buf, las.encode())
         t = (ctypes.c_double * 8)()
         out = os.path.join(td, "out")
@@ -263,8 +265,12 @@ def main():
     import potreeconverter_amd as pc
 
     L = pc.lib()
-    n = args.points
-    g = pc.LasGen(1, n, seed=args.seed)
+    c3 = args.config == "c3"
+    preset = 2 if c3 else 1
+    n = args.points or (1_000_000_000 if c3 else 200_000_000)
+    if c3 and args.plain_points == 20_000_000:
+        args.plain_points = 0  # the per-node plain path is measured on the c1 workload
+    g = pc.LasGen(preset, n, seed=args.seed)
     rl = g.record_length
     d_las = ctypes.c_void_p()
     pc._check(L.pcg_device_alloc(ctypes.byref(d_las), n * rl + 64))
@@ -336,9 +342,26 @@ def main():
     plain = None
     if args.plain_points > 0:
         plain = plain_vs_batched(min(args.plain_points, n), args.seed, args.method)
-    cpu = None if args.no_cpu else cpu_baseline(min(args.cpu_points, n), args.seed, args.method)
+    cpu = None if args.no_cpu else cpu_baseline(min(args.cpu_points, n), args.seed, args.method, preset, n)
+    workload = (f"{n} pts synthetic LAS PDRF2 5x5 km (offset 500000,5000000,0; scale 0.001), "
+                f"terrain fBm + 20 buildings, seed {args.seed}")
+    metric = "Mpts/s count-sort+octree+Poisson at 200M pts, batched vs plain; % HBM peak"
+    if c3:
+        workload = (f"{n} pts precision scene PDRF2 (offset 1e6 m, scale 0.0001; 10% near-duplicate clusters "
+                    f"jittered +-1 unit, 5% on a lattice whose pitch is the level-4 node spacing), seed {args.seed}")
+        metric = "Mpts/s count-sort+octree+Poisson at 1B pts (precision scene); FP64 re-check fraction"
+    precision = None
+    if c3 or args.method != "random":
+        keys = (8, 9, 10, 11, 12, 14, 15, 7, 17, 18)
+        precision = {
+            "fp64_rechecks": int(last[11]), "fp32_pair_tests": int(last[12]),
+            "recheck_fraction": last[11] / last[12] if last[12] else None,
+            "max_fp32_rel_err": last[13],
+            "points_conserved": (int(last[7]) == n * bpp_out) if args.method == "poisson" else None,
+            "counters_equal_over_steps": all(all(p[k] == per[0][k] for k in keys) for p in per),
+        }
     line = {
-        "metric": "Mpts/s count-sort+octree+Poisson at 200M pts, batched vs plain; % HBM peak",
+        "metric": metric,
         "value": round(value, 3),
         "unit": "Mpts/s",
         "n_gpus": 1,
@@ -351,8 +374,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{n} pts synthetic LAS PDRF2 5x5 km (offset 500000,5000000,0; scale 0.001), "
-                        f"terrain fBm + 20 buildings, seed {args.seed}",
+            "workload": workload,
             "points": n,
             "method": args.method,
             "encoding": args.encoding,
@@ -382,6 +404,7 @@ def main():
             "algorithmic_bytes": chunk_bytes,
             "kernel_ms": round(kern_s * 1e3, 4),
         },
+        "precision": precision,
         "plain_vs_batched": plain,
         "cpu_baseline": cpu,
     }

@@ -109,6 +109,14 @@ int pcg_set_option(const char* key, int64_t value) {
     g_sample_groups = (int32_t)value;
     return 0;
   }
+  if (k == "sample_sets") {
+    if (value < 0 || value > kMaxSampleSets) {
+      set_error("sample_sets must be in [0, " + std::to_string(kMaxSampleSets) + "]");
+      return 1;
+    }
+    g_sample_sets = (int32_t)value;
+    return 0;
+  }
   if (k == "distribute_tail") {
     g_distribute_tail = value != 0;
     return 0;

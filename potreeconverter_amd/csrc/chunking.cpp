@@ -88,6 +88,20 @@ void* workspace(int slot, size_t bytes) {
   return e.p;
 }
 
+size_t workspace_bytes(int slot) { return slot >= 0 && slot < WS_COUNT ? g_ws[slot].bytes : 0; }
+
+size_t workspace_total() {
+  size_t t = 0;
+  for (const WsEntry& e : g_ws) t += e.bytes;
+  return t;
+}
+
+void release_workspace_slot(int slot) {
+  if (slot < 0 || slot >= WS_COUNT || !g_ws[slot].p) return;
+  (void)hipFree(g_ws[slot].p);
+  g_ws[slot] = WsEntry();
+}
+
 void release_workspace() {
   for (WsEntry& e : g_ws) {
     if (e.p) (void)hipFree(e.p);

@@ -37,10 +37,16 @@ enum WorkspaceSlot : int {
   WS_GROUP0 = WS_LEVEL0 + 16,  // sampling group g, buffer k: WS_GROUP0 + 8 * g + k
   WS_COUNT = WS_GROUP0 + 8 * 8
 };
-constexpr int kMaxSampleGroups = 8;
+constexpr int kMaxSampleGroups = 8;  // streams of the chunk-subtree sampling
+constexpr int kMaxSampleSets = 64;   // sets of chunk subtrees sampled on those streams
 inline int ws_group_slot(int g, int k) { return WS_GROUP0 + 8 * g + k; }
 void* workspace(int slot, size_t bytes);
 void release_workspace();
+// bytes held by one slot / by all slots; frees one slot (its next use
+// allocates again)
+size_t workspace_bytes(int slot);
+size_t workspace_total();
+void release_workspace_slot(int slot);
 
 struct DeviceChunks {
   uint8_t* d_records = nullptr;  // numPoints * bpp bytes, grouped by chunk (ascending ID)

@@ -164,6 +164,7 @@ bool g_plain_sampling = false;
 int64_t g_random_seed = -1;
 int32_t g_ring_min_nodes = 512;
 int32_t g_sample_groups = 0;
+int32_t g_sample_sets = 0;
 
 void IndexResult::release() {
   d_octree = nullptr;  // a workspace buffer
@@ -289,6 +290,18 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     if (!sub && tree.nodes[id].count >= kLeafCap)
       segs.push_back({id, tree.nodes[id].off, tree.nodes[id].count, tree.nodes[id].min, tree.nodes[id].max});
   if (!segs.empty()) {
+    // the previous job's sampling buffers and octree image are dead: give
+    // them back first when the build buffers would not fit beside them
+    const size_t buildBytes = ((size_t)N * bpp + (size_t)N * 4) * 9 / 8 + ((size_t)1 << 30);
+    if (workspace_bytes(WS_OCT_B) < (size_t)N * bpp) {
+      size_t freeB = 0, totalB = 0;
+      PCG_CHECK(hipMemGetInfo(&freeB, &totalB));
+      if (freeB < buildBytes) {
+        release_workspace_slot(WS_ARENA);
+        release_workspace_slot(WS_OCTREE);
+        for (int k = 0; k < kMaxSampleGroups * 8; k++) release_workspace_slot(WS_GROUP0 + k);
+      }
+    }
     if (bufB.alloc((size_t)N * bpp + 64) || keysA.alloc((size_t)N * 2 + 64) || keysB.alloc((size_t)N * 2 + 64) ||
         dcounter.alloc(16))
       return 1;
@@ -461,7 +474,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   // of one set overlap the wide levels of another.  The nodes above the
   // chunk roots follow on the library stream once every set is done.  With
   // one group every sampled node is in the level lists of that group.
-  int G = 1;
+  int S = 1;  // streams of the chunk-subtree sampling
   if (!g_plain_sampling && !levelStats && chunkRoots.size() > 1) {
     int want = g_sample_groups;
     if (want <= 0) {
@@ -471,7 +484,49 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
       const int queues = q && atoi(q) > 0 ? atoi(q) : 4;
       want = std::max(1, std::min(queues - 1, kMaxSampleGroups));
     }
-    G = (int)std::min<size_t>((size_t)want, chunkRoots.size());
+    S = (int)std::min<size_t>((size_t)want, chunkRoots.size());
+  }
+  // every output range is rounded up to 16 bytes (9 ranges per node at most)
+  const uint64_t arenaCap = (uint64_t)N * bpp * (avg ? 4 : 3) + (1u << 20) + (uint64_t)numNodes * 9 * 16;
+  // Sets: the chunk subtrees are split into G >= S sets, set g sampled on
+  // stream g % S with that stream's buffers.  A stream's buffers hold one
+  // level of one set at a time (the scratch and neighbour pool restart at
+  // every level), so the sets of a stream interleave level by level and the
+  // buffers are sized by the stream's largest set.  More sets than streams
+  // bound those buffers when the job would not fit in device memory
+  // otherwise (waves of chunk subtrees); "sample_sets" forces a count.
+  int G = S;
+  if (!g_plain_sampling && !levelStats && chunkRoots.size() > 1) {
+    const int maxSets = (int)std::min<size_t>(chunkRoots.size(), (size_t)kMaxSampleSets);
+    if (g_sample_sets > 0) {
+      G = std::max(S, std::min(g_sample_sets, maxSets));
+    } else {
+      size_t freeB = 0, totalB = 0;
+      PCG_CHECK(hipMemGetInfo(&freeB, &totalB));
+      // memory the sampling reuses or may release: the group buffers, the
+      // arena and the octree image of the previous job, and the octree-build
+      // and chunking buffers of this one
+      size_t held = workspace_bytes(WS_ARENA) + workspace_bytes(WS_OCTREE);
+      for (int k = 0; k < kMaxSampleGroups * 8; k++) held += workspace_bytes(WS_GROUP0 + k);
+      static const int kRelease[] = {WS_OCT_B, WS_OCT_KEYS_A, WS_OCT_KEYS_B, WS_OCT_M, WS_OCT_SCAN,
+                                     WS_POINT_IDS, WS_CHUNK_IDS16, WS_TILE_M, WS_SCAN};
+      size_t releasable = 0;
+      for (int k : kRelease) releasable += workspace_bytes(k);
+      const double margin = 2e9 + 0.03 * (double)totalB;
+      // per-candidate buffer bytes of a stream (scratch bound + neighbour pool)
+      const int mId = avg ? METHOD_POISSON_AVERAGE : (rnd ? METHOD_RANDOM : METHOD_POISSON);
+      const double perCand = (double)sample_scratch_bound(1 << 20, 1, mId) / (1 << 20) + 16.0;
+      const double image = (double)N * bpp * (avg ? 2.0 : 1.0);
+      auto need = [&](int sets) {
+        return 1.125 * ((double)arenaCap + image + S * perCand * (double)((N + sets - 1) / sets));
+      };
+      const double avail = (double)freeB + (double)held - margin;
+      if (need(S) > avail && releasable > 0) {
+        for (int k : kRelease) release_workspace_slot(k);
+      }
+      const double avail2 = avail + (need(S) > avail ? (double)releasable : 0.0);
+      while (G < maxSets && need(G) > avail2) G = std::min(maxSets, G * 2);
+    }
   }
   std::vector<int32_t> groupOf(numNodes, G > 1 ? -1 : 0);  // -1: above the chunk roots
   std::vector<int64_t> groupPoints(G, 0);
@@ -542,8 +597,6 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
       seed = (unsigned)std::chrono::system_clock::now().time_since_epoch().count();
     if (shuffle_leaves(shufDesc.as<ShuffleLeaf>(), (int64_t)shuffle.size(), bpp, seed, s)) return 1;
   }
-  // every output range is rounded up to 16 bytes (9 ranges per node at most)
-  uint64_t arenaCap = (uint64_t)N * bpp * (avg ? 4 : 3) + (1u << 20) + (uint64_t)numNodes * 9 * 16;
   // all level lists in one upload: per group, deepest level first
   std::vector<int32_t> allLevels;
   std::vector<std::vector<size_t>> levelOffset(G + 1, std::vector<size_t>(maxLevel + 1, 0));
@@ -566,7 +619,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
           tops(ws_group_slot(g, 6)) {}
   };
   std::vector<GroupMem> gm;
-  for (int g = 0; g < G; g++) gm.emplace_back(g);
+  for (int k = 0; k < S; k++) gm.emplace_back(k);
   SampleParams sp;
   memset(&sp, 0, sizeof(sp));
   for (int a = 0; a < 3; a++) { sp.scale[a] = attributes.posScale[a]; sp.offset[a] = attributes.posOffset[a]; }
@@ -602,13 +655,22 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     p.nbr_pool_cap = poolCap;
     p.nbr_pool_top = (unsigned long long*)(m.tops.as<uint64_t>() + 2);
     // the ring needs enough nodes per launch to fill the GPU; concurrent
-    // groups share it
-    p.ring_min_nodes = std::max(1, g_ring_min_nodes / G);
+    // streams share it
+    p.ring_min_nodes = std::max(1, g_ring_min_nodes / S);
     return 0;
   };
+  // each stream's buffers for the largest of its sets
   std::vector<SampleParams> gp(G);
-  for (int g = 0; g < G; g++)
-    if (group_params(gm[g], maxLevelNodes[g], groupPoints[g], gp[g])) return 1;
+  for (int k = 0; k < S; k++) {
+    int64_t mn = 1, mc = 0;
+    for (int g = k; g < G; g += S) {
+      mn = std::max(mn, maxLevelNodes[g]);
+      mc = std::max(mc, groupPoints[g]);
+    }
+    SampleParams base;
+    if (group_params(gm[k], mn, mc, base)) return 1;
+    for (int g = k; g < G; g += S) gp[g] = base;
+  }
   if (!allLevels.empty())
     PCG_CHECK(hipMemcpyAsync(dlist.p, allLevels.data(), sizeof(int32_t) * allLevels.size(), hipMemcpyHostToDevice, s));
   PCG_CHECK(hipMemcpyAsync(dinfo.p, info.data(), sizeof(NodeInfo) * numNodes, hipMemcpyHostToDevice, s));
@@ -726,7 +788,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     if (!sub) walk_phase1();
   } else {
     std::vector<hipStream_t> gs;
-    if (group_streams(G, gs)) return 1;
+    if (group_streams(S, gs)) return 1;
     hipEvent_t ev_start = group_event(0);
     if (!ev_start) return 1;
     PCG_CHECK(hipEventRecord(ev_start, s));
@@ -735,16 +797,16 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     // never queued behind a wait for another group
     // levels are enqueued round-robin over the groups so that every group
     // starts at once
-    for (int g = 0; g < G; g++) PCG_CHECK(hipStreamWaitEvent(gs[g], ev_start, 0));
+    for (int k = 0; k < S; k++) PCG_CHECK(hipStreamWaitEvent(gs[k], ev_start, 0));
     for (int l = maxLevel; l >= 0; l--)
       for (int g = 0; g < G; g++)
-        if (run_level(g, l, gm[g], gp[g], gs[g])) return 1;
-    for (int g = 0; g < G; g++) {
-      hipEvent_t ev_done = group_event(1 + g);
+        if (run_level(g, l, gm[g % S], gp[g], gs[g % S])) return 1;
+    for (int k = 0; k < S; k++) {
+      hipEvent_t ev_done = group_event(1 + k);
       if (!ev_done) return 1;
-      PCG_CHECK(hipEventRecord(ev_done, gs[g]));
+      PCG_CHECK(hipEventRecord(ev_done, gs[k]));
     }
-    for (int g = 0; g < G; g++) PCG_CHECK(hipStreamWaitEvent(s, group_event(1 + g), 0));
+    for (int k = 0; k < S; k++) PCG_CHECK(hipStreamWaitEvent(s, group_event(1 + k), 0));
     walk_phase1();
     // the nodes above the chunk roots: their candidates are points the
     // chunk roots kept
@@ -965,6 +1027,13 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   out.t_assemble = now_s() - ts1;
   out.t_setup = tb0 - t0;
   out.t_total = now_s() - t0;
+  if (const char* mr = getenv("PCG_MEM_REPORT"); mr && mr[0] == '1') {
+    // diagnostic: device memory held by the library's buffers after the job
+    fprintf(stderr, "[memory] %.3f GB in workspace buffers:", workspace_total() / 1e9);
+    for (int k = 0; k < WS_COUNT; k++)
+      if (workspace_bytes(k) >= (size_t)1 << 26) fprintf(stderr, " slot%d=%.2fGB", k, workspace_bytes(k) / 1e9);
+    fprintf(stderr, "\n");
+  }
   return 0;
 }
 

@@ -51,6 +51,11 @@ extern int32_t g_ring_min_nodes;
 // (1 .. kMaxSampleGroups; 1: one level-synchronous pass over the whole tree;
 // 0: one set per hardware queue besides the library stream's).
 extern int32_t g_sample_groups;
+// Sets of chunk subtrees (sample_groups streams take them in turn, each
+// with buffers for its largest set): 0 = as many as streams unless the job
+// needs more sets to fit in device memory; n = max(streams, n), at most
+// kMaxSampleSets and the chunk count.
+extern int32_t g_sample_sets;
 // Seed of every SamplerRandom leaf shuffle (0 .. 2^32-1), or -1 for the
 // reference's choice, the system clock at the start of indexing.
 extern int64_t g_random_seed;

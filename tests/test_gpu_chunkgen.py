@@ -117,6 +117,31 @@ def test_full_chunk_grid_matches_oracle(method, tmp_path):
         assert filecmp.cmp(gpu / f, gpu2 / f, shallow=False), f
 
 
+def test_sample_sets_on_the_full_chunk_grid(tmp_path):
+    """4096 chunk subtrees in 64 sets on 3 streams and in 7 sets on one
+    stream: the oracle's bytes (Poisson-average)."""
+    g = pc.ChunkGen(4096, 300, 1.16, 50, 100_000, seed=12)
+    cpu = tmp_path / "cpu"
+    g.write(cpu)
+    oracle_index(cpu, "poisson_average", "c2")
+    buf = DeviceBuffer(g.total * g.RECORD_BYTES)
+    L = pc.lib()
+    try:
+        meta = g.device(buf.ptr.value)
+        for streams, sets in ((3, 64), (1, 7)):
+            print(f"full grid sets: streams={streams} sets={sets}", flush=True)
+            assert L.pcg_set_option(b"sample_groups", streams) == 0
+            assert L.pcg_set_option(b"sample_sets", sets) == 0
+            out = tmp_path / f"s{streams}_{sets}"
+            pc.do_indexing_device(buf.ptr.value, g.names, g.counts, meta, "poisson_average", name="c2", outdir=out)
+            for f in ("octree.bin", "hierarchy.bin", "metadata.json"):
+                assert filecmp.cmp(cpu / f, out / f, shallow=False), (streams, sets, f)
+    finally:
+        L.pcg_set_option(b"sample_groups", 0)
+        L.pcg_set_option(b"sample_sets", 0)
+        buf.free()
+
+
 def test_device_indexing_rejects_bad_chunk_lists(tmp_path):
     g = pc.ChunkGen(8, 500, 0.5, 100, 1000, seed=1)
     buf = DeviceBuffer(g.total * g.RECORD_BYTES)

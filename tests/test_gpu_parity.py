@@ -324,6 +324,27 @@ def test_sample_groups_agree(tmp_path):
         _set("sample_groups", 0)
 
 
+def test_sample_sets_agree(tmp_path):
+    """More chunk-subtree sets than streams (sets interleaved level by level
+    on a stream, sharing its buffers): the oracle's bytes."""
+    las = tmp_path / "in.las"
+    make_las(las, 0, 2_000_000, 9, False)
+    cpu = tmp_path / "cpu"
+    oracle_convert(las, cpu, "poisson")
+    try:
+        for groups, sets in ((3, 12), (2, 64), (1, 5)):
+            print(f"sets case streams={groups} sets={sets}", flush=True)
+            _set("sample_groups", groups)
+            _set("sample_sets", sets)
+            out = tmp_path / f"s{groups}_{sets}"
+            pc.convert(las, out, "poisson")
+            res = same_files(cpu, out, ["octree.bin", "hierarchy.bin", "metadata.json"])
+            assert all(res.values()), (groups, sets, res)
+    finally:
+        _set("sample_groups", 0)
+        _set("sample_sets", 0)
+
+
 SCENE_CASES = [(sc, m) for sc in ("stacks", "dups", "spacing", "sphere") for m in ("poisson", "poisson_average")]
 
 
