@@ -35,10 +35,13 @@ STATS = 24
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--config", default="c1", choices=["c1", "c2", "c3"],
+    ap.add_argument("--config", default="c1", choices=["c1", "c2", "c3", "c4"],
                     help="c1: 200M-point LAS tile, chunking + indexing; c2: indexing of 4096 pre-chunked chunks; "
                          "c3: the precision scene (offset 1e6 m, scale 0.0001, near-duplicate clusters, lattices "
-                         "at the node spacing) at 1B points, chunking + indexing")
+                         "at the node spacing) at 1B points, chunking + indexing; c4: a serving loop of "
+                         "Poisson-arriving conversion jobs (potreeconverter_amd.serve)")
+    ap.add_argument("--jobs", type=int, default=200, help="c4: number of jobs")
+    ap.add_argument("--rate", type=float, default=20.0, help="c4: mean arrivals per second")
     ap.add_argument("--c2-plain-median", type=int, default=2_500,
                     help="c2: median chunk size of the batched-vs-plain comparison (0 skips it)")
     ap.add_argument("--sample-groups", type=int, default=None,
@@ -248,6 +251,39 @@ def c2_indexing(args):
     print(json.dumps(line))
 
 
+def c4_serving(args):
+    """Poisson arrivals of conversion jobs (100k .. 50M points, log-uniform),
+    HBM-budget admission, device-resident conversion; one JSON line."""
+    from potreeconverter_amd import serve
+
+    # a short stream first: code paths and buffers warm
+    serve.serve(num_jobs=max(args.warmup, 0) * 4, rate=args.rate, min_points=100_000, max_points=2_000_000,
+                seed=args.seed + 1, method=args.method)
+    r = serve.serve(num_jobs=args.jobs, rate=args.rate, min_points=100_000, max_points=50_000_000, seed=args.seed,
+                    method=args.method)
+    line = {
+        "metric": "serving loop: aggregate Mpts/s and p50/p99 job latency (Poisson arrivals, HBM admission)",
+        "value": r["aggregate_Mpts_per_s"],
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.jobs,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["wall_s"] / args.jobs * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.jobs} jobs, Poisson arrivals at {args.rate}/s, 500 m LAS tiles (preset 0, "
+                               f"per-job seeds) of 100k..50M points log-uniform, -m {args.method}",
+                   "jobs": args.jobs, "rate": args.rate, "method": args.method},
+        "serving": r,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -261,6 +297,9 @@ def main():
         pc._check(pc.lib().pcg_set_option(b"sample_groups", args.sample_groups))
     if args.config == "c2":
         c2_indexing(args)
+        return
+    if args.config == "c4":
+        c4_serving(args)
         return
     import potreeconverter_amd as pc
 

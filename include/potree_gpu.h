@@ -46,6 +46,17 @@ int pcg_release(void);
  * "random_seed" (-1 or 0 .. 2^32-1, default -1): seed of every leaf shuffle
  * of -m random (SamplerRandom, Converter/include/sampler_random.h:102-104,
  * where it is system_clock::now()); -1 takes the clock as the reference does.
+ * "sample_sets" (0-64, default 0): the chunk subtrees are split into this
+ * many sets (at least one per stream), the streams taking them in turn with
+ * buffers sized for their largest set (same results; 0 = one set per stream
+ * unless the job needs more sets to fit in device memory, as at 1B points).
+ * "chunk_grid" (0, 128, 256, 512) and "chunk_cap" (0 or >= 1) force the
+ * chunker's grid size and per-chunk point cap, which the reference derives
+ * from the point count (chunker_countsort_laszip.cpp:1209-1220; 0 = that
+ * rule): a small input then takes the paths of a large one.
+ * "distribute_tail" (0/1, default 1): 1 places points with the tile
+ * transcode (16-bit chunk ids, aligned 16-byte run stores) where the record
+ * layout allows, 0 with the byte-wise scatter (same results).
  * The other options have no reference counterpart. */
 int pcg_set_option(const char* key, int64_t value);
 
@@ -185,6 +196,11 @@ int pcg_device_alloc(void** ptr, uint64_t bytes);
 int pcg_device_free(void* ptr);
 int pcg_copy(void* dst, const void* src, uint64_t bytes, int32_t kind);
 int pcg_synchronize(void);
+/* Free and total device memory (hipMemGetInfo) and the bytes the library
+ * holds in its reusable buffers (none of it is freed between jobs unless
+ * pcg_release is called or a job needs the room). */
+int pcg_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
+uint64_t pcg_workspace_bytes(void);
 
 /* Synthetic LAS generator (not part of the reference): preset 0 = 500 m
  * tile, 1 = 5 km tile in UTM-like coordinates, 2 = precision scene (scale

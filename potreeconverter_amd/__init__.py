@@ -73,6 +73,9 @@ def lib():
                                          c.c_void_p, c.c_void_p, c.c_int32, c.c_int32, c.POINTER(c.c_double),
                                          c.POINTER(c.c_double), c.c_double, c.c_void_p, c.c_void_p, c.c_int64,
                                          c.c_void_p, c.c_void_p, c.POINTER(c.c_int32)]
+        L.pcg_device_memory.argtypes = [c.POINTER(c.c_uint64), c.POINTER(c.c_uint64)]
+        L.pcg_workspace_bytes.argtypes = []
+        L.pcg_workspace_bytes.restype = c.c_uint64
         L.pcg_release.argtypes = []
         L.pcg_set_option.argtypes = [c.c_char_p, c.c_int64]
         for name in ("pcg_set_option", "pcg_release", "pcg_convert", "pcg_do_chunking", "pcg_do_indexing", "pcg_count_cells", "pcg_count_cells_noalloc", "pcg_convert_device",
@@ -81,7 +84,7 @@ def lib():
                      "pcg_lasgen_configure",
                      "pcg_lasgen_write", "pcg_lasgen_device", "pcg_lasgen_header", "pcg_init",
                      "pcg_do_indexing_device", "pcg_chunkgen_params", "pcg_chunkgen_counts", "pcg_chunkgen_write",
-                     "pcg_chunkgen_device", "pcg_sample_subtree"):
+                     "pcg_chunkgen_device", "pcg_sample_subtree", "pcg_device_memory"):
             getattr(L, name).restype = c.c_int
         _lib = L
     return _lib

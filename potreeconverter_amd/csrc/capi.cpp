@@ -425,6 +425,17 @@ int pcg_sample_subtree(const char* method, int32_t num_nodes, const int32_t* chi
   return 0;
 }
 
+int pcg_device_memory(uint64_t* free_bytes, uint64_t* total_bytes) {
+  if (ensure_device()) return 1;
+  size_t f = 0, t = 0;
+  PCG_CHECK(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return 0;
+}
+
+uint64_t pcg_workspace_bytes(void) { return workspace_total(); }
+
 int pcg_device_alloc(void** ptr, uint64_t bytes) {
   if (ensure_device()) return 1;
   PCG_CHECK(hipMalloc(ptr, bytes ? bytes : 16));

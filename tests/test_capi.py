@@ -91,7 +91,8 @@ def test_every_binding_matches_the_header():
                 ok = ctypes.sizeof(got) == SIZES[want] and (want != "double") == (got is not ctypes.c_double)
             if not ok:
                 bad.append(f"{name}: parameter {i} is {want}, bound as {got.__name__}")
-        want_ret = {"int": ctypes.c_int, "int32_t": ctypes.c_int32, "void": None, "const char*": ctypes.c_char_p}[ret]
+        want_ret = {"int": ctypes.c_int, "int32_t": ctypes.c_int32, "uint64_t": ctypes.c_uint64, "void": None,
+                    "const char*": ctypes.c_char_p}[ret]
         if fn.restype is not want_ret and not (want_ret is not None and fn.restype is not None
                                                 and ctypes.sizeof(fn.restype) == ctypes.sizeof(want_ret)):
             bad.append(f"{name}: returns {ret}, bound as {fn.restype}")

@@ -105,3 +105,20 @@ def test_chunkgen_records_stay_in_their_cell(tmp_path):
         assert abs(gauss.mean()) < 0.1 and abs(gauss.std() - 1) < 0.1
     meta = open(tmp_path / "chunks" / "metadata.json").read()
     assert '"name": "gauss"' in meta and '"name": "uniform8"' in meta
+
+
+def test_serving_job_stream():
+    """The serving loop's job stream: Poisson arrivals at the given rate and
+    log-uniform sizes, reproducible from the seed."""
+    from potreeconverter_amd.serve import make_jobs, percentile
+    a = make_jobs(4000, 20.0, 100_000, 50_000_000, 3)
+    b = make_jobs(4000, 20.0, 100_000, 50_000_000, 3)
+    assert [(j.arrival, j.points) for j in a] == [(j.arrival, j.points) for j in b]
+    gaps = np.diff([0.0] + [j.arrival for j in a])
+    assert abs(gaps.mean() * 20.0 - 1) < 0.05 and abs(gaps.std() / gaps.mean() - 1) < 0.06
+    logn = np.log([j.points for j in a])
+    assert logn.min() >= np.log(100_000) - 1e-9 and logn.max() <= np.log(50_000_000) + 1e-9
+    # log-uniform: each decade-ish quarter of the log range holds a quarter
+    q = np.histogram(logn, bins=4, range=(np.log(100_000), np.log(50_000_000)))[0] / len(a)
+    assert np.all(np.abs(q - 0.25) < 0.03)
+    assert percentile([1, 2, 3, 4], 50) == 2.5 and percentile([5], 99) == 5
