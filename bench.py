@@ -57,6 +57,7 @@ def parse():
     ap.add_argument("--random-seed", type=int, default=0, help="seed of -m random's leaf shuffles")
     ap.add_argument("--cpu-points", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-io", action="store_true", help="skip the upload / octree write timings")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--plain-points", type=int, default=20_000_000,
                     help="size of the batched-vs-plain sampling comparison (0 skips it)")
@@ -65,8 +66,25 @@ def parse():
     return ap.parse_args()
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(points, seed, method, preset=1, total=200_000_000):
-    """Times the CPU oracle on the first `points` records of the same stream."""
+    """Times the CPU side on the first `points` records of the same stream:
+    doChunking by the oracle (one thread; the reference chunker cannot be
+    built here: it needs LASzip's CMake-generated header) and doIndexing by
+    the reference indexer itself (oracle/_ref/ref_indexer, built from the
+    reference sources), with its own task pool of hardware_concurrency + 4
+    threads over the chunks (indexer.cpp:1638-1707).  Without the reference
+    build the oracle's single-threaded indexer is timed instead."""
     import potreeconverter_amd as pc
 
     ocl = os.path.join(HERE, "oracle", "_build", "liboracle.so")
@@ -74,8 +92,11 @@ def cpu_baseline(points, seed, method, preset=1, total=200_000_000):
         return None
     O = ctypes.CDLL(ocl)
     O.orc_convert.argtypes = [ctypes.c_char_p] * 3 + [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+    ref = os.path.join(HERE, "oracle", "_ref", "ref_indexer")
     g = pc.LasGen(preset, total, seed=seed)
     g.n = points
+    cpus = len(os.sched_getaffinity(0))
+    hw = os.cpu_count() or cpus
     # generate the prefix of the full stream and write it as its own LAS file
     with tempfile.TemporaryDirectory() as td:
         las = os.path.join(td, "sample.las")
@@ -84,20 +105,73 @@ def cpu_baseline(points, seed, method, preset=1, total=200_000_000):
         pc.lib().pcg_lasgen_write(g.buf, las.encode())
         t = (ctypes.c_double * 8)()
         out = os.path.join(td, "out")
-        t0 = time.time()
-        rc = O.orc_convert(las.encode(), out.encode(), method.encode(), 0, 0, t)
-        wall = time.time() - t0
-        if rc != 0:
-            return None
-        secs = t[2] + t[3]
+        if os.path.exists(ref):
+            rc = O.orc_convert(las.encode(), out.encode(), method.encode(), 1, 1, t)
+            if rc != 0:
+                return None
+            chunk_s = t[2]
+            t0 = time.time()
+            r = subprocess.run([ref, out, method, "sample", str(points), "DEFAULT"], stdout=subprocess.DEVNULL,
+                               stderr=subprocess.DEVNULL)
+            index_s = time.time() - t0
+            if r.returncode != 0:
+                return None
+            kind, cores = "reference", cpus
+            how = (f"doChunking by the oracle on 1 thread {chunk_s:.2f}s + doIndexing by the reference indexer "
+                   f"{index_s:.2f}s ({hw + 4} threads on {cpus} available CPUs, incl. its chunk file reads and "
+                   f"octree.bin / hierarchy writes)")
+        else:
+            t0 = time.time()
+            rc = O.orc_convert(las.encode(), out.encode(), method.encode(), 0, 0, t)
+            if rc != 0:
+                return None
+            chunk_s, index_s = t[2], t[3]
+            kind, cores = "port", 1
+            how = f"oracle on 1 thread: chunking {chunk_s:.2f}s + indexing {index_s:.2f}s"
     return {
-        "value": points / secs / 1e6,
+        "value": points / (chunk_s + index_s) / 1e6,
         "unit": "Mpts/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": f"first {points} points of the same generated stream; chunking {t[2]:.2f}s + indexing {t[3]:.2f}s "
-                  f"(wall {wall:.1f}s incl. file read/write)",
+        "cores": cores,
+        "kind": kind,
+        "sample": f"first {points} points of the same generated stream; " + how,
+        "cpu_model": cpu_model(),
+        "parts_s": {"chunking": round(chunk_s, 3), "indexing": round(index_s, 3)},
+        "chunking_Mpts_per_s_1_thread": round(points / chunk_s / 1e6, 2),
+        "indexing_Mpts_per_s": round(points / index_s / 1e6, 2),
     }
+
+
+def io_times(L, d_las, n, rl, header, method):
+    """PCIe and file I/O around one conversion, outside the timed steps:
+    the LAS records host -> device (pageable host memory), and the octree
+    written to disk (octree.bin downloaded from HBM + hierarchy.bin +
+    metadata.json) as the extra wall time of a conversion with an output
+    directory over one without."""
+    import shutil
+
+    import potreeconverter_amd as pc
+
+    out = {}
+    host = ctypes.create_string_buffer(n * rl)
+    pc._check(L.pcg_copy(host, d_las, n * rl, 1))
+    t0 = time.perf_counter()
+    pc._check(L.pcg_copy(d_las, host, n * rl, 0))
+    out["upload_s"] = round(time.perf_counter() - t0, 4)
+    out["upload_GBps"] = round(n * rl / out["upload_s"] / 1e9, 2)
+    del host
+    stats = (ctypes.c_double * STATS)()
+    with tempfile.TemporaryDirectory() as td:
+        if shutil.disk_usage(td).free < 3 * n * (rl + 1):
+            return out
+        t0 = time.perf_counter()
+        pc._check(L.pcg_convert_device(d_las, n, header, method.encode(), None, stats))
+        plain = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        pc._check(L.pcg_convert_device(d_las, n, header, method.encode(), os.path.join(td, "o").encode(), stats))
+        written = time.perf_counter() - t0
+        out["download_and_write_s"] = round(written - plain, 4)
+        out["octree_bytes"] = int(stats[7])
+    return out
 
 
 def plain_vs_batched(points, seed, method):
@@ -382,6 +456,7 @@ def main():
     if args.plain_points > 0:
         plain = plain_vs_batched(min(args.plain_points, n), args.seed, args.method)
     cpu = None if args.no_cpu else cpu_baseline(min(args.cpu_points, n), args.seed, args.method, preset, n)
+    io = None if (args.no_io or c3) else io_times(L, d_las, n, rl, header, args.method)
     workload = (f"{n} pts synthetic LAS PDRF2 5x5 km (offset 500000,5000000,0; scale 0.001), "
                 f"terrain fBm + 20 buildings, seed {args.seed}")
     metric = "Mpts/s count-sort+octree+Poisson at 200M pts, batched vs plain; % HBM peak"
@@ -444,6 +519,7 @@ def main():
             "kernel_ms": round(kern_s * 1e3, 4),
         },
         "precision": precision,
+        "io_s": io,
         "plain_vs_batched": plain,
         "cpu_baseline": cpu,
     }
