@@ -87,6 +87,34 @@ int pcg_convert_encoded(const char* las_path, const char* outdir, const char* me
  * (<id>.bin + metadata.json) exactly as the reference chunker's contract. */
 int pcg_do_chunking(const char* las_path, const char* outdir, double* timings);
 
+/* doChunking with the reference's arguments
+ * (chunker_countsort_laszip::doChunking(sources, targetDir, min, max, state,
+ * outputAttributes, monitor), Converter/include/chunker_countsort_laszip.h:19,
+ * called at Converter/src/main.cpp:353): num_sources LAS files; min / max
+ * (both or neither, 3 doubles each) the box computeStats gives the chunker
+ * (main.cpp:215-284; NULL: computed from the headers the same way);
+ * attributes (num_attributes names, NULL / 0 for all) the --attributes
+ * filter from which the output attributes are derived as
+ * computeOutputAttributes(sources, requestedAttributes) derives them
+ * (PotreeConverter.h:195-315: union of the sources' attributes, smallest
+ * LAS scale, position first then the requested names that exist).  Each
+ * chunk holds the first source's points, then the second's, ... (input
+ * order inside a source).  Extra-bytes attributes that the filter or a mix
+ * of point formats would move are refused: the reference writes them at
+ * their offset in the source's layout (chunker_countsort_laszip.cpp:579-582),
+ * outside their output attribute. */
+int pcg_do_chunking_sources(const char* const* las_paths, int32_t num_sources, const char* outdir, const double* min,
+                            const double* max, const char* const* attributes, int32_t num_attributes,
+                            double* timings);
+
+/* Both phases for several sources with the --attributes filter (main.cpp's
+ * curateSources / computeOutputAttributes / computeStats, chunking and
+ * indexing; the name in metadata.json is the first path's stem);
+ * method, encoding, flags and timings as pcg_convert_encoded. */
+int pcg_convert_sources(const char* const* las_paths, int32_t num_sources, const char* outdir, const char* method,
+                        const char* encoding, const char* const* attributes, int32_t num_attributes, int32_t flags,
+                        double* timings);
+
 /* Phase 2 only (indexer::doIndexing, Converter/include/indexer.h:379) on an
  * existing outdir/chunks/ written by either chunker. */
 int pcg_do_indexing(const char* outdir, const char* method, const char* name, int64_t points_total,

@@ -22,6 +22,7 @@
 #include <iomanip>
 #include <limits>
 #include <map>
+#include <set>
 #include <memory>
 #include <random>
 #include <sstream>
@@ -178,21 +179,6 @@ static bool attributes_for_format(const lasfile::Header& h, std::vector<Attr>& l
     a.offset = V3(e.offset[0], e.offset[1], e.offset[2]);
     list.push_back(a);
   }
-  return true;
-}
-
-// PotreeConverter.h:29-76 computeScaleOffset, :195-315 (single source)
-static bool output_attributes(const lasfile::Header& h, Attrs& out) {
-  if (!attributes_for_format(h, out.list)) return false;
-  out.finish();
-  V3 min(h.min[0], h.min[1], h.min[2]), max(h.max[0], h.max[1], h.max[2]);
-  V3 scale(h.scale[0], h.scale[1], h.scale[2]);
-  V3 size = max - min;
-  scale.x = std::max(scale.x, size.x / pow(2.0, 30.0));
-  scale.y = std::max(scale.y, size.y / pow(2.0, 30.0));
-  scale.z = std::max(scale.z, size.z / pow(2.0, 30.0));
-  out.posScale = scale;
-  out.posOffset = min;
   return true;
 }
 
@@ -412,36 +398,66 @@ struct Transcoder {
   int fmt = 2;
   int inBpp = 26;
   Attrs* out = nullptr;
-  int firstExtraIndex = 9, extraSrc = 0, extraDst = 0, extraBytes = 0;
-  // handler of each standard attribute (name lookup of :532-544 done once)
+  int firstExtraIndex = 9, extraSrc = 0;
+  // handler of each standard attribute of the source (name lookup of
+  // :532-544 done once), its destination offset and output attribute
   enum H { NONE, INTENSITY, RET, NRET, CLASS, SAR, SA, UD, PSID, GPS, CFLAGS, RGB };
-  std::vector<int> handler, offset;
+  std::vector<int> handler, offset, target;
+  struct Extra { int src, dst, size, target; };
+  std::vector<Extra> extras;
 
-  void init(const lasfile::Header& h, Attrs& o) {
+  // `o` is the output layout (the union of the sources' attributes, maybe
+  // filtered); the handlers come from the source's own attribute list
+  bool init(const lasfile::Header& h, Attrs& o) {
     fmt = h.point_format;
     inBpp = h.record_length;
     out = &o;
+    std::vector<Attr> in;
+    if (!attributes_for_format(h, in)) return false;
     static const int core[] = {20, 28, 26, 34, 57, 63, 30, 36};  // record size without extra bytes
     extraSrc = core[fmt];
-    // :561-577 the first extra attribute of each format and its destination
-    // offset: the summed size of the attributes before it
+    // :561-577 the first extra attribute of each format
     static const int firstIdx[] = {8, 9, 9, 10, 14, 15, 10, 11};
     firstExtraIndex = firstIdx[fmt];
-    extraDst = 0;
-    for (int i = 0; i < firstExtraIndex; i++) extraDst += o.list[i].size;
-    extraBytes = 0;
-    for (size_t i = firstExtraIndex; i < o.list.size(); i++) extraBytes += o.list[i].size;
     static const std::pair<const char*, int> names[] = {
         {"intensity", INTENSITY}, {"return number", RET}, {"number of returns", NRET}, {"classification", CLASS},
         {"scan angle rank", SAR}, {"scan angle", SA}, {"user data", UD}, {"point source id", PSID},
         {"gps-time", GPS}, {"classification flags", CFLAGS}, {"rgb", RGB}};
     handler.assign(firstExtraIndex, NONE);
     offset.assign(firstExtraIndex, 0);
+    target.assign(firstExtraIndex, -1);
+    auto index_of = [&](const std::string& n) {
+      for (size_t k = 0; k < o.list.size(); k++) if (o.list[k].name == n) return (int)k;
+      return -1;
+    };
     for (int i = 1; i < firstExtraIndex; i++) {
+      const int t = index_of(in[i].name);
+      if (t < 0) continue;  // not in the output: no handler (:540-543)
       for (auto& nm : names)
-        if (o.list[i].name == nm.first) handler[i] = nm.second;
-      offset[i] = o.offsetOf(o.list[i].name);
+        if (in[i].name == nm.first) handler[i] = nm.second;
+      offset[i] = o.offsetOf(in[i].name);
+      target[i] = t;
     }
+    // extra attributes (:579-651): an included one is written at its offset
+    // in the source's layout (the source's standard sizes plus the included
+    // extra attributes before it); where that is not its output offset the
+    // reference writes outside the attribute, which is refused
+    extras.clear();
+    int refDst = 0, src = extraSrc;
+    for (int i = 0; i < firstExtraIndex; i++) refDst += in[i].size;
+    for (size_t i = firstExtraIndex; i < in.size(); i++) {
+      const int t = index_of(in[i].name);
+      if (t >= 0) {
+        if (o.offsetOf(in[i].name) != refDst) {
+          g_error = "extra attribute " + in[i].name + " would be written outside its output attribute";
+          return false;
+        }
+        extras.push_back(Extra{src, refDst, in[i].size, t});
+        refDst += in[i].size;
+      }
+      src += in[i].size;
+    }
+    return true;
   }
 
   static void mm(Attr* a, double v) { a->min.x = std::min(a->min.x, v); a->max.x = std::max(a->max.x, v); }
@@ -469,7 +485,8 @@ struct Transcoder {
     // the standard handlers that exist for this format's attributes
     // (:391-546); attributes without one (the wave packet) stay zero
     for (int i = 1; i < firstExtraIndex; i++) {
-      Attr* a = &o.list[i];
+      if (target[i] < 0) continue;
+      Attr* a = &o.list[target[i]];
       const int off = offset[i];
       switch (handler[i]) {
         case INTENSITY: memcpy(dst + off, &p.intensity, 2); mm(a, double(p.intensity)); break;
@@ -494,12 +511,10 @@ struct Transcoder {
         default: break;  // no handler (:534-546): bytes stay 0, min/max stay +-inf
       }
     }
-    // extra attributes :579-651
-    int so = 0, ao = extraDst;
-    for (size_t i = firstExtraIndex; i < o.list.size(); i++) {
-      Attr* a = &o.list[i];
-      memcpy(dst + ao, src + extraSrc + so, a->size);
-      const uint8_t* ptr = src + extraSrc + so;
+    for (const Extra& e : extras) {
+      Attr* a = &o.list[e.target];
+      memcpy(dst + e.dst, src + e.src, e.size);
+      const uint8_t* ptr = src + e.src;
       if (a->numElements == 1) {
         double x = elem(a->type, ptr);
         a->min.x = std::min(a->min.x, x); a->max.x = std::max(a->max.x, x);
@@ -512,8 +527,6 @@ struct Transcoder {
         a->min.x = std::min(a->min.x, x); a->min.y = std::min(a->min.y, y); a->min.z = std::min(a->min.z, z);
         a->max.x = std::max(a->max.x, x); a->max.y = std::max(a->max.y, y); a->max.z = std::max(a->max.z, z);
       }
-      ao += a->size;
-      so += a->size;
     }
   }
 };
@@ -581,34 +594,99 @@ struct ChunkerOut {
 // grid size and chunk cap forced by orc_set_chunking (0: the reference's rule)
 static int64_t g_force_grid = 0, g_force_cap = 0;
 
-// doChunking (chunker_countsort_laszip.cpp:1206-1261) for one source with
-// the whole file in memory.  State.pointsTotal = header point count.
-static int do_chunking(const std::string& lasPath, ChunkerOut& co) {
+// computeOutputAttributes(sources, requestedAttributes)
+// (PotreeConverter.h:195-315): the union of the sources' attribute lists by
+// name in source order, the smallest LAS scale and the headers' bounds
+// through computeScaleOffset (:29-76), then position and the requested names
+// that exist, in the order requested
+static bool output_attributes_sources(const std::vector<lasfile::Header>& hs, const std::vector<std::string>& requested,
+                                      Attrs& out) {
+  std::vector<Attr> full;
+  V3 scaleMin(INF, INF, INF), min(INF, INF, INF), max(-INF, -INF, -INF);
+  for (const auto& h : hs) {
+    std::vector<Attr> list;
+    if (!attributes_for_format(h, list)) return false;
+    for (auto& a : list) {
+      bool seen = false;
+      for (auto& b : full) seen = seen || b.name == a.name;
+      if (!seen) full.push_back(a);
+    }
+    scaleMin.x = std::min(scaleMin.x, h.scale[0]); scaleMin.y = std::min(scaleMin.y, h.scale[1]);
+    scaleMin.z = std::min(scaleMin.z, h.scale[2]);
+    min.x = std::min(min.x, h.min[0]); min.y = std::min(min.y, h.min[1]); min.z = std::min(min.z, h.min[2]);
+    max.x = std::max(max.x, h.max[0]); max.y = std::max(max.y, h.max[1]); max.z = std::max(max.z, h.max[2]);
+  }
+  out.list = full;
+  if (!requested.empty()) {
+    std::vector<std::string> should{"position"};
+    should.insert(should.end(), requested.begin(), requested.end());
+    out.list.clear();
+    for (auto& nm : should)
+      for (auto& a : full)
+        if (a.name == nm) { out.list.push_back(a); break; }
+  }
+  out.finish();
+  V3 size = max - min, scale = scaleMin;
+  scale.x = std::max(scale.x, size.x / pow(2.0, 30.0));
+  scale.y = std::max(scale.y, size.y / pow(2.0, 30.0));
+  scale.z = std::max(scale.z, size.z / pow(2.0, 30.0));
+  out.posScale = scale;
+  out.posOffset = min;
+  return true;
+}
+
+// doChunking (chunker_countsort_laszip.cpp:1206-1261) over the sources in
+// order, each file in memory.  cubeMin / cubeMax: the box main.cpp's
+// computeStats (:215-284) hands over, computed here from the headers when
+// null; the chunker cubes it once more (:205-207, :1252-1254).  A chunk holds
+// the first source's points, then the second's, ... (input order inside a
+// source), the canonical order of the reference's task pool.
+static int do_chunking_sources(const std::vector<std::string>& paths, const double* cubeMin, const double* cubeMax,
+                               const std::vector<std::string>& requested, ChunkerOut& co) {
   double t0 = now_s();
-  lasfile::Header h;
-  if (!lasfile::read_header(lasPath, h, &g_error)) return 1;
-  if (!output_attributes(h, co.attrs)) return 1;
-  // main.cpp:215-284 computeStats: cubic box from the header bounds; the
-  // chunker receives (min, min + cube) and cubes it once more
-  // (chunker_countsort_laszip.cpp:205-207, :1252-1254)
-  V3 min(h.min[0], h.min[1], h.min[2]), hmax(h.max[0], h.max[1], h.max[2]);
-  double statsCube = (hmax - min).max();
-  if (statsCube == 0.0) { g_error = "invalid bounding box"; return 123; }
-  V3 max = min + statsCube;
+  std::vector<lasfile::Header> hs(paths.size());
+  for (size_t i = 0; i < paths.size(); i++)
+    if (!lasfile::read_header(paths[i], hs[i], &g_error)) return 1;
+  if (hs.empty()) { g_error = "no sources"; return 1; }
+  if (!output_attributes_sources(hs, requested, co.attrs)) return 1;
+  V3 min, max;
+  if (cubeMin && cubeMax) {
+    min = V3(cubeMin[0], cubeMin[1], cubeMin[2]);
+    max = V3(cubeMax[0], cubeMax[1], cubeMax[2]);
+  } else {
+    V3 hmin(INF, INF, INF), hmax(-INF, -INF, -INF);
+    for (auto& h : hs) {
+      hmin.x = std::min(hmin.x, h.min[0]); hmin.y = std::min(hmin.y, h.min[1]); hmin.z = std::min(hmin.z, h.min[2]);
+      hmax.x = std::max(hmax.x, h.max[0]); hmax.y = std::max(hmax.y, h.max[1]); hmax.z = std::max(hmax.z, h.max[2]);
+    }
+    double statsCube = (hmax - hmin).max();
+    if (statsCube == 0.0) { g_error = "invalid bounding box"; return 123; }
+    min = hmin;
+    max = hmin + statsCube;
+  }
   double cube = (max - min).max();
-  int64_t N = (int64_t)h.num_points;
+  if (!(cube > 0.0)) { g_error = "invalid bounding box"; return 123; }
+  int64_t N = 0;
+  for (auto& h : hs) N += (int64_t)h.num_points;
   int64_t maxPointsPerChunk = g_force_cap > 0 ? g_force_cap : std::min(N / 20, int64_t(10000000));
   int64_t G = g_force_grid > 0 ? g_force_grid : (N < 100000000 ? 128 : (N < 500000000 ? 256 : 512));
-  std::vector<uint8_t> las((size_t)N * h.record_length);
-  if (!lasfile::read_points(lasPath, h, 0, N, las.data())) { g_error = "short read"; return 1; }
-  V3 cubeMax = min + cube;
-
-  std::vector<int32_t> grid(G * G * G);
+  V3 cubeMaxV = min + cube;
+  std::vector<std::vector<uint8_t>> las(hs.size());
+  for (size_t i = 0; i < hs.size(); i++) {
+    las[i].resize((size_t)hs[i].num_points * hs[i].record_length);
+    if (!lasfile::read_points(paths[i], hs[i], 0, hs[i].num_points, las[i].data())) { g_error = "short read"; return 1; }
+  }
+  // countPointsInCells (:131-361): every source into one grid
+  std::vector<int32_t> grid(G * G * G, 0), part(G * G * G);
   double mn[3] = {min.x, min.y, min.z}, mx[3] = {max.x, max.y, max.z};
   double ps[3] = {co.attrs.posScale.x, co.attrs.posScale.y, co.attrs.posScale.z};
   double po[3] = {co.attrs.posOffset.x, co.attrs.posOffset.y, co.attrs.posOffset.z};
-  int rc = orc_count_cells(las.data(), N, h.record_length, h.scale, h.offset, ps, po, mn, mx, (int32_t)G, grid.data());
-  if (rc) return rc;
+  for (size_t i = 0; i < hs.size(); i++) {
+    int rc = orc_count_cells(las[i].data(), (int64_t)hs[i].num_points, hs[i].record_length, hs[i].scale,
+                             hs[i].offset, ps, po, mn, mx, (int32_t)G, part.data());
+    if (rc) return rc;
+    for (size_t k = 0; k < grid.size(); k++) grid[k] += part[k];
+  }
   double t1 = now_s();
   co.tCount = t1 - t0;
 
@@ -618,29 +696,32 @@ static int do_chunking(const std::string& lasPath, ChunkerOut& co) {
   for (size_t c = 0; c < co.chunks.size(); c++) co.records[c].reserve(co.chunks[c].numPoints * co.attrs.bytes);
 
   // distributePoints (:660-1003): per-thread attribute copies reset min/max
-  // (:385-389) then merge into the caller's copy (:914-932); with one batch
-  // sequence the merged result equals a single running min/max.
-  Transcoder tc;
-  tc.init(h, co.attrs);
+  // (:385-389) then merge into the caller's copy (:914-932); the merged result
+  // equals a single running min/max.
   for (auto& a : co.attrs.list) a.histogram.assign(256, 0);
   int bpp = co.attrs.bytes;
   std::vector<uint8_t> rec(bpp);
-  for (int64_t i = 0; i < N; i++) {
-    const uint8_t* src = las.data() + (size_t)i * h.record_length;
-    double xyz[3];
-    int32_t q[3];
-    requantize(src, h.scale, h.offset, co.attrs.posScale, co.attrs.posOffset, xyz, q);
-    std::fill(rec.begin(), rec.end(), 0);
-    tc.apply(src, q, xyz, rec.data());
-    int64_t idx = cell_of(q[0], q[1], q[2], co.attrs.posScale, co.attrs.posOffset, min, cube, G, nullptr);
-    int32_t ci = lut[idx];
-    if (ci < 0) { g_error = "point to node lookup failed"; return 123; }
-    co.records[ci].insert(co.records[ci].end(), rec.begin(), rec.end());
+  for (size_t s = 0; s < hs.size(); s++) {
+    const lasfile::Header& h = hs[s];
+    Transcoder tc;
+    if (!tc.init(h, co.attrs)) return 1;
+    for (int64_t i = 0; i < (int64_t)h.num_points; i++) {
+      const uint8_t* src = las[s].data() + (size_t)i * h.record_length;
+      double xyz[3];
+      int32_t q[3];
+      requantize(src, h.scale, h.offset, co.attrs.posScale, co.attrs.posOffset, xyz, q);
+      std::fill(rec.begin(), rec.end(), 0);
+      tc.apply(src, q, xyz, rec.data());
+      int64_t idx = cell_of(q[0], q[1], q[2], co.attrs.posScale, co.attrs.posOffset, min, cube, G, nullptr);
+      int32_t ci = lut[idx];
+      if (ci < 0) { g_error = "point to node lookup failed"; return 123; }
+      co.records[ci].insert(co.records[ci].end(), rec.begin(), rec.end());
+    }
   }
   double t2 = now_s();
   co.tDistribute = t2 - t1;
   co.min = min;
-  co.max = cubeMax;
+  co.max = cubeMaxV;
   co.tTotal = t2 - t0;
   return 0;
 }
@@ -1671,6 +1752,19 @@ static int do_indexing(std::vector<ChunkIn>& chunks, Attrs attributes, V3 min, V
 
   if (compressFailed) return 1;
   int64_t firstChunkSize = 0;
+  // HierarchyBuilder.h:213-226 exits with 123 when a hierarchy chunk's root
+  // (a name of 1 + 4k characters) was not written while nodes below it were;
+  // :317-326 dereferences the missing node for the batches below the root
+  {
+    std::set<std::string> written;
+    for (auto& r : ix.hrecords) written.insert(r.name);
+    for (auto& nm : written)
+      for (size_t len = 5; len < nm.size(); len += 4)
+        if (!written.count(nm.substr(0, len))) {
+          g_error = "didn't find proxy node " + nm.substr(0, len);
+          return 123;
+        }
+  }
   auto hier = build_hierarchy_file(ix.hrecords, firstChunkSize);
   std::string meta = create_metadata(ix, name, pointsTotal, firstChunkSize);
   if (tIndex) *tIndex = now_s() - t0;
@@ -1791,10 +1885,11 @@ static bool read_chunks(const std::string& outdir, std::vector<ChunkIn>& chunks,
 
 }  // namespace orc
 
-extern "C" int orc_convert_encoded(const char* las_path, const char* outdir, const char* method,
-                                   const char* encoding, int32_t keep_chunks, int32_t chunk_only, double* timings) {
+static int convert_sources(const std::vector<std::string>& paths, const double* cmin, const double* cmax,
+                           const std::vector<std::string>& requested, const char* outdir, const char* method,
+                           const char* encoding, int32_t keep_chunks, int32_t chunk_only, double* timings) {
   ChunkerOut co;
-  int rc = do_chunking(las_path, co);
+  int rc = do_chunking_sources(paths, cmin, cmax, requested, co);
   if (rc) return rc;
   std::string od = outdir;
   fs::create_directories(od);
@@ -1811,7 +1906,9 @@ extern "C" int orc_convert_encoded(const char* las_path, const char* outdir, con
   if (timings) { timings[0] = co.tCount; timings[1] = co.tDistribute; timings[2] = co.tTotal; timings[3] = 0; }
   if (chunk_only) return 0;
   std::vector<ChunkIn> chunks;
+  int64_t total = 0;
   for (size_t c = 0; c < co.chunks.size(); c++) {
+    total += (int64_t)co.records[c].size() / co.attrs.bytes;
     if (co.records[c].empty()) continue;
     ChunkIn ci;
     ci.id = co.chunks[c].id;
@@ -1820,13 +1917,27 @@ extern "C" int orc_convert_encoded(const char* las_path, const char* outdir, con
     ci.points = std::make_shared<std::vector<uint8_t>>(std::move(co.records[c]));
     chunks.push_back(ci);
   }
-  lasfile::Header h;
-  lasfile::read_header(las_path, h, nullptr);
-  std::string name = fs::path(las_path).stem().string();
+  // the name is the first source's file stem (main.cpp:151-203)
+  std::string name = fs::path(paths[0]).stem().string();
   double tIndex = 0;
-  rc = do_indexing(chunks, co.attrs, co.min, co.max, method, name, (int64_t)h.num_points, od, &tIndex, encoding);
+  rc = do_indexing(chunks, co.attrs, co.min, co.max, method, name, total, od, &tIndex, encoding);
   if (timings) timings[3] = tIndex;
   return rc;
+}
+
+extern "C" int orc_convert_encoded(const char* las_path, const char* outdir, const char* method,
+                                   const char* encoding, int32_t keep_chunks, int32_t chunk_only, double* timings) {
+  return convert_sources({las_path}, nullptr, nullptr, {}, outdir, method, encoding, keep_chunks, chunk_only, timings);
+}
+
+extern "C" int orc_convert_sources(const char* const* las_paths, int32_t num_sources, const double* min,
+                                   const double* max, const char* const* attributes, int32_t num_attributes,
+                                   const char* outdir, const char* method, int32_t keep_chunks, int32_t chunk_only,
+                                   double* timings) {
+  std::vector<std::string> paths, req;
+  for (int32_t i = 0; i < num_sources; i++) paths.push_back(las_paths[i]);
+  for (int32_t i = 0; attributes && i < num_attributes; i++) req.push_back(attributes[i]);
+  return convert_sources(paths, min, max, req, outdir, method, "DEFAULT", keep_chunks, chunk_only, timings);
 }
 
 extern "C" int orc_convert(const char* las_path, const char* outdir, const char* method, int32_t keep_chunks,

@@ -41,6 +41,14 @@ int64_t orc_create_lut(const int32_t* grid, int32_t grid_size, int64_t max_point
 int orc_convert(const char* las_path, const char* outdir, const char* method, int32_t keep_chunks,
                 int32_t chunk_only, double* timings);
 
+// orc_convert over several LAS files with the --attributes filter
+// (computeOutputAttributes(sources, requestedAttributes),
+// PotreeConverter.h:195-315) and the chunker's box (min / max from
+// computeStats, main.cpp:215-284; both null: computed from the headers).
+int orc_convert_sources(const char* const* las_paths, int32_t num_sources, const double* min, const double* max,
+                        const char* const* attributes, int32_t num_attributes, const char* outdir,
+                        const char* method, int32_t keep_chunks, int32_t chunk_only, double* timings);
+
 // doIndexing alone on an existing outdir/chunks directory written by the
 // chunker (chunks are read in ascending ID order).
 int orc_index(const char* outdir, const char* method, const char* name, double* timings);

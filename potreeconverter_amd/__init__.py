@@ -76,6 +76,11 @@ def lib():
         L.pcg_device_memory.argtypes = [c.POINTER(c.c_uint64), c.POINTER(c.c_uint64)]
         L.pcg_workspace_bytes.argtypes = []
         L.pcg_workspace_bytes.restype = c.c_uint64
+        L.pcg_do_chunking_sources.argtypes = [c.POINTER(c.c_char_p), c.c_int32, c.c_char_p, c.POINTER(c.c_double),
+                                              c.POINTER(c.c_double), c.POINTER(c.c_char_p), c.c_int32,
+                                              c.POINTER(c.c_double)]
+        L.pcg_convert_sources.argtypes = [c.POINTER(c.c_char_p), c.c_int32, c.c_char_p, c.c_char_p, c.c_char_p,
+                                          c.POINTER(c.c_char_p), c.c_int32, c.c_int32, c.POINTER(c.c_double)]
         L.pcg_release.argtypes = []
         L.pcg_set_option.argtypes = [c.c_char_p, c.c_int64]
         for name in ("pcg_set_option", "pcg_release", "pcg_convert", "pcg_do_chunking", "pcg_do_indexing", "pcg_count_cells", "pcg_count_cells_noalloc", "pcg_convert_device",
@@ -84,7 +89,8 @@ def lib():
                      "pcg_lasgen_configure",
                      "pcg_lasgen_write", "pcg_lasgen_device", "pcg_lasgen_header", "pcg_init",
                      "pcg_do_indexing_device", "pcg_chunkgen_params", "pcg_chunkgen_counts", "pcg_chunkgen_write",
-                     "pcg_chunkgen_device", "pcg_sample_subtree", "pcg_device_memory"):
+                     "pcg_chunkgen_device", "pcg_sample_subtree", "pcg_device_memory",
+                     "pcg_do_chunking_sources", "pcg_convert_sources"):
             getattr(L, name).restype = c.c_int
         _lib = L
     return _lib
@@ -220,6 +226,34 @@ def convert(las_path, outdir, method="poisson", keep_chunks=False, encoding=None
     else:
         _check(lib().pcg_convert_encoded(str(las_path).encode(), str(outdir).encode(), method.encode(),
                                          encoding.encode(), flags, t))
+    return list(t)
+
+
+def _strings(values):
+    values = list(values or [])
+    return (ctypes.c_char_p * max(len(values), 1))(*[str(v).encode() for v in values]), len(values)
+
+
+def do_chunking_sources(las_paths, outdir, cube_min=None, cube_max=None, attributes=None):
+    """doChunking(sources, targetDir, min, max, ..., outputAttributes from
+    the --attributes filter) over several LAS files."""
+    paths, n = _strings(las_paths)
+    names, na = _strings(attributes)
+    t = (ctypes.c_double * 8)()
+    mn = (ctypes.c_double * 3)(*cube_min) if cube_min is not None else None
+    mx = (ctypes.c_double * 3)(*cube_max) if cube_max is not None else None
+    _check(lib().pcg_do_chunking_sources(paths, n, str(outdir).encode(), mn, mx, names if na else None, na, t))
+    return list(t)
+
+
+def convert_sources(las_paths, outdir, method="poisson", attributes=None, encoding=None, keep_chunks=False):
+    """PotreeConverter <las files...> -o <outdir> -m <method> [--attributes ...]."""
+    paths, n = _strings(las_paths)
+    names, na = _strings(attributes)
+    t = (ctypes.c_double * 8)()
+    _check(lib().pcg_convert_sources(paths, n, str(outdir).encode(), method.encode(),
+                                     encoding.encode() if encoding else None, names if na else None, na,
+                                     1 if keep_chunks else 0, t))
     return list(t)
 
 

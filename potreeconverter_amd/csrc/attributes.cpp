@@ -212,6 +212,141 @@ bool output_attributes_for(const lasfile::Header& h, Attributes& out, ChunkerPar
   return true;
 }
 
+bool output_attributes_for_sources(const std::vector<lasfile::Header>& hs, const std::vector<std::string>& requested,
+                                   Attributes& out, std::vector<ChunkerParams>& params,
+                                   std::vector<std::vector<std::pair<std::string, int>>>& statTargets) {
+  if (hs.empty()) {
+    set_error("no sources");
+    return false;
+  }
+  std::vector<Attributes> own(hs.size());
+  params.assign(hs.size(), ChunkerParams());
+  statTargets.assign(hs.size(), {});
+  // PotreeConverter.h:195-258: the union of the sources' attribute lists by
+  // name (first definition wins; sources in the caller's order), the smallest
+  // LAS scale and the bounds of all headers
+  std::vector<Attribute> full;
+  Vec3 scaleMin(INFINITY, INFINITY, INFINITY), mn(INFINITY, INFINITY, INFINITY), mx(-INFINITY, -INFINITY, -INFINITY);
+  for (size_t s = 0; s < hs.size(); s++) {
+    if (!output_attributes_for(hs[s], own[s], params[s])) return false;
+    for (const Attribute& a : own[s].list) {
+      bool seen = false;
+      for (const Attribute& b : full) seen = seen || b.name == a.name;
+      if (!seen) full.push_back(a);
+    }
+    for (int k = 0; k < 3; k++) {
+      scaleMin[k] = std::min(scaleMin[k], hs[s].scale[k]);
+      mn[k] = std::min(mn[k], hs[s].min[k]);
+      mx[k] = std::max(mx[k], hs[s].max[k]);
+    }
+  }
+  // :260-283: position first, then the requested names that exist, in the
+  // order requested (a name requested twice is listed twice)
+  std::vector<Attribute> list = full;
+  if (!requested.empty()) {
+    std::vector<std::string> should{"position"};
+    should.insert(should.end(), requested.begin(), requested.end());
+    list.clear();
+    for (const std::string& nm : should)
+      for (const Attribute& a : full)
+        if (a.name == nm) {
+          list.push_back(a);
+          break;
+        }
+  }
+  out = Attributes(list);
+  // computeScaleOffset (:29-76)
+  Vec3 size = mx - mn, scale = scaleMin;
+  for (int k = 0; k < 3; k++) scale[k] = std::max(scale[k], size[k] / pow(2.0, 30.0));
+  out.posScale = scale;
+  out.posOffset = mn;
+  for (size_t s = 0; s < hs.size(); s++) {
+    const Attributes& A = own[s];
+    ChunkerParams& P = params[s];
+    // the source's own layout -> the output layout, by attribute name
+    // (createAttributeHandlers, chunker_countsort_laszip.cpp:380-546): ops of
+    // attributes the output lacks are dropped
+    auto owner = [&](int dst, std::string& name, int& base) {
+      int o = 0;
+      for (const Attribute& a : A.list) {
+        if (dst >= o && dst < o + a.size) {
+          name = a.name;
+          base = o;
+          return true;
+        }
+        o += a.size;
+      }
+      return false;
+    };
+    size_t numStandard = 0;
+    {
+      ChunkerParams tmp;
+      Attributes a2;
+      lasfile::Header h0 = hs[s];
+      h0.extra.clear();
+      output_attributes_for(h0, a2, tmp);
+      numStandard = a2.list.size();
+    }
+    int extraStart = 0;
+    for (size_t i = 0; i < numStandard; i++) extraStart += A.list[i].size;
+    std::vector<TranscodeOp> ops;
+    for (int i = 0; i < P.num_ops; i++) {
+      TranscodeOp op = P.ops[i];
+      if (op.dst >= extraStart) continue;  // the extra bytes, below
+      std::string name;
+      int base = 0;
+      if (!owner(op.dst, name, base)) continue;
+      const int o = out.getOffset(name);
+      if (o < 0) continue;
+      op.dst = o + (op.dst - base);
+      ops.push_back(op);
+    }
+    // extra attributes (:548-651): the reference writes each included one at
+    // the offset it has in the SOURCE's layout (the sizes of the source's
+    // standard attributes plus those of the included extra attributes before
+    // it), not at the output attribute's offset; a layout where the two
+    // differ would write outside the attribute, so it is refused
+    static const int core[] = {20, 28, 26, 34, 57, 63, 30, 36};
+    int src = core[hs[s].point_format], refDst = extraStart;
+    for (size_t i = numStandard; i < A.list.size(); i++) {
+      const Attribute& a = A.list[i];
+      const int o = out.getOffset(a.name);
+      if (o >= 0) {
+        if (o != refDst) {
+          set_error("extra attribute \"" + a.name + "\" of source " + std::to_string(s) + " would land at byte " +
+                    std::to_string(refDst) + " of the output record instead of its attribute's byte " +
+                    std::to_string(o) + " (chunker_countsort_laszip.cpp:579-582 writes it at its source-layout "
+                    "offset): attribute filters and source mixes that move extra bytes are not supported");
+          return false;
+        }
+        ops.push_back(TranscodeOp{src, o, a.size, TC_COPY});
+        refDst += a.size;
+      }
+      src += a.size;
+    }
+    if ((int)ops.size() > kMaxOps) {
+      set_error("too many attribute copies for one source");
+      return false;
+    }
+    P.num_ops = (int32_t)ops.size();
+    for (size_t i = 0; i < ops.size(); i++) P.ops[i] = ops[i];
+    P.bpp_out = out.bytes;
+    if (!out.get("classification")) P.class_src = -1;
+    // which output attribute component each statistic of this source folds
+    // (the count kernel's order: position, then every attribute with a
+    // handler, one per component)
+    auto& st = statTargets[s];
+    for (int k = 0; k < 3; k++) st.emplace_back("position", k);
+    for (size_t i = 1; i < A.list.size(); i++) {
+      const Attribute& a = A.list[i];
+      if (!has_statistics(a)) continue;
+      const int comps = a.numElements < 3 ? a.numElements : 3;
+      for (int e = 0; e < comps; e++) st.emplace_back(out.get(a.name) ? a.name : std::string(), e);
+    }
+  }
+  return true;
+}
+
 std::string fmt17(double v) {
   char buf[64];
   if (std::isinf(v)) return v > 0 ? "inf" : "-inf";

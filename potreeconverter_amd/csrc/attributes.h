@@ -81,6 +81,15 @@ struct Attributes {
 // unsupported point formats.
 bool output_attributes_for(const lasfile::Header& h, Attributes& out, ChunkerParams& params);
 
+// The output attributes of several sources with an optional filter
+// (computeOutputAttributes(sources, requestedAttributes),
+// PotreeConverter.h:195-315) and each source's transcode / statistics tables
+// into that layout; statTargets[s][k] names the output attribute (empty:
+// not in the output) and component that statistic k of source s folds.
+bool output_attributes_for_sources(const std::vector<lasfile::Header>& hs, const std::vector<std::string>& requested,
+                                   Attributes& out, std::vector<ChunkerParams>& params,
+                                   std::vector<std::vector<std::pair<std::string, int>>>& statTargets);
+
 // chunks/metadata.json text (chunker_countsort_laszip.cpp:1005-1068).
 // whether the chunker folds min/max for the attribute (the reference has a
 // handler for it)

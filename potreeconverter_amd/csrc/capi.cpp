@@ -59,6 +59,84 @@ int pcg_do_chunking(const char* las_path, const char* outdir, double* timings) {
   return rc;
 }
 
+static void fill_timings(double* t, const DeviceChunks& dc, const IndexResult& r, double tWrite);
+
+// every source file's records in HBM (freed by the caller)
+static int load_sources(const char* const* las_paths, int32_t num_sources, std::vector<ChunkSource>& src,
+                        std::vector<uint8_t*>& bufs) {
+  if (num_sources < 1 || !las_paths) {
+    set_error("no source files");
+    return 1;
+  }
+  for (int32_t i = 0; i < num_sources; i++) {
+    lasfile::Header h;
+    uint8_t* d = nullptr;
+    int rc = load_las_to_device(las_paths[i], h, &d);
+    if (d) bufs.push_back(d);
+    if (rc) return rc;
+    src.push_back(ChunkSource{d, h});
+  }
+  return 0;
+}
+
+static void free_sources(std::vector<uint8_t*>& bufs) {
+  for (uint8_t* d : bufs) (void)hipFree(d);
+  bufs.clear();
+}
+
+static std::vector<std::string> name_list(const char* const* names, int32_t n) {
+  std::vector<std::string> v;
+  for (int32_t i = 0; names && i < n; i++) v.push_back(names[i] ? names[i] : "");
+  return v;
+}
+
+int pcg_do_chunking_sources(const char* const* las_paths, int32_t num_sources, const char* outdir, const double* min,
+                            const double* max, const char* const* attributes, int32_t num_attributes,
+                            double* timings) {
+  if (ensure_device()) return 1;
+  if ((min == nullptr) != (max == nullptr)) {
+    set_error("pcg_do_chunking_sources: give both min and max, or neither");
+    return 1;
+  }
+  std::vector<ChunkSource> src;
+  std::vector<uint8_t*> bufs;
+  int rc = load_sources(las_paths, num_sources, src, bufs);
+  DeviceChunks dc;
+  if (!rc) rc = gpu_chunking_sources(src, min, max, name_list(attributes, num_attributes), dc);
+  free_sources(bufs);
+  if (!rc) rc = write_chunk_files(dc, outdir);
+  if (timings) { timings[0] = dc.t_count; timings[1] = dc.t_distribute; timings[2] = dc.t_total; }
+  dc.release();
+  return rc;
+}
+
+int pcg_convert_sources(const char* const* las_paths, int32_t num_sources, const char* outdir, const char* method,
+                        const char* encoding, const char* const* attributes, int32_t num_attributes, int32_t flags,
+                        double* timings) {
+  if (ensure_device()) return 1;
+  std::vector<ChunkSource> src;
+  std::vector<uint8_t*> bufs;
+  int rc = load_sources(las_paths, num_sources, src, bufs);
+  DeviceChunks dc;
+  if (!rc) rc = gpu_chunking_sources(src, nullptr, nullptr, name_list(attributes, num_attributes), dc);
+  free_sources(bufs);
+  if (rc) { dc.release(); return rc; }
+  fs::create_directories(outdir);
+  if (flags & 1) {
+    rc = write_chunk_files(dc, outdir);
+    if (rc) { dc.release(); return rc; }
+  }
+  IndexResult r;
+  // the name is the first source's file stem (main.cpp:151-203)
+  const std::string name = fs::path(las_paths[0]).stem().string();
+  rc = gpu_indexing(dc, method, name, dc.pointsTotal, r, encoding ? encoding : "DEFAULT");
+  if (!rc) rc = write_index_files(r, outdir);
+  fill_timings(timings, dc, r, 0.0);
+  dc.release();
+  r.release();
+  return rc;
+}
+
 static void fill_timings(double* t, const DeviceChunks& dc, const IndexResult& r, double tWrite) {
   (void)tWrite;
   if (!t) return;

@@ -44,6 +44,17 @@ bool distribute_tail_fits(int32_t num_chunks);
 int chunker_distribute_tail(const uint8_t* d_las, const uint32_t* d_cells, uint16_t* d_ids, int64_t n,
                             const int32_t* d_lut, int32_t num_chunks, uint32_t* d_M, void* d_scan_ws,
                             uint8_t* d_out, int32_t* d_err, hipStream_t stream);
+// The two halves of a distribute pass, per source: the tile histogram of
+// one source's points (its tiles are columns tile_base .. of the chunk x
+// tile matrix d_M, whose rows hold `stride` tiles; with d_ids16 the chunk
+// ids go there as 16 bits, else over the cells) and, after the scan of the
+// whole matrix, the placement of that source's records (the tile transcode
+// with d_ids16, the byte-wise scatter without).  Tiles of 4096 points.
+int chunker_tile_hist(uint32_t* d_cells, uint16_t* d_ids16, int64_t n, const int32_t* d_lut, int32_t num_chunks,
+                      uint32_t* d_M, int64_t tile_base, int64_t stride, int32_t* d_err, hipStream_t stream);
+int chunker_place(const uint8_t* d_las, const uint32_t* d_ids32, const uint16_t* d_ids16, int64_t n,
+                  int32_t num_chunks, const uint32_t* d_M, int64_t tile_base, int64_t stride, uint8_t* d_out,
+                  hipStream_t stream);
 int lasgen_device(const LasGenParams& p, int64_t first, int64_t count, uint8_t* d_out, int32_t* d_qminmax,
                   hipStream_t stream);
 

@@ -1310,56 +1310,61 @@ int chunker_lut_fill(const int64_t* d_chunks, int64_t num_chunks, int32_t level_
   return 0;
 }
 
-int chunker_distribute(const uint8_t* d_las, uint32_t* d_ids, int64_t n, const int32_t* d_lut, int32_t num_chunks,
-                       int32_t tile_size, uint32_t* d_M, void* d_scan_ws, uint8_t* d_out, int32_t* d_err,
-                       hipStream_t stream) {
-  int64_t tiles = (n + tile_size - 1) / tile_size;
-  if (tiles == 0) return 0;
-  if (tile_size % (4 * kBlock) != 0 || ((uintptr_t)d_ids & 15) != 0) {
-    set_error("chunk id tiles must be multiples of 1024 points on a 16-byte aligned array");
-    return 1;
-  }
-  size_t hist_lds = num_chunks <= 8192 ? sizeof(uint32_t) * num_chunks : 0;
-  if (num_chunks > 8192) PCG_CHECK(hipMemsetAsync(d_M, 0, sizeof(uint32_t) * tiles * num_chunks, stream));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_hist<uint32_t>), dim3((unsigned)tiles), dim3(kBlock), hist_lds, stream,
-                     d_ids, d_ids, n, tile_size, d_lut, num_chunks, tiles, d_M, d_err);
-  PCG_CHECK(hipGetLastError());
-  if (exclusive_scan_u32(d_M, tiles * num_chunks, d_scan_ws, nullptr, stream)) return 1;
-  int key_bits = 1;
-  while ((1 << key_bits) < num_chunks) key_bits++;
-  size_t cnt_lds = scatter_lds_bytes(num_chunks, g_bpp_in, g_bpp_out);
-  if (cnt_lds > 150 * 1024) {
-    set_error("too many chunks for the LDS scatter counters");
-    return 1;
-  }
-  // the byte-wise transcode of any record layout (the layouts of point
-  // formats 0-3 and 6-7 normally take chunker_distribute_tail)
-  hipLaunchKernelGGL(k_scatter, dim3((unsigned)tiles), dim3(kBlock), cnt_lds, stream, d_las, d_ids, n, tile_size,
-                     num_chunks, key_bits, tiles, d_M, d_out);
-  PCG_CHECK(hipGetLastError());
-  return 0;
-}
-
 bool distribute_tail_fits(int32_t num_chunks) {
   return (g_tail_layout == 0 || g_tail_layout == 1) && g_bpp_in <= 64 && num_chunks < (int32_t)kNoChunk16 &&
          distribute_lds_bytes(num_chunks, g_bpp_in, g_bpp_out) <= 150 * 1024;
 }
 
-int chunker_distribute_tail(const uint8_t* d_las, const uint32_t* d_cells, uint16_t* d_ids, int64_t n,
-                            const int32_t* d_lut, int32_t num_chunks, uint32_t* d_M, void* d_scan_ws,
-                            uint8_t* d_out, int32_t* d_err, hipStream_t stream) {
+int chunker_tile_hist(uint32_t* d_cells, uint16_t* d_ids16, int64_t n, const int32_t* d_lut, int32_t num_chunks,
+                      uint32_t* d_M, int64_t tile_base, int64_t stride, int32_t* d_err, hipStream_t stream) {
   const int64_t tiles = (n + kDistTile - 1) / kDistTile;
   if (tiles == 0) return 0;
+  if (tile_base < 0 || tile_base + tiles > stride || ((uintptr_t)d_cells & 15) != 0 ||
+      (d_ids16 && ((uintptr_t)d_ids16 & 7) != 0)) {
+    set_error("tile histogram: tiles outside the matrix or misaligned id arrays");
+    return 1;
+  }
+  const size_t hist_lds = num_chunks <= 8192 ? sizeof(uint32_t) * num_chunks : 0;
+  if (d_ids16) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_hist<uint16_t>), dim3((unsigned)tiles), dim3(kBlock), hist_lds, stream,
+                       d_cells, d_ids16, n, kDistTile, d_lut, num_chunks, stride, d_M + tile_base, d_err);
+  } else {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_hist<uint32_t>), dim3((unsigned)tiles), dim3(kBlock), hist_lds, stream,
+                       d_cells, d_cells, n, kDistTile, d_lut, num_chunks, stride, d_M + tile_base, d_err);
+  }
+  PCG_CHECK(hipGetLastError());
+  return 0;
+}
+
+int chunker_place(const uint8_t* d_las, const uint32_t* d_ids32, const uint16_t* d_ids16, int64_t n,
+                  int32_t num_chunks, const uint32_t* d_M, int64_t tile_base, int64_t stride, uint8_t* d_out,
+                  hipStream_t stream) {
+  const int64_t tiles = (n + kDistTile - 1) / kDistTile;
+  if (tiles == 0) return 0;
+  if (tile_base < 0 || tile_base + tiles > stride) {
+    set_error("placement: tiles outside the matrix");
+    return 1;
+  }
+  const uint32_t* M = d_M + tile_base;
+  if (!d_ids16) {
+    // the byte-wise transcode of any record layout (the layouts of point
+    // formats 0-3 and 6-7 normally take the tile transcode)
+    int key_bits = 1;
+    while ((1 << key_bits) < num_chunks) key_bits++;
+    const size_t cnt_lds = scatter_lds_bytes(num_chunks, g_bpp_in, g_bpp_out);
+    if (cnt_lds > 150 * 1024) {
+      set_error("too many chunks for the LDS scatter counters");
+      return 1;
+    }
+    hipLaunchKernelGGL(k_scatter, dim3((unsigned)tiles), dim3(kBlock), cnt_lds, stream, d_las, d_ids32, n, kDistTile,
+                       num_chunks, key_bits, stride, M, d_out);
+    PCG_CHECK(hipGetLastError());
+    return 0;
+  }
   if (!distribute_tail_fits(num_chunks)) {
     set_error("record layout or chunk count outside the tile transcode");
     return 1;
   }
-  size_t hist_lds = num_chunks <= 8192 ? sizeof(uint32_t) * num_chunks : 0;
-  if (num_chunks > 8192) PCG_CHECK(hipMemsetAsync(d_M, 0, sizeof(uint32_t) * tiles * num_chunks, stream));
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tile_hist<uint16_t>), dim3((unsigned)tiles), dim3(kBlock), hist_lds, stream,
-                     d_cells, d_ids, n, kDistTile, d_lut, num_chunks, tiles, d_M, d_err);
-  PCG_CHECK(hipGetLastError());
-  if (exclusive_scan_u32(d_M, tiles * num_chunks, d_scan_ws, nullptr, stream)) return 1;
   const size_t lds = distribute_lds_bytes(num_chunks, g_bpp_in, g_bpp_out);
   static bool attr = false;
   if (!attr) {
@@ -1375,7 +1380,7 @@ int chunker_distribute_tail(const uint8_t* d_las, const uint32_t* d_cells, uint1
   }
 #define PCG_DIST(E, V)                                                                                    \
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_distribute<E, V>), dim3((unsigned)tiles), dim3(kBlock), lds, stream, d_las, \
-                     d_ids, n, num_chunks, tiles, d_M, d_out)
+                     d_ids16, n, num_chunks, stride, M, d_out)
   const bool short_rec = g_bpp_in <= 32;
   if (g_tail_layout == 0) {
     if (short_rec) PCG_DIST(false, prefetch_vecs(32));
@@ -1387,6 +1392,37 @@ int chunker_distribute_tail(const uint8_t* d_las, const uint32_t* d_cells, uint1
 #undef PCG_DIST
   PCG_CHECK(hipGetLastError());
   return 0;
+}
+
+int chunker_distribute(const uint8_t* d_las, uint32_t* d_ids, int64_t n, const int32_t* d_lut, int32_t num_chunks,
+                       int32_t tile_size, uint32_t* d_M, void* d_scan_ws, uint8_t* d_out, int32_t* d_err,
+                       hipStream_t stream) {
+  const int64_t tiles = (n + kDistTile - 1) / kDistTile;
+  if (tiles == 0) return 0;
+  if (tile_size != kDistTile) {
+    set_error("chunk id tiles are 4096 points");
+    return 1;
+  }
+  if (num_chunks > 8192) PCG_CHECK(hipMemsetAsync(d_M, 0, sizeof(uint32_t) * tiles * num_chunks, stream));
+  if (chunker_tile_hist(d_ids, nullptr, n, d_lut, num_chunks, d_M, 0, tiles, d_err, stream)) return 1;
+  if (exclusive_scan_u32(d_M, tiles * num_chunks, d_scan_ws, nullptr, stream)) return 1;
+  return chunker_place(d_las, d_ids, nullptr, n, num_chunks, d_M, 0, tiles, d_out, stream);
+}
+
+int chunker_distribute_tail(const uint8_t* d_las, const uint32_t* d_cells, uint16_t* d_ids, int64_t n,
+                            const int32_t* d_lut, int32_t num_chunks, uint32_t* d_M, void* d_scan_ws,
+                            uint8_t* d_out, int32_t* d_err, hipStream_t stream) {
+  const int64_t tiles = (n + kDistTile - 1) / kDistTile;
+  if (tiles == 0) return 0;
+  if (!distribute_tail_fits(num_chunks)) {
+    set_error("record layout or chunk count outside the tile transcode");
+    return 1;
+  }
+  if (num_chunks > 8192) PCG_CHECK(hipMemsetAsync(d_M, 0, sizeof(uint32_t) * tiles * num_chunks, stream));
+  if (chunker_tile_hist(const_cast<uint32_t*>(d_cells), d_ids, n, d_lut, num_chunks, d_M, 0, tiles, d_err, stream))
+    return 1;
+  if (exclusive_scan_u32(d_M, tiles * num_chunks, d_scan_ws, nullptr, stream)) return 1;
+  return chunker_place(d_las, nullptr, d_ids, n, num_chunks, d_M, 0, tiles, d_out, stream);
 }
 
 int lasgen_device(const LasGenParams& p, int64_t first, int64_t count, uint8_t* d_out, int32_t* d_qminmax,

@@ -156,25 +156,64 @@ struct DevBuf {
 };
 
 int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& out) {
+  return gpu_chunking_sources(std::vector<ChunkSource>{ChunkSource{d_las, h}}, nullptr, nullptr, {}, out);
+}
+
+int gpu_chunking_sources(const std::vector<ChunkSource>& src, const double* cubeMin, const double* cubeMax,
+                         const std::vector<std::string>& requested, DeviceChunks& out) {
   if (ensure_device()) return 1;
   Device& dev = device();
   hipStream_t s = dev.stream;
   double t0 = now_s();
-  ChunkerParams P;
   out.release();
-  if (!output_attributes_for(h, out.attributes, P)) return 1;
+  const int S = (int)src.size();
+  if (S < 1) {
+    set_error("no sources");
+    return 1;
+  }
+  // output attributes of all sources and each source's transcode and
+  // statistics tables (PotreeConverter.h:195-315)
+  std::vector<lasfile::Header> hs;
+  for (const ChunkSource& c : src) hs.push_back(c.header);
+  std::vector<ChunkerParams> Ps;
+  std::vector<std::vector<std::pair<std::string, int>>> statTargets;
+  if (!output_attributes_for_sources(hs, requested, out.attributes, Ps, statTargets)) return 1;
   Attributes& A = out.attributes;
 
-  // main.cpp:215-284 (computeStats) then chunker_countsort_laszip.cpp:205-207
-  Vec3 hmin(h.min[0], h.min[1], h.min[2]), hmax(h.max[0], h.max[1], h.max[2]);
-  double statsCube = (hmax - hmin).max();
-  if (statsCube == 0.0) {
+  // main.cpp:215-284 (computeStats over the sources' headers, or the
+  // caller's box) then chunker_countsort_laszip.cpp:205-207
+  Vec3 hmin, hmax;
+  if (cubeMin && cubeMax) {
+    hmin = Vec3(cubeMin[0], cubeMin[1], cubeMin[2]);
+    hmax = Vec3(cubeMax[0], cubeMax[1], cubeMax[2]);
+  } else {
+    hmin = Vec3(INFINITY, INFINITY, INFINITY);
+    hmax = Vec3(-INFINITY, -INFINITY, -INFINITY);
+    for (const auto& h : hs)
+      for (int k = 0; k < 3; k++) {
+        hmin[k] = std::min(hmin[k], h.min[k]);
+        hmax[k] = std::max(hmax[k], h.max[k]);
+      }
+    const double statsCube = (hmax - hmin).max();
+    if (statsCube == 0.0) {
+      set_error("invalid bounding box. at least one axis has a size of zero.");
+      return 123;
+    }
+    hmax = hmin + statsCube;
+  }
+  double cube = (hmax - hmin).max();
+  if (!(cube > 0.0)) {
     set_error("invalid bounding box. at least one axis has a size of zero.");
     return 123;
   }
-  Vec3 smax = hmin + statsCube;
-  double cube = (smax - hmin).max();
-  int64_t N = (int64_t)h.num_points;
+  int64_t N = 0;
+  std::vector<int64_t> firstPoint(S + 1, 0), idOff(S + 1, 0);
+  for (int i = 0; i < S; i++) {
+    N += (int64_t)hs[i].num_points;
+    firstPoint[i + 1] = N;
+    // each source's id slice starts 16-byte aligned (4 ids)
+    idOff[i + 1] = idOff[i] + (((int64_t)hs[i].num_points + 3) & ~3ll);
+  }
   if (N >= (int64_t)4000000000LL) {
     set_error("point counts of 4e9 and above are not supported");
     return 1;
@@ -190,41 +229,51 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
   int32_t Lmax = G == 128 ? 7 : (G == 256 ? 8 : 9);
   out.gridSize = G;
   out.maxPointsPerChunk = cap;
-  for (int a = 0; a < 3; a++) {
-    P.las_scale[a] = h.scale[a];
-    P.las_offset[a] = h.offset[a];
-    P.pos_scale[a] = A.posScale[a];
-    P.pos_offset[a] = A.posOffset[a];
-    P.min[a] = hmin[a];
+  for (int i = 0; i < S; i++) {
+    ChunkerParams& P = Ps[i];
+    for (int a = 0; a < 3; a++) {
+      P.las_scale[a] = hs[i].scale[a];
+      P.las_offset[a] = hs[i].offset[a];
+      P.pos_scale[a] = A.posScale[a];
+      P.pos_offset[a] = A.posOffset[a];
+      P.min[a] = hmin[a];
+    }
+    P.cube = cube;
+    P.grid_size = G;
   }
-  P.cube = cube;
-  P.grid_size = G;
-  if (chunker_set_params(P, s)) return 1;
 
   const int64_t cells = (int64_t)G * G * G;
   DevBuf grid, stats, hist, err, ids;
-  if (grid.alloc(WS_GRID, sizeof(int32_t) * cells) || stats.alloc(WS_STATS, sizeof(uint64_t) * 2 * kMaxStats) ||
-      ids.alloc(WS_POINT_IDS, sizeof(uint32_t) * std::max<int64_t>(N, 1) + 64) ||
+  if (grid.alloc(WS_GRID, sizeof(int32_t) * cells) ||
+      stats.alloc(WS_STATS, sizeof(uint64_t) * 2 * kMaxStats * S) ||
+      ids.alloc(WS_POINT_IDS, sizeof(uint32_t) * std::max<int64_t>(idOff[S], 1) + 64) ||
       hist.alloc(WS_HIST, sizeof(unsigned long long) * 256) || err.alloc(WS_ERR, sizeof(int32_t) * 4))
     return 1;
   PCG_CHECK(hipMemsetAsync(grid.p, 0, sizeof(int32_t) * cells, s));
-  PCG_CHECK(hipMemsetAsync(stats.p, 0xFF, sizeof(uint64_t) * kMaxStats, s));
-  PCG_CHECK(hipMemsetAsync(stats.as<uint64_t>() + kMaxStats, 0, sizeof(uint64_t) * kMaxStats, s));
+  for (int i = 0; i < S; i++) {
+    PCG_CHECK(hipMemsetAsync(stats.as<uint64_t>() + 2 * kMaxStats * i, 0xFF, sizeof(uint64_t) * kMaxStats, s));
+    PCG_CHECK(hipMemsetAsync(stats.as<uint64_t>() + 2 * kMaxStats * i + kMaxStats, 0, sizeof(uint64_t) * kMaxStats, s));
+  }
   PCG_CHECK(hipMemsetAsync(hist.p, 0, sizeof(unsigned long long) * 256, s));
   PCG_CHECK(hipMemsetAsync(err.p, 0, sizeof(int32_t) * 4, s));
   static hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   if (!ev[0])
     for (int i = 0; i < 4; i++) PCG_CHECK(hipEventCreate(&ev[i]));
-  PCG_CHECK(hipEventRecord(ev[0], s));
-  if (chunker_count(d_las, N, grid.as<int32_t>(), ids.as<uint32_t>(), stats.as<uint64_t>(), stats.as<uint64_t>() + kMaxStats,
-                    hist.as<unsigned long long>(), err.as<int32_t>(), dev.num_cus, s))
-    return 1;
+  // countPointsInCells (:131-361): every source's points into one grid
+  for (int i = 0; i < S; i++) {
+    if (chunker_set_params(Ps[i], s)) return 1;
+    if (i == 0) PCG_CHECK(hipEventRecord(ev[0], s));
+    uint64_t* st = stats.as<uint64_t>() + 2 * kMaxStats * i;
+    if (chunker_count(src[i].d_las, (int64_t)hs[i].num_points, grid.as<int32_t>(), ids.as<uint32_t>() + idOff[i], st,
+                      st + kMaxStats, hist.as<unsigned long long>(), err.as<int32_t>(), dev.num_cus, s))
+      return 1;
+  }
   PCG_CHECK(hipEventRecord(ev[1], s));
   int32_t herr[4];
-  uint64_t hstats[2 * kMaxStats];
+  std::vector<uint64_t> hstats((size_t)2 * kMaxStats * S);
   unsigned long long hhist[256];
   PCG_CHECK(hipMemcpyAsync(herr, err.p, sizeof(herr), hipMemcpyDeviceToHost, s));
-  PCG_CHECK(hipMemcpyAsync(hstats, stats.p, sizeof(hstats), hipMemcpyDeviceToHost, s));
+  PCG_CHECK(hipMemcpyAsync(hstats.data(), stats.p, sizeof(uint64_t) * hstats.size(), hipMemcpyDeviceToHost, s));
   PCG_CHECK(hipMemcpyAsync(hhist, hist.p, sizeof(hhist), hipMemcpyDeviceToHost, s));
   PCG_CHECK(hipStreamSynchronize(s));
   if (herr[0] & 1) {
@@ -234,30 +283,27 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
   double t1 = now_s();
   out.t_count = t1 - t0;
 
-  // attribute min/max (chunker_countsort_laszip.cpp:385-389, :914-932);
-  // position uses the decoded coordinate, monotone in the stored integer
-  {
-    int si = 0;
-    auto mn = [&](int k) { return from_ordered_bits(hstats[k]); };
-    auto mx = [&](int k) { return from_ordered_bits(hstats[kMaxStats + k]); };
-    Attribute& pos = A.list[0];
-    for (int a = 0; a < 3; a++) {
-      pos.min[a] = h.scale[a] * mn(si) + h.offset[a];
-      pos.max[a] = h.scale[a] * mx(si) + h.offset[a];
-      si++;
-    }
-    for (size_t i = 1; i < A.list.size(); i++) {
-      Attribute& at = A.list[i];
-      if (!has_statistics(at)) continue;
-      int comps = at.numElements < 3 ? at.numElements : 3;
-      for (int e = 0; e < comps; e++) {
-        if (N > 0) { at.min[e] = mn(si); at.max[e] = mx(si); }
-        si++;
+  // attribute min/max (chunker_countsort_laszip.cpp:385-389, :914-932),
+  // merged over the sources; position uses the decoded coordinate, monotone
+  // in the stored integer
+  for (int i = 0; i < S; i++) {
+    if (hs[i].num_points == 0) continue;
+    const uint64_t* hsi = hstats.data() + 2 * kMaxStats * i;
+    for (size_t k = 0; k < statTargets[i].size(); k++) {
+      const auto& tg = statTargets[i][k];
+      Attribute* at = tg.first.empty() ? nullptr : A.get(tg.first);
+      if (!at) continue;
+      double mn = from_ordered_bits(hsi[k]), mx = from_ordered_bits(hsi[kMaxStats + k]);
+      if (tg.first == "position") {
+        mn = hs[i].scale[tg.second] * mn + hs[i].offset[tg.second];
+        mx = hs[i].scale[tg.second] * mx + hs[i].offset[tg.second];
       }
+      at->min[tg.second] = std::min(at->min[tg.second], mn);
+      at->max[tg.second] = std::max(at->max[tg.second], mx);
     }
-    Attribute* cls = A.get("classification");
-    if (cls) for (int i = 0; i < 256; i++) cls->histogram[i] = (int64_t)hhist[i];
   }
+  if (Attribute* cls = A.get("classification"))
+    for (int i = 0; i < 256; i++) cls->histogram[i] = (int64_t)hhist[i];
 
   // pyramid + chunk cells (:1074-1183)
   std::vector<DevBuf> levels(Lmax);
@@ -317,29 +363,43 @@ int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& o
   if (chunker_lut_fill(dchunks.as<int64_t>(), (int64_t)out.chunks.size(), Lmax, lut.as<int32_t>(), s)) return 1;
   levels.clear();
 
-  // distribute (:660-1003)
-  const int32_t tile = 4096;
-  int64_t tiles = (N + tile - 1) / tile;
-  int32_t C = (int32_t)out.chunks.size();
+  // distribute (:660-1003): the tiles of every source side by side in one
+  // chunk x tile matrix, so that the scan puts each chunk's points in source
+  // order, then input order
+  const int32_t C = (int32_t)out.chunks.size();
+  std::vector<int64_t> tileBase(S + 1, 0);
+  for (int i = 0; i < S; i++) tileBase[i + 1] = tileBase[i] + ((int64_t)hs[i].num_points + 4095) / 4096;
+  const int64_t tiles = tileBase[S];
   out.d_records = (uint8_t*)workspace(WS_RECORDS, (size_t)N * out.bpp + 64);
   out.owned = false;
   if (!out.d_records) return 1;
   // the distribute kernels write every byte of every output record (unset bytes 0)
-  DevBuf M, ws;
+  DevBuf M, ws, ids16;
   if (M.alloc(WS_TILE_M, sizeof(uint32_t) * std::max<int64_t>(1, tiles * C)) ||
       ws.alloc(WS_SCAN, scan_workspace_bytes(tiles * C)))
     return 1;
-  if (g_distribute_tail && distribute_tail_fits(C)) {
-    DevBuf ids16;
-    if (ids16.alloc(WS_CHUNK_IDS16, sizeof(uint16_t) * std::max<int64_t>(N, 1) + 64)) return 1;
-    PCG_CHECK(hipEventRecord(ev[2], s));
-    if (chunker_distribute_tail(d_las, ids.as<uint32_t>(), ids16.as<uint16_t>(), N, lut.as<int32_t>(), C,
-                                M.as<uint32_t>(), ws.p, out.d_records, err.as<int32_t>(), s))
+  std::vector<char> tail(S, 0);
+  bool anyTail = false;
+  for (int i = 0; i < S; i++) {
+    if (chunker_set_params(Ps[i], s)) return 1;
+    tail[i] = g_distribute_tail && distribute_tail_fits(C);
+    anyTail = anyTail || tail[i];
+  }
+  if (anyTail && ids16.alloc(WS_CHUNK_IDS16, sizeof(uint16_t) * std::max<int64_t>(idOff[S], 1) + 64)) return 1;
+  PCG_CHECK(hipEventRecord(ev[2], s));
+  if (C > 8192 && tiles > 0) PCG_CHECK(hipMemsetAsync(M.p, 0, sizeof(uint32_t) * tiles * C, s));
+  for (int i = 0; i < S; i++) {
+    if (S > 1 && chunker_set_params(Ps[i], s)) return 1;
+    if (chunker_tile_hist(ids.as<uint32_t>() + idOff[i], tail[i] ? ids16.as<uint16_t>() + idOff[i] : nullptr,
+                          (int64_t)hs[i].num_points, lut.as<int32_t>(), C, M.as<uint32_t>(), tileBase[i], tiles,
+                          err.as<int32_t>(), s))
       return 1;
-  } else {
-    PCG_CHECK(hipEventRecord(ev[2], s));
-    if (chunker_distribute(d_las, ids.as<uint32_t>(), N, lut.as<int32_t>(), C, tile, M.as<uint32_t>(), ws.p,
-                           out.d_records, err.as<int32_t>(), s))
+  }
+  if (tiles > 0 && exclusive_scan_u32(M.as<uint32_t>(), tiles * C, ws.p, nullptr, s)) return 1;
+  for (int i = 0; i < S; i++) {
+    if (S > 1 && chunker_set_params(Ps[i], s)) return 1;
+    if (chunker_place(src[i].d_las, ids.as<uint32_t>() + idOff[i], tail[i] ? ids16.as<uint16_t>() + idOff[i] : nullptr,
+                      (int64_t)hs[i].num_points, C, M.as<uint32_t>(), tileBase[i], tiles, out.d_records, s))
       return 1;
   }
   PCG_CHECK(hipEventRecord(ev[3], s));

@@ -96,6 +96,21 @@ extern bool g_distribute_tail;
 // doChunking on LAS records already resident in HBM (header describes them).
 int gpu_chunking(const uint8_t* d_las, const lasfile::Header& h, DeviceChunks& out);
 
+// One source of a multi-source doChunking: its records in HBM and its header.
+struct ChunkSource {
+  const uint8_t* d_las;
+  lasfile::Header header;
+};
+// doChunking(sources, targetDir, min, max, state, outputAttributes, monitor)
+// (Converter/include/chunker_countsort_laszip.h:19) on sources resident in
+// HBM: cubeMin / cubeMax (both or neither) are the box main.cpp's
+// computeStats hands over (else computed from the headers the same way),
+// `requested` the --attributes filter (PotreeConverter.h:260-283; empty:
+// every attribute of every source).  Chunks hold each source's points in
+// source order, then input order.
+int gpu_chunking_sources(const std::vector<ChunkSource>& src, const double* cubeMin, const double* cubeMax,
+                         const std::vector<std::string>& requested, DeviceChunks& out);
+
 // Writes chunks/<id>.bin + chunks/metadata.json like the reference chunker.
 int write_chunk_files(const DeviceChunks& dc, const std::string& targetDir);
 

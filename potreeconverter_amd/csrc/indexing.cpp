@@ -1017,6 +1017,23 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
       cn.byteSize = recSize[id];
       out.completed.push_back(cn);
     }
+    // every hierarchy chunk's root (names of 1 + 4k characters) must be
+    // among the written nodes: HierarchyBuilder points to it from the batch
+    // above (HierarchyBuilder.h:213-226 exits with 123 when it is missing;
+    // :317-326 dereferences it).  Poisson-average unlinks a node without
+    // writing it when its parent accepts all its points (e.g. a sparse
+    // cluster in a large cube), which leaves its written descendants orphaned.
+    {
+      std::unordered_map<std::string, bool> written;
+      for (const CompletedNode& c : out.completed) written[c.key ? node_name(c.key) : c.name] = true;
+      for (const auto& kv : written)
+        for (size_t len = 5; len < kv.first.size(); len += 4)
+          if (!written.count(kv.first.substr(0, len))) {
+            set_error("didn't find proxy node " + kv.first.substr(0, len) + " (node " + kv.first +
+                      " was written, its hierarchy chunk root was not)");
+            return 123;
+          }
+    }
     out.hierarchy = build_hierarchy_bin(out.completed, out.firstChunkSize);
   }
   out.metadata = create_metadata_json(attributes, dc.min, dc.max, spacing, name, pointsTotal, out.firstChunkSize,

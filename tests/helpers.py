@@ -45,6 +45,31 @@ def oracle_convert(las, outdir, method, keep_chunks=False, chunk_only=False, enc
     return list(t)
 
 
+def oracle_convert_sources(paths, outdir, method, attributes=None, cube_min=None, cube_max=None, keep_chunks=False,
+                           chunk_only=False, check=True):
+    """The oracle's doChunking(sources, ..., min, max, ...) + doIndexing with
+    the --attributes filter; returns the status (asserts 0 with check)."""
+    O = oracle()
+    if not hasattr(O, "_sources_bound"):
+        O.orc_convert_sources.argtypes = [ctypes.POINTER(ctypes.c_char_p), ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_char_p), ctypes.c_int32, ctypes.c_char_p,
+                                          ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_double)]
+        O._sources_bound = True
+    ps = (ctypes.c_char_p * len(paths))(*[str(p).encode() for p in paths])
+    at = list(attributes or [])
+    names = (ctypes.c_char_p * max(len(at), 1))(*[a.encode() for a in at])
+    mn = (ctypes.c_double * 3)(*cube_min) if cube_min is not None else None
+    mx = (ctypes.c_double * 3)(*cube_max) if cube_max is not None else None
+    t = (ctypes.c_double * 8)()
+    rc = O.orc_convert_sources(ps, len(paths), mn, mx, names if at else None, len(at), str(outdir).encode(),
+                               method.encode(), int(keep_chunks), int(chunk_only), t)
+    if check:
+        assert rc == 0, O.orc_last_error()
+    return rc
+
+
 def oracle_index(outdir, method, name=""):
     """indexer::doIndexing of the oracle on outdir/chunks."""
     t = (ctypes.c_double * 8)()

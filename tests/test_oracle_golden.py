@@ -19,7 +19,7 @@ from scenes import make_scene
 # indexer fixtures (per-node digests); chunks_metadata_* and json_numbers pin
 # the chunk metadata text (tests/test_json_dump.py)
 GOLDEN = sorted(p for p in glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.json"))
-                if not os.path.basename(p).startswith(("chunks_metadata_", "json_")))
+                if not os.path.basename(p).startswith(("chunks_metadata_", "json_", "subtree_")))
 
 
 @pytest.mark.parametrize("fixture", GOLDEN, ids=[os.path.basename(p)[:-5] for p in GOLDEN])
