@@ -46,6 +46,8 @@ def parse():
                     help="c2: median chunk size of the batched-vs-plain comparison (0 skips it)")
     ap.add_argument("--sample-groups", type=int, default=None,
                     help="library option sample_groups (streams of the chunk-subtree sampling; default: the library's)")
+    ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
+                    help="library option (pcg_set_option) set before the run; repeatable")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--points", type=int, default=None, help="default: 200M (c1), 1B (c3)")
@@ -369,6 +371,10 @@ def main():
     if args.sample_groups is not None:
         import potreeconverter_amd as pc
         pc._check(pc.lib().pcg_set_option(b"sample_groups", args.sample_groups))
+    for kv in args.option:
+        import potreeconverter_amd as pc
+        key, value = kv.split("=", 1)
+        pc._check(pc.lib().pcg_set_option(key.encode(), int(value)))
     if args.config == "c2":
         c2_indexing(args)
         return

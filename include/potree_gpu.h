@@ -57,6 +57,9 @@ int pcg_release(void);
  * "distribute_tail" (0/1, default 1): 1 places points with the tile
  * transcode (16-bit chunk ids, aligned 16-byte run stores) where the record
  * layout allows, 0 with the byte-wise scatter (same results).
+ * "sample_order" (0/1, default 0): launch rounds of the node sampling by
+ * octree level, deepest first (0), or by height, a node in the first round
+ * after all its children (1) (same results).
  * The other options have no reference counterpart. */
 int pcg_set_option(const char* key, int64_t value);
 

@@ -195,6 +195,14 @@ int pcg_set_option(const char* key, int64_t value) {
     g_sample_sets = (int32_t)value;
     return 0;
   }
+  if (k == "sample_order") {
+    if (value != 0 && value != 1) {
+      set_error("sample_order must be 0 (octree levels) or 1 (heights)");
+      return 1;
+    }
+    g_sample_order = (int32_t)value;
+    return 0;
+  }
   if (k == "distribute_tail") {
     g_distribute_tail = value != 0;
     return 0;

@@ -4,7 +4,8 @@
 //    of the 32^3 counting sort (octree.hip); the host turns the realized
 //    cells into the node tree (expandTo, indexer.cpp:915-947).
 // 2. Sampling: the whole tree (root, nodes above the chunk roots, chunk
-//    subtrees) is sampled bottom-up one level per launch (sampler.hip).
+//    subtrees) is sampled bottom-up one round per launch sequence, a round
+//    being an octree level or a height (sampler.hip).
 //    Per-node results depend only on the children's states, so this equals
 //    the reference's per-chunk post-order followed by the chunk-root phase.
 // 3. Canonical completion order (per chunk in ascending ID, post-order,
@@ -165,6 +166,7 @@ int64_t g_random_seed = -1;
 int32_t g_ring_min_nodes = 512;
 int32_t g_sample_groups = 0;
 int32_t g_sample_sets = 0;
+int32_t g_sample_order = 0;
 
 void IndexResult::release() {
   d_octree = nullptr;  // a workspace buffer
@@ -549,13 +551,42 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   } else {
     groupPoints[0] = N;
   }
-  // level lists: group g in levels[g], the nodes above the chunk roots in
+  // Launch rounds.  A node samples the records its children kept, so any
+  // order that puts every node after its children gives the same records;
+  // the output is assembled in the canonical order afterwards.  By octree
+  // level (deepest first) a round also waits for unrelated deeper subtrees
+  // (dense spots reach many levels below the surrounding terrain); by height
+  // (1 + the largest height of its children, 0 for a leaf) each node runs in
+  // the first round after its children's, so the leaves' parents of every
+  // depth start at once.  Nodes of one height are never ancestor and
+  // descendant, so a round's candidates are distinct points, as a level's.
+  // Rounds are indexed like levels: round r runs when the loop reaches
+  // index maxLevel + 1 - r (heights run from 1 to at most maxLevel).
+  std::vector<int32_t> roundOf(numNodes, 0);
+  if (g_sample_order == 1) {
+    std::vector<std::vector<int32_t>> byLevel(maxLevel + 1);
+    for (int32_t i = 0; i < numNodes; i++) byLevel[info[i].level].push_back(i);
+    std::vector<int32_t> height(numNodes, 0);
+    for (int l = maxLevel; l >= 0; l--)
+      for (int32_t i : byLevel[l]) {
+        int32_t h = 0;
+        for (int c = 0; c < 8; c++) {
+          const int32_t ch = tree.nodes[i].children[c];
+          if (ch >= 0) h = std::max(h, height[ch] + 1);
+        }
+        height[i] = h;
+        roundOf[i] = maxLevel + 1 - h;
+      }
+  } else {
+    for (int32_t i = 0; i < numNodes; i++) roundOf[i] = info[i].level;
+  }
+  // round lists: group g in levels[g], the nodes above the chunk roots in
   // levels[G] (empty with one group)
   std::vector<std::vector<std::vector<int32_t>>> levels(G + 1, std::vector<std::vector<int32_t>>(maxLevel + 1));
   for (int32_t i = 0; i < numNodes; i++) {
     if (!tree.nodes[i].hasChildren() || (singleChunk && tree.nodes[i].upper)) continue;
     const int g = groupOf[i] >= 0 ? groupOf[i] : G;
-    levels[g][info[i].level].push_back(i);
+    levels[g][roundOf[i]].push_back(i);
   }
   std::vector<int64_t> maxLevelNodes(G + 1, 1);
   for (int g = 0; g <= G; g++)
@@ -633,6 +664,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   sp.arena_cap = arenaCap;
   sp.arena_top = gtop.as<uint64_t>();
   sp.counters = counters.as<unsigned long long>();
+  sp.ring_blocks = 3 * device().num_cus;  // three k_ring blocks fit a CU's LDS
   // a group's buffers for levels of at most `maxNodes` nodes and `cand`
   // candidates in all
   auto group_params = [&](GroupMem& m, int64_t maxNodes, int64_t cand, SampleParams& p) -> int {
@@ -654,6 +686,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     p.nbr_pool = m.pool.as<uint32_t>();
     p.nbr_pool_cap = poolCap;
     p.nbr_pool_top = (unsigned long long*)(m.tops.as<uint64_t>() + 2);
+    p.ring_ctl = (uint32_t*)(m.tops.as<uint64_t>() + 3);
     // the ring needs enough nodes per launch to fill the GPU; concurrent
     // streams share it
     p.ring_min_nodes = std::max(1, g_ring_min_nodes / S);
@@ -685,6 +718,11 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
       if (lst.empty()) return 0;
       sampled += (int64_t)lst.size();
       PCG_CHECK(hipMemsetAsync(m.tops.as<uint64_t>() + 1, 0, 16, st));  // scratch and pool tops
+      double tLevel = 0.0;
+      if (levelStats) {
+        PCG_CHECK(hipStreamSynchronize(st));
+        tLevel = now_s();
+      }
       if (g_plain_sampling) {
         // one launch sequence per node (the scratch of every node of the level
         // stays allocated until the level is done, as in the batched mode)
@@ -698,20 +736,29 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
       }
       if (levelStats) {
         // diagnostic summary of the level (PCG_LEVEL_STATS=1), costs a sync
+        PCG_CHECK(hipStreamSynchronize(st));
+        tLevel = now_s() - tLevel;
         std::vector<NodeWork> wk(lst.size());
         PCG_CHECK(stream_copy(wk.data(), m.work.p, sizeof(NodeWork) * wk.size(), hipMemcpyDeviceToHost));
         int64_t cand = 0, maxn = 0, flagged[6] = {0, 0, 0, 0, 0, 0};
+        std::vector<int64_t> ns;
         for (const NodeWork& w : wk) {
+          ns.push_back(w.n);
           cand += w.n;
           maxn = std::max<int64_t>(maxn, w.n);
           for (int b = 0; b < 6; b++) flagged[b] += (w.flags >> b) & 1;
         }
         unsigned long long hc7 = 0;
         PCG_CHECK(stream_copy(&hc7, counters.as<unsigned long long>() + 7, 8, hipMemcpyDeviceToHost));
+        std::sort(ns.begin(), ns.end());
+        fprintf(stderr, "[%s %d] %.3f ms, candidates per node p50 %lld p90 %lld p99 %lld\n",
+                g_sample_order ? "round" : "level", g_sample_order ? maxLevel + 1 - l : l, 1e3 * tLevel,
+                (long long)ns[ns.size() / 2], (long long)ns[ns.size() * 9 / 10], (long long)ns[ns.size() * 99 / 100]);
         fprintf(stderr,
-                "[level %d] nodes %zu candidates %lld max %lld exact-sort %lld seq-accept %lld seq-average %lld "
+                "[%s %d] nodes %zu candidates %lld max %lld exact-sort %lld seq-accept %lld seq-average %lld "
                 "overflow %lld many-neighbours %lld cell-path %lld (long lists so far %llu)\n",
-                l, lst.size(), (long long)cand, (long long)maxn, (long long)flagged[0], (long long)flagged[1],
+                g_sample_order ? "round" : "level", g_sample_order ? maxLevel + 1 - l : l, lst.size(),
+                (long long)cand, (long long)maxn, (long long)flagged[0], (long long)flagged[1],
                 (long long)flagged[2], (long long)flagged[3], (long long)flagged[4], (long long)flagged[5], hc7);
       }
     }

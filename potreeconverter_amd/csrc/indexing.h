@@ -56,6 +56,10 @@ extern int32_t g_sample_groups;
 // needs more sets to fit in device memory; n = max(streams, n), at most
 // kMaxSampleSets and the chunk count.
 extern int32_t g_sample_sets;
+// Launch rounds of the node sampling: 0 = octree levels, deepest first;
+// 1 = heights (a round holds every node whose children were all sampled in
+// earlier rounds: leaves' parents first, whatever their depth).
+extern int32_t g_sample_order;
 // Seed of every SamplerRandom leaf shuffle (0 .. 2^32-1), or -1 for the
 // reference's choice, the system clock at the start of indexing.
 extern int64_t g_random_seed;

@@ -77,7 +77,7 @@ struct SampleParams {
   uint64_t scratch_cap;
   uint64_t* scratch_top;
   int32_t ring_min_nodes;  // levels with fewer nodes decide acceptance on the cell-table path
-  int32_t pad2;
+  int32_t ring_blocks;     // most k_ring blocks of a launch (the blocks the GPU holds at once)
   uint32_t* item_start;    // num_nodes + 1 entries: work items of the level
   OutSeg* segs;            // 9 per node of the level
   uint32_t* seg_start;     // 9 * num_nodes + 1 entries: copy items per segment
@@ -91,6 +91,7 @@ struct SampleParams {
                                  // [8] runs of equal distances whose libstdc++ order failed
                                  // its window check (an error); 16 entries
   double random_limit;     // SamplerRandom: 0.7 * sqrt(3), the cell-centre distance bound
+  uint32_t* ring_ctl;      // ring work queue: [0] length, [1] next ticket (device)
 };
 
 // Samples every node of `nodes` (all of one level, children already done).

@@ -74,7 +74,7 @@ __host__ __device__ inline uint32_t hash_cap(int64_t n, int method) {
 }
 
 struct Layout {
-  int32_t *X, *Y, *Z;
+  int4* xyz;         // main order: X, Y, Z and 0, one 16-byte load per candidate
   double* add;       // main order
   uint32_t *keyA, *idxA, *keyB, *idxB;
   double* sadd;      // sorted order
@@ -120,9 +120,7 @@ __host__ __device__ inline uint64_t layout_carve(uint8_t* base, int64_t n, int m
     return off + 64;
   }
   const bool avg = method == METHOD_POISSON_AVERAGE;
-  L->X = (int32_t*)take(4 * n);
-  L->Y = (int32_t*)take(4 * n);
-  L->Z = (int32_t*)take(4 * n);
+  L->xyz = (int4*)take(16 * n);
   L->add = (double*)take(8 * n);
   L->keyA = (uint32_t*)take(4 * n);
   L->idxA = (uint32_t*)take(4 * n);
@@ -367,10 +365,11 @@ __device__ bool tie_group_conflicts(const Layout& L, const NodeGeom& g, const Sa
   for (int64_t a = p; a < e; a++) {
     for (int64_t b = a + 1; b < e; b++) {
       uint32_t ia = L.idxA[a], ib = L.idxA[b];
-      double xa = coord(L.X[ia], P.scale[0], P.offset[0]), ya = coord(L.Y[ia], P.scale[1], P.offset[1]),
-             za = coord(L.Z[ia], P.scale[2], P.offset[2]);
-      double xb = coord(L.X[ib], P.scale[0], P.offset[0]), yb = coord(L.Y[ib], P.scale[1], P.offset[1]),
-             zb = coord(L.Z[ib], P.scale[2], P.offset[2]);
+      const int4 pa = L.xyz[ia], pb = L.xyz[ib];
+      double xa = coord(pa.x, P.scale[0], P.offset[0]), ya = coord(pa.y, P.scale[1], P.offset[1]),
+             za = coord(pa.z, P.scale[2], P.offset[2]);
+      double xb = coord(pb.x, P.scale[0], P.offset[0]), yb = coord(pb.y, P.scale[1], P.offset[1]),
+             zb = coord(pb.z, P.scale[2], P.offset[2]);
       double dx = xa - xb, dy = ya - yb, dz = za - zb;
       double dd = dx * dx + dy * dy + dz * dz;
       if (!(dd < g.s2)) continue;
@@ -463,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
              z = coord(Z, P.scale[2], P.offset[2]);
       double ax = x - g.cx, ay = y - g.cy, az = z - g.cz;
       double add = ax * ax + ay * ay + az * az;
-      L.X[j] = X; L.Y[j] = Y; L.Z[j] = Z;
+      L.xyz[j] = make_int4(X, Y, Z, 0);
       L.add[j] = add;
       float f = (float)add;
       uint32_t fb;
@@ -1057,7 +1056,8 @@ __device__ inline double dd64_xyz(int32_t ax, int32_t ay, int32_t az, int32_t bx
 }
 
 __device__ inline double dd64_pair(const Layout& L, uint32_t a, uint32_t b, const SampleParams& P) {
-  return dd64_xyz(L.X[a], L.Y[a], L.Z[a], L.X[b], L.Y[b], L.Z[b], P);
+  const int4 pa = L.xyz[a], pb = L.xyz[b];
+  return dd64_xyz(pa.x, pa.y, pa.z, pb.x, pb.y, pb.z, P);
 }
 
 // exact "dd < spacing^2" through an FP32 filter on integer deltas; the
@@ -1168,7 +1168,10 @@ __global__ __launch_bounds__(kBlock) void k_cells(const int32_t* __restrict__ no
     for (int u = 0; u < 4; u++) {
       const int64_t p = p0 + u * kBlock + tid;
       X4[u] = Y4[u] = Z4[u] = 0;
-      if (p < n) { X4[u] = L.X[j4[u]]; Y4[u] = L.Y[j4[u]]; Z4[u] = L.Z[j4[u]]; }
+      if (p < n) {
+        const int4 q = L.xyz[j4[u]];
+        X4[u] = q.x; Y4[u] = q.y; Z4[u] = q.z;
+      }
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -1211,7 +1214,8 @@ __global__ __launch_bounds__(kBlock) void k_cells(const int32_t* __restrict__ no
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       if (e0 + u * kBlock >= n) continue;
-      v4[u] = make_int4(L.X[j4[u]], L.Y[j4[u]], L.Z[j4[u]], (int32_t)p4[u]);
+      const int4 q = L.xyz[j4[u]];
+      v4[u] = make_int4(q.x, q.y, q.z, (int32_t)p4[u]);
       if (avg) c4[u] = L.cell16[j4[u]];
       else a4[u] = L.sadd[p4[u]];
     }
@@ -1485,7 +1489,8 @@ __device__ void accept_node(const int32_t* __restrict__ nodes, const NodeInfo* _
         }
       } else {
         const uint32_t j = L.idxA[p];
-        const int32_t X = L.X[j], Y = L.Y[j], Z = L.Z[j];
+        const int4 q = L.xyz[j];
+        const int32_t X = q.x, Y = q.y, Z = q.z;
         const double x = coord(X, P.scale[0], P.offset[0]), y = coord(Y, P.scale[1], P.offset[1]),
                      z = coord(Z, P.scale[2], P.offset[2]);
         int lo16[3], hi16[3];
@@ -1584,15 +1589,29 @@ __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ n
 // coordinates.
 constexpr int kRingCap = 512;     // accepted points in the shell (power of two)
 constexpr int kRingSlots = 1024;  // cell chains (direct-mapped, power of two)
-constexpr int kRingWaves = 4;     // nodes per k_ring block
+constexpr int kRingWaves = 4;     // wavefronts per k_ring block
+constexpr int16_t kRingEnd = INT16_MIN;
 
+// 13 KB per wavefront, so that three blocks of four fit the 160 KB of a CU.
+// `add` holds the squared distance rounded up to FP32: it only decides that
+// an entry is too close to the centre to reject (add < limit), and rounding
+// up keeps every such decision true of the exact value.  `next` is the
+// entry's index minus the next one in its chain (a chain older than 32 767
+// entries is dead: the ring holds 512).  Each entry's sorted position, read
+// only for the FP64 re-check, is kept in the node's scratch.
 struct RingLds {
   float x[kRingCap], y[kRingCap], z[kRingCap];
-  double add[kRingCap];
-  int32_t next[kRingCap];
-  int32_t pos[kRingCap];  // sorted position (FP64 re-check)
+  float add[kRingCap];
+  int16_t next[kRingCap];
   int32_t head[kRingSlots];
 };
+static_assert(3 * kRingWaves * sizeof(RingLds) <= 160 * 1024, "three k_ring blocks per CU");
+
+__device__ inline float round_up_f32(double a) {
+  float f = (float)a;
+  if ((double)f < a) f = __uint_as_float(__float_as_uint(f) + 1u);  // a >= 0
+  return f;
+}
 
 struct RingGeom {
   float off, inv_cs;  // cell = floor((v + off) * inv_cs), v relative to the centre
@@ -1610,14 +1629,9 @@ __device__ inline uint32_t ring_slot(int cx, int cy, int cz) {
 }
 static_assert(kRingSlots == 1024, "ring_slot hashes cells to 10 bits");
 
-__global__ __launch_bounds__(64 * kRingWaves) void k_ring(const int32_t* __restrict__ nodes,
-                                                           const NodeInfo* __restrict__ infos,
-                                                           NodeWork* __restrict__ work, int32_t num_nodes,
-                                                           SampleParams P) {
-  __shared__ RingLds s_ring[kRingWaves];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int k = __builtin_amdgcn_readfirstlane(blockIdx.x * kRingWaves + wave);
-  if (k >= num_nodes) return;
+__device__ void ring_node(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
+                          NodeWork* __restrict__ work, const int k, const SampleParams& P, RingLds& R) {
+  const int lane = threadIdx.x & 63;
   const NodeWork& w = work[k];
   if (w.flags & (NW_OVERFLOW | NW_CELL_PATH)) return;
   const int64_t n = w.n;
@@ -1633,7 +1647,7 @@ __global__ __launch_bounds__(64 * kRingWaves) void k_ring(const int32_t* __restr
   const float sq = (float)(g.spacing * (1.0 + 1e-4));
   unsigned long long rechecks = 0, tests = 0;
   float maxerr = 0.f;
-  RingLds& R = s_ring[wave];
+  int32_t* const rpos = (int32_t*)L.keyB;  // sorted position of each ring entry
   for (int i = lane; i < kRingSlots; i += 64) R.head[i] = -1;
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   int32_t total = 0, tail = 0;  // accepted points appended / first live one (wave-uniform)
@@ -1650,7 +1664,8 @@ __global__ __launch_bounds__(64 * kRingWaves) void k_ring(const int32_t* __restr
   double a = 0.0;
   if (lane < n) {
     const uint32_t j = L.idxA[lane];
-    X = L.X[j]; Y = L.Y[j]; Z = L.Z[j];
+    const int4 q = L.xyz[j];
+    X = q.x; Y = q.y; Z = q.z;
     a = L.sadd[lane];
   }
   // the next window's candidates are loaded while this one is decided; the
@@ -1663,7 +1678,8 @@ __global__ __launch_bounds__(64 * kRingWaves) void k_ring(const int32_t* __restr
     int32_t nX = 0, nY = 0, nZ = 0;
     double na = 0.0;
     if (p + 64 < n) {
-      nX = L.X[jn]; nY = L.Y[jn]; nZ = L.Z[jn];
+      const int4 q = L.xyz[jn];
+      nX = q.x; nY = q.y; nZ = q.z;
       na = L.sadd[p + 64];
     }
     jn = p + 128 < n ? L.idxA[p + 128] : 0u;
@@ -1678,7 +1694,7 @@ __global__ __launch_bounds__(64 * kRingWaves) void k_ring(const int32_t* __restr
     const double thr = lim0 >= 0.0 ? lim0 * lim0 : 0.0;
     while (tail < total) {
       const int32_t idx = tail + lane;
-      const bool dead = idx < total && R.add[idx & (kRingCap - 1)] < thr;
+      const bool dead = idx < total && (double)R.add[idx & (kRingCap - 1)] < thr;
       const int d = __popcll(__ballot(dead));
       tail += d;
       if (d < 64) break;
@@ -1712,18 +1728,20 @@ __global__ __launch_bounds__(64 * kRingWaves) void k_ring(const int32_t* __restr
             if (idx < 0 || idx < tail - 64) continue;
             any = true;
             const int e = idx & (kRingCap - 1);
-            cur[t] = R.next[e];
+            const int16_t nx = R.next[e];
+            cur[t] = nx == kRingEnd ? -1 : idx - nx;
             // chains are newest first up to reordering inside one window,
             // so nothing live follows an entry older than tail - 64
-            if (idx < tail || R.add[e] < limSq) continue;
+            if (idx < tail || (double)R.add[e] < limSq) continue;
             tests++;
             const float dx = R.x[e] - xf, dy = R.y[e] - yf, dz = R.z[e] - zf;
             const float d32 = dx * dx + dy * dy + dz * dz;
             if (d32 > G.band_hi) continue;
             bool within = d32 < G.band_lo;
             if (!within) {
-              const uint32_t jq = L.idxA[R.pos[e]];
-              within = exact(L.X[jq], L.Y[jq], L.Z[jq], X, Y, Z, d32);
+              const uint32_t jq = L.idxA[rpos[e]];
+              const int4 q = L.xyz[jq];
+              within = exact(q.x, q.y, q.z, X, Y, Z, d32);
             }
             if (within) rej = true;
           }
@@ -1799,8 +1817,11 @@ __global__ __launch_bounds__(64 * kRingWaves) void k_ring(const int32_t* __restr
     if (accepted) {
       const int32_t idx = total + __popcll(accm & lt);
       const int e = idx & (kRingCap - 1);
-      R.x[e] = xf; R.y[e] = yf; R.z[e] = zf; R.add[e] = a; R.pos[e] = (int32_t)p;
-      R.next[e] = atomicExch(&R.head[ring_slot(ring_cell(xf, G), ring_cell(yf, G), ring_cell(zf, G))], idx);
+      R.x[e] = xf; R.y[e] = yf; R.z[e] = zf; R.add[e] = round_up_f32(a);
+      rpos[e] = (int32_t)p;
+      const int32_t prev = atomicExch(&R.head[ring_slot(ring_cell(xf, G), ring_cell(yf, G), ring_cell(zf, G))], idx);
+      // inside one window a chain may point at a newer entry (at most 63 on)
+      R.next[e] = (prev < 0 || idx - prev > 32767) ? kRingEnd : (int16_t)(idx - prev);
     }
     total += kacc;
     X = nX; Y = nY; Z = nZ; a = na;
@@ -1811,6 +1832,60 @@ __global__ __launch_bounds__(64 * kRingWaves) void k_ring(const int32_t* __restr
   F.tests = tests;
   F.maxerr = maxerr;
   flush_filter(F, P);
+}
+
+// Work queue of the ring: the launch's nodes that the ring decides, largest
+// first by power-of-two classes of their candidate counts, so that the
+// longest nodes start first.  ctl[0] = queue length, ctl[1] = next ticket.
+__global__ __launch_bounds__(1024) void k_ring_queue(const NodeWork* __restrict__ work, int32_t num_nodes,
+                                                     int32_t* __restrict__ queue, uint32_t* __restrict__ ctl) {
+  __shared__ uint32_t s_cnt[33], s_base[33];
+  const int tid = threadIdx.x;
+  if (tid < 33) s_cnt[tid] = 0;
+  __syncthreads();
+  for (int k = tid; k < num_nodes; k += 1024) {
+    const NodeWork& w = work[k];
+    if (w.flags & (NW_OVERFLOW | NW_CELL_PATH)) continue;
+    const int c = w.n > 0 ? 64 - __builtin_clzll((unsigned long long)w.n) : 0;
+    atomicAdd(&s_cnt[c < 32 ? c : 32], 1u);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t run = 0;
+    for (int c = 32; c >= 0; c--) {
+      s_base[c] = run;
+      run += s_cnt[c];
+    }
+    ctl[0] = run;
+    ctl[1] = 0;
+  }
+  __syncthreads();
+  for (int k = tid; k < num_nodes; k += 1024) {
+    const NodeWork& w = work[k];
+    if (w.flags & (NW_OVERFLOW | NW_CELL_PATH)) continue;
+    const int c = w.n > 0 ? 64 - __builtin_clzll((unsigned long long)w.n) : 0;
+    queue[atomicAdd(&s_base[c < 32 ? c : 32], 1u)] = k;
+  }
+}
+
+// Persistent form: every wavefront takes the next node of the queue until
+// the queue is drained, so a wavefront that finishes a small node starts
+// another while a large one is still running next to it.
+__global__ __launch_bounds__(64 * kRingWaves) void k_ring(const int32_t* __restrict__ nodes,
+                                                           const NodeInfo* __restrict__ infos,
+                                                           NodeWork* __restrict__ work,
+                                                           const int32_t* __restrict__ queue,
+                                                           uint32_t* __restrict__ ctl, SampleParams P) {
+  __shared__ RingLds s_ring[kRingWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t len = ctl[0];
+  while (true) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(&ctl[1], 1u);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= len) break;
+    ring_node(nodes, infos, work, __builtin_amdgcn_readfirstlane(queue[t]), P, s_ring[wave]);
+  }
 }
 
 // Accepted entries of a Poisson-average node in cell order, with pre[e] =
@@ -1997,7 +2072,8 @@ __device__ void accept_node_capped(const Layout& L, const NodeGeom& g, int64_t n
         const double lim = cd - g.spacing;
         const double limSq = lim * lim;
         const uint32_t j = L.idxA[p];
-        const int32_t X = L.X[j], Y = L.Y[j], Z = L.Z[j];
+        const int4 q = L.xyz[j];
+        const int32_t X = q.x, Y = q.y, Z = q.z;
         const double x = coord(X, P.scale[0], P.offset[0]), y = coord(Y, P.scale[1], P.offset[1]),
                      z = coord(Z, P.scale[2], P.offset[2]);
         const CellBox cb = cell_box(g, x, y, z);
@@ -2079,8 +2155,9 @@ __global__ void k_sequential(const int32_t* __restrict__ nodes, const NodeInfo* 
   for (int c = 0; c < 4096; c++) { L.hkey[c] = kNil; L.hhead[c] = kNil; }
   for (int64_t p = 0; p < n; p++) {
     uint32_t j = L.idxA[p];
-    double x = coord(L.X[j], P.scale[0], P.offset[0]), y = coord(L.Y[j], P.scale[1], P.offset[1]),
-           z = coord(L.Z[j], P.scale[2], P.offset[2]);
+    const int4 pj = L.xyz[j];
+    double x = coord(pj.x, P.scale[0], P.offset[0]), y = coord(pj.y, P.scale[1], P.offset[1]),
+           z = coord(pj.z, P.scale[2], P.offset[2]);
     int lo[3], hi[3];
     range16(g, x, y, z, lo, hi);
     bool accept = true;
@@ -2105,8 +2182,9 @@ __global__ void k_sequential(const int32_t* __restrict__ nodes, const NodeInfo* 
   }
   for (int64_t p = 0; p < n; p++) {
     uint32_t j = L.idxA[p];
-    double x = coord(L.X[j], P.scale[0], P.offset[0]), y = coord(L.Y[j], P.scale[1], P.offset[1]),
-           z = coord(L.Z[j], P.scale[2], P.offset[2]);
+    const int4 pj = L.xyz[j];
+    double x = coord(pj.x, P.scale[0], P.offset[0]), y = coord(pj.y, P.scale[1], P.offset[1]),
+           z = coord(pj.z, P.scale[2], P.offset[2]);
     int lo[3], hi[3];
     range16(g, x, y, z, lo, hi);
     for (int cx = lo[0]; cx <= hi[0]; cx++)
@@ -2831,8 +2909,13 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   hipLaunchKernelGGL(k_exact_sort, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p, p.counters);
   PCG_CHECK(hipGetLastError());
   if (p.method == METHOD_POISSON && num_nodes >= p.ring_min_nodes) {
-    hipLaunchKernelGGL(k_ring, dim3((unsigned)((num_nodes + kRingWaves - 1) / kRingWaves)), dim3(64 * kRingWaves),
-                       0, s, d_nodes, d_info, d_work, num_nodes, p);
+    // the queue takes the item table, which k_plan fills only afterwards
+    hipLaunchKernelGGL(k_ring_queue, dim3(1), dim3(1024), 0, s, d_work, num_nodes, (int32_t*)p.item_start,
+                       p.ring_ctl);
+    PCG_CHECK(hipGetLastError());
+    const int blocks = std::max(1, std::min((num_nodes + kRingWaves - 1) / kRingWaves, p.ring_blocks));
+    hipLaunchKernelGGL(k_ring, dim3((unsigned)blocks), dim3(64 * kRingWaves), 0, s, d_nodes, d_info, d_work,
+                       (const int32_t*)p.item_start, p.ring_ctl, p);
     PCG_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_cells, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p);
