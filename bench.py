@@ -48,6 +48,8 @@ def parse():
                     help="library option sample_groups (streams of the chunk-subtree sampling; default: the library's)")
     ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
                     help="library option (pcg_set_option) set before the run; repeatable")
+    ap.add_argument("--autotune", action="store_true",
+                    help="c1/c3: search the sampling launch options on the workload before the timed steps")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--points", type=int, default=None, help="default: 200M (c1), 1B (c3)")
@@ -423,6 +425,11 @@ def main():
             pc._check(L.pcg_convert_device_encoded(d_las, n, header, args.method.encode(), args.encoding.encode(),
                                                    None, stats))
 
+    tuned = None
+    if args.autotune:
+        # the launch options searched on this workload before the timed steps
+        chosen, best_s, default_s = pc.autotune_device(d_las.value, n, header, args.method, passes=2, repeats=2)
+        tuned = {"options": chosen, "best_ms": round(best_s * 1e3, 3), "start_ms": round(default_s * 1e3, 3)}
     for _ in range(args.warmup):
         step()
     pc._check(L.pcg_synchronize())
@@ -525,6 +532,7 @@ def main():
             "kernel_ms": round(kern_s * 1e3, 4),
         },
         "precision": precision,
+        "autotune": tuned,
         "io_s": io,
         "plain_vs_batched": plain,
         "cpu_baseline": cpu,

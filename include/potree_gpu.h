@@ -60,8 +60,32 @@ int pcg_release(void);
  * "sample_order" (0/1, default 0): launch rounds of the node sampling by
  * octree level, deepest first (0), or by height, a node in the first round
  * after all its children (1) (same results).
+ * Launch shapes of the node sampling (same results for every value):
+ * "ring_blocks_per_cu" (1-3, default 3) persistent ring blocks per CU,
+ * "copy_blocks" (default 8192) and "neighbor_blocks" (default 4096) grids of
+ * the level-wide record copy and neighbour search, "avg_blocks" (default
+ * 2048) workgroups of a Poisson-average colour-sum launch.
  * The other options have no reference counterpart. */
 int pcg_set_option(const char* key, int64_t value);
+
+/* Current value of an option of pcg_set_option.  No reference counterpart. */
+int pcg_get_option(const char* key, int64_t* value);
+
+/* Autotuner of the runtime options above.  It converts the caller's
+ * workload (records in HBM, as pcg_convert_device, no files written) once
+ * per configuration and searches the options pcg_autotune_param_name(0 ..
+ * pcg_autotune_params() - 1) names by coordinate descent: each option over
+ * its candidate values with the others fixed, a value kept when the job is
+ * at least 1 % faster, for at most `passes` rounds; a configuration's time
+ * is the fastest of `repeats` conversions.  The best values stay set and are
+ * written to chosen[] (pcg_autotune_params() entries); best_s and default_s
+ * (nullable) receive the job's seconds with them and with the options as
+ * they were.  Every option it tries gives the same output bytes.  No
+ * reference counterpart (the reference has no launch parameters). */
+int32_t pcg_autotune_params(void);
+const char* pcg_autotune_param_name(int32_t i);
+int pcg_autotune_device(const void* d_las, int64_t n, const uint8_t* las_header, const char* method,
+                        int32_t passes, int32_t repeats, int64_t* chosen, double* best_s, double* default_s);
 
 /* Phase 1 + phase 2 from a LAS file, the drop-in for the two calls at
  * Converter/src/main.cpp:353 (chunker_countsort_laszip::doChunking) and

@@ -83,6 +83,13 @@ def lib():
                                           c.POINTER(c.c_char_p), c.c_int32, c.c_int32, c.POINTER(c.c_double)]
         L.pcg_release.argtypes = []
         L.pcg_set_option.argtypes = [c.c_char_p, c.c_int64]
+        L.pcg_get_option.argtypes = [c.c_char_p, c.POINTER(c.c_int64)]
+        L.pcg_autotune_params.argtypes = []
+        L.pcg_autotune_params.restype = c.c_int32
+        L.pcg_autotune_param_name.argtypes = [c.c_int32]
+        L.pcg_autotune_param_name.restype = c.c_char_p
+        L.pcg_autotune_device.argtypes = [c.c_void_p, c.c_int64, c.c_void_p, c.c_char_p, c.c_int32, c.c_int32,
+                                          c.POINTER(c.c_int64), c.POINTER(c.c_double), c.POINTER(c.c_double)]
         for name in ("pcg_set_option", "pcg_release", "pcg_convert", "pcg_do_chunking", "pcg_do_indexing", "pcg_count_cells", "pcg_count_cells_noalloc", "pcg_convert_device",
                      "pcg_convert_encoded", "pcg_do_indexing_encoded", "pcg_convert_device_encoded",
                      "pcg_device_alloc", "pcg_device_free", "pcg_copy", "pcg_synchronize", "pcg_lasgen_preset",
@@ -90,7 +97,7 @@ def lib():
                      "pcg_lasgen_write", "pcg_lasgen_device", "pcg_lasgen_header", "pcg_init",
                      "pcg_do_indexing_device", "pcg_chunkgen_params", "pcg_chunkgen_counts", "pcg_chunkgen_write",
                      "pcg_chunkgen_device", "pcg_sample_subtree", "pcg_device_memory",
-                     "pcg_do_chunking_sources", "pcg_convert_sources"):
+                     "pcg_do_chunking_sources", "pcg_convert_sources", "pcg_get_option", "pcg_autotune_device"):
             getattr(L, name).restype = c.c_int
         _lib = L
     return _lib
@@ -227,6 +234,30 @@ def convert(las_path, outdir, method="poisson", keep_chunks=False, encoding=None
         _check(lib().pcg_convert_encoded(str(las_path).encode(), str(outdir).encode(), method.encode(),
                                          encoding.encode(), flags, t))
     return list(t)
+
+
+def set_option(key, value):
+    _check(lib().pcg_set_option(key.encode(), int(value)))
+
+
+def get_option(key):
+    v = ctypes.c_int64()
+    _check(lib().pcg_get_option(key.encode(), ctypes.byref(v)))
+    return v.value
+
+
+def autotune_device(d_las, n, header, method="poisson", passes=2, repeats=2):
+    """Searches the sampling launch options on the caller's workload (records
+    in HBM) and leaves the fastest set; returns ({option: value}, seconds with
+    them, seconds with the options as they were)."""
+    L = lib()
+    k = L.pcg_autotune_params()
+    chosen = (ctypes.c_int64 * k)()
+    best, default = ctypes.c_double(), ctypes.c_double()
+    _check(L.pcg_autotune_device(ctypes.c_void_p(d_las), n, header, method.encode(), passes, repeats, chosen,
+                                 ctypes.byref(best), ctypes.byref(default)))
+    names = [L.pcg_autotune_param_name(i).decode() for i in range(k)]
+    return dict(zip(names, list(chosen))), best.value, default.value
 
 
 def _strings(values):

@@ -1,6 +1,9 @@
 // extern "C" entry points (include/potree_gpu.h).
 #include <string.h>
 
+#include <algorithm>
+#include <chrono>
+
 #include <filesystem>
 #include <string>
 #include <vector>
@@ -203,6 +206,24 @@ int pcg_set_option(const char* key, int64_t value) {
     g_sample_order = (int32_t)value;
     return 0;
   }
+  struct Shape {
+    const char* name;
+    int32_t* value;
+    int64_t lo, hi;
+  };
+  const Shape shapes[] = {{"ring_blocks_per_cu", &g_ring_blocks_per_cu, 1, 3},
+                          {"copy_blocks", &g_copy_blocks, 256, 1 << 20},
+                          {"neighbor_blocks", &g_neighbor_blocks, 256, 1 << 20},
+                          {"avg_blocks", &g_avg_blocks, 64, 1 << 20}};
+  for (const Shape& sh : shapes)
+    if (k == sh.name) {
+      if (value < sh.lo || value > sh.hi) {
+        set_error(k + " must be in [" + std::to_string(sh.lo) + ", " + std::to_string(sh.hi) + "]");
+        return 1;
+      }
+      *sh.value = (int32_t)value;
+      return 0;
+    }
   if (k == "distribute_tail") {
     g_distribute_tail = value != 0;
     return 0;
@@ -227,7 +248,115 @@ int pcg_set_option(const char* key, int64_t value) {
   return 1;
 }
 
+int pcg_get_option(const char* key, int64_t* value) {
+  const std::string k = key ? key : "";
+  if (!value) {
+    set_error("pcg_get_option: null value pointer");
+    return 1;
+  }
+  const std::pair<const char*, int64_t> all[] = {
+      {"plain_sampling", g_plain_sampling ? 1 : 0}, {"ring_min_nodes", g_ring_min_nodes},
+      {"random_seed", g_random_seed},               {"sample_groups", g_sample_groups},
+      {"sample_sets", g_sample_sets},               {"sample_order", g_sample_order},
+      {"ring_blocks_per_cu", g_ring_blocks_per_cu}, {"copy_blocks", g_copy_blocks},
+      {"neighbor_blocks", g_neighbor_blocks},       {"avg_blocks", g_avg_blocks},
+      {"distribute_tail", g_distribute_tail ? 1 : 0}, {"chunk_grid", g_chunk_grid},
+      {"chunk_cap", g_chunk_cap}};
+  for (const auto& kv : all)
+    if (k == kv.first) {
+      *value = kv.second;
+      return 0;
+    }
+  set_error("unknown option: " + k);
+  return 1;
+}
+
 static std::string encoding_or_default(const char* encoding) { return encoding ? encoding : "DEFAULT"; }
+
+// The runtime options the autotuner searches, in the order of its output,
+// with the values it tries (every value gives the same output bytes).
+namespace {
+struct TuneParam {
+  const char* name;
+  std::vector<int64_t> values;
+};
+const std::vector<TuneParam>& tune_params() {
+  static const std::vector<TuneParam> p = {
+      {"sample_groups", {1, 2, 3, 4}},
+      {"sample_order", {0, 1}},
+      {"ring_min_nodes", {64, 128, 256, 512, 1024, 2048, 4096}},
+      {"ring_blocks_per_cu", {1, 2, 3}},
+      {"copy_blocks", {2048, 4096, 8192, 16384}},
+      {"neighbor_blocks", {1024, 2048, 4096, 8192}},
+      {"avg_blocks", {512, 1024, 2048, 4096}},
+  };
+  return p;
+}
+}  // namespace
+
+int32_t pcg_autotune_params(void) { return (int32_t)tune_params().size(); }
+
+const char* pcg_autotune_param_name(int32_t i) {
+  const auto& p = tune_params();
+  return i >= 0 && i < (int32_t)p.size() ? p[i].name : nullptr;
+}
+
+int pcg_autotune_device(const void* d_las, int64_t n, const uint8_t* las_header, const char* method,
+                        int32_t passes, int32_t repeats, int64_t* chosen, double* best_s, double* default_s) {
+  if (ensure_device()) return 1;
+  if (!d_las || n <= 0 || !las_header || !method || passes < 1 || passes > 8 || repeats < 1 || repeats > 8) {
+    set_error("pcg_autotune_device: needs device records, n > 0, a header, a method, 1-8 passes and repeats");
+    return 1;
+  }
+  const auto& params = tune_params();
+  std::vector<int64_t> cur(params.size());
+  for (size_t i = 0; i < params.size(); i++)
+    if (pcg_get_option(params[i].name, &cur[i])) return 1;
+  // one configuration: the fastest of `repeats` conversions (chunking +
+  // indexing in HBM, no files), wall time to the end of the job
+  auto trial = [&](double& t) -> int {
+    t = 1e30;
+    for (int r = 0; r < repeats; r++) {
+      const auto t0 = std::chrono::steady_clock::now();
+      if (pcg_convert_device(d_las, n, las_header, method, nullptr, nullptr)) return 1;
+      const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      t = std::min(t, dt);
+    }
+    return 0;
+  };
+  auto apply = [&](size_t i, int64_t v) { return pcg_set_option(params[i].name, v); };
+  double best = 0;
+  if (trial(best)) return 1;  // the first conversion also sizes the workspace
+  if (trial(best)) return 1;
+  if (default_s) *default_s = best;
+  // coordinate descent: each parameter over its values with the others
+  // fixed, a change kept when it is at least 1 % faster
+  for (int pass = 0; pass < passes; pass++) {
+    bool changed = false;
+    for (size_t i = 0; i < params.size(); i++) {
+      for (int64_t v : params[i].values) {
+        if (v == cur[i]) continue;
+        if (apply(i, v)) return 1;
+        double t = 0;
+        if (trial(t)) {
+          apply(i, cur[i]);
+          return 1;
+        }
+        if (t < 0.99 * best) {
+          best = t;
+          cur[i] = v;
+          changed = true;
+        }
+      }
+      if (apply(i, cur[i])) return 1;
+    }
+    if (!changed) break;
+  }
+  if (chosen)
+    for (size_t i = 0; i < params.size(); i++) chosen[i] = cur[i];
+  if (best_s) *best_s = best;
+  return 0;
+}
 
 int pcg_convert(const char* las_path, const char* outdir, const char* method, int32_t flags, double* timings) {
   return pcg_convert_encoded(las_path, outdir, method, nullptr, flags, timings);

@@ -167,6 +167,10 @@ int32_t g_ring_min_nodes = 512;
 int32_t g_sample_groups = 0;
 int32_t g_sample_sets = 0;
 int32_t g_sample_order = 0;
+int32_t g_ring_blocks_per_cu = 3;
+int32_t g_copy_blocks = 8192;
+int32_t g_neighbor_blocks = 4096;
+int32_t g_avg_blocks = 2048;
 
 void IndexResult::release() {
   d_octree = nullptr;  // a workspace buffer
@@ -664,7 +668,10 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   sp.arena_cap = arenaCap;
   sp.arena_top = gtop.as<uint64_t>();
   sp.counters = counters.as<unsigned long long>();
-  sp.ring_blocks = 3 * device().num_cus;  // three k_ring blocks fit a CU's LDS
+  sp.ring_blocks = g_ring_blocks_per_cu * device().num_cus;  // at most three k_ring blocks fit a CU's LDS
+  sp.copy_blocks = g_copy_blocks;
+  sp.neighbor_blocks = g_neighbor_blocks;
+  sp.avg_blocks = g_avg_blocks;
   // a group's buffers for levels of at most `maxNodes` nodes and `cand`
   // candidates in all
   auto group_params = [&](GroupMem& m, int64_t maxNodes, int64_t cand, SampleParams& p) -> int {

@@ -60,6 +60,13 @@ extern int32_t g_sample_sets;
 // 1 = heights (a round holds every node whose children were all sampled in
 // earlier rounds: leaves' parents first, whatever their depth).
 extern int32_t g_sample_order;
+// Launch shapes of the node sampling (same results for every value): k_ring
+// blocks per CU (1-3), grids of the level-wide record copy and neighbour
+// search, workgroups of a Poisson-average colour-sum launch.
+extern int32_t g_ring_blocks_per_cu;
+extern int32_t g_copy_blocks;
+extern int32_t g_neighbor_blocks;
+extern int32_t g_avg_blocks;
 // Seed of every SamplerRandom leaf shuffle (0 .. 2^32-1), or -1 for the
 // reference's choice, the system clock at the start of indexing.
 extern int64_t g_random_seed;

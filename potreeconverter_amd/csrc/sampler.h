@@ -92,6 +92,10 @@ struct SampleParams {
                                  // its window check (an error); 16 entries
   double random_limit;     // SamplerRandom: 0.7 * sqrt(3), the cell-centre distance bound
   uint32_t* ring_ctl;      // ring work queue: [0] length, [1] next ticket (device)
+  int32_t copy_blocks;     // grid of the level-wide record copy
+  int32_t neighbor_blocks; // grid of the level-wide neighbour search
+  int32_t avg_blocks;      // workgroups of the Poisson-average colour sums per launch
+  int32_t pad3;
 };
 
 // Samples every node of `nodes` (all of one level, children already done).

@@ -44,8 +44,6 @@ constexpr uint32_t kNil = 0xFFFFFFFFu;
 constexpr int kMaxNbr = 16;  // earlier neighbours kept per candidate
 constexpr uint8_t kNbrOverflow = 255;  // list too long for the pool: search again
 constexpr uint8_t kNbrPooled = 254;    // list in the level's neighbour pool
-constexpr int kNeighborBlocks = 4096;  // grid of the level-wide neighbour search
-constexpr int kCopyBlocks = 8192;      // grid of the level-wide record copy
 constexpr int kAccWords = 2048;         // LDS accepted bits per wave in k_accept (65536 candidates)
 
 __device__ inline int32_t ld_i32(const uint8_t* p) {
@@ -2900,7 +2898,7 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
     PCG_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_tile_plan, dim3(1), dim3(1024), 0, s, d_work, num_nodes, p.seg_start);
     PCG_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_copy_tiles, dim3(kCopyBlocks), dim3(kBlock), 0, s, d_work, num_nodes, p.seg_start, p);
+    hipLaunchKernelGGL(k_copy_tiles, dim3((unsigned)p.copy_blocks), dim3(kBlock), 0, s, d_work, num_nodes, p.seg_start, p);
     PCG_CHECK(hipGetLastError());
     return 0;
   }
@@ -2922,7 +2920,7 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   PCG_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, s, d_work, num_nodes, p.item_start);
   PCG_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_neighbors, dim3(kNeighborBlocks), dim3(kBlock), 0, s, d_nodes, d_info, d_work, num_nodes,
+  hipLaunchKernelGGL(k_neighbors, dim3((unsigned)p.neighbor_blocks), dim3(kBlock), 0, s, d_nodes, d_info, d_work, num_nodes,
                      p.item_start, p);
   PCG_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_accept, dim3((unsigned)((num_nodes + kWaves - 1) / kWaves)), dim3(kBlock), 0, s, d_nodes,
@@ -2930,8 +2928,8 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   PCG_CHECK(hipGetLastError());
 
   if (p.method == METHOD_POISSON_AVERAGE) {
-    // about 2048 workgroups in all, at most 64 per node
-    const unsigned slices = (unsigned)std::max(1, std::min(64, 2048 / std::max(1, (int)num_nodes)));
+    // about avg_blocks workgroups in all, at most 64 per node
+    const unsigned slices = (unsigned)std::max(1, std::min(64, p.avg_blocks / std::max(1, (int)num_nodes)));
     hipLaunchKernelGGL(k_avg_compact, grid, dim3(kBlock), 0, s, d_work, p);
     PCG_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_avg_colors, dim3((unsigned)num_nodes, slices), dim3(kBlock), 0, s, d_nodes, d_info, d_work,
@@ -2944,7 +2942,7 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   PCG_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_tile_plan, dim3(1), dim3(1024), 0, s, d_work, num_nodes, p.seg_start);
   PCG_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_copy_tiles, dim3(kCopyBlocks), dim3(kBlock), 0, s, d_work, num_nodes, p.seg_start, p);
+  hipLaunchKernelGGL(k_copy_tiles, dim3((unsigned)p.copy_blocks), dim3(kBlock), 0, s, d_work, num_nodes, p.seg_start, p);
   PCG_CHECK(hipGetLastError());
   return 0;
 }
