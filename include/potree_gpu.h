@@ -60,6 +60,10 @@ int pcg_release(void);
  * "sample_order" (0/1, default 0): launch rounds of the node sampling by
  * octree level, deepest first (0), or by height, a node in the first round
  * after all its children (1) (same results).
+ * "sample_graph" (0/1, default 0): 1 captures the sampling of the chunk
+ * subtrees (every set's launch sequence on its stream, forked from and
+ * joined to the library stream) in one graph per job and launches it (same
+ * results).
  * Launch shapes of the node sampling (same results for every value):
  * "ring_blocks_per_cu" (1-3, default 3) persistent ring blocks per CU,
  * "copy_blocks" (default 8192) and "neighbor_blocks" (default 4096) grids of

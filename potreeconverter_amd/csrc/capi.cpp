@@ -224,6 +224,10 @@ int pcg_set_option(const char* key, int64_t value) {
       *sh.value = (int32_t)value;
       return 0;
     }
+  if (k == "sample_graph") {
+    g_sample_graph = value != 0;
+    return 0;
+  }
   if (k == "distribute_tail") {
     g_distribute_tail = value != 0;
     return 0;
@@ -258,6 +262,7 @@ int pcg_get_option(const char* key, int64_t* value) {
       {"plain_sampling", g_plain_sampling ? 1 : 0}, {"ring_min_nodes", g_ring_min_nodes},
       {"random_seed", g_random_seed},               {"sample_groups", g_sample_groups},
       {"sample_sets", g_sample_sets},               {"sample_order", g_sample_order},
+      {"sample_graph", g_sample_graph ? 1 : 0},
       {"ring_blocks_per_cu", g_ring_blocks_per_cu}, {"copy_blocks", g_copy_blocks},
       {"neighbor_blocks", g_neighbor_blocks},       {"avg_blocks", g_avg_blocks},
       {"distribute_tail", g_distribute_tail ? 1 : 0}, {"chunk_grid", g_chunk_grid},
@@ -284,6 +289,7 @@ const std::vector<TuneParam>& tune_params() {
   static const std::vector<TuneParam> p = {
       {"sample_groups", {1, 2, 3, 4}},
       {"sample_order", {0, 1}},
+      {"sample_graph", {0, 1}},
       {"ring_min_nodes", {64, 128, 256, 512, 1024, 2048, 4096}},
       {"ring_blocks_per_cu", {1, 2, 3}},
       {"copy_blocks", {2048, 4096, 8192, 16384}},

@@ -167,6 +167,7 @@ int32_t g_ring_min_nodes = 512;
 int32_t g_sample_groups = 0;
 int32_t g_sample_sets = 0;
 int32_t g_sample_order = 0;
+bool g_sample_graph = false;
 int32_t g_ring_blocks_per_cu = 3;
 int32_t g_copy_blocks = 8192;
 int32_t g_neighbor_blocks = 4096;
@@ -718,13 +719,39 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   PCG_CHECK(hipMemsetAsync(gtop.p, 0, 64, s));
   PCG_CHECK(hipMemsetAsync(counters.p, 0, 128, s));
   int64_t sampled = 0;
+  // the captured graph of the chunk-subtree sets, destroyed when the job
+  // ends (after waiting for it); a capture left open by an error is closed
+  struct SampleGraph {
+    hipStream_t capturing = nullptr, launched = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int launch() {
+      hipStream_t st = capturing;
+      capturing = nullptr;
+      PCG_CHECK(hipStreamEndCapture(st, &graph));
+      PCG_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      PCG_CHECK(hipGraphLaunch(exec, st));
+      launched = st;
+      return 0;
+    }
+    ~SampleGraph() {
+      if (capturing) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(capturing, &g);
+        if (g) (void)hipGraphDestroy(g);
+      }
+      if (launched) (void)hipStreamSynchronize(launched);
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+    }
+  } sampleGraph;
   // one level of one list set on one stream
   auto run_level = [&](int lg, int l, GroupMem& m, const SampleParams& p, hipStream_t st) -> int {
     {
       const auto& lst = levels[lg][l];
       if (lst.empty()) return 0;
       sampled += (int64_t)lst.size();
-      PCG_CHECK(hipMemsetAsync(m.tops.as<uint64_t>() + 1, 0, 16, st));  // scratch and pool tops
+      if (sample_zero(m.tops.as<uint64_t>() + 1, 2, st)) return 1;  // scratch and pool tops
       double tLevel = 0.0;
       if (levelStats) {
         PCG_CHECK(hipStreamSynchronize(st));
@@ -845,6 +872,16 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     if (group_streams(S, gs)) return 1;
     hipEvent_t ev_start = group_event(0);
     if (!ev_start) return 1;
+    for (int k = 0; k < S; k++)
+      if (!group_event(1 + k)) return 1;
+    // With "sample_graph" the launch sequence of every set (streams forked
+    // from and joined back to the library stream by events) is captured
+    // into one graph, instantiated and launched on the library stream.
+    const bool graph = g_sample_graph && !g_plain_sampling && !levelStats;
+    if (graph) {
+      PCG_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+      sampleGraph.capturing = s;
+    }
     PCG_CHECK(hipEventRecord(ev_start, s));
     // every group's work is enqueued before the library stream's waits, so
     // that a stream sharing a hardware queue with the library stream is
@@ -855,12 +892,9 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     for (int l = maxLevel; l >= 0; l--)
       for (int g = 0; g < G; g++)
         if (run_level(g, l, gm[g % S], gp[g], gs[g % S])) return 1;
-    for (int k = 0; k < S; k++) {
-      hipEvent_t ev_done = group_event(1 + k);
-      if (!ev_done) return 1;
-      PCG_CHECK(hipEventRecord(ev_done, gs[k]));
-    }
+    for (int k = 0; k < S; k++) PCG_CHECK(hipEventRecord(group_event(1 + k), gs[k]));
     for (int k = 0; k < S; k++) PCG_CHECK(hipStreamWaitEvent(s, group_event(1 + k), 0));
+    if (graph && sampleGraph.launch()) return 1;
     walk_phase1();
     // the nodes above the chunk roots: their candidates are points the
     // chunk roots kept

@@ -60,6 +60,9 @@ extern int32_t g_sample_sets;
 // 1 = heights (a round holds every node whose children were all sampled in
 // earlier rounds: leaves' parents first, whatever their depth).
 extern int32_t g_sample_order;
+// Capture the chunk-subtree sampling (every set's launch sequence on its
+// stream) in a graph and launch that instead of the launches one by one.
+extern bool g_sample_graph;
 // Launch shapes of the node sampling (same results for every value): k_ring
 // blocks per CU (1-3), grids of the level-wide record copy and neighbour
 // search, workgroups of a Poisson-average colour-sum launch.

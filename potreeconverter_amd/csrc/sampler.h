@@ -104,6 +104,10 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
 
 // Scratch bytes for one node of n candidates.
 uint64_t sample_scratch_bytes(int64_t n, int method);
+
+// Clears n <= 64 words with a kernel (no memset node, so that the sampling
+// launch sequence can be captured in a graph).
+int sample_zero(uint64_t* d, int32_t n, hipStream_t s);
 // Scratch bytes that cover any level of `max_nodes` nodes holding at most
 // `candidates` candidates in total.
 uint64_t sample_scratch_bound(int64_t candidates, int64_t max_nodes, int method);

@@ -2947,6 +2947,22 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   return 0;
 }
 
+namespace {
+__global__ void k_zero_u64(uint64_t* __restrict__ p, int32_t n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+}  // namespace
+
+int sample_zero(uint64_t* d, int32_t n, hipStream_t s) {
+  if (n < 0 || n > 64) {
+    set_error("sample_zero: at most 64 words");
+    return 1;
+  }
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, d, n);
+  PCG_CHECK(hipGetLastError());
+  return 0;
+}
+
 uint64_t sample_scratch_bytes(int64_t n, int method) { return layout_bytes(n, method); }
 
 uint64_t sample_scratch_bound(int64_t candidates, int64_t max_nodes, int method) {

@@ -12,7 +12,7 @@ from helpers import make_las, oracle_convert
 pytestmark = pytest.mark.gpu
 
 FILES = ["octree.bin", "hierarchy.bin", "metadata.json"]
-TUNED = ["sample_groups", "sample_order", "ring_min_nodes", "ring_blocks_per_cu", "copy_blocks",
+TUNED = ["sample_groups", "sample_order", "sample_graph", "ring_min_nodes", "ring_blocks_per_cu", "copy_blocks",
          "neighbor_blocks", "avg_blocks"]
 
 
