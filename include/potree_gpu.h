@@ -68,7 +68,12 @@ int pcg_release(void);
  * "ring_blocks_per_cu" (1-3, default 3) persistent ring blocks per CU,
  * "copy_blocks" (default 8192) and "neighbor_blocks" (default 4096) grids of
  * the level-wide record copy and neighbour search, "avg_blocks" (default
- * 2048) workgroups of a Poisson-average colour-sum launch.
+ * 2048) workgroups of a Poisson-average colour-sum launch,
+ * "bucket_min_nodes" (default 512) launches of at least this many nodes sort
+ * each node's candidates by distance buckets, smaller ones by radix passes.
+ * "set_policy" (0/1, default 0): chunk subtrees split into sets by their
+ * points (0) or with the subtrees deeper than where most points end in sets
+ * of their own (1) (same results).
  * The other options have no reference counterpart. */
 int pcg_set_option(const char* key, int64_t value);
 

@@ -214,7 +214,8 @@ int pcg_set_option(const char* key, int64_t value) {
   const Shape shapes[] = {{"ring_blocks_per_cu", &g_ring_blocks_per_cu, 1, 3},
                           {"copy_blocks", &g_copy_blocks, 256, 1 << 20},
                           {"neighbor_blocks", &g_neighbor_blocks, 256, 1 << 20},
-                          {"avg_blocks", &g_avg_blocks, 64, 1 << 20}};
+                          {"avg_blocks", &g_avg_blocks, 64, 1 << 20},
+                          {"bucket_min_nodes", &g_bucket_min_nodes, 1, 1 << 30}};
   for (const Shape& sh : shapes)
     if (k == sh.name) {
       if (value < sh.lo || value > sh.hi) {
@@ -224,6 +225,14 @@ int pcg_set_option(const char* key, int64_t value) {
       *sh.value = (int32_t)value;
       return 0;
     }
+  if (k == "set_policy") {
+    if (value != 0 && value != 1) {
+      set_error("set_policy must be 0 (points) or 1 (deep subtrees apart)");
+      return 1;
+    }
+    g_set_policy = (int32_t)value;
+    return 0;
+  }
   if (k == "sample_graph") {
     g_sample_graph = value != 0;
     return 0;
@@ -262,9 +271,10 @@ int pcg_get_option(const char* key, int64_t* value) {
       {"plain_sampling", g_plain_sampling ? 1 : 0}, {"ring_min_nodes", g_ring_min_nodes},
       {"random_seed", g_random_seed},               {"sample_groups", g_sample_groups},
       {"sample_sets", g_sample_sets},               {"sample_order", g_sample_order},
-      {"sample_graph", g_sample_graph ? 1 : 0},
+      {"sample_graph", g_sample_graph ? 1 : 0},      {"set_policy", g_set_policy},
       {"ring_blocks_per_cu", g_ring_blocks_per_cu}, {"copy_blocks", g_copy_blocks},
       {"neighbor_blocks", g_neighbor_blocks},       {"avg_blocks", g_avg_blocks},
+      {"bucket_min_nodes", g_bucket_min_nodes},
       {"distribute_tail", g_distribute_tail ? 1 : 0}, {"chunk_grid", g_chunk_grid},
       {"chunk_cap", g_chunk_cap}};
   for (const auto& kv : all)
@@ -290,11 +300,13 @@ const std::vector<TuneParam>& tune_params() {
       {"sample_groups", {1, 2, 3, 4}},
       {"sample_order", {0, 1}},
       {"sample_graph", {0, 1}},
+      {"set_policy", {0, 1}},
       {"ring_min_nodes", {64, 128, 256, 512, 1024, 2048, 4096}},
       {"ring_blocks_per_cu", {1, 2, 3}},
       {"copy_blocks", {2048, 4096, 8192, 16384}},
       {"neighbor_blocks", {1024, 2048, 4096, 8192}},
       {"avg_blocks", {512, 1024, 2048, 4096}},
+      {"bucket_min_nodes", {128, 512, 2048, 1 << 30}},
   };
   return p;
 }

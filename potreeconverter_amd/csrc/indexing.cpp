@@ -168,10 +168,12 @@ int32_t g_sample_groups = 0;
 int32_t g_sample_sets = 0;
 int32_t g_sample_order = 0;
 bool g_sample_graph = false;
+int32_t g_set_policy = 0;
 int32_t g_ring_blocks_per_cu = 3;
 int32_t g_copy_blocks = 8192;
 int32_t g_neighbor_blocks = 4096;
 int32_t g_avg_blocks = 2048;
+int32_t g_bucket_min_nodes = 512;
 
 void IndexResult::release() {
   d_octree = nullptr;  // a workspace buffer
@@ -541,8 +543,42 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
     std::vector<size_t> crs(chunkRoots.size());
     for (size_t i = 0; i < crs.size(); i++) crs[i] = i;
     std::stable_sort(crs.begin(), crs.end(), [&](size_t x, size_t y) { return chunkRootPoints[x] > chunkRootPoints[y]; });
+    // Set policy 1: the chunk subtrees that reach below the level where most
+    // points end (dense spots: long chains of narrow, latency-bound levels)
+    // take sets of their own, in proportion to their points, so that their
+    // chains run beside the wide levels of the others instead of every set
+    // starting with narrow levels at once.
+    std::vector<int> deepest(chunkRoots.size(), 0);
+    int setsDeep = 0;
+    if (g_set_policy == 1) {
+      std::map<int, int64_t> pointsAt;
+      for (size_t ci = 0; ci < chunkRoots.size(); ci++) {
+        std::vector<int32_t> stack{chunkRoots[ci]};
+        while (!stack.empty()) {
+          const int32_t id = stack.back();
+          stack.pop_back();
+          deepest[ci] = std::max(deepest[ci], tree.nodes[id].level);
+          for (int c = 0; c < 8; c++)
+            if (tree.nodes[id].children[c] >= 0) stack.push_back(tree.nodes[id].children[c]);
+        }
+        pointsAt[deepest[ci]] += chunkRootPoints[ci];
+      }
+      int mode = 0;
+      int64_t best = -1;
+      for (auto& kv : pointsAt)
+        if (kv.second > best) { best = kv.second; mode = kv.first; }
+      int64_t deepPoints = 0;
+      for (size_t ci = 0; ci < chunkRoots.size(); ci++)
+        if (deepest[ci] > mode) deepPoints += chunkRootPoints[ci];
+      if (deepPoints > 0 && deepPoints < N)
+        setsDeep = std::max(1, std::min(G - 1, (int)std::lround((double)G * (double)deepPoints / (double)N)));
+      for (size_t ci = 0; ci < chunkRoots.size(); ci++) deepest[ci] = deepest[ci] > mode ? 1 : 0;
+    }
     for (size_t ci : crs) {
-      int g = (int)(std::min_element(groupPoints.begin(), groupPoints.end()) - groupPoints.begin());
+      // longest processing time first within the chunk's share of the sets
+      const bool deep = setsDeep > 0 && deepest[ci];
+      const int lo = setsDeep == 0 ? 0 : (deep ? 0 : setsDeep), hi = setsDeep == 0 ? G : (deep ? setsDeep : G);
+      int g = (int)(std::min_element(groupPoints.begin() + lo, groupPoints.begin() + hi) - groupPoints.begin());
       groupPoints[g] += chunkRootPoints[ci];
       std::vector<int32_t> stack{chunkRoots[ci]};
       while (!stack.empty()) {
@@ -673,6 +709,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   sp.copy_blocks = g_copy_blocks;
   sp.neighbor_blocks = g_neighbor_blocks;
   sp.avg_blocks = g_avg_blocks;
+  sp.bucket_min_nodes = g_bucket_min_nodes;
   // a group's buffers for levels of at most `maxNodes` nodes and `cand`
   // candidates in all
   auto group_params = [&](GroupMem& m, int64_t maxNodes, int64_t cand, SampleParams& p) -> int {
@@ -785,6 +822,18 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
         unsigned long long hc7 = 0;
         PCG_CHECK(stream_copy(&hc7, counters.as<unsigned long long>() + 7, 8, hipMemcpyDeviceToHost));
         std::sort(ns.begin(), ns.end());
+        {
+          // the largest nodes outside the ring (cell path, exact sort)
+          std::vector<std::pair<int64_t, uint32_t>> big;
+          for (const NodeWork& w : wk)
+            if (w.flags & (NW_CELL_PATH | NW_EXACT_SORT)) big.push_back({w.n, w.flags});
+          std::sort(big.begin(), big.end(), [](auto& a, auto& b) { return a.first > b.first; });
+          std::string txt;
+          for (size_t i = 0; i < big.size() && i < 6; i++)
+            txt += " " + std::to_string(big[i].first) + "/" + std::to_string(big[i].second);
+          fprintf(stderr, "[%s %d] largest flagged nodes (candidates/flags):%s\n", g_sample_order ? "round" : "level",
+                  g_sample_order ? maxLevel + 1 - l : l, txt.c_str());
+        }
         fprintf(stderr, "[%s %d] %.3f ms, candidates per node p50 %lld p90 %lld p99 %lld\n",
                 g_sample_order ? "round" : "level", g_sample_order ? maxLevel + 1 - l : l, 1e3 * tLevel,
                 (long long)ns[ns.size() / 2], (long long)ns[ns.size() * 9 / 10], (long long)ns[ns.size() * 99 / 100]);

@@ -63,6 +63,11 @@ extern int32_t g_sample_order;
 // Capture the chunk-subtree sampling (every set's launch sequence on its
 // stream) in a graph and launch that instead of the launches one by one.
 extern bool g_sample_graph;
+// How chunk subtrees are split into sets: 0 = longest processing time first
+// on their points over all sets; 1 = subtrees deeper than where most points
+// end get sets of their own (in proportion to their points), the others the
+// rest.
+extern int32_t g_set_policy;
 // Launch shapes of the node sampling (same results for every value): k_ring
 // blocks per CU (1-3), grids of the level-wide record copy and neighbour
 // search, workgroups of a Poisson-average colour-sum launch.
@@ -70,6 +75,9 @@ extern int32_t g_ring_blocks_per_cu;
 extern int32_t g_copy_blocks;
 extern int32_t g_neighbor_blocks;
 extern int32_t g_avg_blocks;
+// Sampling launches of at least this many nodes sort each node's candidates
+// by distance buckets, smaller ones by radix passes (same order).
+extern int32_t g_bucket_min_nodes;
 // Seed of every SamplerRandom leaf shuffle (0 .. 2^32-1), or -1 for the
 // reference's choice, the system clock at the start of indexing.
 extern int64_t g_random_seed;

@@ -95,7 +95,7 @@ struct SampleParams {
   int32_t copy_blocks;     // grid of the level-wide record copy
   int32_t neighbor_blocks; // grid of the level-wide neighbour search
   int32_t avg_blocks;      // workgroups of the Poisson-average colour sums per launch
-  int32_t pad3;
+  int32_t bucket_min_nodes;  // launches of at least this many nodes bucket-sort their candidates
 };
 
 // Samples every node of `nodes` (all of one level, children already done).

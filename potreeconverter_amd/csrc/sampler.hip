@@ -388,14 +388,70 @@ __device__ bool tie_group_conflicts(const Layout& L, const NodeGeom& g, const Sa
 }
 
 // ---------------------------------------------------------------- prepare
+// Bucket sort of a node's candidates: the squared distance to the centre is
+// at most a quarter of the box diagonal squared, so floor(add * B / that)
+// (clamped) is a bucket index that never decreases with the distance, and
+// for points on a surface the buckets fill about evenly.  One counting pass
+// during the gather, one scatter of (distance, index) and an insertion sort
+// of each bucket by (distance, index) give the same order as the radix sort
+// below, with the payload moved once.  Nodes with a bucket of more than
+// kBucketMaxRun candidates (dense clusters) take the radix sort.
+constexpr int kBuckets = 8192;      // most buckets of a node (LDS counters)
+constexpr int kBucketMaxRun = 96;   // largest bucket sorted by one lane
+constexpr int64_t kBucketMaxN = 65536;
+
+// (distance, index) pairs [st, st + k) of sadd / keyB, k <= 8, sorted by an
+// odd-even transposition network; indices to idxA
+__device__ inline void bucket_sort8(const Layout& L, uint32_t st, uint32_t k) {
+  if (k == 0) return;
+  if (k == 1) {
+    L.idxA[st] = L.keyB[st];
+    return;
+  }
+  double d[8];
+  uint32_t v[8];
+#pragma unroll
+  for (uint32_t t = 0; t < 8; t++) {
+    d[t] = t < k ? L.sadd[st + t] : __builtin_inf();
+    v[t] = t < k ? L.keyB[st + t] : 0xFFFFFFFFu;
+  }
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+#pragma unroll
+    for (int t = r & 1; t + 1 < 8; t += 2) {
+      const bool sw = d[t] > d[t + 1] || (d[t] == d[t + 1] && v[t] > v[t + 1]);
+      const double dl = sw ? d[t + 1] : d[t], dh = sw ? d[t] : d[t + 1];
+      const uint32_t vl = sw ? v[t + 1] : v[t], vh = sw ? v[t] : v[t + 1];
+      d[t] = dl; d[t + 1] = dh;
+      v[t] = vl; v[t + 1] = vh;
+    }
+  }
+#pragma unroll
+  for (uint32_t t = 0; t < 8; t++)
+    if (t < k) {
+      L.sadd[st + t] = d[t];
+      L.idxA[st + t] = v[t];
+    }
+}
+
+__device__ inline uint32_t dist_bucket(double add, double scale, uint32_t B) {
+  const double t = add * scale;
+  return t < (double)(B - 1) ? (uint32_t)t : B - 1;
+}
+
 __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
                                                      const NodeState* __restrict__ states, NodeWork* __restrict__ work,
                                                      SampleParams P) {
   __shared__ int64_t s_cs[9];
   __shared__ const uint8_t* s_cptr[8];
   __shared__ uint8_t* s_scr;
-  __shared__ RadixShared s_rad;
+  __shared__ union {
+    RadixShared rad;
+    uint32_t bucket[kBuckets];
+  } s_u;
+  RadixShared& s_rad = s_u.rad;
   __shared__ uint32_t s_h4[4][256];  // histograms of the four key bytes, counted while gathering
+  __shared__ uint32_t s_maxb, s_wsum[kWaves];
   const int k = blockIdx.x;
   const NodeInfo info = infos[nodes[k]];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -433,8 +489,23 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
   const int64_t n = s_cs[8];
   Layout L = make_layout(s_scr, n, P.method);
   NodeGeom g = node_geom(info, P);
+  // bucket sort for nodes up to kBucketMaxN candidates (about two per
+  // bucket up to 16 384 candidates), the radix sort otherwise
+  const double maxAdd = 0.25 * (g.size[0] * g.size[0] + g.size[1] * g.size[1] + g.size[2] * g.size[2]);
+  uint32_t B = 64;
+  while ((int64_t)B * 2 < n && B < (uint32_t)kBuckets) B <<= 1;
+  // (launches of few nodes, where one node's latency decides, keep the
+  // radix sort: its passes are shorter for a single large node)
+  bool bucketPath = n >= 2 && n <= kBucketMaxN && maxAdd > 0.0 && (int)gridDim.x >= P.bucket_min_nodes;
+  const double bscale = bucketPath ? (double)B / maxAdd : 0.0;
+  if (bucketPath) {
+    for (uint32_t i = tid; i < B; i += kBlock) s_u.bucket[i] = 0;
+    if (tid == 0) s_maxb = 0;
+    __syncthreads();
+  }
   // gather (sampler_poisson.h:82-108), four records per thread in flight;
-  // the byte histograms of the sort keys are counted on the way
+  // the bucket counts or the byte histograms of the sort keys are counted
+  // on the way
   for (int64_t j0 = 0; j0 < n; j0 += 4 * kBlock) {
     int32_t X4[4], Y4[4], Z4[4];
     const uint8_t* r4[4];
@@ -462,15 +533,19 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
       double add = ax * ax + ay * ay + az * az;
       L.xyz[j] = make_int4(X, Y, Z, 0);
       L.add[j] = add;
-      float f = (float)add;
-      uint32_t fb;
-      __builtin_memcpy(&fb, &f, 4);
-      L.keyA[j] = fb;
-      L.idxA[j] = (uint32_t)j;
-      atomicAdd(&s_h4[0][fb & 255u], 1u);
-      atomicAdd(&s_h4[1][(fb >> 8) & 255u], 1u);
-      atomicAdd(&s_h4[2][(fb >> 16) & 255u], 1u);
-      atomicAdd(&s_h4[3][fb >> 24], 1u);
+      if (bucketPath) {
+        atomicAdd(&s_u.bucket[dist_bucket(add, bscale, B)], 1u);
+      } else {
+        float f = (float)add;
+        uint32_t fb;
+        __builtin_memcpy(&fb, &f, 4);
+        L.keyA[j] = fb;
+        L.idxA[j] = (uint32_t)j;
+        atomicAdd(&s_h4[0][fb & 255u], 1u);
+        atomicAdd(&s_h4[1][(fb >> 8) & 255u], 1u);
+        atomicAdd(&s_h4[2][(fb >> 16) & 255u], 1u);
+        atomicAdd(&s_h4[3][fb >> 24], 1u);
+      }
       if (P.method == METHOD_POISSON_AVERAGE) {
         L.cell16[j] = pack16(cell16_axis(x, g.min[0], g.size[0]), cell16_axis(y, g.min[1], g.size[1]),
                              cell16_axis(z, g.min[2], g.size[2]));
@@ -482,13 +557,105 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
     }
   }
   __syncthreads();
+  if (bucketPath) {
+    // the largest bucket decides between the bucket sort and the radix sort
+    uint32_t m = 0;
+    for (uint32_t i = tid; i < B; i += kBlock) m = s_u.bucket[i] > m ? s_u.bucket[i] : m;
+    if (m > (uint32_t)kBucketMaxRun) atomicMax(&s_maxb, m);
+    __syncthreads();
+    if (s_maxb) {
+      // radix sort after all: its keys and indices, counted by its passes
+      bucketPath = false;
+      for (int64_t j = tid; j < n; j += kBlock) {
+        float f = (float)L.add[j];
+        uint32_t fb;
+        __builtin_memcpy(&fb, &f, 4);
+        L.keyA[j] = fb;
+        L.idxA[j] = (uint32_t)j;
+      }
+      __syncthreads();
+    }
+  }
+  if (bucketPath) {
+    // exclusive prefix of the bucket counts: B / kBlock consecutive buckets
+    // per thread (B is a power of two of at least 64)
+    const uint32_t per = B >= (uint32_t)kBlock ? B / kBlock : 1u;
+    const uint32_t b0 = (uint32_t)tid * per;
+    uint32_t mine = 0;
+    if (b0 < B)
+      for (uint32_t i = 0; i < per; i++) mine += s_u.bucket[b0 + i];
+    uint32_t incl = mine;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t o = __shfl_up(incl, d);
+      if (lane >= d) incl += o;
+    }
+    if (lane == 63) s_wsum[wave] = incl;
+    __syncthreads();
+    uint32_t run = incl - mine;
+    for (int w = 0; w < wave; w++) run += s_wsum[w];
+    if (b0 < B)
+      for (uint32_t i = 0; i < per; i++) {
+        const uint32_t c = s_u.bucket[b0 + i];
+        s_u.bucket[b0 + i] = run;
+        run += c;
+      }
+    __syncthreads();
+    // scatter (distance, index) to the buckets; each counter ends at its
+    // bucket's end
+    for (int64_t j0 = 0; j0 < n; j0 += 4 * kBlock) {
+      double a4[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t j = j0 + u * kBlock + tid;
+        a4[u] = j < n ? L.add[j] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t j = j0 + u * kBlock + tid;
+        if (j >= n) continue;
+        const uint32_t pos = atomicAdd(&s_u.bucket[dist_bucket(a4[u], bscale, B)], 1u);
+        L.sadd[pos] = a4[u];
+        L.keyB[pos] = (uint32_t)j;
+      }
+    }
+    __syncthreads();
+    // each bucket in (distance, index) order by one lane: up to eight
+    // entries loaded together and sorted in registers, longer buckets by
+    // insertion in place
+    for (uint32_t b = tid; b < B; b += kBlock) {
+      const uint32_t e = s_u.bucket[b], st = b ? s_u.bucket[b - 1] : 0u;
+      if (e - st <= 8) {
+        bucket_sort8(L, st, e - st);
+        continue;
+      }
+      for (uint32_t i = st + 1; i < e; i++) {
+        const double dv = L.sadd[i];
+        const uint32_t v = L.keyB[i];
+        uint32_t q = i;
+        while (q > st) {
+          const double dq = L.sadd[q - 1];
+          const uint32_t vq = L.keyB[q - 1];
+          if (!(dq > dv || (dq == dv && vq > v))) break;
+          L.sadd[q] = dq;
+          L.keyB[q] = vq;
+          q--;
+        }
+        L.sadd[q] = dv;
+        L.keyB[q] = v;
+      }
+      for (uint32_t i = st; i < e; i++) L.idxA[i] = L.keyB[i];
+    }
+    __syncthreads();
+  } else {
   // stable LSD radix sort on float(add) bits
   uint32_t *kc = L.keyA, *ic = L.idxA, *ko = L.keyB, *io = L.idxB;
   // the float key, 4 passes; runs of equal keys (distinct doubles that round
   // to the same float, or equal distances) are put in (double add, main
   // index) order below
+  const bool precounted = n > kBucketMaxN || maxAdd <= 0.0 || n < 2 || (int)gridDim.x < P.bucket_min_nodes;
   for (int pass = 0; pass < 4; pass++) {
-    if (!block_radix_pass(kc, ic, ko, io, n, 8 * pass, 255u, s_rad, s_h4[pass])) continue;
+    if (!block_radix_pass(kc, ic, ko, io, n, 8 * pass, 255u, s_rad, precounted ? s_h4[pass] : nullptr)) continue;
     uint32_t* t = kc; kc = ko; ko = t;
     t = ic; ic = io; io = t;
   }
@@ -573,6 +740,7 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
     }
   }
   __syncthreads();
+  }
   // equal distances: the reference's unstable sort decides their order;
   // it matters only if two of them can exclude each other
   bool conflict = false;

@@ -271,9 +271,9 @@ def test_plain_sampling_matches_oracle(method, tmp_path):
 
 
 def test_ring_and_cell_paths_agree_at_20m(tmp_path):
-    """20M points, where the widest levels use the ring by default: the same
-    bytes as with every level on the cell-table path and every level on the
-    ring."""
+    """20M points, where the widest levels use the ring and the bucket sort
+    by default: the same bytes as with every level on the cell-table path and
+    bucket-sorted, and every level on the ring and radix-sorted."""
     L = pc.lib()
     n = 20_000_000
     g = pc.LasGen(1, n, seed=3)
@@ -290,15 +290,17 @@ def test_ring_and_cell_paths_agree_at_20m(tmp_path):
     stats = (ctypes.c_double * 24)()
     outs = []
     try:
-        for k, ring_min in enumerate((512, 1 << 30, 1)):
-            print(f"20M run ring_min_nodes={ring_min}", flush=True)
+        for k, (ring_min, bucket_min) in enumerate(((512, 512), (1 << 30, 1), (1, 1 << 30))):
+            print(f"20M run ring_min_nodes={ring_min} bucket_min_nodes={bucket_min}", flush=True)
             _set("ring_min_nodes", ring_min)
+            _set("bucket_min_nodes", bucket_min)
             o = tmp_path / f"run{k}"
             rc = L.pcg_convert_device(d, n, hdr, b"poisson", str(o).encode(), stats)
             assert rc == 0, L.pcg_last_error()
             outs.append(o)
     finally:
         _set("ring_min_nodes", 512)
+        _set("bucket_min_nodes", 512)
         L.pcg_device_free(d)
         L.pcg_device_free(dq)
     for o in outs[1:]:
