@@ -2391,13 +2391,33 @@ __global__ __launch_bounds__(kBlock) void k_route(const int32_t* __restrict__ no
   const bool avg = P.method == METHOD_POISSON_AVERAGE;
   const bool seq_avg = (w.flags & NW_SEQ_AVERAGE) != 0;
   Layout L = make_layout(w.scratch, n, P.method);
-  uint32_t* inv = L.keyB;  // main index -> sorted position
-  for (int64_t p = tid; p < n; p += kBlock) inv[L.idxA[p]] = (uint32_t)p;
+  // accepted flags in main order: scattered from sorted order (Poisson), or
+  // looked up through the inverse permutation, which Poisson-average needs
+  // for its colour sums
+  uint32_t* inv = L.keyB;  // main index -> sorted position (Poisson-average)
+  for (int64_t p0 = 0; p0 < n; p0 += 4 * kBlock) {
+    uint32_t j4[4];
+    uint8_t a4[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = p0 + u * kBlock + tid;
+      j4[u] = p < n ? L.idxA[p] : 0u;
+      a4[u] = (!avg && p < n) ? L.acc[p] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = p0 + u * kBlock + tid;
+      if (p >= n) continue;
+      if (avg) inv[j4[u]] = (uint32_t)p;
+      else L.accMain[j4[u]] = a4[u];
+    }
+  }
   if (tid < 8) s_acc[tid] = 0;
   __syncthreads();
   for (int c = 0; c < 8; c++) {
     unsigned long long cnt = 0;
-    for (int64_t j = w.child_start[c] + tid; j < w.child_start[c + 1]; j += kBlock) cnt += L.acc[inv[j]];
+    for (int64_t j = w.child_start[c] + tid; j < w.child_start[c + 1]; j += kBlock)
+      cnt += avg ? L.acc[inv[j]] : L.accMain[j];
     if (cnt) atomicAdd((unsigned long long*)&s_acc[c], cnt);
   }
   __syncthreads();
@@ -2458,10 +2478,13 @@ __global__ __launch_bounds__(kBlock) void k_route(const int32_t* __restrict__ no
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int64_t j = b0 + u * kBlock + tid;
-      q4[u] = j < n ? inv[j] : 0u;
+      q4[u] = (avg && j < n) ? inv[j] : 0u;
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) a4[u] = b0 + u * kBlock + tid < n && L.acc[q4[u]];
+    for (int u = 0; u < 4; u++) {
+      const int64_t j = b0 + u * kBlock + tid;
+      a4[u] = j < n && (avg ? L.acc[q4[u]] : L.accMain[j]);
+    }
     uint64_t bal[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -2484,7 +2507,7 @@ __global__ __launch_bounds__(kBlock) void k_route(const int32_t* __restrict__ no
         const int64_t local = j - w.child_start[c];
         const int64_t apos = run + woff + before_in_wave;
         if ((local & 63) == 0) L.tileAcc[tile_base[c] + (local >> 6)] = (uint32_t)apos;
-        L.accMain[j] = a ? 1 : 0;
+        if (avg) L.accMain[j] = a ? 1 : 0;
         if (a) {
           if (avg) {
             long long rs[3], ws;
