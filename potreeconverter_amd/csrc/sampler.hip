@@ -1898,11 +1898,14 @@ __device__ void ring_node(const int32_t* __restrict__ nodes, const NodeInfo* __r
             cur[t] = nx == kRingEnd ? -1 : idx - nx;
             // chains are newest first up to reordering inside one window,
             // so nothing live follows an entry older than tail - 64
-            if (idx < tail || (double)R.add[e] < limSq) continue;
+            if (idx < tail) continue;
             tests++;
             const float dx = R.x[e] - xf, dy = R.y[e] - yf, dz = R.z[e] - zf;
             const float d32 = dx * dx + dy * dy + dz * dz;
             if (d32 > G.band_hi) continue;
+            // an entry nearer the centre than limSq cannot reject (checked
+            // only for the few entries inside the band's upper edge)
+            if ((double)R.add[e] < limSq) continue;
             bool within = d32 < G.band_lo;
             if (!within) {
               const uint32_t jq = L.idxA[rpos[e]];
@@ -1926,14 +1929,18 @@ __device__ void ring_node(const int32_t* __restrict__ nodes, const NodeInfo* __r
       const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xf), q));
       const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(yf), q));
       const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(zf), q));
-      const double qa = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(a), q),
-                                         __builtin_amdgcn_readlane(__double2loint(a), q));
       const float dx = qx - xf, dy = qy - yf, dz = qz - zf;
       const float d32 = dx * dx + dy * dy + dz * dz;
-      const bool cand = lane > q && !(qa < limSq);
+      // most earlier candidates of a window lie far away: the rest of the
+      // test only where some later lane has q within the band's upper edge
+      const bool nearq = lane > q && !(d32 > G.band_hi);
+      if (__ballot(nearq) == 0) continue;
+      const double qa = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(a), q),
+                                         __builtin_amdgcn_readlane(__double2loint(a), q));
+      const bool cand = nearq && !(qa < limSq);
       const uint64_t bit = 1ull << q;
       in_mask |= (cand && d32 < G.band_lo) ? bit : 0ull;
-      m_band |= (cand && !(d32 < G.band_lo) && !(d32 > G.band_hi)) ? bit : 0ull;
+      m_band |= (cand && !(d32 < G.band_lo)) ? bit : 0ull;
     }
     if (!valid || rej) { in_mask = 0; m_band = 0; }
     tests += (uint64_t)__popcll(surv & lt);
