@@ -531,6 +531,16 @@ def main():
             "algorithmic_bytes": chunk_bytes,
             "kernel_ms": round(kern_s * 1e3, 4),
         },
+        "sampling_roofline": {
+            "bound": "hbm",
+            "kernel": "node sampling (k_prepare .. k_copy_tiles, three set streams)",
+            "algorithmic_bytes_per_candidate": 66,
+            "candidates": int(last[10]),
+            "achieved": round(66 * last[10] / last[5] / 1e9, 1) if last[5] > 0 else None,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(66 * last[10] / last[5] / 1e9 / HBM_PEAK_GBS, 4) if last[5] > 0 else None,
+        },
         "precision": precision,
         "autotune": tuned,
         "io_s": io,
