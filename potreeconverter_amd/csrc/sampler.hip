@@ -45,6 +45,7 @@ constexpr int kMaxNbr = 16;  // earlier neighbours kept per candidate
 constexpr uint8_t kNbrOverflow = 255;  // list too long for the pool: search again
 constexpr uint8_t kNbrPooled = 254;    // list in the level's neighbour pool
 constexpr int kAccWords = 2048;         // LDS accepted bits per wave in k_accept (65536 candidates)
+constexpr int kAccDeepNodes = 2048;     // k_accept launches up to this many nodes prefetch three windows
 
 __device__ inline int32_t ld_i32(const uint8_t* p) {
   return (int32_t)((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24));
@@ -63,6 +64,21 @@ __device__ inline void ld_xyz(const uint8_t* p, int32_t& X, int32_t& Y, int32_t&
 }
 
 __host__ __device__ inline uint64_t align16(uint64_t v) { return (v + 15) & ~15ull; }
+
+// Pointers into the node scratch come from memory, so the compiler cannot
+// tell their address space and emits flat instructions; a cast to a
+// global-address pointer makes the loads and stores through it global ones.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ inline gptr<T> gbl(T* p) { return (gptr<T>)p; }
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+// 16 bytes through a global pointer (the HIP vector types cannot be assigned
+// from a global-address reference)
+__device__ inline uint4 ld_g16(gptr<const uint4> p) {
+  const u32x4_t v = *(gptr<const u32x4_t>)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
 
 __host__ __device__ inline uint32_t hash_cap(int64_t n, int method) {
   uint32_t h = 64;
@@ -1569,8 +1585,13 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(const int32_t* __restrict_
 // window; inside the window the dependencies are resolved by ballots.  The
 // accepted flags of the node live in LDS (global memory past 65536
 // candidates).
+template <int kAccAhead, class BitsPtr>
+__device__ void accept_windows(NodeWork* __restrict__ work, int k, const SampleParams& P, const Layout& L,
+                               const NodeGeom& g, Filter& F, const int64_t n, BitsPtr bits);
+
 // Greedy acceptance of node k by one wavefront; the accepted bits live in
 // `lds_bits` (lds_words words) when they fit, in global memory otherwise.
+template <int kAccAhead>
 __device__ void accept_node(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
                             NodeWork* __restrict__ work, int k, const SampleParams& P, uint32_t* lds_bits,
                             int64_t lds_words) {
@@ -1583,159 +1604,214 @@ __device__ void accept_node(const int32_t* __restrict__ nodes, const NodeInfo* _
   const bool avg = P.method == METHOD_POISSON_AVERAGE;
   Filter F = make_filter(g, P);
   const bool lds = n <= lds_words * 32;
-  uint32_t* bits = lds ? lds_bits : L.keyA;
+  if (lds)
+    accept_windows<kAccAhead>(work, k, P, L, g, F, n, lds_bits);
+  else
+    accept_windows<kAccAhead>(work, k, P, L, g, F, n, gbl(L.keyA));
+  flush_filter(F, P);
+}
+
+// The window loop of accept_node with the accepted bits at `bits`, an LDS
+// or a global pointer, so that each copy of the loop reads them with LDS or
+// global instructions.  The node's arrays are read through global-address
+// pointers: the flat instructions that a generic pointer compiles to also
+// count as LDS operations, and a wait for the accepted-bit reads would then
+// wait for every prefetch in flight too.
+template <int kAccAhead, class BitsPtr>
+__device__ void accept_windows(NodeWork* __restrict__ work, int k, const SampleParams& P, const Layout& L,
+                               const NodeGeom& g, Filter& F, const int64_t n, BitsPtr bits) {
+  const int lane = threadIdx.x & 63;
+  const bool avg = P.method == METHOD_POISSON_AVERAGE;
+  const gptr<const uint8_t> gCnt = gbl((const uint8_t*)L.nbrCount);
+  const gptr<const double> gSadd = gbl((const double*)L.sadd);
+  const gptr<const uint4> gNbr = gbl((const uint4*)L.nbr);
+  const gptr<uint8_t> gAcc = gbl(L.acc);
+  if (n <= 0) return;
   const int64_t words = (n + 31) / 32;
   for (int64_t i = lane; i < words; i += 64) bits[i] = 0;
-  if (!lds) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-  // the next window's neighbour rows are loaded while this one is decided
-  uint4 nr0 = make_uint4(0, 0, 0, 0), nr1 = nr0, nr2 = nr0, nr3 = nr0;
-  int ncnt = 0;
-  double nsadd = 0.0;
-  auto prefetch = [&](int64_t p) {
-    ncnt = 0;
-    nsadd = 0.0;
-    if (p < n) {
-      ncnt = L.nbrCount[p];
-      nsadd = L.sadd[p];
-      const uint4* rw = (const uint4*)(L.nbr + (uint64_t)p * kMaxNbr);
-      nr0 = rw[0]; nr1 = rw[1]; nr2 = rw[2]; nr3 = rw[3];
+  // the bits are written and read by this wavefront only: ordering its own
+  // accesses needs no cache maintenance
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  // The window loop is bound by memory latency (one wavefront per node), so
+  // the neighbour rows, counts and distances of the next kAccAhead windows
+  // are in flight while a window is decided; the slots are unrolled so that
+  // no register of a load in flight is moved.  The value the 10 001-check
+  // test of a window reads is loaded with its window.
+  struct Slot {
+    uint4 r0, r1, r2, r3;
+    int cnt;
+    double sadd, capv;
+  };
+  Slot sl[kAccAhead];
+#pragma unroll
+  for (int s = 0; s < kAccAhead; s++) sl[s].r0 = sl[s].r1 = sl[s].r2 = sl[s].r3 = make_uint4(0, 0, 0, 0);
+  // Every lane loads on every window, from a clamped position when it lies
+  // past the node (its values are then unused): with no branch around the
+  // loads, the number of loads issued after a slot's is fixed, and the wait
+  // for a slot leaves the younger slots' loads in flight instead of waiting
+  // for all of them.
+  auto prefetch = [&](Slot& s, int64_t w0) {
+    const int64_t p = w0 + lane < n ? w0 + lane : n - 1;
+    s.cnt = gCnt[p];
+    s.sadd = gSadd[p];
+    const gptr<const uint4> rw = gNbr + (uint64_t)p * (kMaxNbr / 4);
+    s.r0 = ld_g16(rw); s.r1 = ld_g16(rw + 1); s.r2 = ld_g16(rw + 2); s.r3 = ld_g16(rw + 3);
+    if (kAccAhead > 1) {
+      const int64_t hi = (w0 + 64 < n ? w0 + 64 : n);
+      const int64_t c = hi - 10002;
+      s.capv = gSadd[c < 0 ? 0 : (c < n ? c : n - 1)];
     }
   };
-  prefetch(lane);
-  for (int64_t w0 = 0; w0 < n; w0 += 64) {
-    const int64_t p = w0 + lane;
-    const bool valid = p < n;
-    uint64_t in_mask = 0;
-    bool rej = false;
-    double limSq = 1e308;
-    const int cnt = ncnt;
-    const double sa = nsadd;
-    const uint32_t row[16] = {nr0.x, nr0.y, nr0.z, nr0.w, nr1.x, nr1.y, nr1.z, nr1.w,
-                              nr2.x, nr2.y, nr2.z, nr2.w, nr3.x, nr3.y, nr3.z, nr3.w};
-    prefetch(p + 64);
-    if (valid) {
-      if (!avg) {
-        double cd = __builtin_sqrt(sa);
-        double lim = cd - g.spacing;
-        limSq = lim * lim;
-      }
-      if (cnt != kNbrOverflow && cnt != kNbrPooled) {
-        // every entry tested (no early exit), so that the accepted-bit loads
-        // are issued together; a rejected candidate's in-window mask is not
-        // used
-        uint32_t hit = 0;
 #pragma unroll
-        for (int i = 0; i < kMaxNbr; i++) {
-          const uint32_t q = row[i];
-          const bool use = i < cnt;
-          if (use && (int64_t)q >= w0) in_mask |= 1ull << (q - w0);
-          const bool earlier = use && (int64_t)q < w0;
-          const uint32_t word = earlier ? bits[q >> 5] : 0u;
-          hit |= (word >> (q & 31)) & 1u;
+  for (int s = 0; s < kAccAhead; s++) prefetch(sl[s], 64 * s);
+  bool cap_may_bind = false;
+  for (int64_t wb = 0; wb < n; wb += 64 * kAccAhead) {
+#pragma unroll
+    for (int s = 0; s < kAccAhead; s++) {
+      const int64_t w0 = wb + 64 * s;
+      if (w0 >= n) break;
+      const int64_t p = w0 + lane;
+      const bool valid = p < n;
+      uint64_t in_mask = 0;
+      bool rej = false;
+      double limSq = 1e308;
+      const int cnt = sl[s].cnt;
+      const double sa = sl[s].sadd, capv = sl[s].capv;
+      const uint32_t row[16] = {sl[s].r0.x, sl[s].r0.y, sl[s].r0.z, sl[s].r0.w, sl[s].r1.x, sl[s].r1.y,
+                                sl[s].r1.z, sl[s].r1.w, sl[s].r2.x, sl[s].r2.y, sl[s].r2.z, sl[s].r2.w,
+                                sl[s].r3.x, sl[s].r3.y, sl[s].r3.z, sl[s].r3.w};
+      prefetch(sl[s], w0 + 64 * kAccAhead);
+      if (valid) {
+        if (!avg) {
+          double cd = __builtin_sqrt(sa);
+          double lim = cd - g.spacing;
+          limSq = lim * lim;
         }
-        rej = hit != 0;
-      } else if (cnt == kNbrPooled) {
-        // a long list: four entries in flight per step
-        const uint32_t* lst = P.nbr_pool + ((uint64_t)row[0] | ((uint64_t)row[1] << 32));
-        const int len = (int)row[2];
-        for (int i0 = 0; i0 < len && !rej; i0 += 4) {
-          uint32_t q4[4];
+        if (cnt != kNbrOverflow && cnt != kNbrPooled) {
+          // every entry tested (no early exit), so that the accepted-bit loads
+          // are issued together; a rejected candidate's in-window mask is not
+          // used
+          uint32_t hit = 0;
 #pragma unroll
-          for (int u = 0; u < 4; u++) q4[u] = i0 + u < len ? lst[i0 + u] : 0xFFFFFFFFu;
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            const uint32_t q = q4[u];
-            if (q == 0xFFFFFFFFu || rej) continue;
-            if ((int64_t)q >= w0) {
-              in_mask |= 1ull << (q - w0);
-            } else if ((bits[q >> 5] >> (q & 31)) & 1u) {
-              rej = true;
-            }
+          for (int i = 0; i < kMaxNbr; i++) {
+            const uint32_t q = row[i];
+            const bool use = i < cnt;
+            if (use && (int64_t)q >= w0) in_mask |= 1ull << (q - w0);
+            const bool earlier = use && (int64_t)q < w0;
+            const uint32_t word = earlier ? bits[q >> 5] : 0u;
+            hit |= (word >> (q & 31)) & 1u;
           }
-        }
-      } else {
-        const uint32_t j = L.idxA[p];
-        const int4 q = L.xyz[j];
-        const int32_t X = q.x, Y = q.y, Z = q.z;
-        const double x = coord(X, P.scale[0], P.offset[0]), y = coord(Y, P.scale[1], P.offset[1]),
-                     z = coord(Z, P.scale[2], P.offset[2]);
-        int lo16[3], hi16[3];
-        if (avg) range16(g, x, y, z, lo16, hi16);
-        const CellBox cb = cell_box(g, x, y, z);
-        for (int cz = cb.lz; cz <= cb.hz && !rej; cz++)
-          for (int cy = cb.ly; cy <= cb.hy && !rej; cy++)
-            for (int cx = cb.lx; cx <= cb.hx && !rej; cx++) {
-              const uint32_t slot = table_find(L, gkey(cx, cy, cz));
-              if (slot == kNil) continue;
-              const uint4 t = L.htab[slot];
-              for (uint32_t f = t.y; f < t.z; f++) {
-                const int4 o = L.ent[f];
-                const uint32_t q = (uint32_t)o.w;
-                if ((int64_t)q >= p) break;
-                if ((int64_t)q < w0 && !((bits[q >> 5] >> (q & 31)) & 1u)) continue;
-                if (!avg) {
-                  if (L.eAdd[f] < limSq) continue;
-                } else {
-                  if (!in_range16(L.eC16[f], lo16, hi16)) continue;
-                }
-                if (!within_xyz(o.x, o.y, o.z, X, Y, Z, P, F)) continue;
-                if ((int64_t)q < w0) { rej = true; break; }
+          rej = hit != 0;
+        } else if (cnt == kNbrPooled) {
+          // a long list: four entries in flight per step
+          const uint32_t* lst = P.nbr_pool + ((uint64_t)row[0] | ((uint64_t)row[1] << 32));
+          const int len = (int)row[2];
+          for (int i0 = 0; i0 < len && !rej; i0 += 4) {
+            uint32_t q4[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) q4[u] = i0 + u < len ? lst[i0 + u] : 0xFFFFFFFFu;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const uint32_t q = q4[u];
+              if (q == 0xFFFFFFFFu || rej) continue;
+              if ((int64_t)q >= w0) {
                 in_mask |= 1ull << (q - w0);
+              } else if ((bits[q >> 5] >> (q & 31)) & 1u) {
+                rej = true;
               }
             }
-      }
-    }
-    // in-window dependencies: the lowest undecided lane always resolves
-    bool decided = !valid || rej;
-    bool accepted = false;
-    uint64_t dec = __ballot(decided), accm = 0;
-    while (dec != ~0ull) {
-      if (!decided) {
-        if (in_mask & accm) {
-          decided = true;
-        } else if ((in_mask & ~dec) == 0) {
-          decided = true;
-          accepted = true;
+          }
+        } else {
+          const uint32_t j = L.idxA[p];
+          const int4 q = L.xyz[j];
+          const int32_t X = q.x, Y = q.y, Z = q.z;
+          const double x = coord(X, P.scale[0], P.offset[0]), y = coord(Y, P.scale[1], P.offset[1]),
+                       z = coord(Z, P.scale[2], P.offset[2]);
+          int lo16[3], hi16[3];
+          if (avg) range16(g, x, y, z, lo16, hi16);
+          const CellBox cb = cell_box(g, x, y, z);
+          for (int cz = cb.lz; cz <= cb.hz && !rej; cz++)
+            for (int cy = cb.ly; cy <= cb.hy && !rej; cy++)
+              for (int cx = cb.lx; cx <= cb.hx && !rej; cx++) {
+                const uint32_t slot = table_find(L, gkey(cx, cy, cz));
+                if (slot == kNil) continue;
+                const uint4 t = L.htab[slot];
+                for (uint32_t f = t.y; f < t.z; f++) {
+                  const int4 o = L.ent[f];
+                  const uint32_t q = (uint32_t)o.w;
+                  if ((int64_t)q >= p) break;
+                  if ((int64_t)q < w0 && !((bits[q >> 5] >> (q & 31)) & 1u)) continue;
+                  if (!avg) {
+                    if (L.eAdd[f] < limSq) continue;
+                  } else {
+                    if (!in_range16(L.eC16[f], lo16, hi16)) continue;
+                  }
+                  if (!within_xyz(o.x, o.y, o.z, X, Y, Z, P, F)) continue;
+                  if ((int64_t)q < w0) { rej = true; break; }
+                  in_mask |= 1ull << (q - w0);
+                }
+              }
         }
       }
-      dec = __ballot(decided);
-      accm = __ballot(accepted);
-    }
-    if (valid) L.acc[p] = accepted ? 1 : 0;
-    if (lane == 0) {
-      bits[w0 >> 5] = (uint32_t)accm;
-      if ((w0 >> 5) + 1 < words) bits[(w0 >> 5) + 1] = (uint32_t)(accm >> 32);
-    }
-    if (!lds) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-    if (!avg && w0 + 64 > 10001) {
-      // the 10 001-check window of the reference scan may bind when more
-      // than 10 000 candidates lie in [min limSq of the window, add)
-      double ml = limSq;
-#pragma unroll
-      for (int d = 32; d >= 1; d >>= 1) {
-        double o = __shfl_xor(ml, d);
-        ml = o < ml ? o : ml;
+      // in-window dependencies: the lowest undecided lane always resolves
+      bool decided = !valid || rej;
+      bool accepted = false;
+      uint64_t dec = __ballot(decided), accm = 0;
+      while (dec != ~0ull) {
+        if (!decided) {
+          if (in_mask & accm) {
+            decided = true;
+          } else if ((in_mask & ~dec) == 0) {
+            decided = true;
+            accepted = true;
+          }
+        }
+        dec = __ballot(decided);
+        accm = __ballot(accepted);
       }
-      // the first sadd >= ml lies more than 10 000 positions before hi - 1
-      // exactly when sadd[hi - 10 002] >= ml (sadd ascending): one load
-      const int64_t hi = (w0 + 64 < n ? w0 + 64 : n);
-      const int64_t m = hi - 10001;
-      if (m >= 1 && lane == 0 && !(L.sadd[m - 1] < ml)) atomicOr(&work[k].flags, (uint32_t)NW_SEQ_ACCEPT);
+      if (lane == 0) {
+        bits[w0 >> 5] = (uint32_t)accm;
+        if ((w0 >> 5) + 1 < words) bits[(w0 >> 5) + 1] = (uint32_t)(accm >> 32);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      if (!avg && w0 + 64 > 10001) {
+        // the 10 001-check window of the reference scan may bind when more
+        // than 10 000 candidates lie in [min limSq of the window, add)
+        double ml = limSq;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+          double o = __shfl_xor(ml, d);
+          ml = o < ml ? o : ml;
+        }
+        // the first sadd >= ml lies more than 10 000 positions before hi - 1
+        // exactly when sadd[hi - 10 002] >= ml (sadd ascending): one value,
+        // loaded with the window when several windows are in flight
+        const int64_t hi = (w0 + 64 < n ? w0 + 64 : n);
+        const int64_t m = hi - 10001;
+        if (m >= 1 && lane == 0 && !((kAccAhead > 1 ? capv : gSadd[m - 1]) < ml)) cap_may_bind = true;
+      }
     }
   }
-  flush_filter(F, P);
+  // no store in the window loop (a store there would count among the loads
+  // that the slot waits see): the cap flag and the accepted bytes afterwards
+  if (cap_may_bind) atomicOr(&work[k].flags, (uint32_t)NW_SEQ_ACCEPT);
+  for (int64_t i = lane; i < n; i += 64) gAcc[i] = (uint8_t)((bits[i >> 5] >> (i & 31)) & 1u);
 }
 
 // one wavefront per node (accepted bits past kAccWords * 32 candidates in
 // global memory: a kernel holding a large LDS share would wait for CUs that
 // the other sampling streams keep busy)
+// kAccAhead windows of neighbour rows in flight: 3 for launches of few
+// nodes (each wavefront alone on its SIMD, bound by latency), 1 for wide
+// launches (more wavefronts per SIMD hide the latency instead)
+template <int kAccAhead>
 __global__ __launch_bounds__(kBlock) void k_accept(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
                                                     NodeWork* __restrict__ work, int32_t num_nodes, SampleParams P) {
   __shared__ uint32_t s_bits[kWaves][kAccWords];
   const int wave = threadIdx.x >> 6;
   const int k = __builtin_amdgcn_readfirstlane(blockIdx.x * kWaves + wave);
   if (k >= num_nodes) return;
-  accept_node(nodes, infos, work, k, P, s_bits[wave], kAccWords);
+  accept_node<kAccAhead>(nodes, infos, work, k, P, s_bits[wave], kAccWords);
 }
 
 // ------------------------------------------------------ accepted shell ring
@@ -3121,8 +3197,12 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   hipLaunchKernelGGL(k_neighbors, dim3((unsigned)p.neighbor_blocks), dim3(kBlock), 0, s, d_nodes, d_info, d_work, num_nodes,
                      p.item_start, p);
   PCG_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_accept, dim3((unsigned)((num_nodes + kWaves - 1) / kWaves)), dim3(kBlock), 0, s, d_nodes,
-                     d_info, d_work, num_nodes, p);
+  if (num_nodes <= kAccDeepNodes)
+    hipLaunchKernelGGL(k_accept<3>, dim3((unsigned)((num_nodes + kWaves - 1) / kWaves)), dim3(kBlock), 0, s, d_nodes,
+                       d_info, d_work, num_nodes, p);
+  else
+    hipLaunchKernelGGL(k_accept<1>, dim3((unsigned)((num_nodes + kWaves - 1) / kWaves)), dim3(kBlock), 0, s, d_nodes,
+                       d_info, d_work, num_nodes, p);
   PCG_CHECK(hipGetLastError());
 
   if (p.method == METHOD_POISSON_AVERAGE) {
