@@ -170,6 +170,7 @@ int32_t g_sample_order = 0;
 bool g_sample_graph = false;
 int32_t g_set_policy = 0;
 int32_t g_ring_blocks_per_cu = 3;
+int32_t g_accept_pipe_nodes = 1024;
 int32_t g_copy_blocks = 8192;
 int32_t g_neighbor_blocks = 4096;
 int32_t g_avg_blocks = 2048;
@@ -710,6 +711,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   sp.neighbor_blocks = g_neighbor_blocks;
   sp.avg_blocks = g_avg_blocks;
   sp.bucket_min_nodes = g_bucket_min_nodes;
+  sp.accept_pipe_nodes = g_accept_pipe_nodes;
   // a group's buffers for levels of at most `maxNodes` nodes and `cand`
   // candidates in all
   auto group_params = [&](GroupMem& m, int64_t maxNodes, int64_t cand, SampleParams& p) -> int {
@@ -965,6 +967,10 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   for (int i = 0; i < 8; i++) out.counters[i] = hc[i];
   if (hc[8] != 0) {
     set_error("equal-distance order check failed in " + std::to_string(hc[8]) + " runs");
+    return 1;
+  }
+  if (hc[9] != 0) {
+    set_error("pipelined acceptance gave up waiting in " + std::to_string(hc[9]) + " nodes");
     return 1;
   }
   if (hc[6] != 0) {

@@ -89,13 +89,16 @@ struct SampleParams {
                                  // [6] nodes that ran out of scratch or arena,
                                  // [7] candidates with more than 16 earlier neighbours,
                                  // [8] runs of equal distances whose libstdc++ order failed
-                                 // its window check (an error); 16 entries
+                                 // its window check (an error), [9] pipelined acceptances that
+                                 // gave up waiting (an error); 16 entries
   double random_limit;     // SamplerRandom: 0.7 * sqrt(3), the cell-centre distance bound
   uint32_t* ring_ctl;      // ring work queue: [0] length, [1] next ticket (device)
   int32_t copy_blocks;     // grid of the level-wide record copy
   int32_t neighbor_blocks; // grid of the level-wide neighbour search
   int32_t avg_blocks;      // workgroups of the Poisson-average colour sums per launch
   int32_t bucket_min_nodes;  // launches of at least this many nodes bucket-sort their candidates
+  int32_t accept_pipe_nodes; // launches of at most this many nodes take the pipelined acceptance
+  int32_t pad2;
 };
 
 // Samples every node of `nodes` (all of one level, children already done).

@@ -1585,6 +1585,66 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(const int32_t* __restrict_
 // window; inside the window the dependencies are resolved by ballots.  The
 // accepted flags of the node live in LDS (global memory past 65536
 // candidates).
+// The rare forms of one candidate's test in a window: a neighbour list
+// longer than a row (walked from the level's pool, four entries in flight per
+// step) or one that overflowed the pool (the candidate's cells searched in the
+// cell table).  Sets `rej` when an accepted earlier window's neighbour is
+// found, and the in-window neighbours in `in_mask`.
+template <class BitsPtr>
+__device__ inline void lane_slow(const SampleParams& P, const Layout& L, const NodeGeom& g, Filter& F, bool avg,
+                                 int64_t p, int64_t w0, int cnt, uint32_t r0, uint32_t r1, uint32_t r2, double limSq,
+                                 BitsPtr bits, uint64_t& in_mask, bool& rej) {
+  if (cnt == kNbrPooled) {
+    const uint32_t* lst = P.nbr_pool + ((uint64_t)r0 | ((uint64_t)r1 << 32));
+    const int len = (int)r2;
+    for (int i0 = 0; i0 < len && !rej; i0 += 4) {
+      uint32_t q4[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) q4[u] = i0 + u < len ? lst[i0 + u] : 0xFFFFFFFFu;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const uint32_t q = q4[u];
+        if (q == 0xFFFFFFFFu || rej) continue;
+        if ((int64_t)q >= w0) {
+          in_mask |= 1ull << (q - w0);
+        } else if ((bits[q >> 5] >> (q & 31)) & 1u) {
+          rej = true;
+        }
+      }
+    }
+    return;
+  }
+  const uint32_t j = L.idxA[p];
+  const int4 q = L.xyz[j];
+  const int32_t X = q.x, Y = q.y, Z = q.z;
+  const double x = coord(X, P.scale[0], P.offset[0]), y = coord(Y, P.scale[1], P.offset[1]),
+               z = coord(Z, P.scale[2], P.offset[2]);
+  int lo16[3], hi16[3];
+  if (avg) range16(g, x, y, z, lo16, hi16);
+  const CellBox cb = cell_box(g, x, y, z);
+  for (int cz = cb.lz; cz <= cb.hz && !rej; cz++)
+    for (int cy = cb.ly; cy <= cb.hy && !rej; cy++)
+      for (int cx = cb.lx; cx <= cb.hx && !rej; cx++) {
+        const uint32_t slot = table_find(L, gkey(cx, cy, cz));
+        if (slot == kNil) continue;
+        const uint4 t = L.htab[slot];
+        for (uint32_t f = t.y; f < t.z; f++) {
+          const int4 o = L.ent[f];
+          const uint32_t qp = (uint32_t)o.w;
+          if ((int64_t)qp >= p) break;
+          if ((int64_t)qp < w0 && !((bits[qp >> 5] >> (qp & 31)) & 1u)) continue;
+          if (!avg) {
+            if (L.eAdd[f] < limSq) continue;
+          } else {
+            if (!in_range16(L.eC16[f], lo16, hi16)) continue;
+          }
+          if (!within_xyz(o.x, o.y, o.z, X, Y, Z, P, F)) continue;
+          if ((int64_t)qp < w0) { rej = true; break; }
+          in_mask |= 1ull << (qp - w0);
+        }
+      }
+}
+
 template <int kAccAhead, class BitsPtr>
 __device__ void accept_windows(NodeWork* __restrict__ work, int k, const SampleParams& P, const Layout& L,
                                const NodeGeom& g, Filter& F, const int64_t n, BitsPtr bits);
@@ -1702,55 +1762,8 @@ __device__ void accept_windows(NodeWork* __restrict__ work, int k, const SampleP
             hit |= (word >> (q & 31)) & 1u;
           }
           rej = hit != 0;
-        } else if (cnt == kNbrPooled) {
-          // a long list: four entries in flight per step
-          const uint32_t* lst = P.nbr_pool + ((uint64_t)row[0] | ((uint64_t)row[1] << 32));
-          const int len = (int)row[2];
-          for (int i0 = 0; i0 < len && !rej; i0 += 4) {
-            uint32_t q4[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) q4[u] = i0 + u < len ? lst[i0 + u] : 0xFFFFFFFFu;
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const uint32_t q = q4[u];
-              if (q == 0xFFFFFFFFu || rej) continue;
-              if ((int64_t)q >= w0) {
-                in_mask |= 1ull << (q - w0);
-              } else if ((bits[q >> 5] >> (q & 31)) & 1u) {
-                rej = true;
-              }
-            }
-          }
         } else {
-          const uint32_t j = L.idxA[p];
-          const int4 q = L.xyz[j];
-          const int32_t X = q.x, Y = q.y, Z = q.z;
-          const double x = coord(X, P.scale[0], P.offset[0]), y = coord(Y, P.scale[1], P.offset[1]),
-                       z = coord(Z, P.scale[2], P.offset[2]);
-          int lo16[3], hi16[3];
-          if (avg) range16(g, x, y, z, lo16, hi16);
-          const CellBox cb = cell_box(g, x, y, z);
-          for (int cz = cb.lz; cz <= cb.hz && !rej; cz++)
-            for (int cy = cb.ly; cy <= cb.hy && !rej; cy++)
-              for (int cx = cb.lx; cx <= cb.hx && !rej; cx++) {
-                const uint32_t slot = table_find(L, gkey(cx, cy, cz));
-                if (slot == kNil) continue;
-                const uint4 t = L.htab[slot];
-                for (uint32_t f = t.y; f < t.z; f++) {
-                  const int4 o = L.ent[f];
-                  const uint32_t q = (uint32_t)o.w;
-                  if ((int64_t)q >= p) break;
-                  if ((int64_t)q < w0 && !((bits[q >> 5] >> (q & 31)) & 1u)) continue;
-                  if (!avg) {
-                    if (L.eAdd[f] < limSq) continue;
-                  } else {
-                    if (!in_range16(L.eC16[f], lo16, hi16)) continue;
-                  }
-                  if (!within_xyz(o.x, o.y, o.z, X, Y, Z, P, F)) continue;
-                  if ((int64_t)q < w0) { rej = true; break; }
-                  in_mask |= 1ull << (q - w0);
-                }
-              }
+          lane_slow(P, L, g, F, avg, p, w0, cnt, row[0], row[1], row[2], limSq, bits, in_mask, rej);
         }
       }
       // in-window dependencies: the lowest undecided lane always resolves
@@ -1801,6 +1814,252 @@ __device__ void accept_windows(NodeWork* __restrict__ work, int k, const SampleP
 // one wavefront per node (accepted bits past kAccWords * 32 candidates in
 // global memory: a kernel holding a large LDS share would wait for CUs that
 // the other sampling streams keep busy)
+// ------------------------------------------------- pipelined acceptance
+// In a launch of few nodes each k_accept wavefront sits alone on its SIMD and
+// its window loop is bound by the latency of each step: the neighbour rows'
+// loads, the masks built from them and the accepted-bit lookups.  Here one
+// workgroup takes a node: kPipeProducers producer wavefronts prepare the
+// windows ahead into a ring of LDS slots, and one consumer wavefront decides
+// the windows in order.  For each candidate a producer writes its in-window
+// neighbours as a mask and its earlier neighbours as a list, except those in
+// windows the consumer has already decided, which it looks up itself: an
+// accepted one rejects the candidate at once.  The consumer looks up the
+// rest, resolves the window by ballots (as accept_windows) and publishes the
+// window's accepted bits.  The 10 001-check test needs no decision and is
+// made by the producers.  Lists kept in the pool and overflowed lists are
+// walked by the consumer (lane_slow).  Nodes above kAccWords * 32 candidates
+// keep their accepted bits in global memory and take accept_windows.
+constexpr int kPipeProducers = 7;
+constexpr int kPipeSlots = 16;
+constexpr uint32_t kPipeRejected = 0x100u;  // slot kind: rejected by a decided window
+constexpr uint32_t kPipeSpin = 1u << 22;   // bound of every wait (an error when reached)
+// Empty entry of a slot's list.  Position 65 535 is never an earlier
+// neighbour (a pipelined node has at most 65 536 candidates), and its bit is
+// zero whenever it is looked up: every word of the bits is cleared, and a
+// node that has the position decides it in its last window, after every
+// lookup.
+constexpr uint32_t kPipeNone = 0xFFFFu;
+// A slot holds, per lane, the in-window mask and the kind of its list in one
+// 16-byte word, and the earlier neighbours left to look up as 16 16-bit
+// positions (kPipeNone where there is none) in two: the consumer reads a
+// lane's whole slot with three 16-byte LDS reads, then looks up all 16 bits
+// at once.
+struct PipeLds {
+  uint4 head[kPipeSlots][64];      // mask low, mask high, kind (0, kPipeRejected, kNbrPooled, kNbrOverflow), 0
+  uint4 early[kPipeSlots][64][2];  // 16 positions; a pooled list's row words 0-2 in the first three dwords
+  uint32_t ready[kPipeSlots];      // window index + 1 of the slot's contents
+  uint32_t consumed;              // windows decided
+  uint32_t abort;
+  uint32_t bits[kAccWords];
+};
+
+__device__ inline uint32_t lds_acquire(uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline void lds_release(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Waits until *p >= want (wave-uniform); false when the workgroup gave up.
+__device__ inline bool pipe_wait(PipeLds& S, uint32_t* p, uint32_t want) {
+  for (uint32_t spin = 0;; spin++) {
+    if (lds_acquire(p) >= want) return true;
+    if (__hip_atomic_load(&S.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return false;
+    if (spin >= kPipeSpin) {
+      __hip_atomic_store(&S.abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+__device__ void pipe_produce(NodeWork* __restrict__ work, int k, const SampleParams& P, const Layout& L,
+                             const NodeGeom& g, const int64_t n, PipeLds& S, const int pw) {
+  const int lane = threadIdx.x & 63;
+  const bool avg = P.method == METHOD_POISSON_AVERAGE;
+  const gptr<const uint8_t> gCnt = gbl((const uint8_t*)L.nbrCount);
+  const gptr<const double> gSadd = gbl((const double*)L.sadd);
+  const gptr<const uint4> gNbr = gbl((const uint4*)L.nbr);
+  const int64_t nw = (n + 63) / 64;
+  bool cap_may_bind = false;
+  for (int64_t j = pw; j < nw; j += kPipeProducers) {
+    const int sl = (int)(j % kPipeSlots);
+    const int64_t w0 = 64 * j;
+    const int64_t p = w0 + lane;
+    const bool valid = p < n;
+    const int64_t pc = valid ? p : n - 1;
+    const int cnt = gCnt[pc];
+    const double sa = gSadd[pc];
+    const gptr<const uint4> rw = gNbr + (uint64_t)pc * (kMaxNbr / 4);
+    const uint4 r0 = ld_g16(rw), r1 = ld_g16(rw + 1), r2 = ld_g16(rw + 2), r3 = ld_g16(rw + 3);
+    if (!avg && w0 + 64 > 10001) {
+      // the 10 001-check window of the reference scan may bind (the test of
+      // accept_windows)
+      double limSq = 1e308;
+      if (valid) {
+        const double lim = __builtin_sqrt(sa) - g.spacing;
+        limSq = lim * lim;
+      }
+      double ml = limSq;
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) {
+        const double o = __shfl_xor(ml, d);
+        ml = o < ml ? o : ml;
+      }
+      const int64_t hi = (w0 + 64 < n ? w0 + 64 : n);
+      const int64_t m = hi - 10001;
+      if (m >= 1 && lane == 0 && !(gSadd[m - 1] < ml)) cap_may_bind = true;
+    }
+    // the slot is free once the window kPipeSlots before is decided; the
+    // windows before that one are decided too, and their bits are final
+    const int64_t freeAt = j - kPipeSlots + 1;
+    if (!pipe_wait(S, &S.consumed, freeAt > 0 ? (uint32_t)freeAt : 0u)) break;
+    const int64_t decidedEnd = 64 * (int64_t)lds_acquire(&S.consumed);  // candidates of decided windows
+    uint32_t info = 0;
+    uint64_t in_mask = 0;
+    uint32_t e[8] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu,
+                     0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    if (valid) {
+      if (cnt != kNbrOverflow && cnt != kNbrPooled) {
+        const uint32_t row[16] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w,
+                                  r2.x, r2.y, r2.z, r2.w, r3.x, r3.y, r3.z, r3.w};
+        uint32_t hit = 0;
+#pragma unroll
+        for (int i = 0; i < kMaxNbr; i++) {
+          const uint32_t q = row[i];
+          const bool use = i < cnt;
+          if (use && (int64_t)q >= w0) in_mask |= 1ull << (q - w0);
+          const bool old = use && (int64_t)q < decidedEnd;
+          // every lookup is made (kPipeNone's bit is zero) so that none
+          // waits alone in a branch
+          const uint32_t qo = old ? q : kPipeNone;
+          hit |= (S.bits[qo >> 5] >> (qo & 31)) & 1u;
+          const uint32_t keep = (use && !old && (int64_t)q < w0) ? q : kPipeNone;
+          e[i >> 1] = (i & 1) ? ((e[i >> 1] & 0xFFFFu) | (keep << 16)) : ((e[i >> 1] & 0xFFFF0000u) | keep);
+        }
+        info = hit ? kPipeRejected : 0u;
+      } else {
+        info = (uint32_t)cnt;
+        e[0] = r0.x;
+        e[1] = r0.y;
+        e[2] = r0.z;
+      }
+    }
+    S.head[sl][lane] = make_uint4((uint32_t)in_mask, (uint32_t)(in_mask >> 32), info, 0u);
+    S.early[sl][lane][0] = make_uint4(e[0], e[1], e[2], e[3]);
+    S.early[sl][lane][1] = make_uint4(e[4], e[5], e[6], e[7]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) lds_release(&S.ready[sl], (uint32_t)(j + 1));
+  }
+  if (cap_may_bind) atomicOr(&work[k].flags, (uint32_t)NW_SEQ_ACCEPT);
+}
+
+__device__ void pipe_consume(const SampleParams& P, const Layout& L, const NodeGeom& g, Filter& F,
+                             const int64_t n, PipeLds& S) {
+  const int lane = threadIdx.x & 63;
+  const bool avg = P.method == METHOD_POISSON_AVERAGE;
+  const int64_t nw = (n + 63) / 64;
+  uint32_t* bits = S.bits;
+  for (int64_t j = 0; j < nw; j++) {
+    const int sl = (int)(j % kPipeSlots);
+    if (!pipe_wait(S, &S.ready[sl], (uint32_t)(j + 1))) break;
+    const int64_t w0 = 64 * j;
+    const int64_t p = w0 + lane;
+    const bool valid = p < n;
+    const uint4 h = S.head[sl][lane];
+    const uint4 ea = S.early[sl][lane][0], eb = S.early[sl][lane][1];
+    const uint32_t info = h.z;
+    uint64_t in_mask = ((uint64_t)h.y << 32) | h.x;
+    bool rej = false;
+    if (valid) {
+      if (info == kPipeRejected) {
+        rej = true;
+      } else if (info == 0u) {
+        // all 16 lookups issued together: an empty entry looks up
+        // kPipeNone's bit, which is zero
+        const uint32_t ew[8] = {ea.x, ea.y, ea.z, ea.w, eb.x, eb.y, eb.z, eb.w};
+        uint32_t hit = 0;
+#pragma unroll
+        for (int i = 0; i < kMaxNbr; i++) {
+          const uint32_t q = (i & 1) ? (ew[i >> 1] >> 16) : (ew[i >> 1] & 0xFFFFu);
+          hit |= (bits[q >> 5] >> (q & 31)) & 1u;
+        }
+        rej = hit != 0;
+      } else {
+        const uint32_t a = ea.x, b = ea.y, c = ea.z;
+        double limSq = 1e308;
+        if (!avg) {
+          const double lim = __builtin_sqrt(L.sadd[p]) - g.spacing;
+          limSq = lim * lim;
+        }
+        lane_slow(P, L, g, F, avg, p, w0, (int)info, a, b, c, limSq, bits, in_mask, rej);
+      }
+    }
+    // in-window dependencies: the lowest undecided lane always resolves
+    bool decided = !valid || rej;
+    bool accepted = false;
+    uint64_t dec = __ballot(decided), accm = 0;
+    while (dec != ~0ull) {
+      if (!decided) {
+        if (in_mask & accm) {
+          decided = true;
+        } else if ((in_mask & ~dec) == 0) {
+          decided = true;
+          accepted = true;
+        }
+      }
+      dec = __ballot(decided);
+      accm = __ballot(accepted);
+    }
+    if (lane == 0) {
+      bits[2 * j] = (uint32_t)accm;
+      bits[2 * j + 1] = (uint32_t)(accm >> 32);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) lds_release(&S.consumed, (uint32_t)(j + 1));
+  }
+}
+
+__global__ __launch_bounds__(64 * (1 + kPipeProducers)) void k_accept_pipe(const int32_t* __restrict__ nodes,
+                                                                          const NodeInfo* __restrict__ infos,
+                                                                          NodeWork* __restrict__ work,
+                                                                          SampleParams P) {
+  __shared__ PipeLds S;
+  const int k = blockIdx.x;
+  const NodeWork w = work[k];
+  if ((w.flags & NW_OVERFLOW) || !(w.flags & NW_CELL_PATH)) return;
+  const int64_t n = w.n;
+  const Layout L = make_layout(w.scratch, n, P.method);
+  const NodeGeom g = node_geom(infos[nodes[k]], P);
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  if (n > (int64_t)kAccWords * 32) {
+    if (wave == 0) {
+      Filter F = make_filter(g, P);
+      accept_windows<3>(work, k, P, L, g, F, n, gbl(L.keyA));
+      flush_filter(F, P);
+    }
+    return;
+  }
+  // every word, also past the node's: kPipeNone's bit must read zero
+  for (int i = threadIdx.x; i < kAccWords; i += blockDim.x) S.bits[i] = 0;
+  if (threadIdx.x < kPipeSlots) S.ready[threadIdx.x] = 0;
+  if (threadIdx.x == 0) { S.consumed = 0; S.abort = 0; }
+  __syncthreads();
+  Filter F = make_filter(g, P);
+  if (wave == 0) pipe_consume(P, L, g, F, n, S);
+  else pipe_produce(work, k, P, L, g, n, S, wave - 1);
+  __syncthreads();
+  if (S.abort) {
+    if (threadIdx.x == 0) atomicAdd(&P.counters[9], 1ull);
+    return;
+  }
+  const gptr<uint8_t> gAcc = gbl(L.acc);
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) gAcc[i] = (uint8_t)((S.bits[i >> 5] >> (i & 31)) & 1u);
+  if (wave == 0) flush_filter(F, P);
+}
+
 // kAccAhead windows of neighbour rows in flight: 3 for launches of few
 // nodes (each wavefront alone on its SIMD, bound by latency), 1 for wide
 // launches (more wavefronts per SIMD hide the latency instead)
@@ -3197,7 +3456,9 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   hipLaunchKernelGGL(k_neighbors, dim3((unsigned)p.neighbor_blocks), dim3(kBlock), 0, s, d_nodes, d_info, d_work, num_nodes,
                      p.item_start, p);
   PCG_CHECK(hipGetLastError());
-  if (num_nodes <= kAccDeepNodes)
+  if (num_nodes <= p.accept_pipe_nodes)
+    hipLaunchKernelGGL(k_accept_pipe, grid, dim3(64 * (1 + kPipeProducers)), 0, s, d_nodes, d_info, d_work, p);
+  else if (num_nodes <= kAccDeepNodes)
     hipLaunchKernelGGL(k_accept<3>, dim3((unsigned)((num_nodes + kWaves - 1) / kWaves)), dim3(kBlock), 0, s, d_nodes,
                        d_info, d_work, num_nodes, p);
   else
