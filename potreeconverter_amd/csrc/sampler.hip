@@ -1827,30 +1827,47 @@ __device__ void accept_windows(NodeWork* __restrict__ work, int k, const SampleP
 // rest, resolves the window by ballots (as accept_windows) and publishes the
 // window's accepted bits.  The 10 001-check test needs no decision and is
 // made by the producers.  Lists kept in the pool and overflowed lists are
-// walked by the consumer (lane_slow).  Nodes above kAccWords * 32 candidates
-// keep their accepted bits in global memory and take accept_windows.
+// walked by the consumer (lane_slow).  Two forms: 16-bit positions and 16
+// slots for nodes of at most 65 536 candidates, 32-bit positions and 8 slots
+// (and 32 KB of bits) up to 262 144; larger nodes keep their accepted bits in
+// global memory and take accept_windows.
 constexpr int kPipeProducers = 7;
-constexpr int kPipeSlots = 16;
 constexpr uint32_t kPipeRejected = 0x100u;  // slot kind: rejected by a decided window
+constexpr uint32_t kPipeMask = 0x200u;      // slot kind: earlier neighbours as a mask
+constexpr int kPipeSpan = 512;              // positions a mask covers: the window's 512 before it
 constexpr uint32_t kPipeSpin = 1u << 22;   // bound of every wait (an error when reached)
-// Empty entry of a slot's list.  Position 65 535 is never an earlier
-// neighbour (a pipelined node has at most 65 536 candidates), and its bit is
-// zero whenever it is looked up: every word of the bits is cleared, and a
-// node that has the position decides it in its last window, after every
-// lookup.
-constexpr uint32_t kPipeNone = 0xFFFFu;
+template <bool kWide>
+struct PipeCfg {
+  // the windows not yet decided when a slot is filled lie within the
+  // kPipeSpan positions before it
+  static constexpr int kSlots = 8;
+  static_assert(64 * (kSlots - 1) <= kPipeSpan, "a mask covers every window not yet decided");
+  static constexpr int kWords = kWide ? 8192 : 2048;  // LDS bits: the most candidates / 32
+  static constexpr int kVecs = kWide ? 4 : 2;         // 16-byte words of a lane's 16 positions
+  // Empty entry of a slot's list: the last position of the bits.  It is
+  // never an earlier neighbour (a node has at most kWords * 32 candidates),
+  // and its bit is zero whenever it is looked up: every word of the bits is
+  // cleared, and a node that has the position decides it in its last
+  // window, after every lookup.
+  static constexpr uint32_t kNone = (uint32_t)kWords * 32u - 1u;
+};
 // A slot holds, per lane, the in-window mask and the kind of its list in one
-// 16-byte word, and the earlier neighbours left to look up as 16 16-bit
-// positions (kPipeNone where there is none) in two: the consumer reads a
-// lane's whole slot with three 16-byte LDS reads, then looks up all 16 bits
-// at once.
+// 16-byte word, and the earlier neighbours left to look up as 16 positions
+// (kNone where there is none): the consumer reads a lane's whole slot with
+// three or five 16-byte LDS reads, then looks up all 16 bits at once.
+template <bool kWide>
 struct PipeLds {
-  uint4 head[kPipeSlots][64];      // mask low, mask high, kind (0, kPipeRejected, kNbrPooled, kNbrOverflow), 0
-  uint4 early[kPipeSlots][64][2];  // 16 positions; a pooled list's row words 0-2 in the first three dwords
-  uint32_t ready[kPipeSlots];      // window index + 1 of the slot's contents
-  uint32_t consumed;              // windows decided
+  using C = PipeCfg<kWide>;
+  uint4 head[C::kSlots][64];  // mask low, mask high, kind (0, kPipeRejected, kPipeMask, kNbrPooled, kNbrOverflow), 0
+  // 16 positions (kind 0), or for a list longer than 16 entries (dense
+  // clusters) its entries in windows not yet decided as a mask of the
+  // kPipeSpan positions before the window (kPipeMask); an overflowed list's
+  // row words 0-2 in the first three dwords
+  uint4 early[C::kSlots][64][kPipeSpan / 128];
+  uint32_t ready[C::kSlots];             // window index + 1 of the slot's contents
+  uint32_t consumed;                     // windows decided
   uint32_t abort;
-  uint32_t bits[kAccWords];
+  uint32_t bits[C::kWords];
 };
 
 __device__ inline uint32_t lds_acquire(uint32_t* p) {
@@ -1861,20 +1878,34 @@ __device__ inline void lds_release(uint32_t* p, uint32_t v) {
 }
 
 // Waits until *p >= want (wave-uniform); false when the workgroup gave up.
-__device__ inline bool pipe_wait(PipeLds& S, uint32_t* p, uint32_t want) {
+__device__ inline bool pipe_wait(uint32_t* abort, uint32_t* p, uint32_t want) {
   for (uint32_t spin = 0;; spin++) {
     if (lds_acquire(p) >= want) return true;
-    if (__hip_atomic_load(&S.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return false;
+    if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return false;
     if (spin >= kPipeSpin) {
-      __hip_atomic_store(&S.abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       return false;
     }
     __builtin_amdgcn_s_sleep(1);
   }
 }
 
+// entry i of a lane's 16 positions in its dwords
+template <bool kWide>
+__device__ inline uint32_t pipe_entry(const uint32_t* e, int i) {
+  if constexpr (kWide) return e[i];
+  else return (i & 1) ? (e[i >> 1] >> 16) : (e[i >> 1] & 0xFFFFu);
+}
+template <bool kWide>
+__device__ inline void pipe_set_entry(uint32_t* e, int i, uint32_t v) {
+  if constexpr (kWide) e[i] = v;
+  else e[i >> 1] = (i & 1) ? ((e[i >> 1] & 0xFFFFu) | (v << 16)) : ((e[i >> 1] & 0xFFFF0000u) | v);
+}
+
+template <bool kWide>
 __device__ void pipe_produce(NodeWork* __restrict__ work, int k, const SampleParams& P, const Layout& L,
-                             const NodeGeom& g, const int64_t n, PipeLds& S, const int pw) {
+                             const NodeGeom& g, const int64_t n, PipeLds<kWide>& S, const int pw) {
+  using C = PipeCfg<kWide>;
   const int lane = threadIdx.x & 63;
   const bool avg = P.method == METHOD_POISSON_AVERAGE;
   const gptr<const uint8_t> gCnt = gbl((const uint8_t*)L.nbrCount);
@@ -1883,7 +1914,7 @@ __device__ void pipe_produce(NodeWork* __restrict__ work, int k, const SamplePar
   const int64_t nw = (n + 63) / 64;
   bool cap_may_bind = false;
   for (int64_t j = pw; j < nw; j += kPipeProducers) {
-    const int sl = (int)(j % kPipeSlots);
+    const int sl = (int)(j % C::kSlots);
     const int64_t w0 = 64 * j;
     const int64_t p = w0 + lane;
     const bool valid = p < n;
@@ -1910,15 +1941,16 @@ __device__ void pipe_produce(NodeWork* __restrict__ work, int k, const SamplePar
       const int64_t m = hi - 10001;
       if (m >= 1 && lane == 0 && !(gSadd[m - 1] < ml)) cap_may_bind = true;
     }
-    // the slot is free once the window kPipeSlots before is decided; the
+    // the slot is free once the window kSlots before is decided; the
     // windows before that one are decided too, and their bits are final
-    const int64_t freeAt = j - kPipeSlots + 1;
-    if (!pipe_wait(S, &S.consumed, freeAt > 0 ? (uint32_t)freeAt : 0u)) break;
+    const int64_t freeAt = j - C::kSlots + 1;
+    if (!pipe_wait(&S.abort, &S.consumed, freeAt > 0 ? (uint32_t)freeAt : 0u)) break;
     const int64_t decidedEnd = 64 * (int64_t)lds_acquire(&S.consumed);  // candidates of decided windows
     uint32_t info = 0;
     uint64_t in_mask = 0;
-    uint32_t e[8] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu,
-                     0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    uint32_t e[4 * C::kVecs];
+#pragma unroll
+    for (int i = 0; i < 4 * C::kVecs; i++) e[i] = kWide ? C::kNone : (C::kNone | (C::kNone << 16));
     if (valid) {
       if (cnt != kNbrOverflow && cnt != kNbrPooled) {
         const uint32_t row[16] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w,
@@ -1930,14 +1962,44 @@ __device__ void pipe_produce(NodeWork* __restrict__ work, int k, const SamplePar
           const bool use = i < cnt;
           if (use && (int64_t)q >= w0) in_mask |= 1ull << (q - w0);
           const bool old = use && (int64_t)q < decidedEnd;
-          // every lookup is made (kPipeNone's bit is zero) so that none
-          // waits alone in a branch
-          const uint32_t qo = old ? q : kPipeNone;
+          // every lookup is made (kNone's bit is zero) so that none waits
+          // alone in a branch
+          const uint32_t qo = old ? q : C::kNone;
           hit |= (S.bits[qo >> 5] >> (qo & 31)) & 1u;
-          const uint32_t keep = (use && !old && (int64_t)q < w0) ? q : kPipeNone;
-          e[i >> 1] = (i & 1) ? ((e[i >> 1] & 0xFFFFu) | (keep << 16)) : ((e[i >> 1] & 0xFFFF0000u) | keep);
+          pipe_set_entry<kWide>(e, i, (use && !old && (int64_t)q < w0) ? q : C::kNone);
         }
         info = hit ? kPipeRejected : 0u;
+      } else if (cnt == kNbrPooled) {
+        // a long list (dense clusters): walked here, four entries in flight
+        // per step, like a row.  Its entries in windows not yet decided go
+        // to the lane's mask (they all lie in the kPipeSpan positions before
+        // the window: the windows before kSlots - 1 ago are decided).
+        const gptr<const uint32_t> lst = gbl((const uint32_t*)P.nbr_pool) + ((uint64_t)r0.x | ((uint64_t)r0.y << 32));
+        const int len = (int)r0.z;
+        const int64_t base = w0 - kPipeSpan > 0 ? w0 - kPipeSpan : 0;
+        uint4* const sp = S.early[sl][lane];
+#pragma unroll
+        for (int v = 0; v < kPipeSpan / 128; v++) sp[v] = make_uint4(0u, 0u, 0u, 0u);
+        uint32_t hit = 0;
+        for (int i0 = 0; i0 < len && !hit; i0 += 4) {
+          uint32_t q4[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) q4[u] = lst[i0 + u < len ? i0 + u : len - 1];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const uint32_t q = q4[u];
+            const bool use = i0 + u < len;
+            if (use && (int64_t)q >= w0) in_mask |= 1ull << (q - w0);
+            const bool old = use && (int64_t)q < decidedEnd;
+            const uint32_t qo = old ? q : C::kNone;
+            hit |= (S.bits[qo >> 5] >> (qo & 31)) & 1u;
+            if (use && !old && (int64_t)q < w0) {
+              const uint32_t r = (uint32_t)(q - base);
+              atomicOr(&((uint32_t*)sp)[r >> 5], 1u << (r & 31));
+            }
+          }
+        }
+        info = hit ? kPipeRejected : kPipeMask;
       } else {
         info = (uint32_t)cnt;
         e[0] = r0.x;
@@ -1946,8 +2008,9 @@ __device__ void pipe_produce(NodeWork* __restrict__ work, int k, const SamplePar
       }
     }
     S.head[sl][lane] = make_uint4((uint32_t)in_mask, (uint32_t)(in_mask >> 32), info, 0u);
-    S.early[sl][lane][0] = make_uint4(e[0], e[1], e[2], e[3]);
-    S.early[sl][lane][1] = make_uint4(e[4], e[5], e[6], e[7]);
+    if (info != kPipeMask)
+#pragma unroll
+      for (int v = 0; v < C::kVecs; v++) S.early[sl][lane][v] = make_uint4(e[4 * v], e[4 * v + 1], e[4 * v + 2], e[4 * v + 3]);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) lds_release(&S.ready[sl], (uint32_t)(j + 1));
@@ -1955,45 +2018,60 @@ __device__ void pipe_produce(NodeWork* __restrict__ work, int k, const SamplePar
   if (cap_may_bind) atomicOr(&work[k].flags, (uint32_t)NW_SEQ_ACCEPT);
 }
 
+template <bool kWide>
 __device__ void pipe_consume(const SampleParams& P, const Layout& L, const NodeGeom& g, Filter& F,
-                             const int64_t n, PipeLds& S) {
+                             const int64_t n, PipeLds<kWide>& S) {
+  using C = PipeCfg<kWide>;
   const int lane = threadIdx.x & 63;
   const bool avg = P.method == METHOD_POISSON_AVERAGE;
   const int64_t nw = (n + 63) / 64;
   uint32_t* bits = S.bits;
   for (int64_t j = 0; j < nw; j++) {
-    const int sl = (int)(j % kPipeSlots);
-    if (!pipe_wait(S, &S.ready[sl], (uint32_t)(j + 1))) break;
+    const int sl = (int)(j % C::kSlots);
+    if (!pipe_wait(&S.abort, &S.ready[sl], (uint32_t)(j + 1))) break;
     const int64_t w0 = 64 * j;
     const int64_t p = w0 + lane;
     const bool valid = p < n;
     const uint4 h = S.head[sl][lane];
-    const uint4 ea = S.early[sl][lane][0], eb = S.early[sl][lane][1];
+    uint32_t ew[4 * C::kVecs];
+#pragma unroll
+    for (int v = 0; v < C::kVecs; v++) {
+      const uint4 t = S.early[sl][lane][v];
+      ew[4 * v] = t.x; ew[4 * v + 1] = t.y; ew[4 * v + 2] = t.z; ew[4 * v + 3] = t.w;
+    }
     const uint32_t info = h.z;
     uint64_t in_mask = ((uint64_t)h.y << 32) | h.x;
     bool rej = false;
     if (valid) {
       if (info == kPipeRejected) {
         rej = true;
+      } else if (info == kPipeMask) {
+        const uint4* sp = S.early[sl][lane];
+        const uint32_t* bw = bits + ((w0 - kPipeSpan > 0 ? w0 - kPipeSpan : 0) >> 5);
+        uint32_t hit = 0;
+#pragma unroll
+        for (int v = 0; v < kPipeSpan / 128; v++) {
+          const uint4 m = sp[v];
+          hit |= (m.x & bw[4 * v]) | (m.y & bw[4 * v + 1]) | (m.z & bw[4 * v + 2]) | (m.w & bw[4 * v + 3]);
+        }
+        rej = hit != 0;
       } else if (info == 0u) {
-        // all 16 lookups issued together: an empty entry looks up
-        // kPipeNone's bit, which is zero
-        const uint32_t ew[8] = {ea.x, ea.y, ea.z, ea.w, eb.x, eb.y, eb.z, eb.w};
+        // all 16 lookups issued together: an empty entry looks up kNone's
+        // bit, which is zero
         uint32_t hit = 0;
 #pragma unroll
         for (int i = 0; i < kMaxNbr; i++) {
-          const uint32_t q = (i & 1) ? (ew[i >> 1] >> 16) : (ew[i >> 1] & 0xFFFFu);
+          const uint32_t q = pipe_entry<kWide>(ew, i);
           hit |= (bits[q >> 5] >> (q & 31)) & 1u;
         }
         rej = hit != 0;
       } else {
-        const uint32_t a = ea.x, b = ea.y, c = ea.z;
         double limSq = 1e308;
         if (!avg) {
           const double lim = __builtin_sqrt(L.sadd[p]) - g.spacing;
           limSq = lim * lim;
         }
-        lane_slow(P, L, g, F, avg, p, w0, (int)info, a, b, c, limSq, bits, in_mask, rej);
+        lane_slow(P, L, g, F, avg, p, w0, (int)info, ew[0], ew[1], ew[2], limSq, bits, in_mask, rej);
       }
     }
     // in-window dependencies: the lowest undecided lane always resolves
@@ -2022,19 +2100,25 @@ __device__ void pipe_consume(const SampleParams& P, const Layout& L, const NodeG
   }
 }
 
+// kWide == false: nodes of at most 65 536 candidates (the others return);
+// kWide: the nodes above that, with accept_windows past 262 144.
+template <bool kWide>
 __global__ __launch_bounds__(64 * (1 + kPipeProducers)) void k_accept_pipe(const int32_t* __restrict__ nodes,
                                                                           const NodeInfo* __restrict__ infos,
                                                                           NodeWork* __restrict__ work,
                                                                           SampleParams P) {
-  __shared__ PipeLds S;
+  using C = PipeCfg<kWide>;
+  __shared__ PipeLds<kWide> S;
   const int k = blockIdx.x;
   const NodeWork w = work[k];
   if ((w.flags & NW_OVERFLOW) || !(w.flags & NW_CELL_PATH)) return;
   const int64_t n = w.n;
+  constexpr int64_t kNarrowMax = (int64_t)PipeCfg<false>::kWords * 32;
+  if (kWide ? n <= kNarrowMax : n > kNarrowMax) return;
   const Layout L = make_layout(w.scratch, n, P.method);
   const NodeGeom g = node_geom(infos[nodes[k]], P);
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  if (n > (int64_t)kAccWords * 32) {
+  if (n > (int64_t)C::kWords * 32) {
     if (wave == 0) {
       Filter F = make_filter(g, P);
       accept_windows<3>(work, k, P, L, g, F, n, gbl(L.keyA));
@@ -2042,14 +2126,14 @@ __global__ __launch_bounds__(64 * (1 + kPipeProducers)) void k_accept_pipe(const
     }
     return;
   }
-  // every word, also past the node's: kPipeNone's bit must read zero
-  for (int i = threadIdx.x; i < kAccWords; i += blockDim.x) S.bits[i] = 0;
-  if (threadIdx.x < kPipeSlots) S.ready[threadIdx.x] = 0;
+  // every word, also past the node's: kNone's bit must read zero
+  for (int i = threadIdx.x; i < C::kWords; i += blockDim.x) S.bits[i] = 0;
+  if (threadIdx.x < C::kSlots) S.ready[threadIdx.x] = 0;
   if (threadIdx.x == 0) { S.consumed = 0; S.abort = 0; }
   __syncthreads();
   Filter F = make_filter(g, P);
-  if (wave == 0) pipe_consume(P, L, g, F, n, S);
-  else pipe_produce(work, k, P, L, g, n, S, wave - 1);
+  if (wave == 0) pipe_consume<kWide>(P, L, g, F, n, S);
+  else pipe_produce<kWide>(work, k, P, L, g, n, S, wave - 1);
   __syncthreads();
   if (S.abort) {
     if (threadIdx.x == 0) atomicAdd(&P.counters[9], 1ull);
@@ -3456,9 +3540,11 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   hipLaunchKernelGGL(k_neighbors, dim3((unsigned)p.neighbor_blocks), dim3(kBlock), 0, s, d_nodes, d_info, d_work, num_nodes,
                      p.item_start, p);
   PCG_CHECK(hipGetLastError());
-  if (num_nodes <= p.accept_pipe_nodes)
-    hipLaunchKernelGGL(k_accept_pipe, grid, dim3(64 * (1 + kPipeProducers)), 0, s, d_nodes, d_info, d_work, p);
-  else if (num_nodes <= kAccDeepNodes)
+  if (num_nodes <= p.accept_pipe_nodes) {
+    hipLaunchKernelGGL(k_accept_pipe<false>, grid, dim3(64 * (1 + kPipeProducers)), 0, s, d_nodes, d_info, d_work, p);
+    PCG_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_accept_pipe<true>, grid, dim3(64 * (1 + kPipeProducers)), 0, s, d_nodes, d_info, d_work, p);
+  } else if (num_nodes <= kAccDeepNodes)
     hipLaunchKernelGGL(k_accept<3>, dim3((unsigned)((num_nodes + kWaves - 1) / kWaves)), dim3(kBlock), 0, s, d_nodes,
                        d_info, d_work, num_nodes, p);
   else
