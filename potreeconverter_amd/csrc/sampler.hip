@@ -987,15 +987,97 @@ __device__ int64_t block_partition(PairArr a, uint32_t* posLe, uint32_t* posGe, 
   return cut;
 }
 
+// Wave-wide unguarded partition of [f+1, l) around the pivot at f: the
+// rule of block_partition (the k-th element >= pivot from the left swaps with
+// the k-th element <= pivot from the right while the former lies left of the
+// latter; the cut is the first left element without a partner or the K-th
+// right element, whichever is first) with one wavefront, 64 elements per
+// step, and no workgroup barrier.  Segments of different wavefronts use
+// disjoint ranges of posLe / posGe.
+__device__ int64_t wave_partition(PairArr a, uint32_t* posLe, uint32_t* posGe, int64_t f, int64_t l) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const uint64_t me = 1ull << lane;
+  const double pv = a.K[f];
+  // four steps' loads in flight per iteration (the loop is bound by their
+  // latency); out-of-range lanes read the pivot
+  constexpr int U = 4;
+  int64_t leT = 0;
+  for (int64_t b = f + 1; b < l; b += 64 * U) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = b + 64 * u + lane;
+      v[u] = a.K[i < l ? i : f];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) leT += __popcll(__ballot(b + 64 * u + lane < l && !(pv < v[u])));
+  }
+  int64_t cG = 0, cL = 0, kc = 0, fail = INT64_MAX;
+  for (int64_t b0 = f + 1; b0 < l; b0 += 64 * U) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = b0 + 64 * u + lane;
+      v[u] = a.K[i < l ? i : f];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t b = b0 + 64 * u;
+      const int64_t i = b + lane;
+      const bool in = i < l;
+      const bool ge = in && !(v[u] < pv), le = in && !(pv < v[u]);
+      const uint64_t bg = __ballot(ge), bl = __ballot(le);
+      if (le) posLe[f + cL + __popcll(bl & lt)] = (uint32_t)(i - f);
+      bool ok = false;
+      if (ge) {
+        const int64_t kk = cG + __popcll(bg & lt) + 1;
+        const int64_t leIncl = cL + __popcll(bl & (lt | me));
+        ok = (leT - leIncl) >= kk;
+        if (ok) posGe[f + kk - 1] = (uint32_t)(i - f);
+      }
+      const uint64_t okm = __ballot(ok), badm = __ballot(ge && !ok);
+      kc += __popcll(okm);
+      // ok only falls with the rank, so the first element without a
+      // partner is the lowest bad lane of the first step that has one
+      if (badm && fail == INT64_MAX) fail = b + __builtin_ctzll(badm);
+      cG += __popcll(bg);
+      cL += __popcll(bl);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (int64_t kk = lane; kk < kc; kk += 64) pair_swap(a, f + posGe[f + kk], f + posLe[f + leT - 1 - kk]);
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (kc == 0) return fail;
+  const int64_t rK = f + posLe[f + leT - kc];
+  return fail < rK ? fail : rK;
+}
+
+// median-of-three pivot of [f, l) moved to f (__move_median_to_first), by
+// the calling lane
+__device__ inline void pair_pivot_to_first(PairArr a, int64_t f, int64_t l) {
+  const int64_t mid = f + (l - f) / 2, x = f + 1, y = mid, z = l - 1;
+  const double kx = a.K[x], ky = a.K[y], kz = a.K[z];
+  int64_t pick;
+  if (kx < ky) pick = (ky < kz) ? y : ((kx < kz) ? z : x);
+  else pick = (kx < kz) ? x : ((ky < kz) ? z : y);
+  pair_swap(a, f, pick);
+}
+
 __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict__ nodes,
                                                         const NodeInfo* __restrict__ infos, NodeWork* __restrict__ work,
                                                         SampleParams P, unsigned long long* counters) {
-  constexpr int kSmall = 128;     // segments up to this size go to one lane each
-  constexpr int kList = 1024;     // segments per level
-  constexpr int kDeferred = 4096; // segments handed to single lanes
+  constexpr int kSmall = 128;      // segments up to this size: staged in LDS, replayed by one lane
+  constexpr int kWaveSeg = 8192;   // segments up to this size: partitioned by one wavefront
+  constexpr int kList = 256;       // larger segments per level (block-wide partitions)
+  constexpr int kWaveList = 1024;  // segments handed to the wavefronts
+  constexpr int kStack = 64;       // frames of a wavefront's partition loop
   __shared__ int32_t s_cf[kList], s_cl[kList], s_cd[kList];
   __shared__ int32_t s_nf[kList], s_nl[kList], s_nd[kList];
-  __shared__ int32_t s_df[kDeferred], s_dl[kDeferred], s_dd[kDeferred];
+  __shared__ int32_t s_df[kWaveList], s_dl[kWaveList], s_dd[kWaveList];
+  __shared__ int32_t s_sf[kWaves][kStack], s_sl[kWaves][kStack], s_sd[kWaves][kStack];
   __shared__ int s_nc, s_nn, s_nd_count, s_bad, s_any;
   __shared__ PartShared sh;
   __shared__ double s_segK[kWaves][kSmall];
@@ -1058,13 +1140,14 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
     int64_t d = 0;
     for (int64_t m = n; m > 1; m >>= 1) d++;
     if (n > 16 && s_any) {
-      if (n > kSmall) { s_cf[0] = 0; s_cl[0] = (int32_t)n; s_cd[0] = (int32_t)(2 * d); s_nc = 1; }
+      if (n > kWaveSeg) { s_cf[0] = 0; s_cl[0] = (int32_t)n; s_cd[0] = (int32_t)(2 * d); s_nc = 1; }
       else { s_df[0] = 0; s_dl[0] = (int32_t)n; s_dd[0] = (int32_t)(2 * d); s_nd_count = 1; }
     }
   }
   __syncthreads();
-  // levels of the recursion: every segment of a level is partitioned by the
-  // whole block; both parts continue with the decremented depth
+  // levels of the recursion above kWaveSeg: every segment of a level is
+  // partitioned by the whole block; both parts continue with the
+  // decremented depth, the parts up to kWaveSeg on the wavefronts below
   for (int level = 0; level < 128 && s_nc > 0 && !s_bad; level++) {
     if (tid == 0) s_nn = 0;
     __syncthreads();
@@ -1074,7 +1157,7 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
       int depth = s_cd[c];
       if (depth == 0) {
         if (tid == 0) {
-          if (s_nd_count < kDeferred) { s_df[s_nd_count] = (int32_t)f; s_dl[s_nd_count] = (int32_t)l; s_dd[s_nd_count] = 0; s_nd_count++; }
+          if (s_nd_count < kWaveList) { s_df[s_nd_count] = (int32_t)f; s_dl[s_nd_count] = (int32_t)l; s_dd[s_nd_count] = 0; s_nd_count++; }
           else s_bad = 1;
         }
         __syncthreads();
@@ -1107,9 +1190,9 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
           for (int side = 0; side < 2; side++) {
             int64_t sz = pl[side] - pf[side];
             if (sz <= 16 || marks[side] == 0) continue;
-            if (sz > kSmall && s_nn < kList) {
+            if (sz > kWaveSeg && s_nn < kList) {
               s_nf[s_nn] = (int32_t)pf[side]; s_nl[s_nn] = (int32_t)pl[side]; s_nd[s_nn] = depth - 1; s_nn++;
-            } else if (sz <= kSmall && s_nd_count < kDeferred) {
+            } else if (sz <= kWaveSeg && s_nd_count < kWaveList) {
               s_df[s_nd_count] = (int32_t)pf[side]; s_dl[s_nd_count] = (int32_t)pl[side]; s_dd[s_nd_count] = depth - 1;
               s_nd_count++;
             } else {
@@ -1139,34 +1222,99 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
     }
     __syncthreads();
   } else {
-    // small segments: the rest of the partition loop, one wave each with the
-    // segment staged in LDS and replayed by one lane (the replay is a chain
-    // of dependent element accesses: LDS latency instead of memory latency);
-    // segments longer than the stage (depth-limit ones) on one lane in place
+    // segments up to kWaveSeg: the rest of the partition loop, one wavefront
+    // each, with its own stack of the parts still to do (the right part is
+    // pushed, the left one continues, as in pair_partition_loop; parts
+    // without a marked element are dropped); parts up to kSmall staged in
+    // LDS and replayed by one lane (the replay is a chain of dependent
+    // element accesses: LDS latency instead of memory latency), parts at
+    // the depth limit heap-sorted by one lane in place
     const int lane = tid & 63, wave = tid >> 6;
-    for (int d = wave; d < s_nd_count; d += kWaves) {
-      const int64_t f = s_df[d], l = s_dl[d];
-      const int len = (int)(l - f);
-      if (len > kSmall) {
-        if (lane == 0) pair_partition_loop(a, f, l, s_dd[d]);
-        continue;
+    bool wbad = false;
+    for (int d = wave; d < s_nd_count && !wbad; d += kWaves) {
+      int sp = 0;
+      s_sf[wave][0] = s_df[d]; s_sl[wave][0] = s_dl[d]; s_sd[wave][0] = s_dd[d];
+      sp = 1;
+      while (sp > 0 && !wbad) {
+        sp--;
+        int64_t f = s_sf[wave][sp], l = s_sl[wave][sp];
+        int depth = s_sd[wave][sp];
+        while (l - f > 16) {
+          if (depth == 0) {
+            if (lane == 0) pair_heap_sort(a, f, l - f);
+            break;
+          }
+          const int len = (int)(l - f);
+          if (len <= kSmall) {
+            for (int i = lane; i < len; i += 64) {
+              s_segK[wave][i] = a.K[f + i];
+              s_segH[wave][i] = a.H[f + i];
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            if (lane == 0) pair_partition_loop(PairArr{s_segK[wave], s_segH[wave]}, 0, len, depth);
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            for (int i = lane; i < len; i += 64) {
+              a.K[f + i] = s_segK[wave][i];
+              a.H[f + i] = s_segH[wave][i];
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            break;
+          }
+          depth--;
+          if (lane == 0) pair_pivot_to_first(a, f, l);
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+          const int64_t cut = wave_partition(a, posLe, posGe, f, l);
+          if (cut <= f || cut >= l) {  // cannot happen for a median-of-three pivot; stay bounded
+            wbad = true;
+            break;
+          }
+          uint64_t mL = 0, mR = 0;
+          for (int64_t b = f; b < l; b += 256) {
+            uint32_t h[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int64_t i = b + 64 * u + lane;
+              h[u] = a.H[i < l ? i : f];
+            }
+            uint8_t mk[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) mk[u] = mark[h[u]];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int64_t i = b + 64 * u + lane;
+              const bool m = i < l && mk[u] != 0;
+              mL += __popcll(__ballot(m && i < cut));
+              mR += __popcll(__ballot(m && i >= cut));
+            }
+          }
+          if (l - cut > 16 && mR) {
+            if (sp >= kStack) { wbad = true; break; }
+            if (lane == 0) { s_sf[wave][sp] = (int32_t)cut; s_sl[wave][sp] = (int32_t)l; s_sd[wave][sp] = depth; }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            sp++;
+          }
+          if (!mL) break;
+          l = cut;
+        }
       }
-      for (int i = lane; i < len; i += 64) {
-        s_segK[wave][i] = a.K[f + i];
-        s_segH[wave][i] = a.H[f + i];
-      }
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-      if (lane == 0) pair_partition_loop(PairArr{s_segK[wave], s_segH[wave]}, 0, len, s_dd[d]);
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-      for (int i = lane; i < len; i += 64) {
-        a.K[f + i] = s_segK[wave][i];
-        a.H[f + i] = s_segH[wave][i];
-      }
-      __builtin_amdgcn_wave_barrier();
     }
+    if (wbad && lane == 0) s_bad = 1;
     __syncthreads();
+    if (s_bad) {
+      // a wavefront's stack or partition failed: replay the whole sort on
+      // one lane (same result)
+      if (tid == 0) {
+        for (int64_t i = 0; i < n; i++) L.idxB[i] = (uint32_t)i;
+        lstd::sort(L.idxB, n, lstd::KeyLess{L.add});
+        for (int64_t i = 0; i < n; i++) L.sadd[i] = L.add[L.idxB[i]];
+      }
+      __syncthreads();
+    }
   }
   // Positions after the partition phase, then each marked run of equal
   // distances [p, e) (radix order in idxA) is put in the order of those
