@@ -97,7 +97,8 @@ struct SampleParams {
   int32_t neighbor_blocks; // grid of the level-wide neighbour search
   int32_t avg_blocks;      // workgroups of the Poisson-average colour sums per launch
   int32_t bucket_min_nodes;  // launches of at least this many nodes bucket-sort their candidates
-  int32_t accept_pipe_nodes; // launches of at most this many nodes take the pipelined acceptance
+  int32_t accept_pipe_nodes; // launches of at most this many nodes, and the ring's launches (Poisson),
+                             // take the pipelined acceptance; 0: never
   int32_t pad2;
 };
 

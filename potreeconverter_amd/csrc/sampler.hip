@@ -3688,7 +3688,11 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   hipLaunchKernelGGL(k_neighbors, dim3((unsigned)p.neighbor_blocks), dim3(kBlock), 0, s, d_nodes, d_info, d_work, num_nodes,
                      p.item_start, p);
   PCG_CHECK(hipGetLastError());
-  if (num_nodes <= p.accept_pipe_nodes) {
+  // pipelined: launches of few nodes, and launches the ring decides (their
+  // cell-path nodes are the few whose shell outgrew the ring, often the
+  // level's largest)
+  const bool ring_level = p.method == METHOD_POISSON && num_nodes >= p.ring_min_nodes;
+  if (p.accept_pipe_nodes > 0 && (num_nodes <= p.accept_pipe_nodes || ring_level)) {
     hipLaunchKernelGGL(k_accept_pipe<false>, grid, dim3(64 * (1 + kPipeProducers)), 0, s, d_nodes, d_info, d_work, p);
     PCG_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_accept_pipe<true>, grid, dim3(64 * (1 + kPipeProducers)), 0, s, d_nodes, d_info, d_work, p);
