@@ -216,7 +216,8 @@ int pcg_set_option(const char* key, int64_t value) {
                           {"neighbor_blocks", &g_neighbor_blocks, 256, 1 << 20},
                           {"avg_blocks", &g_avg_blocks, 64, 1 << 20},
                           {"bucket_min_nodes", &g_bucket_min_nodes, 1, 1 << 30},
-                          {"accept_pipe_nodes", &g_accept_pipe_nodes, 0, 1 << 30}};
+                          {"accept_pipe_nodes", &g_accept_pipe_nodes, 0, 1 << 30},
+                          {"ring_max_cand", &g_ring_max_cand, 64, 1 << 30}};
   for (const Shape& sh : shapes)
     if (k == sh.name) {
       if (value < sh.lo || value > sh.hi) {
@@ -276,6 +277,7 @@ int pcg_get_option(const char* key, int64_t* value) {
       {"ring_blocks_per_cu", g_ring_blocks_per_cu}, {"copy_blocks", g_copy_blocks},
       {"neighbor_blocks", g_neighbor_blocks},       {"avg_blocks", g_avg_blocks},
       {"bucket_min_nodes", g_bucket_min_nodes},     {"accept_pipe_nodes", g_accept_pipe_nodes},
+      {"ring_max_cand", g_ring_max_cand},
       {"distribute_tail", g_distribute_tail ? 1 : 0}, {"chunk_grid", g_chunk_grid},
       {"chunk_cap", g_chunk_cap}};
   for (const auto& kv : all)
@@ -309,6 +311,7 @@ const std::vector<TuneParam>& tune_params() {
       {"avg_blocks", {512, 1024, 2048, 4096}},
       {"bucket_min_nodes", {128, 512, 2048, 1 << 30}},
       {"accept_pipe_nodes", {0, 256, 1024, 4096}},
+      {"ring_max_cand", {32768, 65536, 1 << 30}},
   };
   return p;
 }

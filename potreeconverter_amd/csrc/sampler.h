@@ -99,7 +99,7 @@ struct SampleParams {
   int32_t bucket_min_nodes;  // launches of at least this many nodes bucket-sort their candidates
   int32_t accept_pipe_nodes; // launches of at most this many nodes, and the ring's launches (Poisson),
                              // take the pipelined acceptance; 0: never
-  int32_t pad2;
+  int32_t ring_max_cand;     // nodes above this many candidates take the cell-table path
 };
 
 // Samples every node of `nodes` (all of one level, children already done).

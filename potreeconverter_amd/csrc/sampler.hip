@@ -487,7 +487,9 @@ __global__ __launch_bounds__(kBlock) void k_prepare(const int32_t* __restrict__ 
     w.n = n;
     for (int c = 0; c < 9; c++) w.child_start[c] = s_cs[c];
     for (int c = 0; c < 8; c++) w.csrc[c] = s_cptr[c];
-    w.flags = (P.method == METHOD_POISSON_AVERAGE || (int)gridDim.x < P.ring_min_nodes) ? NW_CELL_PATH : 0;
+    w.flags = (P.method == METHOD_POISSON_AVERAGE || (int)gridDim.x < P.ring_min_nodes || n > P.ring_max_cand)
+                  ? NW_CELL_PATH
+                  : 0;
     w.pad = 0;
     if (off + bytes > P.scratch_cap) {
       w.flags = NW_OVERFLOW;

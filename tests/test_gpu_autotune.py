@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 FILES = ["octree.bin", "hierarchy.bin", "metadata.json"]
 TUNED = ["sample_groups", "sample_order", "sample_graph", "set_policy", "ring_min_nodes", "ring_blocks_per_cu", "copy_blocks",
-         "neighbor_blocks", "avg_blocks", "bucket_min_nodes", "accept_pipe_nodes"]
+         "neighbor_blocks", "avg_blocks", "bucket_min_nodes", "accept_pipe_nodes", "ring_max_cand"]
 
 
 def same(a, b):
@@ -32,7 +32,7 @@ SHAPES = [
     {"ring_blocks_per_cu": 1, "copy_blocks": 256, "neighbor_blocks": 256, "avg_blocks": 64, "bucket_min_nodes": 1,
      "accept_pipe_nodes": 0},
     {"ring_min_nodes": 1, "sample_order": 1, "copy_blocks": 65536, "neighbor_blocks": 16384, "avg_blocks": 8192,
-     "accept_pipe_nodes": 1 << 30},
+     "accept_pipe_nodes": 1 << 30, "ring_max_cand": 4096},
     {"sample_groups": 1, "sample_order": 1, "ring_min_nodes": 64, "ring_blocks_per_cu": 2,
      "bucket_min_nodes": 1 << 30},
 ]
@@ -55,7 +55,8 @@ def test_launch_shapes_agree(method, tmp_path, keep_options):
         for k in shape:
             pc.set_option(k, {"ring_blocks_per_cu": 3, "copy_blocks": 8192, "neighbor_blocks": 4096,
                               "avg_blocks": 2048, "ring_min_nodes": 512, "sample_order": 0,
-                              "sample_groups": 0, "bucket_min_nodes": 512, "accept_pipe_nodes": 1024}[k])
+                              "sample_groups": 0, "bucket_min_nodes": 512, "accept_pipe_nodes": 1024,
+                              "ring_max_cand": 1 << 30}[k])
 
 
 def test_autotune_keeps_bytes(tmp_path, keep_options):

@@ -36,7 +36,7 @@ def test_autotune_parameter_list():
     L = pc.lib()
     names = [L.pcg_autotune_param_name(i).decode() for i in range(L.pcg_autotune_params())]
     assert names == ["sample_groups", "sample_order", "sample_graph", "set_policy", "ring_min_nodes", "ring_blocks_per_cu", "copy_blocks",
-                     "neighbor_blocks", "avg_blocks", "bucket_min_nodes", "accept_pipe_nodes"]
+                     "neighbor_blocks", "avg_blocks", "bucket_min_nodes", "accept_pipe_nodes", "ring_max_cand"]
     assert L.pcg_autotune_param_name(len(names)) is None
     for n in names:
         pc.get_option(n)
