@@ -707,6 +707,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   sp.arena_cap = arenaCap;
   sp.arena_top = gtop.as<uint64_t>();
   sp.counters = counters.as<unsigned long long>();
+  sp.num_cus = device().num_cus;
   sp.ring_blocks = g_ring_blocks_per_cu * device().num_cus;  // at most three k_ring blocks fit a CU's LDS
   sp.copy_blocks = g_copy_blocks;
   sp.neighbor_blocks = g_neighbor_blocks;

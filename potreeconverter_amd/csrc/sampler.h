@@ -69,7 +69,7 @@ struct SampleParams {
   int32_t bpp;
   int32_t rgb_offset;
   int32_t method;
-  int32_t pad;
+  int32_t num_cus;         // compute units of the device (grids sized to the GPU)
   uint8_t* arena;          // record arena
   uint64_t arena_cap;      // bytes
   uint64_t* arena_top;     // bytes used (device counter)

@@ -2250,48 +2250,96 @@ __device__ void pipe_consume(const SampleParams& P, const Layout& L, const NodeG
   }
 }
 
-// kWide == false: nodes of at most 65 536 candidates (the others return);
-// kWide: the nodes above that, with accept_windows past 262 144.
+// The cell-path nodes of a launch for the pipelined acceptance, in two
+// lists in `list` (num_nodes entries): the nodes of at most 65 536
+// candidates from the front, the larger ones from the back; counts[0] and
+// counts[1] their lengths.  The pipelined kernels then take a grid sized to
+// the GPU over their list instead of a workgroup (and its LDS) per node of
+// the launch: in a launch the ring decides, most nodes are not theirs.
+__global__ __launch_bounds__(1024) void k_pipe_select(const NodeWork* __restrict__ work, int32_t num_nodes,
+                                                      int32_t* __restrict__ list, uint32_t* __restrict__ counts) {
+  __shared__ uint32_t s_w[2][16];
+  __shared__ uint32_t s_run[2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  constexpr int64_t kNarrowMax = (int64_t)PipeCfg<false>::kWords * 32;
+  if (tid < 2) s_run[tid] = 0;
+  __syncthreads();
+  for (int b = 0; b < num_nodes; b += 1024) {
+    const int k = b + tid;
+    bool narrow = false, wide = false;
+    if (k < num_nodes) {
+      const uint32_t f = work[k].flags;
+      if ((f & (NW_OVERFLOW | NW_CELL_PATH)) == NW_CELL_PATH) {
+        narrow = work[k].n <= kNarrowMax;
+        wide = !narrow;
+      }
+    }
+    const uint64_t bn = __ballot(narrow), bw = __ballot(wide);
+    if (lane == 0) { s_w[0][wave] = (uint32_t)__popcll(bn); s_w[1][wave] = (uint32_t)__popcll(bw); }
+    __syncthreads();
+    uint32_t on = s_run[0], ow = s_run[1], tn = 0, tw = 0;
+    for (int q = 0; q < 16; q++) {
+      if (q < wave) { on += s_w[0][q]; ow += s_w[1][q]; }
+      tn += s_w[0][q];
+      tw += s_w[1][q];
+    }
+    if (narrow) list[on + __popcll(bn & lt)] = k;
+    if (wide) list[num_nodes - 1 - (int)(ow + __popcll(bw & lt))] = k;
+    __syncthreads();
+    if (tid == 0) { s_run[0] += tn; s_run[1] += tw; }
+    __syncthreads();
+  }
+  if (tid == 0) { counts[0] = s_run[0]; counts[1] = s_run[1]; }
+}
+
+// kWide == false: the nodes of k_pipe_select's first list (at most 65 536
+// candidates); kWide: its second list, with accept_windows past 262 144.
+// Each workgroup takes the list's entries blockIdx.x, + gridDim.x, ...
 template <bool kWide>
 __global__ __launch_bounds__(64 * (1 + kPipeProducers)) void k_accept_pipe(const int32_t* __restrict__ nodes,
                                                                           const NodeInfo* __restrict__ infos,
                                                                           NodeWork* __restrict__ work,
-                                                                          SampleParams P) {
+                                                                          const int32_t* __restrict__ list,
+                                                                          const uint32_t* __restrict__ counts,
+                                                                          int32_t num_nodes, SampleParams P) {
   using C = PipeCfg<kWide>;
   __shared__ PipeLds<kWide> S;
-  const int k = blockIdx.x;
-  const NodeWork w = work[k];
-  if ((w.flags & NW_OVERFLOW) || !(w.flags & NW_CELL_PATH)) return;
-  const int64_t n = w.n;
-  constexpr int64_t kNarrowMax = (int64_t)PipeCfg<false>::kWords * 32;
-  if (kWide ? n <= kNarrowMax : n > kNarrowMax) return;
-  const Layout L = make_layout(w.scratch, n, P.method);
-  const NodeGeom g = node_geom(infos[nodes[k]], P);
+  const uint32_t count = counts[kWide ? 1 : 0];
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  if (n > (int64_t)C::kWords * 32) {
-    if (wave == 0) {
-      Filter F = make_filter(g, P);
-      accept_windows<3>(work, k, P, L, g, F, n, gbl(L.keyA));
-      flush_filter(F, P);
+  for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
+    const int k = kWide ? list[num_nodes - 1 - (int)e] : list[e];
+    const NodeWork w = work[k];
+    const int64_t n = w.n;
+    const Layout L = make_layout(w.scratch, n, P.method);
+    const NodeGeom g = node_geom(infos[nodes[k]], P);
+    if (n > (int64_t)C::kWords * 32) {
+      if (wave == 0) {
+        Filter F = make_filter(g, P);
+        accept_windows<3>(work, k, P, L, g, F, n, gbl(L.keyA));
+        flush_filter(F, P);
+      }
+      continue;
     }
-    return;
+    // every word, also past the node's: kNone's bit must read zero
+    for (int i = threadIdx.x; i < C::kWords; i += blockDim.x) S.bits[i] = 0;
+    if (threadIdx.x < C::kSlots) S.ready[threadIdx.x] = 0;
+    if (threadIdx.x == 0) { S.consumed = 0; S.abort = 0; }
+    __syncthreads();
+    Filter F = make_filter(g, P);
+    if (wave == 0) pipe_consume<kWide>(P, L, g, F, n, S);
+    else pipe_produce<kWide>(work, k, P, L, g, n, S, wave - 1);
+    __syncthreads();
+    if (S.abort) {
+      if (threadIdx.x == 0) atomicAdd(&P.counters[9], 1ull);
+    } else {
+      const gptr<uint8_t> gAcc = gbl(L.acc);
+      for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+        gAcc[i] = (uint8_t)((S.bits[i >> 5] >> (i & 31)) & 1u);
+      if (wave == 0) flush_filter(F, P);
+    }
+    __syncthreads();  // the next node reuses S
   }
-  // every word, also past the node's: kNone's bit must read zero
-  for (int i = threadIdx.x; i < C::kWords; i += blockDim.x) S.bits[i] = 0;
-  if (threadIdx.x < C::kSlots) S.ready[threadIdx.x] = 0;
-  if (threadIdx.x == 0) { S.consumed = 0; S.abort = 0; }
-  __syncthreads();
-  Filter F = make_filter(g, P);
-  if (wave == 0) pipe_consume<kWide>(P, L, g, F, n, S);
-  else pipe_produce<kWide>(work, k, P, L, g, n, S, wave - 1);
-  __syncthreads();
-  if (S.abort) {
-    if (threadIdx.x == 0) atomicAdd(&P.counters[9], 1ull);
-    return;
-  }
-  const gptr<uint8_t> gAcc = gbl(L.acc);
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) gAcc[i] = (uint8_t)((S.bits[i >> 5] >> (i & 31)) & 1u);
-  if (wave == 0) flush_filter(F, P);
 }
 
 // kAccAhead windows of neighbour rows in flight: 3 for launches of few
@@ -3695,9 +3743,18 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   // level's largest)
   const bool ring_level = p.method == METHOD_POISSON && num_nodes >= p.ring_min_nodes;
   if (p.accept_pipe_nodes > 0 && (num_nodes <= p.accept_pipe_nodes || ring_level)) {
-    hipLaunchKernelGGL(k_accept_pipe<false>, grid, dim3(64 * (1 + kPipeProducers)), 0, s, d_nodes, d_info, d_work, p);
+    // the lists in the item table (k_neighbors has read it), their counts
+    // in the ring's control words (the ring is done)
+    int32_t* list = (int32_t*)p.item_start;
+    uint32_t* counts = p.ring_ctl;
+    hipLaunchKernelGGL(k_pipe_select, dim3(1), dim3(1024), 0, s, d_work, num_nodes, list, counts);
     PCG_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_accept_pipe<true>, grid, dim3(64 * (1 + kPipeProducers)), 0, s, d_nodes, d_info, d_work, p);
+    const int cus = p.num_cus > 0 ? p.num_cus : 256;
+    hipLaunchKernelGGL(k_accept_pipe<false>, dim3((unsigned)std::min(num_nodes, 3 * cus)), dim3(64 * (1 + kPipeProducers)),
+                       0, s, d_nodes, d_info, d_work, (const int32_t*)list, (const uint32_t*)counts, num_nodes, p);
+    PCG_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_accept_pipe<true>, dim3((unsigned)std::min(num_nodes, cus)), dim3(64 * (1 + kPipeProducers)), 0,
+                       s, d_nodes, d_info, d_work, (const int32_t*)list, (const uint32_t*)counts, num_nodes, p);
   } else if (num_nodes <= kAccDeepNodes)
     hipLaunchKernelGGL(k_accept<3>, dim3((unsigned)((num_nodes + kWaves - 1) / kWaves)), dim3(kBlock), 0, s, d_nodes,
                        d_info, d_work, num_nodes, p);
