@@ -79,6 +79,16 @@ __device__ inline uint4 ld_g16(gptr<const uint4> p) {
   const u32x4_t v = *(gptr<const u32x4_t>)p;
   return make_uint4(v.x, v.y, v.z, v.w);
 }
+typedef int32_t i32x4_t __attribute__((ext_vector_type(4)));
+__device__ inline int4 ld_g16i(gptr<const int4> p) {
+  const i32x4_t v = *(gptr<const i32x4_t>)p;
+  return make_int4(v.x, v.y, v.z, v.w);
+}
+__device__ inline void st_g16i(gptr<int4> p, int4 v) {
+  i32x4_t t;
+  t.x = v.x; t.y = v.y; t.z = v.z; t.w = v.w;
+  *(gptr<i32x4_t>)p = t;
+}
 
 __host__ __device__ inline uint32_t hash_cap(int64_t n, int method) {
   uint32_t h = 64;
@@ -1472,6 +1482,7 @@ __device__ inline CellBox cell_box(const NodeGeom& g, double x, double y, double
 // Cell table of a node: candidates sorted by their 2*spacing cell (stable,
 // so ascending sorted position inside a cell) with their coordinates
 // copied alongside, and a hash from cell to its range.
+constexpr int kCellsU = 8;  // candidates per thread in flight in k_cells' gathers
 __global__ __launch_bounds__(kBlock) void k_cells(const int32_t* __restrict__ nodes, const NodeInfo* __restrict__ infos,
                                                    NodeWork* __restrict__ work, SampleParams P) {
   __shared__ RadixShared s_rad;
@@ -1487,33 +1498,34 @@ __global__ __launch_bounds__(kBlock) void k_cells(const int32_t* __restrict__ no
   const double cs = 2.0 * g.spacing;
   for (int i = tid; i < 3 * 256; i += kBlock) (&s_h3[0][0])[i] = 0;
   __syncthreads();
-  // four candidates per thread in flight
-  for (int64_t p0 = 0; p0 < n; p0 += 4 * kBlock) {
-    uint32_t j4[4];
+  // kCellsU candidates per thread in flight (the loop is bound by the
+  // latency of its two dependent loads); positions past the node read
+  // position 0 and are not used
+  const gptr<const uint32_t> gIdxA = gbl((const uint32_t*)L.idxA);
+  const gptr<const int4> gXyz = gbl((const int4*)L.xyz);
+  for (int64_t p0 = 0; p0 < n; p0 += kCellsU * kBlock) {
+    uint32_t j4[kCellsU];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < kCellsU; u++) {
       const int64_t p = p0 + u * kBlock + tid;
-      j4[u] = p < n ? L.idxA[p] : 0u;
+      j4[u] = gIdxA[p < n ? p : 0];
     }
-    int32_t X4[4], Y4[4], Z4[4];
+    int32_t X4[kCellsU], Y4[kCellsU], Z4[kCellsU];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < kCellsU; u++) {
       const int64_t p = p0 + u * kBlock + tid;
-      X4[u] = Y4[u] = Z4[u] = 0;
-      if (p < n) {
-        const int4 q = L.xyz[j4[u]];
-        X4[u] = q.x; Y4[u] = q.y; Z4[u] = q.z;
-      }
+      const int4 q = ld_g16i(gXyz + j4[u]);
+      X4[u] = q.x; Y4[u] = q.y; Z4[u] = q.z;
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < kCellsU; u++) {
       const int64_t p = p0 + u * kBlock + tid;
       if (p >= n) continue;
       double x = coord(X4[u], P.scale[0], P.offset[0]), y = coord(Y4[u], P.scale[1], P.offset[1]),
              z = coord(Z4[u], P.scale[2], P.offset[2]);
       const uint32_t key = gkey(gcell(x, g.min[0], cs), gcell(y, g.min[1], cs), gcell(z, g.min[2], cs));
-      L.keyA[p] = key;
-      L.idxB[p] = (uint32_t)p;
+      gbl(L.keyA)[p] = key;
+      gbl(L.idxB)[p] = (uint32_t)p;
       atomicAdd(&s_h3[0][key & 127u], 1u);
       atomicAdd(&s_h3[1][(key >> 7) & 127u], 1u);
       atomicAdd(&s_h3[2][(key >> 14) & 127u], 1u);
@@ -1527,49 +1539,72 @@ __global__ __launch_bounds__(kBlock) void k_cells(const int32_t* __restrict__ no
     uint32_t* t = kc; kc = ko; ko = t;
     t = vc; vc = vo; vo = t;
   }
-  // cell-ordered entries, four per thread with each level of the index
+  // cell-ordered entries, kCellsU per thread with each level of the index
   // chain (sorted position -> main index -> coordinates) loaded together
-  for (int64_t e0 = tid; e0 < n; e0 += 4 * kBlock) {
-    uint32_t p4[4], k4[4], kp4[4], j4[4];
+  // (positions past the node read position 0 and are not used)
+  const gptr<const uint32_t> gVc = gbl((const uint32_t*)vc), gKc = gbl((const uint32_t*)kc);
+  for (int64_t e0 = tid; e0 < n; e0 += kCellsU * kBlock) {
+    uint32_t p4[kCellsU], k4[kCellsU], kp4[kCellsU], j4[kCellsU];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < kCellsU; u++) {
       const int64_t e = e0 + u * kBlock;
-      p4[u] = e < n ? vc[e] : 0u;
-      k4[u] = e < n ? kc[e] : 0u;
-      kp4[u] = (e > 0 && e < n) ? kc[e - 1] : ~0u;
+      const int64_t ec = e < n ? e : 0;
+      p4[u] = gVc[ec];
+      k4[u] = gKc[ec];
+      kp4[u] = gKc[ec > 0 ? ec - 1 : 0];
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) j4[u] = e0 + u * kBlock < n ? L.idxA[p4[u]] : 0u;
-    int4 v4[4];
-    double a4[4];
-    uint32_t c4[4];
+    for (int u = 0; u < kCellsU; u++) j4[u] = gIdxA[p4[u]];
+    int4 v4[kCellsU];
+    double a4[kCellsU];
+    uint32_t c4[kCellsU];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      if (e0 + u * kBlock >= n) continue;
-      const int4 q = L.xyz[j4[u]];
+    for (int u = 0; u < kCellsU; u++) {
+      const int4 q = ld_g16i(gXyz + j4[u]);
       v4[u] = make_int4(q.x, q.y, q.z, (int32_t)p4[u]);
-      if (avg) c4[u] = L.cell16[j4[u]];
-      else a4[u] = L.sadd[p4[u]];
+      if (avg) c4[u] = gbl((const uint32_t*)L.cell16)[j4[u]];
+      else a4[u] = gbl((const double*)L.sadd)[p4[u]];
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < kCellsU; u++) {
       const int64_t e = e0 + u * kBlock;
       if (e >= n) continue;
-      L.ent[e] = v4[u];
-      if (avg) L.eC16[e] = c4[u];
-      else L.eAdd[e] = a4[u];
-      const uint32_t key = k4[u];
-      if (e == 0 || kp4[u] != key) {
-        uint32_t slot = ghash(key, L.H);
-        while (atomicCAS(&L.htab[slot].x, kNil, key) != kNil) slot = (slot + 1) & (L.H - 1);
-        L.htab[slot].y = (uint32_t)e;
+      st_g16i(gbl(L.ent) + e, v4[u]);
+      if (avg) gbl(L.eC16)[e] = c4[u];
+      else gbl(L.eAdd)[e] = a4[u];
+    }
+    // each cell's first entry claims the cell's hash slot and writes its
+    // range: the claims of all kCellsU entries issued together (a key is
+    // claimed once: keys ascend in kc), probing on only for collisions; the
+    // cell's end found by galloping over kc meanwhile
+    uint32_t slot[kCellsU];
+    uint64_t pend = 0;
+#pragma unroll
+    for (int u = 0; u < kCellsU; u++) {
+      const int64_t e = e0 + u * kBlock;
+      slot[u] = ghash(k4[u], L.H);
+      if (e < n && (e == 0 || kp4[u] != k4[u])) pend |= 1ull << u;
+    }
+    uint32_t end4[kCellsU];
+#pragma unroll
+    for (int u = 0; u < kCellsU; u++)
+      end4[u] = ((pend >> u) & 1) ? (uint32_t)run_end(kc, e0 + u * kBlock, n) : 0u;
+    while (pend) {
+      uint32_t old[kCellsU];
+#pragma unroll
+      for (int u = 0; u < kCellsU; u++) old[u] = ((pend >> u) & 1) ? atomicCAS(&L.htab[slot[u]].x, kNil, k4[u]) : 0u;
+#pragma unroll
+      for (int u = 0; u < kCellsU; u++) {
+        if (!((pend >> u) & 1)) continue;
+        if (old[u] == kNil) {
+          L.htab[slot[u]].y = (uint32_t)(e0 + u * kBlock);
+          L.htab[slot[u]].z = end4[u];
+          pend &= ~(1ull << u);
+        } else {
+          slot[u] = (slot[u] + 1) & (L.H - 1);
+        }
       }
     }
-  }
-  __syncthreads();
-  for (int64_t e = tid; e < n; e += kBlock) {
-    const uint32_t key = kc[e];
-    if (e + 1 == n || kc[e + 1] != key) L.htab[table_find(L, key)].z = (uint32_t)(e + 1);
   }
 }
 
