@@ -2465,7 +2465,7 @@ __device__ void ring_node(const int32_t* __restrict__ nodes, const NodeInfo* __r
   const float sq = (float)(g.spacing * (1.0 + 1e-4));
   unsigned long long rechecks = 0, tests = 0;
   float maxerr = 0.f;
-  int32_t* const rpos = (int32_t*)L.keyB;  // sorted position of each ring entry
+  const gptr<int32_t> rpos = gbl((int32_t*)L.keyB);  // sorted position of each ring entry
   for (int i = lane; i < kRingSlots; i += 64) R.head[i] = -1;
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   int32_t total = 0, tail = 0;  // accepted points appended / first live one (wave-uniform)
@@ -2478,29 +2478,38 @@ __device__ void ring_node(const int32_t* __restrict__ nodes, const NodeInfo* __r
     maxerr = e > maxerr ? e : maxerr;
     return d64 < G.s2;
   };
-  int32_t X = 0, Y = 0, Z = 0;
-  double a = 0.0;
-  if (lane < n) {
-    const uint32_t j = L.idxA[lane];
-    const int4 q = L.xyz[j];
+  if (n <= 0) return;
+  // the node's arrays through global-address pointers (a flat load would
+  // also count as an LDS operation, and every wait of the chain walks would
+  // wait for the next windows' loads too); every lane loads, from a clamped
+  // position past the node (values unused), so the loads take no branch
+  const gptr<const uint32_t> gIdx = gbl((const uint32_t*)L.idxA);
+  const gptr<const int4> gXyz = gbl((const int4*)L.xyz);
+  const gptr<const double> gSadd = gbl((const double*)L.sadd);
+  auto clampn = [n](int64_t q) { return q < n ? q : n - 1; };
+  int32_t X, Y, Z;
+  double a;
+  {
+    const uint32_t j = gIdx[clampn(lane)];
+    const int4 q = ld_g16i(gXyz + j);
     X = q.x; Y = q.y; Z = q.z;
-    a = L.sadd[lane];
+    a = gSadd[clampn(lane)];
   }
   // the next window's candidates are loaded while this one is decided; the
   // window after it has its main indices loaded already, so no load waits
   // on another load of the same window
-  uint32_t jn = lane + 64 < n ? L.idxA[lane + 64] : 0u;
+  uint32_t jn = gIdx[clampn(lane + 64)];
   for (int64_t w0 = 0; w0 < n; w0 += 64) {
     const int64_t p = w0 + lane;
     const bool valid = p < n;
-    int32_t nX = 0, nY = 0, nZ = 0;
-    double na = 0.0;
-    if (p + 64 < n) {
-      const int4 q = L.xyz[jn];
+    int32_t nX, nY, nZ;
+    double na;
+    {
+      const int4 q = ld_g16i(gXyz + jn);
       nX = q.x; nY = q.y; nZ = q.z;
-      na = L.sadd[p + 64];
+      na = gSadd[clampn(p + 64)];
     }
-    jn = p + 128 < n ? L.idxA[p + 128] : 0u;
+    jn = gIdx[clampn(p + 128)];
     const double cd = __builtin_sqrt(a);
     const double lim = cd - g.spacing;
     const double limSq = lim * lim;
@@ -2560,8 +2569,8 @@ __device__ void ring_node(const int32_t* __restrict__ nodes, const NodeInfo* __r
             if ((double)R.add[e] < limSq) continue;
             bool within = d32 < G.band_lo;
             if (!within) {
-              const uint32_t jq = L.idxA[rpos[e]];
-              const int4 q = L.xyz[jq];
+              const uint32_t jq = gIdx[rpos[e]];
+              const int4 q = ld_g16i(gXyz + jq);
               within = exact(q.x, q.y, q.z, X, Y, Z, d32);
             }
             if (within) rej = true;
@@ -2630,7 +2639,7 @@ __device__ void ring_node(const int32_t* __restrict__ nodes, const NodeInfo* __r
       dec = __ballot(decided);
       accm = __ballot(accepted);
     }
-    if (valid) L.acc[p] = accepted ? 1 : 0;
+    if (valid) gbl(L.acc)[p] = accepted ? 1 : 0;
     // append the window's accepted points; entries back to tail - 64 must
     // stay intact for the chain walks
     const int32_t kacc = __popcll(accm);
