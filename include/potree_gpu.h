@@ -70,7 +70,12 @@ int pcg_release(void);
  * the level-wide record copy and neighbour search, "avg_blocks" (default
  * 2048) workgroups of a Poisson-average colour-sum launch,
  * "bucket_min_nodes" (default 512) launches of at least this many nodes sort
- * each node's candidates by distance buckets, smaller ones by radix passes.
+ * each node's candidates by distance buckets, smaller ones by radix passes,
+ * "accept_pipe_nodes" (default 1024, 0 = never) launches of at most this
+ * many nodes, and every launch the ring decides, take the pipelined
+ * acceptance (a workgroup per cell-path node), larger ones one wavefront
+ * per node, "ring_max_cand" (default 2^30) nodes above this many candidates
+ * take the cell-table path instead of the ring.
  * "set_policy" (0/1, default 0): chunk subtrees split into sets by their
  * points (0) or with the subtrees deeper than where most points end in sets
  * of their own (1) (same results).
