@@ -75,7 +75,9 @@ int pcg_release(void);
  * many nodes, and every launch the ring decides, take the pipelined
  * acceptance (a workgroup per cell-path node), larger ones one wavefront
  * per node, "ring_max_cand" (default 2^30) nodes above this many candidates
- * take the cell-table path instead of the ring.
+ * take the cell-table path instead of the ring, "pipe_form" (0-2, default
+ * 0; a test option) the pipelined acceptance by node size (0), every node in
+ * its 32-bit-position form (1) or by one wavefront per node (2).
  * "set_policy" (0/1, default 0): chunk subtrees split into sets by their
  * points (0) or with the subtrees deeper than where most points end in sets
  * of their own (1) (same results).

@@ -217,7 +217,8 @@ int pcg_set_option(const char* key, int64_t value) {
                           {"avg_blocks", &g_avg_blocks, 64, 1 << 20},
                           {"bucket_min_nodes", &g_bucket_min_nodes, 1, 1 << 30},
                           {"accept_pipe_nodes", &g_accept_pipe_nodes, 0, 1 << 30},
-                          {"ring_max_cand", &g_ring_max_cand, 64, 1 << 30}};
+                          {"ring_max_cand", &g_ring_max_cand, 64, 1 << 30},
+                          {"pipe_form", &g_pipe_form, 0, 2}};
   for (const Shape& sh : shapes)
     if (k == sh.name) {
       if (value < sh.lo || value > sh.hi) {
@@ -277,7 +278,7 @@ int pcg_get_option(const char* key, int64_t* value) {
       {"ring_blocks_per_cu", g_ring_blocks_per_cu}, {"copy_blocks", g_copy_blocks},
       {"neighbor_blocks", g_neighbor_blocks},       {"avg_blocks", g_avg_blocks},
       {"bucket_min_nodes", g_bucket_min_nodes},     {"accept_pipe_nodes", g_accept_pipe_nodes},
-      {"ring_max_cand", g_ring_max_cand},
+      {"ring_max_cand", g_ring_max_cand},           {"pipe_form", g_pipe_form},
       {"distribute_tail", g_distribute_tail ? 1 : 0}, {"chunk_grid", g_chunk_grid},
       {"chunk_cap", g_chunk_cap}};
   for (const auto& kv : all)

@@ -172,6 +172,7 @@ int32_t g_set_policy = 0;
 int32_t g_ring_blocks_per_cu = 3;
 int32_t g_accept_pipe_nodes = 1024;
 int32_t g_ring_max_cand = 1 << 30;
+int32_t g_pipe_form = 0;
 int32_t g_copy_blocks = 8192;
 int32_t g_neighbor_blocks = 4096;
 int32_t g_avg_blocks = 2048;
@@ -715,6 +716,7 @@ int gpu_indexing(DeviceChunks& dc, const std::string& method, const std::string&
   sp.bucket_min_nodes = g_bucket_min_nodes;
   sp.accept_pipe_nodes = g_accept_pipe_nodes;
   sp.ring_max_cand = g_ring_max_cand;
+  sp.pipe_form = g_pipe_form;
   // a group's buffers for levels of at most `maxNodes` nodes and `cand`
   // candidates in all
   auto group_params = [&](GroupMem& m, int64_t maxNodes, int64_t cand, SampleParams& p) -> int {

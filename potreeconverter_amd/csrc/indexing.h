@@ -80,6 +80,7 @@ extern int32_t g_avg_blocks;
 extern int32_t g_bucket_min_nodes;
 extern int32_t g_accept_pipe_nodes;
 extern int32_t g_ring_max_cand;
+extern int32_t g_pipe_form;
 // Seed of every SamplerRandom leaf shuffle (0 .. 2^32-1), or -1 for the
 // reference's choice, the system clock at the start of indexing.
 extern int64_t g_random_seed;

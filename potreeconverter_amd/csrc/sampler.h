@@ -100,6 +100,9 @@ struct SampleParams {
   int32_t accept_pipe_nodes; // launches of at most this many nodes, and the ring's launches (Poisson),
                              // take the pipelined acceptance; 0: never
   int32_t ring_max_cand;     // nodes above this many candidates take the cell-table path
+  int32_t pipe_form;         // pipelined acceptance: 0 by node size, 1 every node in the 32-bit form,
+                             // 2 every node by one wavefront (test forms, same results)
+  int32_t pad3;
 };
 
 // Samples every node of `nodes` (all of one level, children already done).

@@ -2292,7 +2292,8 @@ __device__ void pipe_consume(const SampleParams& P, const Layout& L, const NodeG
 // the GPU over their list instead of a workgroup (and its LDS) per node of
 // the launch: in a launch the ring decides, most nodes are not theirs.
 __global__ __launch_bounds__(1024) void k_pipe_select(const NodeWork* __restrict__ work, int32_t num_nodes,
-                                                      int32_t* __restrict__ list, uint32_t* __restrict__ counts) {
+                                                      int32_t* __restrict__ list, uint32_t* __restrict__ counts,
+                                                      int32_t form) {
   __shared__ uint32_t s_w[2][16];
   __shared__ uint32_t s_run[2];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -2306,7 +2307,7 @@ __global__ __launch_bounds__(1024) void k_pipe_select(const NodeWork* __restrict
     if (k < num_nodes) {
       const uint32_t f = work[k].flags;
       if ((f & (NW_OVERFLOW | NW_CELL_PATH)) == NW_CELL_PATH) {
-        narrow = work[k].n <= kNarrowMax;
+        narrow = form == 0 && work[k].n <= kNarrowMax;
         wide = !narrow;
       }
     }
@@ -2348,7 +2349,7 @@ __global__ __launch_bounds__(64 * (1 + kPipeProducers)) void k_accept_pipe(const
     const int64_t n = w.n;
     const Layout L = make_layout(w.scratch, n, P.method);
     const NodeGeom g = node_geom(infos[nodes[k]], P);
-    if (n > (int64_t)C::kWords * 32) {
+    if (n > (int64_t)C::kWords * 32 || P.pipe_form == 2) {
       if (wave == 0) {
         Filter F = make_filter(g, P);
         accept_windows<3>(work, k, P, L, g, F, n, gbl(L.keyA));
@@ -3791,7 +3792,7 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
     // in the ring's control words (the ring is done)
     int32_t* list = (int32_t*)p.item_start;
     uint32_t* counts = p.ring_ctl;
-    hipLaunchKernelGGL(k_pipe_select, dim3(1), dim3(1024), 0, s, d_work, num_nodes, list, counts);
+    hipLaunchKernelGGL(k_pipe_select, dim3(1), dim3(1024), 0, s, d_work, num_nodes, list, counts, p.pipe_form);
     PCG_CHECK(hipGetLastError());
     const int cus = p.num_cus > 0 ? p.num_cus : 256;
     hipLaunchKernelGGL(k_accept_pipe<false>, dim3((unsigned)std::min(num_nodes, 3 * cus)), dim3(64 * (1 + kPipeProducers)),

@@ -94,3 +94,29 @@ def test_autotune_keeps_bytes(tmp_path, keep_options):
     finally:
         L.pcg_device_free(d)
         L.pcg_device_free(dq)
+
+
+@pytest.mark.parametrize("method", ["poisson", "poisson_average"])
+def test_pipelined_acceptance_forms_agree(method, tmp_path, keep_options):
+    """Every cell-path node through each form of the acceptance: the
+    pipelined one with 16-bit positions (by size, the default), with 32-bit
+    positions (pipe_form 1), and one wavefront per node with the accepted
+    bits in global memory (pipe_form 2), and k_accept on wide launches
+    (accept_pipe_nodes 0); all levels on the cell-table path."""
+    las = tmp_path / "in.las"
+    make_las(las, 0, 2_000_000, 7, False)
+    cpu = tmp_path / "cpu"
+    oracle_convert(las, cpu, method)
+    saved_form = pc.get_option("pipe_form")
+    try:
+        pc.set_option("ring_min_nodes", 1 << 20)
+        for form, pipe_nodes in ((0, 1 << 30), (1, 1 << 30), (2, 1 << 30), (0, 0)):
+            print(f"acceptance form {form} accept_pipe_nodes={pipe_nodes} {method}", flush=True)
+            pc.set_option("pipe_form", form)
+            pc.set_option("accept_pipe_nodes", pipe_nodes)
+            out = tmp_path / f"f{form}_{pipe_nodes}"
+            pc.convert(las, out, method)
+            res = same(cpu, out)
+            assert all(res.values()), (form, pipe_nodes, res)
+    finally:
+        pc.set_option("pipe_form", saved_form)
