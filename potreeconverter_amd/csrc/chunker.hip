@@ -56,6 +56,20 @@ __device__ inline uint32_t wave_load_records(const uint8_t* __restrict__ src, ui
   return (uint32_t)(b0 - a0);
 }
 
+// 16 bytes of the LAS buffer at a 16-byte aligned offset; bytes at or past
+// `total` read as 0
+__device__ inline uint4 load_vec16(const uint8_t* __restrict__ src, uint64_t total, uint64_t addr) {
+  if (addr + 16 <= total) return *(const uint4*)(src + addr);
+  uint32_t w[4] = {0u, 0u, 0u, 0u};
+  for (int b = 0; b < 16; b++)
+    if (addr + b < total) w[b >> 2] |= (uint32_t)src[addr + b] << (8 * (b & 3));
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// 16-byte vectors per lane that hold one step's records of up to `bpp`
+// bytes each (a step starts anywhere in a 16-byte vector)
+__host__ __device__ constexpr int prefetch_vecs(int bpp) { return (64 * bpp + 30) / (16 * 64) + 1; }
+
 __device__ inline int32_t ld_i32(const uint8_t* p) {
   return (int32_t)((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24));
 }
@@ -142,13 +156,18 @@ __device__ inline uint64_t stat_key_bits(const StatLane& s, uint64_t key) {
 }
 
 // laszip coordinate (scale * X + offset), requantization and the cell of
-// one point: chunker_countsort_laszip.cpp:224-257
-__device__ inline int64_t cell_of_record(const uint8_t* rec, const ChunkerParams& P, bool& inBox, int32_t q[3]) {
+// one point: chunker_countsort_laszip.cpp:224-257.  The grid has at most
+// 512^3 cells, so the cell index and its per-axis parts are int32 (one
+// conversion instruction each; double -> int64 is a long sequence on the
+// GPU).  In the box every per-axis part lies in [0, G - 1]; outside it the
+// point is an error and its cell unused, the clamp only keeps the
+// conversion defined.
+__device__ inline int32_t cell_of_record(const uint8_t* rec, const ChunkerParams& P, bool& inBox, int32_t q[3]) {
   const int32_t G = P.grid_size;
   const double dG = (double)G;
-  int64_t idx = 0;
+  int32_t idx = 0;
   inBox = true;
-  int64_t mul = 1;
+  int32_t mul = 1;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     int32_t X = (int32_t)lds_u32(rec + 4 * a);
@@ -162,15 +181,16 @@ __device__ inline int64_t cell_of_record(const uint8_t* rec, const ChunkerParams
     const double t = v * P.g_over_cube;
     const double ft = __builtin_floor(t);
     const double fr = t - ft;
-    int64_t ic;
+    int32_t ic;
     if (fr > 1e-9 && fr < 1.0 - 1e-9 && t > 1e-9 && t < dG - 1e-9) {
-      ic = (int64_t)ft;
+      ic = (int32_t)ft;
     } else {
       double u = v / P.cube;
       inBox = inBox && u >= 0.0 && u <= 1.0;
       double gu = dG * u;
       double lim = dG - 1.0;
-      ic = (int64_t)(lim < gu ? lim : gu);
+      const double c = lim < gu ? lim : gu;
+      ic = (int32_t)(c > 0.0 ? c : 0.0);
     }
     idx += ic * mul;
     mul *= G;
@@ -352,7 +372,7 @@ __device__ inline void publish_std(const StdAcc& acc, bool wave_seen, int lane, 
 // record's leading dwords are read from LDS once); the extra-byte statistics
 // are spread over the lanes (lane l folds extra statistic l % E of every
 // (64 / E)-th record of the step).  FMT 8: no standard statistics.
-template <int FMT>
+template <int FMT, int PV>
 __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ las, int64_t n,
                                                    int32_t* __restrict__ grid, uint32_t* __restrict__ cells,
                                                    uint64_t* __restrict__ stat_min,
@@ -399,10 +419,54 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ la
   const int64_t step1 = (step0 + per_wave) < steps_total ? (step0 + per_wave) : steps_total;
   const uint64_t low_incl = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
   bool bad = false;
+  // PV > 0: the next step's records as PV 16-byte vectors per lane, loaded
+  // without a branch per lane (lanes past the step reload its last vector)
+  // except in a step that reaches the end of the input, in flight during
+  // the current step and put into the record area after it
+  uint4 pf[PV > 0 ? PV : 1];
+  int pnvec = 0;
+  auto issue = [&](int64_t f) {
+    const int m = (int)((n - f) < 64 ? (n - f) : 64);
+    const uint64_t b0 = (uint64_t)f * bpp, a0 = b0 & ~15ull;
+    const int nvec = (int)((b0 + (uint64_t)m * bpp - a0 + 15) >> 4);
+    pnvec = nvec;
+    if (a0 + 16ull * (uint64_t)nvec <= total) {
+#pragma unroll
+      for (int j = 0; j < PV; j++) {
+        const int v = lane + 64 * j;
+        pf[j] = *(const uint4*)(las + a0 + 16ull * (uint64_t)(v < nvec ? v : nvec - 1));
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < PV; j++) {
+        const int v = lane + 64 * j;
+        pf[j] = v < nvec ? load_vec16(las, total, a0 + 16ull * v) : make_uint4(0u, 0u, 0u, 0u);
+      }
+    }
+  };
+  auto put = [&]() {
+#pragma unroll
+    for (int j = 0; j < PV; j++) {
+      const int v = lane + 64 * j;
+      if (v < pnvec) *(uint4*)(rec_area + 16 * v) = pf[j];
+    }
+  };
+  if constexpr (PV > 0) {
+    if (step0 < step1) {
+      issue(step0 * 64);
+      put();
+    }
+  }
   for (int64_t step = step0; step < step1; step++) {
     const int64_t first = step * 64;
     const int cnt = (int)((n - first) < 64 ? (n - first) : 64);
-    const uint32_t shift = wave_load_records(las, total, first, cnt, bpp, rec_area);
+    uint32_t shift;
+    if constexpr (PV > 0) {
+      if (step + 1 < step1) issue(first + 64);
+      shift = (uint32_t)(((uint64_t)first * bpp) & 15u);
+    } else {
+      shift = wave_load_records(las, total, first, cnt, bpp, rec_area);
+    }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     const bool valid = lane < cnt;
@@ -451,10 +515,22 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ la
       if (!done) atomicAdd(&grid[ekey], (int32_t)run);
     }
     if (P.class_src >= 0) {
+      // one round per distinct class of the step for the first
+      // kMatchRounds classes (a step rarely holds more), the bitwise match
+      // over the lanes still left after that
       const uint32_t cm = (uint32_t)P.class_mask;
       const uint32_t c = valid ? ((uint32_t)rec[P.class_src] & cm) : cm + 1;
-      const uint64_t peers = wave_match(c, cm == 31 ? 6 : 9, ~0ull);
-      if (valid && (peers & (low_incl >> 1)) == 0) atomicAdd(&s_hist[c], (uint32_t)__popcll(peers));
+      uint64_t left = __ballot(valid);
+      for (int r = 0; r < kMatchRounds && left; r++) {
+        const uint32_t v = __builtin_amdgcn_readlane(c, __builtin_ctzll(left));
+        const uint64_t m = __ballot(c == v) & left;
+        if (lane == 0) atomicAdd(&s_hist[v], (uint32_t)__popcll(m));
+        left &= ~m;
+      }
+      if (left) {
+        const uint64_t peers = wave_match(c, cm == 31 ? 6 : 9, left);
+        if (((left >> lane) & 1) && (peers & (low_incl >> 1)) == 0) atomicAdd(&s_hist[c], (uint32_t)__popcll(peers));
+      }
     }
     if (my_stat >= 0) {
       for (int r = my_r0; r < cnt; r += R) {
@@ -466,6 +542,11 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint8_t* __restrict__ la
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if constexpr (PV > 0) {
+      if (step + 1 < step1) put();
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    }
   }
   if (bad) atomicOr(err, 1);
   // keys back to ordered double bits (a lane that saw no record keeps the
@@ -793,19 +874,6 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint8_t* __restrict__ 
 // permute to the record's alignment in the staging area, plain LDS stores
 // for the dwords that lie inside the record and LDS ORs (the area is zeroed
 // per step) for the two shared with the neighbouring records.
-// 16 bytes of the LAS buffer at a 16-byte aligned offset; bytes at or past
-// `total` read as 0
-__device__ inline uint4 load_vec16(const uint8_t* __restrict__ src, uint64_t total, uint64_t addr) {
-  if (addr + 16 <= total) return *(const uint4*)(src + addr);
-  uint32_t w[4] = {0u, 0u, 0u, 0u};
-  for (int b = 0; b < 16; b++)
-    if (addr + b < total) w[b >> 2] |= (uint32_t)src[addr + b] << (8 * (b & 3));
-  return make_uint4(w[0], w[1], w[2], w[3]);
-}
-
-// 16-byte vectors per lane that hold one step's records of up to `bpp`
-// bytes each (a step starts anywhere in a 16-byte vector)
-__host__ __device__ constexpr int prefetch_vecs(int bpp) { return (64 * bpp + 30) / (16 * 64) + 1; }
 
 // ------------------------------------------------------ tile transcode
 // Transcode + scatter for the record layouts of point formats 0-3 and 6-7
@@ -1248,13 +1316,47 @@ int chunker_count(const uint8_t* d_las, int64_t n, int32_t* d_grid, uint32_t* d_
                   hipStream_t stream) {
   int64_t steps = (n + 63) / 64;
   int64_t blocks = (steps + kWavesPerBlock * kCountSteps - 1) / (kWavesPerBlock * kCountSteps);
-  int64_t max_blocks = (int64_t)num_cus * 8;
+  size_t lds = (size_t)kWavesPerBlock * record_area_bytes(g_bpp_in);
+  // records of up to 32 or 64 bytes take the prefetching forms; the grid is
+  // what stays resident at once (every wave walks an equal range of steps,
+  // so a second round of blocks would run on a partly filled GPU)
+  const int pv = g_bpp_in <= 32 ? prefetch_vecs(32) : (g_bpp_in <= 64 ? prefetch_vecs(64) : 0);
+#define PCG_COUNT_FN(F, V) (const void*)k_count<F, V>
+#define PCG_COUNT_PV(F) (pv == prefetch_vecs(32) ? PCG_COUNT_FN(F, prefetch_vecs(32)) \
+                         : (pv == prefetch_vecs(64) ? PCG_COUNT_FN(F, prefetch_vecs(64)) : PCG_COUNT_FN(F, 0)))
+  const void* fn = nullptr;
+  switch (g_point_format) {
+    case 0: fn = PCG_COUNT_PV(0); break;
+    case 1: fn = PCG_COUNT_PV(1); break;
+    case 2: fn = PCG_COUNT_PV(2); break;
+    case 3: fn = PCG_COUNT_PV(3); break;
+    case 4: fn = PCG_COUNT_PV(4); break;
+    case 5: fn = PCG_COUNT_PV(5); break;
+    case 6: fn = PCG_COUNT_PV(6); break;
+    case 7: fn = PCG_COUNT_PV(7); break;
+    default: fn = PCG_COUNT_PV(8); break;
+  }
+#undef PCG_COUNT_PV
+#undef PCG_COUNT_FN
+  int per_cu = 0;
+  PCG_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, lds));
+  int64_t max_blocks = (int64_t)num_cus * (per_cu > 0 ? per_cu : 1);
   if (blocks > max_blocks) blocks = max_blocks;
   if (blocks < 1) blocks = 1;
-  size_t lds = (size_t)kWavesPerBlock * record_area_bytes(g_bpp_in);
 #define PCG_COUNT(F)                                                                                      \
-  hipLaunchKernelGGL(k_count<F>, dim3((unsigned)blocks), dim3(kBlock), lds, stream, d_las, n, d_grid, d_cells, \
-                     d_stat_min, d_stat_max, d_class_hist, d_err)
+  do {                                                                                                    \
+    if (pv == prefetch_vecs(32))                                                                          \
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_count<F, prefetch_vecs(32)>), dim3((unsigned)blocks),          \
+                         dim3(kBlock), lds, stream, d_las, n, d_grid, d_cells, d_stat_min, d_stat_max,      \
+                         d_class_hist, d_err);                                                            \
+    else if (pv == prefetch_vecs(64))                                                                     \
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_count<F, prefetch_vecs(64)>), dim3((unsigned)blocks),          \
+                         dim3(kBlock), lds, stream, d_las, n, d_grid, d_cells, d_stat_min, d_stat_max,      \
+                         d_class_hist, d_err);                                                            \
+    else                                                                                                  \
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_count<F, 0>), dim3((unsigned)blocks), dim3(kBlock), lds,       \
+                         stream, d_las, n, d_grid, d_cells, d_stat_min, d_stat_max, d_class_hist, d_err);  \
+  } while (0)
   switch (g_point_format) {
     case 0: PCG_COUNT(0); break;
     case 1: PCG_COUNT(1); break;
