@@ -1078,10 +1078,81 @@ __device__ inline void pair_pivot_to_first(PairArr a, int64_t f, int64_t l) {
   pair_swap(a, f, pick);
 }
 
+// A wavefront's segment staged in LDS: keys, main indices (bit 31: the
+// element is marked), and the partition's position lists, 16 bytes per
+// element, carved per wavefront from the launch's dynamic LDS.  The
+// partition loop then runs on LDS latency instead of memory latency.
+// Launches of few nodes (one workgroup per CU at most) stage segments of
+// up to kStageNarrow elements, wider ones up to kStageWide (a small LDS
+// footprint keeps several workgroups per CU).
+constexpr int kStageNarrow = 2048;
+constexpr int kStageWide = 128;
+constexpr uint32_t kStageMark = 0x80000000u;
+template <class T>
+using lptr = __attribute__((address_space(3))) T*;
+struct StageLds {
+  lptr<double> K;
+  lptr<uint32_t> H;
+  lptr<uint16_t> le, ge;
+};
+__device__ inline StageLds stage_view(uint8_t* base, int cap) {
+  return StageLds{(lptr<double>)base, (lptr<uint32_t>)(base + 8 * cap), (lptr<uint16_t>)(base + 12 * cap),
+                  (lptr<uint16_t>)(base + 14 * cap)};
+}
+
+// wave_partition on a staged segment: [f+1, l) of K/H around the pivot at
+// f, the same swaps (positions relative to f in le / ge)
+__device__ __attribute__((always_inline)) inline int wave_partition_lds(const StageLds& S, int f, int l) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const uint64_t me = 1ull << lane;
+  const double pv = S.K[f];
+  int leT = 0;
+  for (int b = f + 1; b < l; b += 64) {
+    const int i = b + lane;
+    const double v = S.K[i < l ? i : f];
+    leT += __popcll(__ballot(i < l && !(pv < v)));
+  }
+  int cG = 0, cL = 0, kc = 0, fail = INT32_MAX;
+  for (int b = f + 1; b < l; b += 64) {
+    const int i = b + lane;
+    const bool in = i < l;
+    const double v = S.K[in ? i : f];
+    const bool ge = in && !(v < pv), le = in && !(pv < v);
+    const uint64_t bg = __ballot(ge), bl = __ballot(le);
+    if (le) S.le[cL + __popcll(bl & lt)] = (uint16_t)(i - f);
+    bool ok = false;
+    if (ge) {
+      const int kk = cG + __popcll(bg & lt) + 1;
+      const int leIncl = cL + __popcll(bl & (lt | me));
+      ok = (leT - leIncl) >= kk;
+      if (ok) S.ge[kk - 1] = (uint16_t)(i - f);
+    }
+    const uint64_t okm = __ballot(ok), badm = __ballot(ge && !ok);
+    kc += __popcll(okm);
+    if (badm && fail == INT32_MAX) fail = b + __builtin_ctzll(badm);
+    cG += __popcll(bg);
+    cL += __popcll(bl);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (int kk = lane; kk < kc; kk += 64) {
+    const int i = f + S.ge[kk], j = f + S.le[leT - 1 - kk];
+    const double ki = S.K[i], kj = S.K[j];
+    const uint32_t hi = S.H[i], hj = S.H[j];
+    S.K[i] = kj; S.K[j] = ki;
+    S.H[i] = hj; S.H[j] = hi;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (kc == 0) return fail;
+  const int rK = f + S.le[leT - kc];
+  return fail < rK ? fail : rK;
+}
+
 __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict__ nodes,
                                                         const NodeInfo* __restrict__ infos, NodeWork* __restrict__ work,
-                                                        SampleParams P, unsigned long long* counters) {
-  constexpr int kSmall = 128;      // segments up to this size: staged in LDS, replayed by one lane
+                                                        SampleParams P, unsigned long long* counters, int stage_cap) {
   constexpr int kWaveSeg = 8192;   // segments up to this size: partitioned by one wavefront
   constexpr int kList = 256;       // larger segments per level (block-wide partitions)
   constexpr int kWaveList = 1024;  // segments handed to the wavefronts
@@ -1091,9 +1162,9 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
   __shared__ int32_t s_df[kWaveList], s_dl[kWaveList], s_dd[kWaveList];
   __shared__ int32_t s_sf[kWaves][kStack], s_sl[kWaves][kStack], s_sd[kWaves][kStack];
   __shared__ int s_nc, s_nn, s_nd_count, s_bad, s_any;
+  __shared__ int32_t s_lf[kWaves][kStack], s_ll[kWaves][kStack], s_ld[kWaves][kStack];
   __shared__ PartShared sh;
-  __shared__ double s_segK[kWaves][kSmall];
-  __shared__ uint32_t s_segH[kWaves][kSmall];
+  extern __shared__ __attribute__((aligned(16))) uint8_t s_stage[];  // kWaves x 16 x stage_cap bytes
   const int k = blockIdx.x;
   NodeWork w = work[k];
   if (!(w.flags & NW_EXACT_SORT) || (w.flags & NW_OVERFLOW)) return;
@@ -1237,10 +1308,10 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
     // segments up to kWaveSeg: the rest of the partition loop, one wavefront
     // each, with its own stack of the parts still to do (the right part is
     // pushed, the left one continues, as in pair_partition_loop; parts
-    // without a marked element are dropped); parts up to kSmall staged in
-    // LDS and replayed by one lane (the replay is a chain of dependent
-    // element accesses: LDS latency instead of memory latency), parts at
-    // the depth limit heap-sorted by one lane in place
+    // without a marked element are dropped); parts up to stage_cap staged in
+    // LDS and partitioned there by the same wavefront (LDS latency instead
+    // of memory latency for every pass), parts at the depth limit
+    // heap-sorted by one lane in place
     const int lane = tid & 63, wave = tid >> 6;
     bool wbad = false;
     for (int d = wave; d < s_nd_count && !wbad; d += kWaves) {
@@ -1257,22 +1328,64 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
             break;
           }
           const int len = (int)(l - f);
-          if (len <= kSmall) {
+          if (len <= stage_cap) {
+            // the rest of this segment's partition loop on its LDS copy: the
+            // same steps (wave partitions down to 16 elements, heap sort at
+            // the depth limit, unmarked parts dropped), the marks carried in
+            // bit 31 of the main index
+            const StageLds S = stage_view(s_stage + (size_t)wave * 16 * stage_cap, stage_cap);
             for (int i = lane; i < len; i += 64) {
-              s_segK[wave][i] = a.K[f + i];
-              s_segH[wave][i] = a.H[f + i];
+              const uint32_t h = a.H[f + i];
+              S.K[i] = a.K[f + i];
+              S.H[i] = h | (mark[h] ? kStageMark : 0u);
             }
-            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) { s_lf[wave][0] = 0; s_ll[wave][0] = len; s_ld[wave][0] = depth; }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            if (lane == 0) pair_partition_loop(PairArr{s_segK[wave], s_segH[wave]}, 0, len, depth);
             __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            int lsp = 1;
+            while (lsp > 0 && !wbad) {
+              lsp--;
+              int lf = s_lf[wave][lsp], ll = s_ll[wave][lsp], ld = s_ld[wave][lsp];
+              while (ll - lf > 16) {
+                if (ld == 0) {
+                  if (lane == 0) pair_heap_sort(PairArr{(double*)S.K, (uint32_t*)S.H}, lf, ll - lf);
+                  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                  __builtin_amdgcn_wave_barrier();
+                  break;
+                }
+                ld--;
+                if (lane == 0) pair_pivot_to_first(PairArr{(double*)S.K, (uint32_t*)S.H}, lf, ll);
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                const int cut = wave_partition_lds(S, lf, ll);
+                if (cut <= lf || cut >= ll) {  // cannot happen for a median-of-three pivot; stay bounded
+                  wbad = true;
+                  break;
+                }
+                uint64_t mL = 0, mR = 0;
+                for (int b = lf; b < ll; b += 64) {
+                  const int i = b + lane;
+                  const bool m = i < ll && (S.H[i < ll ? i : lf] & kStageMark) != 0;
+                  mL |= __ballot(m && i < cut);
+                  mR |= __ballot(m && i >= cut);
+                }
+                if (ll - cut > 16 && mR) {
+                  if (lsp >= kStack) { wbad = true; break; }
+                  if (lane == 0) { s_lf[wave][lsp] = cut; s_ll[wave][lsp] = ll; s_ld[wave][lsp] = ld; }
+                  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                  __builtin_amdgcn_wave_barrier();
+                  lsp++;
+                }
+                if (!mL) break;
+                ll = cut;
+              }
+            }
             for (int i = lane; i < len; i += 64) {
-              a.K[f + i] = s_segK[wave][i];
-              a.H[f + i] = s_segH[wave][i];
+              a.K[f + i] = S.K[i];
+              a.H[f + i] = S.H[i] & ~kStageMark;
             }
-            __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
             break;
           }
           depth--;
@@ -3764,7 +3877,14 @@ int sample_level(const int32_t* d_nodes, int32_t num_nodes, const NodeInfo* d_in
   }
   hipLaunchKernelGGL(k_prepare, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_state, d_work, p);
   PCG_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_exact_sort, grid, dim3(kBlock), 0, s, d_nodes, d_info, d_work, p, p.counters);
+  {
+    const int cus = p.num_cus > 0 ? p.num_cus : 256;
+    const int cap = num_nodes <= cus ? kStageNarrow : kStageWide;
+    const size_t lds = (size_t)kWaves * 16 * cap;
+    PCG_CHECK(hipFuncSetAttribute((const void*)k_exact_sort, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)((size_t)kWaves * 16 * kStageNarrow)));
+    hipLaunchKernelGGL(k_exact_sort, grid, dim3(kBlock), lds, s, d_nodes, d_info, d_work, p, p.counters, cap);
+  }
   PCG_CHECK(hipGetLastError());
   if (p.method == METHOD_POISSON && num_nodes >= p.ring_min_nodes) {
     // the queue takes the item table, which k_plan fills only afterwards
