@@ -1186,37 +1186,84 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
   // Only then does the order of ties that cannot exclude each other matter
   // (the sequential form then scans the accepted points in sorted order).
   if (P.method == METHOD_POISSON && n > 10001) {
-    for (int64_t w0 = 64 * (int64_t)tid; w0 < n; w0 += 64 * (int64_t)kBlock) {
-      if (w0 + 64 <= 10001) continue;
-      const int64_t hi = w0 + 64 < n ? w0 + 64 : n;
-      double ml = 1e308;
-      for (int64_t p = w0; p < hi; p++) {
-        const double cd = __builtin_sqrt(L.sadd[p]);
-        const double lim = cd - g.spacing;
-        const double ls = lim * lim;
-        ml = ls < ml ? ls : ml;
+    // one window per wavefront (its candidates on the lanes), four windows'
+    // loads in flight
+    const int lane = tid & 63, wave = tid >> 6;
+    bool cap = false;
+    for (int64_t q0 = wave; 64 * q0 < n; q0 += 4 * kWaves) {
+      double v[4], gate[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t w0 = 64 * (q0 + u * kWaves), p = w0 + lane;
+        const int64_t hi = w0 + 64 < n ? w0 + 64 : n, m = hi - 10001;
+        v[u] = w0 < n ? L.sadd[p < n ? p : n - 1] : 0.0;
+        gate[u] = (w0 < n && w0 + 64 > 10001 && m >= 1) ? L.sadd[m - 1] : 0.0;
       }
-      // more than 10 000 in [ml, add(hi - 1)): the first sadd >= ml lies
-      // before hi - 10 001, i.e. sadd[hi - 10 002] >= ml (sadd ascending)
-      const int64_t m = hi - 10001;
-      if (m >= 1 && !(L.sadd[m - 1] < ml)) s_cap = 1;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t w0 = 64 * (q0 + u * kWaves), p = w0 + lane;
+        const int64_t hi = w0 + 64 < n ? w0 + 64 : n, m = hi - 10001;
+        double ml = 1e308;
+        if (p < n) {
+          const double cd = __builtin_sqrt(v[u]);
+          const double lim = cd - g.spacing;
+          ml = lim * lim;
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+          const double o = __shfl_xor(ml, d);
+          ml = o < ml ? o : ml;
+        }
+        // more than 10 000 in [ml, add(hi - 1)): the first sadd >= ml lies
+        // before hi - 10 001, i.e. sadd[hi - 10 002] >= ml (sadd ascending)
+        if (w0 < n && w0 + 64 > 10001 && m >= 1 && !(gate[u] < ml)) cap = true;
+      }
     }
+    if (cap) s_cap = 1;
   }
   __syncthreads();
   const bool all_ties = s_cap != 0;
-  for (int64_t p = tid; p + 1 < n; p += kBlock) {
-    if (!(L.sadd[p] == L.sadd[p + 1]) || (p > 0 && L.sadd[p - 1] == L.sadd[p])) continue;
-    const int64_t e = run_end(L.sadd, p, n);
-    if (all_ties || tie_group_conflicts(L, g, P, p, e)) {
-      for (int64_t q = p; q < e; q++) mark[L.idxA[q]] = 1;
-      s_any = 1;
+  // tie group heads, four positions per thread with their loads in flight
+  for (int64_t b0 = 0; b0 + 1 < n; b0 += 4 * kBlock) {
+    double dp[4], dc[4], dn[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = b0 + u * kBlock + tid;
+      dc[u] = p < n ? L.sadd[p] : -1.0;
+      dn[u] = p + 1 < n ? L.sadd[p + 1] : -2.0;
+      dp[u] = (p > 0 && p < n) ? L.sadd[p - 1] : -3.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = b0 + u * kBlock + tid;
+      if (p + 1 >= n || !(dc[u] == dn[u]) || dp[u] == dc[u]) continue;
+      const int64_t e = run_end(L.sadd, p, n);
+      if (all_ties || tie_group_conflicts(L, g, P, p, e)) {
+        for (int64_t q = p; q < e; q++) mark[L.idxA[q]] = 1;
+        s_any = 1;
+      }
     }
   }
   __syncthreads();
+  // the partition phase works on (distance, main index) pairs; bit 31 of
+  // the main index marks the element, so that the count of marked elements
+  // on either side of a cut reads the pairs only
   PairArr a{L.sadd, L.idxB};
   uint32_t* posLe = L.keyB;
   uint32_t* posGe = L.keyA;
-  for (int64_t i = tid; i < n; i += kBlock) { a.K[i] = L.add[i]; a.H[i] = (uint32_t)i; }
+  for (int64_t b0 = 0; b0 < n; b0 += 4 * kBlock) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t i = b0 + u * kBlock + tid;
+      v[u] = i < n ? L.add[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t i = b0 + u * kBlock + tid;
+      if (i < n) { a.K[i] = v[u]; a.H[i] = (uint32_t)i | (mark[i] ? kStageMark : 0u); }
+    }
+  }
   if (tid == 0) {
     atomicAdd(&counters[3], 1ull);
     s_nc = 0; s_nd_count = 0; s_bad = 0;
@@ -1258,8 +1305,19 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
       int64_t cut = block_partition(a, posLe, posGe, f, l, sh);
       {
         unsigned long long mL = 0, mR = 0;
-        for (int64_t i = f + tid; i < l; i += kBlock)
-          if (mark[a.H[i]]) { if (i < cut) mL++; else mR++; }
+        for (int64_t b = f; b < l; b += 4 * kBlock) {
+          uint32_t h[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int64_t i = b + u * kBlock + tid;
+            h[u] = i < l ? a.H[i] : 0u;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int64_t i = b + u * kBlock + tid;
+            if (h[u] & kStageMark) { if (i < cut) mL++; else mR++; }
+          }
+        }
         if (mL) atomicAdd(&sh.markL, mL);
         if (mR) atomicAdd(&sh.markR, mR);
       }
@@ -1331,13 +1389,11 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
           if (len <= stage_cap) {
             // the rest of this segment's partition loop on its LDS copy: the
             // same steps (wave partitions down to 16 elements, heap sort at
-            // the depth limit, unmarked parts dropped), the marks carried in
-            // bit 31 of the main index
+            // the depth limit, unmarked parts dropped)
             const StageLds S = stage_view(s_stage + (size_t)wave * 16 * stage_cap, stage_cap);
             for (int i = lane; i < len; i += 64) {
-              const uint32_t h = a.H[f + i];
               S.K[i] = a.K[f + i];
-              S.H[i] = h | (mark[h] ? kStageMark : 0u);
+              S.H[i] = a.H[f + i];
             }
             if (lane == 0) { s_lf[wave][0] = 0; s_ll[wave][0] = len; s_ld[wave][0] = depth; }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -1382,7 +1438,7 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
             }
             for (int i = lane; i < len; i += 64) {
               a.K[f + i] = S.K[i];
-              a.H[f + i] = S.H[i] & ~kStageMark;
+              a.H[f + i] = S.H[i];
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -1405,13 +1461,10 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
               const int64_t i = b + 64 * u + lane;
               h[u] = a.H[i < l ? i : f];
             }
-            uint8_t mk[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) mk[u] = mark[h[u]];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
               const int64_t i = b + 64 * u + lane;
-              const bool m = i < l && mk[u] != 0;
+              const bool m = i < l && (h[u] & kStageMark) != 0;
               mL += __popcll(__ballot(m && i < cut));
               mR += __popcll(__ballot(m && i >= cut));
             }
@@ -1450,9 +1503,39 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
   // Short runs: one lane, insertion by position.  Long runs: one wave walks
   // that window in order and compacts the run's members by ballot.
   // Unmarked runs keep the radix order: no decision depends on it.
-  for (int64_t i = tid; i < n; i += kBlock) posLe[a.H[i]] = (uint32_t)i;
+  for (int64_t b0 = 0; b0 < n; b0 += 4 * kBlock) {
+    uint32_t h[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t i = b0 + u * kBlock + tid;
+      h[u] = i < n ? a.H[i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t i = b0 + u * kBlock + tid;
+      if (i < n) posLe[h[u] & ~kStageMark] = (uint32_t)i;
+    }
+  }
   __syncthreads();
-  for (int64_t p = tid; p < n; p += kBlock) L.sadd[p] = L.add[L.idxA[p]];
+  for (int64_t b0 = 0; b0 < n; b0 += 4 * kBlock) {
+    uint32_t v[4];
+    double d[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = b0 + u * kBlock + tid;
+      v[u] = p < n ? L.idxA[p] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = b0 + u * kBlock + tid;
+      d[u] = p < n ? L.add[v[u]] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t p = b0 + u * kBlock + tid;
+      if (p < n) L.sadd[p] = d[u];
+    }
+  }
   __syncthreads();
   if (s_any) {
     const int lane = tid & 63, wave = tid >> 6;
@@ -1486,7 +1569,7 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
           uint32_t h = 0;
           bool hit = false;
           if (j < w1) {
-            h = a.H[j];
+            h = a.H[j] & ~kStageMark;
             hit = L.add[h] == key;
           }
           const uint64_t hm = __ballot(hit);
