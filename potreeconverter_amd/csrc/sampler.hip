@@ -901,6 +901,38 @@ __device__ void pair_partition_loop(PairArr a, int64_t first, int64_t last, int6
   }
 }
 
+// The kc swaps of a partition (pair k: left position f + posGe[f + k] and
+// right position f + posLe[f + leT - 1 - k]), k = k0, k0 + stride, ...:
+// four pairs per thread with their loads in flight.  Every left position
+// lies before every right one, so no element takes part in two swaps.
+__device__ inline void swap_pairs(PairArr a, const uint32_t* posLe, const uint32_t* posGe, int64_t f, int64_t leT,
+                                  int64_t kc, int64_t k0, int64_t stride) {
+  for (int64_t kb = k0; kb < kc; kb += 4 * stride) {
+    int64_t I[4], J[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      int64_t k = kb + u * stride;
+      k = k < kc ? k : kc - 1;
+      I[u] = f + posGe[f + k];
+      J[u] = f + posLe[f + leT - 1 - k];
+    }
+    double KI[4], KJ[4];
+    uint32_t HI[4], HJ[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      KI[u] = a.K[I[u]]; KJ[u] = a.K[J[u]];
+      HI[u] = a.H[I[u]]; HJ[u] = a.H[J[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (kb + u * stride < kc) {
+        a.K[I[u]] = KJ[u]; a.K[J[u]] = KI[u];
+        a.H[I[u]] = HJ[u]; a.H[J[u]] = HI[u];
+      }
+    }
+  }
+}
+
 // Block-wide unguarded partition of [f+1, l) around the pivot at f: the
 // k-th element >= pivot from the left swaps with the k-th element <= pivot
 // from the right while the former lies left of the latter; the cut is the
@@ -981,11 +1013,7 @@ __device__ int64_t block_partition(PairArr a, uint32_t* posLe, uint32_t* posGe, 
     __syncthreads();
   }
   const int64_t kc = (int64_t)sh.kc;
-  for (int64_t k = tid; k < kc; k += kBlock) {
-    int64_t i = f + posGe[f + k];
-    int64_t j = f + posLe[f + leT - 1 - k];
-    pair_swap(a, i, j);
-  }
+  swap_pairs(a, posLe, posGe, f, leT, kc, tid, kBlock);
   __syncthreads();
   const int64_t fail = (int64_t)sh.fail;
   int64_t cut;
@@ -1059,7 +1087,7 @@ __device__ int64_t wave_partition(PairArr a, uint32_t* posLe, uint32_t* posGe, i
   }
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  for (int64_t kk = lane; kk < kc; kk += 64) pair_swap(a, f + posGe[f + kk], f + posLe[f + leT - 1 - kk]);
+  swap_pairs(a, posLe, posGe, f, leT, kc, lane, 64);
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
   if (kc == 0) return fail;
