@@ -265,7 +265,8 @@ struct RadixShared {
 // `pre` (LDS, 256 counts) is the digit's histogram when the caller counted
 // it already; otherwise the pass counts.  Returns false and writes nothing
 // when every key has the same digit.
-__device__ bool block_radix_pass(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+template <bool kPrefetch = false>
+__device__ __attribute__((always_inline)) inline bool block_radix_pass(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                  uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n, int shift,
                                  uint32_t mask, RadixShared& sh, const uint32_t* pre = nullptr) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -309,22 +310,36 @@ __device__ bool block_radix_pass(const uint32_t* __restrict__ kin, const uint32_
   }
   __syncthreads();
   if (sh.skip) return false;
-  for (int64_t b0 = 0; b0 < n; b0 += (int64_t)kRadixE * kBlock) {
-    uint32_t key[kRadixE], val[kRadixE], d[kRadixE];
-    uint64_t peers[kRadixE];
-    bool valid[kRadixE];
+  // a tile's keys and values (positions past the node read the last one);
+  // kPrefetch: the next tile's are loaded while this one is ranked and
+  // scattered (more registers), otherwise after the scatter
+  uint32_t key[kRadixE], val[kRadixE];
+  auto load_tile = [&](int64_t b0, uint32_t* kk, uint32_t* vv) {
 #pragma unroll
     for (int e = 0; e < kRadixE; e++) {
       const int64_t j = b0 + e * kBlock + tid;
-      valid[e] = j < n;
-      key[e] = valid[e] ? kin[j] : 0;
-      val[e] = valid[e] ? vin[j] : 0;
+      const int64_t jc = j < n ? j : n - 1;
+      kk[e] = kin[jc];
+      vv[e] = vin[jc];
     }
+  };
+  if (n > 0) load_tile(0, key, val);
+  for (int64_t b0 = 0; b0 < n; b0 += (int64_t)kRadixE * kBlock) {
+    uint32_t d[kRadixE];
+    uint64_t peers[kRadixE];
+    bool valid[kRadixE];
+#pragma unroll
+    for (int e = 0; e < kRadixE; e++) valid[e] = b0 + e * kBlock + tid < n;
 #pragma unroll
     for (int e = 0; e < kRadixE; e++) {
       d[e] = (key[e] >> shift) & mask;
       peers[e] = wave_match8(d[e], __ballot(valid[e]));
       if (valid[e] && (peers[e] & lt) == 0) sh.wc[e][wave][d[e]] = (uint32_t)__popcll(peers[e]);
+    }
+    uint32_t nkey[kRadixE], nval[kRadixE];
+    if constexpr (kPrefetch) {
+      const int64_t nb = b0 + (int64_t)kRadixE * kBlock;
+      load_tile(nb < n ? nb : b0, nkey, nval);
     }
     __syncthreads();
     // per digit (one per thread): exclusive prefix over (sub-tile, wave)
@@ -349,6 +364,16 @@ __device__ bool block_radix_pass(const uint32_t* __restrict__ kin, const uint32_
       const uint32_t pos = sh.wp[e][wave][d[e]] + (uint32_t)__popcll(peers[e] & lt);
       kout[pos] = key[e];
       vout[pos] = val[e];
+    }
+    if constexpr (kPrefetch) {
+#pragma unroll
+      for (int e = 0; e < kRadixE; e++) {
+        key[e] = nkey[e];
+        val[e] = nval[e];
+      }
+    } else {
+      const int64_t nb = b0 + (int64_t)kRadixE * kBlock;
+      if (nb < n) load_tile(nb, key, val);
     }
   }
   __syncthreads();
@@ -903,31 +928,34 @@ __device__ void pair_partition_loop(PairArr a, int64_t first, int64_t last, int6
 
 // The kc swaps of a partition (pair k: left position f + posGe[f + k] and
 // right position f + posLe[f + leT - 1 - k]), k = k0, k0 + stride, ...:
-// four pairs per thread with their loads in flight.  Every left position
+// U pairs per thread with their loads in flight.  Every left position
 // lies before every right one, so no element takes part in two swaps.
+template <int U>
 __device__ inline void swap_pairs(PairArr a, const uint32_t* posLe, const uint32_t* posGe, int64_t f, int64_t leT,
                                   int64_t kc, int64_t k0, int64_t stride) {
-  for (int64_t kb = k0; kb < kc; kb += 4 * stride) {
-    int64_t I[4], J[4];
+  const gptr<double> K = gbl(a.K);
+  const gptr<uint32_t> H = gbl(a.H);
+  for (int64_t kb = k0; kb < kc; kb += U * stride) {
+    uint32_t I[U], J[U];  // positions (node-relative, below 2^31)
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < U; u++) {
       int64_t k = kb + u * stride;
       k = k < kc ? k : kc - 1;
-      I[u] = f + posGe[f + k];
-      J[u] = f + posLe[f + leT - 1 - k];
+      I[u] = (uint32_t)f + posGe[f + k];
+      J[u] = (uint32_t)f + posLe[f + leT - 1 - k];
     }
-    double KI[4], KJ[4];
-    uint32_t HI[4], HJ[4];
+    double KI[U], KJ[U];
+    uint32_t HI[U], HJ[U];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      KI[u] = a.K[I[u]]; KJ[u] = a.K[J[u]];
-      HI[u] = a.H[I[u]]; HJ[u] = a.H[J[u]];
+    for (int u = 0; u < U; u++) {
+      KI[u] = K[I[u]]; KJ[u] = K[J[u]];
+      HI[u] = H[I[u]]; HJ[u] = H[J[u]];
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < U; u++) {
       if (kb + u * stride < kc) {
-        a.K[I[u]] = KJ[u]; a.K[J[u]] = KI[u];
-        a.H[I[u]] = HJ[u]; a.H[J[u]] = HI[u];
+        K[I[u]] = KJ[u]; K[J[u]] = KI[u];
+        H[I[u]] = HJ[u]; H[J[u]] = HI[u];
       }
     }
   }
@@ -1013,7 +1041,7 @@ __device__ int64_t block_partition(PairArr a, uint32_t* posLe, uint32_t* posGe, 
     __syncthreads();
   }
   const int64_t kc = (int64_t)sh.kc;
-  swap_pairs(a, posLe, posGe, f, leT, kc, tid, kBlock);
+  swap_pairs<4>(a, posLe, posGe, f, leT, kc, tid, kBlock);
   __syncthreads();
   const int64_t fail = (int64_t)sh.fail;
   int64_t cut;
@@ -1087,7 +1115,7 @@ __device__ int64_t wave_partition(PairArr a, uint32_t* posLe, uint32_t* posGe, i
   }
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  swap_pairs(a, posLe, posGe, f, leT, kc, lane, 64);
+  swap_pairs<2>(a, posLe, posGe, f, leT, kc, lane, 64);
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
   if (kc == 0) return fail;
@@ -1759,7 +1787,7 @@ __global__ __launch_bounds__(kBlock) void k_cells(const int32_t* __restrict__ no
   __syncthreads();
   uint32_t *kc = L.keyA, *vc = L.idxB, *ko = L.keyB, *vo = L.next;
   for (int pass = 0; pass < 3; pass++) {
-    if (!block_radix_pass(kc, vc, ko, vo, n, 7 * pass, 127u, s_rad, s_h3[pass])) continue;
+    if (!block_radix_pass<true>(kc, vc, ko, vo, n, 7 * pass, 127u, s_rad, s_h3[pass])) continue;
     uint32_t* t = kc; kc = ko; ko = t;
     t = vc; vc = vo; vo = t;
   }
