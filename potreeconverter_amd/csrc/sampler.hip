@@ -1447,9 +1447,21 @@ __global__ __launch_bounds__(kBlock) void k_exact_sort(const int32_t* __restrict
             // same steps (wave partitions down to 16 elements, heap sort at
             // the depth limit, unmarked parts dropped)
             const StageLds S = stage_view(s_stage + (size_t)wave * 16 * stage_cap, stage_cap);
-            for (int i = lane; i < len; i += 64) {
-              S.K[i] = a.K[f + i];
-              S.H[i] = a.H[f + i];
+            for (int i0 = 0; i0 < len; i0 += 4 * 64) {
+              double kv[4];
+              uint32_t hv[4];
+#pragma unroll
+              for (int u = 0; u < 4; u++) {
+                const int i = i0 + 64 * u + lane;
+                const int ic = i < len ? i : len - 1;
+                kv[u] = a.K[f + ic];
+                hv[u] = a.H[f + ic];
+              }
+#pragma unroll
+              for (int u = 0; u < 4; u++) {
+                const int i = i0 + 64 * u + lane;
+                if (i < len) { S.K[i] = kv[u]; S.H[i] = hv[u]; }
+              }
             }
             if (lane == 0) { s_lf[wave][0] = 0; s_ll[wave][0] = len; s_ld[wave][0] = depth; }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
